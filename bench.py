@@ -1,0 +1,253 @@
+#!/usr/bin/env python3
+"""Benchmark of the finite-volume hot path on one MI355X.
+
+Workload: 2D Newtonian Euler, 4096 x 4096 FP64 cells on [0, 1]^2, gamma 1.4,
+SLIC + van Leer, dimensionally split X(dt/2) Y(dt) X(dt/2), CFL 0.9,
+transmissive boundaries, seeded four-quadrant initial data.  A step is one
+full time step: the CFL time step (max wave speed, fused into the last
+sweep) and the three sweeps.  The state is resident in HBM before timing.
+
+    python bench.py [--steps 100] [--warmup 5] [--size 4096] [--no-cpu]
+
+Prints one JSON line.  value = cell-updates/s; roofline = the dominant
+kernel's algorithmic bytes per launch (80 B per cell per sweep: five FP64
+primitives read and five written) over its average launch duration, timed
+with HIP events on the solver's stream; cpu_baseline = the reference's own
+sources (oracle/_ref, serial build) timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s & %HBM peak (2D SLIC FP64 4096²); adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_SWEEP = 80.0      # 5 FP64 read + 5 FP64 written
+SWEEPS_PER_STEP = 3
+KIND_NAMES = ["step_head", "sweep_x", "sweep_y", "sweep_x_wavespeed"]
+
+
+class HipEvents:
+    """hipEvent_t timing through the HIP runtime (events on a given stream)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def new(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def gpu_leg(args):
+    from cosmos_amd import NEWTONIAN, VANLEER, Solver, native, workloads
+
+    n = args.size
+    W = workloads.quadrants_2d(n, seed=args.seed)
+    cells = n * n
+    dx = 1.0 / n
+    ev = HipEvents()
+    with Solver(NEWTONIAN, 2, n, n, 1.4) as s:
+        s.upload(W)
+        del W
+        stream = s.stream
+
+        # timed region: K full steps after W warm-up steps (device-resident loop)
+        s.begin(2, dx, 0.9, math.inf, args.warmup + args.steps, 0.0, VANLEER)
+        s.advance(args.warmup)
+        s.synchronize()
+        e0, e1 = ev.new(), ev.new()
+        t0 = time.perf_counter()
+        ev.record(e0, stream)
+        s.advance(args.steps)
+        ev.record(e1, stream)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        ms_total = ev.elapsed_ms(e0, e1)
+        t_reached, steps_done, _ = s.progress()
+        assert steps_done == args.warmup + args.steps, steps_done
+        state_ok = bool(np.isfinite(s.download()[:: max(1, n // 64), :: max(1, n // 64)]).all())
+
+        # the same steps again with every kernel bracketed by HIP events on the stream
+        s.begin(2, dx, 0.9, math.inf, args.warmup + args.steps, 0.0, VANLEER)
+        s.advance(args.warmup)
+        s.synchronize()
+        lib = native.load()
+        lib.cosmos_profile_enable(s._h, 1)
+        s.advance(args.steps)
+        tot = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int * 4)()
+        native.check(lib.cosmos_profile_read(s._h, tot, cnt), s._h)
+        lib.cosmos_profile_enable(s._h, 0)
+        kernels = {}
+        for k, name in enumerate(KIND_NAMES):
+            if cnt[k]:
+                kernels[name] = {"launches": cnt[k], "avg_ms": tot[k] / cnt[k], "total_ms": tot[k]}
+
+        # achievable bandwidth: a plain copy kernel moving the first half of the
+        # (now finished) state buffer onto its second half
+        planes, pitch = s.state_planes()
+        nbytes = 5 * n * pitch * 8
+        src = planes[0]
+        half = (nbytes // 2) // 16 * 16
+        c0, c1 = ev.new(), ev.new()
+        for _ in range(2):
+            native.check(lib.cosmos_device_copy(ctypes.c_void_p(src + half), ctypes.c_void_p(src), half,
+                                                ctypes.c_void_p(stream)), None)
+        ev.record(c0, stream)
+        reps = 5
+        for _ in range(reps):
+            native.check(lib.cosmos_device_copy(ctypes.c_void_p(src + half), ctypes.c_void_p(src), half,
+                                                ctypes.c_void_p(stream)), None)
+        ev.record(c1, stream)
+        copy_ms = ev.elapsed_ms(c0, c1) / reps
+        copy_gbs = 2.0 * half / (copy_ms * 1e-3) / 1e9
+
+    ms_per_step = ms_total / args.steps
+    cell_updates_per_s = cells / (ms_per_step * 1e-3)
+    dominant = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])
+    dom_name, dom = dominant
+    alg_bytes = BYTES_PER_CELL_SWEEP * cells
+    achieved = alg_bytes / (dom["avg_ms"] * 1e-3) / 1e9
+    return {
+        "ms_per_step": ms_per_step,
+        "wall_s": wall,
+        "cell_updates_per_s": cell_updates_per_s,
+        "step_hbm_gbs": cell_updates_per_s * BYTES_PER_CELL_SWEEP * SWEEPS_PER_STEP / 1e9,
+        "kernels": kernels,
+        "dominant": dom_name,
+        "dominant_alg_bytes": alg_bytes,
+        "dominant_avg_ms": dom["avg_ms"],
+        "dominant_achieved_gbs": achieved,
+        "copy_gbs": copy_gbs,
+        "t_reached": t_reached,
+        "state_finite": state_ok,
+    }
+
+
+def cpu_leg(args):
+    """The reference's own sources (serial build, oracle/_ref) on a bounded
+    sample of the same workload; the C restatement (oracle/liboracle.so,
+    OpenMP) on one full-size step for comparison."""
+    from cosmos_amd import workloads
+    import oracle
+
+    out = {}
+    ref = oracle.load_ref()
+    n = args.cpu_size
+    W = workloads.quadrants_2d(n, seed=args.seed)
+    if ref is not None:
+        t0 = time.perf_counter()
+        _, _, steps = ref.run(W, 0, 2, 1.0 / n, 0.9, max_steps=args.cpu_steps)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": n * n * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+            "sample": f"reference sources (serial build) on the same 4-quadrant SLIC workload at {n}x{n}, "
+                      f"{steps} full steps, {dt:.1f} s",
+        }
+    orc = oracle.load_oracle()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    orc.set_threads(threads)
+    N = args.size
+    WF = workloads.quadrants_2d(N, seed=args.seed)
+    t0 = time.perf_counter()
+    _, _, steps = orc.run(WF, 0, 2, 1.0 / N, 0.9, max_steps=1)
+    dt = time.perf_counter() - t0
+    port = {"value": N * N * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"C restatement (OpenMP over rows) on the full {N}x{N} workload, {steps} step, {dt:.1f} s"}
+    if "cpu_baseline" not in out:
+        out["cpu_baseline"] = port
+    else:
+        out["cpu_port"] = port
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+    except Exception:
+        model = platform.processor()
+    out["cpu_host"] = {"model": model, "nproc": os.cpu_count(), "omp_threads": threads}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-size", type=int, default=384)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    g = gpu_leg(args)
+    line = {
+        "metric": METRIC,
+        "value": g["cell_updates_per_s"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": g["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D Newtonian SLIC, van Leer, {args.size}x{args.size} FP64, gamma 1.4, CFL 0.9, "
+                               f"transmissive, seeded 4-quadrant (seed {args.seed})",
+                   "grid": [args.size, args.size], "order": 2, "limiter": "van Leer", "cfl": 0.9,
+                   "parallelism": "single GPU"},
+        "hbm_fraction": g["step_hbm_gbs"] / HBM_PEAK_GBS,
+        "step_algorithmic_gbs": g["step_hbm_gbs"],
+        "copy_kernel_gbs": g["copy_gbs"],
+        "hbm_fraction_of_copy": g["step_hbm_gbs"] / g["copy_gbs"],
+        "roofline": {
+            "bound": "hbm",
+            "kernel": g["dominant"],
+            "achieved": g["dominant_achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": g["dominant_achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "kernels": g["kernels"],
+        "timing": "HIP events on the solver stream; per-kernel averages from a second identical region with "
+                  "every launch bracketed by events",
+    }
+    if not args.no_cpu:
+        line.update(cpu_leg(args))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

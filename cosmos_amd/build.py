@@ -1,0 +1,54 @@
+"""Builds the gfx950 HIP library of the finite-volume update in-tree.
+
+    python -m cosmos_amd.build            # libcosmos_fv.so next to this file
+
+The library is plain hipcc output (no torch extension): the C-ABI in
+include/cosmos_fv.h over the kernels in csrc/.  FP64 code is compiled with
+-ffp-contract=off (hipcc contracts by default for HIP) so that the only fused
+multiply-adds are the explicit ones, and without fast-math, so divisions and
+square roots are correctly rounded.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcosmos_fv.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("COSMOS_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = ["fv_kernels.hip", "fv_capi.hip"]
+
+FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+]
+
+
+def build(verbose: bool = False) -> str:
+    """Compile every source into libcosmos_fv.so; returns the library path."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

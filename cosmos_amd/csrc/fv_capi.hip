@@ -1,0 +1,547 @@
+// C ABI of the finite-volume update (declared in include/cosmos_fv.h).
+//
+// A cosmos_solver owns two device buffers of five SoA primitive planes and
+// ping-pongs between them, one sweep at a time; the device-resident
+// StepState carries t, dt and the iteration counter of a solve loop so that
+// steps are queued without host round trips.  No C++ exception crosses the
+// ABI: every entry point catches and converts to a status code.
+#include <climits>
+#include <cstdint>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmos_fv.h"
+#include "fv_kernels.hpp"
+
+using cfv::ConstPlanes;
+using cfv::Planes;
+using cfv::StepState;
+using cfv::SweepLaunch;
+
+struct cosmos_solver {
+    int physics = 0;
+    int dim = 2;
+    int nx = 0, ny = 0;
+    long pitch = 0;
+    cfv::PhysParams P{};
+    double* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    StepState* st = nullptr;   // device
+    unsigned long long* ws_scratch = nullptr;  // device, for cosmos_max_wave_speed
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // solve-loop parameters set by cosmos_begin.  Steps are queued without
+    // knowing on the host whether the device loop has stopped; a stopped step
+    // writes nothing, so the current buffer is fixed up from the device's step
+    // count (settle) before the host next touches the state.
+    int run_order = 2;
+    double run_bias = 0.0;
+    int run_limiter = 1;
+    bool run_ready = false;
+    int cur_begin = 0;
+    long queued = 0;
+    bool unsettled = false;
+    // optional per-kernel timing of queued steps (HIP events around each launch)
+    bool profiling = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(cosmos_solver* s, int code, const std::string& msg) {
+    if (s) s->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+#define HIPCHK(s, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail((s), COSMOS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+cfv::PhysParams make_params(double g) {
+    cfv::PhysParams P{};
+    P.gamma = g;
+    P.gm1 = g - 1.0;
+    P.c2p_tol = 1e-8;
+    P.c2p_maxit = 1000;
+    P.sr_c0 = -1.0 / (g * g);
+    P.sr_c1f = (g - 1.0) / (g * g);
+    P.sr_gm1g = (g - 1.0) / g;
+    P.sr_gm2g = (g - 2.0) / g;
+    P.sr_gg1 = g / (g - 1.0);
+    P.sr_4gm1g = 4.0 * ((g - 1.0) / g);
+    return P;
+}
+
+Planes planes_of(cosmos_solver* s, int b) {
+    Planes p;
+    size_t plane = (size_t)s->ny * s->pitch;
+    for (int k = 0; k < 5; ++k) p.p[k] = s->buf[b] + k * plane;
+    return p;
+}
+
+ConstPlanes cplanes_of(cosmos_solver* s, int b) {
+    ConstPlanes p;
+    Planes q = planes_of(s, b);
+    for (int k = 0; k < 5; ++k) p.p[k] = q.p[k];
+    return p;
+}
+
+__global__ void k_aos_to_soa(const double* aos, Planes out, int nx, int ny, long pitch) {
+    long total = (long)nx * ny;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+        long i = c / nx, j = c - i * nx;
+        for (int k = 0; k < 5; ++k) out.p[k][i * pitch + j] = aos[5 * c + k];
+    }
+}
+
+__global__ void k_soa_to_aos(ConstPlanes in, double* aos, int nx, int ny, long pitch) {
+    long total = (long)nx * ny;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+        long i = c / nx, j = c - i * nx;
+        for (int k = 0; k < 5; ++k) aos[5 * c + k] = in.p[k][i * pitch + j];
+    }
+}
+
+dim3 grid_for(long cells) {
+    long b = (cells + 255) / 256;
+    if (b > 16384) b = 16384;
+    if (b < 1) b = 1;
+    return dim3((unsigned)b);
+}
+
+SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, int limiter) {
+    SweepLaunch L{};
+    L.in = cplanes_of(s, in_buf);
+    L.out = planes_of(s, in_buf ^ 1);
+    L.nx = s->nx;
+    L.ny = s->ny;
+    L.pitch = s->pitch;
+    L.bias = bias;
+    L.limiter = limiter;
+    L.order = order;
+    L.physics = s->physics;
+    L.dim = s->dim;
+    L.P = s->P;
+    L.step = nullptr;
+    L.use_full = 0;
+    L.fuse_ws = 0;
+    return L;
+}
+
+int check_order(cosmos_solver* s, int order) {
+    if (order != 1 && order != 2) return fail(s, COSMOS_ERR_ARGUMENT, "order must be 1 (FORCE) or 2 (SLIC)");
+    if (order == 2 && (s->nx < 2 || (s->dim == 2 && s->ny < 2)))
+        return fail(s, COSMOS_ERR_ARGUMENT, "SLIC needs at least 2 cells along each swept axis");
+    return COSMOS_OK;
+}
+
+// Kernel kinds reported by cosmos_profile_read.
+enum { kProfStep = 0, kProfSweepX = 1, kProfSweepY = 2, kProfSweepXWave = 3, kProfKinds = 4 };
+
+// Launches f() on the handle's stream, bracketed by HIP events when profiling.
+template <class F>
+hipError_t timed(cosmos_solver* s, int kind, F&& f) {
+    if (!s->profiling) return f();
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
+    if (e == hipSuccess) e = hipEventRecord(a, s->stream);
+    if (e == hipSuccess) e = f();
+    if (e == hipSuccess) e = hipEventRecord(b, s->stream);
+    if (e != hipSuccess) {
+        if (a) hipEventDestroy(a);
+        if (b) hipEventDestroy(b);
+        return e;
+    }
+    s->prof.push_back({kind, {a, b}});
+    return hipSuccess;
+}
+
+int run_sweep(cosmos_solver* s, const SweepLaunch& L, int axis) {
+    int kind = axis == 1 ? kProfSweepY : (L.fuse_ws ? kProfSweepXWave : kProfSweepX);
+    hipError_t e = timed(s, kind, [&] {
+        return axis == 0 ? cfv::launch_sweep_x(L, s->stream) : cfv::launch_sweep_y(L, s->stream);
+    });
+    if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("sweep launch: ") + hipGetErrorString(e));
+    return COSMOS_OK;
+}
+
+int sweeps_per_step(const cosmos_solver* s) { return s->dim == 2 ? 3 : 1; }
+
+// Queue one solve-loop iteration reading its dt from the device StepState.
+int queue_step(cosmos_solver* s) {
+    hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_step_begin(s->st, s->stream); });
+    if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(e));
+    const int spp = sweeps_per_step(s);
+    const int b0 = (int)((s->cur_begin + (long)spp * s->queued) & 1);
+    for (int k = 0; k < spp; ++k) {
+        SweepLaunch L = base_launch(s, b0 ^ (k & 1), s->run_order, s->run_bias, s->run_limiter);
+        L.step = s->st;
+        L.use_full = (spp == 1) || (k == 1);
+        L.fuse_ws = (k == spp - 1);
+        int rc = run_sweep(s, L, (spp == 3 && k == 1) ? 1 : 0);
+        if (rc) return rc;
+    }
+    s->queued += 1;
+    s->unsettled = true;
+    return COSMOS_OK;
+}
+
+// After queued solve steps: wait, and point cur at the buffer that holds the
+// state after the steps the device actually took.
+int settle(cosmos_solver* s) {
+    if (!s->unsettled) return COSMOS_OK;
+    StepState h{};
+    HIPCHK(s, hipMemcpyAsync(&h, s->st, sizeof h, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->cur = (int)((s->cur_begin + (long)sweeps_per_step(s) * h.active_steps) & 1);
+    s->unsettled = false;
+    return COSMOS_OK;
+}
+
+template <class F>
+int guarded(cosmos_solver* s, F&& f) {
+    if (!s) return fail(nullptr, COSMOS_ERR_ARGUMENT, "null solver handle");
+    try {
+        hipError_t e = hipSetDevice(s->device);
+        if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(s, COSMOS_ERR_DEVICE, "host allocation failed");
+    } catch (...) {
+        return fail(s, COSMOS_ERR_STATE, "unexpected exception");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, double adiabatic_index) {
+    if (!out) return fail(nullptr, COSMOS_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    if (physics != COSMOS_NEWTONIAN && physics != COSMOS_RELATIVISTIC)
+        return fail(nullptr, COSMOS_ERR_ARGUMENT, "physics must be COSMOS_NEWTONIAN or COSMOS_RELATIVISTIC");
+    if (dim != 1 && dim != 2) return fail(nullptr, COSMOS_ERR_ARGUMENT, "dim must be 1 or 2");
+    if (dim == 1) ny = 1;
+    if (nx < 1 || ny < 1) return fail(nullptr, COSMOS_ERR_ARGUMENT, "grid must have at least one cell");
+    if ((long)nx * ny > (1L << 31)) return fail(nullptr, COSMOS_ERR_ARGUMENT, "grid too large");
+    cosmos_solver* s = nullptr;
+    try {
+        s = new cosmos_solver();
+    } catch (...) {
+        return fail(nullptr, COSMOS_ERR_DEVICE, "host allocation failed");
+    }
+    s->physics = physics;
+    s->dim = dim;
+    s->nx = nx;
+    s->ny = ny;
+    s->pitch = ((long)nx + 15) / 16 * 16;
+    s->P = make_params(adiabatic_index);
+    hipError_t e = hipGetDevice(&s->device);
+    size_t bytes = sizeof(double) * 5 * (size_t)ny * s->pitch;
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->buf[0], bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->buf[1], bytes);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->st, sizeof(StepState));
+    if (e == hipSuccess) e = hipMalloc((void**)&s->ws_scratch, sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(s->buf[0], 0, bytes);
+    if (e == hipSuccess) e = hipMemset(s->buf[1], 0, bytes);
+    if (e == hipSuccess) e = hipMemset(s->st, 0, sizeof(StepState));
+    if (e != hipSuccess) {
+        std::string msg = std::string("device setup: ") + hipGetErrorString(e);
+        cosmos_destroy(s);
+        return fail(nullptr, COSMOS_ERR_DEVICE, msg);
+    }
+    *out = s;
+    return COSMOS_OK;
+}
+
+int cosmos_destroy(cosmos_solver* s) {
+    if (!s) return COSMOS_OK;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (auto& p : s->prof) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    hipFree(s->buf[0]);
+    hipFree(s->buf[1]);
+    hipFree(s->st);
+    hipFree(s->ws_scratch);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return COSMOS_OK;
+}
+
+const char* cosmos_last_error(const cosmos_solver* s) {
+    return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations) {
+    return guarded(s, [&] {
+        if (!(tolerance > 0.0) || max_iterations < 1)
+            return fail(s, COSMOS_ERR_ARGUMENT, "con2prim tolerance must be > 0 and max_iterations >= 1");
+        s->P.c2p_tol = tolerance;
+        s->P.c2p_maxit = max_iterations;
+        return COSMOS_OK;
+    });
+}
+
+static int upload_impl(cosmos_solver* s, const double* prim, hipMemcpyKind kind) {
+    if (!prim) return fail(s, COSMOS_ERR_ARGUMENT, "null state pointer");
+    int rc = settle(s);
+    if (rc) return rc;
+    long cells = (long)s->nx * s->ny;
+    double* stage = s->buf[s->cur ^ 1];  // 5 * ny * pitch >= 5 * cells doubles
+    HIPCHK(s, hipMemcpyAsync(stage, prim, sizeof(double) * 5 * cells, kind, s->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, grid_for(cells), dim3(256), 0, s->stream, stage, planes_of(s, s->cur), s->nx,
+                       s->ny, s->pitch);
+    HIPCHK(s, hipGetLastError());
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->run_ready = false;
+    return COSMOS_OK;
+}
+
+static int download_impl(cosmos_solver* s, double* prim, hipMemcpyKind kind) {
+    if (!prim) return fail(s, COSMOS_ERR_ARGUMENT, "null state pointer");
+    int rc = settle(s);
+    if (rc) return rc;
+    long cells = (long)s->nx * s->ny;
+    double* stage = s->buf[s->cur ^ 1];
+    hipLaunchKernelGGL(k_soa_to_aos, grid_for(cells), dim3(256), 0, s->stream, cplanes_of(s, s->cur), stage, s->nx,
+                       s->ny, s->pitch);
+    HIPCHK(s, hipGetLastError());
+    HIPCHK(s, hipMemcpyAsync(prim, stage, sizeof(double) * 5 * cells, kind, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return COSMOS_OK;
+}
+
+int cosmos_upload(cosmos_solver* s, const double* prim) {
+    return guarded(s, [&] { return upload_impl(s, prim, hipMemcpyHostToDevice); });
+}
+int cosmos_download(cosmos_solver* s, double* prim) {
+    return guarded(s, [&] { return download_impl(s, prim, hipMemcpyDeviceToHost); });
+}
+int cosmos_upload_device(cosmos_solver* s, const double* prim_device) {
+    return guarded(s, [&] { return upload_impl(s, prim_device, hipMemcpyDeviceToDevice); });
+}
+int cosmos_download_device(cosmos_solver* s, double* prim_device) {
+    return guarded(s, [&] { return download_impl(s, prim_device, hipMemcpyDeviceToDevice); });
+}
+
+int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
+                 int slope_limiter) {
+    return guarded(s, [&] {
+        int rc = check_order(s, order);
+        if (rc) return rc;
+        if (axis != 0 && axis != 1) return fail(s, COSMOS_ERR_ARGUMENT, "axis must be 0 (x) or 1 (y)");
+        if (axis == 1 && s->dim == 1) return fail(s, COSMOS_ERR_ARGUMENT, "a 1D solver has no y sweep");
+        rc = settle(s);
+        if (rc) return rc;
+        SweepLaunch L = base_launch(s, s->cur, order, bias, slope_limiter);
+        L.coef.dt_dx = time_step / cell_spacing;
+        L.coef.dx_dt = cell_spacing / time_step;
+        L.coef.hdt_dx = 0.5 * (time_step / cell_spacing);
+        s->run_ready = false;
+        rc = run_sweep(s, L, axis);
+        if (rc) return rc;
+        s->cur ^= 1;
+        return COSMOS_OK;
+    });
+}
+
+static int max_ws_impl(cosmos_solver* s, unsigned long long* bits_dev) {
+    int rc = settle(s);
+    if (rc) return rc;
+    hipError_t e = cfv::launch_zero_u64(bits_dev, s->stream);
+    if (e == hipSuccess)
+        e = cfv::launch_wavespeed(cplanes_of(s, s->cur), s->nx, s->ny, s->pitch, s->physics, s->dim, s->P, bits_dev,
+                                  s->stream);
+    if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("wave speed launch: ") + hipGetErrorString(e));
+    return COSMOS_OK;
+}
+
+int cosmos_max_wave_speed(cosmos_solver* s, double* out) {
+    return guarded(s, [&] {
+        if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "out is null");
+        int rc = max_ws_impl(s, s->ws_scratch);
+        if (rc) return rc;
+        unsigned long long bits = 0;
+        HIPCHK(s, hipMemcpyAsync(&bits, s->ws_scratch, sizeof bits, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        double v;
+        std::memcpy(&v, &bits, sizeof v);
+        *out = v;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, double current_time,
+                            double final_time, int current_iteration, double* out) {
+    return guarded(s, [&] {
+        if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "out is null");
+        double smax = 0.0;
+        int rc = cosmos_max_wave_speed(s, &smax);
+        if (rc) return rc;
+        // EulerSolvers::computeStableTimeStep (Solvers/EulerSolvers.cpp:130-145)
+        double dt = cfl * (cell_spacing / smax);
+        if (current_iteration <= 5) dt *= 0.2;
+        if ((current_time + dt) > final_time) dt = final_time - current_time;
+        *out = dt;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
+                 double bias, int slope_limiter) {
+    return guarded(s, [&] {
+        int rc = check_order(s, order);
+        if (rc) return rc;
+        StepState h{};
+        h.t = 0.0;
+        h.tf = final_time;
+        h.cfl = cfl;
+        h.dx = cell_spacing;
+        h.iter = 0;
+        h.max_steps = max_steps < 0 ? INT_MAX : max_steps;
+        h.done = 0;
+        h.active_steps = 0;
+        HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
+        rc = max_ws_impl(s, &s->st->ws_bits);
+        if (rc) return rc;
+        s->run_order = order;
+        s->run_bias = bias;
+        s->run_limiter = slope_limiter;
+        s->run_ready = true;
+        s->cur_begin = s->cur;
+        s->queued = 0;
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_advance(cosmos_solver* s, int n_steps) {
+    return guarded(s, [&] {
+        if (!s->run_ready) return fail(s, COSMOS_ERR_STATE, "cosmos_advance before cosmos_begin");
+        for (int k = 0; k < n_steps; ++k) {
+            int rc = queue_step(s);
+            if (rc) return rc;
+        }
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_progress(cosmos_solver* s, double* time, int* steps, int* stopped) {
+    return guarded(s, [&] {
+        StepState h{};
+        HIPCHK(s, hipMemcpyAsync(&h, s->st, sizeof h, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (time) *time = h.t;
+        if (steps) *steps = h.active_steps;
+        if (stopped) *stopped = h.done || !(h.t < h.tf) || h.iter >= h.max_steps;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_solve(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
+                 double bias, int slope_limiter, double* time_out, int* steps_out) {
+    return guarded(s, [&] {
+        int rc = cosmos_begin(s, order, cell_spacing, cfl, final_time, max_steps, bias, slope_limiter);
+        if (rc) return rc;
+        int stopped = 0, steps = 0;
+        double t = 0.0;
+        const int chunk = max_steps >= 0 && max_steps < 64 ? (max_steps > 0 ? max_steps : 1) : 64;
+        while (true) {
+            rc = cosmos_advance(s, chunk);
+            if (rc) return rc;
+            rc = cosmos_progress(s, &t, &steps, &stopped);
+            if (rc) return rc;
+            if (stopped) break;
+        }
+        if (time_out) *time_out = t;
+        if (steps_out) *steps_out = steps;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_synchronize(cosmos_solver* s) {
+    return guarded(s, [&] {
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return COSMOS_OK;
+    });
+}
+
+void* cosmos_stream(cosmos_solver* s) { return s ? (void*)s->stream : nullptr; }
+
+int cosmos_profile_enable(cosmos_solver* s, int on) {
+    return guarded(s, [&] {
+        s->profiling = on != 0;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_profile_read(cosmos_solver* s, double* total_ms, int* launches) {
+    return guarded(s, [&] {
+        if (!total_ms || !launches) return fail(s, COSMOS_ERR_ARGUMENT, "null output");
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        for (int k = 0; k < kProfKinds; ++k) {
+            total_ms[k] = 0.0;
+            launches[k] = 0;
+        }
+        for (auto& p : s->prof) {
+            float ms = 0.0f;
+            HIPCHK(s, hipEventElapsedTime(&ms, p.second.first, p.second.second));
+            total_ms[p.first] += ms;
+            launches[p.first] += 1;
+            hipEventDestroy(p.second.first);
+            hipEventDestroy(p.second.second);
+        }
+        s->prof.clear();
+        return COSMOS_OK;
+    });
+}
+
+namespace {
+// A plain streaming copy (16-byte vectors), used to calibrate bandwidth and
+// hardware byte counters against a kernel of known traffic.
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, long n2) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+int cosmos_device_copy(void* dst, const void* src, long bytes, void* stream) {
+    if (!dst || !src || bytes <= 0 || (bytes % 16) != 0 || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return fail(nullptr, COSMOS_ERR_ARGUMENT, "copy needs 16-byte aligned pointers and a multiple of 16 bytes");
+    long n2 = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const double2*)src, (double2*)dst, n2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(nullptr, COSMOS_ERR_DEVICE, std::string("copy launch: ") + hipGetErrorString(e));
+    return COSMOS_OK;
+}
+
+int cosmos_state_planes(cosmos_solver* s, double** planes5, long* pitch) {
+    return guarded(s, [&] {
+        if (!planes5) return fail(s, COSMOS_ERR_ARGUMENT, "planes5 is null");
+        int rc = settle(s);
+        if (rc) return rc;
+        Planes p = planes_of(s, s->cur);
+        for (int k = 0; k < 5; ++k) planes5[k] = p.p[k];
+        if (pitch) *pitch = s->pitch;
+        return COSMOS_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,74 @@
+// Launch interface between the C-ABI host layer and the gfx950 sweep kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fv_physics.hpp"
+
+namespace cfv {
+
+// Per-sweep coefficients: dt/dx, dx/dt and 0.5*(dt/dx), with dt the sweep's
+// step (0.5*dt for the outer Strang x sweeps).  Formed exactly as the
+// reference forms them at each use (Solvers/EulerSolvers.cpp:163-172,
+// Solvers/EulerFirstOrderSolver.cpp:3-19, 96-101).
+struct SweepCoef {
+    double dt_dx;
+    double dx_dt;
+    double hdt_dx;
+    double pad;
+};
+
+// Device-resident time-stepping state for solve/solve2D-style loops
+// (Solvers/EulerSecondOrderSolver.cpp:133-178): t, the iteration counter
+// (which drives the 0.2x damping of the first six steps), the final time and
+// the step cap.  The sweeps of a step read their coefficients from here, so
+// an N-step loop needs no host round trip and can be captured in a graph.
+struct StepState {
+    double t;
+    double tf;
+    double dt;
+    double cfl;
+    double dx;
+    unsigned long long ws_bits;  // running max wave speed (bit pattern; >= +0)
+    int iter;
+    int max_steps;
+    int done;
+    int active_steps;            // steps actually taken by this context
+    SweepCoef half;              // the outer x sweeps of a 2D step: 0.5 * dt
+    SweepCoef full;              // the y sweep of a 2D step and the 1D sweep: dt
+};
+
+struct Planes {
+    double* p[5];
+};
+
+struct ConstPlanes {
+    const double* p[5];
+};
+
+struct SweepLaunch {
+    ConstPlanes in;
+    Planes out;
+    int nx, ny;
+    long pitch;                 // doubles per row
+    double bias;
+    int limiter;
+    int order;                  // 1 = FORCE, 2 = SLIC
+    int physics;                // 0 Newtonian, 1 SR
+    int dim;                    // 1 or 2
+    PhysParams P;
+    SweepCoef coef;             // used when step == nullptr
+    StepState* step;            // device state: coefficients and done flag
+    int use_full;               // which StepState coefficients (0 half, 1 full)
+    int fuse_ws;                // accumulate max wave speed of the output into step->ws_bits
+};
+
+// All return hipSuccess or the launch error.
+hipError_t launch_sweep_x(const SweepLaunch& L, hipStream_t s);
+hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
+hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
+                            const PhysParams& P, unsigned long long* out_bits, hipStream_t s);
+hipError_t launch_step_begin(StepState* st, hipStream_t s);
+hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+
+}  // namespace cfv

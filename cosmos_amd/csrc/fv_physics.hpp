@@ -1,0 +1,279 @@
+// Device-side physics for the finite-volume sweeps (gfx950, FP64).
+//
+// Every function reproduces the reference's IEEE operation sequence, so that
+// results are bitwise equal to the reference CPU path.  Divisions that share
+// a denominator use one correctly rounded reciprocal y = RN(1/b) and the
+// Markstein correction q' = RN(q - RN(b*q - a) * y), q = RN(a*y), which
+// returns RN(a/b) exactly for normal operands (the same bits as the division
+// it replaces, including the sign of zero) for 3 FP64 ops instead of a full
+// division sequence.  Code is compiled with -ffp-contract=off; the only fused
+// multiply-adds are the explicit ones below.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cfv {
+
+// Physics parameters, computed once on the host with the reference's
+// expressions (EulerMaterialParameters is passed by value everywhere, so the
+// derived constants are the same values at every use).
+struct PhysParams {
+    double gamma;      // adiabatic index
+    double gm1;        // gamma - 1.0
+    double c2p_tol;    // SR con2prim stopping tolerance (reference: 1e-8)
+    int c2p_maxit;     // SR con2prim iteration cap (reference: 1000)
+    int pad;
+    // SR con2prim constants (RelativisticEuler/RelativisticEulerStateVector.cpp:52-57)
+    double sr_c0;      // -1.0 / (gamma * gamma)
+    double sr_c1f;     // (gamma - 1.0) / (gamma * gamma)
+    double sr_gm1g;    // (gamma - 1.0) / gamma
+    double sr_gm2g;    // (gamma - 2.0) / gamma
+    double sr_gg1;     // gamma / (gamma - 1.0)
+    double sr_4gm1g;   // 4.0 * ((gamma - 1.0) / gamma)
+};
+
+// libstdc++'s std::min / std::max: min(a, b) = (b < a) ? b : a.
+__device__ __forceinline__ double min_std(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double max_std(double a, double b) { return (a < b) ? b : a; }
+
+// RN(a / b) from y = RN(1 / b).
+__device__ __forceinline__ double div_by(double a, double b, double y) {
+    double q = a * y;
+    double r = __builtin_fma(b, q, -a);
+    return __builtin_fma(-r, y, q);
+}
+
+// ---------------------------------------------------------------------------
+// Newtonian ideal gas
+// ---------------------------------------------------------------------------
+struct Newtonian {
+    static constexpr int kPhysics = 0;
+
+    // EulerStateVector::computeConservedVariableVector (Euler/EulerStateVector.cpp:55-66, 129-134).
+    __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
+        double vsq = ((W[1] * W[1]) + (W[2] * W[2])) + (W[3] * W[3]);
+        double e = W[4] / (P.gm1 * W[0]);
+        U[0] = W[0];
+        U[1] = W[0] * W[1];
+        U[2] = W[0] * W[2];
+        U[3] = W[0] * W[3];
+        U[4] = W[0] * ((0.5 * vsq) + e);
+    }
+
+    // Primitive recovery shared by setConservedVariableVector (Euler/EulerStateVector.cpp:30-40)
+    // and compute{X,Y}FluxVector(cons) (:68-117).  R = (rho, u, v, w, p), E = U4 / rho.
+    // irho = RN(1 / U0) may be supplied when already known.
+    __device__ static __forceinline__ void recover(const double* U, double* R, double& E, double irho,
+                                                   const PhysParams& P) {
+        double rho = U[0];
+        double u = div_by(U[1], rho, irho);
+        double v = div_by(U[2], rho, irho);
+        double w = div_by(U[3], rho, irho);
+        double vsq = ((u * u) + (v * v)) + (w * w);
+        E = div_by(U[4], rho, irho);
+        double e = E - (0.5 * vsq);
+        R[0] = rho;
+        R[1] = u;
+        R[2] = v;
+        R[3] = w;
+        R[4] = (e * P.gm1) * rho;
+    }
+
+    // Flux from recovered primitives (Euler/EulerStateVector.cpp:82-86 x, :110-114 y).
+    template <int AXIS>
+    __device__ static __forceinline__ void flux_of(const double* R, double E, double* F) {
+        double rho = R[0], u = R[1], v = R[2], w = R[3], p = R[4];
+        if (AXIS == 0) {
+            F[0] = rho * u;
+            F[1] = (rho * (u * u)) + p;
+            F[2] = rho * (u * v);
+            F[3] = rho * (u * w);
+            F[4] = (rho * (u * E)) + (u * p);
+        } else {
+            F[0] = rho * v;
+            F[1] = rho * (v * u);
+            F[2] = (rho * (v * v)) + p;
+            F[3] = rho * (v * w);
+            F[4] = (rho * (v * E)) + (v * p);
+        }
+    }
+
+    // c2p: primitives of U.
+    __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P) {
+        double E;
+        recover(U, W, E, irho, P);
+    }
+
+    // F(U) (the static cons flux).
+    template <int AXIS>
+    __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P) {
+        double R[5], E;
+        recover(U, R, E, irho, P);
+        flux_of<AXIS>(R, E, F);
+    }
+
+    __device__ static __forceinline__ double inv_density(const double* U) { return 1.0 / U[0]; }
+
+    // EulerEquationOfState::computeSoundSpeed (Euler/EulerEquationOfState.cpp:17-22) and the
+    // per-cell wave speed of computeMaximumWaveSpeed{,2D} (Solvers/EulerSolvers.cpp:87-128):
+    // 1D |u| + |c|; 2D max(|u|, |v| + |c|).
+    template <int DIM>
+    __device__ static __forceinline__ double wavespeed(const double* W, const PhysParams& P) {
+        double c = __builtin_sqrt((P.gamma * W[4]) / W[0]);
+        if (DIM == 1) return __builtin_fabs(W[1]) + __builtin_fabs(c);
+        return max_std(__builtin_fabs(W[1]), __builtin_fabs(W[2]) + __builtin_fabs(c));
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Special-relativistic ideal gas
+// ---------------------------------------------------------------------------
+struct Relativistic {
+    static constexpr int kPhysics = 1;
+
+    // RelativisticEulerStateVector::computeConservedVariableVector
+    // (RelativisticEuler/RelativisticEulerStateVector.cpp:116-136).
+    __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
+        double rho = W[0], u = W[1], v = W[2], w = W[3], p = W[4];
+        double vsq = ((u * u) + (v * v)) + (w * w);
+        double lf = 1.0 / __builtin_sqrt(1.0 - vsq);
+        if (vsq > 1.0 - 1e-8) lf = 1.0 / __builtin_sqrt(1.0 - 1e-8);
+        double h = 1.0 + ((p / rho) * P.sr_gg1);
+        double rhw = (rho * h) * (lf * lf);
+        U[0] = rho * lf;
+        U[1] = rhw * u;
+        U[2] = rhw * v;
+        U[3] = rhw * w;
+        U[4] = (rhw - p) - (rho * lf);
+    }
+
+    // con2prim (RelativisticEuler/RelativisticEulerStateVector.cpp:30-101, duplicated in the
+    // flux functions :138-309): quartic Newton from 1.0, stop when |x - x_new| < tol keeping
+    // the old iterate, floors 1e-8 on rho and p (after the velocities are formed).
+    // R = (rho, vx, vy, vz, p) floored; lf = W; h = specific enthalpy.
+    __device__ static __forceinline__ void con2prim(const double* U, double* R, double& lf_out, double& h_out,
+                                                    const PhysParams& P) {
+        double D = U[0], Sx = U[1], Sy = U[2], Sz = U[3], tau = U[4];
+        double q = ((tau + D) * (tau + D)) - (((Sx * Sx) + (Sy * Sy)) + (Sz * Sz));
+        double sq = (q < 1e-8) ? __builtin_sqrt(1e-8) : __builtin_sqrt(q);
+        double dc = D / sq;
+        double ec = (D + tau) / sq;
+        double g = P.gamma;
+        double c1 = (-2.0 * dc) * P.sr_c1f;
+        double c2 = ((P.sr_gm2g * ((ec * ec) - 1.0)) + 1.0) - (((dc * dc) * P.sr_gm1g) * P.sr_gm1g);
+        double c4 = (ec * ec) - 1.0;
+        double eta = (2.0 * dc) * P.sr_gm1g;
+        double c0 = P.sr_c0;
+        double c2x2 = 2.0 * c2, c4x4 = 4.0 * c4, e3c4 = (3.0 * eta) * c4;
+        (void)g;
+        double x = 1.0;
+        for (int it = 0; it < P.c2p_maxit; ++it) {
+            double x2 = x * x;
+            double x3 = x2 * x;
+            double f = ((((c4 * x3) * (x - eta)) + (c2 * x2)) + (c1 * x)) + c0;
+            double fd = ((c1 + (c2x2 * x)) + (c4x4 * x3)) - (e3c4 * x2);
+            double xn = x - (f / fd);
+            if (__builtin_fabs(x - xn) < P.c2p_tol) break;
+            x = xn;
+        }
+        double lf = ((0.5 * ec) * x) * (1.0 + __builtin_sqrt(1.0 + (P.sr_4gm1g * ((1.0 - (dc * x)) / ((ec * ec) * (x * x))))));
+        double h = 1.0 / (dc * x);
+        double rho = D / lf;
+        double rhw = (rho * h) * (lf * lf);
+        double irhw = 1.0 / rhw;
+        R[1] = div_by(Sx, rhw, irhw);
+        R[2] = div_by(Sy, rhw, irhw);
+        R[3] = div_by(Sz, rhw, irhw);
+        double p = (rhw - D) - tau;
+        R[0] = (rho < 1e-8) ? 1e-8 : rho;
+        R[4] = (p < 1e-8) ? 1e-8 : p;
+        lf_out = lf;
+        h_out = h;
+    }
+
+    __device__ static __forceinline__ void c2p(const double* U, double* W, double, const PhysParams& P) {
+        double lf, h;
+        con2prim(U, W, lf, h, P);
+    }
+
+    // compute{X,Y}FluxVector(cons) (RelativisticEuler/RelativisticEulerStateVector.cpp:210-219, 299-308),
+    // using the floored density and pressure.
+    template <int AXIS>
+    __device__ static __forceinline__ void flux(const double* U, double* F, double, const PhysParams& P) {
+        double R[5], lf, h;
+        con2prim(U, R, lf, h, P);
+        double rho = R[0], p = R[4];
+        double rhw = (rho * h) * (lf * lf);
+        double vn = R[1 + AXIS];
+        F[0] = (rho * lf) * vn;
+        F[1] = rhw * (R[1] * vn);
+        F[2] = rhw * (R[2] * vn);
+        F[3] = rhw * (R[3] * vn);
+        F[1 + AXIS] = F[1 + AXIS] + p;
+        F[4] = (((rhw - p) - (rho * lf)) * vn) + (p * vn);
+    }
+
+    __device__ static __forceinline__ double inv_density(const double*) { return 0.0; }
+
+    // RelativisticEulerEquationOfState::computeSoundSpeed
+    // (RelativisticEuler/RelativisticEulerEquationOfState.cpp:10-18) and the wave speed of
+    // RelativisticEulerSolvers::computeMaximumWaveSpeed{,2D} (Solvers/RelativisticEulerSolvers.cpp:89-130):
+    // 1D |u| + |c|; 2D max(|u|, |v|) + |c|.
+    template <int DIM>
+    __device__ static __forceinline__ double wavespeed(const double* W, const PhysParams& P) {
+        double num = (P.gamma * W[4]) / W[0];
+        double den = 1.0 + ((W[4] / W[0]) * P.sr_gg1);
+        double c = __builtin_sqrt(num / den);
+        if (DIM == 1) return __builtin_fabs(W[1]) + __builtin_fabs(c);
+        return max_std(__builtin_fabs(W[1]), __builtin_fabs(W[2])) + __builtin_fabs(c);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Slope limiters (Solvers/SlopeLimiters.cpp)
+// ---------------------------------------------------------------------------
+
+// computeSlopeLimiter (:3-69) for a general bias.
+__device__ __forceinline__ double limiter_general(double r, double bias, int limiter) {
+    double xi = 2.0 / ((1.0 - bias) + ((1.0 + bias) * r));
+    if (limiter == 0) {
+        if (r < 0.0) return 0.0;
+        if (r >= 0.0 && r < 0.5) return 2.0 * r;
+        if (r >= 0.5 && r < 1.0) return 1.0;
+        return min_std(min_std(r, xi), 2.0);
+    }
+    if (limiter == 1) {
+        if (r < 0.0) return 0.0;
+        return min_std((2.0 * r) / (1.0 + r), xi);
+    }
+    if (r < 0.0) return 0.0;
+    if (r >= 0.0 && r < 1.0) return r;
+    return min_std(1.0, xi);
+}
+
+// Van Leer at bias 0: both candidates 2r/(1+r) and 2/(1+r) share the
+// denominator 1+r (bitwise: (1-0)+(1+0)r = 1+r).  For r <= 1 the first is the
+// smaller (RN is monotone), else the second, so one division of the selected
+// numerator gives std::min's result; r = +inf gives inf/inf = NaN in the
+// reference's first operand, which std::min then returns.
+__device__ __forceinline__ double vanleer_bias0(double r) {
+    if (r < 0.0) return 0.0;
+    double num = (r <= 1.0) ? (2.0 * r) : 2.0;
+    if (r == __builtin_inf()) num = __builtin_nan("");
+    return num / (1.0 + r);
+}
+
+// computeSlopeVector (:71-101) for one component.
+template <int LIMITER_MODE>  // 0: general (runtime limiter and bias), 1: van Leer at bias 0
+__device__ __forceinline__ double slope_component(double ul, double um, double ur, double bias, int limiter) {
+    double dl = um - ul;
+    double dr = ur - um;
+    double s = ((0.5 * (1.0 + bias)) * dl) + ((0.5 * (1.0 - bias)) * dr);
+    double num = (__builtin_fabs(dl) < 1e-5) ? 1e-5 : dl;
+    double den = (__builtin_fabs(dr) < 1e-5) ? 1e-5 : dr;
+    double r = num / den;
+    double phi = (LIMITER_MODE == 1) ? vanleer_bias0(r) : limiter_general(r, bias, limiter);
+    return s * phi;
+}
+
+}  // namespace cfv

@@ -1,0 +1,75 @@
+"""ctypes binding of the C-ABI in include/cosmos_fv.h.
+
+The HIP library is the only implementation: if libcosmos_fv.so is missing
+or fails to load, every call raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcosmos_fv.so")
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_handle = ctypes.c_void_p
+
+# name -> (restype, argtypes); mirrors include/cosmos_fv.h.
+SIGNATURES = {
+    "cosmos_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double]),
+    "cosmos_destroy": (ctypes.c_int, [_handle]),
+    "cosmos_last_error": (ctypes.c_char_p, [_handle]),
+    "cosmos_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
+    "cosmos_upload": (ctypes.c_int, [_handle, _c_double_p]),
+    "cosmos_download": (ctypes.c_int, [_handle, _c_double_p]),
+    "cosmos_upload_device": (ctypes.c_int, [_handle, ctypes.c_void_p]),
+    "cosmos_download_device": (ctypes.c_int, [_handle, ctypes.c_void_p]),
+    "cosmos_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_int]),
+    "cosmos_max_wave_speed": (ctypes.c_int, [_handle, _c_double_p]),
+    "cosmos_stable_time_step": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_int, _c_double_p]),
+    "cosmos_begin": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "cosmos_advance": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cosmos_progress": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, _c_int_p]),
+    "cosmos_solve": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p]),
+    "cosmos_synchronize": (ctypes.c_int, [_handle]),
+    "cosmos_stream": (ctypes.c_void_p, [_handle]),
+    "cosmos_profile_enable": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cosmos_profile_read": (ctypes.c_int, [_handle, _c_double_p, _c_int_p]),
+    "cosmos_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p]),
+    "cosmos_state_planes": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_long)]),
+}
+
+_lib = None
+
+
+class CosmosError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise CosmosError(f"{path} not found: build it with `python -m cosmos_amd.build` "
+                          "(the HIP library is the only implementation of this path)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != 0:
+        msg = load().cosmos_last_error(handle)
+        raise CosmosError(f"cosmos error {rc}: {msg.decode() if msg else ''}")

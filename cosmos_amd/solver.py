@@ -1,0 +1,134 @@
+"""Python handle over one device grid (a thin wrapper of include/cosmos_fv.h).
+
+Method names follow the reference's static solver methods they stand for:
+    sweep             compute{X,Y}SLICTimeStep2D / computeSLICTimeStep and the FORCE ones
+    max_wave_speed    computeMaximumWaveSpeed{,2D}
+    stable_time_step  computeStableTimeStep{,2D}
+    solve             solve / solve2D (to a final time, or a fixed step count)
+States are numpy float64 arrays of primitives, shape (ny, nx, 5) in 2D or
+(nx, 5) in 1D, k in (rho, u, v, w, p): vector<vector<StateVector>> flattened.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import native
+
+NEWTONIAN = 0
+RELATIVISTIC = 1
+SUPERBEE, VANLEER, MINBEE = 0, 1, 2
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    def __init__(self, physics: int, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4):
+        self._lib = native.load()
+        self.physics, self.dim = physics, dim
+        self.nx, self.ny = nx, (ny if dim == 2 else 1)
+        h = ctypes.c_void_p()
+        rc = self._lib.cosmos_create(ctypes.byref(h), physics, dim, nx, self.ny, adiabatic_index)
+        native.check(rc, None)
+        self._h = h
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.cosmos_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        native.check(rc, self._h)
+
+    @property
+    def shape(self):
+        return (self.ny, self.nx, 5) if self.dim == 2 else (self.nx, 5)
+
+    # -- state ---------------------------------------------------------------
+    def upload(self, prim: np.ndarray) -> None:
+        a = np.ascontiguousarray(prim, dtype=np.float64)
+        if a.size != self.nx * self.ny * 5:
+            raise ValueError(f"state has {a.size} values, grid needs {self.nx * self.ny * 5}")
+        self._check(self._lib.cosmos_upload(self._h, _dp(a)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        self._check(self._lib.cosmos_download(self._h, _dp(out)))
+        return out
+
+    def upload_device(self, ptr: int) -> None:
+        self._check(self._lib.cosmos_upload_device(self._h, ctypes.c_void_p(ptr)))
+
+    def download_device(self, ptr: int) -> None:
+        self._check(self._lib.cosmos_download_device(self._h, ctypes.c_void_p(ptr)))
+
+    def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
+        self._check(self._lib.cosmos_set_con2prim(self._h, tolerance, max_iterations))
+
+    # -- numerics ---------------------------------------------------------------
+    def sweep(self, order: int, axis: int, cell_spacing: float, time_step: float, bias: float = 0.0,
+              slope_limiter: int = VANLEER) -> None:
+        self._check(self._lib.cosmos_sweep(self._h, order, axis, cell_spacing, time_step, bias, slope_limiter))
+
+    def max_wave_speed(self) -> float:
+        v = ctypes.c_double()
+        self._check(self._lib.cosmos_max_wave_speed(self._h, ctypes.byref(v)))
+        return v.value
+
+    def stable_time_step(self, cell_spacing: float, cfl: float, current_time: float, final_time: float,
+                         current_iteration: int) -> float:
+        v = ctypes.c_double()
+        self._check(self._lib.cosmos_stable_time_step(self._h, cell_spacing, cfl, current_time, final_time,
+                                                      current_iteration, ctypes.byref(v)))
+        return v.value
+
+    def begin(self, order: int, cell_spacing: float, cfl: float, final_time: float = math.inf,
+              max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER) -> None:
+        self._check(self._lib.cosmos_begin(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
+                                           slope_limiter))
+
+    def advance(self, n_steps: int) -> None:
+        self._check(self._lib.cosmos_advance(self._h, n_steps))
+
+    def progress(self):
+        t, n, stopped = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.cosmos_progress(self._h, ctypes.byref(t), ctypes.byref(n), ctypes.byref(stopped)))
+        return t.value, n.value, bool(stopped.value)
+
+    def solve(self, order: int, cell_spacing: float, cfl: float, final_time: float = math.inf,
+              max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER):
+        """Returns (time reached, steps taken)."""
+        t, n = ctypes.c_double(), ctypes.c_int()
+        self._check(self._lib.cosmos_solve(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
+                                           slope_limiter, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def synchronize(self) -> None:
+        self._check(self._lib.cosmos_synchronize(self._h))
+
+    @property
+    def stream(self) -> int:
+        return self._lib.cosmos_stream(self._h) or 0
+
+    def state_planes(self):
+        arr = (ctypes.c_void_p * 5)()
+        pitch = ctypes.c_long()
+        self._check(self._lib.cosmos_state_planes(self._h, arr, ctypes.byref(pitch)))
+        return [arr[k] for k in range(5)], pitch.value

@@ -1,0 +1,122 @@
+"""Seeded synthetic initial conditions for the benchmark and parity workloads.
+
+Random draws use SplitMix64 with U[a, b] = a + (b - a) * (top 53 bits / 2**53),
+so the same seed gives the same grid in every language.  All states are
+primitive (rho, u, v, w, p) arrays, shape (ny, nx, 5) or (n, 5).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_MASK = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & _MASK
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK
+        return z ^ (z >> 31)
+
+    def uniform(self, a: float, b: float) -> float:
+        return a + (b - a) * ((self.next_u64() >> 11) / float(1 << 53))
+
+
+def sod_1d(n: int = 400) -> np.ndarray:
+    """Toro test 1 (Sod) on [0, 1]: (1, 0, 1) where x_i = i / n < 0.5, else (0.125, 0, 0.1)."""
+    W = np.empty((n, 5))
+    left = np.arange(n) < (n // 2)
+    W[left] = (1.0, 0.0, 0.0, 0.0, 1.0)
+    W[~left] = (0.125, 0.0, 0.0, 0.0, 0.1)
+    return W
+
+
+def quadrants_2d(n: int, seed: int, ny: int | None = None) -> np.ndarray:
+    """Four-quadrant piecewise-constant Newtonian data.  Quadrant
+    q = 2 * [i >= ny/2] + [j >= nx/2]; the states of q = 0..3 are drawn in
+    order (rho, u, v, p) with rho, p ~ U[0.1, 2], u, v ~ U[-1, 1], w = 0."""
+    nx, ny = n, (ny if ny is not None else n)
+    rng = SplitMix64(seed)
+    states = []
+    for _ in range(4):
+        rho = rng.uniform(0.1, 2.0)
+        u = rng.uniform(-1.0, 1.0)
+        v = rng.uniform(-1.0, 1.0)
+        p = rng.uniform(0.1, 2.0)
+        states.append((rho, u, v, 0.0, p))
+    W = np.empty((ny, nx, 5))
+    top = np.arange(ny)[:, None] >= ny // 2
+    right = np.arange(nx)[None, :] >= nx // 2
+    q = 2 * top.astype(int) + right.astype(int)
+    for k in range(4):
+        W[q == k] = states[k]
+    return W
+
+
+def sr_blast_2d(n: int, seed: int, vmax: float = 0.9) -> np.ndarray:
+    """Seeded special-relativistic cylindrical blast.  rho = 1 everywhere;
+    p = p_in ~ U[100, 1000] where sqrt((i - n/2)^2 + (j - n/2)^2) < 0.2 n, else 1.
+    Velocity: the sum of 4 smooth Fourier modes (integer wave numbers 1..3,
+    seeded phases, amplitudes and directions), scaled so that max |v| = vmax."""
+    rng = SplitMix64(seed)
+    p_in = rng.uniform(100.0, 1000.0)
+    i = np.arange(n)[:, None].astype(float)
+    j = np.arange(n)[None, :].astype(float)
+    x = (j + 0.5) / n
+    y = (i + 0.5) / n
+    vx = np.zeros((n, n))
+    vy = np.zeros((n, n))
+    for _ in range(4):
+        kx = 1 + int(rng.next_u64() % 3)
+        ky = 1 + int(rng.next_u64() % 3)
+        phase = rng.uniform(0.0, 2.0 * math.pi)
+        amp = rng.uniform(0.5, 1.0)
+        theta = rng.uniform(0.0, 2.0 * math.pi)
+        wave = amp * np.sin(2.0 * math.pi * (kx * x + ky * y) + phase)
+        vx += wave * math.cos(theta)
+        vy += wave * math.sin(theta)
+    speed = np.sqrt(vx * vx + vy * vy)
+    scale = vmax / speed.max()
+    W = np.empty((n, n, 5))
+    W[..., 0] = 1.0
+    W[..., 1] = vx * scale
+    W[..., 2] = vy * scale
+    W[..., 3] = 0.0
+    r = np.sqrt((i - n // 2) ** 2 + (j - n // 2) ** 2)
+    W[..., 4] = np.where(r < 0.2 * n, p_in, 1.0)
+    return W
+
+
+def sr_riemann_1d(n: int, seed: int, vmax: float = 0.8) -> np.ndarray:
+    """Seeded special-relativistic Riemann problem: left/right rho, p ~ U[0.1, 10],
+    normal velocity ~ U[-vmax, vmax], interface at the centre."""
+    rng = SplitMix64(seed)
+    W = np.empty((n, 5))
+    left = np.arange(n) < n // 2
+    for side in (left, ~left):
+        rho = rng.uniform(0.1, 10.0)
+        u = rng.uniform(-vmax, vmax)
+        p = rng.uniform(0.1, 10.0)
+        W[side] = (rho, u, 0.0, 0.0, p)
+    return W
+
+
+def random_states(count: int, seed: int, physics: int = 0) -> np.ndarray:
+    """Random physical primitive states for known-answer tests."""
+    rng = SplitMix64(seed)
+    W = np.empty((count, 5))
+    for c in range(count):
+        rho = rng.uniform(0.1, 2.0)
+        p = rng.uniform(0.1, 2.0)
+        if physics == 0:
+            vel = [rng.uniform(-1.0, 1.0) for _ in range(3)]
+        else:
+            vel = [rng.uniform(-0.5, 0.5) for _ in range(3)]
+        W[c] = (rho, vel[0], vel[1], vel[2], p)
+    return W

@@ -1,0 +1,131 @@
+/* cosmos_fv.h -- C ABI of the MI355X (gfx950) finite-volume update.
+ *
+ * The reference (Cosmos) exposes this path as static C++ class methods, with
+ * no FFI.  Each entry point below replaces one of those methods; the comment
+ * names the reference interface (file:line).  The C++ drop-in classes in
+ * cosmos_amd/dropin/ keep the reference's class names and signatures and are
+ * implemented on top of these functions.
+ *
+ * Conventions
+ *   - A solver handle owns the device state of one grid: five SoA FP64
+ *     planes of primitives (rho, u, v, w, p), the state of record as in the
+ *     reference, plus a second buffer for the sweep output.
+ *   - Host arrays are row-major AoS primitives prim[(i * nx + j) * 5 + k],
+ *     i = row (y), j = column (x): vector<vector<StateVector>> flattened.
+ *   - physics: COSMOS_NEWTONIAN (Euler/), COSMOS_RELATIVISTIC (RelativisticEuler/).
+ *   - limiter: 0 SuperBee, 1 van Leer, anything else MinBee
+ *     (SlopeLimiters::computeSlopeLimiter, Solvers/SlopeLimiters.cpp:55-69).
+ *   - Every function returns COSMOS_OK (0) or a negative code and never
+ *     throws; cosmos_last_error() describes the last failure.  Numerical
+ *     faults (NaN, con2prim cap) propagate as values, as in the reference.
+ *   - A handle is not thread-safe; separate handles are independent.  Work is
+ *     queued on the handle's HIP stream; functions that return host values
+ *     synchronise it.
+ *   - All arithmetic is FP64 with the reference's operation order; results
+ *     are bitwise equal to the reference CPU path.
+ */
+#ifndef COSMOS_FV_H
+#define COSMOS_FV_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COSMOS_OK 0
+#define COSMOS_ERR_ARGUMENT (-1)
+#define COSMOS_ERR_DEVICE (-2)
+#define COSMOS_ERR_STATE (-3)
+
+#define COSMOS_NEWTONIAN 0
+#define COSMOS_RELATIVISTIC 1
+
+typedef struct cosmos_solver cosmos_solver;
+
+/* Create a solver for an nx-by-ny grid (dim 2) or an nx-cell line (dim 1,
+ * ny ignored).  adiabatic_index plays EulerMaterialParameters
+ * (Euler/EulerMaterialParameters.cpp:3-21). */
+int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, double adiabatic_index);
+int cosmos_destroy(cosmos_solver* s);
+
+/* Message for the last failure on s (s == NULL: the last cosmos_create
+ * failure on this thread). */
+const char* cosmos_last_error(const cosmos_solver* s);
+
+/* Special-relativistic con2prim controls (default: the reference's stopping
+ * tolerance 1e-8 and cap of 1000 iterations,
+ * RelativisticEuler/RelativisticEulerStateVector.cpp:59-80). */
+int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations);
+
+/* State transfer.  *_device variants take device pointers to the same AoS
+ * layout (for callers that already hold the grid in GPU memory). */
+int cosmos_upload(cosmos_solver* s, const double* prim);
+int cosmos_download(cosmos_solver* s, double* prim);
+int cosmos_upload_device(cosmos_solver* s, const double* prim_device);
+int cosmos_download_device(cosmos_solver* s, double* prim_device);
+
+/* One dimensionally split sweep in place on the device state.
+ *   order 2, dim 2: EulerSecondOrderSolver::compute{X,Y}SLICTimeStep2D
+ *                   (Solvers/EulerSecondOrderSolver.cpp:48-100)
+ *   order 2, dim 1: EulerSecondOrderSolver::computeSLICTimeStep (:27-46)
+ *   order 1, dim 2: EulerFirstOrderSolver::compute{X,Y}FORCETimeStep2D
+ *                   (Solvers/EulerFirstOrderSolver.cpp:120-174)
+ *   order 1, dim 1: EulerFirstOrderSolver::computeFORCETimeStep (:103-118)
+ * and the RelativisticEuler* counterparts
+ * (Solvers/RelativisticEulerSecondOrderSolver.cpp, Solvers/RelativisticEulerFirstOrderSolver.cpp).
+ * axis 0 = x, 1 = y.  time_step is the sweep's step (the caller passes
+ * 0.5 * dt for the outer Strang sweeps, as solve2D does). */
+int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
+                 int slope_limiter);
+
+/* EulerSolvers::computeMaximumWaveSpeed{,2D} (Solvers/EulerSolvers.cpp:87-128),
+ * RelativisticEulerSolvers::computeMaximumWaveSpeed{,2D}
+ * (Solvers/RelativisticEulerSolvers.cpp:89-130). */
+int cosmos_max_wave_speed(cosmos_solver* s, double* out);
+
+/* EulerSolvers::computeStableTimeStep{,2D} (Solvers/EulerSolvers.cpp:130-161). */
+int cosmos_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, double current_time,
+                            double final_time, int current_iteration, double* out);
+
+/* The loop of solve / solve2D (Solvers/EulerSecondOrderSolver.cpp:133-178,
+ * Solvers/EulerFirstOrderSolver.cpp:220-265 and the RelativisticEuler*
+ * counterparts) run on the device: while t < final_time, dt from the CFL
+ * condition (with the reference's 0.2x damping of iterations 0..5 and the
+ * clamp to final_time), then X(dt/2) Y(dt) X(dt/2) in 2D or one sweep in 1D.
+ *   cosmos_begin   resets t and the iteration counter (solve starts at 0)
+ *   cosmos_advance queues up to n more steps, asynchronously
+ *   cosmos_progress synchronises and reports t, steps taken and whether the
+ *                  loop has stopped (t >= final_time or max_steps reached)
+ * cosmos_solve = begin + advance until stopped.  max_steps < 0 means no cap;
+ * final_time = +inf with a cap gives a fixed number of steps. */
+int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
+                 double bias, int slope_limiter);
+int cosmos_advance(cosmos_solver* s, int n_steps);
+int cosmos_progress(cosmos_solver* s, double* time, int* steps, int* stopped);
+int cosmos_solve(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
+                 double bias, int slope_limiter, double* time_out, int* steps_out);
+
+/* Wait for all queued work on the handle's stream. */
+int cosmos_synchronize(cosmos_solver* s);
+/* The handle's HIP stream (hipStream_t), for callers that time or order work. */
+void* cosmos_stream(cosmos_solver* s);
+/* Instrumentation (no counterpart in the reference): when enabled, every
+ * kernel queued by cosmos_advance is bracketed by HIP events on the handle's
+ * stream.  cosmos_profile_read synchronises and returns, for kinds
+ * 0 step head (dt), 1 x sweep, 2 y sweep, 3 x sweep with the fused wave-speed
+ * maximum, the summed milliseconds and launch counts since the last read. */
+int cosmos_profile_enable(cosmos_solver* s, int on);
+int cosmos_profile_read(cosmos_solver* s, double* total_ms4, int* launches4);
+
+/* A plain device copy of `bytes` (a multiple of 16, 16-byte aligned) on a HIP
+ * stream (NULL: the default stream), for bandwidth and counter calibration
+ * (no counterpart in the reference). */
+int cosmos_device_copy(void* dst, const void* src, long bytes, void* stream);
+
+/* Device pointers of the current state planes (rho, u, v, w, p) and their
+ * row pitch in elements; valid until the next sweep, step or destroy. */
+int cosmos_state_planes(cosmos_solver* s, double** planes5, long* pitch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

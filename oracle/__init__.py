@@ -1,0 +1,174 @@
+"""CPU checkers for the finite-volume path (TEST INFRASTRUCTURE ONLY).
+
+    load_oracle()  the C restatement, oracle/liboracle.so (orc_* functions)
+    load_ref()     the reference's own sources built by oracle/Makefile,
+                   oracle/_ref/libcosmos_ref.so (ref_* functions)
+
+Both expose the same signatures (see cosmos_oracle.h); wrap() gives them a
+common numpy-facing interface.  Only tests, the golden-fixture script,
+__graft_entry__.smoke() and the CPU-baseline leg of bench.py use this module,
+as the checker; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libcosmos_ref.so")
+REFERENCE_ROOT = "/root/reference"
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.POINTER(ctypes.c_double)
+
+_SIGS = {
+    "sweep": (_I, [_I, _I, _I, _I, _P, _I, _I, _D, _D, _D, _I, _D]),
+    "max_wavespeed": (_D, [_I, _I, _P, _I, _I, _D]),
+    "stable_dt": (_D, [_I, _I, _P, _I, _I, _D, _D, _D, _D, _I, _D]),
+    "stable_dt_scalar": (_D, [_D, _D, _D, _I]),
+    "run": (_I, [_I, _I, _I, _P, _I, _I, _D, _D, _D, _I, _D, _I, _D, _P]),
+    "slic_flux": (None, [_I, _I, _P, _P, _P, _P, _D, _D, _D, _I, _D, _P]),
+    "force_flux": (None, [_I, _I, _P, _P, _D, _D, _D, _P]),
+    "half_step": (None, [_I, _I, _P, _P, _P, _D, _D, _D, _I, _I, _D, _P]),
+    "p2c": (None, [_I, _P, _P, _D]),
+    "c2p": (None, [_I, _P, _P, _D]),
+    "flux": (None, [_I, _I, _P, _P, _D]),
+    "limiter": (_D, [_D, _D, _I]),
+    "slope": (None, [_P, _P, _P, _D, _I, _P]),
+}
+
+
+def build(ref: bool = True) -> None:
+    """Build liboracle.so, and _ref/ when the reference sources are present."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if ref and os.path.isdir(REFERENCE_ROOT):
+        subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_P)
+
+
+class CpuPath:
+    """numpy-facing wrapper of liboracle.so (prefix orc_) or the reference build (ref_)."""
+
+    def __init__(self, path: str, prefix: str):
+        self.path, self.prefix = path, prefix
+        self.lib = ctypes.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(self.lib, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+        if prefix == "orc_":
+            self.lib.orc_set_c2p.argtypes = [_D, _I]
+            self.lib.orc_set_threads.argtypes = [_I]
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    @staticmethod
+    def _dims(W):
+        if W.ndim == 2:
+            return 1, W.shape[0], 1
+        return 2, W.shape[1], W.shape[0]
+
+    def sweep(self, W, physics, order, axis, dx, dt, bias=0.0, limiter=1, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dim, nx, ny = self._dims(a)
+        rc = self._f("sweep")(physics, order, axis, dim, _dp(a), nx, ny, dx, dt, bias, limiter, gamma)
+        if rc:
+            raise ValueError("sweep rejected its arguments")
+        return a
+
+    def max_wavespeed(self, W, physics, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        return self._f("max_wavespeed")(physics, dim, _dp(a), nx, ny, gamma)
+
+    def stable_dt(self, W, physics, dx, cfl, t, tf, iteration, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        return self._f("stable_dt")(physics, dim, _dp(a), nx, ny, dx, cfl, t, tf, iteration, gamma)
+
+    def stable_dt_scalar(self, dt, t, tf, iteration):
+        return self._f("stable_dt_scalar")(dt, t, tf, iteration)
+
+    def run(self, W, physics, order, dx, cfl, tf=math.inf, max_steps=2 ** 31 - 1, bias=0.0, limiter=1,
+            gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dim, nx, ny = self._dims(a)
+        t = _D()
+        n = self._f("run")(physics, order, dim, _dp(a), nx, ny, dx, cfl, tf, max_steps, bias, limiter, gamma,
+                           ctypes.byref(t))
+        return a, t.value, n
+
+    def slic_flux(self, physics, axis, ll, l, r, rr, dx, dt, bias=0.0, limiter=1, gamma=1.4):
+        out = np.empty(5)
+        args = [np.ascontiguousarray(x, dtype=np.float64) for x in (ll, l, r, rr)]
+        self._f("slic_flux")(physics, axis, *[_dp(x) for x in args], dx, dt, bias, limiter, gamma, _dp(out))
+        return out
+
+    def force_flux(self, physics, axis, l, r, dx, dt, gamma=1.4):
+        out = np.empty(5)
+        a, b = (np.ascontiguousarray(x, dtype=np.float64) for x in (l, r))
+        self._f("force_flux")(physics, axis, _dp(a), _dp(b), dx, dt, gamma, _dp(out))
+        return out
+
+    def half_step(self, physics, axis, l, m, r, dx, dt, bias, limiter, side, gamma=1.4):
+        out = np.empty(5)
+        a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (l, m, r))
+        self._f("half_step")(physics, axis, _dp(a), _dp(b), _dp(c), dx, dt, bias, limiter, side, gamma, _dp(out))
+        return out
+
+    def p2c(self, physics, w, gamma=1.4):
+        out = np.empty(5)
+        a = np.ascontiguousarray(w, dtype=np.float64)
+        self._f("p2c")(physics, _dp(a), _dp(out), gamma)
+        return out
+
+    def c2p(self, physics, u, gamma=1.4):
+        out = np.empty(5)
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        self._f("c2p")(physics, _dp(a), _dp(out), gamma)
+        return out
+
+    def flux(self, physics, axis, u, gamma=1.4):
+        out = np.empty(5)
+        a = np.ascontiguousarray(u, dtype=np.float64)
+        self._f("flux")(physics, axis, _dp(a), _dp(out), gamma)
+        return out
+
+    def limiter(self, r, bias, limiter):
+        return self._f("limiter")(r, bias, limiter)
+
+    def slope(self, ul, um, ur, bias, limiter):
+        out = np.empty(5)
+        a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (ul, um, ur))
+        self._f("slope")(_dp(a), _dp(b), _dp(c), bias, limiter, _dp(out))
+        return out
+
+    # oracle-only controls
+    def set_con2prim(self, tol, max_iterations):
+        self.lib.orc_set_c2p(tol, max_iterations)
+
+    def set_threads(self, n):
+        self.lib.orc_set_threads(n)
+
+
+def load_oracle() -> CpuPath:
+    if not os.path.exists(ORACLE_SO):
+        raise FileNotFoundError(f"{ORACLE_SO} missing: run `make -C oracle`")
+    return CpuPath(ORACLE_SO, "orc_")
+
+
+def load_ref() -> CpuPath | None:
+    """The reference build, or None where it has not been built."""
+    if not os.path.exists(REF_SO):
+        return None
+    return CpuPath(REF_SO, "ref_")

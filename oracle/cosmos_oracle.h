@@ -1,0 +1,51 @@
+/* CPU restatement of the Cosmos finite-volume arithmetic (test infrastructure).
+ *
+ * Only tests, the golden-fixture scripts and the CPU-baseline leg of bench.py
+ * load this library, as the checker.  The product path never calls it.
+ *
+ * States are flat row-major AoS arrays of primitives,
+ * prim[(i * nx + j) * 5 + k], i = row (y), j = column (x),
+ * k in (rho, u, v, w, p).  physics: 0 = Newtonian, 1 = special relativistic.
+ * Every function has the same signature as its ref_* counterpart in
+ * ref_driver.cpp so that the two can be compared call for call.
+ */
+#ifndef COSMOS_ORACLE_H
+#define COSMOS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int orc_sweep(int physics, int order, int axis, int dim, double* prim, int nx, int ny, double dx,
+              double dt, double bias, int limiter, double gamma);
+double orc_max_wavespeed(int physics, int dim, const double* prim, int nx, int ny, double gamma);
+double orc_stable_dt(int physics, int dim, const double* prim, int nx, int ny, double dx, double cfl,
+                     double t, double tf, int iteration, double gamma);
+double orc_stable_dt_scalar(double dt, double t, double tf, int iteration);
+int orc_run(int physics, int order, int dim, double* prim, int nx, int ny, double dx, double cfl,
+            double tf, int max_steps, double bias, int limiter, double gamma, double* t_out);
+void orc_slic_flux(int physics, int axis, const double* ll, const double* l, const double* r,
+                   const double* rr, double dx, double dt, double bias, int limiter, double gamma,
+                   double* out);
+void orc_force_flux(int physics, int axis, const double* l, const double* r, double dx, double dt,
+                    double gamma, double* out);
+void orc_half_step(int physics, int axis, const double* l, const double* m, const double* r, double dx,
+                   double dt, double bias, int limiter, int side, double gamma, double* out);
+void orc_p2c(int physics, const double* w, double* u, double gamma);
+void orc_c2p(int physics, const double* u, double* w, double gamma);
+void orc_flux(int physics, int axis, const double* u, double* f, double gamma);
+double orc_limiter(double r, double bias, int limiter);
+void orc_slope(const double* ul, const double* um, const double* ur, double bias, int limiter, double* out);
+
+/* Special-relativistic con2prim Newton controls.  The reference hard-codes a
+ * stopping tolerance of 1e-8 and at most 1000 iterations
+ * (RelativisticEuler/RelativisticEulerStateVector.cpp:59-80); other values
+ * select the tighter "config mode".  Returns the previous settings. */
+void orc_set_c2p(double tol, int max_iterations);
+/* Number of OpenMP threads used by sweeps (0 = runtime default). */
+void orc_set_threads(int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,340 @@
+// Flat-array driver around the reference's own solver classes.
+//
+// This file is test infrastructure: it is compiled together with the
+// reference's unmodified C++ sources (see oracle/Makefile) into
+// oracle/_ref/libcosmos_ref.so, and only tests, golden-fixture scripts and the
+// CPU-baseline leg of bench.py load it.  It never ships in the product path.
+//
+// Every entry point takes primitive states as a flat row-major AoS array
+// prim[(i * nx + j) * 5 + k], i = row (y), j = column (x), k in
+// (rho, u, v, w, p), converts them to the reference's state-vector objects,
+// calls the reference's static method, and converts back.  Physics codes:
+// 0 = Newtonian (Euler/), 1 = special relativistic (RelativisticEuler/).
+
+#include <cstring>
+#include <iostream>
+#include <sstream>
+#include <vector>
+
+#include "Solvers/EulerFirstOrderSolver.hpp"
+#include "Solvers/EulerSecondOrderSolver.hpp"
+#include "Solvers/RelativisticEulerFirstOrderSolver.hpp"
+#include "Solvers/RelativisticEulerSecondOrderSolver.hpp"
+
+namespace {
+
+// The reference's solve/solve2D print one status line per iteration
+// (Solvers/EulerSolvers.cpp:266-269); silence it while a driver call runs.
+struct QuietCout {
+    std::ostringstream sink;
+    std::streambuf* saved;
+    QuietCout() : saved(std::cout.rdbuf(sink.rdbuf())) {}
+    ~QuietCout() { std::cout.rdbuf(saved); }
+};
+
+struct NewtonianTraits {
+    typedef EulerStateVector State;
+    typedef EulerSolvers Helpers;
+    typedef EulerFirstOrderSolver First;
+    typedef EulerSecondOrderSolver Second;
+};
+
+struct RelativisticTraits {
+    typedef RelativisticEulerStateVector State;
+    typedef RelativisticEulerSolvers Helpers;
+    typedef RelativisticEulerFirstOrderSolver First;
+    typedef RelativisticEulerSecondOrderSolver Second;
+};
+
+template <class S>
+S make_state(const double* w) { return S(w[0], w[1], w[2], w[3], w[4]); }
+
+template <class S>
+void store_state(S& s, double* w) {
+    w[0] = s.getDensity(); w[1] = s.getXVelocity(); w[2] = s.getYVelocity();
+    w[3] = s.getZVelocity(); w[4] = s.getPressure();
+}
+
+template <class S>
+std::vector<S> to_1d(const double* prim, int n) {
+    std::vector<S> cells;
+    cells.reserve(n);
+    for (int j = 0; j < n; ++j) cells.push_back(make_state<S>(prim + 5 * j));
+    return cells;
+}
+
+template <class S>
+void from_1d(std::vector<S>& cells, double* prim) {
+    for (size_t j = 0; j < cells.size(); ++j) store_state(cells[j], prim + 5 * j);
+}
+
+template <class S>
+std::vector<std::vector<S> > to_2d(const double* prim, int nx, int ny) {
+    std::vector<std::vector<S> > cells(ny);
+    for (int i = 0; i < ny; ++i) cells[i] = to_1d<S>(prim + 5 * (size_t)i * nx, nx);
+    return cells;
+}
+
+template <class S>
+void from_2d(std::vector<std::vector<S> >& cells, double* prim) {
+    size_t nx = cells[0].size();
+    for (size_t i = 0; i < cells.size(); ++i) from_1d(cells[i], prim + 5 * i * nx);
+}
+
+template <class T>
+void sweep(int order, int axis, int dim, double* prim, int nx, int ny, double dx, double dt,
+           double bias, int limiter, EulerMaterialParameters mp) {
+    typedef typename T::State S;
+    if (dim == 1) {
+        std::vector<S> cells = to_1d<S>(prim, nx);
+        if (order == 1) T::First::computeFORCETimeStep(cells, dx, dt, mp);
+        else T::Second::computeSLICTimeStep(cells, dx, dt, bias, limiter, mp);
+        from_1d(cells, prim);
+        return;
+    }
+    std::vector<std::vector<S> > cells = to_2d<S>(prim, nx, ny);
+    if (order == 1) {
+        if (axis == 0) T::First::computeXFORCETimeStep2D(cells, dx, dt, mp);
+        else T::First::computeYFORCETimeStep2D(cells, dx, dt, mp);
+    } else {
+        if (axis == 0) T::Second::computeXSLICTimeStep2D(cells, dx, dt, bias, limiter, mp);
+        else T::Second::computeYSLICTimeStep2D(cells, dx, dt, bias, limiter, mp);
+    }
+    from_2d(cells, prim);
+}
+
+template <class T>
+double max_wavespeed(int dim, const double* prim, int nx, int ny, EulerMaterialParameters mp) {
+    typedef typename T::State S;
+    if (dim == 1) {
+        std::vector<S> cells = to_1d<S>(prim, nx);
+        return T::Helpers::computeMaximumWaveSpeed(cells, mp);
+    }
+    std::vector<std::vector<S> > cells = to_2d<S>(prim, nx, ny);
+    return T::Helpers::computeMaximumWaveSpeed2D(cells, mp);
+}
+
+template <class T>
+double stable_dt(int dim, const double* prim, int nx, int ny, double dx, double cfl, double t,
+                 double tf, int iteration, EulerMaterialParameters mp) {
+    typedef typename T::State S;
+    if (dim == 1) {
+        std::vector<S> cells = to_1d<S>(prim, nx);
+        return T::Helpers::computeStableTimeStep(cells, dx, cfl, t, tf, iteration, mp);
+    }
+    std::vector<std::vector<S> > cells = to_2d<S>(prim, nx, ny);
+    return T::Helpers::computeStableTimeStep2D(cells, dx, cfl, t, tf, iteration, mp);
+}
+
+// The loop of solve/solve2D (Solvers/EulerSecondOrderSolver.cpp:133-178 and
+// Solvers/EulerFirstOrderSolver.cpp:220-265), capped at max_steps iterations
+// so that a fixed step count can be run (pass finalTime = +inf for no cap).
+template <class T>
+int run(int order, int dim, double* prim, int nx, int ny, double dx, double cfl, double tf,
+        int max_steps, double bias, int limiter, EulerMaterialParameters mp, double* t_out) {
+    typedef typename T::State S;
+    double t = 0.0;
+    int it = 0;
+    if (dim == 1) {
+        std::vector<S> cells = to_1d<S>(prim, nx);
+        while (t < tf && it < max_steps) {
+            double dt = T::Helpers::computeStableTimeStep(cells, dx, cfl, t, tf, it, mp);
+            if (order == 1) T::First::computeFORCETimeStep(cells, dx, dt, mp);
+            else T::Second::computeSLICTimeStep(cells, dx, dt, bias, limiter, mp);
+            t += dt;
+            it += 1;
+        }
+        from_1d(cells, prim);
+    } else {
+        std::vector<std::vector<S> > cells = to_2d<S>(prim, nx, ny);
+        while (t < tf && it < max_steps) {
+            double dt = T::Helpers::computeStableTimeStep2D(cells, dx, cfl, t, tf, it, mp);
+            if (order == 1) {
+                T::First::computeXFORCETimeStep2D(cells, dx, dt * 0.5, mp);
+                T::First::computeYFORCETimeStep2D(cells, dx, dt, mp);
+                T::First::computeXFORCETimeStep2D(cells, dx, dt * 0.5, mp);
+            } else {
+                T::Second::computeXSLICTimeStep2D(cells, dx, 0.5 * dt, bias, limiter, mp);
+                T::Second::computeYSLICTimeStep2D(cells, dx, dt, bias, limiter, mp);
+                T::Second::computeXSLICTimeStep2D(cells, dx, 0.5 * dt, bias, limiter, mp);
+            }
+            t += dt;
+            it += 1;
+        }
+        from_2d(cells, prim);
+    }
+    if (t_out) *t_out = t;
+    return it;
+}
+
+// The reference's own solve/solve2D, run to finalTime.
+template <class T>
+void solve(int order, int dim, double* prim, int nx, int ny, double dx, double cfl, double tf,
+           double bias, int limiter, EulerMaterialParameters mp) {
+    typedef typename T::State S;
+    QuietCout quiet;
+    if (dim == 1) {
+        std::vector<S> cells = to_1d<S>(prim, nx);
+        std::vector<S> out = (order == 1) ? T::First::solve(cells, dx, cfl, tf, mp)
+                                          : T::Second::solve(cells, dx, cfl, tf, bias, limiter, mp);
+        from_1d(out, prim);
+    } else {
+        std::vector<std::vector<S> > cells = to_2d<S>(prim, nx, ny);
+        std::vector<std::vector<S> > out =
+            (order == 1) ? T::First::solve2D(cells, dx, cfl, tf, mp)
+                         : T::Second::solve2D(cells, dx, cfl, tf, bias, limiter, mp);
+        from_2d(out, prim);
+    }
+}
+
+template <class T>
+void slic_flux(int axis, const double* ll, const double* l, const double* r, const double* rr,
+               double dx, double dt, double bias, int limiter, EulerMaterialParameters mp,
+               double* out) {
+    typedef typename T::State S;
+    std::vector<double> f =
+        axis == 0 ? T::Second::computeXSLICFlux(make_state<S>(ll), make_state<S>(l), make_state<S>(r),
+                                                make_state<S>(rr), dx, dt, bias, limiter, mp)
+                  : T::Second::computeYSLICFlux(make_state<S>(ll), make_state<S>(l), make_state<S>(r),
+                                                make_state<S>(rr), dx, dt, bias, limiter, mp);
+    for (int k = 0; k < 5; ++k) out[k] = f[k];
+}
+
+template <class T>
+void force_flux(int axis, const double* l, const double* r, double dx, double dt,
+                EulerMaterialParameters mp, double* out) {
+    typedef typename T::State S;
+    std::vector<double> f = axis == 0 ? T::First::computeXFORCEFlux(make_state<S>(l), make_state<S>(r), dx, dt, mp)
+                                      : T::First::computeYFORCEFlux(make_state<S>(l), make_state<S>(r), dx, dt, mp);
+    for (int k = 0; k < 5; ++k) out[k] = f[k];
+}
+
+template <class T>
+void half_step(int axis, const double* l, const double* m, const double* r, double dx, double dt,
+               double bias, int limiter, int side, EulerMaterialParameters mp, double* out) {
+    typedef typename T::State S;
+    S s = axis == 0 ? T::Helpers::evolveStateByHalfXTimeStep(make_state<S>(l), make_state<S>(m), make_state<S>(r),
+                                                             dx, dt, bias, limiter, side, mp)
+                    : T::Helpers::evolveStateByHalfYTimeStep(make_state<S>(l), make_state<S>(m), make_state<S>(r),
+                                                             dx, dt, bias, limiter, side, mp);
+    store_state(s, out);
+}
+
+template <class S>
+void p2c(const double* w, double* u, EulerMaterialParameters mp) {
+    S s = make_state<S>(w);
+    std::vector<double> c = s.computeConservedVariableVector(mp);
+    for (int k = 0; k < 5; ++k) u[k] = c[k];
+}
+
+template <class S>
+void c2p(const double* u, double* w, EulerMaterialParameters mp) {
+    S s;
+    s.setConservedVariableVector(std::vector<double>(u, u + 5), mp);
+    store_state(s, w);
+}
+
+template <class S>
+void flux(int axis, const double* u, double* f, EulerMaterialParameters mp) {
+    std::vector<double> c(u, u + 5);
+    std::vector<double> r = axis == 0 ? S::computeXFluxVector(c, mp) : S::computeYFluxVector(c, mp);
+    for (int k = 0; k < 5; ++k) f[k] = r[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_sweep(int physics, int order, int axis, int dim, double* prim, int nx, int ny, double dx,
+              double dt, double bias, int limiter, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) sweep<NewtonianTraits>(order, axis, dim, prim, nx, ny, dx, dt, bias, limiter, mp);
+    else sweep<RelativisticTraits>(order, axis, dim, prim, nx, ny, dx, dt, bias, limiter, mp);
+    return 0;
+}
+
+double ref_max_wavespeed(int physics, int dim, const double* prim, int nx, int ny, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    return physics == 0 ? max_wavespeed<NewtonianTraits>(dim, prim, nx, ny, mp)
+                        : max_wavespeed<RelativisticTraits>(dim, prim, nx, ny, mp);
+}
+
+double ref_stable_dt(int physics, int dim, const double* prim, int nx, int ny, double dx, double cfl,
+                     double t, double tf, int iteration, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    return physics == 0 ? stable_dt<NewtonianTraits>(dim, prim, nx, ny, dx, cfl, t, tf, iteration, mp)
+                        : stable_dt<RelativisticTraits>(dim, prim, nx, ny, dx, cfl, t, tf, iteration, mp);
+}
+
+double ref_stable_dt_scalar(double dt, double t, double tf, int iteration) {
+    return EulerSolvers::computeStableTimeStep(dt, t, tf, iteration);
+}
+
+int ref_run(int physics, int order, int dim, double* prim, int nx, int ny, double dx, double cfl,
+            double tf, int max_steps, double bias, int limiter, double gamma, double* t_out) {
+    EulerMaterialParameters mp(gamma);
+    return physics == 0
+               ? run<NewtonianTraits>(order, dim, prim, nx, ny, dx, cfl, tf, max_steps, bias, limiter, mp, t_out)
+               : run<RelativisticTraits>(order, dim, prim, nx, ny, dx, cfl, tf, max_steps, bias, limiter, mp, t_out);
+}
+
+int ref_solve(int physics, int order, int dim, double* prim, int nx, int ny, double dx, double cfl,
+              double tf, double bias, int limiter, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) solve<NewtonianTraits>(order, dim, prim, nx, ny, dx, cfl, tf, bias, limiter, mp);
+    else solve<RelativisticTraits>(order, dim, prim, nx, ny, dx, cfl, tf, bias, limiter, mp);
+    return 0;
+}
+
+void ref_slic_flux(int physics, int axis, const double* ll, const double* l, const double* r,
+                   const double* rr, double dx, double dt, double bias, int limiter, double gamma,
+                   double* out) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) slic_flux<NewtonianTraits>(axis, ll, l, r, rr, dx, dt, bias, limiter, mp, out);
+    else slic_flux<RelativisticTraits>(axis, ll, l, r, rr, dx, dt, bias, limiter, mp, out);
+}
+
+void ref_force_flux(int physics, int axis, const double* l, const double* r, double dx, double dt,
+                    double gamma, double* out) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) force_flux<NewtonianTraits>(axis, l, r, dx, dt, mp, out);
+    else force_flux<RelativisticTraits>(axis, l, r, dx, dt, mp, out);
+}
+
+void ref_half_step(int physics, int axis, const double* l, const double* m, const double* r, double dx,
+                   double dt, double bias, int limiter, int side, double gamma, double* out) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) half_step<NewtonianTraits>(axis, l, m, r, dx, dt, bias, limiter, side, mp, out);
+    else half_step<RelativisticTraits>(axis, l, m, r, dx, dt, bias, limiter, side, mp, out);
+}
+
+void ref_p2c(int physics, const double* w, double* u, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) p2c<EulerStateVector>(w, u, mp);
+    else p2c<RelativisticEulerStateVector>(w, u, mp);
+}
+
+void ref_c2p(int physics, const double* u, double* w, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) c2p<EulerStateVector>(u, w, mp);
+    else c2p<RelativisticEulerStateVector>(u, w, mp);
+}
+
+void ref_flux(int physics, int axis, const double* u, double* f, double gamma) {
+    EulerMaterialParameters mp(gamma);
+    if (physics == 0) flux<EulerStateVector>(axis, u, f, mp);
+    else flux<RelativisticEulerStateVector>(axis, u, f, mp);
+}
+
+double ref_limiter(double r, double bias, int limiter) {
+    return SlopeLimiters::computeSlopeLimiter(r, bias, limiter);
+}
+
+void ref_slope(const double* ul, const double* um, const double* ur, double bias, int limiter, double* out) {
+    std::vector<double> s = SlopeLimiters::computeSlopeVector(std::vector<double>(ul, ul + 5),
+                                                              std::vector<double>(um, um + 5),
+                                                              std::vector<double>(ur, ur + 5), bias, limiter);
+    for (int k = 0; k < 5; ++k) out[k] = s[k];
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library through the C-ABI)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import load_oracle
+
+    return load_oracle()
+
+
+@pytest.fixture(scope="session")
+def ref():
+    from oracle import load_ref
+
+    r = load_ref()
+    if r is None:
+        pytest.skip("reference build oracle/_ref not present")
+    return r
+
+
+def golden(name: str):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def same(a, b) -> bool:
+    """Bitwise-equal values (NaN at the same places counts as equal)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return a.shape == b.shape and bool(np.array_equal(a, b, equal_nan=True))
+
+
+def rel_linf(a, b) -> float:
+    """Largest |a - b| over the largest |b|, per component, maximised."""
+    a = np.asarray(a).reshape(-1, 5)
+    b = np.asarray(b).reshape(-1, 5)
+    worst = 0.0
+    for k in range(5):
+        scale = np.max(np.abs(b[:, k]))
+        diff = np.max(np.abs(a[:, k] - b[:, k]))
+        if scale > 0:
+            worst = max(worst, diff / scale)
+        elif diff > 0:
+            worst = max(worst, np.inf)
+    return worst

@@ -1,0 +1,120 @@
+"""Writes the golden fixtures in tests/golden/*.npz from the reference's own
+sources, compiled by oracle/Makefile into oracle/_ref/libcosmos_ref.so.
+
+Run in a container that has the reference checked out (it is not needed
+afterwards; the fixtures are committed):
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Every fixture stores inputs and expected outputs as raw float64 arrays.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from cosmos_amd import workloads  # noqa: E402
+from oracle import load_ref  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def main() -> None:
+    ref = load_ref()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libcosmos_ref.so is not built (make -C oracle ref)")
+
+    # 1. Toro test 1 (Sod), 400 cells, SLIC + van Leer, CFL 0.8, to t = 0.25 through the
+    #    reference's own EulerSecondOrderSolver::solve, and the same with FORCE.
+    W0 = workloads.sod_1d(400)
+    fx = {"input": W0}
+    D, I, P = ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    ref.lib.ref_solve.argtypes = [I, I, I, P, I, I, D, D, D, D, I, D]
+    for order in (1, 2):
+        a, t, n = ref.run(W0, 0, order, 1.0 / 400, 0.8, tf=0.25)
+        # the driver loop must agree with the reference's own solve()
+        s = W0.copy()
+        ref.lib.ref_solve(0, order, 1, s.ctypes.data_as(P), 400, 1, 1.0 / 400, 0.8, 0.25, 0.0, 1, 1.4)
+        assert np.array_equal(s, a), "driver loop differs from EulerSecondOrderSolver::solve"
+        fx[f"order{order}_output"] = a
+        fx[f"order{order}_time"] = np.array(t)
+        fx[f"order{order}_steps"] = np.array(n)
+    np.savez_compressed(os.path.join(OUT, "sod_1d.npz"), **fx)
+
+    # 2. 2D Newtonian four-quadrant problem, 24x20 cells (ragged), 3 Strang steps, every
+    #    limiter and both orders, CFL 0.9.
+    fx = {}
+    W0 = workloads.quadrants_2d(24, seed=0, ny=20)
+    fx["input"] = W0
+    for order in (1, 2):
+        for lim in ((0, 1, 2) if order == 2 else (1,)):
+            a, t, n = ref.run(W0, 0, order, 1.0 / 24, 0.9, max_steps=3, limiter=lim)
+            fx[f"o{order}_l{lim}_output"] = a
+            fx[f"o{order}_l{lim}_time"] = np.array(t)
+    # biased SLIC (bias 0.3, van Leer) to cover the general limiter path
+    a, t, n = ref.run(W0, 0, 2, 1.0 / 24, 0.9, max_steps=3, bias=0.3, limiter=1)
+    fx["o2_l1_bias03_output"] = a
+    np.savez_compressed(os.path.join(OUT, "newtonian_2d.npz"), **fx)
+
+    # 3. Special relativistic: a 1D Riemann problem (200 cells, 20 steps, CFL 0.5) and a
+    #    2D blast (20x20, 3 steps), gamma 5/3.
+    fx = {}
+    W1 = workloads.sr_riemann_1d(200, seed=3)
+    fx["riemann_input"] = W1
+    for order in (1, 2):
+        a, t, n = ref.run(W1, 1, order, 1.0 / 200, 0.5, max_steps=20, gamma=5.0 / 3.0)
+        fx[f"riemann_o{order}_output"] = a
+        fx[f"riemann_o{order}_time"] = np.array(t)
+    W2 = workloads.sr_blast_2d(20, seed=4, vmax=0.9)
+    fx["blast_input"] = W2
+    a, t, n = ref.run(W2, 1, 2, 1.0 / 20, 0.5, max_steps=3, gamma=5.0 / 3.0)
+    fx["blast_o2_output"] = a
+    fx["blast_o2_time"] = np.array(t)
+    np.savez_compressed(os.path.join(OUT, "relativistic.npz"), **fx)
+
+    # 4. Known answers of single calls.
+    fx = {}
+    r = np.concatenate([np.linspace(-2.0, 4.0, 121), [0.0, 0.5, 1.0, 2.0, 1e300, -1e-300, 1e-300]])
+    fx["limiter_r"] = r
+    for lim in (0, 1, 2):
+        for bias, tag in ((0.0, "0"), (0.3, "p3"), (-0.5, "m5")):
+            fx[f"limiter_{lim}_{tag}"] = np.array([ref.limiter(x, bias, lim) for x in r])
+    for phys, g in ((0, 1.4), (1, 5.0 / 3.0)):
+        Ws = workloads.random_states(64, seed=10 + phys, physics=phys)
+        fx[f"p{phys}_states"] = Ws
+        fx[f"p{phys}_p2c"] = np.array([ref.p2c(phys, w, g) for w in Ws])
+        Us = fx[f"p{phys}_p2c"]
+        fx[f"p{phys}_c2p"] = np.array([ref.c2p(phys, u, g) for u in Us])
+        fx[f"p{phys}_fluxx"] = np.array([ref.flux(phys, 0, u, g) for u in Us])
+        fx[f"p{phys}_fluxy"] = np.array([ref.flux(phys, 1, u, g) for u in Us])
+        sl = []
+        hs = []
+        for k in range(0, 60, 4):
+            for axis in (0, 1):
+                sl.append(ref.slic_flux(phys, axis, Ws[k], Ws[k + 1], Ws[k + 2], Ws[k + 3], 0.01, 0.004, 0.0, 1, g))
+                hs.append(ref.half_step(phys, axis, Ws[k], Ws[k + 1], Ws[k + 2], 0.01, 0.004, 0.0, 1, k % 2, g))
+        fx[f"p{phys}_slic_flux"] = np.array(sl)
+        fx[f"p{phys}_half_step"] = np.array(hs)
+        fx[f"p{phys}_force_flux"] = np.array(
+            [ref.force_flux(phys, k % 2, Ws[k], Ws[k + 1], 0.01, 0.004, g) for k in range(0, 60, 2)])
+    U = workloads.random_states(30, seed=20, physics=0)
+    fx["slope_inputs"] = U
+    fx["slope_out"] = np.array([ref.slope(U[k], U[k + 1], U[k + 2], b, lim)
+                                for k in range(0, 27, 3) for lim in (0, 1, 2) for b in (0.0, 0.3)])
+    fx["dt_scalar"] = np.array([ref.stable_dt_scalar(0.01, t, tf, it)
+                                for (t, tf, it) in ((0.0, 1.0, 0), (0.5, 1.0, 5), (0.5, 1.0, 6), (0.995, 1.0, 7),
+                                                    (0.0, math.inf, 3))])
+    np.savez_compressed(os.path.join(OUT, "single_calls.npz"), **fx)
+    for f in sorted(os.listdir(OUT)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(OUT, f)))
+
+
+if __name__ == "__main__":
+    main()

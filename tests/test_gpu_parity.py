@@ -1,0 +1,202 @@
+"""GPU parity: the HIP library (through the C-ABI) against the CPU oracle on
+the same seeded inputs.  The bar is bitwise equality of every primitive
+value (the kernels keep the reference's IEEE operation order), which is
+stronger than the 1e-11 (Newtonian) / 1e-9 (SR) relative bound."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, rel_linf, same
+from cosmos_amd import NEWTONIAN, RELATIVISTIC, Solver, workloads
+
+pytestmark = pytest.mark.gpu
+
+G_N, G_SR = 1.4, 5.0 / 3.0
+
+
+def gpu_run(W, physics, order, dx, cfl, tf=math.inf, max_steps=-1, bias=0.0, limiter=1, gamma=G_N):
+    dim = 1 if W.ndim == 2 else 2
+    nx = W.shape[-2]
+    ny = W.shape[0] if dim == 2 else 1
+    with Solver(physics, dim, nx, ny, gamma) as s:
+        s.upload(W)
+        t, n = s.solve(order, dx, cfl, tf, max_steps, bias, limiter)
+        return s.download(), t, n
+
+
+def test_sod_config1_matches_golden():
+    g = golden("sod_1d.npz")
+    for order in (1, 2):
+        out, t, n = gpu_run(g["input"], NEWTONIAN, order, 1.0 / 400, 0.8, tf=0.25)
+        assert same(out, g[f"order{order}_output"]), rel_linf(out, g[f"order{order}_output"])
+        assert t == float(g[f"order{order}_time"]) and n == int(g[f"order{order}_steps"])
+
+
+def test_newtonian_2d_matches_golden():
+    g = golden("newtonian_2d.npz")
+    for order in (1, 2):
+        for lim in ((0, 1, 2) if order == 2 else (1,)):
+            out, t, n = gpu_run(g["input"], NEWTONIAN, order, 1.0 / 24, 0.9, max_steps=3, limiter=lim)
+            assert same(out, g[f"o{order}_l{lim}_output"]), (order, lim)
+            assert t == float(g[f"o{order}_l{lim}_time"])
+    out, _, _ = gpu_run(g["input"], NEWTONIAN, 2, 1.0 / 24, 0.9, max_steps=3, bias=0.3, limiter=1)
+    assert same(out, g["o2_l1_bias03_output"])
+
+
+def test_relativistic_matches_golden():
+    g = golden("relativistic.npz")
+    for order in (1, 2):
+        out, t, _ = gpu_run(g["riemann_input"], RELATIVISTIC, order, 1.0 / 200, 0.5, max_steps=20, gamma=G_SR)
+        assert same(out, g[f"riemann_o{order}_output"]), order
+        assert t == float(g[f"riemann_o{order}_time"])
+    out, t, _ = gpu_run(g["blast_input"], RELATIVISTIC, 2, 1.0 / 20, 0.5, max_steps=3, gamma=G_SR)
+    assert same(out, g["blast_o2_output"])
+
+
+# shapes: minimum SLIC grid, ragged sizes that do not divide the x tile (252) or the
+# y march (64 rows), a single column of x tiles, and a wide strip.
+SHAPES = [(2, 2), (3, 5), (17, 9), (300, 257), (253, 65), (700, 3)]
+
+
+@pytest.mark.parametrize("nx,ny", SHAPES)
+@pytest.mark.parametrize("order", [1, 2])
+def test_newtonian_sweeps_bitwise(orc, nx, ny, order):
+    W = workloads.quadrants_2d(nx, seed=11, ny=ny)
+    dx = 1.0 / max(nx, ny)
+    for axis in (0, 1):
+        for lim, bias in ((1, 0.0), (0, 0.0), (2, 0.0), (1, 0.3)):
+            if order == 1 and (lim, bias) != (1, 0.0):
+                continue
+            with Solver(NEWTONIAN, 2, nx, ny, G_N) as s:
+                s.upload(W)
+                s.sweep(order, axis, dx, 0.2 * dx, bias, lim)
+                out = s.download()
+            exp = orc.sweep(W, 0, order, axis, dx, 0.2 * dx, bias, lim, G_N)
+            assert same(out, exp), (axis, lim, bias, rel_linf(out, exp))
+
+
+@pytest.mark.parametrize("nx,ny", [(5, 4), (300, 257), (64, 130)])
+def test_relativistic_sweeps_bitwise(orc, nx, ny):
+    W = workloads.sr_blast_2d(max(nx, ny), seed=12)[:ny, :nx].copy()
+    dx = 1.0 / max(nx, ny)
+    for order in (1, 2):
+        for axis in (0, 1):
+            with Solver(RELATIVISTIC, 2, nx, ny, G_SR) as s:
+                s.upload(W)
+                s.sweep(order, axis, dx, 0.1 * dx, 0.0, 1)
+                out = s.download()
+            exp = orc.sweep(W, 1, order, axis, dx, 0.1 * dx, 0.0, 1, G_SR)
+            assert same(out, exp), (order, axis, rel_linf(out, exp))
+
+
+def test_wavespeed_and_stable_dt_exact(orc):
+    for physics, gamma, W in ((0, G_N, workloads.quadrants_2d(97, seed=3, ny=61)),
+                              (1, G_SR, workloads.sr_blast_2d(80, seed=5))):
+        with Solver(physics, 2, W.shape[1], W.shape[0], gamma) as s:
+            s.upload(W)
+            assert s.max_wave_speed() == orc.max_wavespeed(W, physics, gamma)
+            for it, tf in ((0, 1.0), (6, 1.0), (7, 1e-6), (2, math.inf)):
+                assert s.stable_time_step(0.01, 0.9, 0.0, tf, it) == orc.stable_dt(W, physics, 0.01, 0.9, 0.0, tf, it,
+                                                                                  gamma)
+        W1 = W[W.shape[0] // 2]
+        with Solver(physics, 1, W1.shape[0], 1, gamma) as s:
+            s.upload(W1)
+            assert s.max_wave_speed() == orc.max_wavespeed(W1, physics, gamma)
+
+
+@pytest.mark.parametrize("physics,gamma,n,steps", [(0, G_N, 256, 40), (1, G_SR, 128, 12)])
+@pytest.mark.parametrize("order", [1, 2])
+def test_solve_fixed_steps_bitwise(orc, physics, gamma, n, steps, order):
+    W = workloads.quadrants_2d(n, seed=1) if physics == 0 else workloads.sr_blast_2d(n, seed=2)
+    cfl = 0.9 if physics == 0 else 0.5
+    out, t, k = gpu_run(W, physics, order, 1.0 / n, cfl, max_steps=steps, gamma=gamma)
+    exp, te, ke = orc.run(W, physics, order, 1.0 / n, cfl, max_steps=steps, gamma=gamma)
+    assert k == ke == steps and t == te
+    assert same(out, exp), rel_linf(out, exp)
+
+
+def test_solve_to_time_and_resume(orc):
+    """Run to a final time (the loop stops and clamps dt), then a second solve
+    restarts t and the damping counter like the reference's solve()."""
+    W = workloads.quadrants_2d(96, seed=8, ny=80)
+    with Solver(NEWTONIAN, 2, 96, 80, G_N) as s:
+        s.upload(W)
+        t, k = s.solve(2, 1 / 96, 0.9, 0.02)
+        out1 = s.download()
+        t2, k2 = s.solve(2, 1 / 96, 0.9, 0.01)
+        out2 = s.download()
+    e1, te, ke = orc.run(W, 0, 2, 1 / 96, 0.9, tf=0.02)
+    assert same(out1, e1) and t == te == 0.02 and k == ke
+    e2, te2, ke2 = orc.run(e1, 0, 2, 1 / 96, 0.9, tf=0.01)
+    assert same(out2, e2) and t2 == te2 and k2 == ke2
+
+
+def test_advance_in_chunks_matches_one_run(orc):
+    W = workloads.quadrants_2d(128, seed=21)
+    with Solver(NEWTONIAN, 2, 128, 128, G_N) as s:
+        s.upload(W)
+        s.begin(2, 1 / 128, 0.9, math.inf, 10)
+        s.advance(3)
+        s.advance(4)
+        s.advance(9)  # the loop stops at 10; the rest are no-ops
+        t, n, stopped = s.progress()
+        out = s.download()
+    exp, te, ne = orc.run(W, 0, 2, 1 / 128, 0.9, max_steps=10)
+    assert n == 10 and stopped and t == te and same(out, exp)
+
+
+def test_config2_1024_100_steps_bitwise(orc):
+    """The 4-quadrant workload at 1024^2 for the full 100 steps, bitwise."""
+    W = workloads.quadrants_2d(1024, seed=0)
+    out, t, k = gpu_run(W, NEWTONIAN, 2, 1.0 / 1024, 0.9, max_steps=100)
+    exp, te, ke = orc.run(W, 0, 2, 1.0 / 1024, 0.9, max_steps=100)
+    assert k == ke == 100 and t == te
+    assert same(out, exp), rel_linf(out, exp)
+
+
+def test_config2_full_size_bitwise(orc):
+    """4096^2, three full Strang steps, bitwise against the oracle."""
+    W = workloads.quadrants_2d(4096, seed=1)
+    out, t, k = gpu_run(W, NEWTONIAN, 2, 1.0 / 4096, 0.9, max_steps=3)
+    exp, te, ke = orc.run(W, 0, 2, 1.0 / 4096, 0.9, max_steps=3)
+    assert t == te
+    assert same(out, exp), rel_linf(out, exp)
+
+
+def test_config3_relativistic_reduced_bitwise(orc):
+    """The seeded SR blast at 512^2 for 20 steps (reference con2prim constants)."""
+    W = workloads.sr_blast_2d(512, seed=0)
+    out, t, k = gpu_run(W, RELATIVISTIC, 2, 1.0 / 512, 0.5, max_steps=20, gamma=G_SR)
+    exp, te, ke = orc.run(W, 1, 2, 1.0 / 512, 0.5, max_steps=20, gamma=G_SR)
+    assert t == te and same(out, exp), rel_linf(out, exp)
+
+
+def test_relativistic_config_mode_con2prim(orc):
+    """Tighter con2prim (tol 1e-12, <= 50 iterations) on both sides."""
+    W = workloads.sr_blast_2d(96, seed=6)
+    orc.set_con2prim(1e-12, 50)
+    try:
+        exp, te, _ = orc.run(W, 1, 2, 1.0 / 96, 0.5, max_steps=5, gamma=G_SR)
+    finally:
+        orc.set_con2prim(1e-8, 1000)
+    with Solver(RELATIVISTIC, 2, 96, 96, G_SR) as s:
+        s.set_con2prim(1e-12, 50)
+        s.upload(W)
+        t, _ = s.solve(2, 1.0 / 96, 0.5, math.inf, 5)
+        out = s.download()
+    assert t == te and same(out, exp)
+
+
+def test_nan_propagates_like_reference(orc):
+    """A NaN cell is carried as a value (the reference has no checks), and the
+    wave-speed maximum skips it."""
+    W = workloads.quadrants_2d(40, seed=2)
+    W[10, 10, 4] = np.nan
+    with Solver(NEWTONIAN, 2, 40, 40, G_N) as s:
+        s.upload(W)
+        assert s.max_wave_speed() == orc.max_wavespeed(W, 0, G_N)
+        s.sweep(2, 0, 1 / 40, 0.002)
+        out = s.download()
+    exp = orc.sweep(W, 0, 2, 0, 1 / 40, 0.002)
+    assert same(out, exp) and np.isnan(out).any()

@@ -1,0 +1,281 @@
+"""The CPU oracle (oracle/cosmos_oracle.c) pinned three ways:
+  1. bitwise against the reference's own sources (oracle/_ref) on the same inputs;
+  2. bitwise against the committed golden fixtures written from that build;
+  3. against closed forms and invariants that a dropped term, wrong sign,
+     wrong index or transposed operand would break.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, same
+from cosmos_amd import workloads
+
+G_N, G_SR = 1.4, 5.0 / 3.0
+
+
+# ---------------------------------------------------------------------------
+# 1. oracle == reference build
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("physics,gamma", [(0, G_N), (1, G_SR)])
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("limiter", [0, 1, 2])
+def test_run_2d_matches_reference(ref, orc, physics, gamma, order, limiter):
+    if order == 1 and limiter != 1:
+        pytest.skip("FORCE has no limiter")
+    W = workloads.quadrants_2d(18, seed=5 + limiter, ny=13)
+    if physics == 1:
+        W[..., 1:3] *= 0.6
+    a = ref.run(W, physics, order, 1.0 / 18, 0.8, max_steps=4, limiter=limiter, gamma=gamma)
+    b = orc.run(W, physics, order, 1.0 / 18, 0.8, max_steps=4, limiter=limiter, gamma=gamma)
+    assert same(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+
+
+@pytest.mark.parametrize("physics,gamma", [(0, G_N), (1, G_SR)])
+@pytest.mark.parametrize("order", [1, 2])
+def test_run_1d_to_time_matches_reference(ref, orc, physics, gamma, order):
+    W = workloads.sod_1d(64) if physics == 0 else workloads.sr_riemann_1d(64, seed=9)
+    a = ref.run(W, physics, order, 1.0 / 64, 0.8, tf=0.05, gamma=gamma)
+    b = orc.run(W, physics, order, 1.0 / 64, 0.8, tf=0.05, gamma=gamma)
+    assert same(a[0], b[0]) and a[1] == b[1] == 0.05 and a[2] == b[2]
+
+
+@pytest.mark.parametrize("bias", [0.0, 0.3, -0.4])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_sweeps_match_reference(ref, orc, bias, axis):
+    W = workloads.quadrants_2d(10, seed=2, ny=7)
+    for physics, gamma in ((0, G_N), (1, G_SR)):
+        Wp = W.copy()
+        if physics == 1:
+            Wp[..., 1:3] *= 0.5
+        for order in (1, 2):
+            a = ref.sweep(Wp, physics, order, axis, 0.1, 0.02, bias, 1, gamma)
+            b = orc.sweep(Wp, physics, order, axis, 0.1, 0.02, bias, 1, gamma)
+            assert same(a, b), (physics, order)
+
+
+def test_wavespeed_and_dt_match_reference(ref, orc):
+    for physics, gamma in ((0, G_N), (1, G_SR)):
+        W = workloads.quadrants_2d(16, seed=4)
+        if physics == 1:
+            W[..., 1:3] *= 0.5
+        assert ref.max_wavespeed(W, physics, gamma) == orc.max_wavespeed(W, physics, gamma)
+        assert ref.max_wavespeed(W[0], physics, gamma) == orc.max_wavespeed(W[0], physics, gamma)
+        for it in (0, 5, 6):
+            for tf in (1.0, 1e-4, math.inf):
+                assert ref.stable_dt(W, physics, 1 / 16, 0.9, 0.0, tf, it, gamma) == \
+                    orc.stable_dt(W, physics, 1 / 16, 0.9, 0.0, tf, it, gamma)
+
+
+def test_single_calls_match_reference(ref, orc):
+    rng = np.random.default_rng(7)
+    for physics, gamma in ((0, G_N), (1, G_SR)):
+        Ws = workloads.random_states(40, seed=31 + physics, physics=physics)
+        for k in range(0, 36, 3):
+            l, m, r, rr = Ws[k], Ws[k + 1], Ws[k + 2], Ws[k + 3]
+            for axis in (0, 1):
+                assert same(ref.slic_flux(physics, axis, l, m, r, rr, 0.02, 0.005, 0.0, 1, gamma),
+                            orc.slic_flux(physics, axis, l, m, r, rr, 0.02, 0.005, 0.0, 1, gamma))
+                assert same(ref.force_flux(physics, axis, l, m, 0.02, 0.005, gamma),
+                            orc.force_flux(physics, axis, l, m, 0.02, 0.005, gamma))
+            u = ref.p2c(physics, l, gamma)
+            assert same(u, orc.p2c(physics, l, gamma))
+            assert same(ref.c2p(physics, u, gamma), orc.c2p(physics, u, gamma))
+    for r in rng.normal(size=200) * 3:
+        for lim in (0, 1, 2):
+            for bias in (0.0, 0.25):
+                assert ref.limiter(r, bias, lim) == orc.limiter(r, bias, lim)
+
+
+def test_sr_unphysical_state_matches_reference(ref, orc):
+    """|S| > tau + D hits the q floor, the Newton cap and the rho/p floors."""
+    bad = [np.array([1.0, 5.0, 0.0, 0.0, 0.5]), np.array([1e-9, 0.0, 0.0, 0.0, 1e-12]),
+           np.array([1.0, 0.0, 0.0, 0.0, -0.9])]
+    for u in bad:
+        assert same(ref.c2p(1, u, G_SR), orc.c2p(1, u, G_SR))
+        assert same(ref.flux(1, 0, u, G_SR), orc.flux(1, 0, u, G_SR))
+
+
+# ---------------------------------------------------------------------------
+# 2. oracle == golden fixtures (these need no reference build)
+# ---------------------------------------------------------------------------
+def test_golden_sod(orc):
+    g = golden("sod_1d.npz")
+    for order in (1, 2):
+        a, t, n = orc.run(g["input"], 0, order, 1.0 / 400, 0.8, tf=0.25)
+        assert same(a, g[f"order{order}_output"])
+        assert t == float(g[f"order{order}_time"]) and n == int(g[f"order{order}_steps"])
+
+
+def test_golden_newtonian_2d(orc):
+    g = golden("newtonian_2d.npz")
+    for order in (1, 2):
+        for lim in ((0, 1, 2) if order == 2 else (1,)):
+            a, t, n = orc.run(g["input"], 0, order, 1.0 / 24, 0.9, max_steps=3, limiter=lim)
+            assert same(a, g[f"o{order}_l{lim}_output"]) and t == float(g[f"o{order}_l{lim}_time"])
+    a, t, n = orc.run(g["input"], 0, 2, 1.0 / 24, 0.9, max_steps=3, bias=0.3, limiter=1)
+    assert same(a, g["o2_l1_bias03_output"])
+
+
+def test_golden_relativistic(orc):
+    g = golden("relativistic.npz")
+    for order in (1, 2):
+        a, t, n = orc.run(g["riemann_input"], 1, order, 1.0 / 200, 0.5, max_steps=20, gamma=G_SR)
+        assert same(a, g[f"riemann_o{order}_output"]) and t == float(g[f"riemann_o{order}_time"])
+    a, t, n = orc.run(g["blast_input"], 1, 2, 1.0 / 20, 0.5, max_steps=3, gamma=G_SR)
+    assert same(a, g["blast_o2_output"]) and t == float(g["blast_o2_time"])
+
+
+def test_golden_single_calls(orc):
+    g = golden("single_calls.npz")
+    r = g["limiter_r"]
+    for lim in (0, 1, 2):
+        for bias, tag in ((0.0, "0"), (0.3, "p3"), (-0.5, "m5")):
+            assert same([orc.limiter(x, bias, lim) for x in r], g[f"limiter_{lim}_{tag}"])
+    for phys, gamma in ((0, G_N), (1, G_SR)):
+        Ws = g[f"p{phys}_states"]
+        assert same([orc.p2c(phys, w, gamma) for w in Ws], g[f"p{phys}_p2c"])
+        Us = g[f"p{phys}_p2c"]
+        assert same([orc.c2p(phys, u, gamma) for u in Us], g[f"p{phys}_c2p"])
+        assert same([orc.flux(phys, 0, u, gamma) for u in Us], g[f"p{phys}_fluxx"])
+        assert same([orc.flux(phys, 1, u, gamma) for u in Us], g[f"p{phys}_fluxy"])
+        sl, hs = [], []
+        for k in range(0, 60, 4):
+            for axis in (0, 1):
+                sl.append(orc.slic_flux(phys, axis, Ws[k], Ws[k + 1], Ws[k + 2], Ws[k + 3], 0.01, 0.004, 0.0, 1,
+                                        gamma))
+                hs.append(orc.half_step(phys, axis, Ws[k], Ws[k + 1], Ws[k + 2], 0.01, 0.004, 0.0, 1, k % 2, gamma))
+        assert same(sl, g[f"p{phys}_slic_flux"]) and same(hs, g[f"p{phys}_half_step"])
+        assert same([orc.force_flux(phys, k % 2, Ws[k], Ws[k + 1], 0.01, 0.004, gamma) for k in range(0, 60, 2)],
+                    g[f"p{phys}_force_flux"])
+    U = g["slope_inputs"]
+    assert same([orc.slope(U[k], U[k + 1], U[k + 2], b, lim)
+                 for k in range(0, 27, 3) for lim in (0, 1, 2) for b in (0.0, 0.3)], g["slope_out"])
+    assert same([orc.stable_dt_scalar(0.01, t, tf, it)
+                 for (t, tf, it) in ((0.0, 1.0, 0), (0.5, 1.0, 5), (0.5, 1.0, 6), (0.995, 1.0, 7),
+                                     (0.0, math.inf, 3))], g["dt_scalar"])
+
+
+# ---------------------------------------------------------------------------
+# 3. closed forms and invariants
+# ---------------------------------------------------------------------------
+def test_limiter_closed_forms(orc):
+    for r in [-3.0, -1e-3, 0.0, 0.2, 0.5, 0.75, 1.0, 1.5, 3.0, 10.0]:
+        xi = 2.0 / (1.0 + r)
+        sb = 0.0 if r < 0 else (2 * r if r < 0.5 else (1.0 if r < 1 else min(r, xi, 2.0)))
+        vl = 0.0 if r < 0 else min(2 * r / (1 + r), xi)
+        mb = 0.0 if r < 0 else (r if r < 1 else min(1.0, xi))
+        assert orc.limiter(r, 0.0, 0) == pytest.approx(sb, abs=1e-15)
+        assert orc.limiter(r, 0.0, 1) == pytest.approx(vl, abs=1e-15)
+        assert orc.limiter(r, 0.0, 2) == pytest.approx(mb, abs=1e-15)
+        assert orc.limiter(r, 0.0, 7) == orc.limiter(r, 0.0, 2)  # any other code is MinBee
+    # Toro's SLIC form applies the limiter to the averaged slope, so at bias 0
+    # van Leer is symmetric under r -> 1/r: min(2r/(1+r), 2/(1+r)) = phi(1/r).
+    for r in [0.1, 0.5, 2.0, 4.0]:
+        assert orc.limiter(r, 0.0, 1) == pytest.approx(orc.limiter(1 / r, 0.0, 1), rel=1e-15)
+
+
+def test_slope_clamp_discards_sign(orc):
+    """|delta| < 1e-5 is replaced by +1e-5: a tiny negative right difference
+    with a positive left difference gives a positive ratio (limited slope),
+    where the unclamped ratio would be negative (zero slope)."""
+    ul = np.array([0.0, 0, 0, 0, 0])
+    um = np.array([1.0, 0, 0, 0, 0])
+    ur = np.array([1.0 - 5e-6, 0, 0, 0, 0])
+    s = orc.slope(ul, um, ur, 0.0, 1)
+    r = 1.0 / 1e-5
+    assert s[0] == pytest.approx(0.5 * (1.0 - 5e-6) * min(2 * r / (1 + r), 2 / (1 + r)), rel=1e-14)
+    assert s[0] > 0
+
+
+def test_newtonian_prim_cons_closed_form(orc):
+    w = np.array([1.3, 0.4, -0.2, 0.1, 0.9])
+    u = orc.p2c(0, w, G_N)
+    assert u[0] == 1.3 and u[1] == pytest.approx(1.3 * 0.4) and u[3] == pytest.approx(0.13)
+    assert u[4] == pytest.approx(0.9 / 0.4 + 0.5 * 1.3 * (0.16 + 0.04 + 0.01), rel=1e-15)
+    assert np.allclose(orc.c2p(0, u, G_N), w, rtol=1e-15, atol=1e-16)
+    f = orc.flux(0, 0, u, G_N)
+    assert f[0] == pytest.approx(1.3 * 0.4) and f[1] == pytest.approx(1.3 * 0.16 + 0.9)
+    assert f[4] == pytest.approx(0.4 * (u[4] + 0.9), rel=1e-14)
+    fy = orc.flux(0, 1, u, G_N)
+    assert fy[0] == pytest.approx(1.3 * -0.2) and fy[2] == pytest.approx(1.3 * 0.04 + 0.9)
+
+
+def test_sr_con2prim_round_trip(orc):
+    """prim -> cons -> prim returns the state to within the Newton tolerance,
+    and the converged x satisfies the quartic's root (residual small)."""
+    for w in workloads.random_states(50, seed=77, physics=1):
+        u = orc.p2c(1, w, G_SR)
+        back = orc.c2p(1, u, G_SR)
+        assert np.allclose(back, w, rtol=1e-6, atol=1e-7)
+    orc.set_con2prim(1e-14, 200)
+    try:
+        for w in workloads.random_states(20, seed=78, physics=1):
+            back = orc.c2p(1, orc.p2c(1, w, G_SR), G_SR)
+            assert np.allclose(back, w, rtol=1e-12, atol=1e-13)
+    finally:
+        orc.set_con2prim(1e-8, 1000)
+
+
+def test_sr_closed_form_at_rest(orc):
+    """v = 0: D = rho, S = 0, tau = rho h - p - rho = p / (gamma - 1)."""
+    w = np.array([2.0, 0.0, 0.0, 0.0, 3.0])
+    u = orc.p2c(1, w, G_SR)
+    assert u[0] == 2.0 and u[1] == 0.0 and u[4] == pytest.approx(3.0 / (G_SR - 1.0), rel=1e-15)
+
+
+@pytest.mark.parametrize("physics,gamma", [(0, G_N), (1, G_SR)])
+@pytest.mark.parametrize("order", [1, 2])
+def test_uniform_state_is_preserved_bitwise(orc, physics, gamma, order):
+    W = np.empty((9, 11, 5))
+    W[...] = (0.7, 0.3, -0.2, 0.0, 1.1) if physics == 0 else (0.7, 0.3, -0.2, 0.0, 1.1)
+    U = orc.p2c(physics, W[0, 0], gamma)
+    ref_state = orc.c2p(physics, U, gamma)  # one p2c/c2p round trip per sweep
+    out = orc.sweep(W, physics, order, 0, 0.1, 0.01, 0.0, 1, gamma)
+    assert np.all(out == ref_state)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_conservation_telescopes(orc, order):
+    """Interior fluxes cancel: after an x sweep the change in the total of each
+    conserved variable equals dt/dx (F_left boundary - F_right boundary); with
+    a mirrored/copied boundary and a state that is uniform near both ends,
+    the boundary fluxes are F(U_edge) at each end."""
+    n = 40
+    W = np.empty((1, n, 5))
+    W[0, :] = (1.0, 0.0, 0.0, 0.0, 1.0)
+    W[0, 15:25] = (0.5, 0.2, 0.1, 0.0, 0.4)
+    dx, dt = 1.0 / n, 0.2 / n
+    U0 = np.array([orc.p2c(0, w, G_N) for w in W[0]])
+    out = orc.sweep(W, 0, order, 0, dx, dt)
+    U1 = np.array([orc.p2c(0, w, G_N) for w in out[0]])
+    # both ends are at rest with equal pressure: F_left = F_right, so totals are conserved
+    assert np.allclose(U1.sum(axis=0), U0.sum(axis=0), rtol=0, atol=1e-12)
+
+
+def test_sod_against_exact_riemann_solution(orc):
+    """Sanity (not bitwise): the second-order Sod solution is close to the exact
+    Riemann solution (Toro test 1: p* = 0.30313, u* = 0.92745)."""
+    g = golden("sod_1d.npz")
+    W = g["order2_output"]
+    x = (np.arange(400) + 0.5) / 400
+    # star region between contact (x = 0.5 + 0.92745 t) and shock (x = 0.5 + 1.75216 t) at t = 0.25
+    sel = (x > 0.5 + 0.93 * 0.25 + 0.02) & (x < 0.5 + 1.752 * 0.25 - 0.02)
+    assert np.allclose(W[sel, 4], 0.30313, atol=2e-3)
+    assert np.allclose(W[sel, 1], 0.92745, atol=5e-3)
+
+
+def test_dt_damping_and_clamp(orc):
+    assert orc.stable_dt_scalar(0.01, 0.0, 1.0, 5) == 0.01 * 0.2
+    assert orc.stable_dt_scalar(0.01, 0.0, 1.0, 6) == 0.01
+    assert orc.stable_dt_scalar(0.01, 0.995, 1.0, 9) == 1.0 - 0.995
+
+
+def test_newtonian_2d_wavespeed_grouping(orc):
+    """2D Newtonian wave speed is max(|u|, |v| + c) (Solvers/EulerSolvers.cpp:117-118)."""
+    W = np.array([[[1.0, 3.0, 0.1, 0.0, 1.0]]])
+    c = math.sqrt(1.4)
+    assert orc.max_wavespeed(W, 0, G_N) == max(3.0, 0.1 + c)
+    W1 = np.array([[1.0, 3.0, 0.1, 0.0, 1.0]])
+    assert orc.max_wavespeed(W1, 0, G_N) == 3.0 + c
