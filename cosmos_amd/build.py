@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libcosmos_fv.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("COSMOS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fv_kernels.hip", "fv_capi.hip"]
+SOURCES = ["fv_kernels.hip", "fv_capi.hip", "fv_pointwise.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",
@@ -36,17 +36,34 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source into libcosmos_fv.so; returns the library path."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+ROOT = os.path.dirname(HERE)
+DROPIN = os.path.join(HERE, "dropin")
+DROPIN_LIB = os.path.join(HERE, "libcosmos_dropin.so")
+DROPIN_CHECK = os.path.join(ROOT, "tests", "dropin", "dropin_check")
+CXX = os.environ.get("CXX", "g++")
+
+
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+        raise RuntimeError(f"{cmd[0]} failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the HIP library (libcosmos_fv.so), the C++ drop-in classes over
+    it (libcosmos_dropin.so) and the drop-in check program; returns the HIP
+    library path."""
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *FLAGS, "-o", tmp, *srcs], verbose)
     os.replace(tmp, LIB)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + DROPIN, "-I" + os.path.join(DROPIN, "compat")]
+    _run([CXX, "-std=c++14", "-O2", "-fPIC", "-shared", "-Wall", *inc, "-o", DROPIN_LIB,
+          os.path.join(DROPIN, "cosmos_dropin.cpp"), "-L" + HERE, "-lcosmos_fv", "-Wl,-rpath,$ORIGIN"], verbose)
+    _run([CXX, "-std=c++14", "-O2", "-Wall", *inc, "-o", DROPIN_CHECK, DROPIN_CHECK + ".cpp", "-L" + HERE,
+          "-lcosmos_dropin", "-lcosmos_fv", "-Wl,-rpath," + HERE], verbose)
     return LIB
 
 

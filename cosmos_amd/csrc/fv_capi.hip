@@ -31,7 +31,12 @@ struct cosmos_solver {
     double* buf[2] = {nullptr, nullptr};
     int cur = 0;
     StepState* st = nullptr;   // device
-    unsigned long long* ws_scratch = nullptr;  // device, for cosmos_max_wave_speed
+    unsigned long long* st_slots = nullptr;    // device, max wave speed slots of the solve loop
+    unsigned long long* ws_scratch = nullptr;  // device, slots for cosmos_max_wave_speed
+    int* flag = nullptr;                       // device, scratch flag
+    double* stage = nullptr;                   // device, AoS staging for transfers (lazily allocated)
+    bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
+    int mode = COSMOS_MODE_EXACT;
     hipStream_t stream = nullptr;
     int device = 0;
     // solve-loop parameters set by cosmos_begin.  Steps are queued without
@@ -80,6 +85,7 @@ cfv::PhysParams make_params(double g) {
     P.sr_gm2g = (g - 2.0) / g;
     P.sr_gg1 = g / (g - 1.0);
     P.sr_4gm1g = 4.0 * ((g - 1.0) / g);
+    P.inv_gm1 = 1.0 / (g - 1.0);
     return P;
 }
 
@@ -132,6 +138,8 @@ SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, in
     L.order = order;
     L.physics = s->physics;
     L.dim = s->dim;
+    L.w_zero = s->w_zero ? 1 : 0;
+    L.fast = s->mode == COSMOS_MODE_FAST ? 1 : 0;
     L.P = s->P;
     L.step = nullptr;
     L.use_full = 0;
@@ -255,10 +263,16 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     if (e == hipSuccess) e = hipMalloc((void**)&s->buf[0], bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->buf[1], bytes);
     if (e == hipSuccess) e = hipMalloc((void**)&s->st, sizeof(StepState));
-    if (e == hipSuccess) e = hipMalloc((void**)&s->ws_scratch, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMemset(s->buf[0], 0, bytes);
-    if (e == hipSuccess) e = hipMemset(s->buf[1], 0, bytes);
-    if (e == hipSuccess) e = hipMemset(s->st, 0, sizeof(StepState));
+    if (e == hipSuccess) e = hipMalloc((void**)&s->st_slots, sizeof(unsigned long long) * cfv::kSlotWords);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->flag, sizeof(int));
+    // Zero on the handle's own stream and wait: the stream is non-blocking, so
+    // a fill queued on the null stream would not be ordered before the first
+    // upload into these buffers.
+    if (e == hipSuccess) e = hipMemsetAsync(s->buf[0], 0, bytes, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->buf[1], 0, bytes, s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->st, 0, sizeof(StepState), s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         std::string msg = std::string("device setup: ") + hipGetErrorString(e);
         cosmos_destroy(s);
@@ -279,7 +293,10 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->buf[0]);
     hipFree(s->buf[1]);
     hipFree(s->st);
+    hipFree(s->st_slots);
     hipFree(s->ws_scratch);
+    hipFree(s->flag);
+    hipFree(s->stage);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return COSMOS_OK;
@@ -287,6 +304,15 @@ int cosmos_destroy(cosmos_solver* s) {
 
 const char* cosmos_last_error(const cosmos_solver* s) {
     return s ? s->err.c_str() : g_create_error.c_str();
+}
+
+int cosmos_set_mode(cosmos_solver* s, int mode) {
+    return guarded(s, [&] {
+        if (mode != COSMOS_MODE_EXACT && mode != COSMOS_MODE_FAST)
+            return fail(s, COSMOS_ERR_ARGUMENT, "mode must be COSMOS_MODE_EXACT or COSMOS_MODE_FAST");
+        s->mode = mode;
+        return COSMOS_OK;
+    });
 }
 
 int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations) {
@@ -299,17 +325,36 @@ int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations) 
     });
 }
 
+static int ensure_stage(cosmos_solver* s) {
+    if (s->stage) return COSMOS_OK;
+    HIPCHK(s, hipMalloc((void**)&s->stage, sizeof(double) * 5 * (size_t)s->nx * s->ny));
+    return COSMOS_OK;
+}
+
 static int upload_impl(cosmos_solver* s, const double* prim, hipMemcpyKind kind) {
     if (!prim) return fail(s, COSMOS_ERR_ARGUMENT, "null state pointer");
     int rc = settle(s);
     if (rc) return rc;
+    rc = ensure_stage(s);
+    if (rc) return rc;
     long cells = (long)s->nx * s->ny;
-    double* stage = s->buf[s->cur ^ 1];  // 5 * ny * pitch >= 5 * cells doubles
-    HIPCHK(s, hipMemcpyAsync(stage, prim, sizeof(double) * 5 * cells, kind, s->stream));
-    hipLaunchKernelGGL(k_aos_to_soa, grid_for(cells), dim3(256), 0, s->stream, stage, planes_of(s, s->cur), s->nx,
+    HIPCHK(s, hipMemcpyAsync(s->stage, prim, sizeof(double) * 5 * cells, kind, s->stream));
+    hipLaunchKernelGGL(k_aos_to_soa, grid_for(cells), dim3(256), 0, s->stream, s->stage, planes_of(s, s->cur), s->nx,
                        s->ny, s->pitch);
     HIPCHK(s, hipGetLastError());
+    // Is the z-velocity plane identically zero?  Then the sweeps skip it (its
+    // terms are exact zeros), and the other buffer's plane is zeroed to match.
+    int nonzero = 1;
+    HIPCHK(s, hipMemsetAsync(s->flag, 0, sizeof(int), s->stream));
+    HIPCHK(s, cfv::launch_any_nonzero(planes_of(s, s->cur).p[3], s->nx, s->ny, s->pitch, s->flag, s->stream));
+    HIPCHK(s, hipMemcpyAsync(&nonzero, s->flag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    s->w_zero = (nonzero == 0);
+    if (s->w_zero) {
+        HIPCHK(s, hipMemsetAsync(planes_of(s, s->cur ^ 1).p[3], 0, sizeof(double) * (size_t)s->ny * s->pitch,
+                                 s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+    }
     s->run_ready = false;
     return COSMOS_OK;
 }
@@ -318,12 +363,13 @@ static int download_impl(cosmos_solver* s, double* prim, hipMemcpyKind kind) {
     if (!prim) return fail(s, COSMOS_ERR_ARGUMENT, "null state pointer");
     int rc = settle(s);
     if (rc) return rc;
+    rc = ensure_stage(s);
+    if (rc) return rc;
     long cells = (long)s->nx * s->ny;
-    double* stage = s->buf[s->cur ^ 1];
-    hipLaunchKernelGGL(k_soa_to_aos, grid_for(cells), dim3(256), 0, s->stream, cplanes_of(s, s->cur), stage, s->nx,
-                       s->ny, s->pitch);
+    hipLaunchKernelGGL(k_soa_to_aos, grid_for(cells), dim3(256), 0, s->stream, cplanes_of(s, s->cur), s->stage,
+                       s->nx, s->ny, s->pitch);
     HIPCHK(s, hipGetLastError());
-    HIPCHK(s, hipMemcpyAsync(prim, stage, sizeof(double) * 5 * cells, kind, s->stream));
+    HIPCHK(s, hipMemcpyAsync(prim, s->stage, sizeof(double) * 5 * cells, kind, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return COSMOS_OK;
 }
@@ -362,12 +408,12 @@ int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, dou
     });
 }
 
-static int max_ws_impl(cosmos_solver* s, unsigned long long* bits_dev) {
+static int max_ws_impl(cosmos_solver* s, unsigned long long* slots_dev) {
     int rc = settle(s);
     if (rc) return rc;
-    hipError_t e = cfv::launch_zero_u64(bits_dev, s->stream);
+    hipError_t e = cfv::launch_zero_slots(slots_dev, s->stream);
     if (e == hipSuccess)
-        e = cfv::launch_wavespeed(cplanes_of(s, s->cur), s->nx, s->ny, s->pitch, s->physics, s->dim, s->P, bits_dev,
+        e = cfv::launch_wavespeed(cplanes_of(s, s->cur), s->nx, s->ny, s->pitch, s->physics, s->dim, s->P, slots_dev,
                                   s->stream);
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("wave speed launch: ") + hipGetErrorString(e));
     return COSMOS_OK;
@@ -378,12 +424,11 @@ int cosmos_max_wave_speed(cosmos_solver* s, double* out) {
         if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "out is null");
         int rc = max_ws_impl(s, s->ws_scratch);
         if (rc) return rc;
-        unsigned long long bits = 0;
-        HIPCHK(s, hipMemcpyAsync(&bits, s->ws_scratch, sizeof bits, hipMemcpyDeviceToHost, s->stream));
+        std::vector<unsigned long long> words(cfv::kSlotWords);
+        HIPCHK(s, hipMemcpyAsync(words.data(), s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords,
+                                 hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
-        double v;
-        std::memcpy(&v, &bits, sizeof v);
-        *out = v;
+        *out = cfv::slots_max_host(words.data());
         return COSMOS_OK;
     });
 }
@@ -418,8 +463,9 @@ int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, d
         h.max_steps = max_steps < 0 ? INT_MAX : max_steps;
         h.done = 0;
         h.active_steps = 0;
+        h.ws_slots = s->st_slots;
         HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
-        rc = max_ws_impl(s, &s->st->ws_bits);
+        rc = max_ws_impl(s, s->st_slots);
         if (rc) return rc;
         s->run_order = order;
         s->run_bias = bias;

@@ -22,6 +22,12 @@
 // neighbour exchange between stages (252 outputs per block for SLIC).
 // y sweeps: one thread per column, each marching down a segment of rows with a
 // three-cell register window; every load and store is a coalesced row access.
+//
+// Kernels are templated on the physics (with its W0 flag: the z-velocity
+// plane is identically zero and is neither read nor written; fast mode
+// only), the order, the limiter mode (1: van Leer at bias 0 with one division
+// per component, 2: the fast-mode van Leer, 0: any limiter and bias) and
+// whether the output's max wave speed is merged.
 #include "fv_kernels.hpp"
 
 namespace cfv {
@@ -43,28 +49,36 @@ __device__ __forceinline__ int clamp1(int c, int n) {
     return c >= n ? n - 1 : c;
 }
 
+template <bool W0>
 __device__ __forceinline__ void load_cell(const ConstPlanes& in, long idx, double* W) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) W[k] = in.p[k][idx];
+    for (int k = 0; k < 5; ++k) W[k] = (W0 && k == 3) ? 0.0 : in.p[k][idx];
 }
 
+template <bool W0>
 __device__ __forceinline__ void store_cell(const Planes& out, long idx, const double* W) {
 #pragma unroll
-    for (int k = 0; k < 5; ++k) out.p[k][idx] = W[k];
+    for (int k = 0; k < 5; ++k)
+        if (!(W0 && k == 3)) out.p[k][idx] = W[k];
 }
 
 // Half-step evolution of one cell from its conserved neighbours
 // (EulerSolvers::evolveStateByHalf{X,Y}TimeStep, Solvers/EulerSolvers.cpp:191-223),
 // then the p2c and flux of both evolved face states as the FORCE flux of
 // the neighbouring interfaces will need them (Solvers/EulerFirstOrderSolver.cpp:52-59).
-template <class Phys, int AXIS, int LIMMODE>
+template <class Phys, int AXIS, int LIMMODE, bool W0>
 __device__ __forceinline__ void half_step(const double* ul, const double* um, const double* ur, const SweepCoef& cf,
                                           double bias, int limiter, const PhysParams& P, double* Q0, double* G0,
                                           double* Q1, double* G1) {
     double xl[5], xr[5], fl[5], fr[5], a[5], b[5], p0[5], p1[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        double s = slope_component<LIMMODE>(ul[k], um[k], ur[k], bias, limiter);
+        if (W0 && k == 3) {
+            xl[k] = xr[k] = 0.0;
+            continue;
+        }
+        double s = LIMMODE == 2 ? slope_component_fast_vl(ul[k], um[k], ur[k])
+                                : slope_component<LIMMODE>(ul[k], um[k], ur[k], bias, limiter);
         xl[k] = um[k] - (0.5 * s);
         xr[k] = um[k] + (0.5 * s);
     }
@@ -72,54 +86,70 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
     Phys::template flux<AXIS>(xr, fr, Phys::inv_density(xr), P);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        double ev = cf.hdt_dx * (fl[k] - fr[k]);
-        a[k] = xl[k] + ev;
-        b[k] = xr[k] + ev;
+        double ev = (W0 && k == 3) ? 0.0 : cf.hdt_dx * (fl[k] - fr[k]);
+        a[k] = (W0 && k == 3) ? 0.0 : xl[k] + ev;
+        b[k] = (W0 && k == 3) ? 0.0 : xr[k] + ev;
     }
     // P0 = c2p(a) (side 0), P1 = c2p(b) (side 1); p2c(P) has density P[0] = a[0] (resp.
     // b[0]), so the flux of p2c(P) divides by the same density.
     double ia = Phys::inv_density(a), ib = Phys::inv_density(b);
-    Phys::c2p(a, p0, ia, P);
-    Phys::c2p(b, p1, ib, P);
-    Phys::p2c(p0, Q0, P);
-    Phys::p2c(p1, Q1, P);
-    Phys::template flux<AXIS>(Q0, G0, ia, P);
-    Phys::template flux<AXIS>(Q1, G1, ib, P);
+    if constexpr (Phys::kFast) {
+        // fast mode: the face state's conserved vector is a (resp. b) itself
+        Phys::template c2p_flux<AXIS>(a, p0, G0, ia, P);
+        Phys::template c2p_flux<AXIS>(b, p1, G1, ib, P);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Q0[k] = a[k];
+            Q1[k] = b[k];
+        }
+    } else {
+        Phys::c2p(a, p0, ia, P);
+        Phys::c2p(b, p1, ib, P);
+        Phys::p2c(p0, Q0, P);
+        Phys::p2c(p1, Q1, P);
+        Phys::template flux<AXIS>(Q0, G0, ia, P);
+        Phys::template flux<AXIS>(Q1, G1, ib, P);
+    }
 }
 
 // FORCE flux of one interface from p2c and flux of its two states
 // (computeLaxFriedrichsFlux, computeRichtmyerFlux, computeFORCEFlux:
 // Solvers/EulerFirstOrderSolver.cpp:3-24).
-template <class Phys, int AXIS>
+template <class Phys, int AXIS, bool W0>
 __device__ __forceinline__ void force_flux(const double* UL, const double* FL, const double* UR, const double* FR,
                                            const SweepCoef& cf, const PhysParams& P, double* F) {
     double us[5], fs[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((UL[k] + UR[k]) + (cf.dt_dx * (FL[k] - FR[k])));
+    for (int k = 0; k < 5; ++k)
+        us[k] = (W0 && k == 3) ? 0.0 : 0.5 * ((UL[k] + UR[k]) + (cf.dt_dx * (FL[k] - FR[k])));
     Phys::template flux<AXIS>(us, fs, Phys::inv_density(us), P);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
+        if (W0 && k == 3) {
+            F[k] = 0.0;
+            continue;
+        }
         double lf = 0.5 * ((FL[k] + FR[k]) + (cf.dx_dt * (UL[k] - UR[k])));
         F[k] = 0.5 * (lf + fs[k]);
     }
 }
 
 // computeFORCEUpdate (Solvers/EulerFirstOrderSolver.cpp:96-101) then c2p.
-template <class Phys>
+template <class Phys, bool W0>
 __device__ __forceinline__ void update_cell(const double* U, const double* Fl, const double* Fr, const SweepCoef& cf,
                                             const PhysParams& P, double* W) {
     double un[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) un[k] = U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
+    for (int k = 0; k < 5; ++k) un[k] = (W0 && k == 3) ? 0.0 : U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
     Phys::c2p(un, W, Phys::inv_density(un), P);
 }
 
-// Block-wide max of non-negative wave speeds, merged into *bits with an
-// integer atomic max (the IEEE bit patterns of non-negative doubles order
-// like the values).  The reference's running max skips NaN (ws > max is
-// false), so NaN never enters.
+// Block-wide max of non-negative wave speeds, merged into one of the slots
+// with an integer atomic max (the IEEE bit patterns of non-negative doubles
+// order like the values).  The reference's running max skips NaN
+// (ws > max is false), so NaN never enters.
 template <int NT>
-__device__ __forceinline__ void block_max_to(double m, unsigned long long* bits) {
+__device__ __forceinline__ void block_max_to(double m, unsigned long long* slots) {
     __shared__ double red[NT / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -127,13 +157,19 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* bits)
         m = (o > m) ? o : m;
     }
     int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = m;
-    __syncthreads();
+    if (NT > 64) {
+        if (lane == 0) red[wave] = m;
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        double b = red[0];
+        double b = m;
+        if (NT > 64) {
+            b = red[0];
 #pragma unroll
-        for (int w = 1; w < NT / 64; ++w) b = (red[w] > b) ? red[w] : b;
-        if (b > 0.0) atomicMax(bits, (unsigned long long)__double_as_longlong(b));
+            for (int w = 1; w < NT / 64; ++w) b = (red[w] > b) ? red[w] : b;
+        }
+        if (b > 0.0)
+            atomicMax(slots + (blockIdx.x % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(b));
     }
 }
 
@@ -149,7 +185,7 @@ __device__ __forceinline__ bool resolve_coef(const SweepLaunch& L, SweepCoef& cf
 // ---------------------------------------------------------------------------
 // x sweep: thread per cell, LDS exchange
 // ---------------------------------------------------------------------------
-template <class Phys, int ORDER, int LIMMODE, int DIM, bool WS>
+template <class Phys, int ORDER, int LIMMODE, int DIM, bool WS, bool W0>
 __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     constexpr int NT = kXThreads;
     constexpr int G = ORDER == 2 ? 2 : 1;
@@ -170,24 +206,26 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const long rowbase = (long)row * L.pitch;
 
     double W[5], U[5], F[5];
-    load_cell(L.in, rowbase + cc, W);
+    load_cell<W0>(L.in, rowbase + cc, W);
     Phys::p2c(W, U, P);
 
     const int sl = s > 0 ? s - 1 : 0;
     const int sr = s < NT - 1 ? s + 1 : NT - 1;
     if constexpr (ORDER == 2) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) sA[k][s] = U[k];
+        for (int k = 0; k < 5; ++k)
+            if (!(W0 && k == 3)) sA[k][s] = U[k];
         __syncthreads();
         double ul[5], ur[5], Q0[5], G0[5], Q1[5], G1[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            ul[k] = sA[k][sl];
-            ur[k] = sA[k][sr];
+            ul[k] = (W0 && k == 3) ? 0.0 : sA[k][sl];
+            ur[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
         }
-        half_step<Phys, 0, LIMMODE>(ul, U, ur, cf, L.bias, L.limiter, P, Q0, G0, Q1, G1);
+        half_step<Phys, 0, LIMMODE, W0>(ul, U, ur, cf, L.bias, L.limiter, P, Q0, G0, Q1, G1);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
+            if (W0 && k == 3) continue;
             sB[k][s] = Q0[k];
             sB[5 + k][s] = G0[k];
         }
@@ -195,19 +233,21 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
         double UR[5], FR[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            UR[k] = sB[k][sr];
-            FR[k] = sB[5 + k][sr];
+            UR[k] = (W0 && k == 3) ? 0.0 : sB[k][sr];
+            FR[k] = (W0 && k == 3) ? 0.0 : sB[5 + k][sr];
         }
-        force_flux<Phys, 0>(Q1, G1, UR, FR, cf, P, F);
+        force_flux<Phys, 0, W0>(Q1, G1, UR, FR, cf, P, F);
         // interface (s, s+1) -> sA (stage A values are no longer read)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) sA[k][s] = F[k];
+        for (int k = 0; k < 5; ++k)
+            if (!(W0 && k == 3)) sA[k][s] = F[k];
         __syncthreads();
     } else {
         double FU[5];
         Phys::template flux<0>(U, FU, Phys::inv_density(U), P);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
+            if (W0 && k == 3) continue;
             sA[k][s] = U[k];
             sA[5 + k][s] = FU[k];
         }
@@ -215,35 +255,36 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
         double UR[5], FR[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            UR[k] = sA[k][sr];
-            FR[k] = sA[5 + k][sr];
+            UR[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
+            FR[k] = (W0 && k == 3) ? 0.0 : sA[5 + k][sr];
         }
-        force_flux<Phys, 0>(U, FU, UR, FR, cf, P, F);
+        force_flux<Phys, 0, W0>(U, FU, UR, FR, cf, P, F);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) sB[k][s] = F[k];
+        for (int k = 0; k < 5; ++k)
+            if (!(W0 && k == 3)) sB[k][s] = F[k];
         __syncthreads();
     }
     const bool valid = (s >= G) && (s < NT - G) && (c < nx);
     double Fl[5], Wn[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) Fl[k] = ORDER == 2 ? sA[k][sl] : sB[k][sl];
-    update_cell<Phys>(U, Fl, F, cf, P, Wn);
-    if (valid) store_cell(L.out, rowbase + c, Wn);
+    for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : (ORDER == 2 ? sA[k][sl] : sB[k][sl]);
+    update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
+    if (valid) store_cell<W0>(L.out, rowbase + c, Wn);
     if constexpr (WS) {
         double m = 0.0;
         if (valid) {
             double w = Phys::template wavespeed<DIM>(Wn, P);
             if (w > m) m = w;
         }
-        block_max_to<NT>(m, &L.step->ws_bits);
+        block_max_to<NT>(m, L.step->ws_slots);
     }
 }
 
 // ---------------------------------------------------------------------------
 // y sweep: thread per column, marching down kYRows rows
 // ---------------------------------------------------------------------------
-template <class Phys, int ORDER, int LIMMODE, bool WS>
-__global__ __launch_bounds__(kYThreads) void k_sweep_y(SweepLaunch L) {
+template <class Phys, int ORDER, int LIMMODE, bool WS, bool W0>
+__global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_y(SweepLaunch L) {
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
     const PhysParams& P = L.P;
@@ -261,23 +302,23 @@ __global__ __launch_bounds__(kYThreads) void k_sweep_y(SweepLaunch L) {
 
     if constexpr (ORDER == 2) {
         double Ua[5], Ub[5], Q1p[5], G1p[5], Fp[5], Wn[5];
-        load_cell(L.in, (long)mirror2(i0 - 2, ny) * pitch + jj, Wn);
+        load_cell<W0>(L.in, (long)mirror2(i0 - 2, ny) * pitch + jj, Wn);
         for (int r = i0 - 2; r <= i1 + 1; ++r) {
             double W[5], U[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) W[k] = Wn[k];
-            if (r < i1 + 1) load_cell(L.in, (long)mirror2(r + 1, ny) * pitch + jj, Wn);
+            if (r < i1 + 1) load_cell<W0>(L.in, (long)mirror2(r + 1, ny) * pitch + jj, Wn);
             Phys::p2c(W, U, P);
             if (r >= i0) {
                 double Q0[5], G0[5], Q1[5], G1[5];
-                half_step<Phys, 1, LIMMODE>(Ua, Ub, U, cf, L.bias, L.limiter, P, Q0, G0, Q1, G1);
+                half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cf, L.bias, L.limiter, P, Q0, G0, Q1, G1);
                 if (r >= i0 + 1) {
                     double F[5];
-                    force_flux<Phys, 1>(Q1p, G1p, Q0, G0, cf, P, F);
+                    force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cf, P, F);
                     if (r >= i0 + 2) {
                         double Wo[5];
-                        update_cell<Phys>(Ua, Fp, F, cf, P, Wo);
-                        if (colok) store_cell(L.out, (long)(r - 2) * pitch + j, Wo);
+                        update_cell<Phys, W0>(Ua, Fp, F, cf, P, Wo);
+                        if (colok) store_cell<W0>(L.out, (long)(r - 2) * pitch + j, Wo);
                         if constexpr (WS) {
                             double w = Phys::template wavespeed<2>(Wo, P);
                             if (colok && w > wsmax) wsmax = w;
@@ -300,21 +341,21 @@ __global__ __launch_bounds__(kYThreads) void k_sweep_y(SweepLaunch L) {
         }
     } else {
         double Up[5], FUp[5], Fp[5], Wn[5];
-        load_cell(L.in, (long)clamp1(i0 - 1, ny) * pitch + jj, Wn);
+        load_cell<W0>(L.in, (long)clamp1(i0 - 1, ny) * pitch + jj, Wn);
         for (int r = i0 - 1; r <= i1; ++r) {
             double W[5], U[5], FU[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) W[k] = Wn[k];
-            if (r < i1) load_cell(L.in, (long)clamp1(r + 1, ny) * pitch + jj, Wn);
+            if (r < i1) load_cell<W0>(L.in, (long)clamp1(r + 1, ny) * pitch + jj, Wn);
             Phys::p2c(W, U, P);
             Phys::template flux<1>(U, FU, Phys::inv_density(U), P);
             if (r >= i0) {
                 double F[5];
-                force_flux<Phys, 1>(Up, FUp, U, FU, cf, P, F);
+                force_flux<Phys, 1, W0>(Up, FUp, U, FU, cf, P, F);
                 if (r >= i0 + 1) {
                     double Wo[5];
-                    update_cell<Phys>(Up, Fp, F, cf, P, Wo);
-                    if (colok) store_cell(L.out, (long)(r - 1) * pitch + j, Wo);
+                    update_cell<Phys, W0>(Up, Fp, F, cf, P, Wo);
+                    if (colok) store_cell<W0>(L.out, (long)(r - 1) * pitch + j, Wo);
                     if constexpr (WS) {
                         double w = Phys::template wavespeed<2>(Wo, P);
                         if (colok && w > wsmax) wsmax = w;
@@ -330,7 +371,7 @@ __global__ __launch_bounds__(kYThreads) void k_sweep_y(SweepLaunch L) {
             }
         }
     }
-    if constexpr (WS) block_max_to<kYThreads>(wsmax, &L.step->ws_bits);
+    if constexpr (WS) block_max_to<kYThreads>(wsmax, L.step->ws_slots);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,17 +379,17 @@ __global__ __launch_bounds__(kYThreads) void k_sweep_y(SweepLaunch L) {
 // ---------------------------------------------------------------------------
 template <class Phys, int DIM>
 __global__ __launch_bounds__(256) void k_wavespeed(ConstPlanes in, int nx, int ny, long pitch, PhysParams P,
-                                                   unsigned long long* bits) {
+                                                   unsigned long long* slots) {
     double m = 0.0;
     long total = (long)nx * ny;
     for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
         long i = idx / nx, j = idx - i * nx;
         double W[5];
-        load_cell(in, i * pitch + j, W);
+        load_cell<false>(in, i * pitch + j, W);
         double w = Phys::template wavespeed<DIM>(W, P);
         if (w > m) m = w;
     }
-    block_max_to<256>(m, bits);
+    block_max_to<256>(m, slots);
 }
 
 // One iteration of the solve loop's head (Solvers/EulerSecondOrderSolver.cpp:162-172 with
@@ -356,14 +397,25 @@ __global__ __launch_bounds__(256) void k_wavespeed(ConstPlanes in, int nx, int n
 //   stop unless t < finalTime (and the step cap is not reached);
 //   dt = CFL * (dx / S_max); dt *= 0.2 while iteration <= 5; clamp to finalTime - t;
 //   t += dt, iteration += 1 (the sweeps of this step read dt from here).
-__global__ void k_step_begin(StepState* st) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One wave: lane k reads and clears slot k.
+__global__ __launch_bounds__(64) void k_step_begin(StepState* st) {
+    const int lane = threadIdx.x;
     if (st->done) return;
-    if (!(st->t < st->tf) || st->iter >= st->max_steps) {
-        st->done = 1;
+    unsigned long long* slots = st->ws_slots;
+    unsigned long long b = lane < kMaxSlots ? slots[lane * kSlotStride] : 0ull;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = o > b ? o : b;
+    }
+    const bool stop = !(st->t < st->tf) || st->iter >= st->max_steps;
+    if (stop) {
+        if (lane == 0) st->done = 1;
         return;
     }
-    double smax = __longlong_as_double((long long)st->ws_bits);
+    if (lane < kMaxSlots) slots[lane * kSlotStride] = 0ull;
+    if (lane != 0) return;
+    double smax = __longlong_as_double((long long)b);
     double dt = st->cfl * (st->dx / smax);
     if (st->iter <= 5) dt *= 0.2;
     if ((st->t + dt) > st->tf) dt = st->tf - st->t;
@@ -375,77 +427,92 @@ __global__ void k_step_begin(StepState* st) {
     st->full.dt_dx = dt / st->dx;
     st->full.dx_dt = st->dx / dt;
     st->full.hdt_dx = 0.5 * (dt / st->dx);
-    st->ws_bits = 0ull;
     st->t = st->t + dt;
     st->iter = st->iter + 1;
     st->active_steps = st->active_steps + 1;
 }
 
-__global__ void k_zero_u64(unsigned long long* p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+__global__ __launch_bounds__(256) void k_zero_slots(unsigned long long* slots) {
+    for (int k = threadIdx.x; k < kSlotWords; k += blockDim.x) slots[k] = 0ull;
 }
 
-template <class Phys, int ORDER, int LIMMODE>
+__global__ __launch_bounds__(256) void k_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag) {
+    long total = (long)nx * ny;
+    int found = 0;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        long i = idx / nx, j = idx - i * nx;
+        if (plane[i * pitch + j] != 0.0) found = 1;
+    }
+    if (__any(found) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+template <class Phys, int ORDER, int LIMMODE, bool W0>
 hipError_t dispatch_x(const SweepLaunch& L, hipStream_t s) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     const int TW = kXThreads - 2 * G;
     const int tiles = (L.nx + TW - 1) / TW;
     dim3 grid((unsigned)(tiles * L.ny)), block(kXThreads);
     if (L.fuse_ws) {
-        if (L.dim == 1) hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 1, true>), grid, block, 0, s, L);
-        else hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 2, true>), grid, block, 0, s, L);
+        if (L.dim == 1) hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 1, true, W0>), grid, block, 0, s, L);
+        else hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 2, true, W0>), grid, block, 0, s, L);
     } else {
-        hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 2, false>), grid, block, 0, s, L);
+        hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 2, false, W0>), grid, block, 0, s, L);
     }
     return hipGetLastError();
 }
 
-template <class Phys, int ORDER, int LIMMODE>
+template <class Phys, int ORDER, int LIMMODE, bool W0>
 hipError_t dispatch_y(const SweepLaunch& L, hipStream_t s) {
     const int tx = (L.nx + kYThreads - 1) / kYThreads;
     const int ty = (L.ny + kYRows - 1) / kYRows;
     dim3 grid((unsigned)(tx * ty)), block(kYThreads);
-    if (L.fuse_ws) hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, true>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, false>), grid, block, 0, s, L);
+    if (L.fuse_ws) hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, true, W0>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, false, W0>), grid, block, 0, s, L);
     return hipGetLastError();
 }
 
-template <class Phys>
-hipError_t dispatch_phys_x(const SweepLaunch& L, hipStream_t s) {
+template <template <bool> class PhysT, int AXIS>
+hipError_t dispatch_axis(const SweepLaunch& L, hipStream_t s) {
     const bool vl0 = (L.limiter == 1) && (L.bias == 0.0);
-    if (L.order == 1) return dispatch_x<Phys, 1, 0>(L, s);
-    return vl0 ? dispatch_x<Phys, 2, 1>(L, s) : dispatch_x<Phys, 2, 0>(L, s);
-}
-
-template <class Phys>
-hipError_t dispatch_phys_y(const SweepLaunch& L, hipStream_t s) {
-    const bool vl0 = (L.limiter == 1) && (L.bias == 0.0);
-    if (L.order == 1) return dispatch_y<Phys, 1, 0>(L, s);
-    return vl0 ? dispatch_y<Phys, 2, 1>(L, s) : dispatch_y<Phys, 2, 0>(L, s);
+#define COSMOS_DISPATCH(ORDER, LM, W0)                                                          \
+    return AXIS == 0 ? dispatch_x<PhysT<W0>, ORDER, LM, W0>(L, s) : dispatch_y<PhysT<W0>, ORDER, LM, W0>(L, s)
+    if (L.fast && L.physics == 0 && L.order == 2 && vl0) {
+#define COSMOS_DISPATCH_FAST(W0)                                                                \
+    return AXIS == 0 ? dispatch_x<NewtonianFastT<W0>, 2, 2, W0>(L, s) : dispatch_y<NewtonianFastT<W0>, 2, 2, W0>(L, s)
+        if (L.w_zero) COSMOS_DISPATCH_FAST(true);
+        COSMOS_DISPATCH_FAST(false);
+#undef COSMOS_DISPATCH_FAST
+    }
+    // Exact mode always carries the z-velocity: skipping it is exact only while
+    // the state stays finite (the reference's rho * (u * w) is NaN once u is).
+    if (L.order == 1) COSMOS_DISPATCH(1, 0, false);
+    if (vl0) COSMOS_DISPATCH(2, 1, false);
+    COSMOS_DISPATCH(2, 0, false);
+#undef COSMOS_DISPATCH
 }
 
 }  // namespace
 
 hipError_t launch_sweep_x(const SweepLaunch& L, hipStream_t s) {
-    return L.physics == 0 ? dispatch_phys_x<Newtonian>(L, s) : dispatch_phys_x<Relativistic>(L, s);
+    return L.physics == 0 ? dispatch_axis<NewtonianT, 0>(L, s) : dispatch_axis<RelativisticT, 0>(L, s);
 }
 
 hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s) {
-    return L.physics == 0 ? dispatch_phys_y<Newtonian>(L, s) : dispatch_phys_y<Relativistic>(L, s);
+    return L.physics == 0 ? dispatch_axis<NewtonianT, 1>(L, s) : dispatch_axis<RelativisticT, 1>(L, s);
 }
 
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
-                            const PhysParams& P, unsigned long long* out_bits, hipStream_t s) {
+                            const PhysParams& P, unsigned long long* slots, hipStream_t s) {
     long total = (long)nx * ny;
     long blocks = (total + 255) / 256;
     if (blocks > 8192) blocks = 8192;
     dim3 grid((unsigned)blocks), block(256);
     if (physics == 0) {
-        if (dim == 1) hipLaunchKernelGGL((k_wavespeed<Newtonian, 1>), grid, block, 0, s, in, nx, ny, pitch, P, out_bits);
-        else hipLaunchKernelGGL((k_wavespeed<Newtonian, 2>), grid, block, 0, s, in, nx, ny, pitch, P, out_bits);
+        if (dim == 1) hipLaunchKernelGGL((k_wavespeed<Newtonian, 1>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
+        else hipLaunchKernelGGL((k_wavespeed<Newtonian, 2>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
     } else {
-        if (dim == 1) hipLaunchKernelGGL((k_wavespeed<Relativistic, 1>), grid, block, 0, s, in, nx, ny, pitch, P, out_bits);
-        else hipLaunchKernelGGL((k_wavespeed<Relativistic, 2>), grid, block, 0, s, in, nx, ny, pitch, P, out_bits);
+        if (dim == 1) hipLaunchKernelGGL((k_wavespeed<Relativistic, 1>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
+        else hipLaunchKernelGGL((k_wavespeed<Relativistic, 2>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
     }
     return hipGetLastError();
 }
@@ -455,8 +522,16 @@ hipError_t launch_step_begin(StepState* st, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, p);
+hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_slots, dim3(1), dim3(256), 0, s, slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag, hipStream_t s) {
+    long total = (long)nx * ny;
+    long blocks = (total + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_any_nonzero, dim3((unsigned)blocks), dim3(256), 0, s, plane, nx, ny, pitch, flag);
     return hipGetLastError();
 }
 

@@ -7,6 +7,14 @@
 
 namespace cfv {
 
+// Running maxima (of non-negative doubles, as IEEE bit patterns) are merged
+// with integer atomics into kMaxSlots words, one per 128-byte line, picked by
+// block index: tens of thousands of blocks then contend on 32 addresses
+// instead of one.  Readers take the max over the slots.
+constexpr int kMaxSlots = 32;
+constexpr int kSlotStride = 16;                 // u64 words per slot (128 B)
+constexpr int kSlotWords = kMaxSlots * kSlotStride;
+
 // Per-sweep coefficients: dt/dx, dx/dt and 0.5*(dt/dx), with dt the sweep's
 // step (0.5*dt for the outer Strang x sweeps).  Formed exactly as the
 // reference forms them at each use (Solvers/EulerSolvers.cpp:163-172,
@@ -29,11 +37,11 @@ struct StepState {
     double dt;
     double cfl;
     double dx;
-    unsigned long long ws_bits;  // running max wave speed (bit pattern; >= +0)
+    unsigned long long* ws_slots;  // max wave speed of the current state (kSlotWords words)
     int iter;
     int max_steps;
     int done;
-    int active_steps;            // steps actually taken by this context
+    int active_steps;            // steps actually taken since the loop began
     SweepCoef half;              // the outer x sweeps of a 2D step: 0.5 * dt
     SweepCoef full;              // the y sweep of a 2D step and the 1D sweep: dt
 };
@@ -56,19 +64,33 @@ struct SweepLaunch {
     int order;                  // 1 = FORCE, 2 = SLIC
     int physics;                // 0 Newtonian, 1 SR
     int dim;                    // 1 or 2
+    int w_zero;                 // the z-velocity plane is identically zero: skip it
+    int fast;                   // fast (re-associated, not bitwise) Newtonian SLIC / van Leer kernels
     PhysParams P;
     SweepCoef coef;             // used when step == nullptr
     StepState* step;            // device state: coefficients and done flag
     int use_full;               // which StepState coefficients (0 half, 1 full)
-    int fuse_ws;                // accumulate max wave speed of the output into step->ws_bits
+    int fuse_ws;                // merge the max wave speed of the output into step->ws_slots
 };
 
 // All return hipSuccess or the launch error.
 hipError_t launch_sweep_x(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
-                            const PhysParams& P, unsigned long long* out_bits, hipStream_t s);
+                            const PhysParams& P, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);
-hipError_t launch_zero_u64(unsigned long long* p, hipStream_t s);
+hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s);
+// flag[0] = 1 if any value of the nx-by-ny plane (row pitch `pitch`) is nonzero, else 0.
+hipError_t launch_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag, hipStream_t s);
+
+// Max over the slots on the host (after copying the kSlotWords words back).
+inline double slots_max_host(const unsigned long long* words) {
+    unsigned long long best = 0;
+    for (int k = 0; k < kMaxSlots; ++k)
+        if (words[k * kSlotStride] > best) best = words[k * kSlotStride];
+    double v;
+    __builtin_memcpy(&v, &best, sizeof v);
+    return v;
+}
 
 }  // namespace cfv

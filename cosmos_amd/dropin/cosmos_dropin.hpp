@@ -1,0 +1,146 @@
+// Drop-in replacements for the reference's solver classes on the hot path,
+// with the same class names, static method names, argument lists and
+// by-value / in-place semantics, implemented over the C-ABI in
+// include/cosmos_fv.h (so every sweep runs as gfx950 HIP kernels).
+//
+//   reference class (file)                                  here
+//   EulerSolvers (Solvers/EulerSolvers.hpp:10-39)           wave speed, stable time step
+//   EulerFirstOrderSolver (Solvers/EulerFirstOrderSolver.hpp:8-47)
+//   EulerSecondOrderSolver (Solvers/EulerSecondOrderSolver.hpp:6-29)
+//   RelativisticEulerSolvers / ...FirstOrderSolver / ...SecondOrderSolver
+//   SlopeLimiters (Solvers/SlopeLimiters.hpp:10-35)         limiter functions and slope vectors
+//
+// The state types come from the host application's headers
+// (EulerStateVector.hpp, RelativisticEulerStateVector.hpp,
+// EulerMaterialParameters.hpp) when the drop-in is built inside it; the
+// compat/ directory supplies minimal stand-ins for standalone builds.
+//
+// Differences, all deliberate: device or argument failures throw
+// std::runtime_error (the reference has no error path); solve/solve2D run
+// the whole loop on the device and do not print the per-iteration status
+// line (EulerSolvers::outputStatus is still provided).
+#ifndef COSMOS_DROPIN_HPP
+#define COSMOS_DROPIN_HPP
+
+#include <vector>
+
+#include "EulerMaterialParameters.hpp"
+#include "EulerStateVector.hpp"
+#include "RelativisticEulerStateVector.hpp"
+
+using std::vector;
+
+class SlopeLimiters {
+public:
+    static double computeGradientRatio(double steepness, double bias);
+    static double computeSuperBeeSlopeLimiter(double steepness, double bias);
+    static double computeVanLeerSlopeLimiter(double steepness, double bias);
+    static double computeMinBeeSlopeLimiter(double steepness, double bias);
+    static double computeSlopeLimiter(double steepness, double bias, int slopeLimiter);
+    static vector<double> computeSlopeVector(vector<double> leftConservedVariableVector,
+                                             vector<double> middleConservedVariableVector,
+                                             vector<double> rightConservedVariableVector, double bias, int slopeLimiter);
+};
+
+class EulerSolvers {
+public:
+    static double computeMaximumWaveSpeed(vector<EulerStateVector>& currentCells, EulerMaterialParameters materialParameters);
+    static double computeMaximumWaveSpeed2D(vector<vector<EulerStateVector> >& currentCells,
+                                            EulerMaterialParameters materialParameters);
+    static double computeStableTimeStep(double timeStep, double currentTime, double finalTime, int currentIteration);
+    static double computeStableTimeStep(vector<EulerStateVector> currentCells, double cellSpacing, double CFLCoefficient,
+                                        double currentTime, double finalTime, int currentIteration,
+                                        EulerMaterialParameters materialParameters);
+    static double computeStableTimeStep2D(vector<vector<EulerStateVector> > currentCells, double cellSpacing,
+                                          double CFLCoefficient, double currentTime, double finalTime,
+                                          int currentIteration, EulerMaterialParameters materialParameters);
+    static void outputStatus(int currentIteration, double currentTime, double timeStep);
+};
+
+class EulerFirstOrderSolver {
+public:
+    static void computeFORCETimeStep(vector<EulerStateVector>& currentCells, double cellSpacing, double timeStep,
+                                     EulerMaterialParameters materialParameters);
+    static void computeXFORCETimeStep2D(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                        double timeStep, EulerMaterialParameters materialParameters);
+    static void computeYFORCETimeStep2D(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                        double timeStep, EulerMaterialParameters materialParameters);
+    static vector<EulerStateVector> solve(vector<EulerStateVector> initialCells, double cellSpacing,
+                                          double CFLCoefficient, double finalTime,
+                                          EulerMaterialParameters materialParameters);
+    static vector<vector<EulerStateVector> > solve2D(vector<vector<EulerStateVector> > initialCells,
+                                                     double cellSpacing, double CFLCoefficient, double finalTime,
+                                                     EulerMaterialParameters materialParameters);
+};
+
+class EulerSecondOrderSolver {
+public:
+    static void computeSLICTimeStep(vector<EulerStateVector>& currentCells, double cellSpacing, double timeStep,
+                                    double bias, int slopeLimiter, EulerMaterialParameters materialParameters);
+    static void computeXSLICTimeStep2D(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                       double timeStep, double bias, int slopeLimiter,
+                                       EulerMaterialParameters materialParameters);
+    static void computeYSLICTimeStep2D(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                       double timeStep, double bias, int slopeLimiter,
+                                       EulerMaterialParameters materialParameters);
+    static vector<EulerStateVector> solve(vector<EulerStateVector> initialCells, double cellSpacing,
+                                          double CFLCoefficient, double finalTime, double bias, int slopeLimiter,
+                                          EulerMaterialParameters materialParameters);
+    static vector<vector<EulerStateVector> > solve2D(vector<vector<EulerStateVector> > initialCells,
+                                                     double cellSpacing, double CFLCoefficient, double finalTime,
+                                                     double bias, int slopeLimiter,
+                                                     EulerMaterialParameters materialParameters);
+};
+
+class RelativisticEulerSolvers {
+public:
+    static double computeMaximumWaveSpeed(vector<RelativisticEulerStateVector>& currentCells,
+                                          EulerMaterialParameters materialParameters);
+    static double computeMaximumWaveSpeed2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                            EulerMaterialParameters materialParameters);
+    static double computeStableTimeStep(vector<RelativisticEulerStateVector> currentCells, double cellSpacing,
+                                        double CFLCoefficient, double currentTime, double finalTime,
+                                        int currentIteration, EulerMaterialParameters materialParameters);
+    static double computeStableTimeStep2D(vector<vector<RelativisticEulerStateVector> > currentCells,
+                                          double cellSpacing, double CFLCoefficient, double currentTime,
+                                          double finalTime, int currentIteration,
+                                          EulerMaterialParameters materialParameters);
+};
+
+class RelativisticEulerFirstOrderSolver {
+public:
+    static void computeFORCETimeStep(vector<RelativisticEulerStateVector>& currentCells, double cellSpacing,
+                                     double timeStep, EulerMaterialParameters materialParameters);
+    static void computeXFORCETimeStep2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                        double cellSpacing, double timeStep, EulerMaterialParameters materialParameters);
+    static void computeYFORCETimeStep2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                        double cellSpacing, double timeStep, EulerMaterialParameters materialParameters);
+    static vector<RelativisticEulerStateVector> solve(vector<RelativisticEulerStateVector> initialCells,
+                                                      double cellSpacing, double CFLCoefficient, double finalTime,
+                                                      EulerMaterialParameters materialParameters);
+    static vector<vector<RelativisticEulerStateVector> > solve2D(
+        vector<vector<RelativisticEulerStateVector> > initialCells, double cellSpacing, double CFLCoefficient,
+        double finalTime, EulerMaterialParameters materialParameters);
+};
+
+class RelativisticEulerSecondOrderSolver {
+public:
+    static void computeSLICTimeStep(vector<RelativisticEulerStateVector>& currentCells, double cellSpacing,
+                                    double timeStep, double bias, int slopeLimiter,
+                                    EulerMaterialParameters materialParameters);
+    static void computeXSLICTimeStep2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                       double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                       EulerMaterialParameters materialParameters);
+    static void computeYSLICTimeStep2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                       double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                       EulerMaterialParameters materialParameters);
+    static vector<RelativisticEulerStateVector> solve(vector<RelativisticEulerStateVector> initialCells,
+                                                      double cellSpacing, double CFLCoefficient, double finalTime,
+                                                      double bias, int slopeLimiter,
+                                                      EulerMaterialParameters materialParameters);
+    static vector<vector<RelativisticEulerStateVector> > solve2D(
+        vector<vector<RelativisticEulerStateVector> > initialCells, double cellSpacing, double CFLCoefficient,
+        double finalTime, double bias, int slopeLimiter, EulerMaterialParameters materialParameters);
+};
+
+#endif

@@ -21,6 +21,7 @@ SIGNATURES = {
                                       ctypes.c_int, ctypes.c_double]),
     "cosmos_destroy": (ctypes.c_int, [_handle]),
     "cosmos_last_error": (ctypes.c_char_p, [_handle]),
+    "cosmos_set_mode": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
     "cosmos_upload": (ctypes.c_int, [_handle, _c_double_p]),
     "cosmos_download": (ctypes.c_int, [_handle, _c_double_p]),
@@ -37,6 +38,9 @@ SIGNATURES = {
     "cosmos_progress": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, _c_int_p]),
     "cosmos_solve": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p]),
+    "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
+    "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
+                                            ctypes.c_int, _c_double_p]),
     "cosmos_synchronize": (ctypes.c_int, [_handle]),
     "cosmos_stream": (ctypes.c_void_p, [_handle]),
     "cosmos_profile_enable": (ctypes.c_int, [_handle, ctypes.c_int]),

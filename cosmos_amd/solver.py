@@ -20,6 +20,7 @@ from . import native
 NEWTONIAN = 0
 RELATIVISTIC = 1
 SUPERBEE, VANLEER, MINBEE = 0, 1, 2
+EXACT, FAST = 0, 1
 
 
 def _dp(a: np.ndarray):
@@ -78,6 +79,10 @@ class Solver:
 
     def download_device(self, ptr: int) -> None:
         self._check(self._lib.cosmos_download_device(self._h, ctypes.c_void_p(ptr)))
+
+    def set_mode(self, mode: int) -> None:
+        """EXACT (bitwise reference arithmetic, default) or FAST (Newtonian SLIC/van Leer)."""
+        self._check(self._lib.cosmos_set_mode(self._h, mode))
 
     def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
         self._check(self._lib.cosmos_set_con2prim(self._h, tolerance, max_iterations))

@@ -39,6 +39,9 @@ extern "C" {
 #define COSMOS_NEWTONIAN 0
 #define COSMOS_RELATIVISTIC 1
 
+#define COSMOS_MODE_EXACT 0
+#define COSMOS_MODE_FAST 1
+
 typedef struct cosmos_solver cosmos_solver;
 
 /* Create a solver for an nx-by-ny grid (dim 2) or an nx-cell line (dim 1,
@@ -50,6 +53,16 @@ int cosmos_destroy(cosmos_solver* s);
 /* Message for the last failure on s (s == NULL: the last cosmos_create
  * failure on this thread). */
 const char* cosmos_last_error(const cosmos_solver* s);
+
+/* Arithmetic mode (no counterpart in the reference).  COSMOS_MODE_EXACT (the
+ * default) keeps the reference's operation order: results are bitwise equal
+ * to the reference CPU path.  COSMOS_MODE_FAST re-associates the Newtonian
+ * second-order van Leer (bias 0) sweeps for throughput (one reciprocal per
+ * primitive recovery, no prim->cons->prim round trips); results then differ
+ * from the reference by rounding only, bounded by the tests at 1e-11 relative
+ * after the 100-step 4096^2 workload.  Other physics/scheme combinations
+ * always run exact. */
+int cosmos_set_mode(cosmos_solver* s, int mode);
 
 /* Special-relativistic con2prim controls (default: the reference's stopping
  * tolerance 1e-8 and cap of 1000 iterations,
@@ -103,6 +116,15 @@ int cosmos_advance(cosmos_solver* s, int n_steps);
 int cosmos_progress(cosmos_solver* s, double* time, int* steps, int* stopped);
 int cosmos_solve(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
                  double bias, int slope_limiter, double* time_out, int* steps_out);
+
+/* Batched point-wise limiter functions on host arrays (evaluated on the device):
+ *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)
+ *                         (Solvers/SlopeLimiters.cpp:3-69)
+ *   cosmos_slope_vector:  out[5i..5i+4] = SlopeLimiters::computeSlopeVector(L_i, M_i, R_i, bias, limiter)
+ *                         on conserved vectors (Solvers/SlopeLimiters.cpp:71-101) */
+int cosmos_slope_limiter(const double* steepness, long n, double bias, int slope_limiter, double* out);
+int cosmos_slope_vector(const double* left, const double* middle, const double* right, long n, double bias,
+                        int slope_limiter, double* out);
 
 /* Wait for all queued work on the handle's stream. */
 int cosmos_synchronize(cosmos_solver* s);

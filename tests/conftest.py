@@ -43,6 +43,32 @@ def same(a, b) -> bool:
     return a.shape == b.shape and bool(np.array_equal(a, b, equal_nan=True))
 
 
+def conserved(W, gamma=1.4, physics=0):
+    """Vectorised prim -> cons (Newtonian; SR with the W clamp) for error norms."""
+    W = np.asarray(W).reshape(-1, 5)
+    rho, u, v, w, p = (W[:, k] for k in range(5))
+    vsq = u * u + v * v + w * w
+    U = np.empty_like(W)
+    if physics == 0:
+        U[:, 0] = rho
+        U[:, 1:4] = rho[:, None] * W[:, 1:4]
+        U[:, 4] = rho * (0.5 * vsq + p / ((gamma - 1.0) * rho))
+    else:
+        lf = 1.0 / np.sqrt(1.0 - np.minimum(vsq, 1.0 - 1e-8))
+        h = 1.0 + (p / rho) * (gamma / (gamma - 1.0))
+        rhw = rho * h * lf * lf
+        U[:, 0] = rho * lf
+        U[:, 1:4] = rhw[:, None] * W[:, 1:4]
+        U[:, 4] = rhw - p - rho * lf
+    return U
+
+
+def cons_rel_linf(a, b, gamma=1.4, physics=0) -> float:
+    """Relative L-infinity difference per conserved variable (max |a - b| over
+    max |b| for each of the five), maximised over the variables."""
+    return rel_linf(conserved(a, gamma, physics), conserved(b, gamma, physics))
+
+
 def rel_linf(a, b) -> float:
     """Largest |a - b| over the largest |b|, per component, maximised."""
     a = np.asarray(a).reshape(-1, 5)

@@ -1,0 +1,83 @@
+// Exercises the drop-in solver classes the way the reference's scenario
+// drivers call them (Tests/EulerTests.cpp:3-30, Tests/RelativisticEulerTests.cpp),
+// writing every result as raw float64 to the file named by argv[1] for
+// tests/test_gpu_dropin.py to compare against the oracle.
+//
+// Sections, in order:
+//   1. EulerSecondOrderSolver::solve, Sod, 400 cells, CFL 0.8, t = 0.25   (400 x 5)
+//   2. EulerFirstOrderSolver::solve on the same input                      (400 x 5)
+//   3. EulerSecondOrderSolver::solve2D, 24 x 20 quadrant input, t = 0.02   (20 x 24 x 5)
+//   4. X(dt/2) Y(dt) X(dt/2) by per-sweep calls, dt from computeStableTimeStep2D (20 x 24 x 5)
+//   5. RelativisticEulerSecondOrderSolver::solve, 200 cells, t = 0.05     (200 x 5)
+//   6. EulerSolvers::computeMaximumWaveSpeed2D and the stable time step    (2)
+//   7. SlopeLimiters::computeSlopeLimiter for r = -1 .. 3, 3 limiters       (3 x 41)
+#include <cstdio>
+#include <vector>
+
+#include "cosmos_dropin.hpp"
+
+using std::vector;
+
+static void put(FILE* f, double x) { fwrite(&x, sizeof x, 1, f); }
+
+template <class S>
+static void put_cells(FILE* f, vector<S>& c) {
+    for (auto& s : c) {
+        put(f, s.getDensity());
+        put(f, s.getXVelocity());
+        put(f, s.getYVelocity());
+        put(f, s.getZVelocity());
+        put(f, s.getPressure());
+    }
+}
+
+int main(int argc, char** argv) {
+    if (argc < 2) return 2;
+    FILE* f = fopen(argv[1], "wb");
+    if (!f) return 3;
+    EulerMaterialParameters mp(1.4);
+
+    vector<EulerStateVector> sod;
+    for (int i = 0; i < 400; ++i)
+        sod.push_back(i < 200 ? EulerStateVector(1.0, 0.0, 0.0, 0.0, 1.0) : EulerStateVector(0.125, 0.0, 0.0, 0.0, 0.1));
+    vector<EulerStateVector> out = EulerSecondOrderSolver::solve(sod, 1.0 / 400, 0.8, 0.25, 0.0, 1, mp);
+    put_cells(f, out);
+    out = EulerFirstOrderSolver::solve(sod, 1.0 / 400, 0.8, 0.25, mp);
+    put_cells(f, out);
+
+    // quadrant states as in the Python workload generator, given explicitly
+    const double q[4][5] = {{1.2, 0.3, -0.4, 0.0, 1.5}, {0.4, -0.2, 0.1, 0.0, 0.6},
+                            {0.9, 0.5, 0.5, 0.0, 0.3}, {1.7, -0.6, -0.1, 0.0, 1.1}};
+    vector<vector<EulerStateVector> > grid(20, vector<EulerStateVector>(24));
+    for (int i = 0; i < 20; ++i)
+        for (int j = 0; j < 24; ++j) {
+            const double* s = q[2 * (i >= 10) + (j >= 12)];
+            grid[i][j] = EulerStateVector(s[0], s[1], s[2], s[3], s[4]);
+        }
+    vector<vector<EulerStateVector> > g2 = EulerSecondOrderSolver::solve2D(grid, 1.0 / 24, 0.9, 0.02, 0.0, 1, mp);
+    for (auto& row : g2) put_cells(f, row);
+
+    vector<vector<EulerStateVector> > g3 = grid;
+    double dt = EulerSolvers::computeStableTimeStep2D(g3, 1.0 / 24, 0.9, 0.0, 1.0, 0, mp);
+    EulerSecondOrderSolver::computeXSLICTimeStep2D(g3, 1.0 / 24, 0.5 * dt, 0.0, 1, mp);
+    EulerSecondOrderSolver::computeYSLICTimeStep2D(g3, 1.0 / 24, dt, 0.0, 1, mp);
+    EulerSecondOrderSolver::computeXSLICTimeStep2D(g3, 1.0 / 24, 0.5 * dt, 0.0, 1, mp);
+    for (auto& row : g3) put_cells(f, row);
+
+    EulerMaterialParameters mps(5.0 / 3.0);
+    vector<RelativisticEulerStateVector> sr;
+    for (int i = 0; i < 200; ++i)
+        sr.push_back(i < 100 ? RelativisticEulerStateVector(10.0, 0.0, 0.0, 0.0, 13.33)
+                             : RelativisticEulerStateVector(1.0, 0.0, 0.0, 0.0, 1e-6));
+    vector<RelativisticEulerStateVector> sro = RelativisticEulerSecondOrderSolver::solve(sr, 1.0 / 200, 0.5, 0.05, 0.0,
+                                                                                         1, mps);
+    put_cells(f, sro);
+
+    put(f, EulerSolvers::computeMaximumWaveSpeed2D(grid, mp));
+    put(f, dt);
+
+    for (int lim = 0; lim < 3; ++lim)
+        for (int k = 0; k <= 40; ++k) put(f, SlopeLimiters::computeSlopeLimiter(-1.0 + 0.1 * k, 0.0, lim));
+    fclose(f);
+    return 0;
+}

@@ -1,0 +1,64 @@
+"""The C++ drop-in classes (cosmos_amd/dropin, same class and method names as
+the reference's solvers) called the way the reference's scenario drivers
+call them, compared bitwise against the oracle.  The check program is built
+by cosmos_amd.build; this test runs it on the GPU and reads its output."""
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, same, golden
+from cosmos_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+EXE = os.path.join(ROOT, "tests", "dropin", "dropin_check")
+
+
+def test_dropin_classes_match_oracle(orc, tmp_path):
+    if not os.path.exists(EXE):
+        pytest.fail("tests/dropin/dropin_check is not built (python -m cosmos_amd.build)")
+    out = tmp_path / "dropin.bin"
+    subprocess.run([EXE, str(out)], check=True, timeout=120)
+    v = np.fromfile(out, dtype=np.float64)
+    pos = 0
+
+    def take(shape):
+        nonlocal pos
+        n = int(np.prod(shape))
+        a = v[pos:pos + n].reshape(shape)
+        pos += n
+        return a
+
+    g = golden("sod_1d.npz")
+    assert same(take((400, 5)), g["order2_output"])
+    assert same(take((400, 5)), g["order1_output"])
+
+    q = [(1.2, 0.3, -0.4, 0.0, 1.5), (0.4, -0.2, 0.1, 0.0, 0.6), (0.9, 0.5, 0.5, 0.0, 0.3), (1.7, -0.6, -0.1, 0.0, 1.1)]
+    W = np.empty((20, 24, 5))
+    for i in range(20):
+        for j in range(24):
+            W[i, j] = q[2 * (i >= 10) + (j >= 12)]
+    exp, _, _ = orc.run(W, 0, 2, 1 / 24, 0.9, tf=0.02)
+    assert same(take((20, 24, 5)), exp)
+    dt = orc.stable_dt(W, 0, 1 / 24, 0.9, 0.0, 1.0, 0)
+    e = orc.sweep(W, 0, 2, 0, 1 / 24, 0.5 * dt)
+    e = orc.sweep(e, 0, 2, 1, 1 / 24, dt)
+    e = orc.sweep(e, 0, 2, 0, 1 / 24, 0.5 * dt)
+    assert same(take((20, 24, 5)), e)
+
+    S = np.empty((200, 5))
+    S[:100] = (10.0, 0.0, 0.0, 0.0, 13.33)
+    S[100:] = (1.0, 0.0, 0.0, 0.0, 1e-6)
+    exp, _, _ = orc.run(S, 1, 2, 1 / 200, 0.5, tf=0.05, gamma=5 / 3)
+    assert same(take((200, 5)), exp)
+
+    ws, dt2 = take((2,))
+    assert ws == orc.max_wavespeed(W, 0) and dt2 == dt
+
+    lim = take((3, 41))
+    for k in range(3):
+        assert same(lim[k], [orc.limiter(-1.0 + 0.1 * i, 0.0, k) for i in range(41)])
+    assert pos == v.size

@@ -200,3 +200,57 @@ def test_nan_propagates_like_reference(orc):
         out = s.download()
     exp = orc.sweep(W, 0, 2, 0, 1 / 40, 0.002)
     assert same(out, exp) and np.isnan(out).any()
+
+
+# ---------------------------------------------------------------------------
+# Fast mode: Newtonian SLIC / van Leer re-associated for throughput.  The bar
+# is the relative L-infinity difference per conserved variable against the
+# reference arithmetic after the full step count: <= 1e-11.
+# ---------------------------------------------------------------------------
+FAST_TOL = 1e-11
+
+
+def fast_run(W, n, steps):
+    from cosmos_amd import FAST
+
+    with Solver(NEWTONIAN, 2, W.shape[1], W.shape[0], G_N) as s:
+        s.set_mode(FAST)
+        s.upload(W)
+        t, k = s.solve(2, 1.0 / n, 0.9, math.inf, steps)
+        return s.download(), t, k
+
+
+@pytest.mark.parametrize("n,seed", [(256, 0), (1024, 0), (1024, 2)])
+def test_fast_mode_100_steps_within_tolerance(orc, n, seed):
+    from conftest import cons_rel_linf
+
+    W = workloads.quadrants_2d(n, seed=seed)
+    out, t, k = fast_run(W, n, 100)
+    exp, te, ke = orc.run(W, 0, 2, 1.0 / n, 0.9, max_steps=100)
+    err = cons_rel_linf(out, exp)
+    assert k == 100 and abs(t - te) <= FAST_TOL * te
+    assert err <= FAST_TOL, err
+
+
+def test_fast_mode_with_nonzero_z_velocity(orc):
+    """The z-velocity plane is carried when it is not identically zero."""
+    from conftest import cons_rel_linf
+
+    W = workloads.quadrants_2d(300, seed=4, ny=200)
+    W[..., 3] = 0.1 * W[..., 1]
+    out, t, k = fast_run(W, 300, 30)
+    exp, te, ke = orc.run(W, 0, 2, 1.0 / 300, 0.9, max_steps=30)
+    assert cons_rel_linf(out, exp) <= FAST_TOL
+
+
+def test_fast_mode_config2_full_size_100_steps(orc):
+    """The benchmark workload itself: 4096^2, 100 steps, fast mode against the
+    reference arithmetic (the oracle, multi-threaded) within 1e-11."""
+    from conftest import cons_rel_linf
+
+    W = workloads.quadrants_2d(4096, seed=0)
+    out, t, k = fast_run(W, 4096, 100)
+    exp, te, ke = orc.run(W, 0, 2, 1.0 / 4096, 0.9, max_steps=100)
+    err = cons_rel_linf(out, exp)
+    print(f"fast mode 4096^2 x 100 steps: rel L-inf per conserved variable {err:.3e}")
+    assert k == 100 and err <= FAST_TOL, err
