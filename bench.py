@@ -37,6 +37,22 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_SWEEP = 80.0      # 5 FP64 read + 5 FP64 written
 SWEEPS_PER_STEP = 3
 KIND_NAMES = ["step_head", "sweep_x", "sweep_y", "sweep_x_wavespeed"]
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweeps_4096_traffic.json")
+
+
+def measured_traffic(size, mode, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by the copy-kernel calibration), or None when the
+    profiled workload is not this one."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+        if d["grid"] != size:
+            return None
+        k = d["kernels"][mode][kernel]
+        return float(k["fetch_bytes"] + k["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 class HipEvents:
@@ -242,7 +258,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": g["dominant_achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.size, args.mode, g["dominant"]),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+            "algorithmic_bytes": BYTES_PER_CELL_SWEEP * args.size * args.size,
         },
         "kernels": g["kernels"],
         "modes": {m: {"cell_updates_per_s": r["cell_updates_per_s"], "ms_per_step": r["ms_per_step"],

@@ -32,6 +32,7 @@ struct cosmos_solver {
     int cur = 0;
     StepState* st = nullptr;   // device
     unsigned long long* st_slots = nullptr;    // device, max wave speed slots of the solve loop
+    double* st_part = nullptr;                 // device, per-block maxima of the solve loop's last sweep
     unsigned long long* ws_scratch = nullptr;  // device, slots for cosmos_max_wave_speed
     int* flag = nullptr;                       // device, scratch flag
     double* stage = nullptr;                   // device, AoS staging for transfers (lazily allocated)
@@ -266,6 +267,8 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     if (e == hipSuccess) e = hipMalloc((void**)&s->st_slots, sizeof(unsigned long long) * cfv::kSlotWords);
     if (e == hipSuccess) e = hipMalloc((void**)&s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords);
     if (e == hipSuccess) e = hipMalloc((void**)&s->flag, sizeof(int));
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&s->st_part, sizeof(double) * (size_t)cfv::sweep_x_blocks(s->nx, s->ny, 2));
     // Zero on the handle's own stream and wait: the stream is non-blocking, so
     // a fill queued on the null stream would not be ordered before the first
     // upload into these buffers.
@@ -294,6 +297,7 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->buf[1]);
     hipFree(s->st);
     hipFree(s->st_slots);
+    hipFree(s->st_part);
     hipFree(s->ws_scratch);
     hipFree(s->flag);
     hipFree(s->stage);
@@ -464,6 +468,9 @@ int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, d
         h.done = 0;
         h.active_steps = 0;
         h.ws_slots = s->st_slots;
+        h.ws_part = s->st_part;
+        h.ws_nparts = 0;
+        h.ws_nparts_sweep = cfv::sweep_x_blocks(s->nx, s->ny, order);
         HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
         rc = max_ws_impl(s, s->st_slots);
         if (rc) return rc;
@@ -562,9 +569,21 @@ int cosmos_profile_read(cosmos_solver* s, double* total_ms, int* launches) {
 namespace {
 // A plain streaming copy (16-byte vectors), used to calibrate bandwidth and
 // hardware byte counters against a kernel of known traffic.
+// Each thread moves four 16-byte vectors, blockDim apart (coalesced), all
+// loads issued before the stores.
 __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, long n2) {
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (long)gridDim.x * blockDim.x)
-        dst[i] = src[i];
+    long base = (long)blockIdx.x * (4 * 256) + threadIdx.x;
+    double2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        long i = base + k * 256;
+        if (i < n2) v[k] = src[i];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        long i = base + k * 256;
+        if (i < n2) dst[i] = v[k];
+    }
 }
 }  // namespace
 
@@ -572,7 +591,10 @@ int cosmos_device_copy(void* dst, const void* src, long bytes, void* stream) {
     if (!dst || !src || bytes <= 0 || (bytes % 16) != 0 || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
         return fail(nullptr, COSMOS_ERR_ARGUMENT, "copy needs 16-byte aligned pointers and a multiple of 16 bytes");
     long n2 = bytes / 16;
-    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, (hipStream_t)stream, (const double2*)src, (double2*)dst, n2);
+    long blocks = (n2 + 1023) / 1024;
+    if (blocks > 0x7fffffffL) return fail(nullptr, COSMOS_ERR_ARGUMENT, "copy too large");
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (const double2*)src,
+                       (double2*)dst, n2);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(nullptr, COSMOS_ERR_DEVICE, std::string("copy launch: ") + hipGetErrorString(e));
     return COSMOS_OK;

@@ -173,6 +173,32 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* slots
     }
 }
 
+// Block-wide max of non-negative wave speeds stored to part[blockIdx.x] (no
+// atomics; the step head reduces the per-block values).  red: NT / 64 doubles
+// of LDS the caller no longer reads.
+template <int NT>
+__device__ __forceinline__ void block_max_store(double m, double* part, double* red) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off, 64);
+        m = (o > m) ? o : m;
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (NT > 64) {
+        if (lane == 0) red[wave] = m;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double b = m;
+        if (NT > 64) {
+            b = red[0];
+#pragma unroll
+            for (int w = 1; w < NT / 64; ++w) b = (red[w] > b) ? red[w] : b;
+        }
+        part[blockIdx.x] = b;
+    }
+}
+
 __device__ __forceinline__ bool resolve_coef(const SweepLaunch& L, SweepCoef& cf) {
     cf = L.coef;
     if (L.step) {
@@ -190,8 +216,13 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     constexpr int NT = kXThreads;
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TW = NT - 2 * G;
-    __shared__ double sA[10][NT];   // stage A output (U; FORCE: U and F)
-    __shared__ double sB[10][NT];   // stage B output (Q0, G0); then the interface fluxes
+    // One LDS arena of 15 rows (30 KB: five blocks per CU).
+    //   SLIC:  sA = rows 0-4   stage A U, then the interface fluxes
+    //          sB = rows 5-14  stage B Q0, G0
+    //   FORCE: sA = rows 0-9   U and F(U);  sB = rows 10-14 the interface fluxes
+    __shared__ double lds[15][NT];
+    double(*sA)[NT] = lds;
+    double(*sB)[NT] = lds + (ORDER == 2 ? 5 : 10);
 
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
@@ -276,7 +307,8 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
             double w = Phys::template wavespeed<DIM>(Wn, P);
             if (w > m) m = w;
         }
-        block_max_to<NT>(m, L.step->ws_slots);
+        // rows no longer read in stage D: sB for SLIC, sA for FORCE
+        block_max_store<NT>(m, L.step->ws_part, ORDER == 2 ? &sB[0][0] : &sA[0][0]);
     }
 }
 
@@ -371,7 +403,7 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(4, 8)
             }
         }
     }
-    if constexpr (WS) block_max_to<kYThreads>(wsmax, L.step->ws_slots);
+    if constexpr (WS) block_max_store<kYThreads>(wsmax, L.step->ws_part, nullptr);
 }
 
 // ---------------------------------------------------------------------------
@@ -397,25 +429,56 @@ __global__ __launch_bounds__(256) void k_wavespeed(ConstPlanes in, int nx, int n
 //   stop unless t < finalTime (and the step cap is not reached);
 //   dt = CFL * (dx / S_max); dt *= 0.2 while iteration <= 5; clamp to finalTime - t;
 //   t += dt, iteration += 1 (the sweeps of this step read dt from here).
-// One wave: lane k reads and clears slot k.
-__global__ __launch_bounds__(64) void k_step_begin(StepState* st) {
-    const int lane = threadIdx.x;
+// The maximum wave speed comes from the slots (written at cosmos_begin and
+// cleared here) and the per-block maxima of the previous step's last sweep.
+constexpr int kHeadThreads = 1024;
+__global__ __launch_bounds__(kHeadThreads) void k_step_begin(StepState* st) {
+    __shared__ double red[kHeadThreads / 64];
+    const int tid = threadIdx.x;
     if (st->done) return;
-    unsigned long long* slots = st->ws_slots;
-    unsigned long long b = lane < kMaxSlots ? slots[lane * kSlotStride] : 0ull;
+    double m = 0.0;
+    if (tid < kMaxSlots) {
+        unsigned long long b = st->ws_slots[tid * kSlotStride];
+        m = __longlong_as_double((long long)b);
+    }
+    const double* part = st->ws_part;
+    const int n = st->ws_nparts;
+    // eight independent loads in flight per thread
+    constexpr int kU = 8;
+    double acc[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) acc[u] = 0.0;
+    int k = tid;
+    for (; k + (kU - 1) * kHeadThreads < n; k += kU * kHeadThreads) {
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            double v = part[k + u * kHeadThreads];
+            acc[u] = (v > acc[u]) ? v : acc[u];
+        }
+    }
+    for (; k < n; k += kHeadThreads) {
+        double v = part[k];
+        acc[0] = (v > acc[0]) ? v : acc[0];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) m = (acc[u] > m) ? acc[u] : m;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = o > b ? o : b;
+        double o = __shfl_xor(m, off, 64);
+        m = (o > m) ? o : m;
     }
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
     const bool stop = !(st->t < st->tf) || st->iter >= st->max_steps;
     if (stop) {
-        if (lane == 0) st->done = 1;
+        if (tid == 0) st->done = 1;
         return;
     }
-    if (lane < kMaxSlots) slots[lane * kSlotStride] = 0ull;
-    if (lane != 0) return;
-    double smax = __longlong_as_double((long long)b);
+    if (tid < kMaxSlots) st->ws_slots[tid * kSlotStride] = 0ull;
+    if (tid != 0) return;
+    double smax = red[0];
+    for (int w = 1; w < kHeadThreads / 64; ++w) smax = (red[w] > smax) ? red[w] : smax;
+    st->ws_nparts = st->ws_nparts_sweep;
     double dt = st->cfl * (st->dx / smax);
     if (st->iter <= 5) dt *= 0.2;
     if ((st->t + dt) > st->tf) dt = st->tf - st->t;
@@ -518,8 +581,13 @@ hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, i
 }
 
 hipError_t launch_step_begin(StepState* st, hipStream_t s) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, st);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(kHeadThreads), 0, s, st);
     return hipGetLastError();
+}
+
+int sweep_x_blocks(int nx, int ny, int order) {
+    const int TW = kXThreads - 2 * (order == 2 ? 2 : 1);
+    return (nx + TW - 1) / TW * ny;
 }
 
 hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s) {

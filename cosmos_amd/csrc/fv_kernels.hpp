@@ -37,7 +37,10 @@ struct StepState {
     double dt;
     double cfl;
     double dx;
-    unsigned long long* ws_slots;  // max wave speed of the current state (kSlotWords words)
+    unsigned long long* ws_slots;  // max wave speed of the state at cosmos_begin (kSlotWords words)
+    double* ws_part;               // per-block maxima written by the last sweep of each step
+    int ws_nparts;                 // entries of ws_part to read at the next step head
+    int ws_nparts_sweep;           // blocks of the last sweep of a step
     int iter;
     int max_steps;
     int done;
@@ -79,6 +82,8 @@ hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);
+// Blocks of an x-sweep launch (the number of per-block maxima it writes).
+int sweep_x_blocks(int nx, int ny, int order);
 hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s);
 // flag[0] = 1 if any value of the nx-by-ny plane (row pitch `pitch`) is nonzero, else 0.
 hipError_t launch_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag, hipStream_t s);

@@ -37,6 +37,21 @@ void orc_flux(int physics, int axis, const double* u, double* f, double gamma);
 double orc_limiter(double r, double bias, int limiter);
 void orc_slope(const double* ul, const double* um, const double* ur, double bias, int limiter, double* out);
 
+/* General relativistic (Kerr-Schild black hole of mass M, spin a, at (bx, by, bz)).
+ * Cell (i, j) of a 2D grid sits at (j*h, i*h); a 1D cell i at (i*h, 0). */
+int orc_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, double h, double dt, double bias,
+                 int limiter, double gamma, double M, double a, double bx, double by, double bz);
+int orc_gr_forcing(int dim, double* prim, int nx, int ny, double h, double dt, double bias, int limiter, double gamma,
+                   double M, double a, double bx, double by, double bz);
+double orc_gr_max_wavespeed(int dim, const double* prim, int nx, int ny, double gamma, double M, double a, double bx,
+                            double by, double bz);
+int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, double cfl, double tf, int max_steps,
+               double bias, int limiter, int subcycles, double gamma, double M, double a, double bx, double by,
+               double bz, double* t_out);
+/* what: 0 p2c, 1 c2p, 2 x flux, 3 y flux, 4 source term, 5 sound speed, 6 metric pieces (44 values) */
+void orc_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M,
+                  double a, double bx, double by, double bz, double* out);
+
 /* Special-relativistic con2prim Newton controls.  The reference hard-codes a
  * stopping tolerance of 1e-8 and at most 1000 iterations
  * (RelativisticEuler/RelativisticEulerStateVector.cpp:59-80); other values

@@ -1,0 +1,789 @@
+/* CPU restatement of the Cosmos general-relativistic (Kerr-Schild) path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle): tests and the golden-fixture
+ * scripts use it as the checker; the product path never calls it.
+ *
+ * Follows BlackHoleEuler/BlackHoleSpacetime.cpp, BlackHoleEuler/BlackHoleEulerStateVector.cpp,
+ * BlackHoleEuler/BlackHoleEulerEigenvalues.cpp, Mathematics/MatrixAlgebra.cpp and
+ * Solvers/BlackHoleEuler{Solvers,FirstOrderSolver,SecondOrderSolver,ForcingSolver}.cpp
+ * operation by operation (explicit parentheses give the reference's
+ * evaluation order), including its quirks: the (r + (y - y_BH)) term of the
+ * Kerr-Schild 4-vector, tau with W rather than W^2 in p2c, flat |S|^2 in
+ * con2prim, the x-flux tolerance 1e-10, the unit-domain wave speed with
+ * swapped coordinates, and the per-call coordinate expressions.
+ */
+#include "cosmos_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+    double M, a, x, y, z;
+} bh_t;
+
+/* ------------------------------------------------------------------------ */
+/* Matrix algebra (Mathematics/MatrixAlgebra.cpp)                           */
+/* ------------------------------------------------------------------------ */
+
+/* computeMatrixDeterminant (:106-143): cofactor expansion along row 0. */
+static double det_n(const double* m, int n) {
+    if (n == 2) return (m[0] * m[3]) - (m[2] * m[1]);
+    double d = 0.0;
+    double cof[16];
+    for (int i = 0; i < n; ++i) {
+        for (int j = 0; j < n - 1; ++j)
+            for (int k = 0; k < n - 1; ++k) cof[j * (n - 1) + k] = (k < i) ? m[(j + 1) * n + k] : m[(j + 1) * n + k + 1];
+        if (i % 2 == 0) d += m[i] * det_n(cof, n - 1);
+        else d -= m[i] * det_n(cof, n - 1);
+    }
+    return d;
+}
+
+/* multiplyMatrices (:56-82): entries accumulate from 0.0 over k. */
+static void matmul(const double* a, const double* b, double* c, int n) {
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < n; ++k) s += a[i * n + k] * b[k * n + j];
+            c[i * n + j] = s;
+        }
+}
+
+static double trace(const double* m, int n) {
+    double t = 0.0;
+    for (int i = 0; i < n; ++i) t += m[i * n + i];
+    return t;
+}
+
+/* computeMatrixInverse (:145-194), Cayley-Hamilton for n = 3, 4.  scalar*I is
+ * formed as scalar*1.0 and scalar*0.0 (multiplyMatrix of the identity). */
+static void inverse3(const double* m, double* out) {
+    double det = det_n(m, 3);
+    double tr = trace(m, 3);
+    double m2[9];
+    matmul(m, m, m2, 3);
+    double tr2 = trace(m2, 3);
+    double c = 0.5 * ((tr * tr) - tr2);
+    double id = 1.0 / det;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double e = c * (i == j ? 1.0 : 0.0);
+            e = e - (tr * m[i * 3 + j]);
+            e = e + m2[i * 3 + j];
+            out[i * 3 + j] = id * e;
+        }
+}
+
+static void inverse4(const double* m, double* out) {
+    double det = det_n(m, 4);
+    double tr = trace(m, 4);
+    double m2[16], m3[16];
+    matmul(m, m, m2, 4);
+    double tr2 = trace(m2, 4);
+    matmul(m2, m, m3, 4);
+    double tr3 = trace(m3, 4);
+    double c3 = (1.0 / 6.0) * ((((tr * tr) * tr) - ((3.0 * tr) * tr2)) + (2.0 * tr3));
+    double c2 = 0.5 * ((tr * tr) - tr2);
+    double id = 1.0 / det;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double e = c3 * (i == j ? 1.0 : 0.0);
+            e = e - (c2 * m[i * 4 + j]);
+            e = e + (tr * m2[i * 4 + j]);
+            e = e - m3[i * 4 + j];
+            out[i * 4 + j] = id * e;
+        }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kerr-Schild spacetime (BlackHoleEuler/BlackHoleSpacetime.cpp)            */
+/* ------------------------------------------------------------------------ */
+
+static double ks_radius(const bh_t* b, double x, double y, double z) {
+    double dx = x - b->x, dy = y - b->y, dz = z - b->z;
+    double ns = ((dx * dx) + (dy * dy)) + (dz * dz);
+    double ma = (b->M * b->M) * (b->a * b->a);
+    return sqrt(0.5 * ((ns - ma) + sqrt(((ns - ma) * (ns - ma)) + (4.0 * (ma * (dz * dz))))));
+}
+
+/* computeKerrSchildScalar (:54-68) */
+static double ks_scalar(const bh_t* b, double x, double y, double z) {
+    double r = ks_radius(b, x, y, z);
+    double dz = z - b->z;
+    double ma = (b->M * b->M) * (b->a * b->a);
+    return (-(b->M * ((r * r) * r))) / ((((r * r) * r) * r) + (ma * (dz * dz)));
+}
+
+/* computeKerrSchildVector (:70-90) */
+static void ks_vector(const bh_t* b, double x, double y, double z, double* l) {
+    double r = ks_radius(b, x, y, z);
+    double dx = x - b->x, dy = y - b->y, dz = z - b->z;
+    double den = (r * r) + ((b->M * b->M) * (b->a * b->a));
+    l[0] = (-((r * dx) + ((b->a * b->M) * dy))) / den;
+    l[1] = (-((r * dy) - ((b->a * b->M) * dx))) / den;
+    l[2] = (-dz) / r;
+}
+
+/* computeKerrSchildSpacetimeVector (:92-113), with (r + (y - y_BH)) at :108 */
+static void ks_vector4(const bh_t* b, double x, double y, double z, double* l) {
+    double r = ks_radius(b, x, y, z);
+    double dx = x - b->x, dy = y - b->y, dz = z - b->z;
+    double den = (r * r) + ((b->M * b->M) * (b->a * b->a));
+    l[0] = -1.0;
+    l[1] = (-((r * dx) + ((b->a * b->M) * dy))) / den;
+    l[2] = (-((r + dy) - ((b->a * b->M) * dx))) / den;
+    l[3] = (-dz) / r;
+}
+
+/* computeKerrSchildScalarDerivative (:115-127) */
+static void ks_scalar_d(const bh_t* b, double x, double y, double z, double h, double* d) {
+    d[0] = (ks_scalar(b, x + (0.5 * h), y, z) - ks_scalar(b, x - (0.5 * h), y, z)) / h;
+    d[1] = (ks_scalar(b, x, y + (0.5 * h), z) - ks_scalar(b, x, y - (0.5 * h), z)) / h;
+    d[2] = (ks_scalar(b, x, y, z + (0.5 * h)) - ks_scalar(b, x, y, z - (0.5 * h))) / h;
+}
+
+/* computeKerrSchildVectorDerivative (:129-151): d[dir][comp] = (1/h)(l+ - l-) */
+static void ks_vector_d(const bh_t* b, double x, double y, double z, double h, double* d) {
+    double p[3], m[3];
+    double ih = 1.0 / h;
+    ks_vector(b, x + (0.5 * h), y, z, p);
+    ks_vector(b, x - (0.5 * h), y, z, m);
+    for (int i = 0; i < 3; ++i) d[0 * 3 + i] = ih * (p[i] - m[i]);
+    ks_vector(b, x, y + (0.5 * h), z, p);
+    ks_vector(b, x, y - (0.5 * h), z, m);
+    for (int i = 0; i < 3; ++i) d[1 * 3 + i] = ih * (p[i] - m[i]);
+    ks_vector(b, x, y, z + (0.5 * h), p);
+    ks_vector(b, x, y, z - (0.5 * h), m);
+    for (int i = 0; i < 3; ++i) d[2 * 3 + i] = ih * (p[i] - m[i]);
+}
+
+/* computeSpatialMetricTensor (:153-186) */
+static void gamma3(const bh_t* b, double x, double y, double z, double* g) {
+    double H = ks_scalar(b, x, y, z);
+    double l[3];
+    ks_vector(b, x, y, z, l);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) g[i * 3 + j] = (i == j ? 1.0 : 0.0) - (((2.0 * H) * l[i]) * l[j]);
+}
+
+/* computeSpacetimeMetricTensor (:188-227) */
+static void metric4(const bh_t* b, double x, double y, double z, double* g) {
+    double H = ks_scalar(b, x, y, z);
+    double l[4];
+    ks_vector4(b, x, y, z, l);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double eta = (i == j) ? (i == 0 ? -1.0 : 1.0) : 0.0;
+            g[i * 4 + j] = eta - (((2.0 * H) * l[i]) * l[j]);
+        }
+}
+
+static double gamma_det(const bh_t* b, double x, double y, double z) {
+    double g[9];
+    gamma3(b, x, y, z, g);
+    return det_n(g, 3);
+}
+
+/* computeLapseFunction (:278-281) */
+static double lapse(const bh_t* b, double x, double y, double z) {
+    return 1.0 / sqrt(1.0 - (2.0 * ks_scalar(b, x, y, z)));
+}
+
+/* computeShiftVector (:283-296) */
+static void shift(const bh_t* b, double x, double y, double z, double* s) {
+    double H = ks_scalar(b, x, y, z);
+    double l[3];
+    ks_vector(b, x, y, z, l);
+    for (int i = 0; i < 3; ++i) s[i] = ((2.0 * H) / (1.0 - (2.0 * H))) * l[i];
+}
+
+/* computeLapseFunctionDerivative (:298-310) */
+static void lapse_d(const bh_t* b, double x, double y, double z, double h, double* d) {
+    d[0] = (lapse(b, x + (0.5 * h), y, z) - lapse(b, x - (0.5 * h), y, z)) / h;
+    d[1] = (lapse(b, x, y + (0.5 * h), z) - lapse(b, x, y - (0.5 * h), z)) / h;
+    d[2] = (lapse(b, x, y, z + (0.5 * h)) - lapse(b, x, y, z - (0.5 * h))) / h;
+}
+
+/* computeShiftVectorDerivative (:312-334): d[dir][comp] */
+static void shift_d(const bh_t* b, double x, double y, double z, double h, double* d) {
+    double p[3], m[3];
+    double ih = 1.0 / h;
+    shift(b, x + (0.5 * h), y, z, p);
+    shift(b, x - (0.5 * h), y, z, m);
+    for (int i = 0; i < 3; ++i) d[0 * 3 + i] = ih * (p[i] - m[i]);
+    shift(b, x, y + (0.5 * h), z, p);
+    shift(b, x, y - (0.5 * h), z, m);
+    for (int i = 0; i < 3; ++i) d[1 * 3 + i] = ih * (p[i] - m[i]);
+    shift(b, x, y, z + (0.5 * h), p);
+    shift(b, x, y, z - (0.5 * h), m);
+    for (int i = 0; i < 3; ++i) d[2 * 3 + i] = ih * (p[i] - m[i]);
+}
+
+/* computeSpatialMetricTensorDerivative (:250-276): d[dir][i][j] */
+static void gamma_d(const bh_t* b, double x, double y, double z, double h, double* d) {
+    double p[9], m[9];
+    double ih = 1.0 / h;
+    gamma3(b, x + (0.5 * h), y, z, p);
+    gamma3(b, x - (0.5 * h), y, z, m);
+    for (int i = 0; i < 9; ++i) d[0 * 9 + i] = ih * (p[i] - m[i]);
+    gamma3(b, x, y + (0.5 * h), z, p);
+    gamma3(b, x, y - (0.5 * h), z, m);
+    for (int i = 0; i < 9; ++i) d[1 * 9 + i] = ih * (p[i] - m[i]);
+    gamma3(b, x, y, z + (0.5 * h), p);
+    gamma3(b, x, y, z - (0.5 * h), m);
+    for (int i = 0; i < 9; ++i) d[2 * 9 + i] = ih * (p[i] - m[i]);
+}
+
+/* computeExtrinsicCurvatureTensor (:336-370) */
+static void curvature(const bh_t* b, double x, double y, double z, double h, double* K) {
+    double al = lapse(b, x, y, z);
+    double H = ks_scalar(b, x, y, z);
+    double l[3], dH[3], dl[9];
+    ks_vector(b, x, y, z, l);
+    ks_scalar_d(b, x, y, z, h, dH);
+    ks_vector_d(b, x, y, z, h, dl);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double k = al * ((((-(l[i] * dH[j])) - (l[j] * dH[i])) - (H * dl[j * 3 + i])) - (H * dl[i * 3 + j]));
+            for (int q = 0; q < 3; ++q) {
+                k += ((al * 2.0) * (H * H)) * (((l[i] * l[q]) * dl[q * 3 + j]) + ((l[j] * l[q]) * dl[q * 3 + i]));
+                k += (((((al * 2.0) * H) * l[i]) * l[j]) * l[q]) * dH[q];
+            }
+            K[i * 3 + j] = k;
+        }
+}
+
+/* inExcisionRegion (:372-386) */
+static int excised(const bh_t* b, double x, double y, double z) {
+    double dx = x - b->x, dy = y - b->y, dz = z - b->z;
+    double r = sqrt(((dx * dx) + (dy * dy)) + (dz * dz));
+    return r <= (b->M * (1.0 + sqrt(1.0 - (b->a * b->a))));
+}
+
+/* ------------------------------------------------------------------------ */
+/* GR state (BlackHoleEuler/BlackHoleEulerStateVector.cpp)                  */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double rho, v[3], p, lf, h;
+} c2p_t;
+
+/* The con2prim shared by setConservedVariableVector (:30-103), the fluxes and
+ * the source term: divide by sqrt(det gamma), then the special-relativistic
+ * quartic Newton with flat |S|^2. */
+static void gr_con2prim(const double* U, double sdet, double g, double tol, c2p_t* o) {
+    double D = U[0] / sdet, Sx = U[1] / sdet, Sy = U[2] / sdet, Sz = U[3] / sdet, tau = U[4] / sdet;
+    double q = ((tau + D) * (tau + D)) - (((Sx * Sx) + (Sy * Sy)) + (Sz * Sz));
+    double dc = D / sqrt(q);
+    double ec = (D + tau) / sqrt(q);
+    if (q < 1e-8) {
+        dc = D / sqrt(1e-8);
+        ec = (D + tau) / sqrt(1e-8);
+    }
+    double gm1g = (g - 1.0) / g;
+    double c0 = -1.0 / (g * g);
+    double c1 = (-2.0 * dc) * ((g - 1.0) / (g * g));
+    double c2 = ((((g - 2.0) / g) * ((ec * ec) - 1.0)) + 1.0) - (((dc * dc) * gm1g) * gm1g);
+    double c4 = (ec * ec) - 1.0;
+    double eta = (2.0 * dc) * gm1g;
+    double x = 1.0;
+    for (int it = 0; it < 1000; ++it) {
+        double x2 = x * x, x3 = x2 * x;
+        double f = ((((c4 * x3) * (x - eta)) + (c2 * x2)) + (c1 * x)) + c0;
+        double fd = ((c1 + ((2.0 * c2) * x)) + ((4.0 * c4) * x3)) - (((3.0 * eta) * c4) * x2);
+        double xn = x - (f / fd);
+        if (fabs(x - xn) < tol) break;
+        x = xn;
+    }
+    double lf = ((0.5 * ec) * x) * (1.0 + sqrt(1.0 + ((4.0 * gm1g) * ((1.0 - (dc * x)) / ((ec * ec) * (x * x))))));
+    double h = 1.0 / (dc * x);
+    double rho = D / lf;
+    double rhw = (rho * h) * (lf * lf);
+    o->v[0] = Sx / rhw;
+    o->v[1] = Sy / rhw;
+    o->v[2] = Sz / rhw;
+    double p = (rhw - D) - tau;
+    o->rho = (rho < 1e-8) ? 1e-8 : rho;
+    o->p = (p < 1e-8) ? 1e-8 : p;
+    o->lf = lf;
+    o->h = h;
+}
+
+/* computeConservedVariableVector (:118-157), tau with W (not W^2) at :154 */
+static void gr_p2c(const bh_t* b, const double* W, double x, double y, double z, double g, double* U) {
+    double gm[9];
+    gamma3(b, x, y, z, gm);
+    double det = det_n(gm, 3);
+    double vsq = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) vsq += (gm[i * 3 + j] * W[1 + i]) * W[1 + j];
+    double lf = 1.0 / sqrt(1.0 - vsq);
+    if (vsq > 1.0 - 1e-8) lf = 1.0 / sqrt(1.0 - 1e-8);
+    double h = 1.0 + ((W[4] / W[0]) * (g / (g - 1.0)));
+    double sd = sqrt(det);
+    U[0] = (sd * W[0]) * lf;
+    for (int k = 0; k < 3; ++k) U[1 + k] = (((sd * W[0]) * h) * (lf * lf)) * W[1 + k];
+    U[4] = sd * ((((W[0] * h) * lf) - W[4]) - (W[0] * lf));
+}
+
+static void gr_c2p(const bh_t* b, const double* U, double x, double y, double z, double g, double* W) {
+    c2p_t o;
+    gr_con2prim(U, sqrt(gamma_det(b, x, y, z)), g, 1e-8, &o);
+    /* setConservedVariableVector keeps the unfloored velocities and floors rho, p */
+    W[0] = o.rho;
+    W[1] = o.v[0];
+    W[2] = o.v[1];
+    W[3] = o.v[2];
+    W[4] = o.p;
+}
+
+/* compute{X,Y}FluxVector(cons) (:159-257, :259-357): x tolerance 1e-10, y 1e-8 */
+static void gr_flux(const bh_t* b, int axis, const double* U, double x, double y, double z, double g, double* F) {
+    double sdet = sqrt(gamma_det(b, x, y, z));
+    double al = lapse(b, x, y, z);
+    double be[3];
+    shift(b, x, y, z, be);
+    c2p_t o;
+    gr_con2prim(U, sdet, g, axis == 0 ? 1e-10 : 1e-8, &o);
+    double rho = o.rho, p = o.p, lf = o.lf, h = o.h;
+    double as = al * sqrt(gamma_det(b, x, y, z));
+    double vt = o.v[axis] - (be[axis] / al);
+    double rhw = (rho * h) * (lf * lf);
+    F[0] = as * ((rho * lf) * vt);
+    for (int k = 0; k < 3; ++k) {
+        double t = rhw * (o.v[k] * vt);
+        if (k == axis) t = t + p;
+        F[1 + k] = as * t;
+    }
+    F[4] = as * ((((rhw - p) - (rho * lf)) * vt) + (p * o.v[axis]));
+}
+
+/* computeSourceTermVector (:359-510) */
+static void gr_source(const bh_t* b, const double* U, double x, double y, double z, double h, double g, double* S) {
+    double sdet = sqrt(gamma_det(b, x, y, z));
+    double al = lapse(b, x, y, z);
+    double be[3];
+    shift(b, x, y, z, be);
+    c2p_t o;
+    gr_con2prim(U, sdet, g, 1e-8, &o);
+    double rho = o.rho, p = o.p, lf = o.lf, hh = o.h;
+    double g4[16], gi[16];
+    metric4(b, x, y, z, g4);
+    inverse4(g4, gi);
+    double u[4];
+    u[0] = lf / al;
+    for (int k = 0; k < 3; ++k) u[1 + k] = ((al * u[0]) * o.v[k]) - (be[k] * u[0]);
+    double T[16];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) T[i * 4 + j] = (((rho * hh) * u[i]) * u[j]) + (p * gi[i * 4 + j]);
+    double da[3], db[9], gm[9], dg[27], K[9];
+    lapse_d(b, x, y, z, h, da);
+    shift_d(b, x, y, z, h, db);
+    gamma3(b, x, y, z, gm);
+    gamma_d(b, x, y, z, h, dg);
+    curvature(b, x, y, z, h, K);
+    double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double vc[3];
+    for (int i = 0; i < 3; ++i) vc[i] = ((gm[i * 3 + 0] * o.v[0]) + (gm[i * 3 + 1] * o.v[1])) + (gm[i * 3 + 2] * o.v[2]);
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            for (int k = 0; k < 3; ++k) {
+                double d = dg[i * 9 + j * 3 + k];
+                s[i + 1] += T[0] * (((0.5 * be[j]) * be[k]) * d);
+                s[i + 1] += (T[j + 1] * be[k]) * d;
+                s[i + 1] += 0.5 * (T[(j + 1) * 4 + (k + 1)] * d);
+            }
+            s[i + 1] += ((((rho * hh) * (lf * lf)) * vc[j]) / al) * db[i * 3 + j];
+        }
+        s[i + 1] -= (T[0] * al) * da[i];
+    }
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            s[4] += T[0] * ((be[i] * be[j]) * K[i * 3 + j]);
+            s[4] += T[i + 1] * ((2.0 * be[j]) * K[i * 3 + j]);
+            s[4] += T[(i + 1) * 4 + (j + 1)] * K[i * 3 + j];
+        }
+        s[4] -= T[0] * (be[i] * da[i]);
+        s[4] -= T[i + 1] * da[i];
+    }
+    double f = al * sqrt(gamma_det(b, x, y, z));
+    for (int k = 0; k < 5; ++k) S[k] = f * s[k];
+}
+
+/* computeSoundSpeed (:519-549) with BlackHoleEulerEigenvalues (:3-96) */
+static double gr_sound_speed(const bh_t* b, const double* W, double x, double y, double z, double g) {
+    double al = lapse(b, x, y, z);
+    double be[3], gm[9], gi[9];
+    shift(b, x, y, z, be);
+    gamma3(b, x, y, z, gm);
+    inverse3(gm, gi);
+    double num = (g * W[4]) / W[0];
+    double den = 1.0 + ((W[4] / W[0]) * (g / (g - 1.0)));
+    double cs = sqrt(num / den);
+    double vsq = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) vsq += (gm[i * 3 + j] * W[1 + i]) * W[1 + j];
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        double v = W[1 + i];
+        double mat = (al * v) - be[i];
+        double pre = al / (1.0 - (vsq * (cs * cs)));
+        double rt = cs * sqrt((1.0 - vsq) * ((gi[i * 3 + i] * (1.0 - (vsq * (cs * cs)))) - ((v * v) * (1.0 - (cs * cs)))));
+        double fast = (pre * ((v * (1.0 - (cs * cs))) + rt)) - be[i];
+        double slow = (pre * ((v * (1.0 - (cs * cs))) - rt)) - be[i];
+        if (mat > s) s = mat;
+        if (fast > s) s = fast;
+        if (slow > s) s = slow;
+    }
+    return s;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Solver building blocks (Solvers/BlackHoleEulerSolvers.cpp, ...)          */
+/* ------------------------------------------------------------------------ */
+
+/* point offset along an axis */
+static void at(int axis, double x, double y, double off, double* px, double* py) {
+    *px = axis == 0 ? x + off : x;
+    *py = axis == 0 ? y : y + off;
+}
+
+/* SlopeLimiters::computeSlopeVector{X,Y} (Solvers/SlopeLimiters.cpp:123-147) */
+static void gr_slope(const bh_t* b, int axis, const double* L, const double* M, const double* R, double x, double y,
+                     double h, double bias, int lim, double g, double* s) {
+    double ul[5], um[5], ur[5], px, py;
+    at(axis, x, y, -h, &px, &py);
+    gr_p2c(b, L, px, py, 0.0, g, ul);
+    gr_p2c(b, M, x, y, 0.0, g, um);
+    at(axis, x, y, h, &px, &py);
+    gr_p2c(b, R, px, py, 0.0, g, ur);
+    orc_slope(ul, um, ur, bias, lim, s);
+}
+
+/* evolveStateByHalf{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:219-307): fluxes
+ * and the evolved-state c2p at x -/+ h*0.5 */
+static void gr_half_step(const bh_t* b, int axis, const double* L, const double* M, const double* R, double h,
+                         double dt, double bias, int lim, int side, double x, double y, double g, double* out) {
+    double s[5], um[5], xl[5], xr[5], fl[5], fr[5], px, py;
+    gr_slope(b, axis, L, M, R, x, y, h, bias, lim, g, s);
+    gr_p2c(b, M, x, y, 0.0, g, um);
+    for (int k = 0; k < 5; ++k) {
+        xl[k] = um[k] - (0.5 * s[k]);
+        xr[k] = um[k] + (0.5 * s[k]);
+    }
+    at(axis, x, y, -(h * 0.5), &px, &py);
+    gr_flux(b, axis, xl, px, py, 0.0, g, fl);
+    at(axis, x, y, h * 0.5, &px, &py);
+    gr_flux(b, axis, xr, px, py, 0.0, g, fr);
+    double f = 0.5 * (dt / h);
+    double e[5];
+    for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (f * (fl[k] - fr[k]));
+    if (axis == 0) {
+        px = side == 0 ? x - (h * 0.5) : x + (h * 0.5);
+        py = y;
+    } else {
+        px = x;
+        py = side == 0 ? y - (h * 0.5) : y + (h * 0.5);
+    }
+    gr_c2p(b, e, px, py, 0.0, g, out);
+}
+
+/* evolveStateByFractional{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:159-217):
+ * fluxes at x -/+ h, c2p at x */
+static void gr_fractional(const bh_t* b, int axis, double frac, const double* L, const double* M, const double* R,
+                          double h, double dt, double bias, int lim, double x, double y, double g, double* out) {
+    double s[5], um[5], xl[5], xr[5], fl[5], fr[5], px, py, e[5];
+    gr_slope(b, axis, L, M, R, x, y, h, bias, lim, g, s);
+    gr_p2c(b, M, x, y, 0.0, g, um);
+    for (int k = 0; k < 5; ++k) {
+        xl[k] = um[k] - (0.5 * s[k]);
+        xr[k] = um[k] + (0.5 * s[k]);
+    }
+    at(axis, x, y, -h, &px, &py);
+    gr_flux(b, axis, xl, px, py, 0.0, g, fl);
+    at(axis, x, y, h, &px, &py);
+    gr_flux(b, axis, xr, px, py, 0.0, g, fr);
+    double f = frac * (dt / h);
+    for (int k = 0; k < 5; ++k) e[k] = um[k] + (f * (fl[k] - fr[k]));
+    gr_c2p(b, e, x, y, 0.0, g, out);
+}
+
+/* BlackHoleEulerFirstOrderSolver::compute{X,Y}FORCEFlux (Solvers/BlackHoleEulerFirstOrderSolver.cpp:3-99):
+ * states p2c'd at the interface -/+ h*0.5, the Richtmyer state's flux at the interface */
+static void gr_force(const bh_t* b, int axis, const double* L, const double* R, double x, double y, double h, double dt,
+                     double g, double* F) {
+    double ul[5], ur[5], fl[5], fr[5], us[5], fs[5], lx, ly, rx, ry;
+    at(axis, x, y, -(h * 0.5), &lx, &ly);
+    at(axis, x, y, h * 0.5, &rx, &ry);
+    gr_p2c(b, L, lx, ly, 0.0, g, ul);
+    gr_p2c(b, R, rx, ry, 0.0, g, ur);
+    gr_flux(b, axis, ul, lx, ly, 0.0, g, fl);
+    gr_flux(b, axis, ur, rx, ry, 0.0, g, fr);
+    for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + ((dt / h) * (fl[k] - fr[k])));
+    gr_flux(b, axis, us, x, y, 0.0, g, fs);
+    for (int k = 0; k < 5; ++k) {
+        double lf = 0.5 * ((fl[k] + fr[k]) + ((h / dt) * (ul[k] - ur[k])));
+        F[k] = 0.5 * (lf + fs[k]);
+    }
+}
+
+/* BlackHoleEulerSecondOrderSolver::compute{X,Y}SLICFlux (Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-35) */
+static void gr_slic(const bh_t* b, int axis, const double* LL, const double* L, const double* R, const double* RR,
+                    double h, double dt, double bias, int lim, double x, double y, double g, double* F) {
+    double pr[5], pl[5], px, py;
+    at(axis, x, y, -(h * 0.5), &px, &py);
+    gr_half_step(b, axis, LL, L, R, h, dt, bias, lim, 1, px, py, g, pr);
+    at(axis, x, y, h * 0.5, &px, &py);
+    gr_half_step(b, axis, L, R, RR, h, dt, bias, lim, 0, px, py, g, pl);
+    gr_force(b, axis, pr, pl, x, y, h, dt, g, F);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Grid sweeps                                                              */
+/* ------------------------------------------------------------------------ */
+
+static const double* cell2(const double* prim, int nx, int ny, int i, int j, int ghosts) {
+    /* transmissive ghosts of insertBoundaryCells2D(.., 1 or 2); i = row, j = column */
+    if (ghosts == 2) {
+        if (j < 0) j = -j - 1;
+        if (j >= nx) j = 2 * nx - 1 - j;
+        if (i < 0) i = -i - 1;
+        if (i >= ny) i = 2 * ny - 1 - i;
+    } else {
+        if (j < 0) j = 0;
+        if (j >= nx) j = nx - 1;
+        if (i < 0) i = 0;
+        if (i >= ny) i = ny - 1;
+    }
+    return prim + 5 * ((long)i * nx + j);
+}
+
+/* excision over the stencil the sweep uses: +-w along x and (2D) along y */
+static int stencil_clear(const bh_t* b, int dim, int i, int j, int w, double h) {
+    if (excised(b, j * h, dim == 2 ? i * h : 0.0, 0.0)) return 0;
+    for (int d = 1; d <= w; ++d) {
+        if (excised(b, (j - d) * h, dim == 2 ? i * h : 0.0, 0.0)) return 0;
+        if (excised(b, (j + d) * h, dim == 2 ? i * h : 0.0, 0.0)) return 0;
+        if (dim == 2) {
+            if (excised(b, j * h, (i - d) * h, 0.0)) return 0;
+            if (excised(b, j * h, (i + d) * h, 0.0)) return 0;
+        }
+    }
+    return 1;
+}
+
+/* BlackHoleEuler{First,Second}OrderSolver::compute*TimeStep{,2D} (FORCE: ..FirstOrderSolver.cpp:101-181,
+ * SLIC: ..SecondOrderSolver.cpp:37-143).  In 1D the coordinate of cell i is i*h (y = 0);
+ * in 2D cell (i, j) is at (j*h, i*h).  The excision stencil is the reference's:
+ * 1D +-1 (FORCE) / +-2 (SLIC) cells; 2D the same along x and y for either sweep. */
+int orc_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, double h, double dt, double bias,
+                 int lim, double gamma, double M, double a, double bx, double by, double bz) {
+    bh_t b = {M, a, bx, by, bz};
+    if (dim == 1) ny = 1, axis = 0;
+    int gh = order == 2 ? 2 : 1;
+    double* out = (double*)malloc(sizeof(double) * 5 * (size_t)nx * ny);
+    if (!out) return -1;
+    memcpy(out, prim, sizeof(double) * 5 * (size_t)nx * ny);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < ny; ++i) {
+        for (int j = 0; j < nx; ++j) {
+            if (!stencil_clear(&b, dim, i, j, gh, h)) continue;
+            int c = axis == 0 ? j : i;
+            double x = dim == 2 ? j * h : j * h, y = dim == 2 ? i * h : 0.0;
+            double U[5], Fl[5], Fr[5], un[5];
+            const double* W = prim + 5 * ((long)i * nx + j);
+            gr_p2c(&b, W, x, y, 0.0, gamma, U);
+#define NB(d) (axis == 0 ? cell2(prim, nx, ny, i, j + (d), gh) : cell2(prim, nx, ny, i + (d), j, gh))
+            double lx, ly, rx, ry;
+            if (axis == 0) {
+                lx = (c - 0.5) * h; ly = y; rx = (c + 0.5) * h; ry = y;
+            } else {
+                lx = x; ly = (c - 0.5) * h; rx = x; ry = (c + 0.5) * h;
+            }
+            if (order == 2) {
+                gr_slic(&b, axis, NB(-2), NB(-1), NB(0), NB(1), h, dt, bias, lim, lx, ly, gamma, Fl);
+                gr_slic(&b, axis, NB(-1), NB(0), NB(1), NB(2), h, dt, bias, lim, rx, ry, gamma, Fr);
+            } else {
+                gr_force(&b, axis, NB(-1), NB(0), lx, ly, h, dt, gamma, Fl);
+                gr_force(&b, axis, NB(0), NB(1), rx, ry, h, dt, gamma, Fr);
+            }
+#undef NB
+            for (int k = 0; k < 5; ++k) un[k] = U[k] + ((dt / h) * (Fl[k] - Fr[k]));
+            gr_c2p(&b, un, x, y, 0.0, gamma, out + 5 * ((long)i * nx + j));
+        }
+    }
+    memcpy(prim, out, sizeof(double) * 5 * (size_t)nx * ny);
+    free(out);
+    return 0;
+}
+
+/* BlackHoleEulerForcingSolver::evolveConservedVariableVector{,2D} (Solvers/BlackHoleEulerForcingSolver.cpp:3-185) */
+static void gr_rk4(const bh_t* b, int dim, const double* uL, const double* uM, const double* uR, const double* uT,
+                   const double* uB, double x, double y, double h, double dt, double bias, int lim, double g,
+                   double* out) {
+    double WL[5], WR[5], WT[5], WB[5], Wm[5], s1[5], s2[5], s3[5], s4[5], k1[5], k2[5], k3[5], tmp[5], e1[5], e2[5],
+        e3[5], um[5];
+    gr_c2p(b, uL, x - h, y, 0.0, g, WL);
+    gr_c2p(b, uR, x + h, y, 0.0, g, WR);
+    if (dim == 2) {
+        gr_c2p(b, uT, x, y - h, 0.0, g, WT);
+        gr_c2p(b, uB, x, y + h, 0.0, g, WB);
+    }
+    const double fr1[3] = {0.5, 0.5, 1.0};   /* 1D fractions of stages 1..3 */
+    const double fx2[3] = {0.25, 0.25, 0.5}; /* 2D x fractions */
+    const double fy2[3] = {0.5, 0.5, 1.0};   /* 2D y fractions */
+    double* stage_src[3] = {s1, s2, s3};
+    double* stage_k[3] = {k1, k2, k3};
+    gr_source(b, uM, x, y, 0.0, h, g, s1);
+    for (int k = 0; k < 5; ++k) k1[k] = dt * s1[k];
+    for (int st = 0; st < 3; ++st) {
+        double* kk = stage_k[st];
+        /* U + 0.5 k (stages 1, 2) or U + k (stage 3) */
+        for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[k]) : uM[k] + kk[k];
+        gr_c2p(b, tmp, x, y, 0.0, g, Wm);
+        if (dim == 1) {
+            gr_fractional(b, 0, fr1[st], WL, Wm, WR, h, dt, bias, lim, x, y, g, e1);
+        } else {
+            gr_fractional(b, 0, fx2[st], WL, Wm, WR, h, dt, bias, lim, x, y, g, e1);
+            gr_fractional(b, 1, fy2[st], WT, e1, WB, h, dt, bias, lim, x, y, g, e2);
+            gr_fractional(b, 0, fx2[st], WL, e2, WR, h, dt, bias, lim, x, y, g, e3);
+            memcpy(e1, e3, sizeof e1);
+        }
+        gr_p2c(b, e1, x, y, 0.0, g, um);
+        double* src = st == 0 ? s2 : (st == 1 ? s3 : s4);
+        gr_source(b, um, x, y, 0.0, h, g, src);
+        double* knext = st == 0 ? k2 : (st == 1 ? k3 : NULL);
+        if (knext) for (int k = 0; k < 5; ++k) knext[k] = dt * src[k];
+    }
+    (void)stage_src;
+    double k4[5];
+    for (int k = 0; k < 5; ++k) k4[k] = dt * s4[k];
+    for (int k = 0; k < 5; ++k)
+        out[k] = uM[k] + ((1.0 / 6.0) * (k1[k] + ((2.0 * k2[k]) + ((2.0 * k3[k]) + k4[k]))));
+}
+
+/* computeRungeKuttaTimeStep{,2D} (Solvers/BlackHoleEulerForcingSolver.cpp:187-250) */
+int orc_gr_forcing(int dim, double* prim, int nx, int ny, double h, double dt, double bias, int lim, double gamma,
+                   double M, double a, double bx, double by, double bz) {
+    bh_t b = {M, a, bx, by, bz};
+    if (dim == 1) ny = 1;
+    double* out = (double*)malloc(sizeof(double) * 5 * (size_t)nx * ny);
+    if (!out) return -1;
+    memcpy(out, prim, sizeof(double) * 5 * (size_t)nx * ny);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < ny; ++i) {
+        for (int j = 0; j < nx; ++j) {
+            if (!stencil_clear(&b, dim, i, j, 1, h)) continue;
+            double x = j * h, y = dim == 2 ? i * h : 0.0;
+            double uL[5], uM[5], uR[5], uT[5], uB[5], un[5];
+            gr_p2c(&b, cell2(prim, nx, ny, i, j - 1, 1), (j - 1) * h, y, 0.0, gamma, uL);
+            gr_p2c(&b, cell2(prim, nx, ny, i, j, 1), x, y, 0.0, gamma, uM);
+            gr_p2c(&b, cell2(prim, nx, ny, i, j + 1, 1), (j + 1) * h, y, 0.0, gamma, uR);
+            if (dim == 2) {
+                gr_p2c(&b, cell2(prim, nx, ny, i - 1, j, 1), x, (i - 1) * h, 0.0, gamma, uT);
+                gr_p2c(&b, cell2(prim, nx, ny, i + 1, j, 1), x, (i + 1) * h, 0.0, gamma, uB);
+            }
+            gr_rk4(&b, dim, uL, uM, uR, uT, uB, x, y, h, dt, bias, lim, gamma, un);
+            gr_c2p(&b, un, x, y, 0.0, gamma, out + 5 * ((long)i * nx + j));
+        }
+    }
+    memcpy(prim, out, sizeof(double) * 5 * (size_t)nx * ny);
+    free(out);
+    return 0;
+}
+
+/* computeMaximumWaveSpeed{,2D} (Solvers/BlackHoleEulerSolvers.cpp:88-140): unit-domain
+ * spacing 1/N, excision at (j/nx, i/ny), sound speed at (i/ny, j/nx). */
+double orc_gr_max_wavespeed(int dim, const double* prim, int nx, int ny, double gamma, double M, double a, double bx,
+                            double by, double bz) {
+    bh_t b = {M, a, bx, by, bz};
+    double best = 0.0;
+    if (dim == 1) {
+        double d = 1.0 / nx;
+        for (int i = 0; i < nx; ++i) {
+            if (excised(&b, i * d, 0.0, 0.0)) continue;
+            double ws = fabs(gr_sound_speed(&b, prim + 5 * i, i * d, 0.0, 0.0, gamma));
+            if (ws > best) best = ws;
+        }
+        return best;
+    }
+    double dr = 1.0 / ny, dc = 1.0 / nx;
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) {
+            if (excised(&b, j * dc, i * dr, 0.0)) continue;
+            double ws = fabs(gr_sound_speed(&b, prim + 5 * ((long)i * nx + j), i * dr, j * dc, 0.0, gamma));
+            if (ws > best) best = ws;
+        }
+    return best;
+}
+
+/* BlackHoleEuler{First,Second}OrderSolver::solve{,2D} loops (..FirstOrderSolver.cpp:183-253,
+ * ..SecondOrderSolver.cpp:145-218), capped at max_steps: forcing(dt/2N) x N, the flux
+ * step (Strang in 2D), forcing(dt/2N) x N.  The forcing uses bias 0 and van Leer. */
+int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, double cfl, double tf, int max_steps,
+               double bias, int lim, int subcycles, double gamma, double M, double a, double bx, double by, double bz,
+               double* t_out) {
+    double t = 0.0;
+    int it = 0;
+    if (dim == 1) ny = 1;
+    while (t < tf && it < max_steps) {
+        double ws = orc_gr_max_wavespeed(dim, prim, nx, ny, gamma, M, a, bx, by, bz);
+        double dt = orc_stable_dt_scalar(cfl * (h / ws), t, tf, it);
+        for (int s = 0; s < subcycles; ++s)
+            orc_gr_forcing(dim, prim, nx, ny, h, 0.5 * (dt / subcycles), 0.0, 1, gamma, M, a, bx, by, bz);
+        if (dim == 1) {
+            orc_gr_sweep(order, 0, 1, prim, nx, 1, h, dt, bias, lim, gamma, M, a, bx, by, bz);
+        } else {
+            orc_gr_sweep(order, 0, 2, prim, nx, ny, h, dt * 0.5, bias, lim, gamma, M, a, bx, by, bz);
+            orc_gr_sweep(order, 1, 2, prim, nx, ny, h, dt, bias, lim, gamma, M, a, bx, by, bz);
+            orc_gr_sweep(order, 0, 2, prim, nx, ny, h, dt * 0.5, bias, lim, gamma, M, a, bx, by, bz);
+        }
+        for (int s = 0; s < subcycles; ++s)
+            orc_gr_forcing(dim, prim, nx, ny, h, 0.5 * (dt / subcycles), 0.0, 1, gamma, M, a, bx, by, bz);
+        t += dt;
+        it += 1;
+    }
+    if (t_out) *t_out = t;
+    return it;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Single-call entry points for known-answer tests                          */
+/* ------------------------------------------------------------------------ */
+void orc_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M, double a,
+                  double bx, double by, double bz, double* out) {
+    bh_t b = {M, a, bx, by, bz};
+    switch (what) {
+        case 0: gr_p2c(&b, in, x, y, z, gamma, out); break;
+        case 1: gr_c2p(&b, in, x, y, z, gamma, out); break;
+        case 2: gr_flux(&b, 0, in, x, y, z, gamma, out); break;
+        case 3: gr_flux(&b, 1, in, x, y, z, gamma, out); break;
+        case 4: gr_source(&b, in, x, y, z, h, gamma, out); break;
+        case 5: out[0] = gr_sound_speed(&b, in, x, y, z, gamma); break;
+        case 6: { /* metric pieces: H, l[3], alpha, beta[3], det gamma, gamma^{ij}[9], g^{mu nu}[16], K[9] */
+            double l[3], be[3], gm[9], gi[9], g4[16], g4i[16], K[9];
+            out[0] = ks_scalar(&b, x, y, z);
+            ks_vector(&b, x, y, z, l);
+            out[1] = l[0]; out[2] = l[1]; out[3] = l[2];
+            out[4] = lapse(&b, x, y, z);
+            shift(&b, x, y, z, be);
+            out[5] = be[0]; out[6] = be[1]; out[7] = be[2];
+            gamma3(&b, x, y, z, gm);
+            out[8] = det_n(gm, 3);
+            inverse3(gm, gi);
+            memcpy(out + 9, gi, sizeof gi);
+            metric4(&b, x, y, z, g4);
+            inverse4(g4, g4i);
+            memcpy(out + 18, g4i, sizeof g4i);
+            curvature(&b, x, y, z, h, K);
+            memcpy(out + 34, K, sizeof K);
+            out[43] = excised(&b, x, y, z);
+            break;
+        }
+        default: break;
+    }
+}

@@ -338,3 +338,162 @@ void ref_slope(const double* ul, const double* um, const double* ur, double bias
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// General relativistic (BlackHoleEuler/): the same flat layout; the black hole
+// is BlackHoleSpacetime(M, a, x, y, z) (BlackHoleEuler/BlackHoleSpacetime.cpp:43-52).
+// ---------------------------------------------------------------------------
+#include "Solvers/BlackHoleEulerFirstOrderSolver.hpp"
+#include "Solvers/BlackHoleEulerForcingSolver.hpp"
+#include "Solvers/BlackHoleEulerSecondOrderSolver.hpp"
+
+extern "C" {
+
+int ref_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, double h, double dt, double bias,
+                 int lim, double gamma, double M, double a, double bx, double by, double bz) {
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    typedef BlackHoleEulerStateVector S;
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        if (order == 1) BlackHoleEulerFirstOrderSolver::computeFORCETimeStep(c, h, dt, mp, bh);
+        else BlackHoleEulerSecondOrderSolver::computeSLICTimeStep(c, h, dt, bias, lim, mp, bh);
+        from_1d(c, prim);
+        return 0;
+    }
+    std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+    if (order == 1) {
+        if (axis == 0) BlackHoleEulerFirstOrderSolver::computeXFORCETimeStep2D(c, h, dt, mp, bh);
+        else BlackHoleEulerFirstOrderSolver::computeYFORCETimeStep2D(c, h, dt, mp, bh);
+    } else {
+        if (axis == 0) BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(c, h, dt, bias, lim, mp, bh);
+        else BlackHoleEulerSecondOrderSolver::computeYSLICTimeStep2D(c, h, dt, bias, lim, mp, bh);
+    }
+    from_2d(c, prim);
+    return 0;
+}
+
+int ref_gr_forcing(int dim, double* prim, int nx, int ny, double h, double dt, double bias, int lim, double gamma,
+                   double M, double a, double bx, double by, double bz) {
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    typedef BlackHoleEulerStateVector S;
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep(c, h, dt, bias, lim, mp, bh);
+        from_1d(c, prim);
+        return 0;
+    }
+    std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+    BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(c, h, dt, bias, lim, mp, bh);
+    from_2d(c, prim);
+    return 0;
+}
+
+double ref_gr_max_wavespeed(int dim, const double* prim, int nx, int ny, double gamma, double M, double a, double bx,
+                            double by, double bz) {
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    typedef BlackHoleEulerStateVector S;
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        return BlackHoleEulerSolvers::computeMaximumWaveSpeed(c, mp, bh);
+    }
+    std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+    return BlackHoleEulerSolvers::computeMaximumWaveSpeed2D(c, mp, bh);
+}
+
+// The solve / solve2D loops (Solvers/BlackHoleEulerSecondOrderSolver.cpp:145-218,
+// Solvers/BlackHoleEulerFirstOrderSolver.cpp:183-253), capped at max_steps.
+int ref_gr_run(int order, int dim, double* prim, int nx, int ny, double h, double cfl, double tf, int max_steps,
+               double bias, int lim, int subcycles, double gamma, double M, double a, double bx, double by, double bz,
+               double* t_out) {
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    typedef BlackHoleEulerStateVector S;
+    double t = 0.0;
+    int it = 0;
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        while (t < tf && it < max_steps) {
+            double dt = BlackHoleEulerSolvers::computeStableTimeStep(c, h, cfl, t, tf, it, mp, bh);
+            for (int s = 0; s < subcycles; ++s)
+                BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep(c, h, 0.5 * (dt / subcycles), 0.0, 1, mp, bh);
+            if (order == 1) BlackHoleEulerFirstOrderSolver::computeFORCETimeStep(c, h, dt, mp, bh);
+            else BlackHoleEulerSecondOrderSolver::computeSLICTimeStep(c, h, dt, bias, lim, mp, bh);
+            for (int s = 0; s < subcycles; ++s)
+                BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep(c, h, 0.5 * (dt / subcycles), 0.0, 1, mp, bh);
+            t += dt;
+            it += 1;
+        }
+        from_1d(c, prim);
+    } else {
+        std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+        while (t < tf && it < max_steps) {
+            double dt = BlackHoleEulerSolvers::computeStableTimeStep2D(c, h, cfl, t, tf, it, mp, bh);
+            for (int s = 0; s < subcycles; ++s)
+                BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(c, h, 0.5 * (dt / subcycles), 0.0, 1, mp, bh);
+            if (order == 1) {
+                BlackHoleEulerFirstOrderSolver::computeXFORCETimeStep2D(c, h, dt * 0.5, mp, bh);
+                BlackHoleEulerFirstOrderSolver::computeYFORCETimeStep2D(c, h, dt, mp, bh);
+                BlackHoleEulerFirstOrderSolver::computeXFORCETimeStep2D(c, h, dt * 0.5, mp, bh);
+            } else {
+                BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(c, h, dt * 0.5, bias, lim, mp, bh);
+                BlackHoleEulerSecondOrderSolver::computeYSLICTimeStep2D(c, h, dt, bias, lim, mp, bh);
+                BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(c, h, dt * 0.5, bias, lim, mp, bh);
+            }
+            for (int s = 0; s < subcycles; ++s)
+                BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(c, h, 0.5 * (dt / subcycles), 0.0, 1, mp, bh);
+            t += dt;
+            it += 1;
+        }
+        from_2d(c, prim);
+    }
+    if (t_out) *t_out = t;
+    return it;
+}
+
+// what: 0 p2c, 1 c2p, 2 x flux, 3 y flux, 4 source term, 5 sound speed,
+// 6 metric pieces (H, l[3], alpha, beta[3], det gamma, gamma^{ij}[9],
+// g^{mu nu}[16], K[9], excised) -- 44 values.
+void ref_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M,
+                  double a, double bx, double by, double bz, double* out) {
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    std::vector<double> v(in, in + 5), r;
+    BlackHoleEulerStateVector s(in[0], in[1], in[2], in[3], in[4]);
+    switch (what) {
+        case 0: r = s.computeConservedVariableVector(x, y, z, mp, bh); break;
+        case 1: {
+            BlackHoleEulerStateVector t;
+            t.setConservedVariableVector(v, x, y, z, mp, bh);
+            store_state(t, out);
+            return;
+        }
+        case 2: r = BlackHoleEulerStateVector::computeXFluxVector(v, x, y, z, mp, bh); break;
+        case 3: r = BlackHoleEulerStateVector::computeYFluxVector(v, x, y, z, mp, bh); break;
+        case 4: r = BlackHoleEulerStateVector::computeSourceTermVector(v, x, y, z, h, mp, bh); break;
+        case 5: out[0] = s.computeSoundSpeed(x, y, z, mp, bh); return;
+        case 6: {
+            out[0] = bh.computeKerrSchildScalar(x, y, z);
+            std::vector<double> l = bh.computeKerrSchildVector(x, y, z);
+            out[1] = l[0]; out[2] = l[1]; out[3] = l[2];
+            out[4] = bh.computeLapseFunction(x, y, z);
+            std::vector<double> be = bh.computeShiftVector(x, y, z);
+            out[5] = be[0]; out[6] = be[1]; out[7] = be[2];
+            out[8] = bh.computeSpatialMetricDeterminant(x, y, z);
+            std::vector<std::vector<double> > gi = bh.computeInverseSpatialMetricTensor(x, y, z);
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[9 + 3 * i + j] = gi[i][j];
+            std::vector<std::vector<double> > g4 = bh.computeInverseSpacetimeMetricTensor(x, y, z);
+            for (int i = 0; i < 4; ++i) for (int j = 0; j < 4; ++j) out[18 + 4 * i + j] = g4[i][j];
+            std::vector<std::vector<double> > K = bh.computeExtrinsicCurvatureTensor(x, y, z, h);
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) out[34 + 3 * i + j] = K[i][j];
+            out[43] = bh.inExcisionRegion(x, y, z) ? 1.0 : 0.0;
+            return;
+        }
+        default: return;
+    }
+    for (size_t k = 0; k < r.size(); ++k) out[k] = r[k];
+}
+
+}  // extern "C"
