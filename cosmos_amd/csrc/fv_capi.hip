@@ -38,6 +38,9 @@ struct cosmos_solver {
     double* stage = nullptr;                   // device, AoS staging for transfers (lazily allocated)
     bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
     int mode = COSMOS_MODE_EXACT;
+    cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
+    double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
+    long gr_fpitch = 0;
     hipStream_t stream = nullptr;
     int device = 0;
     // solve-loop parameters set by cosmos_begin.  Steps are queued without
@@ -88,6 +91,21 @@ cfv::PhysParams make_params(double g) {
     P.sr_4gm1g = 4.0 * ((g - 1.0) / g);
     P.inv_gm1 = 1.0 / (g - 1.0);
     return P;
+}
+
+// BlackHoleSpacetime(M, a, x, y, z) and the constants its accessors form
+// (BlackHoleEuler/BlackHoleSpacetime.cpp:54-113, 372-386).
+cfv::Hole make_hole(double M, double a, double x, double y, double z) {
+    cfv::Hole b{};
+    b.M = M;
+    b.a = a;
+    b.x = x;
+    b.y = y;
+    b.z = z;
+    b.ma = (M * M) * (a * a);
+    b.aM = a * M;
+    b.r_exc = M * (1.0 + std::sqrt(1.0 - (a * a)));
+    return b;
 }
 
 Planes planes_of(cosmos_solver* s, int b) {
@@ -233,6 +251,71 @@ int guarded(cosmos_solver* s, F&& f) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// General relativistic (COSMOS_BLACKHOLE) launches
+// ---------------------------------------------------------------------------
+cfv::GRLaunch gr_launch(cosmos_solver* s, double h) {
+    cfv::GRLaunch L{};
+    L.in = cplanes_of(s, s->cur);
+    L.out = planes_of(s, s->cur ^ 1);
+    L.nx = s->nx;
+    L.ny = s->ny;
+    L.pitch = s->pitch;
+    L.dim = s->dim;
+    L.h = h;
+    L.P = s->P;
+    L.hole = s->hole;
+    if (s->gr_flux) {
+        size_t plane = (size_t)(s->ny + 1) * s->gr_fpitch;
+        for (int k = 0; k < 5; ++k) L.flux.p[k] = s->gr_flux + k * plane;
+        L.fpitch = s->gr_fpitch;
+    }
+    return L;
+}
+
+int ensure_gr_flux(cosmos_solver* s) {
+    if (s->gr_flux) return COSMOS_OK;
+    // x sweeps: ny rows of nx + 1 interfaces; y sweeps: ny + 1 rows of nx
+    s->gr_fpitch = ((long)s->nx + 1 + 15) / 16 * 16;
+    HIPCHK(s, hipMalloc((void**)&s->gr_flux, sizeof(double) * 5 * (size_t)(s->ny + 1) * s->gr_fpitch));
+    return COSMOS_OK;
+}
+
+int gr_sweep(cosmos_solver* s, int order, int axis, double h, double dt, double bias, int limiter) {
+    int rc = ensure_gr_flux(s);
+    if (rc) return rc;
+    cfv::GRLaunch L = gr_launch(s, h);
+    L.order = order;
+    L.bias = bias;
+    L.limiter = limiter;
+    L.coef.dt_dx = dt / h;
+    L.coef.dx_dt = h / dt;
+    L.coef.hdt_dx = 0.5 * (dt / h);
+    hipError_t e = timed(s, axis == 0 ? kProfSweepX : kProfSweepY, [&] { return cfv::launch_gr_sweep(L, axis, s->stream); });
+    if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("GR sweep launch: ") + hipGetErrorString(e));
+    s->cur ^= 1;
+    return COSMOS_OK;
+}
+
+// The stage fractions of evolveConservedVariableVector{,2D}
+// (Solvers/BlackHoleEulerForcingSolver.cpp:21-53, 98-167), each times dt / h as
+// computeFractionalEvolutionVector forms it (Solvers/EulerSolvers.cpp:163-167).
+int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) {
+    cfv::GRLaunch L = gr_launch(s, h);
+    L.bias = bias;
+    L.limiter = limiter;
+    L.dt = dt;
+    static const double f1[3] = {0.5, 0.5, 1.0}, fx[3] = {0.25, 0.25, 0.5}, fy[3] = {0.5, 0.5, 1.0};
+    for (int k = 0; k < 3; ++k) {
+        L.frac_dt_h[k] = (s->dim == 1 ? f1[k] : fx[k]) * (dt / h);
+        L.frac_dt_h_y[k] = fy[k] * (dt / h);
+    }
+    hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_gr_forcing(L, s->stream); });
+    if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("GR forcing launch: ") + hipGetErrorString(e));
+    s->cur ^= 1;
+    return COSMOS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -240,8 +323,9 @@ extern "C" {
 int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, double adiabatic_index) {
     if (!out) return fail(nullptr, COSMOS_ERR_ARGUMENT, "out is null");
     *out = nullptr;
-    if (physics != COSMOS_NEWTONIAN && physics != COSMOS_RELATIVISTIC)
-        return fail(nullptr, COSMOS_ERR_ARGUMENT, "physics must be COSMOS_NEWTONIAN or COSMOS_RELATIVISTIC");
+    if (physics != COSMOS_NEWTONIAN && physics != COSMOS_RELATIVISTIC && physics != COSMOS_BLACKHOLE)
+        return fail(nullptr, COSMOS_ERR_ARGUMENT,
+                    "physics must be COSMOS_NEWTONIAN, COSMOS_RELATIVISTIC or COSMOS_BLACKHOLE");
     if (dim != 1 && dim != 2) return fail(nullptr, COSMOS_ERR_ARGUMENT, "dim must be 1 or 2");
     if (dim == 1) ny = 1;
     if (nx < 1 || ny < 1) return fail(nullptr, COSMOS_ERR_ARGUMENT, "grid must have at least one cell");
@@ -258,6 +342,7 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     s->ny = ny;
     s->pitch = ((long)nx + 15) / 16 * 16;
     s->P = make_params(adiabatic_index);
+    s->hole = make_hole(1.0, 0.0, 0.5, 0.5, 0.0);
     hipError_t e = hipGetDevice(&s->device);
     size_t bytes = sizeof(double) * 5 * (size_t)ny * s->pitch;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -301,6 +386,7 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->ws_scratch);
     hipFree(s->flag);
     hipFree(s->stage);
+    hipFree(s->gr_flux);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return COSMOS_OK;
@@ -325,6 +411,15 @@ int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations) 
             return fail(s, COSMOS_ERR_ARGUMENT, "con2prim tolerance must be > 0 and max_iterations >= 1");
         s->P.c2p_tol = tolerance;
         s->P.c2p_maxit = max_iterations;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_set_spacetime(cosmos_solver* s, double mass, double spin, double x, double y, double z) {
+    return guarded(s, [&] {
+        if (s->physics != COSMOS_BLACKHOLE)
+            return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_set_spacetime needs a COSMOS_BLACKHOLE solver");
+        s->hole = make_hole(mass, spin, x, y, z);
         return COSMOS_OK;
     });
 }
@@ -400,6 +495,9 @@ int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, dou
         if (axis == 1 && s->dim == 1) return fail(s, COSMOS_ERR_ARGUMENT, "a 1D solver has no y sweep");
         rc = settle(s);
         if (rc) return rc;
+        s->run_ready = false;
+        if (s->physics == COSMOS_BLACKHOLE)
+            return gr_sweep(s, order, axis, cell_spacing, time_step, bias, slope_limiter);
         SweepLaunch L = base_launch(s, s->cur, order, bias, slope_limiter);
         L.coef.dt_dx = time_step / cell_spacing;
         L.coef.dx_dt = cell_spacing / time_step;
@@ -416,9 +514,13 @@ static int max_ws_impl(cosmos_solver* s, unsigned long long* slots_dev) {
     int rc = settle(s);
     if (rc) return rc;
     hipError_t e = cfv::launch_zero_slots(slots_dev, s->stream);
-    if (e == hipSuccess)
-        e = cfv::launch_wavespeed(cplanes_of(s, s->cur), s->nx, s->ny, s->pitch, s->physics, s->dim, s->P, slots_dev,
-                                  s->stream);
+    if (e == hipSuccess) {
+        if (s->physics == COSMOS_BLACKHOLE)
+            e = cfv::launch_gr_wavespeed(gr_launch(s, 0.0), slots_dev, s->stream);
+        else
+            e = cfv::launch_wavespeed(cplanes_of(s, s->cur), s->nx, s->ny, s->pitch, s->physics, s->dim, s->P,
+                                      slots_dev, s->stream);
+    }
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("wave speed launch: ") + hipGetErrorString(e));
     return COSMOS_OK;
 }
@@ -458,6 +560,8 @@ int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, d
     return guarded(s, [&] {
         int rc = check_order(s, order);
         if (rc) return rc;
+        if (s->physics == COSMOS_BLACKHOLE)
+            return fail(s, COSMOS_ERR_ARGUMENT, "black-hole solvers step with cosmos_solve_forced");
         StepState h{};
         h.t = 0.0;
         h.tf = final_time;
@@ -525,6 +629,56 @@ int cosmos_solve(cosmos_solver* s, int order, double cell_spacing, double cfl, d
         }
         if (time_out) *time_out = t;
         if (steps_out) *steps_out = steps;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_forcing(cosmos_solver* s, double cell_spacing, double time_step, double bias, int slope_limiter) {
+    return guarded(s, [&] {
+        if (s->physics != COSMOS_BLACKHOLE)
+            return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_forcing needs a COSMOS_BLACKHOLE solver");
+        int rc = settle(s);
+        if (rc) return rc;
+        s->run_ready = false;
+        return gr_forcing(s, cell_spacing, time_step, bias, slope_limiter);
+    });
+}
+
+int cosmos_solve_forced(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time,
+                        int max_steps, double bias, int slope_limiter, int subcycles, double* time_out,
+                        int* steps_out) {
+    return guarded(s, [&] {
+        if (s->physics != COSMOS_BLACKHOLE)
+            return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_solve_forced needs a COSMOS_BLACKHOLE solver");
+        int rc = check_order(s, order);
+        if (rc) return rc;
+        if (subcycles < 0) return fail(s, COSMOS_ERR_ARGUMENT, "subcycles must be >= 0");
+        double t = 0.0;
+        int it = 0;
+        const int cap = max_steps < 0 ? INT_MAX : max_steps;
+        while (t < final_time && it < cap) {
+            double dt = 0.0;
+            rc = cosmos_stable_time_step(s, cell_spacing, cfl, t, final_time, it, &dt);
+            if (rc) return rc;
+            // forcing(dt / 2N) x N, flux step, forcing(dt / 2N) x N, with the forcing's
+            // bias 0 and van Leer fixed as in the reference
+            for (int k = 0; k < subcycles; ++k)
+                if ((rc = gr_forcing(s, cell_spacing, 0.5 * (dt / subcycles), 0.0, 1))) return rc;
+            if (s->dim == 1) {
+                if ((rc = gr_sweep(s, order, 0, cell_spacing, dt, bias, slope_limiter))) return rc;
+            } else {
+                if ((rc = gr_sweep(s, order, 0, cell_spacing, dt * 0.5, bias, slope_limiter))) return rc;
+                if ((rc = gr_sweep(s, order, 1, cell_spacing, dt, bias, slope_limiter))) return rc;
+                if ((rc = gr_sweep(s, order, 0, cell_spacing, dt * 0.5, bias, slope_limiter))) return rc;
+            }
+            for (int k = 0; k < subcycles; ++k)
+                if ((rc = gr_forcing(s, cell_spacing, 0.5 * (dt / subcycles), 0.0, 1))) return rc;
+            t += dt;
+            it += 1;
+        }
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (time_out) *time_out = t;
+        if (steps_out) *steps_out = it;
         return COSMOS_OK;
     });
 }

@@ -144,35 +144,6 @@ __device__ __forceinline__ void update_cell(const double* U, const double* Fl, c
     Phys::c2p(un, W, Phys::inv_density(un), P);
 }
 
-// Block-wide max of non-negative wave speeds, merged into one of the slots
-// with an integer atomic max (the IEEE bit patterns of non-negative doubles
-// order like the values).  The reference's running max skips NaN
-// (ws > max is false), so NaN never enters.
-template <int NT>
-__device__ __forceinline__ void block_max_to(double m, unsigned long long* slots) {
-    __shared__ double red[NT / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        double o = __shfl_xor(m, off, 64);
-        m = (o > m) ? o : m;
-    }
-    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (NT > 64) {
-        if (lane == 0) red[wave] = m;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        double b = m;
-        if (NT > 64) {
-            b = red[0];
-#pragma unroll
-            for (int w = 1; w < NT / 64; ++w) b = (red[w] > b) ? red[w] : b;
-        }
-        if (b > 0.0)
-            atomicMax(slots + (blockIdx.x % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(b));
-    }
-}
-
 // Block-wide max of non-negative wave speeds stored to part[blockIdx.x] (no
 // atomics; the step head reduces the per-block values).  red: NT / 64 doubles
 // of LDS the caller no longer reads.

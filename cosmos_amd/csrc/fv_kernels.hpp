@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fv_gr.hpp"
 #include "fv_physics.hpp"
 
 namespace cfv {
@@ -14,6 +15,35 @@ namespace cfv {
 constexpr int kMaxSlots = 32;
 constexpr int kSlotStride = 16;                 // u64 words per slot (128 B)
 constexpr int kSlotWords = kMaxSlots * kSlotStride;
+
+// Block-wide max of non-negative wave speeds, merged into one of the slots
+// with an integer atomic max (the IEEE bit patterns of non-negative doubles
+// order like the values).  The reference's running max skips NaN
+// (ws > max is false), so NaN never enters.
+template <int NT>
+__device__ __forceinline__ void block_max_to(double m, unsigned long long* slots) {
+    __shared__ double red[NT / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off, 64);
+        m = (o > m) ? o : m;
+    }
+    int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (NT > 64) {
+        if (lane == 0) red[wave] = m;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double b = m;
+        if (NT > 64) {
+            b = red[0];
+#pragma unroll
+            for (int w = 1; w < NT / 64; ++w) b = (red[w] > b) ? red[w] : b;
+        }
+        if (b > 0.0)
+            atomicMax(slots + (blockIdx.x % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(b));
+    }
+}
 
 // Per-sweep coefficients: dt/dx, dx/dt and 0.5*(dt/dx), with dt the sweep's
 // step (0.5*dt for the outer Strang x sweeps).  Formed exactly as the
@@ -87,6 +117,35 @@ int sweep_x_blocks(int nx, int ny, int order);
 hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s);
 // flag[0] = 1 if any value of the nx-by-ny plane (row pitch `pitch`) is nonzero, else 0.
 hipError_t launch_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// General relativistic (Kerr-Schild) sweeps, forcing and wave speed.  Each
+// sweep is two launches: the FORCE / SLIC flux of every interface into the
+// flux planes, then the conservative update of every cell.
+// ---------------------------------------------------------------------------
+struct GRLaunch {
+    ConstPlanes in;
+    Planes out;
+    Planes flux;                // interface fluxes: x sweep [ny][fpitch], y sweep [ny + 1][fpitch]
+    int nx, ny;
+    long pitch;                 // doubles per state row
+    long fpitch;                // doubles per flux row (>= nx + 1)
+    int dim;                    // 1 or 2
+    int order;                  // 1 = FORCE, 2 = SLIC
+    int limiter;
+    double bias;
+    double h;                   // cell spacing (also the finite-difference spacing of the source)
+    SweepCoef coef;             // from the sweep's (or the forcing's) time step
+    double dt;                  // the forcing's time step
+    double frac_dt_h[3];        // RK4 stage fractions times dt / h (1D: 1/2 1/2 1; 2D x: 1/4 1/4 1/2)
+    double frac_dt_h_y[3];      // 2D y: 1/2 1/2 1
+    PhysParams P;
+    Hole hole;
+};
+
+hipError_t launch_gr_sweep(const GRLaunch& L, int axis, hipStream_t s);
+hipError_t launch_gr_forcing(const GRLaunch& L, hipStream_t s);
+hipError_t launch_gr_wavespeed(const GRLaunch& L, unsigned long long* slots, hipStream_t s);
 
 // Max over the slots on the host (after copying the kSlotWords words back).
 inline double slots_max_host(const unsigned long long* words) {

@@ -23,6 +23,8 @@ SIGNATURES = {
     "cosmos_last_error": (ctypes.c_char_p, [_handle]),
     "cosmos_set_mode": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
+    "cosmos_set_spacetime": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_double]),
     "cosmos_upload": (ctypes.c_int, [_handle, _c_double_p]),
     "cosmos_download": (ctypes.c_int, [_handle, _c_double_p]),
     "cosmos_upload_device": (ctypes.c_int, [_handle, ctypes.c_void_p]),
@@ -38,6 +40,10 @@ SIGNATURES = {
     "cosmos_progress": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, _c_int_p]),
     "cosmos_solve": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p]),
+    "cosmos_forcing": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "cosmos_solve_forced": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_double_p,
+                                            _c_int_p]),
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
                                             ctypes.c_int, _c_double_p]),

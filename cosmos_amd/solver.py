@@ -5,6 +5,8 @@ Method names follow the reference's static solver methods they stand for:
     max_wave_speed    computeMaximumWaveSpeed{,2D}
     stable_time_step  computeStableTimeStep{,2D}
     solve             solve / solve2D (to a final time, or a fixed step count)
+    forcing           BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep{,2D}
+    solve_forced      BlackHoleEuler{First,Second}OrderSolver::solve{,2D}
 States are numpy float64 arrays of primitives, shape (ny, nx, 5) in 2D or
 (nx, 5) in 1D, k in (rho, u, v, w, p): vector<vector<StateVector>> flattened.
 """
@@ -19,6 +21,7 @@ from . import native
 
 NEWTONIAN = 0
 RELATIVISTIC = 1
+BLACKHOLE = 2
 SUPERBEE, VANLEER, MINBEE = 0, 1, 2
 EXACT, FAST = 0, 1
 
@@ -87,6 +90,11 @@ class Solver:
     def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
         self._check(self._lib.cosmos_set_con2prim(self._h, tolerance, max_iterations))
 
+    def set_spacetime(self, mass: float = 1.0, spin: float = 0.0, x: float = 0.5, y: float = 0.5,
+                      z: float = 0.0) -> None:
+        """BlackHoleSpacetime(mass, spin, x, y, z) of a BLACKHOLE solver."""
+        self._check(self._lib.cosmos_set_spacetime(self._h, mass, spin, x, y, z))
+
     # -- numerics ---------------------------------------------------------------
     def sweep(self, order: int, axis: int, cell_spacing: float, time_step: float, bias: float = 0.0,
               slope_limiter: int = VANLEER) -> None:
@@ -123,6 +131,18 @@ class Solver:
         t, n = ctypes.c_double(), ctypes.c_int()
         self._check(self._lib.cosmos_solve(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
                                            slope_limiter, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def forcing(self, cell_spacing: float, time_step: float, bias: float = 0.0,
+                slope_limiter: int = VANLEER) -> None:
+        self._check(self._lib.cosmos_forcing(self._h, cell_spacing, time_step, bias, slope_limiter))
+
+    def solve_forced(self, order: int, cell_spacing: float, cfl: float, final_time: float = math.inf,
+                     max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER, subcycles: int = 1):
+        """Returns (time reached, steps taken)."""
+        t, n = ctypes.c_double(), ctypes.c_int()
+        self._check(self._lib.cosmos_solve_forced(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
+                                                  slope_limiter, subcycles, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
 
     def synchronize(self) -> None:

@@ -120,3 +120,74 @@ def random_states(count: int, seed: int, physics: int = 0) -> np.ndarray:
             vel = [rng.uniform(-0.5, 0.5) for _ in range(3)]
         W[c] = (rho, vel[0], vel[1], vel[2], p)
     return W
+
+
+# ---------------------------------------------------------------------------
+# General relativistic (black hole) workloads.  Coordinates follow the
+# reference: cell (i, j) of a 2D grid sits at (j*h, i*h), a 1D cell j at (j*h, 0).
+# A hole is (mass, spin, x, y, z).
+# ---------------------------------------------------------------------------
+def splitmix_uniform(seed: int, count: int) -> np.ndarray:
+    """The first `count` SplitMix64 draws of `seed` as U[0, 1) (vectorised;
+    the same numbers as SplitMix64(seed).uniform(0, 1) called count times)."""
+    k = np.arange(1, count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & _MASK) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
+def gaussian_random_field(n: int, seed: int, correlation: float, extent: float) -> np.ndarray:
+    """Unit-variance periodic Gaussian random field on an n x n grid of side
+    `extent`, with Gaussian correlation exp(-r^2 / (2 l^2)), l = correlation:
+    SplitMix64 white noise (Box-Muller) filtered by exp(-(k l)^2 / 4)."""
+    u = splitmix_uniform(seed, 2 * n * n)
+    u1 = 1.0 - u[0::2]
+    u2 = u[1::2]
+    noise = (np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * math.pi * u2)).reshape(n, n)
+    k = 2.0 * math.pi * np.fft.fftfreq(n, d=extent / n)
+    k2 = k[:, None] ** 2 + k[None, :] ** 2
+    field = np.real(np.fft.ifft2(np.fft.fft2(noise) * np.exp(-k2 * correlation ** 2 / 4.0)))
+    field -= field.mean()
+    return field / field.std()
+
+
+def gr_random_field_2d(n: int, seed: int, mass: float = 1.0, half_width: float = 20.0,
+                       amplitude: float = 0.05, pressure: float = 0.01):
+    """Black-hole accretion of a perturbed uniform medium on [-W, W]^2, W = half_width
+    (in units of the mass): rho = 1 + amplitude * GRF (correlation length 2M),
+    p = pressure, zero velocity.  Returns (W, h, hole) with the hole at the
+    centre, (W, W, 0) in the reference's cell coordinates."""
+    h = 2.0 * half_width / n
+    W = np.zeros((n, n, 5))
+    W[..., 0] = 1.0 + amplitude * gaussian_random_field(n, seed, 2.0 * mass, 2.0 * half_width)
+    W[..., 4] = pressure
+    return W, h, (mass, 0.0, half_width, half_width, 0.0)
+
+
+def gr_accretion_1d(n: int, spin: bool = False):
+    """The reference's 1D (spinning) spherical accretion set-up: rho = 10 for
+    0.1 n <= i <= 0.9 n, else 0.1; p = 0.1; hole (0.05, 0, 0.5, 0, 0) or
+    (0.1, 0.99, 0.5, 0, 0).  Returns (W, h, hole)."""
+    i = np.arange(n)
+    W = np.zeros((n, 5))
+    W[:, 0] = np.where((i >= 0.1 * n) & (i <= 0.9 * n), 10.0, 0.1)
+    W[:, 4] = 0.1
+    hole = (0.1, 0.99, 0.5, 0.0, 0.0) if spin else (0.05, 0.0, 0.5, 0.0, 0.0)
+    return W, 1.0 / n, hole
+
+
+def gr_accretion_2d(n: int, spin: bool = False):
+    """The reference's 2D (spinning) spherical accretion set-up: rho = 10 within
+    0.4 n cells of the centre, else 0.1; p = 0.1; hole (0.05, 0, 0.5, 0.5, 0) or
+    (0.1, 0.99, 0.5, 0.5, 0).  Returns (W, h, hole)."""
+    i = np.arange(n)[:, None].astype(float)
+    j = np.arange(n)[None, :].astype(float)
+    r = np.sqrt((i - 0.5 * n) ** 2 + (j - 0.5 * n) ** 2)
+    W = np.zeros((n, n, 5))
+    W[..., 0] = np.where(r <= 0.4 * n, 10.0, 0.1)
+    W[..., 4] = 0.1
+    hole = (0.1, 0.99, 0.5, 0.5, 0.0) if spin else (0.05, 0.0, 0.5, 0.5, 0.0)
+    return W, 1.0 / n, hole

@@ -12,7 +12,10 @@
  *     reference, plus a second buffer for the sweep output.
  *   - Host arrays are row-major AoS primitives prim[(i * nx + j) * 5 + k],
  *     i = row (y), j = column (x): vector<vector<StateVector>> flattened.
- *   - physics: COSMOS_NEWTONIAN (Euler/), COSMOS_RELATIVISTIC (RelativisticEuler/).
+ *   - physics: COSMOS_NEWTONIAN (Euler/), COSMOS_RELATIVISTIC (RelativisticEuler/),
+ *     COSMOS_BLACKHOLE (BlackHoleEuler/: general relativistic on a fixed
+ *     Kerr-Schild background; cell (i, j) sits at (j*h, i*h), a 1D cell j at
+ *     (j*h, 0), with h the cell_spacing argument, as in the reference).
  *   - limiter: 0 SuperBee, 1 van Leer, anything else MinBee
  *     (SlopeLimiters::computeSlopeLimiter, Solvers/SlopeLimiters.cpp:55-69).
  *   - Every function returns COSMOS_OK (0) or a negative code and never
@@ -38,6 +41,7 @@ extern "C" {
 
 #define COSMOS_NEWTONIAN 0
 #define COSMOS_RELATIVISTIC 1
+#define COSMOS_BLACKHOLE 2
 
 #define COSMOS_MODE_EXACT 0
 #define COSMOS_MODE_FAST 1
@@ -69,6 +73,12 @@ int cosmos_set_mode(cosmos_solver* s, int mode);
  * RelativisticEuler/RelativisticEulerStateVector.cpp:59-80). */
 int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations);
 
+/* The black hole of a COSMOS_BLACKHOLE solver: BlackHoleSpacetime(mass, spin,
+ * x, y, z) (BlackHoleEuler/BlackHoleSpacetime.cpp:43-52; the default, as the
+ * reference's default constructor :3-11, is mass 1, spin 0 at (0.5, 0.5, 0)).
+ * spin is dimensionless (the metric uses mass * spin). */
+int cosmos_set_spacetime(cosmos_solver* s, double mass, double spin, double x, double y, double z);
+
 /* State transfer.  *_device variants take device pointers to the same AoS
  * layout (for callers that already hold the grid in GPU memory). */
 int cosmos_upload(cosmos_solver* s, const double* prim);
@@ -83,8 +93,10 @@ int cosmos_download_device(cosmos_solver* s, double* prim_device);
  *   order 1, dim 2: EulerFirstOrderSolver::compute{X,Y}FORCETimeStep2D
  *                   (Solvers/EulerFirstOrderSolver.cpp:120-174)
  *   order 1, dim 1: EulerFirstOrderSolver::computeFORCETimeStep (:103-118)
- * and the RelativisticEuler* counterparts
- * (Solvers/RelativisticEulerSecondOrderSolver.cpp, Solvers/RelativisticEulerFirstOrderSolver.cpp).
+ * and the RelativisticEuler* and BlackHoleEuler* counterparts
+ * (Solvers/RelativisticEulerSecondOrderSolver.cpp, Solvers/RelativisticEulerFirstOrderSolver.cpp,
+ * Solvers/BlackHoleEulerSecondOrderSolver.cpp:37-143, Solvers/BlackHoleEulerFirstOrderSolver.cpp:101-181;
+ * black-hole cells whose stencil touches the excision region keep their state).
  * axis 0 = x, 1 = y.  time_step is the sweep's step (the caller passes
  * 0.5 * dt for the outer Strang sweeps, as solve2D does). */
 int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
@@ -92,10 +104,14 @@ int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, dou
 
 /* EulerSolvers::computeMaximumWaveSpeed{,2D} (Solvers/EulerSolvers.cpp:87-128),
  * RelativisticEulerSolvers::computeMaximumWaveSpeed{,2D}
- * (Solvers/RelativisticEulerSolvers.cpp:89-130). */
+ * (Solvers/RelativisticEulerSolvers.cpp:89-130),
+ * BlackHoleEulerSolvers::computeMaximumWaveSpeed{,2D} (Solvers/BlackHoleEulerSolvers.cpp:88-140;
+ * like the reference it places the cells on the unit square for this, whatever
+ * the cell spacing, with x and y exchanged in the eigenvalue evaluation). */
 int cosmos_max_wave_speed(cosmos_solver* s, double* out);
 
-/* EulerSolvers::computeStableTimeStep{,2D} (Solvers/EulerSolvers.cpp:130-161). */
+/* EulerSolvers::computeStableTimeStep{,2D} (Solvers/EulerSolvers.cpp:130-161) and its
+ * RelativisticEuler / BlackHoleEuler counterparts (Solvers/BlackHoleEulerSolvers.cpp:142-157). */
 int cosmos_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, double current_time,
                             double final_time, int current_iteration, double* out);
 
@@ -116,6 +132,22 @@ int cosmos_advance(cosmos_solver* s, int n_steps);
 int cosmos_progress(cosmos_solver* s, double* time, int* steps, int* stopped);
 int cosmos_solve(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
                  double bias, int slope_limiter, double* time_out, int* steps_out);
+
+/* COSMOS_BLACKHOLE only.  One RK4 source-term step in place:
+ * BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep{,2D}
+ * (Solvers/BlackHoleEulerForcingSolver.cpp:187-250). */
+int cosmos_forcing(cosmos_solver* s, double cell_spacing, double time_step, double bias, int slope_limiter);
+
+/* COSMOS_BLACKHOLE only.  The loop of BlackHoleEuler{First,Second}OrderSolver::solve{,2D}
+ * (Solvers/BlackHoleEulerFirstOrderSolver.cpp:183-253,
+ * Solvers/BlackHoleEulerSecondOrderSolver.cpp:145-218): per step dt from the CFL
+ * condition, `subcycles` forcing steps of dt / (2 subcycles) (bias 0 and van
+ * Leer, as the reference hard-codes), the flux step (Strang X Y X in 2D), and
+ * the same forcing again.  max_steps < 0 means no cap.  The host reads the
+ * wave speed once per step. */
+int cosmos_solve_forced(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time,
+                        int max_steps, double bias, int slope_limiter, int subcycles, double* time_out,
+                        int* steps_out);
 
 /* Batched point-wise limiter functions on host arrays (evaluated on the device):
  *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)

@@ -41,7 +41,16 @@ _SIGS = {
     "flux": (None, [_I, _I, _P, _P, _D]),
     "limiter": (_D, [_D, _D, _I]),
     "slope": (None, [_P, _P, _P, _D, _I, _P]),
+    # general relativistic (black hole M, a at (bx, by, bz))
+    "gr_sweep": (_I, [_I, _I, _I, _P, _I, _I, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D]),
+    "gr_forcing": (_I, [_I, _P, _I, _I, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D]),
+    "gr_max_wavespeed": (_D, [_I, _P, _I, _I, _D, _D, _D, _D, _D, _D]),
+    "gr_run": (_I, [_I, _I, _P, _I, _I, _D, _D, _D, _I, _D, _I, _I, _D, _D, _D, _D, _D, _D, _P]),
+    "gr_point": (None, [_I, _P, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _P]),
 }
+
+#: what-codes of gr_point
+GR_P2C, GR_C2P, GR_XFLUX, GR_YFLUX, GR_SOURCE, GR_SOUND, GR_METRIC = range(7)
 
 
 def build(ref: bool = True) -> None:
@@ -152,6 +161,40 @@ class CpuPath:
         a, b, c = (np.ascontiguousarray(x, dtype=np.float64) for x in (ul, um, ur))
         self._f("slope")(_dp(a), _dp(b), _dp(c), bias, limiter, _dp(out))
         return out
+
+    # ---- general relativistic; hole = (M, a, bx, by, bz) ----
+    def gr_sweep(self, W, order, axis, h, dt, hole, bias=0.0, limiter=1, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dim, nx, ny = self._dims(a)
+        self._f("gr_sweep")(order, axis, dim, _dp(a), nx, ny, h, dt, bias, limiter, gamma, *hole)
+        return a
+
+    def gr_forcing(self, W, h, dt, hole, bias=0.0, limiter=1, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dim, nx, ny = self._dims(a)
+        self._f("gr_forcing")(dim, _dp(a), nx, ny, h, dt, bias, limiter, gamma, *hole)
+        return a
+
+    def gr_max_wavespeed(self, W, hole, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        return self._f("gr_max_wavespeed")(dim, _dp(a), nx, ny, gamma, *hole)
+
+    def gr_run(self, W, order, h, cfl, hole, tf=math.inf, max_steps=2 ** 31 - 1, bias=0.0, limiter=1,
+               subcycles=1, gamma=1.4):
+        a = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dim, nx, ny = self._dims(a)
+        t = _D()
+        n = self._f("gr_run")(order, dim, _dp(a), nx, ny, h, cfl, tf, max_steps, bias, limiter, subcycles, gamma,
+                              *hole, ctypes.byref(t))
+        return a, t.value, n
+
+    def gr_point(self, what, v, x, y, z, h, hole, gamma=1.4):
+        out = np.full(44, np.nan)
+        a = np.ascontiguousarray(v if v is not None else np.zeros(5), dtype=np.float64)
+        self._f("gr_point")(what, _dp(a), x, y, z, h, gamma, *hole, _dp(out))
+        n = {GR_SOUND: 1, GR_METRIC: 44}.get(what, 5)
+        return out[:n]
 
     # oracle-only controls
     def set_con2prim(self, tol, max_iterations):

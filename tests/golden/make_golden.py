@@ -111,6 +111,35 @@ def main() -> None:
                                 for (t, tf, it) in ((0.0, 1.0, 0), (0.5, 1.0, 5), (0.5, 1.0, 6), (0.995, 1.0, 7),
                                                     (0.0, math.inf, 3))])
     np.savez_compressed(os.path.join(OUT, "single_calls.npz"), **fx)
+
+    # 5. General relativistic (black hole): the reference's 2D spinning accretion set-up
+    #    at 16^2 (2 steps of solve2D's loop, one forcing subcycle), its 1D accretion set-up
+    #    at 50 cells (FORCE, 5 steps), and known answers of the point functions.
+    fx = {}
+    W, h, hole = workloads.gr_accretion_2d(16, spin=True)
+    a, t, n = ref.gr_run(W, 2, h, 0.95, hole, tf=0.45, max_steps=2, subcycles=1, gamma=5.0 / 3.0)
+    fx.update(accretion_2d_input=W, accretion_2d_h=np.array(h), accretion_2d_hole=np.array(hole),
+              accretion_2d_output=a, accretion_2d_time=np.array(t))
+    W, h, hole = workloads.gr_accretion_1d(50)
+    a, t, n = ref.gr_run(W, 1, h, 0.95, hole, tf=0.45, max_steps=5, subcycles=1, gamma=5.0 / 3.0)
+    fx.update(accretion_1d_input=W, accretion_1d_h=np.array(h), accretion_1d_hole=np.array(hole),
+              accretion_1d_output=a, accretion_1d_time=np.array(t))
+    rng = np.random.default_rng(31)
+    pts = np.column_stack([rng.uniform(-0.5, 1.5, 24), rng.uniform(-0.5, 1.5, 24), np.zeros(24)])
+    pts[::3, 2] = rng.uniform(-0.2, 0.2, 8)
+    states = workloads.random_states(24, seed=32, physics=1)
+    fx["point_xyz"] = pts
+    fx["point_states"] = states
+    kat_hole = (0.3, 0.7, 0.5, 0.4, 0.05)
+    fx["point_hole"] = np.array(kat_hole)
+    for what, tag in ((0, "p2c"), (5, "sound"), (6, "metric")):
+        fx[f"point_{tag}"] = np.array([ref.gr_point(what, w, *xyz, 0.01, kat_hole, 5.0 / 3.0)
+                                         for w, xyz in zip(states, pts)])
+    U = fx["point_p2c"]
+    for what, tag in ((1, "c2p"), (2, "xflux"), (3, "yflux"), (4, "source")):
+        fx[f"point_{tag}"] = np.array([ref.gr_point(what, u, *xyz, 0.01, kat_hole, 5.0 / 3.0)
+                                         for u, xyz in zip(U, pts)])
+    np.savez_compressed(os.path.join(OUT, "blackhole.npz"), **fx)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
