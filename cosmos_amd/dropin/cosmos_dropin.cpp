@@ -343,3 +343,149 @@ vector<vector<RelativisticEulerStateVector> > RelativisticEulerSecondOrderSolver
     EulerMaterialParameters mp) {
     return solve_2d(COSMOS_RELATIVISTIC, 2, c, dx, cfl, tf, bias, lim, mp.getAdiabaticIndex());
 }
+
+// --- General relativistic (black hole) -------------------------------------------
+namespace {
+
+// The cached handle with the caller's black hole set (a host-side parameter).
+template <class Load, class Cells>
+cosmos_solver* load_gr(Load load, Cells& cells, EulerMaterialParameters& mp, BlackHoleSpacetime& bh,
+                       vector<double>& buf) {
+    cosmos_solver* s = load(COSMOS_BLACKHOLE, cells, mp.getAdiabaticIndex(), buf);
+    check(cosmos_set_spacetime(s, bh.getBlackHoleMass(), bh.getBlackHoleSpin(), bh.getBlackHoleXPosition(),
+                               bh.getBlackHoleYPosition(), bh.getBlackHoleZPosition()),
+          s);
+    return s;
+}
+
+typedef vector<BlackHoleEulerStateVector> GRLine;
+typedef vector<vector<BlackHoleEulerStateVector> > GRGrid;
+
+cosmos_solver* load_gr_1d(GRLine& c, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, vector<double>& buf) {
+    return load_gr(load_1d<BlackHoleEulerStateVector>, c, mp, bh, buf);
+}
+cosmos_solver* load_gr_2d(GRGrid& c, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, vector<double>& buf) {
+    return load_gr(load_2d<BlackHoleEulerStateVector>, c, mp, bh, buf);
+}
+
+void gr_sweep_1d(int order, GRLine& c, double dx, double dt, double bias, int lim, EulerMaterialParameters& mp,
+                 BlackHoleSpacetime& bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    check(cosmos_sweep(s, order, 0, dx, dt, bias, lim), s);
+    store_1d(s, c, buf);
+}
+
+void gr_sweep_2d(int order, int axis, GRGrid& c, double dx, double dt, double bias, int lim,
+                 EulerMaterialParameters& mp, BlackHoleSpacetime& bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    check(cosmos_sweep(s, order, axis, dx, dt, bias, lim), s);
+    store_2d(s, c, buf);
+}
+
+double gr_stable_dt(cosmos_solver* s, double dx, double cfl, double t, double tf, int it) {
+    double dt = 0.0;
+    check(cosmos_stable_time_step(s, dx, cfl, t, tf, it, &dt), s);
+    return dt;
+}
+
+}  // namespace
+
+double BlackHoleEulerSolvers::computeMaximumWaveSpeed(GRLine& c, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    double v = 0.0;
+    check(cosmos_max_wave_speed(s, &v), s);
+    return v;
+}
+double BlackHoleEulerSolvers::computeMaximumWaveSpeed2D(GRGrid& c, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    double v = 0.0;
+    check(cosmos_max_wave_speed(s, &v), s);
+    return v;
+}
+double BlackHoleEulerSolvers::computeStableTimeStep(GRLine c, double dx, double cfl, double t, double tf, int it,
+                                                    EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    return gr_stable_dt(load_gr_1d(c, mp, bh, buf), dx, cfl, t, tf, it);
+}
+double BlackHoleEulerSolvers::computeStableTimeStep2D(GRGrid c, double dx, double cfl, double t, double tf, int it,
+                                                      EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    return gr_stable_dt(load_gr_2d(c, mp, bh, buf), dx, cfl, t, tf, it);
+}
+
+void BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep(GRLine& c, double dx, double dt, double bias, int lim,
+                                                            EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    check(cosmos_forcing(s, dx, dt, bias, lim), s);
+    store_1d(s, c, buf);
+}
+void BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(GRGrid& c, double dx, double dt, double bias, int lim,
+                                                              EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    check(cosmos_forcing(s, dx, dt, bias, lim), s);
+    store_2d(s, c, buf);
+}
+
+void BlackHoleEulerFirstOrderSolver::computeFORCETimeStep(GRLine& c, double dx, double dt, EulerMaterialParameters mp,
+                                                          BlackHoleSpacetime bh) {
+    gr_sweep_1d(1, c, dx, dt, 0.0, 1, mp, bh);
+}
+void BlackHoleEulerFirstOrderSolver::computeXFORCETimeStep2D(GRGrid& c, double dx, double dt,
+                                                             EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    gr_sweep_2d(1, 0, c, dx, dt, 0.0, 1, mp, bh);
+}
+void BlackHoleEulerFirstOrderSolver::computeYFORCETimeStep2D(GRGrid& c, double dx, double dt,
+                                                             EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    gr_sweep_2d(1, 1, c, dx, dt, 0.0, 1, mp, bh);
+}
+GRLine BlackHoleEulerFirstOrderSolver::solve(GRLine c, double dx, double cfl, double tf, int sub,
+                                             EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    check(cosmos_solve_forced(s, 1, dx, cfl, tf, -1, 0.0, 1, sub, nullptr, nullptr), s);
+    store_1d(s, c, buf);
+    return c;
+}
+GRGrid BlackHoleEulerFirstOrderSolver::solve2D(GRGrid c, double dx, double cfl, double tf, int sub,
+                                               EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    check(cosmos_solve_forced(s, 1, dx, cfl, tf, -1, 0.0, 1, sub, nullptr, nullptr), s);
+    store_2d(s, c, buf);
+    return c;
+}
+
+void BlackHoleEulerSecondOrderSolver::computeSLICTimeStep(GRLine& c, double dx, double dt, double bias, int lim,
+                                                          EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    gr_sweep_1d(2, c, dx, dt, bias, lim, mp, bh);
+}
+void BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(GRGrid& c, double dx, double dt, double bias, int lim,
+                                                             EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    gr_sweep_2d(2, 0, c, dx, dt, bias, lim, mp, bh);
+}
+void BlackHoleEulerSecondOrderSolver::computeYSLICTimeStep2D(GRGrid& c, double dx, double dt, double bias, int lim,
+                                                             EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    gr_sweep_2d(2, 1, c, dx, dt, bias, lim, mp, bh);
+}
+GRLine BlackHoleEulerSecondOrderSolver::solve(GRLine c, double dx, double cfl, double tf, double bias, int lim,
+                                              int sub, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    check(cosmos_solve_forced(s, 2, dx, cfl, tf, -1, bias, lim, sub, nullptr, nullptr), s);
+    store_1d(s, c, buf);
+    return c;
+}
+GRGrid BlackHoleEulerSecondOrderSolver::solve2D(GRGrid c, double dx, double cfl, double tf, double bias, int lim,
+                                                int sub, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    vector<double> buf;
+    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    check(cosmos_solve_forced(s, 2, dx, cfl, tf, -1, bias, lim, sub, nullptr, nullptr), s);
+    store_2d(s, c, buf);
+    return c;
+}

@@ -8,10 +8,13 @@
 //   EulerFirstOrderSolver (Solvers/EulerFirstOrderSolver.hpp:8-47)
 //   EulerSecondOrderSolver (Solvers/EulerSecondOrderSolver.hpp:6-29)
 //   RelativisticEulerSolvers / ...FirstOrderSolver / ...SecondOrderSolver
+//   BlackHoleEulerSolvers / ...FirstOrderSolver / ...SecondOrderSolver / ...ForcingSolver
+//     (Solvers/BlackHoleEuler{Solvers,FirstOrderSolver,SecondOrderSolver,ForcingSolver}.hpp)
 //   SlopeLimiters (Solvers/SlopeLimiters.hpp:10-35)         limiter functions and slope vectors
 //
 // The state types come from the host application's headers
 // (EulerStateVector.hpp, RelativisticEulerStateVector.hpp,
+// BlackHoleEulerStateVector.hpp, BlackHoleSpacetime.hpp,
 // EulerMaterialParameters.hpp) when the drop-in is built inside it; the
 // compat/ directory supplies minimal stand-ins for standalone builds.
 //
@@ -24,6 +27,8 @@
 
 #include <vector>
 
+#include "BlackHoleEulerStateVector.hpp"
+#include "BlackHoleSpacetime.hpp"
 #include "EulerMaterialParameters.hpp"
 #include "EulerStateVector.hpp"
 #include "RelativisticEulerStateVector.hpp"
@@ -141,6 +146,78 @@ public:
     static vector<vector<RelativisticEulerStateVector> > solve2D(
         vector<vector<RelativisticEulerStateVector> > initialCells, double cellSpacing, double CFLCoefficient,
         double finalTime, double bias, int slopeLimiter, EulerMaterialParameters materialParameters);
+};
+
+class BlackHoleEulerSolvers {
+public:
+    static double computeMaximumWaveSpeed(vector<BlackHoleEulerStateVector>& currentCells,
+                                          EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static double computeMaximumWaveSpeed2D(vector<vector<BlackHoleEulerStateVector> >& currentCells,
+                                            EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static double computeStableTimeStep(vector<BlackHoleEulerStateVector> currentCells, double cellSpacing,
+                                        double CFLCoefficient, double currentTime, double finalTime,
+                                        int currentIteration, EulerMaterialParameters materialParameters,
+                                        BlackHoleSpacetime blackHole);
+    static double computeStableTimeStep2D(vector<vector<BlackHoleEulerStateVector> > currentCells, double cellSpacing,
+                                          double CFLCoefficient, double currentTime, double finalTime,
+                                          int currentIteration, EulerMaterialParameters materialParameters,
+                                          BlackHoleSpacetime blackHole);
+};
+
+class BlackHoleEulerForcingSolver {
+public:
+    static void computeRungeKuttaTimeStep(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
+                                          double timeStep, double bias, int slopeLimiter,
+                                          EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeRungeKuttaTimeStep2D(vector<vector<BlackHoleEulerStateVector> >& currentCells,
+                                            double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                            EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+};
+
+class BlackHoleEulerFirstOrderSolver {
+public:
+    static void computeFORCETimeStep(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
+                                     double timeStep, EulerMaterialParameters materialParameters,
+                                     BlackHoleSpacetime blackHole);
+    static void computeXFORCETimeStep2D(vector<vector<BlackHoleEulerStateVector> >& currentCells, double cellSpacing,
+                                        double timeStep, EulerMaterialParameters materialParameters,
+                                        BlackHoleSpacetime blackHole);
+    static void computeYFORCETimeStep2D(vector<vector<BlackHoleEulerStateVector> >& currentCells, double cellSpacing,
+                                        double timeStep, EulerMaterialParameters materialParameters,
+                                        BlackHoleSpacetime blackHole);
+    static vector<BlackHoleEulerStateVector> solve(vector<BlackHoleEulerStateVector> initialCells, double cellSpacing,
+                                                   double CFLCoefficient, double finalTime, int subcyclingIterations,
+                                                   EulerMaterialParameters materialParameters,
+                                                   BlackHoleSpacetime blackHole);
+    static vector<vector<BlackHoleEulerStateVector> > solve2D(vector<vector<BlackHoleEulerStateVector> > initialCells,
+                                                              double cellSpacing, double CFLCoefficient,
+                                                              double finalTime, int subcyclingIterations,
+                                                              EulerMaterialParameters materialParameters,
+                                                              BlackHoleSpacetime blackHole);
+};
+
+class BlackHoleEulerSecondOrderSolver {
+public:
+    static void computeSLICTimeStep(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
+                                    double timeStep, double bias, int slopeLimiter,
+                                    EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeXSLICTimeStep2D(vector<vector<BlackHoleEulerStateVector> >& currentCells, double cellSpacing,
+                                       double timeStep, double bias, int slopeLimiter,
+                                       EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeYSLICTimeStep2D(vector<vector<BlackHoleEulerStateVector> >& currentCells, double cellSpacing,
+                                       double timeStep, double bias, int slopeLimiter,
+                                       EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static vector<BlackHoleEulerStateVector> solve(vector<BlackHoleEulerStateVector> initialCells, double cellSpacing,
+                                                   double CFLCoefficient, double finalTime, double bias,
+                                                   int slopeLimiter, int subcyclingIterations,
+                                                   EulerMaterialParameters materialParameters,
+                                                   BlackHoleSpacetime blackHole);
+    static vector<vector<BlackHoleEulerStateVector> > solve2D(vector<vector<BlackHoleEulerStateVector> > initialCells,
+                                                              double cellSpacing, double CFLCoefficient,
+                                                              double finalTime, double bias, int slopeLimiter,
+                                                              int subcyclingIterations,
+                                                              EulerMaterialParameters materialParameters,
+                                                              BlackHoleSpacetime blackHole);
 };
 
 #endif

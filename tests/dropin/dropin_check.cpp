@@ -11,6 +11,13 @@
 //   5. RelativisticEulerSecondOrderSolver::solve, 200 cells, t = 0.05     (200 x 5)
 //   6. EulerSolvers::computeMaximumWaveSpeed2D and the stable time step    (2)
 //   7. SlopeLimiters::computeSlopeLimiter for r = -1 .. 3, 3 limiters       (3 x 41)
+//   8. BlackHoleEulerSecondOrderSolver::solve2D, the spinning 2D accretion set-up
+//      of Tests/BlackHoleEulerTests.cpp at 16 x 16, t = 0.01, one subcycle  (16 x 16 x 5)
+//   9. BlackHoleEulerFirstOrderSolver::solve, the 1D accretion set-up at 60 cells,
+//      t = 0.02, one subcycle                                               (60 x 5)
+//  10. one forcing step + X/Y/X SLIC per-call on the 2D input (dt 0.002)   (16 x 16 x 5)
+//  11. BlackHoleEulerSolvers::computeMaximumWaveSpeed2D of the 2D input    (1)
+#include <cmath>
 #include <cstdio>
 #include <vector>
 
@@ -78,6 +85,39 @@ int main(int argc, char** argv) {
 
     for (int lim = 0; lim < 3; ++lim)
         for (int k = 0; k <= 40; ++k) put(f, SlopeLimiters::computeSlopeLimiter(-1.0 + 0.1 * k, 0.0, lim));
+
+    // black hole (Tests/BlackHoleEulerTests.cpp:3-31, 125-160 set-ups)
+    {
+        const int n = 16;
+        BlackHoleSpacetime bh(0.1, 0.99, 0.5, 0.5, 0.0);
+        vector<vector<BlackHoleEulerStateVector> > g(n, vector<BlackHoleEulerStateVector>(n));
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                double r = std::sqrt(((i - 0.5 * n) * (i - 0.5 * n)) + ((j - 0.5 * n) * (j - 0.5 * n)));
+                g[i][j] = r <= 0.4 * n ? BlackHoleEulerStateVector(10.0, 0.0, 0.0, 0.0, 0.1)
+                                       : BlackHoleEulerStateVector(0.1, 0.0, 0.0, 0.0, 0.1);
+            }
+        vector<vector<BlackHoleEulerStateVector> > g2 =
+            BlackHoleEulerSecondOrderSolver::solve2D(g, 1.0 / n, 0.95, 0.01, 0.0, 1, 1, mps, bh);
+        for (auto& row : g2) put_cells(f, row);
+
+        const int m = 60;
+        BlackHoleSpacetime bh1(0.05, 0.5, 0.0, 0.0);
+        vector<BlackHoleEulerStateVector> line(m);
+        for (int i = 0; i < m; ++i)
+            line[i] = (i >= 0.1 * m && i <= 0.9 * m) ? BlackHoleEulerStateVector(10.0, 0.0, 0.0, 0.0, 0.1)
+                                                     : BlackHoleEulerStateVector(0.1, 0.0, 0.0, 0.0, 0.1);
+        vector<BlackHoleEulerStateVector> l2 = BlackHoleEulerFirstOrderSolver::solve(line, 1.0 / m, 0.95, 0.02, 1, mps, bh1);
+        put_cells(f, l2);
+
+        vector<vector<BlackHoleEulerStateVector> > g3 = g;
+        BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(g3, 1.0 / n, 0.001, 0.0, 1, mps, bh);
+        BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(g3, 1.0 / n, 0.001, 0.0, 1, mps, bh);
+        BlackHoleEulerSecondOrderSolver::computeYSLICTimeStep2D(g3, 1.0 / n, 0.002, 0.0, 1, mps, bh);
+        BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(g3, 1.0 / n, 0.001, 0.0, 1, mps, bh);
+        for (auto& row : g3) put_cells(f, row);
+        put(f, BlackHoleEulerSolvers::computeMaximumWaveSpeed2D(g, mps, bh));
+    }
     fclose(f);
     return 0;
 }

@@ -61,4 +61,18 @@ def test_dropin_classes_match_oracle(orc, tmp_path):
     lim = take((3, 41))
     for k in range(3):
         assert same(lim[k], [orc.limiter(-1.0 + 0.1 * i, 0.0, k) for i in range(41)])
+
+    g = 5.0 / 3.0
+    W2, h2, hole2 = workloads.gr_accretion_2d(16, spin=True)
+    exp, _, _ = orc.gr_run(W2, 2, h2, 0.95, hole2, tf=0.01, subcycles=1, gamma=g)
+    assert same(take((16, 16, 5)), exp)
+    W1, h1, hole1 = workloads.gr_accretion_1d(60)
+    exp, _, _ = orc.gr_run(W1, 1, h1, 0.95, hole1, tf=0.02, subcycles=1, gamma=g)
+    assert same(take((60, 5)), exp)
+    e = orc.gr_forcing(W2, h2, 0.001, hole2, 0.0, 1, g)
+    e = orc.gr_sweep(e, 2, 0, h2, 0.001, hole2, 0.0, 1, g)
+    e = orc.gr_sweep(e, 2, 1, h2, 0.002, hole2, 0.0, 1, g)
+    e = orc.gr_sweep(e, 2, 0, h2, 0.001, hole2, 0.0, 1, g)
+    assert same(take((16, 16, 5)), e)
+    assert take((1,))[0] == orc.gr_max_wavespeed(W2, hole2, g)
     assert pos == v.size
