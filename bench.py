@@ -168,6 +168,91 @@ def gpu_leg(args, mode):
     }
 
 
+def gr_leg(args):
+    """The black-hole workload: 2D accretion of a perturbed medium on [-20M, 20M]^2
+    (M = 1, spin 0, gamma 4/3, excision r <= 2.5M, SLIC + van Leer with one RK4
+    forcing subcycle before and after the Strang flux step, CFL 0.4), through the
+    device loop of cosmos_solve_forced.  The reference's own dynamics on this
+    workload reach NaN after 30-40 steps at every resolution tried (its wave
+    speed is evaluated on the unit square, far from the hole, so the time step
+    ignores the fast near-horizon characteristics); the measured steps stay
+    before that."""
+    from cosmos_amd import BLACKHOLE, Solver, native, workloads
+
+    n = args.gr_size
+    W, h, hole = workloads.gr_random_field_2d(n, seed=args.seed)
+    ev = HipEvents()
+    lib = native.load()
+    with Solver(BLACKHOLE, 2, n, n, 4.0 / 3.0) as s:
+        s.set_spacetime(*hole)
+        s.set_excision(2.5)
+        s.upload(W)
+        stream = s.stream
+        s.solve_forced(2, h, 0.4, max_steps=1, subcycles=1)   # warm-up (not timed)
+        s.upload(W)
+        s.synchronize()
+        e0, e1 = ev.new(), ev.new()
+        t0 = time.perf_counter()
+        ev.record(e0, stream)
+        t_reached, steps = s.solve_forced(2, h, 0.4, max_steps=args.gr_steps, subcycles=1)
+        ev.record(e1, stream)
+        ms_total = ev.elapsed_ms(e0, e1)
+        wall = time.perf_counter() - t0
+        out = s.download()
+        finite = bool(np.isfinite(out).all())
+        # per-kernel split: one more step with events around every launch
+        lib.cosmos_profile_enable(s._h, 1)
+        s.solve_forced(2, h, 0.4, max_steps=1, subcycles=1)
+        tot = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int * 4)()
+        native.check(lib.cosmos_profile_read(s._h, tot, cnt), s._h)
+        lib.cosmos_profile_enable(s._h, 0)
+    names = {0: "forcing", 1: "sweep_x", 2: "sweep_y"}
+    kernels = {names[k]: {"launches": cnt[k], "avg_ms": tot[k] / cnt[k]} for k in names if cnt[k]}
+    ms_per_step = ms_total / steps
+    return {
+        "workload": f"2D black-hole accretion, {n}x{n} FP64 on [-20M,20M]^2, M=1, a=0, gamma 4/3, "
+                    f"excision r<=2.5M, SLIC+van Leer, RK4 forcing (1 subcycle), CFL 0.4, {steps} steps, "
+                    f"perturbed uniform medium (seed {args.seed})",
+        "cell_updates_per_s": n * n / (ms_per_step * 1e-3),
+        "ms_per_step": ms_per_step,
+        "wall_s": wall,
+        "t_reached": t_reached,
+        "state_finite": finite,
+        "kernels": kernels,
+        "bound": "fp64",
+    }
+
+
+def sr_leg(args):
+    """The special-relativistic blast: 2048^2 FP64, gamma 5/3, SLIC + van Leer,
+    CFL 0.5, seeded cylindrical blast with |v| <= 0.9, in the reference's
+    con2prim mode (tolerance 1e-8, <= 1000 iterations) and in the tighter
+    mode (1e-12, <= 50 iterations)."""
+    from cosmos_amd import RELATIVISTIC, VANLEER, Solver, workloads
+
+    n = args.sr_size
+    W = workloads.sr_blast_2d(n, seed=args.seed, vmax=0.9)
+    ev = HipEvents()
+    res = {}
+    for tag, (tol, maxit) in (("reference_c2p", (1e-8, 1000)), ("tight_c2p", (1e-12, 50))):
+        with Solver(RELATIVISTIC, 2, n, n, 5.0 / 3.0) as s:
+            s.set_con2prim(tol, maxit)
+            s.upload(W)
+            stream = s.stream
+            s.begin(2, 1.0 / n, 0.5, math.inf, args.sr_steps, 0.0, VANLEER)
+            s.synchronize()
+            e0, e1 = ev.new(), ev.new()
+            ev.record(e0, stream)
+            s.advance(args.sr_steps)
+            ev.record(e1, stream)
+            ms = ev.elapsed_ms(e0, e1)
+            _, steps, _ = s.progress()
+        res[tag] = {"cell_updates_per_s": n * n * steps / (ms * 1e-3), "ms_per_step": ms / steps, "steps": steps}
+    return {"workload": f"2D special-relativistic blast, {n}x{n} FP64, gamma 5/3, SLIC+van Leer, CFL 0.5, "
+                        f"{args.sr_steps} steps (seed {args.seed})", "bound": "fp64", **res}
+
+
 def cpu_leg(args):
     """The reference's own sources (serial build, oracle/_ref) on a bounded
     sample of the same workload; the C restatement (oracle/liboracle.so,
@@ -188,6 +273,25 @@ def cpu_leg(args):
             "sample": f"reference sources (serial build) on the same 4-quadrant SLIC workload at {n}x{n}, "
                       f"{steps} full steps, {dt:.1f} s",
         }
+        if not args.no_extra:
+            # the same reference sources on bounded samples of the SR and black-hole workloads
+            m = 128
+            Wsr = workloads.sr_blast_2d(m, seed=args.seed, vmax=0.9)
+            t0 = time.perf_counter()
+            _, _, steps = ref.run(Wsr, 1, 2, 1.0 / m, 0.5, max_steps=2, gamma=5.0 / 3.0)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline_relativistic"] = {
+                "value": m * m * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"reference sources (serial) on the SR blast at {m}x{m}, {steps} steps, {dt:.1f} s"}
+            m = 64
+            Wgr, h, hole = workloads.gr_random_field_2d(m, seed=args.seed)
+            t0 = time.perf_counter()
+            _, _, steps = ref.gr_run(Wgr, 2, h, 0.4, hole, max_steps=2, subcycles=1, gamma=4.0 / 3.0)
+            dt = time.perf_counter() - t0
+            out["cpu_baseline_blackhole"] = {
+                "value": m * m * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
+                "sample": f"reference sources (serial) on the black-hole workload at {m}x{m} (its own "
+                          f"excision radius 2M), {steps} steps, {dt:.1f} s"}
     orc = oracle.load_oracle()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     orc.set_threads(threads)
@@ -222,6 +326,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-size", type=int, default=768)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--gr-size", type=int, default=1024)
+    ap.add_argument("--gr-steps", type=int, default=20)
+    ap.add_argument("--sr-size", type=int, default=2048)
+    ap.add_argument("--sr-steps", type=int, default=200)
+    ap.add_argument("--no-extra", action="store_true", help="skip the SR and black-hole legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -272,6 +381,9 @@ def main():
         "timing": "HIP events on the solver stream; per-kernel averages from a second identical region with "
                   "every launch bracketed by events",
     }
+    if not args.no_extra:
+        line["relativistic"] = sr_leg(args)
+        line["blackhole"] = gr_leg(args)
     if not args.no_cpu:
         line.update(cpu_leg(args))
     print(json.dumps(line), flush=True)

@@ -39,6 +39,7 @@ struct cosmos_solver {
     bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
     int mode = COSMOS_MODE_EXACT;
     cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
+    double excision = 0.0;                     // > 0: excision radius override
     double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
     long gr_fpitch = 0;
     hipStream_t stream = nullptr;
@@ -95,7 +96,7 @@ cfv::PhysParams make_params(double g) {
 
 // BlackHoleSpacetime(M, a, x, y, z) and the constants its accessors form
 // (BlackHoleEuler/BlackHoleSpacetime.cpp:54-113, 372-386).
-cfv::Hole make_hole(double M, double a, double x, double y, double z) {
+cfv::Hole make_hole(double M, double a, double x, double y, double z, double excision) {
     cfv::Hole b{};
     b.M = M;
     b.a = a;
@@ -104,7 +105,7 @@ cfv::Hole make_hole(double M, double a, double x, double y, double z) {
     b.z = z;
     b.ma = (M * M) * (a * a);
     b.aM = a * M;
-    b.r_exc = M * (1.0 + std::sqrt(1.0 - (a * a)));
+    b.r_exc = excision > 0.0 ? excision : M * (1.0 + std::sqrt(1.0 - (a * a)));
     return b;
 }
 
@@ -342,7 +343,7 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     s->ny = ny;
     s->pitch = ((long)nx + 15) / 16 * 16;
     s->P = make_params(adiabatic_index);
-    s->hole = make_hole(1.0, 0.0, 0.5, 0.5, 0.0);
+    s->hole = make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);
     hipError_t e = hipGetDevice(&s->device);
     size_t bytes = sizeof(double) * 5 * (size_t)ny * s->pitch;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -419,7 +420,17 @@ int cosmos_set_spacetime(cosmos_solver* s, double mass, double spin, double x, d
     return guarded(s, [&] {
         if (s->physics != COSMOS_BLACKHOLE)
             return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_set_spacetime needs a COSMOS_BLACKHOLE solver");
-        s->hole = make_hole(mass, spin, x, y, z);
+        s->hole = make_hole(mass, spin, x, y, z, s->excision);
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_set_excision(cosmos_solver* s, double radius) {
+    return guarded(s, [&] {
+        if (s->physics != COSMOS_BLACKHOLE)
+            return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_set_excision needs a COSMOS_BLACKHOLE solver");
+        s->excision = radius > 0.0 ? radius : 0.0;
+        s->hole = make_hole(s->hole.M, s->hole.a, s->hole.x, s->hole.y, s->hole.z, s->excision);
         return COSMOS_OK;
     });
 }

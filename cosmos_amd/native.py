@@ -40,6 +40,7 @@ SIGNATURES = {
     "cosmos_progress": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, _c_int_p]),
     "cosmos_solve": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p]),
+    "cosmos_set_excision": (ctypes.c_int, [_handle, ctypes.c_double]),
     "cosmos_forcing": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "cosmos_solve_forced": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_double_p,

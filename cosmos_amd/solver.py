@@ -133,6 +133,10 @@ class Solver:
                                            slope_limiter, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
 
+    def set_excision(self, radius: float) -> None:
+        """Excision radius override of a BLACKHOLE solver (<= 0: the reference's horizon)."""
+        self._check(self._lib.cosmos_set_excision(self._h, radius))
+
     def forcing(self, cell_spacing: float, time_step: float, bias: float = 0.0,
                 slope_limiter: int = VANLEER) -> None:
         self._check(self._lib.cosmos_forcing(self._h, cell_spacing, time_step, bias, slope_limiter))

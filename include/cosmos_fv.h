@@ -79,6 +79,12 @@ int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations);
  * spin is dimensionless (the metric uses mass * spin). */
 int cosmos_set_spacetime(cosmos_solver* s, double mass, double spin, double x, double y, double z);
 
+/* Excision radius of a COSMOS_BLACKHOLE solver (no counterpart in the
+ * reference, which always excises r <= M (1 + sqrt(1 - spin^2)),
+ * BlackHoleEuler/BlackHoleSpacetime.cpp:372-386).  radius > 0 replaces that
+ * radius (same <= test); radius <= 0 restores it. */
+int cosmos_set_excision(cosmos_solver* s, double radius);
+
 /* State transfer.  *_device variants take device pointers to the same AoS
  * layout (for callers that already hold the grid in GPU memory). */
 int cosmos_upload(cosmos_solver* s, const double* prim);

@@ -77,6 +77,7 @@ class CpuPath:
         if prefix == "orc_":
             self.lib.orc_set_c2p.argtypes = [_D, _I]
             self.lib.orc_set_threads.argtypes = [_I]
+            self.lib.orc_gr_set_excision.argtypes = [_D]
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -202,6 +203,10 @@ class CpuPath:
 
     def set_threads(self, n):
         self.lib.orc_set_threads(n)
+
+    def gr_set_excision(self, radius):
+        """Excision radius override (<= 0 restores the reference's horizon radius)."""
+        self.lib.orc_gr_set_excision(radius)
 
 
 def load_oracle() -> CpuPath:

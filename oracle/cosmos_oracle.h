@@ -52,6 +52,9 @@ int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, doubl
 void orc_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M,
                   double a, double bx, double by, double bz, double* out);
 
+/* Excision radius override (<= 0: the reference's horizon radius M (1 + sqrt(1 - a^2))). */
+void orc_gr_set_excision(double radius);
+
 /* Special-relativistic con2prim Newton controls.  The reference hard-codes a
  * stopping tolerance of 1e-8 and at most 1000 iterations
  * (RelativisticEuler/RelativisticEulerStateVector.cpp:59-80); other values

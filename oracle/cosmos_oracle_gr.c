@@ -257,11 +257,17 @@ static void curvature(const bh_t* b, double x, double y, double z, double h, dou
         }
 }
 
-/* inExcisionRegion (:372-386) */
+/* inExcisionRegion (:372-386).  A positive g_excision replaces the horizon
+ * radius M (1 + sqrt(1 - a^2)) (an explicit variant, not in the reference). */
+static double g_excision = 0.0;
+
+void orc_gr_set_excision(double radius) { g_excision = radius; }
+
 static int excised(const bh_t* b, double x, double y, double z) {
     double dx = x - b->x, dy = y - b->y, dz = z - b->z;
     double r = sqrt(((dx * dx) + (dy * dy)) + (dz * dz));
-    return r <= (b->M * (1.0 + sqrt(1.0 - (b->a * b->a))));
+    double rad = g_excision > 0.0 ? g_excision : (b->M * (1.0 + sqrt(1.0 - (b->a * b->a))));
+    return r <= rad;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -140,3 +140,19 @@ def test_gr_rejects_misuse():
             s.forcing(0.1, 0.01)
         with pytest.raises(Exception):
             s.set_spacetime(1.0)
+
+
+def test_gr_excision_override(orc):
+    """The explicit excision-radius variant (r <= 2.5M on the random-field workload)."""
+    W, h, hole = workloads.gr_random_field_2d(48, seed=2)
+    try:
+        orc.gr_set_excision(2.5)
+        exp, te, ne = orc.gr_run(W, 2, h, 0.4, hole, max_steps=3, subcycles=1, gamma=4.0 / 3.0)
+    finally:
+        orc.gr_set_excision(0.0)
+    with solver_for(W, hole, 4.0 / 3.0) as s:
+        s.set_excision(2.5)
+        t, n = s.solve_forced(2, h, 0.4, max_steps=3, subcycles=1)
+        out = s.download()
+    assert same(out, exp), rel_linf(out, exp)
+    assert (t, n) == (te, ne)

@@ -192,3 +192,29 @@ def test_wavespeed_unit_square_placement(orc):
             ws = abs(orc.gr_point(GR_SOUND, W[i, j], i / 9, j / 13, 0.0, 0.01, hole, G)[0])
             best = max(best, ws)
     assert orc.gr_max_wavespeed(W, hole, G) == best
+
+
+def test_excision_radius_override(orc):
+    hole = (0.1, 0.0, 0.5, 0.5, 0.0)
+    try:
+        orc.gr_set_excision(0.35)
+        assert metric(orc, 0.5 + 0.35 * (1.0 - 1e-9), 0.5, 0.0, hole)["excised"] == 1.0
+        assert metric(orc, 0.5, 0.5 + 0.35 * (1.0 + 1e-9), 0.0, hole)["excised"] == 0.0
+        # a sweep leaves every cell whose +-2 stencil meets the region as it was
+        W = grid((20, 20), 9)
+        h = 1.0 / 20
+        out = orc.gr_sweep(W, 2, 0, h, 0.01, hole, 0.0, 1, G)
+
+        def inside(ii, jj):  # the cell centre (jj h, ii h), formed as the sweep forms it
+            dx, dy = jj * h - 0.5, ii * h - 0.5
+            return math.sqrt((dx * dx + dy * dy) + 0.0) <= 0.35
+
+        for i in range(20):
+            for j in range(20):
+                near = any(inside(ii, jj) for ii, jj in
+                           [(i, j)] + [(i, j + d) for d in (-2, -1, 1, 2)] + [(i + d, j) for d in (-2, -1, 1, 2)])
+                assert same(out[i, j], W[i, j]) == near, (i, j)
+    finally:
+        orc.gr_set_excision(0.0)
+    assert metric(orc, 0.5 + 0.2 * (1.0 - 1e-9), 0.5, 0.0, hole)["excised"] == 1.0
+    assert metric(orc, 0.5 + 0.3, 0.5, 0.0, hole)["excised"] == 0.0
