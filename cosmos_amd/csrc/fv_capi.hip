@@ -78,21 +78,6 @@ int fail(cosmos_solver* s, int code, const std::string& msg) {
             return fail((s), COSMOS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-cfv::PhysParams make_params(double g) {
-    cfv::PhysParams P{};
-    P.gamma = g;
-    P.gm1 = g - 1.0;
-    P.c2p_tol = 1e-8;
-    P.c2p_maxit = 1000;
-    P.sr_c0 = -1.0 / (g * g);
-    P.sr_c1f = (g - 1.0) / (g * g);
-    P.sr_gm1g = (g - 1.0) / g;
-    P.sr_gm2g = (g - 2.0) / g;
-    P.sr_gg1 = g / (g - 1.0);
-    P.sr_4gm1g = 4.0 * ((g - 1.0) / g);
-    P.inv_gm1 = 1.0 / (g - 1.0);
-    return P;
-}
 
 // BlackHoleSpacetime(M, a, x, y, z) and the constants its accessors form
 // (BlackHoleEuler/BlackHoleSpacetime.cpp:54-113, 372-386).
@@ -342,7 +327,7 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     s->nx = nx;
     s->ny = ny;
     s->pitch = ((long)nx + 15) / 16 * 16;
-    s->P = make_params(adiabatic_index);
+    s->P = cfv::make_params(adiabatic_index);
     s->hole = make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);
     hipError_t e = hipGetDevice(&s->device);
     size_t bytes = sizeof(double) * 5 * (size_t)ny * s->pitch;

@@ -45,6 +45,24 @@ __device__ __forceinline__ void block_max_to(double m, unsigned long long* slots
     }
 }
 
+// PhysParams of an adiabatic index, each constant formed with the reference's
+// expression (RelativisticEuler/RelativisticEulerStateVector.cpp:52-57).
+inline PhysParams make_params(double g) {
+    PhysParams P{};
+    P.gamma = g;
+    P.gm1 = g - 1.0;
+    P.c2p_tol = 1e-8;
+    P.c2p_maxit = 1000;
+    P.sr_c0 = -1.0 / (g * g);
+    P.sr_c1f = (g - 1.0) / (g * g);
+    P.sr_gm1g = (g - 1.0) / g;
+    P.sr_gm2g = (g - 2.0) / g;
+    P.sr_gg1 = g / (g - 1.0);
+    P.sr_4gm1g = 4.0 * ((g - 1.0) / g);
+    P.inv_gm1 = 1.0 / (g - 1.0);
+    return P;
+}
+
 // Per-sweep coefficients: dt/dx, dx/dt and 0.5*(dt/dx), with dt the sweep's
 // step (0.5*dt for the outer Strang x sweeps).  Formed exactly as the
 // reference forms them at each use (Solvers/EulerSolvers.cpp:163-172,

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libcosmos_fv.so")
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_u8_p = ctypes.POINTER(ctypes.c_ubyte)
 _handle = ctypes.c_void_p
 
 # name -> (restype, argtypes); mirrors include/cosmos_fv.h.
@@ -45,6 +46,21 @@ SIGNATURES = {
     "cosmos_solve_forced": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_double_p,
                                             _c_int_p]),
+    "cosmos_amr_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_double]),
+    "cosmos_amr_destroy": (ctypes.c_int, [_handle]),
+    "cosmos_amr_last_error": (ctypes.c_char_p, [_handle]),
+    "cosmos_amr_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
+    "cosmos_amr_upload": (ctypes.c_int, [_handle, _c_double_p]),
+    "cosmos_amr_download": (ctypes.c_int, [_handle, _c_double_p, _c_double_p, _c_u8_p, _c_u8_p]),
+    "cosmos_amr_refine_flags": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double]),
+    "cosmos_amr_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "cosmos_amr_restrict": (ctypes.c_int, [_handle]),
+    "cosmos_amr_derefine": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double]),
+    "cosmos_amr_solve_level1": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,
+                                                _c_int_p]),
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
                                             ctypes.c_int, _c_double_p]),
@@ -80,7 +96,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int, handle=None) -> None:
+def check(rc: int, handle=None, amr: bool = False) -> None:
     if rc != 0:
-        msg = load().cosmos_last_error(handle)
+        lib = load()
+        msg = lib.cosmos_amr_last_error(handle) if amr else lib.cosmos_last_error(handle)
         raise CosmosError(f"cosmos error {rc}: {msg.decode() if msg else ''}")

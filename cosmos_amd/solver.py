@@ -161,3 +161,81 @@ class Solver:
         pitch = ctypes.c_long()
         self._check(self._lib.cosmos_state_planes(self._h, arr, ctypes.byref(pitch)))
         return [arr[k] for k in range(5)], pitch.value
+
+
+class Amr:
+    """Two-level AMR context (cosmos_amr_*): a coarse grid, a fine grid of twice
+    the resolution and one refinement flag per cell of each."""
+
+    def __init__(self, physics: int, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4):
+        self._lib = native.load()
+        self.physics, self.dim = physics, dim
+        self.nx, self.ny = nx, (ny if dim == 2 else 1)
+        h = ctypes.c_void_p()
+        native.check(self._lib.cosmos_amr_create(ctypes.byref(h), physics, dim, nx, self.ny, adiabatic_index), None,
+                     amr=True)
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.cosmos_amr_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        native.check(rc, self._h, amr=True)
+
+    def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
+        self._check(self._lib.cosmos_amr_set_con2prim(self._h, tolerance, max_iterations))
+
+    def upload(self, prim: np.ndarray) -> None:
+        a = np.ascontiguousarray(prim, dtype=np.float64)
+        if a.size != self.nx * self.ny * 5:
+            raise ValueError("state does not match the coarse grid")
+        self._check(self._lib.cosmos_amr_upload(self._h, _dp(a)))
+
+    def download(self):
+        """(coarse, fine, coarse_flags, fine_flags)"""
+        if self.dim == 1:
+            cs, fs = (self.nx, 5), (2 * self.nx, 5)
+        else:
+            cs, fs = (self.ny, self.nx, 5), (2 * self.ny, 2 * self.nx, 5)
+        c, f = np.empty(cs), np.empty(fs)
+        cf, ff = np.empty(cs[:-1], dtype=np.uint8), np.empty(fs[:-1], dtype=np.uint8)
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        self._check(self._lib.cosmos_amr_download(self._h, _dp(c), _dp(f), cf.ctypes.data_as(u8),
+                                                  ff.ctypes.data_as(u8)))
+        return c, f, cf, ff
+
+    def refine_flags(self, cell_spacing: float, tolerance: float) -> None:
+        self._check(self._lib.cosmos_amr_refine_flags(self._h, cell_spacing, tolerance))
+
+    def sweep(self, level: int, order: int, axis: int, cell_spacing: float, time_step: float, bias: float = 0.0,
+              slope_limiter: int = VANLEER) -> None:
+        self._check(self._lib.cosmos_amr_sweep(self._h, level, order, axis, cell_spacing, time_step, bias,
+                                               slope_limiter))
+
+    def restrict(self) -> None:
+        self._check(self._lib.cosmos_amr_restrict(self._h))
+
+    def derefine(self, cell_spacing: float, tolerance: float) -> None:
+        self._check(self._lib.cosmos_amr_derefine(self._h, cell_spacing, tolerance))
+
+    def solve_level1(self, order: int, cell_spacing: float, cfl: float, final_time: float, tolerance: float,
+                     max_coarse_steps: int = -1):
+        """Returns (time reached, coarse steps taken)."""
+        t, n = ctypes.c_double(), ctypes.c_int()
+        self._check(self._lib.cosmos_amr_solve_level1(self._h, order, cell_spacing, cfl, final_time, tolerance,
+                                                      max_coarse_steps, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value

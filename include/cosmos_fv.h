@@ -155,6 +155,55 @@ int cosmos_solve_forced(cosmos_solver* s, int order, double cell_spacing, double
                         int max_steps, double bias, int slope_limiter, int subcycles, double* time_out,
                         int* steps_out);
 
+/* ------------------------------------------------------------------------
+ * Two-level adaptive mesh refinement (AMR/EulerAMR.cpp, AMR/RelativisticEulerAMR.cpp).
+ * A cosmos_amr context holds a coarse nx-by-ny grid, a fine 2nx-by-2ny grid (1D:
+ * nx and 2nx cells) and one refinement flag per cell of each (bytes 0/1, row
+ * major): a coarse cell whose flag is set is advanced on the coarse grid, a
+ * fine cell whose flag is set on the fine grid.
+ * ------------------------------------------------------------------------ */
+typedef struct cosmos_amr cosmos_amr;
+
+int cosmos_amr_create(cosmos_amr** out, int physics, int dim, int nx, int ny, double adiabatic_index);
+int cosmos_amr_destroy(cosmos_amr* a);
+const char* cosmos_amr_last_error(const cosmos_amr* a);
+/* as cosmos_set_con2prim, for both levels */
+int cosmos_amr_set_con2prim(cosmos_amr* a, double tolerance, int max_iterations);
+/* The coarse state (AoS primitives); the fine grid becomes a copy of each coarse
+ * cell in its children, coarse flags all set, fine flags all clear
+ * (the set-up of solve{,2D}Level1AMR, AMR/EulerAMR.cpp:174-200, 356-392). */
+int cosmos_amr_upload(cosmos_amr* a, const double* prim);
+/* Any of the outputs may be NULL. */
+int cosmos_amr_download(cosmos_amr* a, double* coarse, double* fine, unsigned char* coarse_flags,
+                        unsigned char* fine_flags);
+/* Refinement flags from the coarse state: a coarse cell with any conserved
+ * component k jumping by |dU_k| / cell_spacing >= tolerance to a neighbour
+ * (left, right; 2D also top and bottom, 1-cell copy boundary) clears its own
+ * flag and sets its children's; flags are never cleared here
+ * (AMR/EulerAMR.cpp:202-229, 397-431; the special-relativistic 2D test reads
+ * the diagonal cell as "bottom", AMR/RelativisticEulerAMR.cpp:422). */
+int cosmos_amr_refine_flags(cosmos_amr* a, double cell_spacing, double tolerance);
+/* One sweep of a level (0 coarse, 1 fine) updating only the cells whose flag
+ * is set (compute{,X,Y}{FORCE,SLIC}TimeStep{,2D}AMR, AMR/EulerAMR.cpp:3-167). */
+int cosmos_amr_sweep(cosmos_amr* a, int level, int order, int axis, double cell_spacing, double time_step,
+                     double bias, int slope_limiter);
+/* Coarse cells whose flag is clear take the mean of their children's conserved
+ * vectors; fine cells whose flag is clear take a copy of their parent
+ * (AMR/EulerAMR.cpp:282-308, 494-536). */
+int cosmos_amr_restrict(cosmos_amr* a);
+/* Derefinement: a refined coarse cell whose centre child (1D: child 2j+1; 2D:
+ * (2i+1, 2j+1)) jumps by less than tolerance / 2 over cell_spacing to each fine
+ * neighbour is made coarse again and restricted (AMR/EulerAMR.cpp:310-345, 538-586). */
+int cosmos_amr_derefine(cosmos_amr* a, double cell_spacing, double tolerance);
+/* EulerAMR::solveLevel1AMR / solve2DLevel1AMR and the RelativisticEulerAMR copies
+ * (AMR/EulerAMR.cpp:169-590): per coarse step the CFL step (damped by the
+ * fine-step counter), the refinement flags, the masked coarse flux step, masked
+ * fine sub-steps to the coarse time (less 1e-4), restriction and
+ * derefinement; bias 0 and van Leer, as the reference fixes them.
+ * max_coarse_steps < 0 means no cap. */
+int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
+                            double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
+
 /* Batched point-wise limiter functions on host arrays (evaluated on the device):
  *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)
  *                         (Solvers/SlopeLimiters.cpp:3-69)

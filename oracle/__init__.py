@@ -74,7 +74,16 @@ class CpuPath:
             fn = getattr(self.lib, prefix + name)
             fn.restype = res
             fn.argtypes = args
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        if prefix == "ref_" and hasattr(self.lib, "ref_amr_level1"):
+            self.lib.ref_amr_level1.restype = _I
+            self.lib.ref_amr_level1.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _P, _P, u8, u8]
         if prefix == "orc_":
+            self.lib.orc_amr_refine_flags.restype = None
+            self.lib.orc_amr_refine_flags.argtypes = [_I, _I, _P, _I, _I, _D, _D, _D, u8, u8]
+            self.lib.orc_amr_level1.restype = _I
+            self.lib.orc_amr_level1.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P, _P, u8, u8,
+                                                ctypes.POINTER(_D)]
             self.lib.orc_set_c2p.argtypes = [_D, _I]
             self.lib.orc_set_threads.argtypes = [_I]
             self.lib.orc_gr_set_excision.argtypes = [_D]
@@ -196,6 +205,37 @@ class CpuPath:
         self._f("gr_point")(what, _dp(a), x, y, z, h, gamma, *hole, _dp(out))
         n = {GR_SOUND: 1, GR_METRIC: 44}.get(what, 5)
         return out[:n]
+
+    # ---- two-level AMR ----
+    def amr_level1(self, W, physics, order, dx, cfl, tf, tol, gamma=1.4, max_steps=-1):
+        """(coarse, fine, coarse_flags, fine_flags[, t, steps]) of solve{,2D}Level1AMR."""
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        fshape = (2 * nx, 5) if dim == 1 else (2 * ny, 2 * nx, 5)
+        coarse = np.empty_like(a)
+        fine = np.empty(fshape)
+        cf = np.empty(a.shape[:-1], dtype=np.uint8)
+        ff = np.empty(fshape[:-1], dtype=np.uint8)
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        args = [physics, dim, order, _dp(a), nx, ny, dx, cfl, tf, tol, gamma]
+        if self.prefix == "ref_":
+            self.lib.ref_amr_level1(*args, _dp(coarse), _dp(fine), cf.ctypes.data_as(u8), ff.ctypes.data_as(u8))
+            return coarse, fine, cf, ff
+        t = _D()
+        n = self.lib.orc_amr_level1(*args, max_steps, _dp(coarse), _dp(fine), cf.ctypes.data_as(u8),
+                                    ff.ctypes.data_as(u8), ctypes.byref(t))
+        return coarse, fine, cf, ff, t.value, n
+
+    def amr_refine_flags(self, W, physics, dx, tol, cflags, fflags, gamma=1.4):
+        """One sticky flag pass; returns updated copies of the flag arrays."""
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        cf = np.array(cflags, dtype=np.uint8, copy=True)
+        ff = np.array(fflags, dtype=np.uint8, copy=True)
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        self.lib.orc_amr_refine_flags(physics, dim, _dp(a), nx, ny, dx, tol, gamma, cf.ctypes.data_as(u8),
+                                      ff.ctypes.data_as(u8))
+        return cf, ff
 
     # oracle-only controls
     def set_con2prim(self, tol, max_iterations):

@@ -52,6 +52,17 @@ int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, doubl
 void orc_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M,
                   double a, double bx, double by, double bz, double* out);
 
+/* Two-level AMR (EulerAMR::solve{,2D}Level1AMR and the special-relativistic copies):
+ * coarse nx x ny, fine 2nx x 2ny, flags as 0/1 bytes; at most max_coarse_steps coarse
+ * steps (< 0: none).  Returns the coarse steps taken. */
+int orc_amr_level1(int physics, int dim, int order, const double* prim, int nx, int ny, double dx, double cfl,
+                   double tf, double tol, double gamma, int max_coarse_steps, double* coarse, double* fine,
+                   unsigned char* cflags, unsigned char* fflags, double* t_out);
+
+/* One sticky refinement-flag pass over a coarse state (flags in/out). */
+void orc_amr_refine_flags(int physics, int dim, const double* coarse, int nx, int ny, double dx, double tol,
+                          double gamma, unsigned char* cflags, unsigned char* fflags);
+
 /* Excision radius override (<= 0: the reference's horizon radius M (1 + sqrt(1 - a^2))). */
 void orc_gr_set_excision(double radius);
 

@@ -497,3 +497,49 @@ void ref_gr_point(int what, const double* in, double x, double y, double z, doub
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// AMR level 1 (AMR/EulerAMR.cpp:169-590, AMR/RelativisticEulerAMR.cpp): the reference's
+// own solve{,2D}Level1AMR, run to final_time; coarse [ny][nx], fine [2ny][2nx],
+// flags as 0/1 bytes.
+// ---------------------------------------------------------------------------
+#include "AMR/EulerAMR.hpp"
+#include "AMR/RelativisticEulerAMR.hpp"
+
+namespace {
+template <class AMR, class S>
+void amr_level1(int dim, int order, const double* prim, int nx, int ny, double dx, double cfl, double tf, double tol,
+                double gamma, double* coarse, double* fine, unsigned char* cflags, unsigned char* fflags) {
+    QuietCout quiet;
+    EulerMaterialParameters mp(gamma);
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        auto r = AMR::solveLevel1AMR(c, dx, cfl, tf, tol, order, mp);
+        from_1d(std::get<0>(r), coarse);
+        from_1d(std::get<1>(r), fine);
+        for (int i = 0; i < nx; ++i) cflags[i] = std::get<2>(r)[i] ? 1 : 0;
+        for (int i = 0; i < 2 * nx; ++i) fflags[i] = std::get<3>(r)[i] ? 1 : 0;
+        return;
+    }
+    std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+    auto r = AMR::solve2DLevel1AMR(c, dx, cfl, tf, tol, order, mp);
+    from_2d(std::get<0>(r), coarse);
+    from_2d(std::get<1>(r), fine);
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) cflags[(size_t)i * nx + j] = std::get<2>(r)[i][j] ? 1 : 0;
+    for (int i = 0; i < 2 * ny; ++i)
+        for (int j = 0; j < 2 * nx; ++j) fflags[(size_t)i * 2 * nx + j] = std::get<3>(r)[i][j] ? 1 : 0;
+}
+}  // namespace
+
+extern "C" int ref_amr_level1(int physics, int dim, int order, const double* prim, int nx, int ny, double dx,
+                              double cfl, double tf, double tol, double gamma, double* coarse, double* fine,
+                              unsigned char* cflags, unsigned char* fflags) {
+    if (physics == 0)
+        amr_level1<EulerAMR, EulerStateVector>(dim, order, prim, nx, ny, dx, cfl, tf, tol, gamma, coarse, fine,
+                                               cflags, fflags);
+    else
+        amr_level1<RelativisticEulerAMR, RelativisticEulerStateVector>(dim, order, prim, nx, ny, dx, cfl, tf, tol,
+                                                                       gamma, coarse, fine, cflags, fflags);
+    return 0;
+}

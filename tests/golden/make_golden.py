@@ -25,6 +25,19 @@ from oracle import load_ref  # noqa: E402
 OUT = os.path.dirname(os.path.abspath(__file__))
 
 
+def amr_cases():
+    """(tag, physics, order, input, tolerance, gamma) of the AMR fixtures."""
+    spiky = workloads.quadrants_2d(10, seed=6, ny=8)
+    spiky[..., 1:3] *= 0.5
+    spiky[3, 4] = (1.9, 0.2, -0.3, 0.0, 1.7)
+    spiky[6, 1] = (0.3, -0.4, 0.1, 0.0, 0.4)
+    return [("n1_slic", 0, 2, workloads.sod_1d(40), 5.0, 1.4),
+            ("n2_slic", 0, 2, workloads.quadrants_2d(12, seed=1, ny=10), 5.0, 1.4),
+            ("n2_force", 0, 1, workloads.quadrants_2d(9, seed=2, ny=11), 4.0, 1.4),
+            ("sr1_force", 1, 1, workloads.sr_riemann_1d(40, seed=3), 20.0, 5.0 / 3.0),
+            ("sr2_slic", 1, 2, spiky, 3.0, 5.0 / 3.0)]
+
+
 def main() -> None:
     ref = load_ref()
     if ref is None:
@@ -140,6 +153,16 @@ def main() -> None:
         fx[f"point_{tag}"] = np.array([ref.gr_point(what, u, *xyz, 0.01, kat_hole, 5.0 / 3.0)
                                          for u, xyz in zip(U, pts)])
     np.savez_compressed(os.path.join(OUT, "blackhole.npz"), **fx)
+
+    # 6. Two-level AMR: the reference's solve{,2D}Level1AMR on small Newtonian and
+    #    special-relativistic problems, coarse and fine states and both flag grids.
+    fx = {}
+    for tag, physics, order, W, tol, gamma in amr_cases():
+        nx = W.shape[-2]
+        c, f, cf, ff = ref.amr_level1(W, physics, order, 1.0 / nx, 0.8, 0.02, tol, gamma)
+        fx.update({f"{tag}_input": W, f"{tag}_coarse": c, f"{tag}_fine": f, f"{tag}_cflags": cf,
+                   f"{tag}_fflags": ff, f"{tag}_meta": np.array([physics, order, tol, gamma])})
+    np.savez_compressed(os.path.join(OUT, "amr.npz"), **fx)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -1,0 +1,89 @@
+"""GPU parity of the two-level AMR path (cosmos_amr_*): refinement flags,
+masked sweeps and the level-1 driver against the CPU oracle, bitwise for
+states and flags."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel_linf, same
+from cosmos_amd import Amr, workloads
+from test_oracle_amr import cases
+
+pytestmark = pytest.mark.gpu
+
+G_N, G_SR = 1.4, 5.0 / 3.0
+
+
+def amr_for(W, physics, gamma):
+    dim = 1 if W.ndim == 2 else 2
+    nx = W.shape[-2]
+    ny = W.shape[0] if dim == 2 else 1
+    a = Amr(physics, dim, nx, ny, gamma)
+    a.upload(W)
+    return a
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_amr_level1_bitwise(orc, case):
+    physics, order, W, tol, g = cases()[case]
+    nx = W.shape[-2]
+    exp = orc.amr_level1(W, physics, order, 1.0 / nx, 0.8, 0.02, tol, g)
+    with amr_for(W, physics, g) as a:
+        t, n = a.solve_level1(order, 1.0 / nx, 0.8, 0.02, tol)
+        got = a.download()
+    for k, name in enumerate(("coarse", "fine", "coarse flags", "fine flags")):
+        assert same(got[k], exp[k]), (name, rel_linf(got[k], exp[k]) if k < 2 else None)
+    assert (t, n) == (exp[4], exp[5])
+
+
+def test_amr_matches_golden():
+    g = golden("amr.npz")
+    for tag in sorted({k.rsplit("_", 1)[0] for k in g.files if k.endswith("_meta")}):
+        physics, order, tol, gamma = g[f"{tag}_meta"]
+        W = g[f"{tag}_input"]
+        with amr_for(W, int(physics), gamma) as a:
+            a.solve_level1(int(order), 1.0 / W.shape[-2], 0.8, 0.02, tol)
+            c, f, cf, ff = a.download()
+        assert same(c, g[f"{tag}_coarse"]) and same(f, g[f"{tag}_fine"]), tag
+        assert same(cf, g[f"{tag}_cflags"]) and same(ff, g[f"{tag}_fflags"]), tag
+
+
+@pytest.mark.parametrize("physics,gamma", [(0, G_N), (1, G_SR)])
+@pytest.mark.parametrize("shape", [(37,), (23, 30), (64, 48)])
+def test_amr_flags_and_masked_sweep(orc, physics, gamma, shape):
+    rng = np.random.default_rng(len(shape) * 7 + physics)
+    W = np.empty(tuple(shape) + (5,))
+    W[..., 0] = rng.uniform(0.5, 2.0, shape)
+    W[..., 1:3] = rng.uniform(-0.4, 0.4, tuple(shape) + (2,))
+    W[..., 3] = 0.0
+    W[..., 4] = rng.uniform(0.5, 2.0, shape)
+    nx = shape[-1]
+    dx, tol = 1.0 / nx, 60.0
+    cshape = shape
+    fshape = (2 * shape[0],) if len(shape) == 1 else (2 * shape[0], 2 * shape[1])
+    cf0 = np.ones(cshape, dtype=np.uint8)
+    ff0 = np.zeros(fshape, dtype=np.uint8)
+    ecf, eff = orc.amr_refine_flags(W, physics, dx, tol, cf0, ff0, gamma)
+    assert 0 < (ecf == 0).sum() < ecf.size
+    with amr_for(W, physics, gamma) as a:
+        a.refine_flags(dx, tol)
+        c, f, cf, ff = a.download()
+        assert same(cf, ecf) and same(ff, eff)
+        for axis in ((0,) if len(shape) == 1 else (0, 1)):
+            a.sweep(0, 2, axis, dx, 0.1 * dx)
+        c2, _, _, _ = a.download()
+    exp = W
+    for axis in ((0,) if len(shape) == 1 else (0, 1)):
+        full = orc.sweep(exp, physics, 2, axis, dx, 0.1 * dx, 0.0, 1, gamma)
+        exp = np.where(ecf[..., None] == 1, full, exp)
+    assert same(c2, exp), rel_linf(c2, exp)
+
+
+def test_amr_2d_larger_newtonian(orc):
+    W = workloads.quadrants_2d(48, seed=9, ny=40)
+    exp = orc.amr_level1(W, 0, 2, 1.0 / 48, 0.9, 1.0, 8.0, G_N, max_steps=6)
+    with amr_for(W, 0, G_N) as a:
+        t, n = a.solve_level1(2, 1.0 / 48, 0.9, 1.0, 8.0, 6)
+        got = a.download()
+    for k in range(4):
+        assert same(got[k], exp[k]), k
+    assert (t, n) == (exp[4], exp[5]) and n == 6
