@@ -38,6 +38,7 @@ struct cosmos_solver {
     double* stage = nullptr;                   // device, AoS staging for transfers (lazily allocated)
     bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
     int mode = COSMOS_MODE_EXACT;
+    int y_rows = 0;                            // y-sweep rows per thread (0: by physics)
     cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
     double excision = 0.0;                     // > 0: excision radius override
     double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
@@ -149,6 +150,7 @@ SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, in
     L.step = nullptr;
     L.use_full = 0;
     L.fuse_ws = 0;
+    L.y_rows = s->y_rows;
     return L;
 }
 
@@ -387,6 +389,15 @@ int cosmos_set_mode(cosmos_solver* s, int mode) {
         if (mode != COSMOS_MODE_EXACT && mode != COSMOS_MODE_FAST)
             return fail(s, COSMOS_ERR_ARGUMENT, "mode must be COSMOS_MODE_EXACT or COSMOS_MODE_FAST");
         s->mode = mode;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_set_tile(cosmos_solver* s, int y_rows) {
+    return guarded(s, [&] {
+        if (y_rows != 0 && y_rows != 16 && y_rows != 32 && y_rows != 64)
+            return fail(s, COSMOS_ERR_ARGUMENT, "y_rows must be 0 (default), 16, 32 or 64");
+        s->y_rows = y_rows;
         return COSMOS_OK;
     });
 }

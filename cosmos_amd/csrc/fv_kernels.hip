@@ -35,7 +35,6 @@ namespace {
 
 constexpr int kXThreads = 256;
 constexpr int kYThreads = 64;
-constexpr int kYRows = 64;
 
 __device__ __forceinline__ int mirror2(int c, int n) {
     if (c < 0) c = -c - 1;
@@ -284,9 +283,9 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
 }
 
 // ---------------------------------------------------------------------------
-// y sweep: thread per column, marching down kYRows rows
+// y sweep: thread per column, marching down ROWS rows
 // ---------------------------------------------------------------------------
-template <class Phys, int ORDER, int LIMMODE, bool WS, bool W0>
+template <class Phys, int ORDER, int LIMMODE, bool WS, bool W0, int ROWS>
 __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_y(SweepLaunch L) {
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
@@ -298,8 +297,8 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(4, 8)
     const int j = bx * kYThreads + threadIdx.x;
     const bool colok = j < nx;
     const int jj = colok ? j : nx - 1;
-    const int i0 = by * kYRows;
-    const int i1 = min(i0 + kYRows, ny);
+    const int i0 = by * ROWS;
+    const int i1 = min(i0 + ROWS, ny);
     const long pitch = L.pitch;
     double wsmax = 0.0;
 
@@ -495,13 +494,24 @@ hipError_t dispatch_x(const SweepLaunch& L, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Rows per y-sweep thread: long marches amortise the 4-row halo for the
+// HBM-bound Newtonian sweeps; the compute-bound special-relativistic sweeps
+// need more waves in flight, so shorter marches (more blocks).
+int y_rows_for(const SweepLaunch& L) {
+    if (L.y_rows == 16 || L.y_rows == 32 || L.y_rows == 64) return L.y_rows;
+    return L.physics == 1 ? 16 : 64;
+}
+
 template <class Phys, int ORDER, int LIMMODE, bool W0>
 hipError_t dispatch_y(const SweepLaunch& L, hipStream_t s) {
+    const int rows = L.fuse_ws ? 64 : y_rows_for(L);
     const int tx = (L.nx + kYThreads - 1) / kYThreads;
-    const int ty = (L.ny + kYRows - 1) / kYRows;
+    const int ty = (L.ny + rows - 1) / rows;
     dim3 grid((unsigned)(tx * ty)), block(kYThreads);
-    if (L.fuse_ws) hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, true, W0>), grid, block, 0, s, L);
-    else hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, false, W0>), grid, block, 0, s, L);
+    if (L.fuse_ws) hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, true, W0, 64>), grid, block, 0, s, L);
+    else if (rows == 16) hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, false, W0, 16>), grid, block, 0, s, L);
+    else if (rows == 32) hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, false, W0, 32>), grid, block, 0, s, L);
+    else hipLaunchKernelGGL((k_sweep_y<Phys, ORDER, LIMMODE, false, W0, 64>), grid, block, 0, s, L);
     return hipGetLastError();
 }
 

@@ -122,6 +122,7 @@ struct SweepLaunch {
     StepState* step;            // device state: coefficients and done flag
     int use_full;               // which StepState coefficients (0 half, 1 full)
     int fuse_ws;                // merge the max wave speed of the output into step->ws_slots
+    int y_rows;                 // rows per y-sweep thread (16, 32, 64; 0: by physics)
 };
 
 // All return hipSuccess or the launch error.

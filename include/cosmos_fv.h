@@ -68,6 +68,12 @@ const char* cosmos_last_error(const cosmos_solver* s);
  * always run exact. */
 int cosmos_set_mode(cosmos_solver* s, int mode);
 
+/* Launch shape (no counterpart in the reference): rows of the grid each
+ * y-sweep thread marches down (16, 32 or 64; 0 restores the default, 64 for
+ * Newtonian and 16 for special-relativistic sweeps).  Results do not depend
+ * on it. */
+int cosmos_set_tile(cosmos_solver* s, int y_rows);
+
 /* Special-relativistic con2prim controls (default: the reference's stopping
  * tolerance 1e-8 and cap of 1000 iterations,
  * RelativisticEuler/RelativisticEulerStateVector.cpp:59-80). */

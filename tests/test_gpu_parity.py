@@ -254,3 +254,23 @@ def test_fast_mode_config2_full_size_100_steps(orc):
     err = cons_rel_linf(out, exp)
     print(f"fast mode 4096^2 x 100 steps: rel L-inf per conserved variable {err:.3e}")
     assert k == 100 and err <= FAST_TOL, err
+
+
+@pytest.mark.parametrize("y_rows", [16, 32, 64])
+@pytest.mark.parametrize("physics", [NEWTONIAN, RELATIVISTIC])
+def test_y_sweep_tile_does_not_change_results(orc, y_rows, physics):
+    """The y-sweep march length is a launch shape only: bitwise the same on
+    shapes that do not divide 16, 32 or 64."""
+    gamma = G_N if physics == NEWTONIAN else G_SR
+    for nx, ny in ((37, 70), (130, 33)):
+        W = workloads.quadrants_2d(nx, seed=13, ny=ny)
+        if physics == RELATIVISTIC:
+            W[..., 1:3] *= 0.5
+        dx = 1.0 / max(nx, ny)
+        for order in (1, 2):
+            with Solver(physics, 2, nx, ny, gamma) as s:
+                s.set_tile(y_rows)
+                s.upload(W)
+                s.sweep(order, 1, dx, 0.2 * dx)
+                out = s.download()
+            assert same(out, orc.sweep(W, physics, order, 1, dx, 0.2 * dx, 0.0, 1, gamma)), (nx, ny, order)
