@@ -285,8 +285,17 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
 // ---------------------------------------------------------------------------
 // y sweep: thread per column, marching down ROWS rows
 // ---------------------------------------------------------------------------
+// Minimum waves per SIMD, which caps the VGPRs: 4 (128 VGPRs) for the
+// HBM-bound fast Newtonian sweep, which fits without spilling; the exact
+// Newtonian sweep would spill 44-68 B per lane at that cap and the
+// special-relativistic one 172-200 B (its con2prim state), so they may use
+// up to 168 (3 waves) and 256 (2 waves) VGPRs.
+template <class Phys>
+constexpr int y_min_waves() { return Phys::kPhysics == 1 ? 2 : (Phys::kFast ? 4 : 3); }
+
 template <class Phys, int ORDER, int LIMMODE, bool WS, bool W0, int ROWS>
-__global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_sweep_y(SweepLaunch L) {
+__global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min_waves<Phys>(), 8))) void k_sweep_y(
+    SweepLaunch L) {
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
     const PhysParams& P = L.P;
