@@ -48,6 +48,19 @@ __device__ __forceinline__ int clamp1(int c, int n) {
     return c >= n ? n - 1 : c;
 }
 
+// The hardware hands consecutive workgroups to the 8 XCDs round robin, and
+// each XCD has its own L2.  Renumber so that each XCD works through one
+// contiguous range of tiles: neighbouring tiles, which share halo cache
+// lines, then run on the same XCD at about the same time and the second
+// reader hits in L2 instead of fetching the line from HBM again.  A
+// bijection on [0, n) for any n.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int xcd = b % kXcds, k = b / kXcds;
+    const int base = n / kXcds, extra = n % kXcds;
+    return xcd * base + (xcd < extra ? xcd : extra) + k;
+}
+
 template <bool W0>
 __device__ __forceinline__ void load_cell(const ConstPlanes& in, long idx, double* W) {
 #pragma unroll
@@ -199,8 +212,9 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const PhysParams& P = L.P;
     const int nx = L.nx;
     const int tiles = (nx + TW - 1) / TW;
-    const int row = blockIdx.x / tiles;
-    const int tile = blockIdx.x - row * tiles;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int row = blk / tiles;
+    const int tile = blk - row * tiles;
     const int s = threadIdx.x;
     const int c = tile * TW - G + s;
     const int cc = ORDER == 2 ? mirror2(c, nx) : clamp1(c, nx);
@@ -301,8 +315,9 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min
     const PhysParams& P = L.P;
     const int nx = L.nx, ny = L.ny;
     const int tx = (nx + kYThreads - 1) / kYThreads;
-    const int by = blockIdx.x / tx;
-    const int bx = blockIdx.x - by * tx;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int by = blk / tx;
+    const int bx = blk - by * tx;
     const int j = bx * kYThreads + threadIdx.x;
     const bool colok = j < nx;
     const int jj = colok ? j : nx - 1;
