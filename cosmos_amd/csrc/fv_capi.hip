@@ -39,6 +39,7 @@ struct cosmos_solver {
     bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
     int mode = COSMOS_MODE_EXACT;
     int y_rows = 0;                            // y-sweep rows per thread (0: by physics)
+    int ws_mode = 0;                           // fused wave-speed hand-off (0: default)
     cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
     double excision = 0.0;                     // > 0: excision radius override
     double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
@@ -393,11 +394,14 @@ int cosmos_set_mode(cosmos_solver* s, int mode) {
     });
 }
 
-int cosmos_set_tile(cosmos_solver* s, int y_rows) {
+int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode) {
     return guarded(s, [&] {
         if (y_rows != 0 && y_rows != 16 && y_rows != 32 && y_rows != 64)
             return fail(s, COSMOS_ERR_ARGUMENT, "y_rows must be 0 (default), 16, 32 or 64");
+        if (ws_mode < 0 || ws_mode > 2) return fail(s, COSMOS_ERR_ARGUMENT, "ws_mode must be 0, 1 or 2");
         s->y_rows = y_rows;
+        s->ws_mode = ws_mode;
+        s->run_ready = false;
         return COSMOS_OK;
     });
 }
@@ -581,7 +585,8 @@ int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, d
         h.ws_slots = s->st_slots;
         h.ws_part = s->st_part;
         h.ws_nparts = 0;
-        h.ws_nparts_sweep = cfv::sweep_x_blocks(s->nx, s->ny, order);
+        h.ws_slot_atomics = s->ws_mode != 1;
+        h.ws_nparts_sweep = h.ws_slot_atomics ? 0 : cfv::sweep_x_blocks(s->nx, s->ny, order);
         HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
         rc = max_ws_impl(s, s->st_slots);
         if (rc) return rc;

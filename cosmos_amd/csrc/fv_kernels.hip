@@ -182,6 +182,18 @@ __device__ __forceinline__ void block_max_store(double m, double* part, double* 
     }
 }
 
+// Wave-wide max of non-negative wave speeds merged with one integer atomic
+// max into slot (wave_id mod kMaxSlots) (no LDS, no barrier).
+__device__ __forceinline__ void wave_max_to_slot(double m, unsigned long long* slots, int wave_id) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(m, off, 64);
+        m = (o > m) ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0 && m > 0.0)
+        atomicMax(slots + (wave_id % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(m));
+}
+
 __device__ __forceinline__ bool resolve_coef(const SweepLaunch& L, SweepCoef& cf) {
     cf = L.coef;
     if (L.step) {
@@ -291,8 +303,12 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
             double w = Phys::template wavespeed<DIM>(Wn, P);
             if (w > m) m = w;
         }
-        // rows no longer read in stage D: sB for SLIC, sA for FORCE
-        block_max_store<NT>(m, L.step->ws_part, ORDER == 2 ? &sB[0][0] : &sA[0][0]);
+        if (L.step->ws_slot_atomics) {
+            wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
+        } else {
+            // rows no longer read in stage D: sB for SLIC, sA for FORCE
+            block_max_store<NT>(m, L.step->ws_part, ORDER == 2 ? &sB[0][0] : &sA[0][0]);
+        }
     }
 }
 
@@ -397,7 +413,10 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min
             }
         }
     }
-    if constexpr (WS) block_max_store<kYThreads>(wsmax, L.step->ws_part, nullptr);
+    if constexpr (WS) {
+        if (L.step->ws_slot_atomics) wave_max_to_slot(wsmax, L.step->ws_slots, blockIdx.x);
+        else block_max_store<kYThreads>(wsmax, L.step->ws_part, nullptr);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -88,7 +88,8 @@ struct StepState {
     unsigned long long* ws_slots;  // max wave speed of the state at cosmos_begin (kSlotWords words)
     double* ws_part;               // per-block maxima written by the last sweep of each step
     int ws_nparts;                 // entries of ws_part to read at the next step head
-    int ws_nparts_sweep;           // blocks of the last sweep of a step
+    int ws_nparts_sweep;           // blocks of the last sweep of a step (0 with slot atomics)
+    int ws_slot_atomics;           // the last sweep merges per-wave maxima into ws_slots
     int iter;
     int max_steps;
     int done;

@@ -90,9 +90,11 @@ class Solver:
     def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
         self._check(self._lib.cosmos_set_con2prim(self._h, tolerance, max_iterations))
 
-    def set_tile(self, y_rows: int) -> None:
-        """Rows per y-sweep thread (16, 32, 64; 0 = default for the physics)."""
-        self._check(self._lib.cosmos_set_tile(self._h, y_rows))
+    def set_launch(self, y_rows: int = 0, ws_mode: int = 0) -> None:
+        """Launch shape: rows per y-sweep thread (16, 32, 64; 0 = default for the
+        physics) and the wave-speed hand-off (1 per-block partials, 2 per-wave
+        slot atomics, 0 = default)."""
+        self._check(self._lib.cosmos_set_launch(self._h, y_rows, ws_mode))
 
     def set_spacetime(self, mass: float = 1.0, spin: float = 0.0, x: float = 0.5, y: float = 0.5,
                       z: float = 0.0) -> None:

@@ -68,11 +68,14 @@ const char* cosmos_last_error(const cosmos_solver* s);
  * always run exact. */
 int cosmos_set_mode(cosmos_solver* s, int mode);
 
-/* Launch shape (no counterpart in the reference): rows of the grid each
- * y-sweep thread marches down (16, 32 or 64; 0 restores the default, 64 for
- * Newtonian and 16 for special-relativistic sweeps).  Results do not depend
- * on it. */
-int cosmos_set_tile(cosmos_solver* s, int y_rows);
+/* Launch shape (no counterpart in the reference; results do not depend on it):
+ *   y_rows   rows of the grid each y-sweep thread marches down (16, 32 or 64;
+ *            0: the default, 64 for Newtonian and 16 for special-relativistic)
+ *   ws_mode  how the solve loop's fused wave-speed maximum leaves the last
+ *            sweep of a step: 1 one partial per workgroup, reduced by the
+ *            next step's head; 2 one atomic maximum per wavefront into 32
+ *            slots; 0: the default (2). */
+int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode);
 
 /* Special-relativistic con2prim controls (default: the reference's stopping
  * tolerance 1e-8 and cap of 1000 iterations,

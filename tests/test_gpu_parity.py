@@ -269,7 +269,7 @@ def test_y_sweep_tile_does_not_change_results(orc, y_rows, physics):
         dx = 1.0 / max(nx, ny)
         for order in (1, 2):
             with Solver(physics, 2, nx, ny, gamma) as s:
-                s.set_tile(y_rows)
+                s.set_launch(y_rows)
                 s.upload(W)
                 s.sweep(order, 1, dx, 0.2 * dx)
                 out = s.download()
