@@ -1,0 +1,607 @@
+// Reverse-mode (adjoint) finite-volume update on gfx950: the adjoint of one
+// FORCE / SLIC sweep as one fused kernel, and a checkpointed driver for the
+// adjoint of a whole trajectory of dimensionally split steps (Newtonian).
+//
+// Adjoint of one sweep (per line along the sweep axis; lambda = adjoint of
+// the sweep output O, nu = (dc2p/dU)^T lambda at O):
+//   U_c = p2c(W_c); faces (a_c, b_c) = half step of c from U_{c-1..c+1};
+//   F_{c+1/2} = FORCE(b_c, a_{c+1});  O_c = c2p(U_c + k (F_{c-1/2} - F_{c+1/2}))
+// so  phi_{c+1/2} = k (nu_{c+1} - nu_c)  is the adjoint of each interface flux,
+// FORCE^T gives the adjoints of b_c and a_{c+1}, the half-step adjoint the
+// contributions to U_{c-1..c+1}, and  gU_c = nu_c + sum of contributions,
+// lambda_in_c = (dp2c/dW)^T gU_c.  Ghost cells of the transmissive boundary
+// are copies of interior cells (mirror order for SLIC, edge copy for FORCE,
+// Solvers/EulerSolvers.cpp:3-85), so their adjoints fold back onto those
+// cells.
+//
+// Kernel layout: a 256-cell tile of one line per workgroup with a 3-cell
+// halo (250 outputs), five LDS stages: U and nu; the half steps; the FORCE
+// adjoints (written in place over the face states: each slot is read and
+// written by one thread); the half-step adjoints; the per-cell totals and the
+// ghost fold.  x lines are contiguous; y lines are strided, with
+// neighbouring lines in consecutive workgroups so that they share cache lines.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/cosmos_fv.h"
+#include "fv_adjoint_physics.hpp"
+#include "fv_kernels.hpp"
+
+namespace cfv {
+namespace {
+
+constexpr int kAT = 256;  // tile (threads)
+constexpr int kAH = 3;    // halo
+
+struct AdjLaunch {
+    ConstPlanes W, O, lam;  // sweep input, sweep output, adjoint of the output
+    Planes lam_in;          // adjoint of the input
+    int n, lines;           // cells per line, lines
+    long pitch;
+    int axis, order;
+    adj::Coef C;
+};
+
+__device__ __forceinline__ int mirror_c(int c, int n) {
+    if (c < 0) c = -c - 1;
+    if (c >= n) c = 2 * n - 1 - c;
+    c = c < 0 ? 0 : c;
+    return c >= n ? n - 1 : c;
+}
+__device__ __forceinline__ int clamp_c(int c, int n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
+
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int xcd = b % 8, k = b / 8, base = nb / 8, extra = nb % 8;
+    return xcd * base + (xcd < extra ? xcd : extra) + k;
+}
+
+template <int AXIS, int ORDER>
+__global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
+    __shared__ double sU[5][kAT], sN[5][kAT], sA[5][kAT], sB[5][kAT], sM[5][kAT];
+    const int TW = kAT - 2 * kAH;
+    const int tiles = (L.n + TW - 1) / TW;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    int line, tile;
+    if (AXIS == 0) {
+        line = blk / tiles;
+        tile = blk - line * tiles;
+    } else {
+        tile = blk / L.lines;
+        line = blk - tile * L.lines;
+    }
+    const int s = threadIdx.x;
+    const int c = tile * TW - kAH + s;
+    const int n = L.n;
+    const int cm = ORDER == 2 ? mirror_c(c, n) : clamp_c(c, n);
+    const long addr = AXIS == 0 ? (long)line * L.pitch + cm : (long)cm * L.pitch + line;
+    const adj::Coef& C = L.C;
+
+    // stage 1: U of every position; nu of real cells (0 for ghosts)
+    double W[5], U[5], nu[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) W[k] = L.W.p[k][addr];
+    adj::p2c(W, U, C);
+    const bool real = c >= 0 && c < n;
+    if (real) {
+        double O[5], g[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            O[k] = L.O.p[k][addr];
+            g[k] = L.lam.p[k][addr];
+        }
+        adj::vjp_c2p(O, g, nu, C);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) nu[k] = 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        sU[k][s] = U[k];
+        sN[k][s] = nu[k];
+    }
+    __syncthreads();
+
+    double ul[5], ur[5];
+    if (s > 0 && s < kAT - 1) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            ul[k] = sU[k][s - 1];
+            ur[k] = sU[k][s + 1];
+        }
+    }
+    if constexpr (ORDER == 2) {
+        // stage 2: the half step of each position -> (a, b)
+        if (s > 0 && s < kAT - 1) {
+            double a[5], b[5];
+            adj::half_step<AXIS>(ul, U, ur, C, a, b);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                sA[k][s] = a[k];
+                sB[k][s] = b[k];
+            }
+        }
+        __syncthreads();
+    }
+    // stage 3: adjoint of interface (s, s+1), phi = k (nu_{s+1} - nu_s);
+    // the state adjoints overwrite the slots this thread alone reads
+    if (s > 0 && s < kAT - 2) {
+        double UL[5], UR[5], phi[5], gl[5], gr[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            UL[k] = ORDER == 2 ? sB[k][s] : U[k];
+            UR[k] = ORDER == 2 ? sA[k][s + 1] : ur[k];
+            phi[k] = C.k * (sN[k][s + 1] - nu[k]);
+        }
+        adj::vjp_force<AXIS>(UL, UR, phi, C, gl, gr);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            sB[k][s] = gl[k];
+            sA[k][s + 1] = gr[k];
+        }
+    }
+    __syncthreads();
+    if constexpr (ORDER == 2) {
+        // stage 4: half-step adjoint of each position -> contributions to s-1, s, s+1
+        if (s > 1 && s < kAT - 2) {
+            double ga[5], gb[5], gL[5], gM[5], gR[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                ga[k] = sA[k][s];
+                gb[k] = sB[k][s];
+            }
+            adj::vjp_half_step<AXIS>(ul, U, ur, ga, gb, C, gL, gM, gR);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                sA[k][s] = gL[k];
+                sM[k][s] = gM[k];
+                sB[k][s] = gR[k];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                sA[k][s] = 0.0;
+                sM[k][s] = 0.0;
+                sB[k][s] = 0.0;
+            }
+        }
+        __syncthreads();
+    }
+    // stage 5: the total adjoint of each position's U (into sN, own slot)
+    double tot[5];
+    if (s > 0 && s < kAT - 1) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (ORDER == 2) tot[k] = ((nu[k] + sB[k][s - 1]) + sM[k][s]) + sA[k][s + 1];
+            else tot[k] = (nu[k] + sB[k][s]) + sA[k][s];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) tot[k] = 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sN[k][s] = tot[k];
+    __syncthreads();
+    // stage 6: outputs fold the ghost positions that copy them
+    if (s >= kAH && s < kAT - kAH && real) {
+        if (ORDER == 2) {
+            if (c == 0) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 1];
+            if (c == 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 3];
+            if (c == n - 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 1];
+            if (c == n - 2) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 3];
+        } else {
+            if (c == 0) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 1];
+            if (c == n - 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 1];
+        }
+        double gW[5];
+        adj::vjp_p2c(W, tot, gW, C);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) L.lam_in.p[k][addr] = gW[k];
+    }
+}
+
+dim3 adj_grid(const AdjLaunch& L) {
+    const int TW = kAT - 2 * kAH;
+    const long tiles = (L.n + TW - 1) / TW;
+    return dim3((unsigned)(tiles * L.lines));
+}
+
+hipError_t launch_adjoint_sweep(const AdjLaunch& L, hipStream_t s) {
+    const dim3 g = adj_grid(L), b(kAT);
+    if (L.axis == 0) {
+        if (L.order == 2) hipLaunchKernelGGL((k_adjoint_sweep<0, 2>), g, b, 0, s, L);
+        else hipLaunchKernelGGL((k_adjoint_sweep<0, 1>), g, b, 0, s, L);
+    } else {
+        if (L.order == 2) hipLaunchKernelGGL((k_adjoint_sweep<1, 2>), g, b, 0, s, L);
+        else hipLaunchKernelGGL((k_adjoint_sweep<1, 1>), g, b, 0, s, L);
+    }
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_planes_to_aos(ConstPlanes in, double* aos, int nx, int ny, long pitch) {
+    long total = (long)nx * ny;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+        long i = c / nx, j = c - i * nx;
+        for (int k = 0; k < 5; ++k) aos[5 * c + k] = in.p[k][i * pitch + j];
+    }
+}
+__global__ __launch_bounds__(256) void k_aos_to_planes(const double* aos, Planes out, int nx, int ny, long pitch) {
+    long total = (long)nx * ny;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < total; c += (long)gridDim.x * blockDim.x) {
+        long i = c / nx, j = c - i * nx;
+        for (int k = 0; k < 5; ++k) out.p[k][i * pitch + j] = aos[5 * c + k];
+    }
+}
+
+}  // namespace
+}  // namespace cfv
+
+using cfv::ConstPlanes;
+using cfv::Planes;
+
+// One stored state: five planes of ny x pitch doubles.
+struct AdjState {
+    double* base = nullptr;
+};
+
+struct cosmos_adjoint {
+    int dim = 2, nx = 0, ny = 0;
+    double gamma = 1.4;
+    cosmos_solver* s = nullptr;
+    long pitch = 0;
+    size_t plane = 0;             // doubles per plane
+    double* lam[2] = {nullptr, nullptr};  // adjoint ping-pong (5 planes each)
+    double* stage = nullptr;      // AoS transfer buffer
+    // trajectory
+    int order = 2, limiter = 1;
+    double bias = 0.0, dx = 0.0;
+    std::vector<double> dts;      // recorded time steps
+    int every = 20;               // checkpoint interval
+    std::vector<double*> checkpoints;  // state at steps 0, every, 2 every, ...
+    std::vector<double*> seg;     // per-sweep states of one segment during the reverse pass
+    double* final_state = nullptr;  // the state after the recorded steps (restored after the reverse pass)
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_adj_create_error;
+
+int adj_fail(cosmos_adjoint* a, int code, const std::string& msg) {
+    if (a) a->err = msg;
+    else g_adj_create_error = msg;
+    return code;
+}
+
+#define ADJCHK(a, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return adj_fail((a), COSMOS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int from_solver(cosmos_adjoint* a, int rc) { return rc ? adj_fail(a, rc, cosmos_last_error(a->s)) : rc; }
+
+Planes planes_at(const cosmos_adjoint* a, double* base) {
+    Planes p;
+    for (int k = 0; k < 5; ++k) p.p[k] = base + k * a->plane;
+    return p;
+}
+ConstPlanes cplanes_at(const cosmos_adjoint* a, const double* base) {
+    ConstPlanes p;
+    for (int k = 0; k < 5; ++k) p.p[k] = base + k * a->plane;
+    return p;
+}
+
+hipStream_t stream_of(cosmos_adjoint* a) { return (hipStream_t)cosmos_stream(a->s); }
+
+int current_base(cosmos_adjoint* a, double*& base) {
+    double* planes[5];
+    long pitch = 0;
+    int rc = cosmos_state_planes(a->s, planes, &pitch);
+    if (rc) return from_solver(a, rc);
+    base = planes[0];
+    return COSMOS_OK;
+}
+
+int alloc_state(cosmos_adjoint* a, double*& p) {
+    ADJCHK(a, hipMalloc((void**)&p, sizeof(double) * 5 * a->plane));
+    return COSMOS_OK;
+}
+
+// copy the handle's current state into dst (5 planes)
+int save_state(cosmos_adjoint* a, double* dst) {
+    double* cur;
+    int rc = current_base(a, cur);
+    if (rc) return rc;
+    ADJCHK(a, hipMemcpyAsync(dst, cur, sizeof(double) * 5 * a->plane, hipMemcpyDeviceToDevice, stream_of(a)));
+    return COSMOS_OK;
+}
+
+int load_state(cosmos_adjoint* a, const double* src) {
+    double* cur;
+    int rc = current_base(a, cur);
+    if (rc) return rc;
+    ADJCHK(a, hipMemcpyAsync(cur, src, sizeof(double) * 5 * a->plane, hipMemcpyDeviceToDevice, stream_of(a)));
+    return COSMOS_OK;
+}
+
+cfv::adj::Coef coef(const cosmos_adjoint* a, double dx, double dt, double bias, int limiter) {
+    cfv::adj::Coef C{};
+    C.k = dt / dx;
+    C.kinv = dx / dt;
+    C.c = 0.5 * (dt / dx);
+    C.gm1 = a->gamma - 1.0;
+    C.igm1 = 1.0 / (a->gamma - 1.0);
+    C.bias = bias;
+    C.limiter = limiter;
+    return C;
+}
+
+// lam[dst] <- adjoint of the sweep from state W (planes at w_base) to O (o_base), given lam[src]
+int adjoint_sweep_dev(cosmos_adjoint* a, int order, int axis, double dx, double dt, double bias, int limiter,
+                      const double* w_base, const double* o_base, int src, int dst) {
+    cfv::AdjLaunch L{};
+    L.W = cplanes_at(a, w_base);
+    L.O = cplanes_at(a, o_base);
+    L.lam = cplanes_at(a, a->lam[src]);
+    L.lam_in = planes_at(a, a->lam[dst]);
+    L.n = axis == 0 ? a->nx : a->ny;
+    L.lines = axis == 0 ? a->ny : a->nx;
+    L.pitch = a->pitch;
+    L.axis = axis;
+    L.order = order;
+    L.C = coef(a, dx, dt, bias, limiter);
+    ADJCHK(a, cfv::launch_adjoint_sweep(L, stream_of(a)));
+    return COSMOS_OK;
+}
+
+int lam_upload(cosmos_adjoint* a, const double* host, int which) {
+    long cells = (long)a->nx * a->ny;
+    hipStream_t st = stream_of(a);
+    ADJCHK(a, hipMemcpyAsync(a->stage, host, sizeof(double) * 5 * cells, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(cfv::k_aos_to_planes, dim3(1024), dim3(256), 0, st, (const double*)a->stage,
+                       planes_at(a, a->lam[which]), a->nx, a->ny, a->pitch);
+    ADJCHK(a, hipGetLastError());
+    return COSMOS_OK;
+}
+
+int lam_download(cosmos_adjoint* a, double* host, int which) {
+    long cells = (long)a->nx * a->ny;
+    hipStream_t st = stream_of(a);
+    hipLaunchKernelGGL(cfv::k_planes_to_aos, dim3(1024), dim3(256), 0, st, cplanes_at(a, a->lam[which]), a->stage,
+                       a->nx, a->ny, a->pitch);
+    ADJCHK(a, hipGetLastError());
+    ADJCHK(a, hipMemcpyAsync(host, a->stage, sizeof(double) * 5 * cells, hipMemcpyDeviceToHost, st));
+    ADJCHK(a, hipStreamSynchronize(st));
+    return COSMOS_OK;
+}
+
+// The sweeps of one step with time step dt: (axis, dt factor) in order.
+int step_sweeps(const cosmos_adjoint* a, int* axes, double* fac) {
+    if (a->dim == 1) {
+        axes[0] = 0;
+        fac[0] = 1.0;
+        return 1;
+    }
+    axes[0] = 0;
+    fac[0] = 0.5;
+    axes[1] = 1;
+    fac[1] = 1.0;
+    axes[2] = 0;
+    fac[2] = 0.5;
+    return 3;
+}
+
+// dt * 0.5 for the outer Strang sweeps, as solve2D forms it (Solvers/EulerSecondOrderSolver.cpp:172-174)
+double sweep_dt(double dt, double f) { return f == 1.0 ? dt : dt * 0.5; }
+
+template <class F>
+int adj_guarded(cosmos_adjoint* a, F&& f) {
+    if (!a) return adj_fail(nullptr, COSMOS_ERR_ARGUMENT, "null adjoint handle");
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return adj_fail(a, COSMOS_ERR_DEVICE, "host allocation failed");
+    } catch (...) {
+        return adj_fail(a, COSMOS_ERR_STATE, "unexpected exception");
+    }
+}
+
+void free_list(std::vector<double*>& v) {
+    for (double* p : v) hipFree(p);
+    v.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosmos_adjoint_create(cosmos_adjoint** out, int physics, int dim, int nx, int ny, double adiabatic_index) {
+    if (!out) return adj_fail(nullptr, COSMOS_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    if (physics != COSMOS_NEWTONIAN) return adj_fail(nullptr, COSMOS_ERR_ARGUMENT, "the adjoint supports COSMOS_NEWTONIAN");
+    cosmos_adjoint* a = new (std::nothrow) cosmos_adjoint();
+    if (!a) return adj_fail(nullptr, COSMOS_ERR_DEVICE, "host allocation failed");
+    int rc = cosmos_create(&a->s, physics, dim, nx, ny, adiabatic_index);
+    if (rc) {
+        std::string msg = cosmos_last_error(nullptr);
+        delete a;
+        return adj_fail(nullptr, rc, msg);
+    }
+    a->dim = dim;
+    a->nx = nx;
+    a->ny = dim == 1 ? 1 : ny;
+    a->gamma = adiabatic_index;
+    double* planes[5];
+    rc = cosmos_state_planes(a->s, planes, &a->pitch);
+    if (rc) {
+        std::string msg = cosmos_last_error(a->s);
+        cosmos_adjoint_destroy(a);
+        return adj_fail(nullptr, rc, msg);
+    }
+    a->plane = (size_t)a->ny * a->pitch;
+    hipError_t e = hipMalloc((void**)&a->lam[0], sizeof(double) * 5 * a->plane);
+    if (e == hipSuccess) e = hipMalloc((void**)&a->lam[1], sizeof(double) * 5 * a->plane);
+    if (e == hipSuccess) e = hipMalloc((void**)&a->stage, sizeof(double) * 5 * (size_t)a->nx * a->ny);
+    if (e == hipSuccess) e = hipMemsetAsync(a->lam[0], 0, sizeof(double) * 5 * a->plane, stream_of(a));
+    if (e == hipSuccess) e = hipMemsetAsync(a->lam[1], 0, sizeof(double) * 5 * a->plane, stream_of(a));
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_of(a));
+    if (e != hipSuccess) {
+        std::string msg = hipGetErrorString(e);
+        cosmos_adjoint_destroy(a);
+        return adj_fail(nullptr, COSMOS_ERR_DEVICE, "adjoint setup: " + msg);
+    }
+    *out = a;
+    return COSMOS_OK;
+}
+
+int cosmos_adjoint_destroy(cosmos_adjoint* a) {
+    if (!a) return COSMOS_OK;
+    if (a->s) (void)cosmos_synchronize(a->s);
+    hipFree(a->lam[0]);
+    hipFree(a->lam[1]);
+    hipFree(a->stage);
+    free_list(a->checkpoints);
+    free_list(a->seg);
+    hipFree(a->final_state);
+    cosmos_destroy(a->s);
+    delete a;
+    return COSMOS_OK;
+}
+
+const char* cosmos_adjoint_last_error(const cosmos_adjoint* a) { return a ? a->err.c_str() : g_adj_create_error.c_str(); }
+
+cosmos_solver* cosmos_adjoint_solver(cosmos_adjoint* a) { return a ? a->s : nullptr; }
+
+int cosmos_adjoint_sweep(cosmos_adjoint* a, int order, int axis, double cell_spacing, double time_step, double bias,
+                         int slope_limiter, const double* lambda_out, double* lambda_in) {
+    return adj_guarded(a, [&] {
+        if (!lambda_out || !lambda_in) return adj_fail(a, COSMOS_ERR_ARGUMENT, "null adjoint array");
+        double *w, *o;
+        int rc = current_base(a, w);
+        if (rc) return rc;
+        rc = cosmos_sweep(a->s, order, axis, cell_spacing, time_step, bias, slope_limiter);
+        if (rc) return from_solver(a, rc);
+        if ((rc = current_base(a, o))) return rc;
+        if ((rc = lam_upload(a, lambda_out, 0))) return rc;
+        if ((rc = adjoint_sweep_dev(a, order, axis, cell_spacing, time_step, bias, slope_limiter, w, o, 0, 1)))
+            return rc;
+        return lam_download(a, lambda_in, 1);
+    });
+}
+
+int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, double cfl, int n_steps, double bias,
+                           int slope_limiter, int checkpoint_every) {
+    return adj_guarded(a, [&] {
+        if (n_steps < 1 || checkpoint_every < 1)
+            return adj_fail(a, COSMOS_ERR_ARGUMENT, "n_steps and checkpoint_every must be >= 1");
+        a->order = order;
+        a->dx = cell_spacing;
+        a->bias = bias;
+        a->limiter = slope_limiter;
+        a->every = checkpoint_every;
+        a->dts.clear();
+        free_list(a->checkpoints);
+        free_list(a->seg);
+        int axes[3];
+        double fac[3];
+        const int ns = step_sweeps(a, axes, fac);
+        for (int it = 0; it < n_steps; ++it) {
+            if (it % a->every == 0) {
+                double* cp = nullptr;
+                int rc = alloc_state(a, cp);
+                if (rc) return rc;
+                a->checkpoints.push_back(cp);
+                if ((rc = save_state(a, cp))) return rc;
+            }
+            double dt = 0.0;
+            int rc = cosmos_stable_time_step(a->s, cell_spacing, cfl, 0.0, INFINITY, it, &dt);
+            if (rc) return from_solver(a, rc);
+            a->dts.push_back(dt);
+            for (int k = 0; k < ns; ++k) {
+                rc = cosmos_sweep(a->s, order, axes[k], cell_spacing, sweep_dt(dt, fac[k]), bias, slope_limiter);
+                if (rc) return from_solver(a, rc);
+            }
+        }
+        if (!a->final_state) {
+            int rc = alloc_state(a, a->final_state);
+            if (rc) return rc;
+        }
+        int rc = save_state(a, a->final_state);
+        if (rc) return rc;
+        ADJCHK(a, hipStreamSynchronize(stream_of(a)));
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_adjoint_replay(cosmos_adjoint* a, const double* prim_in, double* prim_out) {
+    return adj_guarded(a, [&] {
+        if (a->dts.empty()) return adj_fail(a, COSMOS_ERR_STATE, "no recorded trajectory (cosmos_adjoint_forward)");
+        int rc = cosmos_upload(a->s, prim_in);
+        if (rc) return from_solver(a, rc);
+        int axes[3];
+        double fac[3];
+        const int ns = step_sweeps(a, axes, fac);
+        for (double dt : a->dts)
+            for (int k = 0; k < ns; ++k) {
+                rc = cosmos_sweep(a->s, a->order, axes[k], a->dx, sweep_dt(dt, fac[k]), a->bias, a->limiter);
+                if (rc) return from_solver(a, rc);
+            }
+        rc = cosmos_download(a->s, prim_out);
+        return from_solver(a, rc);
+    });
+}
+
+int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* grad_in) {
+    return adj_guarded(a, [&] {
+        if (a->dts.empty()) return adj_fail(a, COSMOS_ERR_STATE, "no recorded trajectory (cosmos_adjoint_forward)");
+        if (!grad_out || !grad_in) return adj_fail(a, COSMOS_ERR_ARGUMENT, "null gradient array");
+        int axes[3];
+        double fac[3];
+        const int ns = step_sweeps(a, axes, fac);
+        const int nsteps = (int)a->dts.size();
+        // segment buffers: the input of every sweep of a segment plus its final output
+        const size_t need = (size_t)a->every * ns + 1;
+        while (a->seg.size() < need) {
+            double* p = nullptr;
+            int rc = alloc_state(a, p);
+            if (rc) return rc;
+            a->seg.push_back(p);
+        }
+        int rc = lam_upload(a, grad_out, 0);
+        if (rc) return rc;
+        int cur = 0;
+        for (int sgi = (int)a->checkpoints.size() - 1; sgi >= 0; --sgi) {
+            const int s0 = sgi * a->every, s1 = std::min(nsteps, s0 + a->every);
+            // recompute the segment from its checkpoint, keeping every sweep's input
+            if ((rc = load_state(a, a->checkpoints[sgi]))) return rc;
+            int q = 0;
+            for (int it = s0; it < s1; ++it)
+                for (int k = 0; k < ns; ++k) {
+                    if ((rc = save_state(a, a->seg[q++]))) return rc;
+                    rc = cosmos_sweep(a->s, a->order, axes[k], a->dx, sweep_dt(a->dts[it], fac[k]), a->bias, a->limiter);
+                    if (rc) return from_solver(a, rc);
+                }
+            if ((rc = save_state(a, a->seg[q]))) return rc;
+            // reverse through the segment
+            for (int it = s1 - 1; it >= s0; --it)
+                for (int k = ns - 1; k >= 0; --k) {
+                    const int qi = (it - s0) * ns + k;
+                    rc = adjoint_sweep_dev(a, a->order, axes[k], a->dx, sweep_dt(a->dts[it], fac[k]), a->bias,
+                                           a->limiter, a->seg[qi], a->seg[qi + 1], cur, cur ^ 1);
+                    if (rc) return rc;
+                    cur ^= 1;
+                }
+        }
+        if ((rc = load_state(a, a->final_state))) return rc;
+        return lam_download(a, grad_in, cur);
+    });
+}
+
+}  // extern "C"

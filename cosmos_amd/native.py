@@ -62,6 +62,17 @@ SIGNATURES = {
     "cosmos_amr_solve_level1": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,
                                                 _c_int_p]),
+    "cosmos_adjoint_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_int, ctypes.c_double]),
+    "cosmos_adjoint_destroy": (ctypes.c_int, [_handle]),
+    "cosmos_adjoint_last_error": (ctypes.c_char_p, [_handle]),
+    "cosmos_adjoint_solver": (_handle, [_handle]),
+    "cosmos_adjoint_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, _c_double_p, _c_double_p]),
+    "cosmos_adjoint_forward": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_int, ctypes.c_int]),
+    "cosmos_adjoint_backward": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
+    "cosmos_adjoint_replay": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
                                             ctypes.c_int, _c_double_p]),
@@ -97,8 +108,11 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int, handle=None, amr: bool = False) -> None:
+def check(rc: int, handle=None, amr: bool = False, adjoint: bool = False) -> None:
     if rc != 0:
         lib = load()
-        msg = lib.cosmos_amr_last_error(handle) if amr else lib.cosmos_last_error(handle)
+        if adjoint:
+            msg = lib.cosmos_adjoint_last_error(handle)
+        else:
+            msg = lib.cosmos_amr_last_error(handle) if amr else lib.cosmos_last_error(handle)
         raise CosmosError(f"cosmos error {rc}: {msg.decode() if msg else ''}")

@@ -245,3 +245,84 @@ class Amr:
         self._check(self._lib.cosmos_amr_solve_level1(self._h, order, cell_spacing, cfl, final_time, tolerance,
                                                       max_coarse_steps, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
+
+
+class Adjoint:
+    """Reverse mode of the Newtonian update (cosmos_adjoint_*): a solver plus
+    the recorded trajectory (dt schedule and checkpoints) of a forward run."""
+
+    def __init__(self, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4):
+        self._lib = native.load()
+        self.dim = dim
+        self.nx, self.ny = nx, (ny if dim == 2 else 1)
+        h = ctypes.c_void_p()
+        native.check(self._lib.cosmos_adjoint_create(ctypes.byref(h), NEWTONIAN, dim, nx, self.ny, adiabatic_index),
+                     None, adjoint=True)
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.cosmos_adjoint_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def shape(self):
+        return (self.ny, self.nx, 5) if self.dim == 2 else (self.nx, 5)
+
+    def _check(self, rc: int) -> None:
+        native.check(rc, self._h, adjoint=True)
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != self.nx * self.ny * 5:
+            raise ValueError("array does not match the grid")
+        return a
+
+    def upload(self, prim: np.ndarray) -> None:
+        s = self._lib.cosmos_adjoint_solver(self._h)
+        a = self._arr(prim)
+        native.check(self._lib.cosmos_upload(s, _dp(a)), s)
+
+    def download(self) -> np.ndarray:
+        s = self._lib.cosmos_adjoint_solver(self._h)
+        out = np.empty(self.shape)
+        native.check(self._lib.cosmos_download(s, _dp(out)), s)
+        return out
+
+    def sweep(self, order: int, axis: int, cell_spacing: float, time_step: float, lam_out: np.ndarray,
+              bias: float = 0.0, slope_limiter: int = VANLEER) -> np.ndarray:
+        """Advance the state by one sweep; return (d out / d in)^T lam_out."""
+        lo = self._arr(lam_out)
+        li = np.empty(self.shape)
+        self._check(self._lib.cosmos_adjoint_sweep(self._h, order, axis, cell_spacing, time_step, bias,
+                                                   slope_limiter, _dp(lo), _dp(li)))
+        return li
+
+    def forward(self, order: int, cell_spacing: float, cfl: float, n_steps: int, bias: float = 0.0,
+                slope_limiter: int = VANLEER, checkpoint_every: int = 20) -> None:
+        self._check(self._lib.cosmos_adjoint_forward(self._h, order, cell_spacing, cfl, n_steps, bias,
+                                                     slope_limiter, checkpoint_every))
+
+    def backward(self, grad_out: np.ndarray) -> np.ndarray:
+        go = self._arr(grad_out)
+        gi = np.empty(self.shape)
+        self._check(self._lib.cosmos_adjoint_backward(self._h, _dp(go), _dp(gi)))
+        return gi
+
+    def replay(self, prim_in: np.ndarray) -> np.ndarray:
+        a = self._arr(prim_in)
+        out = np.empty(self.shape)
+        self._check(self._lib.cosmos_adjoint_replay(self._h, _dp(a), _dp(out)))
+        return out

@@ -213,6 +213,42 @@ int cosmos_amr_derefine(cosmos_amr* a, double cell_spacing, double tolerance);
 int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
 
+/* ------------------------------------------------------------------------
+ * Adjoint (reverse mode) of the Newtonian update (no counterpart in the
+ * reference).  A cosmos_adjoint context owns a solver handle for an nx-by-ny
+ * grid (dim 2) or an nx-cell line (dim 1).  It differentiates the update the
+ * reference computes (SLIC or FORCE sweeps, dimensionally split X(dt/2) Y(dt)
+ * X(dt/2) in 2D) with respect to the primitive state, with the time steps
+ * held at the values the forward pass chose (a frozen dt schedule) and the
+ * limiters differentiated on the branch the forward pass took (one-sided
+ * derivatives at their kinks).  Host arrays are AoS primitives, as for
+ * cosmos_upload.
+ * ------------------------------------------------------------------------ */
+typedef struct cosmos_adjoint cosmos_adjoint;
+
+int cosmos_adjoint_create(cosmos_adjoint** out, int physics, int dim, int nx, int ny, double adiabatic_index);
+int cosmos_adjoint_destroy(cosmos_adjoint* a);
+const char* cosmos_adjoint_last_error(const cosmos_adjoint* a);
+/* The context's solver: upload the initial state through it. */
+cosmos_solver* cosmos_adjoint_solver(cosmos_adjoint* a);
+/* One sweep and its adjoint: the solver's state is advanced by the sweep
+ * (as cosmos_sweep), and lambda_in = (d output / d input)^T lambda_out. */
+int cosmos_adjoint_sweep(cosmos_adjoint* a, int order, int axis, double cell_spacing, double time_step, double bias,
+                         int slope_limiter, const double* lambda_out, double* lambda_in);
+/* Advance the solver n_steps steps of the solve / solve2D loop (dt from the CFL
+ * condition with the 0.2x damping of the first six steps, no final time),
+ * recording every dt and a device checkpoint of the state every
+ * checkpoint_every steps. */
+int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, double cfl, int n_steps, double bias,
+                           int slope_limiter, int checkpoint_every);
+/* grad_in = (d W_N / d W_0)^T grad_out for the recorded trajectory: each
+ * segment between checkpoints is recomputed forward, then traversed in
+ * reverse one sweep at a time.  The solver's state is left at W_N. */
+int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* grad_in);
+/* The recorded dt schedule applied to another initial state (for finite-
+ * difference checks); the solver's state is left at the result. */
+int cosmos_adjoint_replay(cosmos_adjoint* a, const double* prim_in, double* prim_out);
+
 /* Batched point-wise limiter functions on host arrays (evaluated on the device):
  *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)
  *                         (Solvers/SlopeLimiters.cpp:3-69)
