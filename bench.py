@@ -253,6 +253,78 @@ def sr_leg(args):
                         f"{args.sr_steps} steps (seed {args.seed})", "bound": "fp64", **res}
 
 
+def adjoint_leg(args):
+    """Adjoint of the Newtonian SLIC update: the 4-quadrant workload at
+    n x n (CFL 0.9, van Leer), args.adj_steps steps with a checkpoint every
+    20.  forward = the device solve loop (exact arithmetic) over the same
+    steps; adjoint = cosmos_adjoint_backward (segment recompute + reverse
+    sweeps).  Gradient agreement: L = sum (rho_N - rho*)^2 on a 256^2 grid over
+    20 steps, central differences along 32 seeded directions with the dt
+    schedule held fixed."""
+    from cosmos_amd import NEWTONIAN, VANLEER, Adjoint, Solver, workloads
+
+    n, steps = args.adj_size, args.adj_steps
+    W = workloads.quadrants_2d(n, seed=args.seed)
+    ev = HipEvents()
+    with Solver(NEWTONIAN, 2, n, n, 1.4) as s:
+        s.upload(W)
+        s.begin(2, 1.0 / n, 0.9, math.inf, steps, 0.0, VANLEER)
+        s.synchronize()
+        e0, e1 = ev.new(), ev.new()
+        ev.record(e0, s.stream)
+        s.advance(steps)
+        ev.record(e1, s.stream)
+        fwd_ms = ev.elapsed_ms(e0, e1)
+    with Adjoint(2, n, n, 1.4) as a:
+        a.upload(W)
+        t0 = time.perf_counter()
+        a.forward(2, 1.0 / n, 0.9, steps, 0.0, VANLEER, 20)
+        rec_s = time.perf_counter() - t0
+        WN = a.download()
+        target = workloads.quadrants_2d(n, seed=args.seed + 1)[..., 0]
+        g = np.zeros_like(WN)
+        g[..., 0] = 2.0 * (WN[..., 0] - target)
+        t0 = time.perf_counter()
+        a.backward(g)
+        bwd_s = time.perf_counter() - t0
+    # gradient agreement on a smaller problem
+    m, ms = 256, 20
+    rng = np.random.default_rng(args.seed + 7)
+    W0 = workloads.quadrants_2d(m, seed=args.seed)
+    W0[..., 0] *= 1.0 + 0.05 * rng.standard_normal((m, m))
+    tgt = workloads.quadrants_2d(m, seed=args.seed + 1)[..., 0]
+    errs = {1e-6: [], 1e-7: []}
+    with Adjoint(2, m, m, 1.4) as a:
+        a.upload(W0)
+        a.forward(2, 1.0 / m, 0.9, ms, 0.0, VANLEER, 5)
+        WN = a.download()
+        g = np.zeros_like(WN)
+        g[..., 0] = 2.0 * (WN[..., 0] - tgt)
+        grad = a.backward(g)
+
+        def loss(X):
+            r = a.replay(X)[..., 0] - tgt
+            return float(np.sum(r * r))
+
+        for _ in range(32):
+            e = rng.standard_normal(W0.shape)
+            exact = float(np.sum(grad * e))
+            for h in errs:
+                fd = (loss(W0 + h * e) - loss(W0 - h * e)) / (2 * h)
+                errs[h].append(abs(fd - exact) / max(abs(exact), 1e-300))
+    return {
+        "workload": f"2D Newtonian SLIC, van Leer, {n}x{n} FP64, CFL 0.9, 4-quadrant (seed {args.seed}), "
+                    f"{steps} steps, checkpoint every 20, dt schedule frozen",
+        "forward_ms": fwd_ms,
+        "forward_recording_s": rec_s,
+        "adjoint_s": bwd_s,
+        "adjoint_forward_ratio": bwd_s * 1e3 / fwd_ms,
+        "fd_check": {f"h={h:g}": {"directions": len(v), "median_rel_err": float(np.median(v)),
+                                  "max_rel_err": float(np.max(v)), "within_1e-6": int(np.sum(np.array(v) < 1e-6))}
+                     for h, v in errs.items()},
+    }
+
+
 def cpu_leg(args):
     """The reference's own sources (serial build, oracle/_ref) on a bounded
     sample of the same workload; the C restatement (oracle/liboracle.so,
@@ -330,7 +402,9 @@ def main():
     ap.add_argument("--gr-steps", type=int, default=20)
     ap.add_argument("--sr-size", type=int, default=2048)
     ap.add_argument("--sr-steps", type=int, default=200)
-    ap.add_argument("--no-extra", action="store_true", help="skip the SR and black-hole legs")
+    ap.add_argument("--adj-size", type=int, default=1024)
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--no-extra", action="store_true", help="skip the SR, black-hole and adjoint legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -384,6 +458,7 @@ def main():
     if not args.no_extra:
         line["relativistic"] = sr_leg(args)
         line["blackhole"] = gr_leg(args)
+        line["adjoint"] = adjoint_leg(args)
     if not args.no_cpu:
         line.update(cpu_leg(args))
     print(json.dumps(line), flush=True)
