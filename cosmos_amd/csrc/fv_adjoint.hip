@@ -264,7 +264,6 @@ struct cosmos_adjoint {
     int every = 20;               // checkpoint interval
     std::vector<double*> checkpoints;  // state at steps 0, every, 2 every, ...
     std::vector<double*> seg;     // per-sweep states of one segment during the reverse pass
-    double* final_state = nullptr;  // the state after the recorded steps (restored after the reverse pass)
     std::string err;
 };
 
@@ -323,14 +322,6 @@ int save_state(cosmos_adjoint* a, double* dst) {
     return COSMOS_OK;
 }
 
-int load_state(cosmos_adjoint* a, const double* src) {
-    double* cur;
-    int rc = current_base(a, cur);
-    if (rc) return rc;
-    ADJCHK(a, hipMemcpyAsync(cur, src, sizeof(double) * 5 * a->plane, hipMemcpyDeviceToDevice, stream_of(a)));
-    return COSMOS_OK;
-}
-
 cfv::adj::Coef coef(const cosmos_adjoint* a, double dx, double dt, double bias, int limiter) {
     cfv::adj::Coef C{};
     C.k = dt / dx;
@@ -379,6 +370,29 @@ int lam_download(cosmos_adjoint* a, double* host, int which) {
     ADJCHK(a, hipGetLastError());
     ADJCHK(a, hipMemcpyAsync(host, a->stage, sizeof(double) * 5 * cells, hipMemcpyDeviceToHost, st));
     ADJCHK(a, hipStreamSynchronize(st));
+    return COSMOS_OK;
+}
+
+// One exact forward sweep from the planes at `in` to the planes at `out`
+// (the kernels and coefficients of cosmos_sweep, without its ping-pong).
+int forward_sweep_dev(cosmos_adjoint* a, int axis, double dt, const double* in, double* out) {
+    cfv::SweepLaunch L{};
+    L.in = cplanes_at(a, in);
+    L.out = planes_at(a, out);
+    L.nx = a->nx;
+    L.ny = a->ny;
+    L.pitch = a->pitch;
+    L.bias = a->bias;
+    L.limiter = a->limiter;
+    L.order = a->order;
+    L.physics = COSMOS_NEWTONIAN;
+    L.dim = a->dim;
+    L.P = cfv::make_params(a->gamma);
+    L.coef.dt_dx = dt / a->dx;
+    L.coef.dx_dt = a->dx / dt;
+    L.coef.hdt_dx = 0.5 * (dt / a->dx);
+    hipError_t e = axis == 0 ? cfv::launch_sweep_x(L, stream_of(a)) : cfv::launch_sweep_y(L, stream_of(a));
+    ADJCHK(a, e);
     return COSMOS_OK;
 }
 
@@ -469,7 +483,6 @@ int cosmos_adjoint_destroy(cosmos_adjoint* a) {
     hipFree(a->stage);
     free_list(a->checkpoints);
     free_list(a->seg);
-    hipFree(a->final_state);
     cosmos_destroy(a->s);
     delete a;
     return COSMOS_OK;
@@ -509,6 +522,9 @@ int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, do
         a->dts.clear();
         free_list(a->checkpoints);
         free_list(a->seg);
+        // the reverse pass recomputes segments with the exact kernels
+        int rc0 = cosmos_set_mode(a->s, COSMOS_MODE_EXACT);
+        if (rc0) return from_solver(a, rc0);
         int axes[3];
         double fac[3];
         const int ns = step_sweeps(a, axes, fac);
@@ -529,12 +545,6 @@ int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, do
                 if (rc) return from_solver(a, rc);
             }
         }
-        if (!a->final_state) {
-            int rc = alloc_state(a, a->final_state);
-            if (rc) return rc;
-        }
-        int rc = save_state(a, a->final_state);
-        if (rc) return rc;
         ADJCHK(a, hipStreamSynchronize(stream_of(a)));
         return COSMOS_OK;
     });
@@ -566,8 +576,9 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
         double fac[3];
         const int ns = step_sweeps(a, axes, fac);
         const int nsteps = (int)a->dts.size();
-        // segment buffers: the input of every sweep of a segment plus its final output
-        const size_t need = (size_t)a->every * ns + 1;
+        // segment buffers: the output of every sweep of a segment (its first
+        // input is the checkpoint itself)
+        const size_t need = (size_t)a->every * ns;
         while (a->seg.size() < need) {
             double* p = nullptr;
             int rc = alloc_state(a, p);
@@ -579,27 +590,23 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
         int cur = 0;
         for (int sgi = (int)a->checkpoints.size() - 1; sgi >= 0; --sgi) {
             const int s0 = sgi * a->every, s1 = std::min(nsteps, s0 + a->every);
-            // recompute the segment from its checkpoint, keeping every sweep's input
-            if ((rc = load_state(a, a->checkpoints[sgi]))) return rc;
+            // recompute the segment from its checkpoint: state q + 1 = sweep q of state q
+            auto state = [&](int q) -> double* { return q == 0 ? a->checkpoints[sgi] : a->seg[q - 1]; };
             int q = 0;
             for (int it = s0; it < s1; ++it)
-                for (int k = 0; k < ns; ++k) {
-                    if ((rc = save_state(a, a->seg[q++]))) return rc;
-                    rc = cosmos_sweep(a->s, a->order, axes[k], a->dx, sweep_dt(a->dts[it], fac[k]), a->bias, a->limiter);
-                    if (rc) return from_solver(a, rc);
-                }
-            if ((rc = save_state(a, a->seg[q]))) return rc;
+                for (int k = 0; k < ns; ++k, ++q)
+                    if ((rc = forward_sweep_dev(a, axes[k], sweep_dt(a->dts[it], fac[k]), state(q), state(q + 1))))
+                        return rc;
             // reverse through the segment
             for (int it = s1 - 1; it >= s0; --it)
                 for (int k = ns - 1; k >= 0; --k) {
                     const int qi = (it - s0) * ns + k;
                     rc = adjoint_sweep_dev(a, a->order, axes[k], a->dx, sweep_dt(a->dts[it], fac[k]), a->bias,
-                                           a->limiter, a->seg[qi], a->seg[qi + 1], cur, cur ^ 1);
+                                           a->limiter, state(qi), state(qi + 1), cur, cur ^ 1);
                     if (rc) return rc;
                     cur ^= 1;
                 }
         }
-        if ((rc = load_state(a, a->final_state))) return rc;
         return lam_download(a, grad_in, cur);
     });
 }

@@ -238,12 +238,14 @@ int cosmos_adjoint_sweep(cosmos_adjoint* a, int order, int axis, double cell_spa
 /* Advance the solver n_steps steps of the solve / solve2D loop (dt from the CFL
  * condition with the 0.2x damping of the first six steps, no final time),
  * recording every dt and a device checkpoint of the state every
- * checkpoint_every steps. */
+ * checkpoint_every steps.  The solver is set to COSMOS_MODE_EXACT: the
+ * reverse pass recomputes segments with the exact kernels. */
 int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, double cfl, int n_steps, double bias,
                            int slope_limiter, int checkpoint_every);
 /* grad_in = (d W_N / d W_0)^T grad_out for the recorded trajectory: each
  * segment between checkpoints is recomputed forward, then traversed in
- * reverse one sweep at a time.  The solver's state is left at W_N. */
+ * reverse one sweep at a time, in buffers of the context: the solver's
+ * state is not touched and stays at W_N. */
 int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* grad_in);
 /* The recorded dt schedule applied to another initial state (for finite-
  * difference checks); the solver's state is left at the result. */
