@@ -4,7 +4,8 @@ The compute path is the HIP library libcosmos_fv.so (C-ABI: include/cosmos_fv.h)
 this package is its Python binding, the build script and the seeded workloads.
 """
 from .native import CosmosError, LIB_PATH, load  # noqa: F401
+from .batch import Batch  # noqa: F401
 from .solver import BLACKHOLE, EXACT, FAST, MINBEE, NEWTONIAN, RELATIVISTIC, SUPERBEE, VANLEER, Adjoint, Amr, Solver  # noqa: F401
 
-__all__ = ["Solver", "Amr", "Adjoint", "CosmosError", "BLACKHOLE", "load", "LIB_PATH", "NEWTONIAN", "RELATIVISTIC", "SUPERBEE", "VANLEER",
+__all__ = ["Solver", "Amr", "Adjoint", "Batch", "CosmosError", "BLACKHOLE", "load", "LIB_PATH", "NEWTONIAN", "RELATIVISTIC", "SUPERBEE", "VANLEER",
            "MINBEE"]

@@ -28,25 +28,15 @@
 // only), the order, the limiter mode (1: van Leer at bias 0 with one division
 // per component, 2: the fast-mode van Leer, 0: any limiter and bias) and
 // whether the output's max wave speed is merged.
-#include "fv_kernels.hpp"
+#include "fv_sweep_core.hpp"
 
 namespace cfv {
 namespace {
 
+using namespace core;
+
 constexpr int kXThreads = 256;
 constexpr int kYThreads = 64;
-
-__device__ __forceinline__ int mirror2(int c, int n) {
-    if (c < 0) c = -c - 1;
-    if (c >= n) c = 2 * n - 1 - c;
-    c = c < 0 ? 0 : c;
-    return c >= n ? n - 1 : c;
-}
-
-__device__ __forceinline__ int clamp1(int c, int n) {
-    c = c < 0 ? 0 : c;
-    return c >= n ? n - 1 : c;
-}
 
 // The hardware hands consecutive workgroups to the 8 XCDs round robin, and
 // each XCD has its own L2.  Renumber so that each XCD works through one
@@ -59,101 +49,6 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     const int xcd = b % kXcds, k = b / kXcds;
     const int base = n / kXcds, extra = n % kXcds;
     return xcd * base + (xcd < extra ? xcd : extra) + k;
-}
-
-template <bool W0>
-__device__ __forceinline__ void load_cell(const ConstPlanes& in, long idx, double* W) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k) W[k] = (W0 && k == 3) ? 0.0 : in.p[k][idx];
-}
-
-template <bool W0>
-__device__ __forceinline__ void store_cell(const Planes& out, long idx, const double* W) {
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        if (!(W0 && k == 3)) out.p[k][idx] = W[k];
-}
-
-// Half-step evolution of one cell from its conserved neighbours
-// (EulerSolvers::evolveStateByHalf{X,Y}TimeStep, Solvers/EulerSolvers.cpp:191-223),
-// then the p2c and flux of both evolved face states as the FORCE flux of
-// the neighbouring interfaces will need them (Solvers/EulerFirstOrderSolver.cpp:52-59).
-template <class Phys, int AXIS, int LIMMODE, bool W0>
-__device__ __forceinline__ void half_step(const double* ul, const double* um, const double* ur, const SweepCoef& cf,
-                                          double bias, int limiter, const PhysParams& P, double* Q0, double* G0,
-                                          double* Q1, double* G1) {
-    double xl[5], xr[5], fl[5], fr[5], a[5], b[5], p0[5], p1[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        if (W0 && k == 3) {
-            xl[k] = xr[k] = 0.0;
-            continue;
-        }
-        double s = LIMMODE == 2 ? slope_component_fast_vl(ul[k], um[k], ur[k])
-                                : slope_component<LIMMODE>(ul[k], um[k], ur[k], bias, limiter);
-        xl[k] = um[k] - (0.5 * s);
-        xr[k] = um[k] + (0.5 * s);
-    }
-    Phys::template flux<AXIS>(xl, fl, Phys::inv_density(xl), P);
-    Phys::template flux<AXIS>(xr, fr, Phys::inv_density(xr), P);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        double ev = (W0 && k == 3) ? 0.0 : cf.hdt_dx * (fl[k] - fr[k]);
-        a[k] = (W0 && k == 3) ? 0.0 : xl[k] + ev;
-        b[k] = (W0 && k == 3) ? 0.0 : xr[k] + ev;
-    }
-    // P0 = c2p(a) (side 0), P1 = c2p(b) (side 1); p2c(P) has density P[0] = a[0] (resp.
-    // b[0]), so the flux of p2c(P) divides by the same density.
-    double ia = Phys::inv_density(a), ib = Phys::inv_density(b);
-    if constexpr (Phys::kFast) {
-        // fast mode: the face state's conserved vector is a (resp. b) itself
-        Phys::template c2p_flux<AXIS>(a, p0, G0, ia, P);
-        Phys::template c2p_flux<AXIS>(b, p1, G1, ib, P);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            Q0[k] = a[k];
-            Q1[k] = b[k];
-        }
-    } else {
-        Phys::c2p(a, p0, ia, P);
-        Phys::c2p(b, p1, ib, P);
-        Phys::p2c(p0, Q0, P);
-        Phys::p2c(p1, Q1, P);
-        Phys::template flux<AXIS>(Q0, G0, ia, P);
-        Phys::template flux<AXIS>(Q1, G1, ib, P);
-    }
-}
-
-// FORCE flux of one interface from p2c and flux of its two states
-// (computeLaxFriedrichsFlux, computeRichtmyerFlux, computeFORCEFlux:
-// Solvers/EulerFirstOrderSolver.cpp:3-24).
-template <class Phys, int AXIS, bool W0>
-__device__ __forceinline__ void force_flux(const double* UL, const double* FL, const double* UR, const double* FR,
-                                           const SweepCoef& cf, const PhysParams& P, double* F) {
-    double us[5], fs[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        us[k] = (W0 && k == 3) ? 0.0 : 0.5 * ((UL[k] + UR[k]) + (cf.dt_dx * (FL[k] - FR[k])));
-    Phys::template flux<AXIS>(us, fs, Phys::inv_density(us), P);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        if (W0 && k == 3) {
-            F[k] = 0.0;
-            continue;
-        }
-        double lf = 0.5 * ((FL[k] + FR[k]) + (cf.dx_dt * (UL[k] - UR[k])));
-        F[k] = 0.5 * (lf + fs[k]);
-    }
-}
-
-// computeFORCEUpdate (Solvers/EulerFirstOrderSolver.cpp:96-101) then c2p.
-template <class Phys, bool W0>
-__device__ __forceinline__ void update_cell(const double* U, const double* Fl, const double* Fr, const SweepCoef& cf,
-                                            const PhysParams& P, double* W) {
-    double un[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) un[k] = (W0 && k == 3) ? 0.0 : U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
-    Phys::c2p(un, W, Phys::inv_density(un), P);
 }
 
 // Block-wide max of non-negative wave speeds stored to part[blockIdx.x] (no
@@ -212,12 +107,7 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TW = NT - 2 * G;
     // One LDS arena of 15 rows (30 KB: five blocks per CU).
-    //   SLIC:  sA = rows 0-4   stage A U, then the interface fluxes
-    //          sB = rows 5-14  stage B Q0, G0
-    //   FORCE: sA = rows 0-9   U and F(U);  sB = rows 10-14 the interface fluxes
     __shared__ double lds[15][NT];
-    double(*sA)[NT] = lds;
-    double(*sB)[NT] = lds + (ORDER == 2 ? 5 : 10);
 
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
@@ -227,87 +117,14 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const int blk = xcd_tile(blockIdx.x, gridDim.x);
     const int row = blk / tiles;
     const int tile = blk - row * tiles;
-    const int s = threadIdx.x;
-    const int c = tile * TW - G + s;
-    const int cc = ORDER == 2 ? mirror2(c, nx) : clamp1(c, nx);
-    const long rowbase = (long)row * L.pitch;
-
-    double W[5], U[5], F[5];
-    load_cell<W0>(L.in, rowbase + cc, W);
-    Phys::p2c(W, U, P);
-
-    const int sl = s > 0 ? s - 1 : 0;
-    const int sr = s < NT - 1 ? s + 1 : NT - 1;
-    if constexpr (ORDER == 2) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            if (!(W0 && k == 3)) sA[k][s] = U[k];
-        __syncthreads();
-        double ul[5], ur[5], Q0[5], G0[5], Q1[5], G1[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            ul[k] = (W0 && k == 3) ? 0.0 : sA[k][sl];
-            ur[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
-        }
-        half_step<Phys, 0, LIMMODE, W0>(ul, U, ur, cf, L.bias, L.limiter, P, Q0, G0, Q1, G1);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            if (W0 && k == 3) continue;
-            sB[k][s] = Q0[k];
-            sB[5 + k][s] = G0[k];
-        }
-        __syncthreads();
-        double UR[5], FR[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            UR[k] = (W0 && k == 3) ? 0.0 : sB[k][sr];
-            FR[k] = (W0 && k == 3) ? 0.0 : sB[5 + k][sr];
-        }
-        force_flux<Phys, 0, W0>(Q1, G1, UR, FR, cf, P, F);
-        // interface (s, s+1) -> sA (stage A values are no longer read)
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            if (!(W0 && k == 3)) sA[k][s] = F[k];
-        __syncthreads();
-    } else {
-        double FU[5];
-        Phys::template flux<0>(U, FU, Phys::inv_density(U), P);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            if (W0 && k == 3) continue;
-            sA[k][s] = U[k];
-            sA[5 + k][s] = FU[k];
-        }
-        __syncthreads();
-        double UR[5], FR[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            UR[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
-            FR[k] = (W0 && k == 3) ? 0.0 : sA[5 + k][sr];
-        }
-        force_flux<Phys, 0, W0>(U, FU, UR, FR, cf, P, F);
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            if (!(W0 && k == 3)) sB[k][s] = F[k];
-        __syncthreads();
-    }
-    const bool valid = (s >= G) && (s < NT - G) && (c < nx);
-    double Fl[5], Wn[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : (ORDER == 2 ? sA[k][sl] : sB[k][sl]);
-    update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
-    if (valid) store_cell<W0>(L.out, rowbase + c, Wn);
+    const double m = x_segment<Phys, ORDER, LIMMODE, DIM, WS, W0, NT>(L.in, L.out, nx, (long)row * L.pitch, tile, cf,
+                                                                     L.bias, L.limiter, P, lds);
     if constexpr (WS) {
-        double m = 0.0;
-        if (valid) {
-            double w = Phys::template wavespeed<DIM>(Wn, P);
-            if (w > m) m = w;
-        }
         if (L.step->ws_slot_atomics) {
             wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
         } else {
             // rows no longer read in stage D: sB for SLIC, sA for FORCE
-            block_max_store<NT>(m, L.step->ws_part, ORDER == 2 ? &sB[0][0] : &sA[0][0]);
+            block_max_store<NT>(m, L.step->ws_part, &lds[ORDER == 2 ? 5 : 0][0]);
         }
     }
 }
@@ -339,80 +156,8 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min
     const int jj = colok ? j : nx - 1;
     const int i0 = by * ROWS;
     const int i1 = min(i0 + ROWS, ny);
-    const long pitch = L.pitch;
-    double wsmax = 0.0;
-
-    if constexpr (ORDER == 2) {
-        double Ua[5], Ub[5], Q1p[5], G1p[5], Fp[5], Wn[5];
-        load_cell<W0>(L.in, (long)mirror2(i0 - 2, ny) * pitch + jj, Wn);
-        for (int r = i0 - 2; r <= i1 + 1; ++r) {
-            double W[5], U[5];
-#pragma unroll
-            for (int k = 0; k < 5; ++k) W[k] = Wn[k];
-            if (r < i1 + 1) load_cell<W0>(L.in, (long)mirror2(r + 1, ny) * pitch + jj, Wn);
-            Phys::p2c(W, U, P);
-            if (r >= i0) {
-                double Q0[5], G0[5], Q1[5], G1[5];
-                half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cf, L.bias, L.limiter, P, Q0, G0, Q1, G1);
-                if (r >= i0 + 1) {
-                    double F[5];
-                    force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cf, P, F);
-                    if (r >= i0 + 2) {
-                        double Wo[5];
-                        update_cell<Phys, W0>(Ua, Fp, F, cf, P, Wo);
-                        if (colok) store_cell<W0>(L.out, (long)(r - 2) * pitch + j, Wo);
-                        if constexpr (WS) {
-                            double w = Phys::template wavespeed<2>(Wo, P);
-                            if (colok && w > wsmax) wsmax = w;
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < 5; ++k) Fp[k] = F[k];
-                }
-#pragma unroll
-                for (int k = 0; k < 5; ++k) {
-                    Q1p[k] = Q1[k];
-                    G1p[k] = G1[k];
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                Ua[k] = Ub[k];
-                Ub[k] = U[k];
-            }
-        }
-    } else {
-        double Up[5], FUp[5], Fp[5], Wn[5];
-        load_cell<W0>(L.in, (long)clamp1(i0 - 1, ny) * pitch + jj, Wn);
-        for (int r = i0 - 1; r <= i1; ++r) {
-            double W[5], U[5], FU[5];
-#pragma unroll
-            for (int k = 0; k < 5; ++k) W[k] = Wn[k];
-            if (r < i1) load_cell<W0>(L.in, (long)clamp1(r + 1, ny) * pitch + jj, Wn);
-            Phys::p2c(W, U, P);
-            Phys::template flux<1>(U, FU, Phys::inv_density(U), P);
-            if (r >= i0) {
-                double F[5];
-                force_flux<Phys, 1, W0>(Up, FUp, U, FU, cf, P, F);
-                if (r >= i0 + 1) {
-                    double Wo[5];
-                    update_cell<Phys, W0>(Up, Fp, F, cf, P, Wo);
-                    if (colok) store_cell<W0>(L.out, (long)(r - 1) * pitch + j, Wo);
-                    if constexpr (WS) {
-                        double w = Phys::template wavespeed<2>(Wo, P);
-                        if (colok && w > wsmax) wsmax = w;
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < 5; ++k) Fp[k] = F[k];
-            }
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                Up[k] = U[k];
-                FUp[k] = FU[k];
-            }
-        }
-    }
+    const double wsmax = y_march<Phys, ORDER, LIMMODE, W0>(L.in, L.out, ny, L.pitch, j, colok, jj, i0, i1, WS, cf,
+                                                          L.bias, L.limiter, P);
     if constexpr (WS) {
         if (L.step->ws_slot_atomics) wave_max_to_slot(wsmax, L.step->ws_slots, blockIdx.x);
         else block_max_store<kYThreads>(wsmax, L.step->ws_part, nullptr);

@@ -9,12 +9,27 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcosmos_fv.so")
+LIB_PATH = os.environ.get("COSMOS_FV_LIB") or os.path.join(HERE, "libcosmos_fv.so")
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_u8_p = ctypes.POINTER(ctypes.c_ubyte)
 _handle = ctypes.c_void_p
+
+
+
+class BatchProblem(ctypes.Structure):
+    """cosmos_batch_problem"""
+    _fields_ = [("physics", ctypes.c_int), ("dim", ctypes.c_int), ("nx", ctypes.c_int), ("ny", ctypes.c_int),
+                ("cell_spacing", ctypes.c_double), ("cfl", ctypes.c_double), ("final_time", ctypes.c_double),
+                ("max_steps", ctypes.c_int), ("adiabatic_index", ctypes.c_double)]
+
+
+class BatchSchedule(ctypes.Structure):
+    """cosmos_batch_schedule"""
+    _fields_ = [("whole_max_cells", ctypes.c_int), ("x_segments_per_task", ctypes.c_int), ("y_rows", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int)]
+
 
 # name -> (restype, argtypes); mirrors include/cosmos_fv.h.
 SIGNATURES = {
@@ -47,6 +62,20 @@ SIGNATURES = {
     "cosmos_solve_forced": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_double_p,
                                             _c_int_p]),
+    "cosmos_batch_default_schedule": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(BatchSchedule)]),
+    "cosmos_batch_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.POINTER(BatchProblem), ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "cosmos_batch_destroy": (ctypes.c_int, [_handle]),
+    "cosmos_batch_last_error": (ctypes.c_char_p, [_handle]),
+    "cosmos_batch_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
+    "cosmos_batch_set_schedule": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(BatchSchedule)]),
+    "cosmos_batch_get_schedule": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.POINTER(BatchSchedule)]),
+    "cosmos_batch_upload": (ctypes.c_int, [_handle, ctypes.c_int, _c_double_p]),
+    "cosmos_batch_download": (ctypes.c_int, [_handle, ctypes.c_int, _c_double_p]),
+    "cosmos_batch_run": (ctypes.c_int, [_handle]),
+    "cosmos_batch_run_timed": (ctypes.c_int, [_handle, _c_double_p]),
+    "cosmos_batch_stats": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, _c_double_p]),
+    "cosmos_batch_autotune": (ctypes.c_int, [_handle, ctypes.c_int, _c_double_p]),
     "cosmos_amr_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_double]),
     "cosmos_amr_destroy": (ctypes.c_int, [_handle]),
@@ -108,10 +137,12 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def check(rc: int, handle=None, amr: bool = False, adjoint: bool = False) -> None:
+def check(rc: int, handle=None, amr: bool = False, adjoint: bool = False, batch: bool = False) -> None:
     if rc != 0:
         lib = load()
-        if adjoint:
+        if batch:
+            msg = lib.cosmos_batch_last_error(handle)
+        elif adjoint:
             msg = lib.cosmos_adjoint_last_error(handle)
         else:
             msg = lib.cosmos_amr_last_error(handle) if amr else lib.cosmos_last_error(handle)

@@ -191,3 +191,40 @@ def gr_accretion_2d(n: int, spin: bool = False):
     W[..., 4] = 0.1
     hole = (0.1, 0.99, 0.5, 0.5, 0.0) if spin else (0.05, 0.0, 0.5, 0.5, 0.0)
     return W, 1.0 / n, hole
+
+
+def riemann_batch(count: int = 1024, seed: int = 0, steps: int = 200, sr_fraction: float = 0.3,
+                  two_d_fraction: float = 0.5, size_1d=(64, 4096), side_2d=(32, 512)):
+    """The batched workload: `count` independent Riemann problems.  Per problem,
+    in order from one SplitMix64 stream: physics (special relativistic with
+    probability sr_fraction, else Newtonian), dimension (2D with probability
+    two_d_fraction), size log-uniform in size_1d cells (1D) or side_2d per side
+    (2D, square), then the left and right states (rho, p ~ U[0.1, 10], normal
+    velocity ~ U[-0.8, 0.8]).  Cells with x_j = j dx < 0.5 (j < nx / 2) take the
+    left state; in 2D the interface is the line x = 0.5 and the normal
+    velocity is u (v = w = 0).  Newtonian: gamma 1.4, CFL 0.9; special
+    relativistic: gamma 5/3, CFL 0.5; dx = 1 / nx; `steps` steps each, no
+    final time.  Returns (problems, states) for Batch / Solver."""
+    rng = SplitMix64(seed)
+    problems, states = [], []
+    for _ in range(count):
+        physics = 1 if rng.uniform(0.0, 1.0) < sr_fraction else 0
+        dim = 2 if rng.uniform(0.0, 1.0) < two_d_fraction else 1
+        lo, hi = size_1d if dim == 1 else side_2d
+        n = int(round(math.exp(math.log(lo) + rng.uniform(0.0, 1.0) * (math.log(hi) - math.log(lo)))))
+        n = min(max(n, lo), hi)
+        left = [rng.uniform(0.1, 10.0), rng.uniform(-0.8, 0.8), rng.uniform(0.1, 10.0)]
+        right = [rng.uniform(0.1, 10.0), rng.uniform(-0.8, 0.8), rng.uniform(0.1, 10.0)]
+        shape = (n, n, 5) if dim == 2 else (n, 5)
+        W = np.empty(shape)
+        lmask = (np.arange(n) < n // 2)
+        for mask, (rho, u, p) in ((lmask, left), (~lmask, right)):
+            if dim == 2:
+                W[:, mask] = (rho, u, 0.0, 0.0, p)
+            else:
+                W[mask] = (rho, u, 0.0, 0.0, p)
+        problems.append({"physics": physics, "dim": dim, "nx": n, "ny": n if dim == 2 else 1,
+                         "cell_spacing": 1.0 / n, "cfl": 0.5 if physics else 0.9, "final_time": math.inf,
+                         "max_steps": steps, "adiabatic_index": 5.0 / 3.0 if physics else 1.4})
+        states.append(W)
+    return problems, states

@@ -281,6 +281,71 @@ int cosmos_device_copy(void* dst, const void* src, long bytes, void* stream);
  * row pitch in elements; valid until the next sweep, step or destroy. */
 int cosmos_state_planes(cosmos_solver* s, double** planes5, long* pitch);
 
+
+/* ------------------------------------------------------------------------
+ * Batched persistent form (no counterpart in the reference: the reference
+ * solves one problem per call).  Many independent Newtonian and
+ * special-relativistic problems, 1D and 2D, of uneven sizes, each advanced
+ * by the solve / solve2D loop (dt from the CFL condition, the 0.2x damping
+ * of iterations 0..5, the final-time clamp) until max_steps steps or
+ * final_time, all in one launch of a captured hipGraph: set-up kernels, then
+ * one persistent kernel per physics that takes tasks from a ticket queue.
+ * Each problem's result is bitwise the one cosmos_solve gives for it (same
+ * order, limiter and bias for the whole batch).  Host arrays are AoS
+ * primitives, as for cosmos_upload.
+ * ------------------------------------------------------------------------ */
+typedef struct cosmos_batch cosmos_batch;
+
+typedef struct {
+    int physics;             /* COSMOS_NEWTONIAN or COSMOS_RELATIVISTIC */
+    int dim;                 /* 1 or 2 */
+    int nx, ny;              /* ny is ignored in 1D */
+    double cell_spacing;     /* dx (both axes in 2D, as solve2D) */
+    double cfl;
+    double final_time;       /* INFINITY: run exactly max_steps steps */
+    int max_steps;           /* 1 .. 1000000 */
+    double adiabatic_index;
+} cosmos_batch_problem;
+
+/* How a physics' problems are cut into tasks (the autotuner's knobs):
+ * problems of at most whole_max_cells cells run start to finish in one
+ * block; larger ones run phase by phase as tasks of x_segments_per_task
+ * x segments (256-thread row pieces, short rows packed side by side) or of
+ * 256 y marches of y_rows rows.  blocks_per_cu: persistent blocks per
+ * compute unit (0: the kernel's occupancy). */
+typedef struct {
+    int whole_max_cells;
+    int x_segments_per_task;
+    int y_rows;
+    int blocks_per_cu;
+} cosmos_batch_schedule;
+
+int cosmos_batch_default_schedule(int physics, cosmos_batch_schedule* out);
+int cosmos_batch_create(cosmos_batch** out, const cosmos_batch_problem* problems, int n, int order, double bias,
+                        int slope_limiter);
+int cosmos_batch_destroy(cosmos_batch* b);
+const char* cosmos_batch_last_error(const cosmos_batch* b);
+/* Special-relativistic con2prim stopping tolerance and iteration cap for the
+ * batch (default: the reference's 1e-8 and 1000). */
+int cosmos_batch_set_con2prim(cosmos_batch* b, double tolerance, int max_iterations);
+int cosmos_batch_set_schedule(cosmos_batch* b, int physics, const cosmos_batch_schedule* s);
+int cosmos_batch_get_schedule(cosmos_batch* b, int physics, cosmos_batch_schedule* s);
+int cosmos_batch_upload(cosmos_batch* b, int problem, const double* prim);
+int cosmos_batch_download(cosmos_batch* b, int problem, double* prim);
+/* Advance every problem from its current state (t = 0, iteration 0) to its
+ * stop, then wait.  cosmos_batch_run_timed also returns the graph's
+ * duration from HIP events around its launch. */
+int cosmos_batch_run(cosmos_batch* b);
+int cosmos_batch_run_timed(cosmos_batch* b, double* ms);
+/* Per problem after a run (each array n entries, any may be NULL): the time
+ * reached, the steps taken and the completion time in microseconds after
+ * the run's first kernel started (device wall clock). */
+int cosmos_batch_stats(cosmos_batch* b, double* time_reached, int* steps_taken, double* finish_us);
+/* Coordinate descent over each physics' schedule (whole_max_cells,
+ * x_segments_per_task, y_rows) on timed runs capped at trial_steps steps;
+ * keeps the fastest.  The problems' states are restored afterwards. */
+int cosmos_batch_autotune(cosmos_batch* b, int trial_steps, double* best_ms);
+
 #ifdef __cplusplus
 }
 #endif
