@@ -253,6 +253,75 @@ def sr_leg(args):
                         f"{args.sr_steps} steps (seed {args.seed})", "bound": "fp64", **res}
 
 
+def batch_leg(args):
+    """The batched form: args.batch_count independent Riemann problems
+    (70 % Newtonian / 30 % special relativistic, half 1D in [64, 4096] cells,
+    half 2D in [32, 512]^2, log-uniform sizes; SLIC + van Leer; CFL 0.9 / 0.5;
+    args.batch_steps steps each) advanced in one hipGraph launch after the
+    schedule autotuner's trials, against the same problems solved one by one
+    through the per-problem API (cosmos_solve per problem, in turn) and with
+    every problem's solve loop queued on its own stream before one wait.
+    Completion times per problem come from the device wall clock."""
+    from cosmos_amd import Solver, VANLEER, workloads
+    from cosmos_amd.batch import Batch, cell_updates
+
+    problems, states = workloads.riemann_batch(args.batch_count, seed=args.seed, steps=args.batch_steps)
+    total = cell_updates(problems, [p["max_steps"] for p in problems])
+    res = {"workload": f"{args.batch_count} Riemann problems (seed {args.seed}): 70% Newtonian (gamma 1.4, CFL 0.9) / "
+                       "30% special relativistic (gamma 5/3, CFL 0.5, reference con2prim), 50% 1D [64, 4096] cells / "
+                       f"50% 2D [32, 512]^2, log-uniform; SLIC + van Leer; {args.batch_steps} steps each",
+           "problems": len(problems), "cells": int(sum(p["nx"] * p["ny"] for p in problems)),
+           "cell_updates": total}
+    with Batch(problems) as b:
+        for k, W in enumerate(states):
+            b.upload(k, W)
+        t0 = time.perf_counter()
+        res["tune_trial_ms"] = b.autotune(args.batch_tune_steps)
+        res["tune_s"] = time.perf_counter() - t0
+        res["schedule"] = {"newtonian": b.schedule(0), "relativistic": b.schedule(1)}
+        b.run()  # builds the graph for the tuned schedule
+        for k, W in enumerate(states):
+            b.upload(k, W)
+        t0 = time.perf_counter()
+        ms = b.run_timed()
+        wall = time.perf_counter() - t0
+        _, steps, fin = b.stats()
+    done = cell_updates(problems, steps)
+    res.update({"ms": ms, "wall_s": wall, "cell_updates_per_s": done / (ms * 1e-3),
+                "p50_ms": float(np.percentile(fin, 50)) * 1e-3, "p99_ms": float(np.percentile(fin, 99)) * 1e-3,
+                "max_ms": float(fin.max()) * 1e-3, "all_steps_taken": bool(np.all(steps == args.batch_steps))})
+    if not args.no_batch_baseline:
+        solvers = []
+        try:
+            for p, W in zip(problems, states):
+                s = Solver(p["physics"], p["dim"], p["nx"], p["ny"], p["adiabatic_index"])
+                solvers.append(s)
+                s.upload(W)
+            t0 = time.perf_counter()
+            for p, s in zip(problems, solvers):
+                s.solve(2, p["cell_spacing"], p["cfl"], math.inf, p["max_steps"], 0.0, VANLEER)
+            seq = time.perf_counter() - t0
+            for s, W in zip(solvers, states):
+                s.upload(W)
+            t0 = time.perf_counter()
+            for p, s in zip(problems, solvers):
+                s.begin(2, p["cell_spacing"], p["cfl"], math.inf, p["max_steps"], 0.0, VANLEER)
+            for p, s in zip(problems, solvers):
+                s.advance(p["max_steps"])
+            for s in solvers:
+                s.synchronize()
+            streams = time.perf_counter() - t0
+        finally:
+            for s in solvers:
+                s.close()
+        res["per_problem_sequential"] = {"s": seq, "cell_updates_per_s": total / seq}
+        res["per_problem_streams"] = {"s": streams, "cell_updates_per_s": total / streams}
+        res["speedup_vs_sequential"] = seq / (ms * 1e-3)
+        res["speedup_vs_streams"] = streams / (ms * 1e-3)
+    res["parity"] = "each problem bitwise equal to the CPU oracle's solve loop (tests/test_gpu_batch.py)"
+    return res
+
+
 def adjoint_leg(args):
     """Adjoint of the Newtonian SLIC update: the 4-quadrant workload at
     n x n (CFL 0.9, van Leer), args.adj_steps steps with a checkpoint every
@@ -416,7 +485,11 @@ def main():
     ap.add_argument("--sr-steps", type=int, default=200)
     ap.add_argument("--adj-size", type=int, default=1024)
     ap.add_argument("--adj-steps", type=int, default=200)
-    ap.add_argument("--no-extra", action="store_true", help="skip the SR, black-hole and adjoint legs")
+    ap.add_argument("--batch-count", type=int, default=1024)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--batch-tune-steps", type=int, default=10)
+    ap.add_argument("--no-batch-baseline", action="store_true", help="skip the per-problem launch baselines")
+    ap.add_argument("--no-extra", action="store_true", help="skip the SR, black-hole, adjoint and batched legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -471,6 +544,7 @@ def main():
         line["relativistic"] = sr_leg(args)
         line["blackhole"] = gr_leg(args)
         line["adjoint"] = adjoint_leg(args)
+        line["batched"] = batch_leg(args)
     if not args.no_cpu:
         line.update(cpu_leg(args))
     print(json.dumps(line), flush=True)

@@ -69,9 +69,11 @@ struct BProb {
     int ntx;          // tiled tasks per x phase
     int ry;           // rows per y march
     int nmarch;       // y marches per phase
+    int ytc;          // y sweep by LDS tiles of ytc columns (0: column marches)
+    int nytc;         // y tiles across a row
+    int nytile;       // y tiles per phase
     int nty;          // tiled tasks per y phase
     int max_steps;
-    int pad;
     double dx, cfl, tf;
     PhysParams P;
 };
@@ -104,16 +106,35 @@ struct BatchArgs {
     double bias;
 };
 
+// A copy of *p held in scalar registers: p is the same for every lane of the
+// wave (a problem record picked by the block's task), but the compiler cannot
+// prove the record unaliased by the kernel's stores, so it would keep the
+// fields in vector registers.
+template <class T>
+__device__ __forceinline__ T uniform_copy(const T* p) {
+    static_assert(sizeof(T) % 4 == 0, "word-sized records only");
+    T v;
+    const int* src = reinterpret_cast<const int*>(p);
+    int* dst = reinterpret_cast<int*>(&v);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+    return v;
+}
+
+// (buffer chosen by a select, not an index: a register copy of the record
+// must not be addressed)
 __device__ __forceinline__ ConstPlanes cplanes(const BProb& pb, int b) {
+    const double* base = b ? pb.buf[1] : pb.buf[0];
     ConstPlanes q;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q.p[k] = pb.buf[b] + k * pb.plane;
+    for (int k = 0; k < 5; ++k) q.p[k] = base + k * pb.plane;
     return q;
 }
 __device__ __forceinline__ Planes planes(const BProb& pb, int b) {
+    double* base = b ? pb.buf[1] : pb.buf[0];
     Planes q;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) q.p[k] = pb.buf[b] + k * pb.plane;
+    for (int k = 0; k < 5; ++k) q.p[k] = base + k * pb.plane;
     return q;
 }
 
@@ -193,6 +214,18 @@ __device__ __forceinline__ double y_one(const BProb& pb, const ConstPlanes& in, 
                                                 pb.P);
 }
 
+// y tile yt of problem pb (YT columns by kBT / YT rows, 2G of them halo).
+template <class Phys, int ORDER, int LIMMODE, int YT>
+__device__ __forceinline__ double y_tile(const BProb& pb, const ConstPlanes& in, const Planes& out, int yt, bool ws,
+                                         const SweepCoef& cf, double bias, int limiter, double (*lds)[kBT]) {
+    constexpr int G = ORDER == 2 ? 2 : 1;
+    constexpr int TRV = kBT / YT - 2 * G;
+    const int br = yt / pb.nytc;
+    const int bc = yt - br * pb.nytc;
+    return y_tile_cells<Phys, ORDER, LIMMODE, false, kBT, YT>(in, out, pb.nx, pb.ny, pb.pitch, bc * YT, br * TRV,
+                                                              ws ? 2 : 0, cf, bias, limiter, pb.P, lds);
+}
+
 struct WholeShared {
     SweepCoef half, full;
     double red[kBT / 64];
@@ -200,9 +233,9 @@ struct WholeShared {
     int cur;
 };
 
-template <class Phys, int ORDER, int LIMMODE>
+template <class Phys, int ORDER, int LIMMODE, int YT>
 __device__ void run_whole(const BatchArgs& A, int p, double (*lds)[kBT], WholeShared& sh) {
-    const BProb& pb = A.prob[p];
+    const BProb pb = uniform_copy(A.prob + p);
     BCtl* c = A.ctl + p;
     if (threadIdx.x == 0) {
         sh.more = !c->stopped;
@@ -228,9 +261,15 @@ __device__ void run_whole(const BatchArgs& A, int p, double (*lds)[kBT], WholeSh
                                                                  A.limiter, lds);
                     m = (w > m) ? w : m;
                 }
-            } else {
+            } else if constexpr (YT == 0) {
                 for (int mi = threadIdx.x; mi < pb.nmarch; mi += kBT) {
                     const double w = y_one<Phys, ORDER, LIMMODE>(pb, in, out, mi, last, cf, A.bias, A.limiter);
+                    m = (w > m) ? w : m;
+                }
+            } else {
+                for (int yt = 0; yt < pb.nytile; ++yt) {
+                    if (yt) __syncthreads();
+                    const double w = y_tile<Phys, ORDER, LIMMODE, YT>(pb, in, out, yt, last, cf, A.bias, A.limiter, lds);
                     m = (w > m) ? w : m;
                 }
             }
@@ -255,9 +294,9 @@ __device__ void run_whole(const BatchArgs& A, int p, double (*lds)[kBT], WholeSh
 }
 
 // One tiled task: phase `step * pps + sub` of problem p, task index `task`.
-template <class Phys, int ORDER, int LIMMODE>
+template <class Phys, int ORDER, int LIMMODE, int YT>
 __device__ void run_task(const BatchArgs& A, int p, int sub, int task, int step, double (*lds)[kBT], int& s_go) {
-    const BProb& pb = A.prob[p];
+    const BProb pb = uniform_copy(A.prob + p);
     BCtl* c = A.ctl + p;
     const int phase = step * pb.pps + sub;
     if (threadIdx.x == 0) {
@@ -295,9 +334,17 @@ __device__ void run_task(const BatchArgs& A, int p, int sub, int task, int step,
                 x_seg<Phys, ORDER, LIMMODE>(pb, in, out, sg, last ? pb.dim : 0, cf, A.bias, A.limiter, lds);
             m = (w > m) ? w : m;
         }
-    } else {
+    } else if constexpr (YT == 0) {
         const int mi = task * kBT + threadIdx.x;
         if (mi < pb.nmarch) m = y_one<Phys, ORDER, LIMMODE>(pb, in, out, mi, last, cf, A.bias, A.limiter);
+    } else {
+        const int t0 = task * pb.xtask;
+        const int t1 = min(t0 + pb.xtask, pb.nytile);
+        for (int yt = t0; yt < t1; ++yt) {
+            if (yt > t0) __syncthreads();
+            const double w = y_tile<Phys, ORDER, LIMMODE, YT>(pb, in, out, yt, last, cf, A.bias, A.limiter, lds);
+            m = (w > m) ? w : m;
+        }
     }
     if (last) {
         m = wave_max(m);
@@ -324,14 +371,15 @@ __device__ void run_task(const BatchArgs& A, int p, int sub, int task, int step,
     }
 }
 
-// Two waves per SIMD (up to 256 VGPRs): the kernel holds both the x and the
-// y sweep, whose exact Newtonian y march alone needs ~142 VGPRs; at three
-// waves it would spill.
-template <class Phys>
-constexpr int batch_min_waves() { return 2; }
+// Waves per SIMD: with the column march (YT = 0) the kernel holds its
+// register window (the exact Newtonian march alone needs ~142 VGPRs, the
+// special-relativistic ~180), so two waves (256 VGPRs); with LDS y tiles
+// both sweeps are the x sweep's thread-per-cell stages, which fit four.
+template <int YT>
+constexpr int batch_min_waves() { return YT == 0 ? 2 : 3; }
 
-template <class Phys, int ORDER, int LIMMODE>
-__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(batch_min_waves<Phys>(), 8))) void k_batch(
+template <class Phys, int ORDER, int LIMMODE, int YT>
+__global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(batch_min_waves<YT>(), 8))) void k_batch(
     BatchArgs A) {
     __shared__ double lds[15][kBT];
     __shared__ WholeShared sh;
@@ -345,16 +393,20 @@ __global__ __launch_bounds__(kBT) __attribute__((amdgpu_waves_per_eu(batch_min_w
             s_ticket = t;
         }
         __syncthreads();
-        const int t = s_ticket;
+        // uniform across the block: keep the task's indices (and so every
+        // problem-record load) in scalar registers
+        const int t = __builtin_amdgcn_readfirstlane(s_ticket);
         __syncthreads();
         if ((long)t >= total) return;
         if (t < A.n_whole) {
-            run_whole<Phys, ORDER, LIMMODE>(A, A.whole_list[t], lds, sh);
+            run_whole<Phys, ORDER, LIMMODE, YT>(A, __builtin_amdgcn_readfirstlane(A.whole_list[t]), lds, sh);
         } else {
             const int u = t - A.n_whole;
             const int step = u / A.level_len;
             const uint2 e = A.level[u - step * A.level_len];
-            run_task<Phys, ORDER, LIMMODE>(A, (int)e.x, (int)(e.y >> 24), (int)(e.y & 0xffffffu), step, lds, s_go);
+            const int p = __builtin_amdgcn_readfirstlane((int)e.x);
+            const int y = __builtin_amdgcn_readfirstlane((int)e.y);
+            run_task<Phys, ORDER, LIMMODE, YT>(A, p, y >> 24, y & 0xffffff, step, lds, s_go);
         }
         __syncthreads();
     }
@@ -439,11 +491,18 @@ __global__ __launch_bounds__(256) void k_batch_planes_to_aos(ConstPlanes in, dou
 
 typedef void (*BatchKernel)(BatchArgs);
 
+template <class Phys, int YT>
+BatchKernel pick_kernel_yt(int order, int limmode) {
+    if (order == 1) return k_batch<Phys, 1, 0, YT>;
+    if (limmode == 1) return k_batch<Phys, 2, 1, YT>;
+    return k_batch<Phys, 2, 0, YT>;
+}
+
 template <class Phys>
-BatchKernel pick_kernel(int order, int limmode) {
-    if (order == 1) return k_batch<Phys, 1, 0>;
-    if (limmode == 1) return k_batch<Phys, 2, 1>;
-    return k_batch<Phys, 2, 0>;
+BatchKernel pick_kernel(int order, int limmode, int ytc) {
+    if (ytc == 16) return pick_kernel_yt<Phys, 16>(order, limmode);
+    if (ytc == 8) return pick_kernel_yt<Phys, 8>(order, limmode);
+    return pick_kernel_yt<Phys, 0>(order, limmode);
 }
 
 }  // namespace
@@ -562,9 +621,25 @@ void plan_problem(const cosmos_batch* b, BProb& pb, const cosmos_batch_schedule&
     pb.ntx = (pb.nxseg + pb.xtask - 1) / pb.xtask;
     pb.ry = std::max(1, cfg.y_rows);
     pb.nmarch = pb.dim == 2 ? (pb.ny + pb.ry - 1) / pb.ry * pb.nx : 0;
-    pb.nty = (pb.nmarch + cfv::kBT - 1) / cfv::kBT;
+    pb.ytc = cfg.y_tile_cols;
+    if (pb.ytc > 0 && pb.dim == 2) {
+        const int trv = cfv::kBT / pb.ytc - 2 * G;
+        pb.nytc = (pb.nx + pb.ytc - 1) / pb.ytc;
+        pb.nytile = pb.nytc * ((pb.ny + trv - 1) / trv);
+        pb.nty = (pb.nytile + pb.xtask - 1) / pb.xtask;
+    } else {
+        pb.nytc = pb.nytile = 0;
+        pb.nty = (pb.nmarch + cfv::kBT - 1) / cfv::kBT;
+    }
     const long cells = (long)pb.nx * pb.ny;
     pb.whole = cells <= (long)cfg.whole_max_cells ? 1 : 0;
+}
+
+cfv::BatchKernel kernel_for(const cosmos_batch* b, int ph) {
+    const int lm = (b->order == 2 && b->limiter == 1 && b->bias == 0.0) ? 1 : 0;
+    const int ytc = b->sched[ph].cfg.y_tile_cols;
+    return ph == 0 ? cfv::pick_kernel<cfv::Newtonian>(b->order, lm, ytc)
+                   : cfv::pick_kernel<cfv::Relativistic>(b->order, lm, ytc);
 }
 
 int build_schedules(cosmos_batch* b) {
@@ -613,8 +688,7 @@ int build_schedules(cosmos_batch* b) {
             BCHK(b, hipMemcpy(S.d_level, S.level.data(), sizeof(uint2) * S.level.size(), hipMemcpyHostToDevice));
         }
         int occ = 0;
-        cfv::BatchKernel k = ph == 0 ? cfv::pick_kernel<cfv::Newtonian>(b->order, b->limiter == 1 && b->bias == 0.0 ? 1 : 0)
-                                     : cfv::pick_kernel<cfv::Relativistic>(b->order, b->limiter == 1 && b->bias == 0.0 ? 1 : 0);
+        cfv::BatchKernel k = kernel_for(b, ph);
         BCHK(b, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k, cfv::kBT, 0));
         const int per_cu = S.cfg.blocks_per_cu > 0 ? S.cfg.blocks_per_cu : std::max(1, occ);
         S.grid = per_cu * b->cus;
@@ -653,7 +727,6 @@ int queue_run(cosmos_batch* b) {
     BCHK(b, hipGetLastError());
     hipLaunchKernelGGL(cfv::k_batch_init_head, gp, blk, 0, b->stream, (const BProb*)b->d_prob, b->d_ctl, b->n);
     BCHK(b, hipGetLastError());
-    const int lm = (b->order == 2 && b->limiter == 1 && b->bias == 0.0) ? 1 : 0;
     const bool both = b->sched[0].present && b->sched[1].present;
     if (both) {
         BCHK(b, hipEventRecord(b->fork, b->stream));
@@ -663,8 +736,7 @@ int queue_run(cosmos_batch* b) {
         PhysSchedule& S = b->sched[ph];
         if (!S.present) continue;
         hipStream_t st = (both && ph == 1) ? b->side : b->stream;
-        cfv::BatchKernel k =
-            ph == 0 ? cfv::pick_kernel<cfv::Newtonian>(b->order, lm) : cfv::pick_kernel<cfv::Relativistic>(b->order, lm);
+        cfv::BatchKernel k = kernel_for(b, ph);
         cfv::BatchArgs A = args_for(b, ph);
         void* kargs[] = {&A};
         BCHK(b, hipLaunchKernel((const void*)k, dim3((unsigned)S.grid), blk, kargs, 0, st));
@@ -737,10 +809,12 @@ extern "C" {
 
 int cosmos_batch_default_schedule(int physics, cosmos_batch_schedule* out) {
     if (!out) return COSMOS_ERR_ARGUMENT;
-    out->whole_max_cells = 4096;
-    out->x_segments_per_task = physics == COSMOS_RELATIVISTIC ? 1 : 4;
-    out->y_rows = physics == COSMOS_RELATIVISTIC ? 8 : 16;
+    // the autotuner's choice on the 1024-problem Riemann workload
+    out->whole_max_cells = physics == COSMOS_RELATIVISTIC ? 4096 : 65536;
+    out->x_segments_per_task = physics == COSMOS_RELATIVISTIC ? 16 : 64;
+    out->y_rows = physics == COSMOS_RELATIVISTIC ? 32 : 64;
     out->blocks_per_cu = 0;
+    out->y_tile_cols = 8;
     return COSMOS_OK;
 }
 
@@ -871,7 +945,8 @@ int cosmos_batch_set_schedule(cosmos_batch* b, int physics, const cosmos_batch_s
         if (physics != COSMOS_NEWTONIAN && physics != COSMOS_RELATIVISTIC)
             return bfail(b, COSMOS_ERR_ARGUMENT, "physics must be Newtonian or relativistic");
         if (!s || s->whole_max_cells < 0 || s->x_segments_per_task < 1 || s->y_rows < 1 || s->y_rows > 4096 ||
-            s->blocks_per_cu < 0 || s->blocks_per_cu > 32)
+            s->blocks_per_cu < 0 || s->blocks_per_cu > 32 ||
+            (s->y_tile_cols != 0 && s->y_tile_cols != 8 && s->y_tile_cols != 16))
             return bfail(b, COSMOS_ERR_ARGUMENT, "invalid schedule");
         b->sched[physics].cfg = *s;
         b->dirty = true;
@@ -1004,17 +1079,21 @@ int cosmos_batch_autotune(cosmos_batch* b, int trial_steps, double* best_ms) {
         double cur_ms = 0.0;
         int rc = trial(cur_ms);
         if (rc) return rc;
-        const int wholes[] = {0, 1024, 4096, 16384, 65536};
-        const int xtasks[] = {1, 2, 4, 8, 16};
-        const int yrows[] = {4, 8, 16, 32, 64};
+        const std::vector<int> cands[4] = {{0, 4096, 16384, 65536, 262144},
+                                           {2, 4, 8, 16, 32, 64},
+                                           {0, 8, 16},
+                                           {8, 16, 32, 64, 128}};
         for (int ph = 0; ph < 2 && !rc; ++ph) {
             if (!b->sched[ph].present) continue;
-            for (int param = 0; param < 3 && !rc; ++param) {
-                const int* cand = param == 0 ? wholes : (param == 1 ? xtasks : yrows);
+            for (int param = 0; param < 4 && !rc; ++param) {
+                const std::vector<int>& cand = cands[param];
                 cosmos_batch_schedule best = b->sched[ph].cfg;
-                for (int k = 0; k < 5; ++k) {
+                if (param == 3 && best.y_tile_cols != 0) continue;  // y_rows only shapes the column march
+                for (size_t k = 0; k < cand.size(); ++k) {
                     cosmos_batch_schedule c = best;
-                    int& field = param == 0 ? c.whole_max_cells : (param == 1 ? c.x_segments_per_task : c.y_rows);
+                    int& field = param == 0 ? c.whole_max_cells
+                                            : (param == 1 ? c.x_segments_per_task
+                                                          : (param == 2 ? c.y_tile_cols : c.y_rows));
                     if (field == cand[k]) continue;
                     field = cand[k];
                     const cosmos_batch_schedule keep = b->sched[ph].cfg;

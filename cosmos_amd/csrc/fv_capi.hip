@@ -39,6 +39,7 @@ struct cosmos_solver {
     bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
     int mode = COSMOS_MODE_EXACT;
     int y_rows = 0;                            // y-sweep rows per thread (0: by physics)
+    int y_tile = -1;                           // y-sweep layout: 0 march, 8 / 16 tile columns (-1: by physics)
     int ws_mode = 0;                           // fused wave-speed hand-off (0: default)
     cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
     double excision = 0.0;                     // > 0: excision radius override
@@ -152,6 +153,7 @@ SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, in
     L.use_full = 0;
     L.fuse_ws = 0;
     L.y_rows = s->y_rows;
+    L.y_tile = s->y_tile;
     return L;
 }
 
@@ -401,6 +403,16 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode) {
         if (ws_mode < 0 || ws_mode > 2) return fail(s, COSMOS_ERR_ARGUMENT, "ws_mode must be 0, 1 or 2");
         s->y_rows = y_rows;
         s->ws_mode = ws_mode;
+        s->run_ready = false;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_set_y_tile(cosmos_solver* s, int columns) {
+    return guarded(s, [&] {
+        if (columns != -1 && columns != 0 && columns != 8 && columns != 16)
+            return fail(s, COSMOS_ERR_ARGUMENT, "y tile columns must be -1 (default), 0 (march), 8 or 16");
+        s->y_tile = columns;
         s->run_ready = false;
         return COSMOS_OK;
     });

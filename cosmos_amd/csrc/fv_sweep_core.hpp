@@ -124,17 +124,19 @@ __device__ __forceinline__ void update_cell(const double* U, const double* Fl, c
     Phys::c2p(un, W, Phys::inv_density(un), P);
 }
 
-// One x-sweep pass of a block of NT threads over a run of consecutive cells
-// of a row (or several short rows packed side by side).  Thread s loads the
-// cell at ld_idx (already mapped through the boundary) into LDS column s; the
-// stencil of thread s uses columns s - 1 and s + 1, so each row needs G = 2
-// (SLIC) or 1 (FORCE) halo threads on both sides, whose outputs are not
-// stored.  valid: the thread's cell is an output, stored at st_idx.  lds: 15
-// rows of NT doubles; the caller must barrier before reusing it.  Returns
-// the wave speed of the output cell (the 1D or 2D form by ws_dim; 0 when
-// ws_dim is 0 or the cell is not an output).
-template <class Phys, int ORDER, int LIMMODE, bool W0, int NT>
-__device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
+// One sweep pass of a block of NT threads over cells laid out in LDS so that
+// the neighbours of thread s along the swept axis are threads s - STRIDE and
+// s + STRIDE: a run of consecutive cells of a row (x sweep, STRIDE 1; several
+// short rows may be packed side by side) or a tile of STRIDE columns (y
+// sweep, one row of the tile per STRIDE threads).  Thread s loads the cell at
+// ld_idx (already mapped through the boundary) into LDS column s; each line
+// needs G = 2 (SLIC) or 1 (FORCE) halo cells at both ends, whose outputs are
+// not stored.  valid: the thread's cell is an output, stored at st_idx.
+// lds: 15 rows of NT doubles; the caller must barrier before reusing it.
+// Returns the wave speed of the output cell (the 1D or 2D form by ws_dim; 0
+// when ws_dim is 0 or the cell is not an output).
+template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>
+__device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
                                           bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
                                           const PhysParams& P, double (*lds)[NT]) {
     //   SLIC:  sA = rows 0-4   stage A U, then the interface fluxes
@@ -148,8 +150,8 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
     load_cell<W0>(in, ld_idx, W);
     Phys::p2c(W, U, P);
 
-    const int sl = s > 0 ? s - 1 : 0;
-    const int sr = s < NT - 1 ? s + 1 : NT - 1;
+    const int sl = s >= STRIDE ? s - STRIDE : 0;
+    const int sr = s < NT - STRIDE ? s + STRIDE : NT - 1;
     if constexpr (ORDER == 2) {
 #pragma unroll
         for (int k = 0; k < 5; ++k)
@@ -161,7 +163,7 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
             ul[k] = (W0 && k == 3) ? 0.0 : sA[k][sl];
             ur[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
         }
-        half_step<Phys, 0, LIMMODE, W0>(ul, U, ur, cf, bias, limiter, P, Q0, G0, Q1, G1);
+        half_step<Phys, AXIS, LIMMODE, W0>(ul, U, ur, cf, bias, limiter, P, Q0, G0, Q1, G1);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             if (W0 && k == 3) continue;
@@ -175,7 +177,7 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
             UR[k] = (W0 && k == 3) ? 0.0 : sB[k][sr];
             FR[k] = (W0 && k == 3) ? 0.0 : sB[5 + k][sr];
         }
-        force_flux<Phys, 0, W0>(Q1, G1, UR, FR, cf, P, F);
+        force_flux<Phys, AXIS, W0>(Q1, G1, UR, FR, cf, P, F);
         // interface (s, s+1) -> sA (stage A values are no longer read)
 #pragma unroll
         for (int k = 0; k < 5; ++k)
@@ -183,7 +185,7 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
         __syncthreads();
     } else {
         double FU[5];
-        Phys::template flux<0>(U, FU, Phys::inv_density(U), P);
+        Phys::template flux<AXIS>(U, FU, Phys::inv_density(U), P);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             if (W0 && k == 3) continue;
@@ -197,7 +199,7 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
             UR[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
             FR[k] = (W0 && k == 3) ? 0.0 : sA[5 + k][sr];
         }
-        force_flux<Phys, 0, W0>(U, FU, UR, FR, cf, P, F);
+        force_flux<Phys, AXIS, W0>(U, FU, UR, FR, cf, P, F);
 #pragma unroll
         for (int k = 0; k < 5; ++k)
             if (!(W0 && k == 3)) sB[k][s] = F[k];
@@ -214,6 +216,35 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
         if (w > m) m = w;
     }
     return m;
+}
+
+template <class Phys, int ORDER, int LIMMODE, bool W0, int NT>
+__device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
+                                          bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
+                                          const PhysParams& P, double (*lds)[NT]) {
+    return line_cells<Phys, ORDER, LIMMODE, W0, NT, 0, 1>(in, out, ld_idx, st_idx, valid, ws_dim, cf, bias, limiter, P,
+                                                           lds);
+}
+
+// One y-sweep tile by a block of NT threads: TC columns by NT / TC rows,
+// thread s at tile row s / TC and column s mod TC, the first and last G rows
+// halo.  Outputs rows [r0, r0 + NT / TC - 2G) of columns [c0, c0 + TC)
+// (clipped to the grid).
+template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int TC>
+__device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Planes& out, int nx, int ny, long pitch,
+                                               int c0, int r0, int ws_dim, const SweepCoef& cf, double bias,
+                                               int limiter, const PhysParams& P, double (*lds)[NT]) {
+    constexpr int G = ORDER == 2 ? 2 : 1;
+    constexpr int TR = NT / TC;
+    const int s = threadIdx.x;
+    const int r = s / TC;
+    const int i = r0 - G + r;
+    const int j = c0 + (s - r * TC);
+    const int jj = j < nx ? j : nx - 1;
+    const int ii = ORDER == 2 ? mirror2(i, ny) : clamp1(i, ny);
+    const bool valid = (r >= G) && (r < TR - G) && (i < ny) && (j < nx);
+    return line_cells<Phys, ORDER, LIMMODE, W0, NT, 1, TC>(in, out, (long)ii * pitch + jj, (long)i * pitch + j, valid,
+                                                            ws_dim, cf, bias, limiter, P, lds);
 }
 
 // One row segment of an x sweep: thread s holds cell tile * (NT - 2G) - G + s

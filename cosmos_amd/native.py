@@ -28,7 +28,7 @@ class BatchProblem(ctypes.Structure):
 class BatchSchedule(ctypes.Structure):
     """cosmos_batch_schedule"""
     _fields_ = [("whole_max_cells", ctypes.c_int), ("x_segments_per_task", ctypes.c_int), ("y_rows", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("y_tile_cols", ctypes.c_int)]
 
 
 # name -> (restype, argtypes); mirrors include/cosmos_fv.h.
@@ -40,6 +40,7 @@ SIGNATURES = {
     "cosmos_set_mode": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
     "cosmos_set_launch": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int]),
+    "cosmos_set_y_tile": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_set_spacetime": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double]),
     "cosmos_upload": (ctypes.c_int, [_handle, _c_double_p]),

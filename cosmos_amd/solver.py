@@ -96,6 +96,10 @@ class Solver:
         slot atomics, 0 = default)."""
         self._check(self._lib.cosmos_set_launch(self._h, y_rows, ws_mode))
 
+    def set_y_tile(self, columns: int = -1) -> None:
+        """y-sweep layout: 0 column march, 8 or 16 LDS tile columns, -1 default by physics."""
+        self._check(self._lib.cosmos_set_y_tile(self._h, columns))
+
     def set_spacetime(self, mass: float = 1.0, spin: float = 0.0, x: float = 0.5, y: float = 0.5,
                       z: float = 0.0) -> None:
         """BlackHoleSpacetime(mass, spin, x, y, z) of a BLACKHOLE solver."""

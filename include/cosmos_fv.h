@@ -76,6 +76,13 @@ int cosmos_set_mode(cosmos_solver* s, int mode);
  *            next step's head; 2 one atomic maximum per wavefront into 32
  *            slots; 0: the default (2). */
 int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode);
+/* y-sweep layout (no counterpart in the reference; results do not depend on
+ * it): 0 one thread per column marching down y_rows rows with a register
+ * window; 8 or 16: one thread per cell over LDS tiles of that many columns
+ * (re-evaluating 4 halo rows per tile for SLIC, at the x sweep's occupancy);
+ * -1: the default (the march for Newtonian, 8-column tiles for special
+ * relativistic). */
+int cosmos_set_y_tile(cosmos_solver* s, int columns);
 
 /* Special-relativistic con2prim controls (default: the reference's stopping
  * tolerance 1e-8 and cap of 1000 iterations,
@@ -310,14 +317,17 @@ typedef struct {
 /* How a physics' problems are cut into tasks (the autotuner's knobs):
  * problems of at most whole_max_cells cells run start to finish in one
  * block; larger ones run phase by phase as tasks of x_segments_per_task
- * x segments (256-thread row pieces, short rows packed side by side) or of
- * 256 y marches of y_rows rows.  blocks_per_cu: persistent blocks per
- * compute unit (0: the kernel's occupancy). */
+ * x segments (256-thread row pieces, short rows packed side by side) and,
+ * for the y sweep, of as many LDS tiles of y_tile_cols columns (8 or 16)
+ * or, with y_tile_cols 0, of 256 column marches of y_rows rows.
+ * blocks_per_cu: persistent blocks per compute unit (0: the kernel's
+ * occupancy). */
 typedef struct {
     int whole_max_cells;
     int x_segments_per_task;
     int y_rows;
     int blocks_per_cu;
+    int y_tile_cols;
 } cosmos_batch_schedule;
 
 int cosmos_batch_default_schedule(int physics, cosmos_batch_schedule* out);
@@ -342,7 +352,8 @@ int cosmos_batch_run_timed(cosmos_batch* b, double* ms);
  * the run's first kernel started (device wall clock). */
 int cosmos_batch_stats(cosmos_batch* b, double* time_reached, int* steps_taken, double* finish_us);
 /* Coordinate descent over each physics' schedule (whole_max_cells,
- * x_segments_per_task, y_rows) on timed runs capped at trial_steps steps;
+ * x_segments_per_task, y_tile_cols, y_rows) on timed runs capped at
+ * trial_steps steps;
  * keeps the fastest.  The problems' states are restored afterwards. */
 int cosmos_batch_autotune(cosmos_batch* b, int trial_steps, double* best_ms);
 

@@ -79,11 +79,17 @@ def run_batch(problems, states, order=2, bias=0.0, limiter=1, schedules=None, ru
 
 SCHEDULES = {
     "default": None,
-    "all_whole": {0: {"whole_max_cells": 1 << 20}, 1: {"whole_max_cells": 1 << 20}},
-    "all_tiled": {0: {"whole_max_cells": 0, "x_segments_per_task": 1, "y_rows": 4},
-                  1: {"whole_max_cells": 0, "x_segments_per_task": 1, "y_rows": 4}},
-    "tiled_odd": {0: {"whole_max_cells": 0, "x_segments_per_task": 3, "y_rows": 7, "blocks_per_cu": 1},
-                  1: {"whole_max_cells": 100, "x_segments_per_task": 2, "y_rows": 5, "blocks_per_cu": 1}},
+    "all_whole": {0: {"whole_max_cells": 1 << 20, "y_tile_cols": 0}, 1: {"whole_max_cells": 1 << 20, "y_tile_cols": 0}},
+    "all_whole_tiles": {0: {"whole_max_cells": 1 << 20, "y_tile_cols": 16},
+                        1: {"whole_max_cells": 1 << 20, "y_tile_cols": 8}},
+    "all_tiled": {0: {"whole_max_cells": 0, "x_segments_per_task": 1, "y_rows": 4, "y_tile_cols": 0},
+                  1: {"whole_max_cells": 0, "x_segments_per_task": 1, "y_rows": 4, "y_tile_cols": 0}},
+    "tiled_odd": {0: {"whole_max_cells": 0, "x_segments_per_task": 3, "y_rows": 7, "blocks_per_cu": 1,
+                      "y_tile_cols": 0},
+                  1: {"whole_max_cells": 100, "x_segments_per_task": 2, "y_rows": 5, "blocks_per_cu": 1,
+                      "y_tile_cols": 0}},
+    "tiles": {0: {"whole_max_cells": 0, "x_segments_per_task": 1, "y_tile_cols": 8},
+              1: {"whole_max_cells": 300, "x_segments_per_task": 3, "y_tile_cols": 16}},
 }
 
 
@@ -107,7 +113,12 @@ def test_batch_matches_oracle(mixed, name):
 def test_batch_other_schemes(orc, order, bias, limiter):
     problems, states = mixed_batch(seed=6, steps=6)
     exp = oracle_runs(orc, problems, states, order, bias, limiter)
-    got, t, steps, _ = run_batch(problems, states, order, bias, limiter, schedules=SCHEDULES["tiled_odd"])
+    for name in ("tiled_odd", "tiles"):
+        got, t, steps, _ = run_batch(problems, states, order, bias, limiter, schedules=SCHEDULES[name])
+        for k, (W, (E, te, ne)) in enumerate(zip(got, exp)):
+            assert same(W, E), (name, k, problems[k], rel_linf(W, E))
+            assert t[k] == te and steps[k] == ne
+    got, t, steps, _ = run_batch(problems, states, order, bias, limiter, schedules=SCHEDULES["all_whole_tiles"])
     for k, (W, (E, te, ne)) in enumerate(zip(got, exp)):
         assert same(W, E), (k, problems[k], rel_linf(W, E))
         assert t[k] == te and steps[k] == ne

@@ -256,6 +256,37 @@ def test_fast_mode_config2_full_size_100_steps(orc):
     assert k == 100 and err <= FAST_TOL, err
 
 
+@pytest.mark.parametrize("columns", [0, 8, 16])
+@pytest.mark.parametrize("physics", [NEWTONIAN, RELATIVISTIC])
+def test_y_sweep_layout_does_not_change_results(orc, columns, physics):
+    """Column march or LDS tiles of 8 / 16 columns for the y sweep: bitwise
+    the oracle's sweep on shapes that do not divide the tiles, for every
+    order and limiter, and through the fused-wave-speed solve loop."""
+    gamma = G_N if physics == NEWTONIAN else G_SR
+    for nx, ny in ((37, 70), (130, 33), (5, 3), (17, 2)):
+        W = workloads.quadrants_2d(nx, seed=14, ny=ny)
+        if physics == RELATIVISTIC:
+            W[..., 1:3] *= 0.5
+        dx = 1.0 / max(nx, ny)
+        for order, limiter, bias in ((1, 1, 0.0), (2, 1, 0.0), (2, 0, 0.4), (2, 2, -0.3)):
+            with Solver(physics, 2, nx, ny, gamma) as s:
+                s.set_y_tile(columns)
+                s.upload(W)
+                s.sweep(order, 1, dx, 0.2 * dx, bias, limiter)
+                out = s.download()
+            exp = orc.sweep(W, physics, order, 1, dx, 0.2 * dx, bias, limiter, gamma)
+            assert same(out, exp), (nx, ny, order, limiter, rel_linf(out, exp))
+    W = workloads.quadrants_2d(48, seed=15, ny=40)
+    if physics == RELATIVISTIC:
+        W[..., 1:3] *= 0.5
+    exp, te, ne = orc.run(W, physics, 2, 1.0 / 48, 0.5, math.inf, 7, 0.0, 1, gamma)
+    with Solver(physics, 2, 48, 40, gamma) as s:
+        s.set_y_tile(columns)
+        s.upload(W)
+        t, n = s.solve(2, 1.0 / 48, 0.5, math.inf, 7)
+        assert same(s.download(), exp) and (t, n) == (te, ne)
+
+
 @pytest.mark.parametrize("y_rows", [16, 32, 64])
 @pytest.mark.parametrize("physics", [NEWTONIAN, RELATIVISTIC])
 def test_y_sweep_tile_does_not_change_results(orc, y_rows, physics):
@@ -269,6 +300,7 @@ def test_y_sweep_tile_does_not_change_results(orc, y_rows, physics):
         dx = 1.0 / max(nx, ny)
         for order in (1, 2):
             with Solver(physics, 2, nx, ny, gamma) as s:
+                s.set_y_tile(0)
                 s.set_launch(y_rows)
                 s.upload(W)
                 s.sweep(order, 1, dx, 0.2 * dx)
