@@ -20,8 +20,8 @@ LIB = os.path.join(HERE, "libcosmos_fv.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("COSMOS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["fv_kernels.hip", "fv_gr.hip", "fv_capi.hip", "fv_amr.hip", "fv_adjoint.hip", "fv_batch.hip",
-           "fv_pointwise.hip"]
+SOURCES = ["fv_kernels.hip", "fv_gr.hip", "fv_capi.hip", "fv_amr.hip", "fv_adjoint.hip", "fv_gr_adjoint.hip",
+           "fv_batch.hip", "fv_pointwise.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

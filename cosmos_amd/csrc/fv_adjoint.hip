@@ -30,6 +30,7 @@
 
 #include "../../include/cosmos_fv.h"
 #include "fv_adjoint_physics.hpp"
+#include "fv_gr_adjoint_launch.hpp"
 #include "fv_kernels.hpp"
 
 namespace cfv {
@@ -250,6 +251,9 @@ struct AdjState {
 };
 
 struct cosmos_adjoint {
+    int physics = COSMOS_NEWTONIAN;
+    int subcycles = 1;            // black hole: forcing sub-steps on each side of the sweeps
+    double* gbuf = nullptr;       // black-hole sweeps: per-cell conserved adjoints (5 planes)
     int dim = 2, nx = 0, ny = 0;
     double gamma = 1.4;
     cosmos_solver* s = nullptr;
@@ -396,24 +400,91 @@ int forward_sweep_dev(cosmos_adjoint* a, int axis, double dt, const double* in, 
     return COSMOS_OK;
 }
 
-// The sweeps of one step with time step dt: (axis, dt factor) in order.
-int step_sweeps(const cosmos_adjoint* a, int* axes, double* fac) {
+// One operation of a step: a sweep along `axis` or (black hole) a forcing
+// step, each with its own time step.
+struct Op {
+    int forcing;
+    int axis;
+    double dt;
+};
+constexpr int kMaxOps = 2 * 8 + 3;
+
+// The operations of one solve-loop step with time step dt, in order:
+// solve / solve2D (Solvers/EulerSecondOrderSolver.cpp:133-178) sweep x (1D) or
+// X(dt * 0.5) Y(dt) X(dt * 0.5) (2D); the black-hole loops
+// (Solvers/BlackHoleEulerSecondOrderSolver.cpp:145-218) wrap the sweeps in
+// `subcycles` forcing steps of 0.5 * (dt / subcycles) on each side.
+int step_ops(const cosmos_adjoint* a, double dt, Op* ops) {
+    int n = 0;
+    const bool bh = a->physics == COSMOS_BLACKHOLE;
+    if (bh)
+        for (int k = 0; k < a->subcycles; ++k) ops[n++] = {1, 0, 0.5 * (dt / a->subcycles)};
     if (a->dim == 1) {
-        axes[0] = 0;
-        fac[0] = 1.0;
-        return 1;
+        ops[n++] = {0, 0, dt};
+    } else {
+        ops[n++] = {0, 0, dt * 0.5};
+        ops[n++] = {0, 1, dt};
+        ops[n++] = {0, 0, dt * 0.5};
     }
-    axes[0] = 0;
-    fac[0] = 0.5;
-    axes[1] = 1;
-    fac[1] = 1.0;
-    axes[2] = 0;
-    fac[2] = 0.5;
-    return 3;
+    if (bh)
+        for (int k = 0; k < a->subcycles; ++k) ops[n++] = {1, 0, 0.5 * (dt / a->subcycles)};
+    return n;
 }
 
-// dt * 0.5 for the outer Strang sweeps, as solve2D forms it (Solvers/EulerSecondOrderSolver.cpp:172-174)
-double sweep_dt(double dt, double f) { return f == 1.0 ? dt : dt * 0.5; }
+// The operation through the public API (on the solver's own state).
+int apply_op(cosmos_adjoint* a, const Op& op) {
+    int rc = op.forcing ? cosmos_forcing(a->s, a->dx, op.dt, 0.0, 1)
+                        : cosmos_sweep(a->s, a->order, op.axis, a->dx, op.dt, a->bias, a->limiter);
+    return rc ? from_solver(a, rc) : COSMOS_OK;
+}
+
+int gr_launch_of(cosmos_adjoint* a, const Op& op, double bias, int limiter, cfv::GRLaunch& L) {
+    int rc = op.forcing ? cfv::gr_forcing_launch(a->s, a->dx, op.dt, 0.0, 1, &L)
+                        : cfv::gr_sweep_launch(a->s, a->order, a->dx, op.dt, bias, limiter, &L);
+    if (rc) return adj_fail(a, rc, "black-hole launch set-up failed");
+    return COSMOS_OK;
+}
+
+// The operation from the planes at `in` to the planes at `out` (the
+// segment recompute of the reverse pass).
+int forward_op_dev(cosmos_adjoint* a, const Op& op, const double* in, double* out) {
+    if (a->physics != COSMOS_BLACKHOLE) return forward_sweep_dev(a, op.axis, op.dt, in, out);
+    cfv::GRLaunch L;
+    int rc = gr_launch_of(a, op, a->bias, a->limiter, L);
+    if (rc) return rc;
+    L.in = cplanes_at(a, in);
+    L.out = planes_at(a, out);
+    ADJCHK(a, op.forcing ? cfv::launch_gr_forcing(L, stream_of(a)) : cfv::launch_gr_sweep(L, op.axis, stream_of(a)));
+    return COSMOS_OK;
+}
+
+// lam[dst] <- the adjoint of operation op from state W (planes at w) to O
+// (planes at o), given lam[src].  Black-hole sweeps re-run the forward sweep
+// (rewriting O with the same values) for its interface fluxes.
+int adjoint_op_dev(cosmos_adjoint* a, const Op& op, int order, double bias, int limiter, const double* w, double* o,
+                   int src, int dst) {
+    if (a->physics != COSMOS_BLACKHOLE)
+        return adjoint_sweep_dev(a, order, op.axis, a->dx, op.dt, bias, limiter, w, o, src, dst);
+    if (!a->gbuf) ADJCHK(a, hipMalloc((void**)&a->gbuf, sizeof(double) * 5 * a->plane));
+    cfv::GRAdjLaunch A{};
+    int rc = gr_launch_of(a, op, bias, limiter, A.L);
+    if (rc) return rc;
+    A.L.order = order;
+    A.L.in = cplanes_at(a, w);
+    A.L.out = planes_at(a, o);
+    A.lam = cplanes_at(a, a->lam[src]);
+    A.lam_in = planes_at(a, a->lam[dst]);
+    A.gbuf = planes_at(a, a->gbuf);
+    hipStream_t st = stream_of(a);
+    if (op.forcing) {
+        ADJCHK(a, hipMemsetAsync(a->lam[dst], 0, sizeof(double) * 5 * a->plane, st));
+        ADJCHK(a, cfv::launch_gr_adjoint_forcing(A, st));
+    } else {
+        ADJCHK(a, cfv::launch_gr_sweep(A.L, op.axis, st));
+        ADJCHK(a, cfv::launch_gr_adjoint_sweep(A, op.axis, st));
+    }
+    return COSMOS_OK;
+}
 
 template <class F>
 int adj_guarded(cosmos_adjoint* a, F&& f) {
@@ -439,7 +510,8 @@ extern "C" {
 int cosmos_adjoint_create(cosmos_adjoint** out, int physics, int dim, int nx, int ny, double adiabatic_index) {
     if (!out) return adj_fail(nullptr, COSMOS_ERR_ARGUMENT, "out is null");
     *out = nullptr;
-    if (physics != COSMOS_NEWTONIAN) return adj_fail(nullptr, COSMOS_ERR_ARGUMENT, "the adjoint supports COSMOS_NEWTONIAN");
+    if (physics != COSMOS_NEWTONIAN && physics != COSMOS_BLACKHOLE)
+        return adj_fail(nullptr, COSMOS_ERR_ARGUMENT, "the adjoint supports COSMOS_NEWTONIAN and COSMOS_BLACKHOLE");
     cosmos_adjoint* a = new (std::nothrow) cosmos_adjoint();
     if (!a) return adj_fail(nullptr, COSMOS_ERR_DEVICE, "host allocation failed");
     int rc = cosmos_create(&a->s, physics, dim, nx, ny, adiabatic_index);
@@ -448,6 +520,7 @@ int cosmos_adjoint_create(cosmos_adjoint** out, int physics, int dim, int nx, in
         delete a;
         return adj_fail(nullptr, rc, msg);
     }
+    a->physics = physics;
     a->dim = dim;
     a->nx = nx;
     a->ny = dim == 1 ? 1 : ny;
@@ -481,6 +554,7 @@ int cosmos_adjoint_destroy(cosmos_adjoint* a) {
     hipFree(a->lam[0]);
     hipFree(a->lam[1]);
     hipFree(a->stage);
+    hipFree(a->gbuf);
     free_list(a->checkpoints);
     free_list(a->seg);
     cosmos_destroy(a->s);
@@ -503,9 +577,40 @@ int cosmos_adjoint_sweep(cosmos_adjoint* a, int order, int axis, double cell_spa
         if (rc) return from_solver(a, rc);
         if ((rc = current_base(a, o))) return rc;
         if ((rc = lam_upload(a, lambda_out, 0))) return rc;
-        if ((rc = adjoint_sweep_dev(a, order, axis, cell_spacing, time_step, bias, slope_limiter, w, o, 0, 1)))
-            return rc;
+        a->dx = cell_spacing;
+        a->order = order;
+        const Op op{0, axis, time_step};
+        if ((rc = adjoint_op_dev(a, op, order, bias, slope_limiter, w, o, 0, 1))) return rc;
         return lam_download(a, lambda_in, 1);
+    });
+}
+
+int cosmos_adjoint_forcing(cosmos_adjoint* a, double cell_spacing, double time_step, const double* lambda_out,
+                           double* lambda_in) {
+    return adj_guarded(a, [&] {
+        if (a->physics != COSMOS_BLACKHOLE)
+            return adj_fail(a, COSMOS_ERR_ARGUMENT, "forcing needs a COSMOS_BLACKHOLE adjoint context");
+        if (!lambda_out || !lambda_in) return adj_fail(a, COSMOS_ERR_ARGUMENT, "null adjoint array");
+        double *w, *o;
+        int rc = current_base(a, w);
+        if (rc) return rc;
+        rc = cosmos_forcing(a->s, cell_spacing, time_step, 0.0, 1);
+        if (rc) return from_solver(a, rc);
+        if ((rc = current_base(a, o))) return rc;
+        if ((rc = lam_upload(a, lambda_out, 0))) return rc;
+        a->dx = cell_spacing;
+        const Op op{1, 0, time_step};
+        if ((rc = adjoint_op_dev(a, op, a->order, 0.0, 1, w, o, 0, 1))) return rc;
+        return lam_download(a, lambda_in, 1);
+    });
+}
+
+int cosmos_adjoint_set_subcycles(cosmos_adjoint* a, int subcycles) {
+    return adj_guarded(a, [&] {
+        if (subcycles < 0 || subcycles > 8) return adj_fail(a, COSMOS_ERR_ARGUMENT, "subcycles must be in [0, 8]");
+        a->subcycles = subcycles;
+        a->dts.clear();
+        return COSMOS_OK;
     });
 }
 
@@ -525,9 +630,7 @@ int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, do
         // the reverse pass recomputes segments with the exact kernels
         int rc0 = cosmos_set_mode(a->s, COSMOS_MODE_EXACT);
         if (rc0) return from_solver(a, rc0);
-        int axes[3];
-        double fac[3];
-        const int ns = step_sweeps(a, axes, fac);
+        Op ops[kMaxOps];
         for (int it = 0; it < n_steps; ++it) {
             if (it % a->every == 0) {
                 double* cp = nullptr;
@@ -540,10 +643,9 @@ int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, do
             int rc = cosmos_stable_time_step(a->s, cell_spacing, cfl, 0.0, INFINITY, it, &dt);
             if (rc) return from_solver(a, rc);
             a->dts.push_back(dt);
-            for (int k = 0; k < ns; ++k) {
-                rc = cosmos_sweep(a->s, order, axes[k], cell_spacing, sweep_dt(dt, fac[k]), bias, slope_limiter);
-                if (rc) return from_solver(a, rc);
-            }
+            const int ns = step_ops(a, dt, ops);
+            for (int k = 0; k < ns; ++k)
+                if ((rc = apply_op(a, ops[k]))) return rc;
         }
         ADJCHK(a, hipStreamSynchronize(stream_of(a)));
         return COSMOS_OK;
@@ -555,14 +657,12 @@ int cosmos_adjoint_replay(cosmos_adjoint* a, const double* prim_in, double* prim
         if (a->dts.empty()) return adj_fail(a, COSMOS_ERR_STATE, "no recorded trajectory (cosmos_adjoint_forward)");
         int rc = cosmos_upload(a->s, prim_in);
         if (rc) return from_solver(a, rc);
-        int axes[3];
-        double fac[3];
-        const int ns = step_sweeps(a, axes, fac);
-        for (double dt : a->dts)
-            for (int k = 0; k < ns; ++k) {
-                rc = cosmos_sweep(a->s, a->order, axes[k], a->dx, sweep_dt(dt, fac[k]), a->bias, a->limiter);
-                if (rc) return from_solver(a, rc);
-            }
+        Op ops[kMaxOps];
+        for (double dt : a->dts) {
+            const int ns = step_ops(a, dt, ops);
+            for (int k = 0; k < ns; ++k)
+                if ((rc = apply_op(a, ops[k]))) return rc;
+        }
         rc = cosmos_download(a->s, prim_out);
         return from_solver(a, rc);
     });
@@ -572,9 +672,8 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
     return adj_guarded(a, [&] {
         if (a->dts.empty()) return adj_fail(a, COSMOS_ERR_STATE, "no recorded trajectory (cosmos_adjoint_forward)");
         if (!grad_out || !grad_in) return adj_fail(a, COSMOS_ERR_ARGUMENT, "null gradient array");
-        int axes[3];
-        double fac[3];
-        const int ns = step_sweeps(a, axes, fac);
+        Op ops[kMaxOps];
+        const int ns = step_ops(a, 1.0, ops);  // operations per step (the count does not depend on dt)
         const int nsteps = (int)a->dts.size();
         // segment buffers: the output of every sweep of a segment (its first
         // input is the checkpoint itself)
@@ -593,19 +692,22 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
             // recompute the segment from its checkpoint: state q + 1 = sweep q of state q
             auto state = [&](int q) -> double* { return q == 0 ? a->checkpoints[sgi] : a->seg[q - 1]; };
             int q = 0;
-            for (int it = s0; it < s1; ++it)
+            for (int it = s0; it < s1; ++it) {
+                step_ops(a, a->dts[it], ops);
                 for (int k = 0; k < ns; ++k, ++q)
-                    if ((rc = forward_sweep_dev(a, axes[k], sweep_dt(a->dts[it], fac[k]), state(q), state(q + 1))))
-                        return rc;
+                    if ((rc = forward_op_dev(a, ops[k], state(q), state(q + 1)))) return rc;
+            }
             // reverse through the segment
-            for (int it = s1 - 1; it >= s0; --it)
+            for (int it = s1 - 1; it >= s0; --it) {
+                step_ops(a, a->dts[it], ops);
                 for (int k = ns - 1; k >= 0; --k) {
                     const int qi = (it - s0) * ns + k;
-                    rc = adjoint_sweep_dev(a, a->order, axes[k], a->dx, sweep_dt(a->dts[it], fac[k]), a->bias,
-                                           a->limiter, state(qi), state(qi + 1), cur, cur ^ 1);
+                    rc = adjoint_op_dev(a, ops[k], a->order, a->bias, a->limiter, state(qi), state(qi + 1), cur,
+                                        cur ^ 1);
                     if (rc) return rc;
                     cur ^= 1;
                 }
+            }
         }
         return lam_download(a, grad_in, cur);
     });

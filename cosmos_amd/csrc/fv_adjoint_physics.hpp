@@ -27,7 +27,7 @@ struct Coef {
 };
 
 // primitives (rho, u, v, w, p) -> conserved (rho, m, E)
-__device__ __forceinline__ void p2c(const double* W, double* U, const Coef& C) {
+CFV_HD void p2c(const double* W, double* U, const Coef& C) {
     const double r = W[0];
     U[0] = r;
     U[1] = r * W[1];
@@ -37,7 +37,7 @@ __device__ __forceinline__ void p2c(const double* W, double* U, const Coef& C) {
 }
 
 // gW = (dU/dW)^T gU at W
-__device__ __forceinline__ void vjp_p2c(const double* W, const double* gU, double* gW, const Coef& C) {
+CFV_HD void vjp_p2c(const double* W, const double* gU, double* gW, const Coef& C) {
     const double r = W[0];
     const double q = 0.5 * ((W[1] * W[1] + W[2] * W[2]) + W[3] * W[3]);
     gW[0] = gU[0] + gU[1] * W[1] + gU[2] * W[2] + gU[3] * W[3] + gU[4] * q;
@@ -48,7 +48,7 @@ __device__ __forceinline__ void vjp_p2c(const double* W, const double* gU, doubl
 }
 
 // primitives of U (rho, velocity, p) with 1/rho
-__device__ __forceinline__ void c2p(const double* U, double* W, double& irho, const Coef& C) {
+CFV_HD void c2p(const double* U, double* W, double& irho, const Coef& C) {
     irho = 1.0 / U[0];
     W[0] = U[0];
     W[1] = U[1] * irho;
@@ -58,7 +58,7 @@ __device__ __forceinline__ void c2p(const double* U, double* W, double& irho, co
 }
 
 // gU = (dW/dU)^T gW, the Jacobian of c2p taken at the primitives W = c2p(U)
-__device__ __forceinline__ void vjp_c2p(const double* W, const double* gW, double* gU, const Coef& C) {
+CFV_HD void vjp_c2p(const double* W, const double* gW, double* gU, const Coef& C) {
     const double ir = 1.0 / W[0];
     const double q = 0.5 * ((W[1] * W[1] + W[2] * W[2]) + W[3] * W[3]);
     const double gp = gW[4] * C.gm1;
@@ -71,7 +71,7 @@ __device__ __forceinline__ void vjp_c2p(const double* W, const double* gW, doubl
 
 // Euler flux along AXIS (0: x, 1: y) of conserved U
 template <int AXIS>
-__device__ __forceinline__ void flux(const double* U, double* F, const Coef& C) {
+CFV_HD void flux(const double* U, double* F, const Coef& C) {
     double W[5], ir;
     c2p(U, W, ir, C);
     const double q = W[1 + AXIS];
@@ -85,7 +85,7 @@ __device__ __forceinline__ void flux(const double* U, double* F, const Coef& C) 
 
 // gU += (dF/dU)^T gF at U
 template <int AXIS>
-__device__ __forceinline__ void vjp_flux_acc(const double* U, const double* gF, double* gU, const Coef& C) {
+CFV_HD void vjp_flux_acc(const double* U, const double* gF, double* gU, const Coef& C) {
     double W[5], ir;
     c2p(U, W, ir, C);
     const double r = W[0], q = W[1 + AXIS], p = W[4], E = U[4];
@@ -109,7 +109,7 @@ __device__ __forceinline__ void vjp_flux_acc(const double* U, const double* gF, 
 
 // Limiter value phi(r) and its derivative on the branch the forward selects
 // (SlopeLimiters::computeSlopeLimiter, Solvers/SlopeLimiters.cpp:3-69).
-__device__ __forceinline__ double limiter_d(double r, double bias, int limiter, double& dphi) {
+CFV_HD double limiter_d(double r, double bias, int limiter, double& dphi) {
     const double den = (1.0 - bias) + (1.0 + bias) * r;
     const double xi = 2.0 / den;
     const double dxi = -2.0 * (1.0 + bias) / (den * den);
@@ -152,7 +152,7 @@ __device__ __forceinline__ double limiter_d(double r, double bias, int limiter, 
 
 // One component of the limited slope Delta = s phi(r) (SlopeLimiters::computeSlopeVector,
 // Solvers/SlopeLimiters.cpp:71-101) and its partial derivatives by dl = m - l and dr = r - m.
-__device__ __forceinline__ double slope_d(double ul, double um, double ur, double bias, int limiter, double& d_dl,
+CFV_HD double slope_d(double ul, double um, double ur, double bias, int limiter, double& d_dl,
                                           double& d_dr) {
     const double dl = um - ul, dr = ur - um;
     const double s = 0.5 * (1.0 + bias) * dl + 0.5 * (1.0 - bias) * dr;
@@ -172,7 +172,7 @@ __device__ __forceinline__ double slope_d(double ul, double um, double ur, doubl
 // Solvers/EulerSolvers.cpp:174-223): the evolved left (side 0) and right
 // (side 1) face states, conserved.
 template <int AXIS>
-__device__ __forceinline__ void half_step(const double* UL, const double* UM, const double* UR, const Coef& C,
+CFV_HD void half_step(const double* UL, const double* UM, const double* UR, const Coef& C,
                                           double* a, double* b) {
     double xl[5], xr[5], fl[5], fr[5];
 #pragma unroll
@@ -195,7 +195,7 @@ __device__ __forceinline__ void half_step(const double* UL, const double* UM, co
 // Adjoint of the half step: from the adjoints ga, gb of its two outputs,
 // the contributions to the adjoints of UL, UM, UR.
 template <int AXIS>
-__device__ __forceinline__ void vjp_half_step(const double* UL, const double* UM, const double* UR, const double* ga,
+CFV_HD void vjp_half_step(const double* UL, const double* UM, const double* UR, const double* ga,
                                               const double* gb, const Coef& C, double* gL, double* gM, double* gR) {
     double xl[5], xr[5], dd0[5], dd1[5];
 #pragma unroll
@@ -232,7 +232,7 @@ __device__ __forceinline__ void vjp_half_step(const double* UL, const double* UM
 // is not needed forward; its adjoint from the flux adjoint phi gives the
 // adjoints of the left and right states.
 template <int AXIS>
-__device__ __forceinline__ void vjp_force(const double* UL, const double* UR, const double* phi, const Coef& C,
+CFV_HD void vjp_force(const double* UL, const double* UR, const double* phi, const Coef& C,
                                           double* gUL, double* gUR) {
     double FL[5], FR[5], us[5];
     flux<AXIS>(UL, FL, C);

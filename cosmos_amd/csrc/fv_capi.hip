@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/cosmos_fv.h"
+#include "fv_gr_adjoint_launch.hpp"
 #include "fv_kernels.hpp"
 
 using cfv::ConstPlanes;
@@ -272,16 +273,23 @@ int ensure_gr_flux(cosmos_solver* s) {
     return COSMOS_OK;
 }
 
-int gr_sweep(cosmos_solver* s, int order, int axis, double h, double dt, double bias, int limiter) {
+int gr_sweep_setup(cosmos_solver* s, int order, double h, double dt, double bias, int limiter, cfv::GRLaunch& L) {
     int rc = ensure_gr_flux(s);
     if (rc) return rc;
-    cfv::GRLaunch L = gr_launch(s, h);
+    L = gr_launch(s, h);
     L.order = order;
     L.bias = bias;
     L.limiter = limiter;
     L.coef.dt_dx = dt / h;
     L.coef.dx_dt = h / dt;
     L.coef.hdt_dx = 0.5 * (dt / h);
+    return COSMOS_OK;
+}
+
+int gr_sweep(cosmos_solver* s, int order, int axis, double h, double dt, double bias, int limiter) {
+    cfv::GRLaunch L;
+    int rc = gr_sweep_setup(s, order, h, dt, bias, limiter, L);
+    if (rc) return rc;
     hipError_t e = timed(s, axis == 0 ? kProfSweepX : kProfSweepY, [&] { return cfv::launch_gr_sweep(L, axis, s->stream); });
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("GR sweep launch: ") + hipGetErrorString(e));
     s->cur ^= 1;
@@ -291,7 +299,7 @@ int gr_sweep(cosmos_solver* s, int order, int axis, double h, double dt, double 
 // The stage fractions of evolveConservedVariableVector{,2D}
 // (Solvers/BlackHoleEulerForcingSolver.cpp:21-53, 98-167), each times dt / h as
 // computeFractionalEvolutionVector forms it (Solvers/EulerSolvers.cpp:163-167).
-int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) {
+cfv::GRLaunch gr_forcing_setup(cosmos_solver* s, double h, double dt, double bias, int limiter) {
     cfv::GRLaunch L = gr_launch(s, h);
     L.bias = bias;
     L.limiter = limiter;
@@ -301,6 +309,11 @@ int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) 
         L.frac_dt_h[k] = (s->dim == 1 ? f1[k] : fx[k]) * (dt / h);
         L.frac_dt_h_y[k] = fy[k] * (dt / h);
     }
+    return L;
+}
+
+int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) {
+    cfv::GRLaunch L = gr_forcing_setup(s, h, dt, bias, limiter);
     hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_gr_forcing(L, s->stream); });
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("GR forcing launch: ") + hipGetErrorString(e));
     s->cur ^= 1;
@@ -308,6 +321,21 @@ int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) 
 }
 
 }  // namespace
+
+namespace cfv {
+
+int gr_sweep_launch(cosmos_solver* s, int order, double h, double dt, double bias, int limiter, GRLaunch* L) {
+    if (!s || s->physics != COSMOS_BLACKHOLE) return COSMOS_ERR_ARGUMENT;
+    return gr_sweep_setup(s, order, h, dt, bias, limiter, *L);
+}
+
+int gr_forcing_launch(cosmos_solver* s, double h, double dt, double bias, int limiter, GRLaunch* L) {
+    if (!s || s->physics != COSMOS_BLACKHOLE) return COSMOS_ERR_ARGUMENT;
+    *L = gr_forcing_setup(s, h, dt, bias, limiter);
+    return COSMOS_OK;
+}
+
+}  // namespace cfv
 
 extern "C" {
 
@@ -424,6 +452,7 @@ int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations) 
             return fail(s, COSMOS_ERR_ARGUMENT, "con2prim tolerance must be > 0 and max_iterations >= 1");
         s->P.c2p_tol = tolerance;
         s->P.c2p_maxit = max_iterations;
+        if (s->physics == COSMOS_BLACKHOLE) s->P.gr_tol = tolerance;
         return COSMOS_OK;
     });
 }

@@ -38,7 +38,7 @@ struct KSPoint {
     double r, dx, dy, dz, H, l0, l1, l2;
 };
 
-__device__ __forceinline__ KSPoint ks_point(const Hole& b, double x, double y, double z) {
+CFV_HD KSPoint ks_point(const Hole& b, double x, double y, double z) {
     KSPoint k;
     k.dx = x - b.x;
     k.dy = y - b.y;
@@ -61,13 +61,13 @@ __device__ __forceinline__ KSPoint ks_point(const Hole& b, double x, double y, d
 
 // The spacetime 4-vector's second spatial component with the reference's
 // (r + (y - y_BH)) term (computeKerrSchildSpacetimeVector :92-113).
-__device__ __forceinline__ double ks_l4y(const Hole& b, const KSPoint& k) {
+CFV_HD double ks_l4y(const Hole& b, const KSPoint& k) {
     return (-((k.r + k.dy) - (b.aM * k.dx))) / ((k.r * k.r) + b.ma);
 }
 
 // Determinant of a 3x3 matrix by cofactor expansion along row 0
 // (MatrixAlgebra::computeMatrixDeterminant, Mathematics/MatrixAlgebra.cpp:106-143).
-__device__ __forceinline__ double det3(const double* m) {
+CFV_HD double det3(const double* m) {
     double d = 0.0;
     d += m[0] * ((m[4] * m[8]) - (m[7] * m[5]));
     d -= m[1] * ((m[3] * m[8]) - (m[6] * m[5]));
@@ -76,7 +76,7 @@ __device__ __forceinline__ double det3(const double* m) {
 }
 
 // gamma_ij = delta_ij - 2 H l_i l_j (computeSpatialMetricTensor :153-186).
-__device__ __forceinline__ void spatial_metric(const KSPoint& k, double* g) {
+CFV_HD void spatial_metric(const KSPoint& k, double* g) {
     const double l[3] = {k.l0, k.l1, k.l2};
     double H2 = 2.0 * k.H;
 #pragma unroll
@@ -86,10 +86,10 @@ __device__ __forceinline__ void spatial_metric(const KSPoint& k, double* g) {
 }
 
 // lapse 1 / sqrt(1 - 2H) (:278-281)
-__device__ __forceinline__ double lapse_of(const KSPoint& k) { return 1.0 / __builtin_sqrt(1.0 - (2.0 * k.H)); }
+CFV_HD double lapse_of(const KSPoint& k) { return 1.0 / __builtin_sqrt(1.0 - (2.0 * k.H)); }
 
 // shift (2H / (1 - 2H)) l (:283-296)
-__device__ __forceinline__ void shift_of(const KSPoint& k, double* be) {
+CFV_HD void shift_of(const KSPoint& k, double* be) {
     double f = (2.0 * k.H) / (1.0 - (2.0 * k.H));
     be[0] = f * k.l0;
     be[1] = f * k.l1;
@@ -104,7 +104,7 @@ struct Geo {
     double beta[3];  // shift
 };
 
-__device__ __forceinline__ Geo geo_at(const Hole& b, double x, double y, double z) {
+CFV_HD Geo geo_at(const Hole& b, double x, double y, double z) {
     Geo G;
     KSPoint k = ks_point(b, x, y, z);
     spatial_metric(k, G.g);
@@ -115,7 +115,7 @@ __device__ __forceinline__ Geo geo_at(const Hole& b, double x, double y, double 
 }
 
 // inExcisionRegion (:372-386)
-__device__ __forceinline__ bool excised(const Hole& b, double x, double y, double z) {
+CFV_HD bool excised(const Hole& b, double x, double y, double z) {
     double dx = x - b.x, dy = y - b.y, dz = z - b.z;
     return __builtin_sqrt(((dx * dx) + (dy * dy)) + (dz * dz)) <= b.r_exc;
 }
@@ -126,7 +126,7 @@ __device__ __forceinline__ bool excised(const Hole& b, double x, double y, doubl
 
 // computeConservedVariableVector (:118-157): densitised by sqrt(det gamma),
 // v^2 = gamma_ij v^i v^j, W clamped, tau with W (not W^2).
-__device__ __forceinline__ void gr_p2c(const double* W, const Geo& G, double* U, const PhysParams& P) {
+CFV_HD void gr_p2c(const double* W, const Geo& G, double* U, const PhysParams& P) {
     double vsq = 0.0;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -145,13 +145,23 @@ __device__ __forceinline__ void gr_p2c(const double* W, const Geo& G, double* U,
 }
 
 // Primitive recovery shared by setConservedVariableVector (:30-103), the
-// fluxes (:159-357) and the source term (:359-517): divide by sqrt(det gamma),
-// then the special-relativistic quartic Newton iteration from 1.0 with the
-// flat |S|^2, stopping when |x - x_new| < tol and keeping the previous
-// iterate; rho and p floored at 1e-8 after the velocities are formed.
-// R = (rho, v^x, v^y, v^z, p); lf = W; h = specific enthalpy.
-__device__ __forceinline__ void gr_con2prim(const double* U, double sdet, double tol, const PhysParams& P, double* R,
-                                            double& lf_out, double& h_out) {
+// fluxes (:159-357) and the source term (:359-517) (iteration cap: the
+// reference's 1000 unless configured): divide by sqrt(det gamma), then the
+// special-relativistic quartic Newton iteration from 1.0 with the flat |S|^2,
+// stopping when |x - x_new| < tol and keeping the previous iterate; rho and p
+// floored at 1e-8 after the velocities are formed.  C2PRec keeps the
+// intermediate values the reverse-mode code differentiates through.
+struct C2PRec {
+    double D, S[3], tau;   // undensitised conserved state
+    double q, sq;          // (tau + D)^2 - |S|^2 and the root used (floored q: sqrt(1e-8))
+    double dc, ec;         // D / sq, (D + tau) / sq
+    double c2, c4, eta;    // quartic coefficients (c1 follows from dc)
+    double x, fd;          // the kept iterate and f'(x) there
+    double lf, h, rho, rhw, p;
+    double R[5];           // (rho, v^x, v^y, v^z, p), rho and p floored
+};
+
+CFV_HD void gr_con2prim_rec(const double* U, double sdet, double tol, const PhysParams& P, C2PRec& c) {
     double is = 1.0 / sdet;
     double D = div_by(U[0], sdet, is);
     double Sx = div_by(U[1], sdet, is);
@@ -169,12 +179,12 @@ __device__ __forceinline__ void gr_con2prim(const double* U, double sdet, double
     double eta = (2.0 * dc) * P.sr_gm1g;
     double c0 = P.sr_c0;
     double c2x2 = 2.0 * c2, c4x4 = 4.0 * c4, e3c4 = (3.0 * eta) * c4;
-    double x = 1.0;
-    for (int it = 0; it < 1000; ++it) {
+    double x = 1.0, fd = 1.0;
+    for (int it = 0; it < P.c2p_maxit; ++it) {
         double x2 = x * x;
         double x3 = x2 * x;
         double f = ((((c4 * x3) * (x - eta)) + (c2 * x2)) + (c1 * x)) + c0;
-        double fd = ((c1 + (c2x2 * x)) + (c4x4 * x3)) - (e3c4 * x2);
+        fd = ((c1 + (c2x2 * x)) + (c4x4 * x3)) - (e3c4 * x2);
         double xn = x - (f / fd);
         if (__builtin_fabs(x - xn) < tol) break;
         x = xn;
@@ -184,29 +194,58 @@ __device__ __forceinline__ void gr_con2prim(const double* U, double sdet, double
     double rho = D / lf;
     double rhw = (rho * h) * (lf * lf);
     double irhw = 1.0 / rhw;
-    R[1] = div_by(Sx, rhw, irhw);
-    R[2] = div_by(Sy, rhw, irhw);
-    R[3] = div_by(Sz, rhw, irhw);
+    c.R[1] = div_by(Sx, rhw, irhw);
+    c.R[2] = div_by(Sy, rhw, irhw);
+    c.R[3] = div_by(Sz, rhw, irhw);
     double p = (rhw - D) - tau;
-    R[0] = (rho < 1e-8) ? 1e-8 : rho;
-    R[4] = (p < 1e-8) ? 1e-8 : p;
-    lf_out = lf;
-    h_out = h;
+    c.R[0] = (rho < 1e-8) ? 1e-8 : rho;
+    c.R[4] = (p < 1e-8) ? 1e-8 : p;
+    c.D = D;
+    c.S[0] = Sx;
+    c.S[1] = Sy;
+    c.S[2] = Sz;
+    c.tau = tau;
+    c.q = q;
+    c.sq = sq;
+    c.dc = dc;
+    c.ec = ec;
+    c.c2 = c2;
+    c.c4 = c4;
+    c.eta = eta;
+    c.x = x;
+    c.fd = fd;
+    c.lf = lf;
+    c.h = h;
+    c.rho = rho;
+    c.rhw = rhw;
+    c.p = p;
 }
 
-// setConservedVariableVector: the primitives (velocities unfloored).
-__device__ __forceinline__ void gr_c2p(const double* U, double sdet, const PhysParams& P, double* W) {
+// R = (rho, v^x, v^y, v^z, p); lf = W; h = specific enthalpy.
+CFV_HD void gr_con2prim(const double* U, double sdet, double tol, const PhysParams& P, double* R, double& lf_out,
+                        double& h_out) {
+    C2PRec c;
+    gr_con2prim_rec(U, sdet, tol, P, c);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) R[k] = c.R[k];
+    lf_out = c.lf;
+    h_out = c.h;
+}
+
+CFV_HD double gr_tol(const PhysParams& P, double reference) { return P.gr_tol > 0.0 ? P.gr_tol : reference; }
+
+CFV_HD void gr_c2p(const double* U, double sdet, const PhysParams& P, double* W) {
     double lf, h;
-    gr_con2prim(U, sdet, 1e-8, P, W, lf, h);
+    gr_con2prim(U, sdet, gr_tol(P, 1e-8), P, W, lf, h);
 }
 
 // compute{X,Y}FluxVector(cons) (:159-257 x, tolerance 1e-10; :259-357 y, 1e-8):
 // alpha sqrt(gamma) (rho W vt, rho h W^2 v_i vt + p delta_i, (rho h W^2 - p - rho W) vt + p v)
 // with vt = v^axis - beta^axis / alpha, from the floored density and pressure.
 template <int AXIS>
-__device__ __forceinline__ void gr_flux(const double* U, const Geo& G, const PhysParams& P, double* F) {
+CFV_HD void gr_flux(const double* U, const Geo& G, const PhysParams& P, double* F) {
     double R[5], lf, h;
-    gr_con2prim(U, G.sdet, AXIS == 0 ? 1e-10 : 1e-8, P, R, lf, h);
+    gr_con2prim(U, G.sdet, gr_tol(P, AXIS == 0 ? 1e-10 : 1e-8), P, R, lf, h);
     double rho = R[0], p = R[4];
     double as = G.alpha * G.sdet;
     double vn = R[1 + AXIS];
@@ -229,7 +268,7 @@ __device__ __forceinline__ void gr_flux(const double* U, const Geo& G, const Phy
 // 4x4 inverse by Cayley-Hamilton (MatrixAlgebra::computeMatrixInverse,
 // Mathematics/MatrixAlgebra.cpp:145-194) with the cofactor determinant;
 // matrix products accumulate from 0.0 over k (:56-82).
-__device__ __forceinline__ double det4(const double* m) {
+CFV_HD double det4(const double* m) {
     double d = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -244,7 +283,7 @@ __device__ __forceinline__ double det4(const double* m) {
     return d;
 }
 
-__device__ __forceinline__ void inverse4(const double* m, double* out) {
+CFV_HD void inverse4(const double* m, double* out) {
     double det = det4(m);
     double tr = 0.0;
     tr += m[0];
@@ -308,7 +347,7 @@ struct SourceGeo {
 // (computeLapseFunctionDerivative :298-310, computeKerrSchildScalarDerivative :115-127)
 // and (1/h) (v(x + h/2) - v(x - h/2)) for vectors and tensors
 // (:129-151, :250-276, :312-334), at offsets x + (0.5 * h).
-__device__ __forceinline__ void source_geo(const Hole& b, double x, double y, double z, double h, SourceGeo& S) {
+CFV_HD void source_geo(const Hole& b, double x, double y, double z, double h, SourceGeo& S) {
     KSPoint c = ks_point(b, x, y, z);
     S.alpha = lapse_of(c);
     shift_of(c, S.beta);
@@ -370,9 +409,9 @@ __device__ __forceinline__ void source_geo(const Hole& b, double x, double y, do
 // u^mu (:438-442), T^{mu nu} = rho h u^mu u^nu + p g^{mu nu} (:444-453), the
 // momentum source from d gamma, beta, T, d beta and d alpha (:478-494), the
 // energy source from K and d alpha (:496-507), all times alpha sqrt(gamma) (:509).
-__device__ __forceinline__ void gr_source(const double* U, const SourceGeo& S, const PhysParams& P, double* out) {
+CFV_HD void gr_source(const double* U, const SourceGeo& S, const PhysParams& P, double* out) {
     double R[5], lf, hh;
-    gr_con2prim(U, S.sdet, 1e-8, P, R, lf, hh);
+    gr_con2prim(U, S.sdet, gr_tol(P, 1e-8), P, R, lf, hh);
     double rho = R[0], p = R[4];
     double al = S.alpha;
     double u[4];
@@ -426,7 +465,7 @@ __device__ __forceinline__ void gr_source(const double* U, const SourceGeo& S, c
 // largest signed material / fast / slow eigenvalue over the three
 // directions, floored at 0.  Only the diagonal of the Cayley-Hamilton
 // inverse spatial metric is used.
-__device__ __forceinline__ double gr_sound_speed(const double* W, const Hole& b, double x, double y, double z,
+CFV_HD double gr_sound_speed(const double* W, const Hole& b, double x, double y, double z,
                                                  const PhysParams& P) {
     KSPoint k = ks_point(b, x, y, z);
     double al = lapse_of(k);

@@ -1,0 +1,347 @@
+// gfx950 kernels of the adjoint (reverse mode) of the general-relativistic
+// update: the FORCE / SLIC sweep with excision
+// (Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-143, Solvers/BlackHoleEulerFirstOrderSolver.cpp:3-191)
+// and the RK4 source-term forcing step
+// (Solvers/BlackHoleEulerForcingSolver.cpp:3-250).
+//
+// Sweep: the forward fluxes are in the flux planes (the caller re-runs the
+// forward sweep first).  k_gra_cells gives each cell's adjoint through its
+// update (c2p, then p2c back to the input primitives; excised stencils pass
+// their adjoint through unchanged) and stores the adjoint of the updated
+// conserved vector; k_gra_ifaces forms each interface flux's adjoint from its
+// two cells' and scatters it through the FORCE / SLIC stencil onto the input
+// cells with FP64 atomic adds (accumulation order is unspecified, so results
+// may differ in the last bits from run to run).
+//
+// Forcing: one thread per cell re-runs the cell's RK4 chain, keeping every
+// stage's states, then walks it back; its own and its four neighbours'
+// adjoints are accumulated with atomic adds into a zeroed output.
+#include "fv_gr_adjoint.hpp"
+#include "fv_gr_adjoint_launch.hpp"
+
+namespace cfv {
+namespace {
+
+constexpr int kGAT = 128;
+
+__device__ __forceinline__ int mirror2_a(int c, int n) {
+    if (c < 0) c = -c - 1;
+    if (c >= n) c = 2 * n - 1 - c;
+    c = c < 0 ? 0 : c;
+    return c >= n ? n - 1 : c;
+}
+
+__device__ __forceinline__ int clamp1_a(int c, int n) {
+    c = c < 0 ? 0 : c;
+    return c >= n ? n - 1 : c;
+}
+
+__device__ __forceinline__ void ld5(const ConstPlanes& in, long idx, double* W) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) W[k] = in.p[k][idx];
+}
+
+__device__ __forceinline__ void st5(const Planes& out, long idx, const double* W) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) out.p[k][idx] = W[k];
+}
+
+__device__ __forceinline__ void add5(const Planes& out, long idx, const double* g) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (g[k] != 0.0) atomicAdd(out.p[k] + idx, g[k]);
+}
+
+__device__ __forceinline__ void zero5(double* g) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) g[k] = 0.0;
+}
+
+// The forward's excision test (stencil_clear in the forward kernels).
+__device__ __forceinline__ bool clear_of(const Hole& b, int dim, int i, int j, int w, double h) {
+    double y = dim == 2 ? i * h : 0.0;
+    if (excised(b, j * h, y, 0.0)) return false;
+    for (int d = 1; d <= w; ++d) {
+        if (excised(b, (j - d) * h, y, 0.0) || excised(b, (j + d) * h, y, 0.0)) return false;
+        if (dim == 2 && (excised(b, j * h, (i - d) * h, 0.0) || excised(b, j * h, (i + d) * h, 0.0))) return false;
+    }
+    return true;
+}
+
+// Per cell of a sweep: the adjoint through the conservative update.
+template <int ORDER, int AXIS>
+__global__ __launch_bounds__(kGAT) void k_gra_cells(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= (long)L.nx * L.ny) return;
+    int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
+    long cell = (long)i * L.pitch + j;
+    double lam[5];
+    ld5(A.lam, cell, lam);
+    double gun[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (clear_of(L.hole, L.dim, i, j, ORDER == 2 ? 2 : 1, L.h)) {
+        double W[5], U[5], un[5];
+        ld5(L.in, cell, W);
+        const Geo g = geo_at(L.hole, j * L.h, L.dim == 2 ? i * L.h : 0.0, 0.0);
+        gr_p2c(W, g, U, L.P);
+        long fl = (long)i * L.fpitch + j;
+        long fr = AXIS == 0 ? fl + 1 : fl + L.fpitch;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) un[k] = U[k] + (L.coef.dt_dx * (L.flux.p[k][fl] - L.flux.p[k][fr]));
+        adjgr::vjp_c2p(un, g.sdet, gr_tol(L.P, 1e-8), L.P, lam, gun);
+        double gW[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        adjgr::vjp_p2c(W, g, L.P, gun, gW);
+        st5(A.lam_in, cell, gW);
+    } else {
+        st5(A.lam_in, cell, lam);
+    }
+    st5(A.gbuf, cell, gun);
+}
+
+// Per interface: the flux adjoint dt/dx (g_right - g_left) of the updated
+// conserved vectors of its two cells, back through the FORCE / SLIC stencil.
+template <int ORDER, int AXIS>
+__global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    const int n = AXIS == 0 ? L.nx : L.ny;
+    const long lines = AXIS == 0 ? L.ny : L.nx;
+    const long per = AXIS == 0 ? (long)(n + 1) : lines;
+    const long total = (long)(n + 1) * lines;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= total) return;
+    int c, i, j;
+    if (AXIS == 0) {
+        i = (int)(idx / per);
+        c = (int)(idx - (long)i * per);
+        j = c;
+    } else {
+        c = (int)(idx / per);
+        j = (int)(idx - (long)c * per);
+        i = c;
+    }
+    auto cell_at = [&](int cc) -> long { return AXIS == 0 ? (long)i * L.pitch + cc : (long)cc * L.pitch + j; };
+    // un_c = U_c + k (F_c - F_{c+1}): dF_c gets k (g_c - g_{c-1})
+    double gF[5];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double gr = c < n ? A.gbuf.p[k][cell_at(c)] : 0.0;
+        const double gl = c > 0 ? A.gbuf.p[k][cell_at(c - 1)] : 0.0;
+        gF[k] = L.coef.dt_dx * (gr - gl);
+        any |= gF[k] != 0.0;
+    }
+    if (!any) return;
+    auto at = [&](int cc) -> long { return cell_at(ORDER == 2 ? mirror2_a(cc, n) : clamp1_a(cc, n)); };
+    double x, y;
+    if (AXIS == 0) {
+        x = (c - 0.5) * L.h;
+        y = L.dim == 2 ? i * L.h : 0.0;
+    } else {
+        x = j * L.h;
+        y = (c - 0.5) * L.h;
+    }
+    const Hole& b = L.hole;
+    if (ORDER == 1) {
+        double WL[5], WR[5], gL[5], gR[5];
+        ld5(L.in, at(c - 1), WL);
+        ld5(L.in, at(c), WR);
+        zero5(gL);
+        zero5(gR);
+        adjgr::vjp_force<AXIS>(WL, WR, x, y, b, L.h, L.coef, L.P, gF, gL, gR);
+        add5(A.lam_in, at(c - 1), gL);
+        add5(A.lam_in, at(c), gR);
+        return;
+    }
+    const double hh = L.h * 0.5;
+    const double xl = AXIS == 0 ? x - hh : x, yl = AXIS == 0 ? y : y - hh;
+    const double xr = AXIS == 0 ? x + hh : x, yr = AXIS == 0 ? y : y + hh;
+    double W0[5], W1[5], W2[5], W3[5], pr[5], pl[5];
+    ld5(L.in, at(c - 2), W0);
+    ld5(L.in, at(c - 1), W1);
+    ld5(L.in, at(c), W2);
+    ld5(L.in, at(c + 1), W3);
+    adjgr::half_fwd<AXIS>(W0, W1, W2, xl, yl, 1, b, L.h, L.coef, L.bias, L.limiter, L.P, pr);
+    adjgr::half_fwd<AXIS>(W1, W2, W3, xr, yr, 0, b, L.h, L.coef, L.bias, L.limiter, L.P, pl);
+    double gpr[5], gpl[5];
+    zero5(gpr);
+    zero5(gpl);
+    adjgr::vjp_force<AXIS>(pr, pl, x, y, b, L.h, L.coef, L.P, gF, gpr, gpl);
+    double g0[5], g1[5], g2[5], g3[5];
+    zero5(g0);
+    zero5(g1);
+    zero5(g2);
+    zero5(g3);
+    adjgr::vjp_half<AXIS>(W0, W1, W2, xl, yl, 1, b, L.h, L.coef, L.bias, L.limiter, L.P, gpr, g0, g1, g2);
+    adjgr::vjp_half<AXIS>(W1, W2, W3, xr, yr, 0, b, L.h, L.coef, L.bias, L.limiter, L.P, gpl, g1, g2, g3);
+    add5(A.lam_in, at(c - 2), g0);
+    add5(A.lam_in, at(c - 1), g1);
+    add5(A.lam_in, at(c), g2);
+    add5(A.lam_in, at(c + 1), g3);
+}
+
+// Per cell: the adjoint of the RK4 forcing (gr forcing kernel), into a zeroed lam_in.
+template <int DIM>
+__global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= (long)L.nx * L.ny) return;
+    int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
+    long cell = (long)i * L.pitch + j;
+    double lam[5];
+    ld5(A.lam, cell, lam);
+    const Hole& b = L.hole;
+    const double h = L.h;
+    if (!clear_of(b, DIM, i, j, 1, h)) {
+        add5(A.lam_in, cell, lam);
+        return;
+    }
+    const PhysParams& P = L.P;
+    const double tol = gr_tol(P, 1e-8);
+    const double x = j * h, y = DIM == 2 ? i * h : 0.0;
+    // forward chain, every stage kept
+    const long nb[4] = {(long)i * L.pitch + clamp1_a(j - 1, L.nx), (long)i * L.pitch + clamp1_a(j + 1, L.nx),
+                        DIM == 2 ? (long)clamp1_a(i - 1, L.ny) * L.pitch + j : cell,
+                        DIM == 2 ? (long)clamp1_a(i + 1, L.ny) * L.pitch + j : cell};
+    const double nbx[4] = {(j - 1) * h, (j + 1) * h, x, x};
+    const double nby[4] = {y, y, DIM == 2 ? (i - 1) * h : y, DIM == 2 ? (i + 1) * h : y};
+    const double sx[4] = {x - h, x + h, x, x};
+    const double sy[4] = {y, y, y - h, y + h};
+    const int nn = DIM == 2 ? 4 : 2;
+    double Wnb[4][5], unb[4][5], Wn[4][5];
+    for (int q = 0; q < nn; ++q) {
+        ld5(L.in, nb[q], Wnb[q]);
+        gr_p2c(Wnb[q], geo_at(b, nbx[q], nby[q], 0.0), unb[q], P);
+        gr_c2p(unb[q], geo_at(b, sx[q], sy[q], 0.0).sdet, P, Wn[q]);
+    }
+    double W[5], uM[5];
+    ld5(L.in, cell, W);
+    const Geo gc = geo_at(b, x, y, 0.0);
+    gr_p2c(W, gc, uM, P);
+    SourceGeo sg;
+    source_geo(b, x, y, 0.0, h, sg);
+    double kk[4][5], tmp[3][5], Wm[3][5], e2[3][5], e3[3][5], e1[3][5], um[3][5], src[5];
+    gr_source(uM, sg, P, src);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
+    for (int st = 0; st < 3; ++st) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) tmp[st][k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
+        gr_c2p(tmp[st], gc.sdet, P, Wm[st]);
+        if (DIM == 1) {
+            adjgr::frac_fwd<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, e1[st]);
+        } else {
+            adjgr::frac_fwd<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, e2[st]);
+            adjgr::frac_fwd<1>(L.frac_dt_h_y[st], Wn[2], e2[st], Wn[3], x, y, b, h, L.bias, L.limiter, P, e3[st]);
+            adjgr::frac_fwd<0>(L.frac_dt_h[st], Wn[0], e3[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, e1[st]);
+        }
+        gr_p2c(e1[st], gc, um[st], P);
+        gr_source(um[st], sg, P, src);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
+    }
+    double un[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
+    // reverse
+    double gun[5], guM[5], gkk[4][5], gWn[4][5];
+    zero5(gun);
+    adjgr::vjp_c2p(un, gc.sdet, tol, P, lam, gun);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        guM[k] = gun[k];
+        gkk[0][k] = (1.0 / 6.0) * gun[k];
+        gkk[1][k] = (1.0 / 6.0) * 2.0 * gun[k];
+        gkk[2][k] = (1.0 / 6.0) * 2.0 * gun[k];
+        gkk[3][k] = (1.0 / 6.0) * gun[k];
+    }
+    for (int q = 0; q < 4; ++q) zero5(gWn[q]);
+    for (int st = 2; st >= 0; --st) {
+        double gsrc[5], gum[5], ge1[5], gWm[5], gtmp[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gkk[st + 1][k];
+        zero5(gum);
+        adjgr::vjp_source(um[st], sg, P, gsrc, gum);
+        zero5(ge1);
+        adjgr::vjp_p2c(e1[st], gc, P, gum, ge1);
+        zero5(gWm);
+        if (DIM == 1) {
+            adjgr::vjp_frac<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, ge1, gWn[0],
+                               gWm, gWn[1]);
+        } else {
+            double ge3[5], ge2[5];
+            zero5(ge3);
+            zero5(ge2);
+            adjgr::vjp_frac<0>(L.frac_dt_h[st], Wn[0], e3[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, ge1, gWn[0],
+                               ge3, gWn[1]);
+            adjgr::vjp_frac<1>(L.frac_dt_h_y[st], Wn[2], e2[st], Wn[3], x, y, b, h, L.bias, L.limiter, P, ge3,
+                               gWn[2], ge2, gWn[3]);
+            adjgr::vjp_frac<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, ge2, gWn[0],
+                               gWm, gWn[1]);
+        }
+        zero5(gtmp);
+        adjgr::vjp_c2p(tmp[st], gc.sdet, tol, P, gWm, gtmp);
+        const double f = st < 2 ? 0.5 : 1.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            guM[k] += gtmp[k];
+            gkk[st][k] += f * gtmp[k];
+        }
+    }
+    {
+        double gsrc[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gkk[0][k];
+        adjgr::vjp_source(uM, sg, P, gsrc, guM);
+    }
+    double gW[5];
+    zero5(gW);
+    adjgr::vjp_p2c(W, gc, P, guM, gW);
+    add5(A.lam_in, cell, gW);
+    for (int q = 0; q < nn; ++q) {
+        double gu[5], gw[5];
+        zero5(gu);
+        zero5(gw);
+        adjgr::vjp_c2p(unb[q], geo_at(b, sx[q], sy[q], 0.0).sdet, tol, P, gWn[q], gu);
+        adjgr::vjp_p2c(Wnb[q], geo_at(b, nbx[q], nby[q], 0.0), P, gu, gw);
+        add5(A.lam_in, nb[q], gw);
+    }
+}
+
+dim3 gat_blocks(long work) { return dim3((unsigned)((work + kGAT - 1) / kGAT)); }
+
+}  // namespace
+
+hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s) {
+    const GRLaunch& L = A.L;
+    const long cells = (long)L.nx * L.ny;
+    const long ifaces = axis == 0 ? (long)(L.nx + 1) * L.ny : (long)(L.ny + 1) * L.nx;
+    const dim3 blk(kGAT);
+    if (L.order == 2) {
+        if (axis == 0) {
+            hipLaunchKernelGGL((k_gra_cells<2, 0>), gat_blocks(cells), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_ifaces<2, 0>), gat_blocks(ifaces), blk, 0, s, A);
+        } else {
+            hipLaunchKernelGGL((k_gra_cells<2, 1>), gat_blocks(cells), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_ifaces<2, 1>), gat_blocks(ifaces), blk, 0, s, A);
+        }
+    } else {
+        if (axis == 0) {
+            hipLaunchKernelGGL((k_gra_cells<1, 0>), gat_blocks(cells), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_ifaces<1, 0>), gat_blocks(ifaces), blk, 0, s, A);
+        } else {
+            hipLaunchKernelGGL((k_gra_cells<1, 1>), gat_blocks(cells), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_ifaces<1, 1>), gat_blocks(ifaces), blk, 0, s, A);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gr_adjoint_forcing(const GRAdjLaunch& A, hipStream_t s) {
+    const long cells = (long)A.L.nx * A.L.ny;
+    if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+    else hipLaunchKernelGGL((k_gra_forcing<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+    return hipGetLastError();
+}
+
+}  // namespace cfv

@@ -31,14 +31,19 @@ struct PhysParams {
     double sr_gg1;     // gamma / (gamma - 1.0)
     double sr_4gm1g;   // 4.0 * ((gamma - 1.0) / gamma)
     double inv_gm1;    // 1.0 / (gamma - 1.0) (fast mode only)
+    double gr_tol;     // > 0: every general-relativistic con2prim stops at this
+                       // tolerance instead of the reference's 1e-10 / 1e-8 (0: reference)
 };
+
+// Device functions that host-side checks also compile.
+#define CFV_HD __host__ __device__ __forceinline__
 
 // libstdc++'s std::min / std::max: min(a, b) = (b < a) ? b : a.
 __device__ __forceinline__ double min_std(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double max_std(double a, double b) { return (a < b) ? b : a; }
 
 // RN(a / b) from y = RN(1 / b).
-__device__ __forceinline__ double div_by(double a, double b, double y) {
+CFV_HD double div_by(double a, double b, double y) {
     double q = a * y;
     double r = __builtin_fma(b, q, -a);
     return __builtin_fma(-r, y, q);

@@ -95,6 +95,8 @@ SIGNATURES = {
     "cosmos_adjoint_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_double]),
     "cosmos_adjoint_destroy": (ctypes.c_int, [_handle]),
+    "cosmos_adjoint_forcing": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
+    "cosmos_adjoint_set_subcycles": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_adjoint_last_error": (ctypes.c_char_p, [_handle]),
     "cosmos_adjoint_solver": (_handle, [_handle]),
     "cosmos_adjoint_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

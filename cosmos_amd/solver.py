@@ -252,17 +252,45 @@ class Amr:
 
 
 class Adjoint:
-    """Reverse mode of the Newtonian update (cosmos_adjoint_*): a solver plus
-    the recorded trajectory (dt schedule and checkpoints) of a forward run."""
+    """Reverse mode of the Newtonian or black-hole update (cosmos_adjoint_*):
+    a solver plus the recorded trajectory (dt schedule and checkpoints) of a
+    forward run."""
 
-    def __init__(self, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4):
+    def __init__(self, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4, physics: int = NEWTONIAN):
         self._lib = native.load()
         self.dim = dim
+        self.physics = physics
         self.nx, self.ny = nx, (ny if dim == 2 else 1)
         h = ctypes.c_void_p()
-        native.check(self._lib.cosmos_adjoint_create(ctypes.byref(h), NEWTONIAN, dim, nx, self.ny, adiabatic_index),
+        native.check(self._lib.cosmos_adjoint_create(ctypes.byref(h), physics, dim, nx, self.ny, adiabatic_index),
                      None, adjoint=True)
         self._h = h
+
+    def _solver(self):
+        return self._lib.cosmos_adjoint_solver(self._h)
+
+    def set_spacetime(self, mass: float = 1.0, spin: float = 0.0, x: float = 0.5, y: float = 0.5,
+                      z: float = 0.0) -> None:
+        s = self._solver()
+        native.check(self._lib.cosmos_set_spacetime(s, mass, spin, x, y, z), s)
+
+    def set_excision(self, radius: float) -> None:
+        s = self._solver()
+        native.check(self._lib.cosmos_set_excision(s, radius), s)
+
+    def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
+        s = self._solver()
+        native.check(self._lib.cosmos_set_con2prim(s, tolerance, max_iterations), s)
+
+    def set_subcycles(self, subcycles: int) -> None:
+        self._check(self._lib.cosmos_adjoint_set_subcycles(self._h, subcycles))
+
+    def forcing(self, cell_spacing: float, time_step: float, lam_out: np.ndarray) -> np.ndarray:
+        """Advance the state by one RK4 forcing step; return (d out / d in)^T lam_out."""
+        lo = self._arr(lam_out)
+        li = np.empty(self.shape)
+        self._check(self._lib.cosmos_adjoint_forcing(self._h, cell_spacing, time_step, _dp(lo), _dp(li)))
+        return li
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -84,9 +84,15 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode);
  * relativistic). */
 int cosmos_set_y_tile(cosmos_solver* s, int columns);
 
-/* Special-relativistic con2prim controls (default: the reference's stopping
- * tolerance 1e-8 and cap of 1000 iterations,
- * RelativisticEuler/RelativisticEulerStateVector.cpp:59-80). */
+/* Con2prim controls.  Special relativistic: the stopping tolerance and
+ * iteration cap (default: the reference's 1e-8 and 1000,
+ * RelativisticEuler/RelativisticEulerStateVector.cpp:59-80).  Black hole: every
+ * con2prim (fluxes, source, state) stops at `tolerance` instead of the
+ * reference's fixed 1e-10 (x fluxes) / 1e-8 (elsewhere), with the same cap;
+ * without this call the reference's values apply.  A tight tolerance keeps
+ * the Newton iteration count from changing under small perturbations, which
+ * finite-difference checks of gradients need (no counterpart in the
+ * reference for the black-hole case). */
 int cosmos_set_con2prim(cosmos_solver* s, double tolerance, int max_iterations);
 
 /* The black hole of a COSMOS_BLACKHOLE solver: BlackHoleSpacetime(mass, spin,
@@ -221,15 +227,17 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
 
 /* ------------------------------------------------------------------------
- * Adjoint (reverse mode) of the Newtonian update (no counterpart in the
- * reference).  A cosmos_adjoint context owns a solver handle for an nx-by-ny
- * grid (dim 2) or an nx-cell line (dim 1).  It differentiates the update the
- * reference computes (SLIC or FORCE sweeps, dimensionally split X(dt/2) Y(dt)
- * X(dt/2) in 2D) with respect to the primitive state, with the time steps
- * held at the values the forward pass chose (a frozen dt schedule) and the
- * limiters differentiated on the branch the forward pass took (one-sided
- * derivatives at their kinks).  Host arrays are AoS primitives, as for
- * cosmos_upload.
+ * Adjoint (reverse mode) of the Newtonian and black-hole updates (no
+ * counterpart in the reference).  A cosmos_adjoint context owns a solver
+ * handle for an nx-by-ny grid (dim 2) or an nx-cell line (dim 1).  It
+ * differentiates the update the reference computes (SLIC or FORCE sweeps,
+ * dimensionally split X(dt/2) Y(dt) X(dt/2) in 2D; for the black hole the
+ * RK4 forcing steps on either side, Solvers/BlackHoleEulerSecondOrderSolver.cpp:181-218)
+ * with respect to the primitive state, with the time steps held at the
+ * values the forward pass chose (a frozen dt schedule), the limiters
+ * differentiated on the branch the forward pass took (one-sided derivatives
+ * at their kinks) and the con2prim Newton solves through their roots.
+ * Host arrays are AoS primitives, as for cosmos_upload.
  * ------------------------------------------------------------------------ */
 typedef struct cosmos_adjoint cosmos_adjoint;
 
@@ -257,6 +265,15 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
 /* The recorded dt schedule applied to another initial state (for finite-
  * difference checks); the solver's state is left at the result. */
 int cosmos_adjoint_replay(cosmos_adjoint* a, const double* prim_in, double* prim_out);
+/* Black hole (COSMOS_BLACKHOLE contexts; set the spacetime and excision on
+ * the context's solver).  One RK4 forcing step of the solver's state
+ * (bias 0 and van Leer, as the solve loop fixes them) and its adjoint, as
+ * cosmos_adjoint_sweep does for a sweep. */
+int cosmos_adjoint_forcing(cosmos_adjoint* a, double cell_spacing, double time_step, const double* lambda_out,
+                           double* lambda_in);
+/* Forcing sub-steps on each side of a step's sweeps (default 1; 0 to 8), as
+ * the subcyclingIterations argument of the black-hole solve loops. */
+int cosmos_adjoint_set_subcycles(cosmos_adjoint* a, int subcycles);
 
 /* Batched point-wise limiter functions on host arrays (evaluated on the device):
  *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)

@@ -1,0 +1,132 @@
+"""The adjoint of the black-hole (Kerr-Schild) update, checked against central
+finite differences of the forward map on the GPU:
+  - per operation, the dot-product test <lam, (S(W + h e) - S(W - h e)) / 2h>
+    = <S'^T lam, e> for an x and a y SLIC / FORCE sweep with excision and for
+    the RK4 forcing step;
+  - per trajectory (forcing, X Y X sweeps, forcing per step), the gradient of
+    L = sum (rho_N - rho*)^2 along seeded directions with the recorded dt
+    schedule held fixed (cosmos_adjoint_replay).
+Con2prim runs with a tight tolerance (1e-13) so that the Newton iteration
+counts, and so the forward map, do not jump inside the finite-difference
+stencil.  States are cell-wise random, which keeps the limiters' kinks away
+from a single operation's stencil: h = 1e-6 relative, bar 1e-6.  Trajectories
+use h = 1e-7 with the 1e-6 bar in at least three of four directions and 1e-3
+in all (a wrong adjoint is off by O(1))."""
+import numpy as np
+import pytest
+
+from cosmos_amd import BLACKHOLE, Adjoint
+
+pytestmark = pytest.mark.gpu
+
+G = 4.0 / 3.0
+TOL = 1e-6
+
+
+def bh_state(n, seed, ny=None):
+    """A perturbed, slowly moving medium around a hole at the centre of
+    [0, 40]^2 in cell coordinates (cell (i, j) at (j h, i h))."""
+    ny = ny or n
+    rng = np.random.default_rng(seed)
+    W = np.empty((ny, n, 5))
+    W[..., 0] = rng.uniform(0.8, 1.2, (ny, n))
+    W[..., 1] = rng.uniform(-0.1, 0.1, (ny, n))
+    W[..., 2] = rng.uniform(-0.1, 0.1, (ny, n))
+    W[..., 3] = 0.0
+    W[..., 4] = rng.uniform(0.05, 0.15, (ny, n))
+    return W, 40.0 / n
+
+
+def make(n, ny=None, spin=0.0, excision=0.0):
+    ny = ny or n
+    a = Adjoint(2, n, ny, G, physics=BLACKHOLE)
+    a.set_spacetime(1.0, spin, 20.0, 20.0, 0.0)
+    if excision:
+        a.set_excision(excision)
+    a.set_con2prim(1e-13, 200)
+    return a
+
+
+def rel(x, y):
+    return abs(x - y) / max(abs(x), abs(y), 1e-300)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_gr_sweep_adjoint_dot_product(order, axis):
+    W, h = bh_state(24, 3, ny=20)
+    rng = np.random.default_rng(4)
+    lam = rng.standard_normal(W.shape)
+    dt = 0.1 * h
+    H = 1e-6
+    with make(24, 20) as a:
+        a.upload(W)
+        g = a.sweep(order, axis, h, dt, lam)
+        for trial in range(3):
+            e = rng.standard_normal(W.shape) * np.abs(W).clip(0.05)
+            exact = float(np.sum(g * e))
+            a.upload(W + H * e)
+            a.sweep(order, axis, h, dt, lam)
+            p = a.download()
+            a.upload(W - H * e)
+            a.sweep(order, axis, h, dt, lam)
+            m = a.download()
+            fd = float(np.sum(lam * (p - m))) / (2.0 * H)
+            assert abs(exact) > 1e-6 and rel(fd, exact) < TOL, (trial, fd, exact)
+
+
+@pytest.mark.parametrize("spin,excision", [(0.0, 0.0), (0.5, 0.0), (0.0, 2.5)])
+def test_gr_forcing_adjoint_dot_product(spin, excision):
+    W, h = bh_state(20, 5, ny=22)
+    rng = np.random.default_rng(6)
+    lam = rng.standard_normal(W.shape)
+    dt = 0.2 * h
+    H = 1e-6
+    with make(20, 22, spin, excision) as a:
+        a.upload(W)
+        g = a.forcing(h, dt, lam)
+        for trial in range(3):
+            e = rng.standard_normal(W.shape) * np.abs(W).clip(0.05)
+            exact = float(np.sum(g * e))
+            a.upload(W + H * e)
+            a.forcing(h, dt, lam)
+            p = a.download()
+            a.upload(W - H * e)
+            a.forcing(h, dt, lam)
+            m = a.download()
+            fd = float(np.sum(lam * (p - m))) / (2.0 * H)
+            assert abs(exact) > 1e-6 and rel(fd, exact) < TOL, (trial, fd, exact)
+
+
+def loss_and_grad(WN, target):
+    r = WN[..., 0] - target
+    g = np.zeros_like(WN)
+    g[..., 0] = 2.0 * r
+    return float(np.sum(r * r)), g
+
+
+@pytest.mark.parametrize("order,every", [(2, 2), (1, 3)])
+def test_gr_trajectory_gradient(order, every):
+    W0, h = bh_state(24, 11)
+    target = bh_state(24, 12)[0][..., 0]
+    with make(24) as a:
+        a.upload(W0)
+        a.forward(order, h, 0.4, 4, 0.0, 1, every)
+        WN = a.download()
+        assert np.all(np.isfinite(WN))
+        _, gout = loss_and_grad(WN, target)
+        grad = a.backward(gout)
+        assert np.array_equal(a.download(), WN)
+        assert np.array_equal(a.replay(W0), WN)
+        rng = np.random.default_rng(13)
+        hh = 1e-7
+        errs = []
+        for _ in range(4):
+            e = rng.standard_normal(W0.shape) * np.abs(W0).clip(0.05)
+            exact = float(np.sum(grad * e))
+            lp = loss_and_grad(a.replay(W0 + hh * e), target)[0]
+            lm = loss_and_grad(a.replay(W0 - hh * e), target)[0]
+            errs.append(rel((lp - lm) / (2 * hh), exact))
+        errs = np.array(errs)
+        assert np.abs(grad).max() > 1e-6
+        assert (errs < TOL).sum() >= 3 and errs.max() < 1e-3, errs
