@@ -322,6 +322,91 @@ def batch_leg(args):
     return res
 
 
+def gr_adjoint_leg(args):
+    """Adjoint of the black-hole update on the differentiable-GR workload:
+    n x n on [-20M, 20M]^2 (hole at the centre, M = 1, a = 0), gamma 4/3,
+    excision r < 2.5M, rho = 1 + 0.05 GRF (correlation length 2M, seed),
+    p = 0.01, v = 0; SLIC + van Leer with one RK4 forcing sub-step on each side
+    of the sweeps, CFL 0.4, args.gra_steps steps, a checkpoint every
+    args.gra_every.  forward = the solve loop (cosmos_solve_forced) over the
+    same steps; adjoint = cosmos_adjoint_backward (segment recompute + reverse)
+    for L = sum (rho_N - rho*)^2, rho* from seed + 1.  The finite-difference
+    check runs the same workload at args.gra_fd_size with a tight con2prim
+    (1e-13), central differences with h = 1e-6 and 1e-8 along 32 seeded
+    directions of the initial density field."""
+    from cosmos_amd import BLACKHOLE, VANLEER, Adjoint, Solver, workloads
+
+    n, steps = args.gra_size, args.gra_steps
+    W, h, hole = workloads.gr_random_field_2d(n, seed=args.seed)
+    target = workloads.gr_random_field_2d(n, seed=args.seed + 1)[0][..., 0]
+    with Solver(BLACKHOLE, 2, n, n, 4.0 / 3.0) as s:
+        s.set_spacetime(*hole)
+        s.set_excision(2.5)
+        s.upload(W)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.solve_forced(2, h, 0.4, math.inf, steps, 0.0, VANLEER, 1)
+        fwd = time.perf_counter() - t0
+    with Adjoint(2, n, n, 4.0 / 3.0, physics=BLACKHOLE) as a:
+        a.set_spacetime(*hole)
+        a.set_excision(2.5)
+        a.upload(W)
+        t0 = time.perf_counter()
+        a.forward(2, h, 0.4, steps, 0.0, VANLEER, args.gra_every)
+        rec = time.perf_counter() - t0
+        WN = a.download()
+        g = np.zeros_like(WN)
+        g[..., 0] = 2.0 * (WN[..., 0] - target)
+        t0 = time.perf_counter()
+        grad = a.backward(g)
+        adj = time.perf_counter() - t0
+        finite = bool(np.all(np.isfinite(WN)) and np.all(np.isfinite(grad)))
+
+    m, fsteps = args.gra_fd_size, args.gra_fd_steps
+    W, h, hole = workloads.gr_random_field_2d(m, seed=args.seed + 7)
+    target = workloads.gr_random_field_2d(m, seed=args.seed + 8)[0][..., 0]
+    errs = []
+    with Adjoint(2, m, m, 4.0 / 3.0, physics=BLACKHOLE) as a:
+        a.set_spacetime(*hole)
+        a.set_excision(2.5)
+        a.set_con2prim(1e-13, 200)
+        a.upload(W)
+        a.forward(2, h, 0.4, fsteps, 0.0, VANLEER, args.gra_every)
+        WN = a.download()
+        g = np.zeros_like(WN)
+        g[..., 0] = 2.0 * (WN[..., 0] - target)
+        gr = a.backward(g)
+        rng = np.random.default_rng(args.seed + 9)
+        dirs = []
+        for _ in range(32):
+            e = np.zeros_like(W)
+            e[..., 0] = rng.standard_normal(W.shape[:-1])  # the initial density field
+            dirs.append(e)
+        fd_by_h = {}
+        for hh in (1e-6, 1e-8):
+            errs = []
+            for e in dirs:
+                exact = float(np.sum(gr * e))
+                lp = float(np.sum((a.replay(W + hh * e)[..., 0] - target) ** 2))
+                lm = float(np.sum((a.replay(W - hh * e)[..., 0] - target) ** 2))
+                fd = (lp - lm) / (2 * hh)
+                errs.append(abs(fd - exact) / max(abs(fd), abs(exact), 1e-300))
+            errs = np.array(errs)
+            fd_by_h[f"h={hh:g}"] = {"median_rel_err": float(np.median(errs)), "max_rel_err": float(errs.max()),
+                                    "within_1e-6": int((errs < 1e-6).sum())}
+    return {"workload": f"2D black-hole accretion, {n}x{n} FP64 on [-20M,20M]^2, M=1, a=0, gamma 4/3, excision "
+                        f"r<2.5M, rho = 1 + 0.05 GRF (seed {args.seed}), p = 0.01, SLIC+van Leer, RK4 forcing "
+                        f"(1 subcycle), CFL 0.4, {steps} steps, checkpoint every {args.gra_every}, dt schedule frozen",
+            "forward_s": fwd, "forward_recording_s": rec, "adjoint_s": adj, "adjoint_forward_ratio": adj / fwd,
+            "state_and_gradient_finite": finite,
+            "fd_check": {"grid": [m, m], "steps": fsteps, "con2prim_tolerance": 1e-13, "directions": 32,
+                         "direction": "seeded normal perturbations of the initial density field",
+                         **fd_by_h,
+                         "note": "differences shrink with h: the reference's slope clamp (|delta| < 1e-5 -> +1e-5, "
+                                 "sign dropped) and van Leer kink make the forward map piecewise; central "
+                                 "differences straddle those pieces at h = 1e-6"}}
+
+
 def adjoint_leg(args):
     """Adjoint of the Newtonian SLIC update: the 4-quadrant workload at
     n x n (CFL 0.9, van Leer), args.adj_steps steps with a checkpoint every
@@ -485,6 +570,11 @@ def main():
     ap.add_argument("--sr-steps", type=int, default=200)
     ap.add_argument("--adj-size", type=int, default=1024)
     ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--gra-size", type=int, default=1024)
+    ap.add_argument("--gra-steps", type=int, default=20)
+    ap.add_argument("--gra-every", type=int, default=20)
+    ap.add_argument("--gra-fd-size", type=int, default=64)
+    ap.add_argument("--gra-fd-steps", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=1024)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--batch-tune-steps", type=int, default=10)
@@ -544,6 +634,7 @@ def main():
         line["relativistic"] = sr_leg(args)
         line["blackhole"] = gr_leg(args)
         line["adjoint"] = adjoint_leg(args)
+        line["adjoint_blackhole"] = gr_adjoint_leg(args)
         line["batched"] = batch_leg(args)
     if not args.no_cpu:
         line.update(cpu_leg(args))

@@ -147,7 +147,7 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
         ld5(L.in, at(c), WR);
         zero5(gL);
         zero5(gR);
-        adjgr::vjp_force<AXIS>(WL, WR, x, y, b, L.h, L.coef, L.P, gF, gL, gR);
+        adjgr::vjp_force_g<AXIS>(WL, WR, adjgr::force_geo<AXIS>(b, x, y, L.h), L.coef, L.P, gF, gL, gR);
         add5(A.lam_in, at(c - 1), gL);
         add5(A.lam_in, at(c), gR);
         return;
@@ -160,19 +160,21 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     ld5(L.in, at(c - 1), W1);
     ld5(L.in, at(c), W2);
     ld5(L.in, at(c + 1), W3);
-    adjgr::half_fwd<AXIS>(W0, W1, W2, xl, yl, 1, b, L.h, L.coef, L.bias, L.limiter, L.P, pr);
-    adjgr::half_fwd<AXIS>(W1, W2, W3, xr, yr, 0, b, L.h, L.coef, L.bias, L.limiter, L.P, pl);
+    // each metric point evaluated once for the forward re-run and the adjoint
+    const adjgr::HalfGeo HL = adjgr::half_geo<AXIS>(b, xl, yl, L.h), HR = adjgr::half_geo<AXIS>(b, xr, yr, L.h);
+    adjgr::half_fwd_g<AXIS>(W0, W1, W2, 1, HL, L.coef, L.bias, L.limiter, L.P, pr);
+    adjgr::half_fwd_g<AXIS>(W1, W2, W3, 0, HR, L.coef, L.bias, L.limiter, L.P, pl);
     double gpr[5], gpl[5];
     zero5(gpr);
     zero5(gpl);
-    adjgr::vjp_force<AXIS>(pr, pl, x, y, b, L.h, L.coef, L.P, gF, gpr, gpl);
+    adjgr::vjp_force_g<AXIS>(pr, pl, adjgr::force_geo<AXIS>(b, x, y, L.h), L.coef, L.P, gF, gpr, gpl);
     double g0[5], g1[5], g2[5], g3[5];
     zero5(g0);
     zero5(g1);
     zero5(g2);
     zero5(g3);
-    adjgr::vjp_half<AXIS>(W0, W1, W2, xl, yl, 1, b, L.h, L.coef, L.bias, L.limiter, L.P, gpr, g0, g1, g2);
-    adjgr::vjp_half<AXIS>(W1, W2, W3, xr, yr, 0, b, L.h, L.coef, L.bias, L.limiter, L.P, gpl, g1, g2, g3);
+    adjgr::vjp_half_g<AXIS>(W0, W1, W2, 1, HL, L.coef, L.bias, L.limiter, L.P, gpr, g0, g1, g2);
+    adjgr::vjp_half_g<AXIS>(W1, W2, W3, 0, HR, L.coef, L.bias, L.limiter, L.P, gpl, g1, g2, g3);
     add5(A.lam_in, at(c - 2), g0);
     add5(A.lam_in, at(c - 1), g1);
     add5(A.lam_in, at(c), g2);
@@ -204,18 +206,24 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
                         DIM == 2 ? (long)clamp1_a(i + 1, L.ny) * L.pitch + j : cell};
     const double nbx[4] = {(j - 1) * h, (j + 1) * h, x, x};
     const double nby[4] = {y, y, DIM == 2 ? (i - 1) * h : y, DIM == 2 ? (i + 1) * h : y};
-    const double sx[4] = {x - h, x + h, x, x};
-    const double sy[4] = {y, y, y - h, y + h};
+    // the metric at the centre and one cell either side, shared by every
+    // fractional sweep of the four stages (the positions the forward forms)
+    const Geo gc = geo_at(b, x, y, 0.0);
+    const Geo gxm = geo_at(b, x - h, y, 0.0), gxp = geo_at(b, x + h, y, 0.0);
+    Geo gym, gyp;
+    if (DIM == 2) {
+        gym = geo_at(b, x, y - h, 0.0);
+        gyp = geo_at(b, x, y + h, 0.0);
+    }
     const int nn = DIM == 2 ? 4 : 2;
     double Wnb[4][5], unb[4][5], Wn[4][5];
     for (int q = 0; q < nn; ++q) {
         ld5(L.in, nb[q], Wnb[q]);
         gr_p2c(Wnb[q], geo_at(b, nbx[q], nby[q], 0.0), unb[q], P);
-        gr_c2p(unb[q], geo_at(b, sx[q], sy[q], 0.0).sdet, P, Wn[q]);
+        gr_c2p(unb[q], (q == 0 ? gxm : q == 1 ? gxp : q == 2 ? gym : gyp).sdet, P, Wn[q]);
     }
     double W[5], uM[5];
     ld5(L.in, cell, W);
-    const Geo gc = geo_at(b, x, y, 0.0);
     gr_p2c(W, gc, uM, P);
     SourceGeo sg;
     source_geo(b, x, y, 0.0, h, sg);
@@ -228,11 +236,11 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
         for (int k = 0; k < 5; ++k) tmp[st][k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
         gr_c2p(tmp[st], gc.sdet, P, Wm[st]);
         if (DIM == 1) {
-            adjgr::frac_fwd<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, e1[st]);
+            adjgr::frac_fwd_g<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, e1[st]);
         } else {
-            adjgr::frac_fwd<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, e2[st]);
-            adjgr::frac_fwd<1>(L.frac_dt_h_y[st], Wn[2], e2[st], Wn[3], x, y, b, h, L.bias, L.limiter, P, e3[st]);
-            adjgr::frac_fwd<0>(L.frac_dt_h[st], Wn[0], e3[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, e1[st]);
+            adjgr::frac_fwd_g<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, e2[st]);
+            adjgr::frac_fwd_g<1>(L.frac_dt_h_y[st], Wn[2], e2[st], Wn[3], gym, gc, gyp, L.bias, L.limiter, P, e3[st]);
+            adjgr::frac_fwd_g<0>(L.frac_dt_h[st], Wn[0], e3[st], Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, e1[st]);
         }
         gr_p2c(e1[st], gc, um[st], P);
         gr_source(um[st], sg, P, src);
@@ -266,18 +274,18 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
         adjgr::vjp_p2c(e1[st], gc, P, gum, ge1);
         zero5(gWm);
         if (DIM == 1) {
-            adjgr::vjp_frac<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, ge1, gWn[0],
-                               gWm, gWn[1]);
+            adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, ge1,
+                                 gWn[0], gWm, gWn[1]);
         } else {
             double ge3[5], ge2[5];
             zero5(ge3);
             zero5(ge2);
-            adjgr::vjp_frac<0>(L.frac_dt_h[st], Wn[0], e3[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, ge1, gWn[0],
-                               ge3, gWn[1]);
-            adjgr::vjp_frac<1>(L.frac_dt_h_y[st], Wn[2], e2[st], Wn[3], x, y, b, h, L.bias, L.limiter, P, ge3,
-                               gWn[2], ge2, gWn[3]);
-            adjgr::vjp_frac<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], x, y, b, h, L.bias, L.limiter, P, ge2, gWn[0],
-                               gWm, gWn[1]);
+            adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wn[0], e3[st], Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, ge1,
+                                 gWn[0], ge3, gWn[1]);
+            adjgr::vjp_frac_g<1>(L.frac_dt_h_y[st], Wn[2], e2[st], Wn[3], gym, gc, gyp, L.bias, L.limiter, P, ge3,
+                                 gWn[2], ge2, gWn[3]);
+            adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wn[0], Wm[st], Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, ge2,
+                                 gWn[0], gWm, gWn[1]);
         }
         zero5(gtmp);
         adjgr::vjp_c2p(tmp[st], gc.sdet, tol, P, gWm, gtmp);
@@ -302,7 +310,7 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
         double gu[5], gw[5];
         zero5(gu);
         zero5(gw);
-        adjgr::vjp_c2p(unb[q], geo_at(b, sx[q], sy[q], 0.0).sdet, tol, P, gWn[q], gu);
+        adjgr::vjp_c2p(unb[q], (q == 0 ? gxm : q == 1 ? gxp : q == 2 ? gym : gyp).sdet, tol, P, gWn[q], gu);
         adjgr::vjp_p2c(Wnb[q], geo_at(b, nbx[q], nby[q], 0.0), P, gu, gw);
         add5(A.lam_in, nb[q], gw);
     }

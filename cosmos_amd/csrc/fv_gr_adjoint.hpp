@@ -268,38 +268,53 @@ CFV_HD void slopes(const double* ul, const double* um, const double* ur, double 
     for (int k = 0; k < 5; ++k) s[k] = adj::slope_d(ul[k], um[k], ur[k], bias, limiter, d0[k], d1[k]);
 }
 
-// FORCE flux at the interface (x, y) from the states WL (at -h/2) and WR (at
-// +h/2): the forward (gr_force) and its adjoint.
+// The metric of a FORCE flux at an interface: at the interface -/+ h/2 and at it.
+struct ForceGeo {
+    Geo l, r, c;
+};
+// The metric of a half step about a centre: the centre -/+ h, the centre, and
+// its faces -/+ h/2.
+struct HalfGeo {
+    Geo m0, m, m1, fl, fr;
+};
+
 template <int AXIS>
-CFV_HD void force_fwd(const double* WL, const double* WR, double x, double y, const Hole& b, double h,
-                      const SweepCoef& cf, const PhysParams& P, double* F) {
+CFV_HD ForceGeo force_geo(const Hole& b, double x, double y, double h) {
     const double hh = h * 0.5;
+    return ForceGeo{geo_shift<AXIS>(b, x, y, -hh), geo_shift<AXIS>(b, x, y, hh), geo_at(b, x, y, 0.0)};
+}
+
+template <int AXIS>
+CFV_HD HalfGeo half_geo(const Hole& b, double x, double y, double h) {
+    return HalfGeo{geo_shift<AXIS>(b, x, y, -h), geo_at(b, x, y, 0.0), geo_shift<AXIS>(b, x, y, h),
+                   geo_shift<AXIS>(b, x, y, -h * 0.5), geo_shift<AXIS>(b, x, y, h * 0.5)};
+}
+
+// FORCE flux at an interface from the states WL (at -h/2) and WR (at +h/2)
+// (gr_force): the forward and its adjoint.
+template <int AXIS>
+CFV_HD void force_fwd_g(const double* WL, const double* WR, const ForceGeo& G, const SweepCoef& cf, const PhysParams& P,
+                        double* F) {
     double ul[5], ur[5], fl[5], fr[5], us[5], fs[5];
-    const Geo gl = geo_shift<AXIS>(b, x, y, -hh);
-    gr_p2c(WL, gl, ul, P);
-    gr_flux<AXIS>(ul, gl, P, fl);
-    const Geo gr = geo_shift<AXIS>(b, x, y, hh);
-    gr_p2c(WR, gr, ur, P);
-    gr_flux<AXIS>(ur, gr, P, fr);
+    gr_p2c(WL, G.l, ul, P);
+    gr_flux<AXIS>(ul, G.l, P, fl);
+    gr_p2c(WR, G.r, ur, P);
+    gr_flux<AXIS>(ur, G.r, P, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (cf.dt_dx * (fl[k] - fr[k])));
-    gr_flux<AXIS>(us, geo_at(b, x, y, 0.0), P, fs);
+    gr_flux<AXIS>(us, G.c, P, fs);
 #pragma unroll
     for (int k = 0; k < 5; ++k) F[k] = 0.5 * ((0.5 * ((fl[k] + fr[k]) + (cf.dx_dt * (ul[k] - ur[k])))) + fs[k]);
 }
 
 template <int AXIS>
-CFV_HD void vjp_force(const double* WL, const double* WR, double x, double y, const Hole& b, double h,
-                      const SweepCoef& cf, const PhysParams& P, const double* gF, double* gWL, double* gWR) {
-    const double hh = h * 0.5;
+CFV_HD void vjp_force_g(const double* WL, const double* WR, const ForceGeo& G, const SweepCoef& cf,
+                        const PhysParams& P, const double* gF, double* gWL, double* gWR) {
     double ul[5], ur[5], fl[5], fr[5], us[5];
-    const Geo gl = geo_shift<AXIS>(b, x, y, -hh);
-    const Geo gr = geo_shift<AXIS>(b, x, y, hh);
-    const Geo gc = geo_at(b, x, y, 0.0);
-    gr_p2c(WL, gl, ul, P);
-    gr_flux<AXIS>(ul, gl, P, fl);
-    gr_p2c(WR, gr, ur, P);
-    gr_flux<AXIS>(ur, gr, P, fr);
+    gr_p2c(WL, G.l, ul, P);
+    gr_flux<AXIS>(ul, G.l, P, fl);
+    gr_p2c(WR, G.r, ur, P);
+    gr_flux<AXIS>(ur, G.r, P, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (cf.dt_dx * (fl[k] - fr[k])));
     double gfs[5], gfl[5], gfr[5], gul[5], gur[5], gus[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -312,7 +327,7 @@ CFV_HD void vjp_force(const double* WL, const double* WR, double x, double y, co
         gul[k] = cf.dx_dt * glf;
         gur[k] = -cf.dx_dt * glf;
     }
-    vjp_flux<AXIS>(us, gc, P, gfs, gus);
+    vjp_flux<AXIS>(us, G.c, P, gfs, gus);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         gul[k] += 0.5 * gus[k];
@@ -320,21 +335,32 @@ CFV_HD void vjp_force(const double* WL, const double* WR, double x, double y, co
         gfl[k] += 0.5 * cf.dt_dx * gus[k];
         gfr[k] -= 0.5 * cf.dt_dx * gus[k];
     }
-    vjp_flux<AXIS>(ul, gl, P, gfl, gul);
-    vjp_flux<AXIS>(ur, gr, P, gfr, gur);
-    vjp_p2c(WL, gl, P, gul, gWL);
-    vjp_p2c(WR, gr, P, gur, gWR);
+    vjp_flux<AXIS>(ul, G.l, P, gfl, gul);
+    vjp_flux<AXIS>(ur, G.r, P, gfr, gur);
+    vjp_p2c(WL, G.l, P, gul, gWL);
+    vjp_p2c(WR, G.r, P, gur, gWR);
 }
 
-// SLIC half step of the middle cell centred at (x, y): the forward
-// (gr_half_step) and its adjoint.
 template <int AXIS>
-CFV_HD void half_fwd(const double* WL, const double* WM, const double* WR, double x, double y, int side, const Hole& b,
-                     double h, const SweepCoef& cf, double bias, int limiter, const PhysParams& P, double* out) {
+CFV_HD void force_fwd(const double* WL, const double* WR, double x, double y, const Hole& b, double h,
+                      const SweepCoef& cf, const PhysParams& P, double* F) {
+    force_fwd_g<AXIS>(WL, WR, force_geo<AXIS>(b, x, y, h), cf, P, F);
+}
+
+template <int AXIS>
+CFV_HD void vjp_force(const double* WL, const double* WR, double x, double y, const Hole& b, double h,
+                      const SweepCoef& cf, const PhysParams& P, const double* gF, double* gWL, double* gWR) {
+    vjp_force_g<AXIS>(WL, WR, force_geo<AXIS>(b, x, y, h), cf, P, gF, gWL, gWR);
+}
+
+// SLIC half step of a middle cell (gr_half_step): the forward and its adjoint.
+template <int AXIS>
+CFV_HD void half_fwd_g(const double* WL, const double* WM, const double* WR, int side, const HalfGeo& G,
+                       const SweepCoef& cf, double bias, int limiter, const PhysParams& P, double* out) {
     double ul[5], um[5], ur[5], s[5], d0[5], d1[5];
-    gr_p2c(WL, geo_shift<AXIS>(b, x, y, -h), ul, P);
-    gr_p2c(WM, geo_at(b, x, y, 0.0), um, P);
-    gr_p2c(WR, geo_shift<AXIS>(b, x, y, h), ur, P);
+    gr_p2c(WL, G.m0, ul, P);
+    gr_p2c(WM, G.m, um, P);
+    gr_p2c(WR, G.m1, ur, P);
     slopes(ul, um, ur, bias, limiter, s, d0, d1);
     double xl[5], xr[5], fl[5], fr[5], e[5];
 #pragma unroll
@@ -342,24 +368,21 @@ CFV_HD void half_fwd(const double* WL, const double* WM, const double* WR, doubl
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    const Geo gl = geo_shift<AXIS>(b, x, y, -h * 0.5);
-    const Geo gr = geo_shift<AXIS>(b, x, y, h * 0.5);
-    gr_flux<AXIS>(xl, gl, P, fl);
-    gr_flux<AXIS>(xr, gr, P, fr);
+    gr_flux<AXIS>(xl, G.fl, P, fl);
+    gr_flux<AXIS>(xr, G.fr, P, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (cf.hdt_dx * (fl[k] - fr[k]));
-    gr_c2p(e, side == 0 ? gl.sdet : gr.sdet, P, out);
+    gr_c2p(e, side == 0 ? G.fl.sdet : G.fr.sdet, P, out);
 }
 
 template <int AXIS>
-CFV_HD void vjp_half(const double* WL, const double* WM, const double* WR, double x, double y, int side, const Hole& b,
-                     double h, const SweepCoef& cf, double bias, int limiter, const PhysParams& P, const double* gout,
-                     double* gWL, double* gWM, double* gWR) {
-    const Geo g0 = geo_shift<AXIS>(b, x, y, -h), gm = geo_at(b, x, y, 0.0), g1 = geo_shift<AXIS>(b, x, y, h);
+CFV_HD void vjp_half_g(const double* WL, const double* WM, const double* WR, int side, const HalfGeo& G,
+                       const SweepCoef& cf, double bias, int limiter, const PhysParams& P, const double* gout,
+                       double* gWL, double* gWM, double* gWR) {
     double ul[5], um[5], ur[5], s[5], d0[5], d1[5];
-    gr_p2c(WL, g0, ul, P);
-    gr_p2c(WM, gm, um, P);
-    gr_p2c(WR, g1, ur, P);
+    gr_p2c(WL, G.m0, ul, P);
+    gr_p2c(WM, G.m, um, P);
+    gr_p2c(WR, G.m1, ur, P);
     slopes(ul, um, ur, bias, limiter, s, d0, d1);
     double xl[5], xr[5], fl[5], fr[5], e[5];
 #pragma unroll
@@ -367,14 +390,12 @@ CFV_HD void vjp_half(const double* WL, const double* WM, const double* WR, doubl
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    const Geo gl = geo_shift<AXIS>(b, x, y, -h * 0.5);
-    const Geo gr = geo_shift<AXIS>(b, x, y, h * 0.5);
-    gr_flux<AXIS>(xl, gl, P, fl);
-    gr_flux<AXIS>(xr, gr, P, fr);
+    gr_flux<AXIS>(xl, G.fl, P, fl);
+    gr_flux<AXIS>(xr, G.fr, P, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (cf.hdt_dx * (fl[k] - fr[k]));
     double ge[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    vjp_c2p(e, side == 0 ? gl.sdet : gr.sdet, gr_tol(P, 1e-8), P, gout, ge);
+    vjp_c2p(e, side == 0 ? G.fl.sdet : G.fr.sdet, gr_tol(P, 1e-8), P, gout, ge);
     double gxl[5], gxr[5], gfl[5], gfr[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -383,8 +404,8 @@ CFV_HD void vjp_half(const double* WL, const double* WM, const double* WR, doubl
         gfl[k] = cf.hdt_dx * ge[k];
         gfr[k] = -cf.hdt_dx * ge[k];
     }
-    vjp_flux<AXIS>(xl, gl, P, gfl, gxl);
-    vjp_flux<AXIS>(xr, gr, P, gfr, gxr);
+    vjp_flux<AXIS>(xl, G.fl, P, gfl, gxl);
+    vjp_flux<AXIS>(xr, G.fr, P, gfr, gxr);
     double gul[5], gum[5], gur[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -393,17 +414,31 @@ CFV_HD void vjp_half(const double* WL, const double* WM, const double* WR, doubl
         gul[k] = -gs * d0[k];
         gur[k] = gs * d1[k];
     }
-    vjp_p2c(WL, g0, P, gul, gWL);
-    vjp_p2c(WM, gm, P, gum, gWM);
-    vjp_p2c(WR, g1, P, gur, gWR);
+    vjp_p2c(WL, G.m0, P, gul, gWL);
+    vjp_p2c(WM, G.m, P, gum, gWM);
+    vjp_p2c(WR, G.m1, P, gur, gWR);
 }
 
-// Fractional sweep of a forcing stage about the cell centre (x, y) with
-// fdt_h = fraction * dt / h (gr_fractional): the forward and its adjoint.
 template <int AXIS>
-CFV_HD void frac_fwd(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
-                     const Hole& b, double h, double bias, int limiter, const PhysParams& P, double* out) {
-    const Geo gl = geo_shift<AXIS>(b, x, y, -h), gc = geo_at(b, x, y, 0.0), gr = geo_shift<AXIS>(b, x, y, h);
+CFV_HD void half_fwd(const double* WL, const double* WM, const double* WR, double x, double y, int side, const Hole& b,
+                     double h, const SweepCoef& cf, double bias, int limiter, const PhysParams& P, double* out) {
+    half_fwd_g<AXIS>(WL, WM, WR, side, half_geo<AXIS>(b, x, y, h), cf, bias, limiter, P, out);
+}
+
+template <int AXIS>
+CFV_HD void vjp_half(const double* WL, const double* WM, const double* WR, double x, double y, int side, const Hole& b,
+                     double h, const SweepCoef& cf, double bias, int limiter, const PhysParams& P, const double* gout,
+                     double* gWL, double* gWM, double* gWR) {
+    vjp_half_g<AXIS>(WL, WM, WR, side, half_geo<AXIS>(b, x, y, h), cf, bias, limiter, P, gout, gWL, gWM, gWR);
+}
+
+// Fractional sweep of a forcing stage about a cell centre with
+// fdt_h = fraction * dt / h (gr_fractional), the metric given at the centre
+// (gc) and one cell either side along the sweep (gl, gr): the forward and its
+// adjoint.
+template <int AXIS>
+CFV_HD void frac_fwd_g(double fdt_h, const double* WL, const double* WM, const double* WR, const Geo& gl, const Geo& gc,
+                       const Geo& gr, double bias, int limiter, const PhysParams& P, double* out) {
     double ul[5], um[5], ur[5], s[5], d0[5], d1[5];
     gr_p2c(WL, gl, ul, P);
     gr_p2c(WM, gc, um, P);
@@ -423,10 +458,9 @@ CFV_HD void frac_fwd(double fdt_h, const double* WL, const double* WM, const dou
 }
 
 template <int AXIS>
-CFV_HD void vjp_frac(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
-                     const Hole& b, double h, double bias, int limiter, const PhysParams& P, const double* gout,
-                     double* gWL, double* gWM, double* gWR) {
-    const Geo gl = geo_shift<AXIS>(b, x, y, -h), gc = geo_at(b, x, y, 0.0), gr = geo_shift<AXIS>(b, x, y, h);
+CFV_HD void vjp_frac_g(double fdt_h, const double* WL, const double* WM, const double* WR, const Geo& gl, const Geo& gc,
+                       const Geo& gr, double bias, int limiter, const PhysParams& P, const double* gout, double* gWL,
+                       double* gWM, double* gWR) {
     double ul[5], um[5], ur[5], s[5], d0[5], d1[5];
     gr_p2c(WL, gl, ul, P);
     gr_p2c(WM, gc, um, P);
@@ -463,6 +497,22 @@ CFV_HD void vjp_frac(double fdt_h, const double* WL, const double* WM, const dou
     vjp_p2c(WL, gl, P, gul, gWL);
     vjp_p2c(WM, gc, P, gum, gWM);
     vjp_p2c(WR, gr, P, gur, gWR);
+}
+
+// The same about (x, y), the metric formed here.
+template <int AXIS>
+CFV_HD void frac_fwd(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
+                     const Hole& b, double h, double bias, int limiter, const PhysParams& P, double* out) {
+    frac_fwd_g<AXIS>(fdt_h, WL, WM, WR, geo_shift<AXIS>(b, x, y, -h), geo_at(b, x, y, 0.0), geo_shift<AXIS>(b, x, y, h),
+                     bias, limiter, P, out);
+}
+
+template <int AXIS>
+CFV_HD void vjp_frac(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
+                     const Hole& b, double h, double bias, int limiter, const PhysParams& P, const double* gout,
+                     double* gWL, double* gWM, double* gWR) {
+    vjp_frac_g<AXIS>(fdt_h, WL, WM, WR, geo_shift<AXIS>(b, x, y, -h), geo_at(b, x, y, 0.0),
+                     geo_shift<AXIS>(b, x, y, h), bias, limiter, P, gout, gWL, gWM, gWR);
 }
 
 }  // namespace adjgr
