@@ -91,7 +91,9 @@ def gpu_leg(args, mode):
     ev = HipEvents()
     with Solver(NEWTONIAN, 2, n, n, 1.4) as s:
         s.set_mode(FAST if mode == "fast" else EXACT)
+        t0 = time.perf_counter()
         s.upload(W)
+        transfer_s = time.perf_counter() - t0
         del W
         stream = s.stream
 
@@ -109,7 +111,11 @@ def gpu_leg(args, mode):
         ms_total = ev.elapsed_ms(e0, e1)
         t_reached, steps_done, _ = s.progress()
         assert steps_done == args.warmup + args.steps, steps_done
-        state_ok = bool(np.isfinite(s.download()[:: max(1, n // 64), :: max(1, n // 64)]).all())
+        t0 = time.perf_counter()
+        out = s.download()
+        transfer_s += time.perf_counter() - t0
+        state_ok = bool(np.isfinite(out[:: max(1, n // 64), :: max(1, n // 64)]).all())
+        del out
 
         # the same steps again with every kernel bracketed by HIP events on the stream
         s.begin(2, dx, 0.9, math.inf, args.warmup + args.steps, 0.0, VANLEER)
@@ -163,6 +169,7 @@ def gpu_leg(args, mode):
         "dominant_avg_ms": dom["avg_ms"],
         "dominant_achieved_gbs": achieved,
         "copy_gbs": copy_gbs,
+        "transfer_s": transfer_s,
         "t_reached": t_reached,
         "state_finite": state_ok,
     }
@@ -609,6 +616,10 @@ def main():
         "step_algorithmic_gbs": g["step_hbm_gbs"],
         "copy_kernel_gbs": g["copy_gbs"],
         "hbm_fraction_of_copy": g["step_hbm_gbs"] / g["copy_gbs"],
+        "pcie_inclusive_cell_updates_per_s": float(args.size) ** 2 * args.steps
+        / (g["ms_per_step"] * 1e-3 * args.steps + g["transfer_s"]),
+        "pcie_note": "the state uploaded from and downloaded to host arrays around the run (what the drop-in "
+                     "shims do); value excludes it (state resident in HBM)",
         "roofline": {
             "bound": "hbm",
             "kernel": g["dominant"],
