@@ -4,16 +4,22 @@
 Workload: 2D Newtonian Euler, 4096 x 4096 FP64 cells on [0, 1]^2, gamma 1.4,
 SLIC + van Leer, dimensionally split X(dt/2) Y(dt) X(dt/2), CFL 0.9,
 transmissive boundaries, seeded four-quadrant initial data.  A step is one
-full time step: the CFL time step (max wave speed, fused into the last
-sweep) and the three sweeps.  The state is resident in HBM before timing.
+full time step: the CFL time step (max wave speed, fused into the step's
+last pass) and the three sweeps.  The state is resident in HBM before timing.
+Fast mode (the headline) runs each step as one fused X Y X kernel; the same
+mode with the three sweeps launched separately is reported beside it.
 
     python bench.py [--steps 100] [--warmup 5] [--size 4096] [--no-cpu]
 
-Prints one JSON line.  value = cell-updates/s; roofline = the dominant
-kernel's algorithmic bytes per launch (80 B per cell per sweep: five FP64
-primitives read and five written) over its average launch duration, timed
-with HIP events on the solver's stream; cpu_baseline = the reference's own
-sources (oracle/_ref, serial build) timed on this host on a bounded sample.
+Prints one JSON line.  value = cell-updates/s.  hbm_fraction = cell-updates/s
+x 240 B (the three unfused sweeps' compulsory traffic: five FP64 primitives
+read and five written per sweep) / 8 TB/s; hbm_fraction_measured = the step
+kernels' HBM bytes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+/ step time / 8 TB/s.  roofline = the dominant kernel's algorithmic bytes per
+launch (80 B per cell: the state read once and written once) over its average
+launch duration, timed with HIP events on the solver's stream; cpu_baseline =
+the reference's own sources (oracle/_ref, serial build) timed on this host on a
+bounded sample.
 """
 from __future__ import annotations
 
@@ -36,7 +42,7 @@ METRIC = "cell-updates/s & %HBM peak (2D SLIC FP64 4096²); adjoint/forward time
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_SWEEP = 80.0      # 5 FP64 read + 5 FP64 written
 SWEEPS_PER_STEP = 3
-KIND_NAMES = ["step_head", "sweep_x", "sweep_y", "sweep_x_wavespeed"]
+KIND_NAMES = ["step_head", "sweep_x", "sweep_y", "sweep_x_wavespeed", "step_fused"]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweeps_4096_traffic.json")
 
 
@@ -53,6 +59,20 @@ def measured_traffic(size, mode, kernel):
         return float(k["fetch_bytes"] + k["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def measured_step_traffic(size, mode, kernels, steps):
+    """HBM bytes per step of the step's sweep kernels (the dt head's few bytes
+    aside) from the committed counter passes, or None if one is missing."""
+    total = 0.0
+    for name, k in kernels.items():
+        if name == "step_head":
+            continue
+        b = measured_traffic(size, mode, name)
+        if b is None:
+            return None
+        total += b * k["launches"] / steps
+    return total
 
 
 class HipEvents:
@@ -82,6 +102,8 @@ class HipEvents:
 
 
 def gpu_leg(args, mode):
+    """mode: "exact", "fast" (fused step kernel) or "fast_sweeps" (fast mode,
+    three sweep launches per step)."""
     from cosmos_amd import EXACT, FAST, NEWTONIAN, VANLEER, Solver, native, workloads
 
     n = args.size
@@ -90,7 +112,8 @@ def gpu_leg(args, mode):
     dx = 1.0 / n
     ev = HipEvents()
     with Solver(NEWTONIAN, 2, n, n, 1.4) as s:
-        s.set_mode(FAST if mode == "fast" else EXACT)
+        s.set_mode(EXACT if mode == "exact" else FAST)
+        s.set_step_fusion(mode == "fast")
         t0 = time.perf_counter()
         s.upload(W)
         transfer_s = time.perf_counter() - t0
@@ -124,8 +147,8 @@ def gpu_leg(args, mode):
         lib = native.load()
         lib.cosmos_profile_enable(s._h, 1)
         s.advance(args.steps)
-        tot = (ctypes.c_double * 4)()
-        cnt = (ctypes.c_int * 4)()
+        tot = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int * 5)()
         native.check(lib.cosmos_profile_read(s._h, tot, cnt), s._h)
         lib.cosmos_profile_enable(s._h, 0)
         kernels = {}
@@ -210,8 +233,8 @@ def gr_leg(args):
         # per-kernel split: one more step with events around every launch
         lib.cosmos_profile_enable(s._h, 1)
         s.solve_forced(2, h, 0.4, max_steps=1, subcycles=1)
-        tot = (ctypes.c_double * 4)()
-        cnt = (ctypes.c_int * 4)()
+        tot = (ctypes.c_double * 5)()
+        cnt = (ctypes.c_int * 5)()
         native.check(lib.cosmos_profile_read(s._h, tot, cnt), s._h)
         lib.cosmos_profile_enable(s._h, 0)
     names = {0: "forcing", 1: "sweep_x", 2: "sweep_y"}
@@ -566,8 +589,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--mode", choices=["fast", "exact"], default="fast",
-                    help="headline arithmetic mode (the other is measured and reported beside it)")
+    ap.add_argument("--mode", choices=["fast", "exact", "fast_sweeps"], default="fast",
+                    help="headline mode (the others are measured and reported beside it)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-size", type=int, default=768)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -593,8 +616,9 @@ def main():
               "(multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
-    modes = {m: gpu_leg(args, m) for m in ("exact", "fast")}
+    modes = {m: gpu_leg(args, m) for m in ("exact", "fast", "fast_sweeps")}
     g = modes[args.mode]
+    step_traffic = measured_step_traffic(args.size, args.mode, g["kernels"], args.steps)
     line = {
         "metric": METRIC,
         "value": g["cell_updates_per_s"],
@@ -614,6 +638,9 @@ def main():
                    "mode": args.mode, "parallelism": "single GPU"},
         "hbm_fraction": g["step_hbm_gbs"] / HBM_PEAK_GBS,
         "step_algorithmic_gbs": g["step_hbm_gbs"],
+        "hbm_fraction_measured": None if step_traffic is None
+        else step_traffic / (g["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "step_measured_bytes_per_cell_update": None if step_traffic is None else step_traffic / args.size ** 2,
         "copy_kernel_gbs": g["copy_gbs"],
         "hbm_fraction_of_copy": g["step_hbm_gbs"] / g["copy_gbs"],
         "pcie_inclusive_cell_updates_per_s": float(args.size) ** 2 * args.steps
@@ -637,7 +664,9 @@ def main():
                   for m, r in modes.items()},
         "parity": {"exact": "bitwise equal to the reference arithmetic (tests/test_gpu_parity.py)",
                    "fast": "relative L-inf per conserved variable <= 1e-11 after 100 steps at 4096^2 "
-                           "(tests/test_gpu_parity.py::test_fast_mode_config2_full_size_100_steps)"},
+                           "(tests/test_gpu_parity.py::test_fast_mode_config2_full_size_100_steps); the fused "
+                           "step kernel is bitwise the three fast sweeps "
+                           "(tests/test_gpu_parity.py::test_fused_step_is_bitwise_the_three_sweeps)"},
         "timing": "HIP events on the solver stream; per-kernel averages from a second identical region with "
                   "every launch bracketed by events",
     }

@@ -42,6 +42,7 @@ struct cosmos_solver {
     int y_rows = 0;                            // y-sweep rows per thread (0: by physics)
     int y_tile = -1;                           // y-sweep layout: 0 march, 8 / 16 tile columns (-1: by physics)
     int ws_mode = 0;                           // fused wave-speed hand-off (0: default)
+    int fuse_step = 1;                         // fast mode: one fused kernel per 2D step where it applies
     cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
     double excision = 0.0;                     // > 0: excision radius override
     double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
@@ -166,7 +167,7 @@ int check_order(cosmos_solver* s, int order) {
 }
 
 // Kernel kinds reported by cosmos_profile_read.
-enum { kProfStep = 0, kProfSweepX = 1, kProfSweepY = 2, kProfSweepXWave = 3, kProfKinds = 4 };
+enum { kProfStep = 0, kProfSweepX = 1, kProfSweepY = 2, kProfSweepXWave = 3, kProfStepFused = 4, kProfKinds = 5 };
 
 // Launches f() on the handle's stream, bracketed by HIP events when profiling.
 template <class F>
@@ -198,12 +199,32 @@ int run_sweep(cosmos_solver* s, const SweepLaunch& L, int axis) {
 
 int sweeps_per_step(const cosmos_solver* s) { return s->dim == 2 ? 3 : 1; }
 
+// Whether the queued 2D step runs as one fused kernel: fast mode, Newtonian
+// SLIC with van Leer and bias 0, at least 8 cells along each axis, wave speed
+// handed over through the slots.  The buffer parity is the three sweeps'
+// (one hand-off instead of three), so settle() is unchanged.
+bool fused_step(const cosmos_solver* s) {
+    return s->fuse_step && s->mode == COSMOS_MODE_FAST && s->physics == COSMOS_NEWTONIAN && s->dim == 2 &&
+           s->run_order == 2 && s->run_limiter == 1 && s->run_bias == 0.0 && s->nx >= 8 && s->ny >= 8 &&
+           s->ws_mode != 1;
+}
+
 // Queue one solve-loop iteration reading its dt from the device StepState.
 int queue_step(cosmos_solver* s) {
     hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_step_begin(s->st, s->stream); });
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(e));
     const int spp = sweeps_per_step(s);
     const int b0 = (int)((s->cur_begin + (long)spp * s->queued) & 1);
+    if (fused_step(s)) {
+        SweepLaunch L = base_launch(s, b0, s->run_order, s->run_bias, s->run_limiter);
+        L.step = s->st;
+        e = timed(s, kProfStepFused, [&] { return cfv::launch_step_fused(L, s->stream); });
+        if (e != hipSuccess)
+            return fail(s, COSMOS_ERR_DEVICE, std::string("fused step launch: ") + hipGetErrorString(e));
+        s->queued += 1;
+        s->unsettled = true;
+        return COSMOS_OK;
+    }
     for (int k = 0; k < spp; ++k) {
         SweepLaunch L = base_launch(s, b0 ^ (k & 1), s->run_order, s->run_bias, s->run_limiter);
         L.step = s->st;
@@ -436,6 +457,15 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode) {
     });
 }
 
+int cosmos_set_step_fusion(cosmos_solver* s, int enable) {
+    return guarded(s, [&] {
+        if (enable != 0 && enable != 1) return fail(s, COSMOS_ERR_ARGUMENT, "step fusion must be 0 or 1");
+        s->fuse_step = enable;
+        s->run_ready = false;
+        return COSMOS_OK;
+    });
+}
+
 int cosmos_set_y_tile(cosmos_solver* s, int columns) {
     return guarded(s, [&] {
         if (columns != -1 && columns != 0 && columns != 8 && columns != 16)
@@ -614,6 +644,10 @@ int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, d
         if (rc) return rc;
         if (s->physics == COSMOS_BLACKHOLE)
             return fail(s, COSMOS_ERR_ARGUMENT, "black-hole solvers step with cosmos_solve_forced");
+        // the steps queued by an earlier run decide which buffer holds the state;
+        // read that before the new run's StepState replaces their count
+        rc = settle(s);
+        if (rc) return rc;
         StepState h{};
         h.t = 0.0;
         h.tf = final_time;

@@ -132,16 +132,17 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
 // ---------------------------------------------------------------------------
 // y sweep: thread per column, marching down ROWS rows
 // ---------------------------------------------------------------------------
-// Minimum waves per SIMD, which caps the VGPRs: 4 (128 VGPRs) for the
-// HBM-bound fast Newtonian sweep, which fits without spilling; the exact
-// Newtonian sweep would spill 44-68 B per lane at that cap and the
-// special-relativistic one 172-200 B (its con2prim state), so they may use
-// up to 168 (3 waves) and 256 (2 waves) VGPRs.
-template <class Phys>
-constexpr int y_min_waves() { return Phys::kPhysics == 1 ? 2 : (Phys::kFast ? 4 : 3); }
+// Minimum waves per SIMD, which caps the VGPRs: 4 (128 VGPRs) for the fast
+// Newtonian sweep without the z-velocity plane, which fits without spilling.
+// At that cap the fast sweep with the plane would spill 20 B per lane, the
+// exact Newtonian sweep 44-68 B and the special-relativistic one 172-200 B
+// (its con2prim state), so they may use up to 168 (3 waves), 168 and 256
+// (2 waves) VGPRs.
+template <class Phys, bool W0>
+constexpr int y_min_waves() { return Phys::kPhysics == 1 ? 2 : (Phys::kFast && W0 ? 4 : 3); }
 
 template <class Phys, int ORDER, int LIMMODE, bool WS, bool W0, int ROWS>
-__global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min_waves<Phys>(), 8))) void k_sweep_y(
+__global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min_waves<Phys, W0>(), 8))) void k_sweep_y(
     SweepLaunch L) {
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
@@ -188,6 +189,118 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
         if (L.step->ws_slot_atomics) wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
         else block_max_store<NT>(m, L.step->ws_part, &lds[ORDER == 2 ? 5 : 0][0]);
     }
+}
+
+// ---------------------------------------------------------------------------
+// One whole 2D step X(dt/2) Y(dt) X(dt/2) in one pass over HBM (fast mode).
+// A block owns a strip of kFuseCols output columns (256 threads: 4 halo
+// columns each side) and marches down RB output rows.  For each input row it
+// runs the first x sweep on the row (LDS exchange, as k_sweep_x), feeds the
+// result into the y sweep's three-row register window (as k_sweep_y), and
+// every y output row goes straight through the second x sweep and out to HBM
+// with the wave-speed maximum.  Per-cell arithmetic is the three separate
+// sweeps', so the result is bitwise theirs; each cell is read and written once
+// per step instead of three times, at the cost of recomputing the first x sweep
+// on 4 halo rows per RB and every sweep on 8 halo columns per 256.
+// Boundaries: columns through mirror2 at load (the first sweep's ghosts); the
+// second sweep's ghost columns are the y outputs of the mirrored columns, which
+// the y sweep produces from the first sweep's outputs of those columns: blocks
+// at the grid's left or right edge copy them across through LDS after the first
+// sweep.  Rows through mirror2 as in the y march.
+// ---------------------------------------------------------------------------
+constexpr int kFuseCols = kXThreads - 8;
+
+template <class Phys, int LIMMODE, bool W0>
+__global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_step_fused(SweepLaunch L,
+                                                                                                     int RB) {
+    constexpr int NT = kXThreads;
+    __shared__ double lds[15][NT];
+    if (L.step->done) return;
+    const SweepCoef ch = L.step->half, cfu = L.step->full;
+    const PhysParams& P = L.P;
+    const int nx = L.nx, ny = L.ny;
+    const long pitch = L.pitch;
+    const int tx = (nx + kFuseCols - 1) / kFuseCols;
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int by = blk / tx;
+    const int bx = blk - by * tx;
+    const int t = threadIdx.x;
+    const int base = bx * kFuseCols - 4;
+    const int p = base + t;              // the thread's column (may lie outside the grid)
+    const int pc = mirror2(p, nx);       // the column it loads
+    const bool edge = base < 0 || base + NT > nx;
+    const int i0 = by * RB;
+    const int i1 = min(i0 + RB, ny);
+
+    // first x sweep of (mirrored) row rr -> Wx
+    auto x1_row = [&](int rr, double* Wx) {
+        double Wr[5];
+        load_cell<W0>(L.in, (long)rr * pitch + pc, Wr);
+        __syncthreads();
+        line_step<Phys, 2, LIMMODE, W0, NT, 0, 1>(Wr, Wx, ch, 0.0, 1, P, lds);
+        if (edge) {
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 5; ++k)
+                if (!(W0 && k == 3)) lds[k][t] = Wx[k];
+            __syncthreads();
+            if (p < 0 || p >= nx) {
+                // the mirror column's slot: in [4, 8) on the left, >= 1 on the
+                // right (>= 3 for the columns an output needs); clamped for the
+                // far columns no output reads
+                int src = pc - base;
+                src = src < 0 ? 0 : (src > NT - 1 ? NT - 1 : src);
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                    if (!(W0 && k == 3)) Wx[k] = lds[k][src];
+            }
+        }
+    };
+
+    double wsmax = 0.0;
+    double Ua[5], Ub[5], Q1p[5], G1p[5], Fp[5], Wn[5];
+    x1_row(mirror2(i0 - 2, ny), Wn);
+#pragma unroll 1
+    for (int r = i0 - 2; r <= i1 + 1; ++r) {
+        double W[5], U[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) W[k] = Wn[k];
+        if (r < i1 + 1) x1_row(mirror2(r + 1, ny), Wn);
+        Phys::p2c(W, U, P);
+        if (r >= i0) {
+            double Q0[5], G0[5], Q1[5], G1[5];
+            half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cfu, 0.0, 1, P, Q0, G0, Q1, G1);
+            if (r >= i0 + 1) {
+                double F[5];
+                force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cfu, P, F);
+                if (r >= i0 + 2) {
+                    // y output of row r - 2 -> second x sweep -> HBM
+                    double Wo[5], Wz[5];
+                    update_cell<Phys, W0>(Ua, Fp, F, cfu, P, Wo);
+                    __syncthreads();
+                    line_step<Phys, 2, LIMMODE, W0, NT, 0, 1>(Wo, Wz, ch, 0.0, 1, P, lds);
+                    if (t >= 4 && t < NT - 4 && p < nx) {
+                        store_cell<W0>(L.out, (long)(r - 2) * pitch + p, Wz);
+                        const double w = Phys::template wavespeed<2>(Wz, P);
+                        wsmax = (w > wsmax) ? w : wsmax;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 5; ++k) Fp[k] = F[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                Q1p[k] = Q1[k];
+                G1p[k] = G1[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            Ua[k] = Ub[k];
+            Ub[k] = U[k];
+        }
+    }
+    wave_max_to_slot(wsmax, L.step->ws_slots, blockIdx.x * (NT / 64) + (t >> 6));
 }
 
 // ---------------------------------------------------------------------------
@@ -399,6 +512,55 @@ hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, i
         else hipLaunchKernelGGL((k_wavespeed<Relativistic, 2>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
     }
     return hipGetLastError();
+}
+
+// Rows per block of the fused step: a block's time grows as about 3 RB + 4
+// row passes (the first x sweep also runs on the 4 halo rows), and the blocks
+// run in ceil(blocks / resident) rounds, so pick RB in [16, 64] minimising
+// rounds * (3 RB + 4); ties go to the larger RB (fewer halo rows).
+int fused_rows(int nx, int ny, int resident) {
+    const long tx = (nx + kFuseCols - 1) / kFuseCols;
+    int best = 64;
+    long best_cost = -1;
+    for (int rb = 64; rb >= 16; --rb) {
+        const long blocks = tx * ((ny + rb - 1) / rb);
+        const long rounds = (blocks + resident - 1) / resident;
+        const long cost = rounds * (3L * rb + 4);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = rb;
+        }
+    }
+    return best;
+}
+
+template <bool W0>
+hipError_t launch_fused_w(const SweepLaunch& L, hipStream_t s) {
+    // resident blocks of the kernel on the current device, once per device
+    static int resident[64] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= 64) return hipErrorInvalidDevice;
+    if (resident[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_step_fused<NewtonianFastT<W0>, 2, W0>,
+                                                             kXThreads, 0);
+        if (e != hipSuccess) return e;
+        resident[dev] = std::max(1, cus * per_cu);
+    }
+    const int rb = fused_rows(L.nx, L.ny, resident[dev]);
+    const int tx = (L.nx + kFuseCols - 1) / kFuseCols;
+    const int ty = (L.ny + rb - 1) / rb;
+    dim3 grid((unsigned)(tx * ty)), block(kXThreads);
+    hipLaunchKernelGGL((k_step_fused<NewtonianFastT<W0>, 2, W0>), grid, block, 0, s, L, rb);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_fused(const SweepLaunch& L, hipStream_t s) {
+    return L.w_zero ? launch_fused_w<true>(L, s) : launch_fused_w<false>(L, s);
 }
 
 hipError_t launch_step_begin(StepState* st, hipStream_t s) {

@@ -133,6 +133,10 @@ hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);
+// One whole 2D step X(dt/2) Y(dt) X(dt/2) of the fast Newtonian SLIC / van
+// Leer / bias 0 update in one kernel (coefficients from L.step, the wave-speed
+// maximum merged into the step's slots).  Needs nx, ny >= 8 and slot atomics.
+hipError_t launch_step_fused(const SweepLaunch& L, hipStream_t s);
 // Blocks of an x-sweep launch (the number of per-block maxima it writes).
 int sweep_x_blocks(int nx, int ny, int order);
 hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s);

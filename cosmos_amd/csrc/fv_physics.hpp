@@ -298,8 +298,13 @@ __device__ __forceinline__ double slope_component(double ul, double um, double u
 // ---------------------------------------------------------------------------
 // Fast mode (Newtonian, opt-in).  The same equations, re-associated for
 // throughput: one reciprocal of the density per recovery (v_rcp_f64 and two
-// Newton steps) with products instead of quotients, total energy
-// E = rho v^2 / 2 + p / (gamma - 1) without a division, and the flux of an
+// Newton steps) with products instead of quotients; fused multiply-adds; the
+// pressure straight from the conserved variables,
+//   p = (gamma - 1) (E - |m|^2 / (2 rho)),  m = rho v;
+// the flux in conserved form,
+//   F = (m_n, m u_n + p e_n, u_n (E + p)),  u_n = m_n / rho,
+// (so F[0] is the momentum itself instead of rho (m_n / rho)); total energy
+// E = rho v^2 / 2 + p / (gamma - 1) without a division; and the flux of an
 // evolved face state taken from the state itself rather than through a
 // prim -> cons -> prim round trip.  Results differ from the reference by
 // rounding only (not bitwise); the tests bound the difference after the
@@ -319,55 +324,52 @@ struct NewtonianFastT {
     static constexpr bool kFast = true;
 
     __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
-        double vsq = (W[1] * W[1]) + (W[2] * W[2]);
-        if (!W0) vsq = vsq + (W[3] * W[3]);
+        double vsq = __builtin_fma(W[1], W[1], W[2] * W[2]);
+        if (!W0) vsq = __builtin_fma(W[3], W[3], vsq);
         U[0] = W[0];
         U[1] = W[0] * W[1];
         U[2] = W[0] * W[2];
         U[3] = W0 ? 0.0 : W[0] * W[3];
-        U[4] = (W[0] * (0.5 * vsq)) + (W[4] * P.inv_gm1);
+        U[4] = __builtin_fma(W[0], 0.5 * vsq, W[4] * P.inv_gm1);
     }
 
-    __device__ static __forceinline__ void recover(const double* U, double* R, double& E, double irho,
-                                                   const PhysParams& P) {
-        double rho = U[0];
-        double u = U[1] * irho;
-        double v = U[2] * irho;
-        double w = W0 ? 0.0 : U[3] * irho;
-        double vsq = (u * u) + (v * v);
-        if (!W0) vsq = vsq + (w * w);
-        E = U[4] * irho;
-        R[0] = rho;
-        R[1] = u;
-        R[2] = v;
-        R[3] = w;
-        R[4] = ((E - (0.5 * vsq)) * P.gm1) * rho;
+    // p = (gamma - 1) (E - |m|^2 irho / 2)
+    __device__ static __forceinline__ double pressure(const double* U, double irho, const PhysParams& P) {
+        double m2 = __builtin_fma(U[1], U[1], U[2] * U[2]);
+        if (!W0) m2 = __builtin_fma(U[3], U[3], m2);
+        return P.gm1 * __builtin_fma(-0.5 * irho, m2, U[4]);
     }
 
+    // flux along AXIS from the conserved vector, its 1 / rho and pressure
     template <int AXIS>
-    __device__ static __forceinline__ void flux_of(const double* R, double E, double* F) {
-        NewtonianT<W0>::template flux_of<AXIS>(R, E, F);
+    __device__ static __forceinline__ void flux_cons(const double* U, double irho, double p, double* F) {
+        const double un = U[1 + AXIS] * irho;
+        F[0] = U[1 + AXIS];
+        F[1] = AXIS == 0 ? __builtin_fma(U[1], un, p) : U[1] * un;
+        F[2] = AXIS == 1 ? __builtin_fma(U[2], un, p) : U[2] * un;
+        F[3] = W0 ? 0.0 : U[3] * un;
+        F[4] = un * (U[4] + p);
     }
 
     __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P) {
-        double E;
-        recover(U, W, E, irho, P);
+        W[0] = U[0];
+        W[1] = U[1] * irho;
+        W[2] = U[2] * irho;
+        W[3] = W0 ? 0.0 : U[3] * irho;
+        W[4] = pressure(U, irho, P);
     }
 
     template <int AXIS>
     __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P) {
-        double R[5], E;
-        recover(U, R, E, irho, P);
-        flux_of<AXIS>(R, E, F);
+        flux_cons<AXIS>(U, irho, pressure(U, irho, P), F);
     }
 
     // primitives and flux of U from one recovery
     template <int AXIS>
     __device__ static __forceinline__ void c2p_flux(const double* U, double* W, double* F, double irho,
                                                     const PhysParams& P) {
-        double E;
-        recover(U, W, E, irho, P);
-        flux_of<AXIS>(W, E, F);
+        c2p(U, W, irho, P);
+        flux_cons<AXIS>(U, irho, W[4], F);
     }
 
     __device__ static __forceinline__ double inv_density(const double* U) { return fast_rcp(U[0]); }

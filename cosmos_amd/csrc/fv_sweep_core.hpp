@@ -59,8 +59,13 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
         }
         double s = LIMMODE == 2 ? slope_component_fast_vl(ul[k], um[k], ur[k])
                                 : slope_component<LIMMODE>(ul[k], um[k], ur[k], bias, limiter);
-        xl[k] = um[k] - (0.5 * s);
-        xr[k] = um[k] + (0.5 * s);
+        if constexpr (Phys::kFast) {
+            xl[k] = __builtin_fma(-0.5, s, um[k]);
+            xr[k] = __builtin_fma(0.5, s, um[k]);
+        } else {
+            xl[k] = um[k] - (0.5 * s);
+            xr[k] = um[k] + (0.5 * s);
+        }
     }
     Phys::template flux<AXIS>(xl, fl, Phys::inv_density(xl), P);
     Phys::template flux<AXIS>(xr, fr, Phys::inv_density(xr), P);
@@ -99,6 +104,23 @@ template <class Phys, int AXIS, bool W0>
 __device__ __forceinline__ void force_flux(const double* UL, const double* FL, const double* UR, const double* FR,
                                            const SweepCoef& cf, const PhysParams& P, double* F) {
     double us[5], fs[5];
+    if constexpr (Phys::kFast) {
+        // U* = (UL + UR) / 2 + (dt / 2dx)(FL - FR);  F = (FL + FR) / 4 + (dx / 4dt)(UL - UR) + F(U*) / 2
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            us[k] = (W0 && k == 3) ? 0.0 : __builtin_fma(cf.hdt_dx, FL[k] - FR[k], 0.5 * (UL[k] + UR[k]));
+        Phys::template flux<AXIS>(us, fs, Phys::inv_density(us), P);
+        const double q = 0.25 * cf.dx_dt;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (W0 && k == 3) {
+                F[k] = 0.0;
+                continue;
+            }
+            F[k] = __builtin_fma(0.5, fs[k], __builtin_fma(q, UL[k] - UR[k], 0.25 * (FL[k] + FR[k])));
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < 5; ++k)
         us[k] = (W0 && k == 3) ? 0.0 : 0.5 * ((UL[k] + UR[k]) + (cf.dt_dx * (FL[k] - FR[k])));
@@ -120,7 +142,11 @@ __device__ __forceinline__ void update_cell(const double* U, const double* Fl, c
                                             const PhysParams& P, double* W) {
     double un[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) un[k] = (W0 && k == 3) ? 0.0 : U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
+    for (int k = 0; k < 5; ++k) {
+        if (W0 && k == 3) un[k] = 0.0;
+        else if constexpr (Phys::kFast) un[k] = __builtin_fma(cf.dt_dx, Fl[k] - Fr[k], U[k]);
+        else un[k] = U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
+    }
     Phys::c2p(un, W, Phys::inv_density(un), P);
 }
 
@@ -135,9 +161,11 @@ __device__ __forceinline__ void update_cell(const double* U, const double* Fl, c
 // lds: 15 rows of NT doubles; the caller must barrier before reusing it.
 // Returns the wave speed of the output cell (the 1D or 2D form by ws_dim; 0
 // when ws_dim is 0 or the cell is not an output).
+// The stages of one such pass on values held by the threads: W, the thread's
+// input primitives, gives Wn, its updated primitives (meaningful where the
+// thread's cell has its full stencil in the block).
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>
-__device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
-                                          bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
+__device__ __forceinline__ void line_step(const double* W, double* Wn, const SweepCoef& cf, double bias, int limiter,
                                           const PhysParams& P, double (*lds)[NT]) {
     //   SLIC:  sA = rows 0-4   stage A U, then the interface fluxes
     //          sB = rows 5-14  stage B Q0, G0
@@ -146,8 +174,7 @@ __device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes
     double(*sB)[NT] = lds + (ORDER == 2 ? 5 : 10);
     const int s = threadIdx.x;
 
-    double W[5], U[5], F[5];
-    load_cell<W0>(in, ld_idx, W);
+    double U[5], F[5];
     Phys::p2c(W, U, P);
 
     const int sl = s >= STRIDE ? s - STRIDE : 0;
@@ -205,10 +232,19 @@ __device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes
             if (!(W0 && k == 3)) sB[k][s] = F[k];
         __syncthreads();
     }
-    double Fl[5], Wn[5];
+    double Fl[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : (ORDER == 2 ? sA[k][sl] : sB[k][sl]);
     update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
+}
+
+template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>
+__device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
+                                          bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
+                                          const PhysParams& P, double (*lds)[NT]) {
+    double W[5], Wn[5];
+    load_cell<W0>(in, ld_idx, W);
+    line_step<Phys, ORDER, LIMMODE, W0, NT, AXIS, STRIDE>(W, Wn, cf, bias, limiter, P, lds);
     if (valid) store_cell<W0>(out, st_idx, Wn);
     double m = 0.0;
     if (ws_dim != 0 && valid) {

@@ -83,6 +83,12 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode);
  * -1: the default (the march for Newtonian, 8-column tiles for special
  * relativistic). */
 int cosmos_set_y_tile(cosmos_solver* s, int columns);
+/* Step fusion (no counterpart in the reference; results do not depend on it):
+ * with 1 (the default), a 2D fast-mode Newtonian SLIC step with van Leer and
+ * bias 0 queued by cosmos_begin / cosmos_advance / cosmos_solve on a grid of at
+ * least 8 x 8 cells runs its X(dt/2) Y(dt) X(dt/2) sweeps as one kernel that
+ * reads and writes each cell once; 0 queues the three sweeps. */
+int cosmos_set_step_fusion(cosmos_solver* s, int enable);
 
 /* Con2prim controls.  Special relativistic: the stopping tolerance and
  * iteration cap (default: the reference's 1e-8 and 1000,
@@ -292,9 +298,12 @@ void* cosmos_stream(cosmos_solver* s);
  * kernel queued by cosmos_advance is bracketed by HIP events on the handle's
  * stream.  cosmos_profile_read synchronises and returns, for kinds
  * 0 step head (dt), 1 x sweep, 2 y sweep, 3 x sweep with the fused wave-speed
- * maximum, the summed milliseconds and launch counts since the last read. */
+ * maximum, 4 fused whole step (cosmos_set_step_fusion), the summed
+ * milliseconds and launch counts since the last read (arrays of
+ * COSMOS_PROFILE_KINDS entries). */
+#define COSMOS_PROFILE_KINDS 5
 int cosmos_profile_enable(cosmos_solver* s, int on);
-int cosmos_profile_read(cosmos_solver* s, double* total_ms4, int* launches4);
+int cosmos_profile_read(cosmos_solver* s, double* total_ms, int* launches);
 
 /* A plain device copy of `bytes` (a multiple of 16, 16-byte aligned) on a HIP
  * stream (NULL: the default stream), for bandwidth and counter calibration

@@ -256,6 +256,83 @@ def test_fast_mode_config2_full_size_100_steps(orc):
     assert k == 100 and err <= FAST_TOL, err
 
 
+def fast_pair(W, steps, tf=math.inf):
+    """The same fast-mode solve with the fused step kernel and with three sweeps."""
+    from cosmos_amd import FAST
+
+    ny, nx = W.shape[:2]
+    res = []
+    for fuse in (True, False):
+        with Solver(NEWTONIAN, 2, nx, ny, G_N) as s:
+            s.set_mode(FAST)
+            s.set_step_fusion(fuse)
+            s.upload(W)
+            t, k = s.solve(2, 1.0 / max(nx, ny), 0.9, tf, steps)
+            res.append((s.download(), t, k))
+    return res
+
+
+@pytest.mark.parametrize("zvel", [False, True])
+def test_fused_step_is_bitwise_the_three_sweeps(zvel):
+    """The fused X(dt/2) Y(dt) X(dt/2) kernel against the three fast sweeps:
+    bitwise equal states, times and step counts, on shapes at and around its
+    248-column strips and 64-row blocks (edge strips that need the mirrored
+    ghost columns on both sides, a single strip narrower than 248, a last
+    strip of one column), with and without the z-velocity plane."""
+    shapes = ((8, 8), (9, 13), (247, 70), (248, 65), (249, 130), (250, 64), (253, 9), (497, 129), (744, 200),
+              (1000, 67))
+    for idx, (nx, ny) in enumerate(shapes):
+        W = workloads.quadrants_2d(nx, seed=20 + idx, ny=ny)
+        if zvel:
+            W[..., 3] = 0.1 * W[..., 2]
+        (a, ta, ka), (b, tb, kb) = fast_pair(W, 9)
+        assert ka == kb == 9 and ta == tb, (nx, ny, ta, tb)
+        assert same(a, b), (nx, ny, rel_linf(a, b))
+    # final-time clamp and early stop inside the queued steps
+    W = workloads.quadrants_2d(300, seed=31, ny=140)
+    (a, ta, ka), (b, tb, kb) = fast_pair(W, 1000, tf=0.01)
+    assert ka == kb and ta == tb and 0 < ka < 1000 and same(a, b)
+
+
+def test_consecutive_solves_continue_from_the_last_state(orc):
+    """A second cosmos_solve starts from the state the first one left (its
+    queued steps settled first), in exact mode bitwise the oracle's two runs,
+    and in fast mode the same with and without the fused step."""
+    from cosmos_amd import FAST
+
+    W = workloads.quadrants_2d(96, seed=33, ny=80)
+    e1, _, _ = orc.run(W, 0, 2, 1.0 / 96, 0.9, max_steps=5)
+    e2, te, ke = orc.run(e1, 0, 2, 1.0 / 96, 0.9, max_steps=7)
+    with Solver(NEWTONIAN, 2, 96, 80, G_N) as s:
+        s.upload(W)
+        s.solve(2, 1.0 / 96, 0.9, math.inf, 5)
+        t, k = s.solve(2, 1.0 / 96, 0.9, math.inf, 7)
+        assert same(s.download(), e2) and (t, k) == (te, ke)
+    outs = []
+    for fuse in (True, False):
+        with Solver(NEWTONIAN, 2, 96, 80, G_N) as s:
+            s.set_mode(FAST)
+            s.set_step_fusion(fuse)
+            s.upload(W)
+            s.solve(2, 1.0 / 96, 0.9, math.inf, 5)
+            outs.append((s.solve(2, 1.0 / 96, 0.9, math.inf, 7), s.download()))
+    from conftest import cons_rel_linf
+
+    assert outs[0][0] == outs[1][0] and same(outs[0][1], outs[1][1])
+    assert cons_rel_linf(outs[0][1], e2) <= FAST_TOL
+
+
+def test_fused_step_full_size_against_oracle(orc):
+    """Fused fast steps at 1024^2 against the reference arithmetic within 1e-11."""
+    from conftest import cons_rel_linf
+
+    W = workloads.quadrants_2d(1024, seed=5)
+    (a, ta, ka), (b, tb, kb) = fast_pair(W, 40)
+    exp, te, ke = orc.run(W, 0, 2, 1.0 / 1024, 0.9, max_steps=40)
+    assert same(a, b) and ka == ke == 40 and abs(ta - te) <= FAST_TOL * te
+    assert cons_rel_linf(a, exp) <= FAST_TOL
+
+
 @pytest.mark.parametrize("columns", [0, 8, 16])
 @pytest.mark.parametrize("physics", [NEWTONIAN, RELATIVISTIC])
 def test_y_sweep_layout_does_not_change_results(orc, columns, physics):
