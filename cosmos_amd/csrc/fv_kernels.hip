@@ -232,10 +232,10 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
     const int i0 = by * RB;
     const int i1 = min(i0 + RB, ny);
 
-    // first x sweep of (mirrored) row rr -> Wx
-    auto x1_row = [&](int rr, double* Wx) {
-        double Wr[5];
-        load_cell<W0>(L.in, (long)rr * pitch + pc, Wr);
+    // index of the thread's cell of (mirrored) input row rr
+    auto at_row = [&](int rr) { return (long)mirror2(rr, ny) * pitch + pc; };
+    // first x sweep of one input row (the thread's primitives Wr) -> Wx
+    auto x1_row = [&](const double* Wr, double* Wx) {
         __syncthreads();
         line_step<Phys, 2, LIMMODE, W0, NT, 0, 1>(Wr, Wx, ch, 0.0, 1, P, lds);
         if (edge) {
@@ -257,15 +257,22 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
         }
     };
 
+    // Input rows i0 - 2 .. i1 + 1; each is loaded one row ahead of its first
+    // x sweep (Wpre), so the load has a whole row's work to arrive.
     double wsmax = 0.0;
-    double Ua[5], Ub[5], Q1p[5], G1p[5], Fp[5], Wn[5];
-    x1_row(mirror2(i0 - 2, ny), Wn);
+    double Ua[5], Ub[5], Q1p[5], G1p[5], Fp[5], Wn[5], Wpre[5];
+    load_cell<W0>(L.in, at_row(i0 - 2), Wpre);
+    x1_row(Wpre, Wn);
+    load_cell<W0>(L.in, at_row(i0 - 1), Wpre);
 #pragma unroll 1
     for (int r = i0 - 2; r <= i1 + 1; ++r) {
         double W[5], U[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) W[k] = Wn[k];
-        if (r < i1 + 1) x1_row(mirror2(r + 1, ny), Wn);
+        if (r < i1 + 1) {
+            x1_row(Wpre, Wn);  // row r + 1
+            if (r < i1) load_cell<W0>(L.in, at_row(r + 2), Wpre);
+        }
         Phys::p2c(W, U, P);
         if (r >= i0) {
             double Q0[5], G0[5], Q1[5], G1[5];

@@ -384,24 +384,30 @@ struct NewtonianFastT {
     }
 };
 
+// minmod(a, b): the argument of smaller magnitude when both have the same
+// sign, else 0, as max(min(a, b), min(max(a, b), 0)) (four FP64 min / max).
+__device__ __forceinline__ double minmod_fast(double a, double b) {
+    return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
+}
+
 // Fast van Leer at bias 0.  With the clamped differences num, den of equal
 // sign, 2r/(1+r) = 2 num/(num+den) and 2/(1+r) = 2 den/(num+den); opposite
 // signs (r < 0) give 0.  The limited slope s phi, s = (dl + dr)/2, then
 // simplifies where neither difference is clamped (num + den = dl + dr): it is
-// the one-sided difference of smaller magnitude.  Where both are clamped,
-// phi = 1 and the slope is s.  Only a single clamped difference needs the
-// quotient.
+// minmod(dl, dr).  Where both are clamped, phi = 1 and the slope is s.  Only a
+// single clamped difference needs the quotient, s 2 minmod(num, den) / (num + den).
 __device__ __forceinline__ double slope_component_fast_vl(double ul, double um, double ur) {
-    double dl = um - ul;
-    double dr = ur - um;
-    double s = (0.5 * dl) + (0.5 * dr);
+    const double dl = um - ul;
+    const double dr = ur - um;
+    const double s = 0.5 * (dl + dr);
     const bool cl = __builtin_fabs(dl) < 1e-5, cr = __builtin_fabs(dr) < 1e-5;
-    double num = cl ? 1e-5 : dl;
-    double den = cr ? 1e-5 : dr;
-    bool same = (num < 0.0) == (den < 0.0);
-    double top = (__builtin_fabs(num) <= __builtin_fabs(den)) ? num : den;
-    double out = cl ? s : (same ? top : 0.0);
-    if (cl != cr) out = same ? s * ((2.0 * top) * fast_rcp(num + den)) : 0.0;
+    double out = (cl && cr) ? s : minmod_fast(dl, dr);
+    if (cl != cr) {
+        const double num = cl ? 1e-5 : dl;
+        const double den = cr ? 1e-5 : dr;
+        const double m = minmod_fast(num, den);
+        out = (m != 0.0) ? s * ((2.0 * m) * fast_rcp(num + den)) : 0.0;
+    }
     return out;
 }
 
