@@ -44,6 +44,11 @@ BYTES_PER_CELL_SWEEP = 80.0      # 5 FP64 read + 5 FP64 written
 SWEEPS_PER_STEP = 3
 KIND_NAMES = ["step_head", "sweep_x", "sweep_y", "sweep_x_wavespeed", "step_fused"]
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "sweeps_4096_traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "step_kernels_4096_valu.json")
+# MI355X at its 2.4 GHz peak engine clock: 256 CUs x 4 SIMDs, one 64-lane VALU
+# instruction (FP64 included) per SIMD every 4 cycles; FP64 FMA peak 78.6 TFLOP/s
+VALU_PEAK_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+FP64_PEAK_TFLOPS = 78.6
 
 
 def measured_traffic(size, mode, kernel):
@@ -59,6 +64,36 @@ def measured_traffic(size, mode, kernel):
         return float(k["fetch_bytes"] + k["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def measured_valu(size, mode, kernel):
+    """VALU wave instructions and FP64 FLOPs per launch of `kernel` from the
+    committed rocprofv3 SQ counter passes, or None when the profiled workload
+    is not this one."""
+    try:
+        with open(VALU_FILE) as f:
+            d = json.load(f)
+        if d["grid"] != size:
+            return None
+        k = d["kernels"][mode][kernel]
+        return float(k["valu_wave_instructions"]), float(k["fp64_flop"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def issue_roofline(size, mode, kernel, avg_ms):
+    """The dominant kernel against the VALU issue rate and the FP64 peak."""
+    v = measured_valu(size, mode, kernel)
+    if v is None:
+        return None
+    instr, flop = v
+    rate = instr / (avg_ms * 1e-3)
+    tflops = flop / (avg_ms * 1e-3) / 1e12
+    return {"valu_instr_per_s": rate, "valu_peak_instr_per_s": VALU_PEAK_INSTR_PER_S,
+            "valu_frac": rate / VALU_PEAK_INSTR_PER_S, "fp64_tflops": tflops, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+            "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+            "source": "rocprofv3 SQ_INSTS_VALU and SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 per launch (committed), over "
+                      "the live average launch time"}
 
 
 def measured_step_traffic(size, mode, kernels, steps):
@@ -657,6 +692,7 @@ def main():
             "traffic": measured_traffic(args.size, args.mode, g["dominant"]),
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "algorithmic_bytes": BYTES_PER_CELL_SWEEP * args.size * args.size,
+            "issue": issue_roofline(args.size, args.mode, g["dominant"], g["dominant_avg_ms"]),
         },
         "kernels": g["kernels"],
         "modes": {m: {"cell_updates_per_s": r["cell_updates_per_s"], "ms_per_step": r["ms_per_step"],

@@ -43,6 +43,7 @@ struct cosmos_solver {
     int y_tile = -1;                           // y-sweep layout: 0 march, 8 / 16 tile columns (-1: by physics)
     int ws_mode = 0;                           // fused wave-speed hand-off (0: default)
     int fuse_step = 1;                         // fast mode: one fused kernel per 2D step where it applies
+    int fuse_rows = 0;                         // its output rows per block (0: by occupancy)
     cfv::Hole hole{};                          // COSMOS_BLACKHOLE: the spacetime
     double excision = 0.0;                     // > 0: excision radius override
     double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
@@ -156,6 +157,7 @@ SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, in
     L.fuse_ws = 0;
     L.y_rows = s->y_rows;
     L.y_tile = s->y_tile;
+    L.fuse_rows = s->fuse_rows;
     return L;
 }
 
@@ -457,10 +459,13 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode) {
     });
 }
 
-int cosmos_set_step_fusion(cosmos_solver* s, int enable) {
+int cosmos_set_step_fusion(cosmos_solver* s, int enable, int rows_per_block) {
     return guarded(s, [&] {
         if (enable != 0 && enable != 1) return fail(s, COSMOS_ERR_ARGUMENT, "step fusion must be 0 or 1");
+        if (rows_per_block != 0 && (rows_per_block < 4 || rows_per_block > 1024))
+            return fail(s, COSMOS_ERR_ARGUMENT, "fused rows per block must be 0 (automatic) or in [4, 1024]");
         s->fuse_step = enable;
+        s->fuse_rows = rows_per_block;
         s->run_ready = false;
         return COSMOS_OK;
     });

@@ -558,7 +558,7 @@ hipError_t launch_fused_w(const SweepLaunch& L, hipStream_t s) {
         if (e != hipSuccess) return e;
         resident[dev] = std::max(1, cus * per_cu);
     }
-    const int rb = fused_rows(L.nx, L.ny, resident[dev]);
+    const int rb = L.fuse_rows > 0 ? L.fuse_rows : fused_rows(L.nx, L.ny, resident[dev]);
     const int tx = (L.nx + kFuseCols - 1) / kFuseCols;
     const int ty = (L.ny + rb - 1) / rb;
     dim3 grid((unsigned)(tx * ty)), block(kXThreads);

@@ -124,6 +124,7 @@ struct SweepLaunch {
     int use_full;               // which StepState coefficients (0 half, 1 full)
     int fuse_ws;                // merge the max wave speed of the output into step->ws_slots
     int y_rows;                 // rows per y-sweep thread (16, 32, 64; 0: by physics)
+    int fuse_rows;  // fused step: output rows per block (0: chosen from the occupancy)
     int y_tile;                 // y-sweep layout: 0 column march, 8 or 16 columns per LDS tile; -1: by physics
 };
 

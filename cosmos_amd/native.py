@@ -41,7 +41,7 @@ SIGNATURES = {
     "cosmos_set_con2prim": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_int]),
     "cosmos_set_launch": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int]),
     "cosmos_set_y_tile": (ctypes.c_int, [_handle, ctypes.c_int]),
-    "cosmos_set_step_fusion": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cosmos_set_step_fusion": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int]),
     "cosmos_set_spacetime": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_double]),
     "cosmos_upload": (ctypes.c_int, [_handle, _c_double_p]),

@@ -100,10 +100,11 @@ class Solver:
         """y-sweep layout: 0 column march, 8 or 16 LDS tile columns, -1 default by physics."""
         self._check(self._lib.cosmos_set_y_tile(self._h, columns))
 
-    def set_step_fusion(self, enable: bool = True) -> None:
+    def set_step_fusion(self, enable: bool = True, rows_per_block: int = 0) -> None:
         """Fast mode, 2D Newtonian SLIC / van Leer / bias 0 on >= 8 x 8 cells: run
-        each queued step as one fused X Y X kernel (default) or as three sweeps."""
-        self._check(self._lib.cosmos_set_step_fusion(self._h, 1 if enable else 0))
+        each queued step as one fused X Y X kernel (default) or as three sweeps;
+        rows_per_block is the fused kernel's launch shape (0: automatic)."""
+        self._check(self._lib.cosmos_set_step_fusion(self._h, 1 if enable else 0, rows_per_block))
 
     def set_spacetime(self, mass: float = 1.0, spin: float = 0.0, x: float = 0.5, y: float = 0.5,
                       z: float = 0.0) -> None:

@@ -87,8 +87,10 @@ int cosmos_set_y_tile(cosmos_solver* s, int columns);
  * with 1 (the default), a 2D fast-mode Newtonian SLIC step with van Leer and
  * bias 0 queued by cosmos_begin / cosmos_advance / cosmos_solve on a grid of at
  * least 8 x 8 cells runs its X(dt/2) Y(dt) X(dt/2) sweeps as one kernel that
- * reads and writes each cell once; 0 queues the three sweeps. */
-int cosmos_set_step_fusion(cosmos_solver* s, int enable);
+ * reads and writes each cell once; 0 queues the three sweeps.  rows_per_block:
+ * output rows of each block of the fused kernel, a launch shape only (0: chosen
+ * from the grid and the kernel's occupancy; else 4 .. 1024). */
+int cosmos_set_step_fusion(cosmos_solver* s, int enable, int rows_per_block);
 
 /* Con2prim controls.  Special relativistic: the stopping tolerance and
  * iteration cap (default: the reference's 1e-8 and 1000,

@@ -256,8 +256,9 @@ def test_fast_mode_config2_full_size_100_steps(orc):
     assert k == 100 and err <= FAST_TOL, err
 
 
-def fast_pair(W, steps, tf=math.inf):
-    """The same fast-mode solve with the fused step kernel and with three sweeps."""
+def fast_pair(W, steps, tf=math.inf, rows=0):
+    """The same fast-mode solve with the fused step kernel (rows per block:
+    rows, 0 automatic) and with three sweeps."""
     from cosmos_amd import FAST
 
     ny, nx = W.shape[:2]
@@ -265,7 +266,7 @@ def fast_pair(W, steps, tf=math.inf):
     for fuse in (True, False):
         with Solver(NEWTONIAN, 2, nx, ny, G_N) as s:
             s.set_mode(FAST)
-            s.set_step_fusion(fuse)
+            s.set_step_fusion(fuse, rows)
             s.upload(W)
             t, k = s.solve(2, 1.0 / max(nx, ny), 0.9, tf, steps)
             res.append((s.download(), t, k))
@@ -288,6 +289,11 @@ def test_fused_step_is_bitwise_the_three_sweeps(zvel):
         (a, ta, ka), (b, tb, kb) = fast_pair(W, 9)
         assert ka == kb == 9 and ta == tb, (nx, ny, ta, tb)
         assert same(a, b), (nx, ny, rel_linf(a, b))
+    # rows per block are a launch shape only
+    for rows in (4, 13, 200):
+        W = workloads.quadrants_2d(253, seed=40 + rows, ny=150)
+        (a, ta, ka), (b, tb, kb) = fast_pair(W, 5, rows=rows)
+        assert ka == kb and ta == tb and same(a, b), rows
     # final-time clamp and early stop inside the queued steps
     W = workloads.quadrants_2d(300, seed=31, ny=140)
     (a, ta, ka), (b, tb, kb) = fast_pair(W, 1000, tf=0.01)
