@@ -5,7 +5,8 @@ this package is its Python binding, the build script and the seeded workloads.
 """
 from .native import CosmosError, LIB_PATH, load  # noqa: F401
 from .batch import Batch  # noqa: F401
-from .solver import BLACKHOLE, EXACT, FAST, MINBEE, NEWTONIAN, RELATIVISTIC, SUPERBEE, VANLEER, Adjoint, Amr, Solver  # noqa: F401
+from .solver import (BLACKHOLE, EXACT, FAST, MINBEE, NEWTONIAN, RELATIVISTIC, SUPERBEE, TAPE_NONE,  # noqa: F401
+                     TAPE_RERUN, TAPE_SEGMENT, VANLEER, Adjoint, Amr, Solver)
 
 __all__ = ["Solver", "Amr", "Adjoint", "Batch", "CosmosError", "BLACKHOLE", "load", "LIB_PATH", "NEWTONIAN", "RELATIVISTIC", "SUPERBEE", "VANLEER",
            "MINBEE"]

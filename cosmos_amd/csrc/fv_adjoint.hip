@@ -268,6 +268,13 @@ struct cosmos_adjoint {
     int every = 20;               // checkpoint interval
     std::vector<double*> checkpoints;  // state at steps 0, every, 2 every, ...
     std::vector<double*> seg;     // per-sweep states of one segment during the reverse pass
+    // black-hole forcing tapes (cosmos_adjoint_set_tape)
+    int tape_mode = COSMOS_TAPE_SEGMENT;
+    long long tape_reserve = 8LL << 30;
+    double* tape = nullptr;            // the re-run tape
+    std::vector<double*> tapes;        // one per forcing step of a segment
+    std::vector<double*> stapes;       // one per sweep of a segment: fluxes (5 planes) and face states (10)
+    long fplane = 0;                   // doubles per flux plane
     std::string err;
 };
 
@@ -445,24 +452,47 @@ int gr_launch_of(cosmos_adjoint* a, const Op& op, double bias, int limiter, cfv:
     return COSMOS_OK;
 }
 
+size_t tape_bytes(const cosmos_adjoint* a) { return sizeof(double) * cfv::kTapePlanes * a->plane; }
+// a black-hole sweep's tape: its interface fluxes and the SLIC face states
+constexpr int kSweepTapePlanes = 15;
+size_t sweep_tape_bytes(const cosmos_adjoint* a) { return sizeof(double) * kSweepTapePlanes * a->fplane; }
+
+void point_sweep_tape(const cosmos_adjoint* a, double* tape, cfv::GRLaunch& L) {
+    for (int k = 0; k < 5; ++k) L.flux.p[k] = tape + k * a->fplane;
+    L.half = L.order == 2 ? tape + 5 * a->fplane : nullptr;
+    L.half_plane = a->fplane;
+}
+
 // The operation from the planes at `in` to the planes at `out` (the
-// segment recompute of the reverse pass).
-int forward_op_dev(cosmos_adjoint* a, const Op& op, const double* in, double* out) {
+// segment recompute of the reverse pass); a forcing step with a tape given
+// writes it too.
+int forward_op_dev(cosmos_adjoint* a, const Op& op, const double* in, double* out, double* tape) {
     if (a->physics != COSMOS_BLACKHOLE) return forward_sweep_dev(a, op.axis, op.dt, in, out);
-    cfv::GRLaunch L;
-    int rc = gr_launch_of(a, op, a->bias, a->limiter, L);
+    cfv::GRAdjLaunch A{};
+    int rc = gr_launch_of(a, op, a->bias, a->limiter, A.L);
     if (rc) return rc;
-    L.in = cplanes_at(a, in);
-    L.out = planes_at(a, out);
-    ADJCHK(a, op.forcing ? cfv::launch_gr_forcing(L, stream_of(a)) : cfv::launch_gr_sweep(L, op.axis, stream_of(a)));
+    A.L.in = cplanes_at(a, in);
+    A.L.out = planes_at(a, out);
+    if (op.forcing && tape) {
+        A.tape = tape;
+        A.tape_plane = (long)a->plane;
+        ADJCHK(a, cfv::launch_gr_forcing_tape(A, stream_of(a)));
+        return COSMOS_OK;
+    }
+    if (!op.forcing && tape) point_sweep_tape(a, tape, A.L);
+    ADJCHK(a, op.forcing ? cfv::launch_gr_forcing(A.L, stream_of(a)) : cfv::launch_gr_sweep(A.L, op.axis, stream_of(a)));
     return COSMOS_OK;
 }
 
 // lam[dst] <- the adjoint of operation op from state W (planes at w) to O
-// (planes at o), given lam[src].  Black-hole sweeps re-run the forward sweep
-// (rewriting O with the same values) for its interface fluxes.
+// (planes at o), given lam[src].  A black-hole sweep reads its interface
+// fluxes and face states from `tape` when given (written by the segment
+// recompute), else re-runs the forward sweep (rewriting O with the same
+// values) for its fluxes; a forcing step reads `tape` when given, else re-runs
+// its chain into the context's tape (rewriting O with the same values) or,
+// with COSMOS_TAPE_NONE, inside the adjoint kernel.
 int adjoint_op_dev(cosmos_adjoint* a, const Op& op, int order, double bias, int limiter, const double* w, double* o,
-                   int src, int dst) {
+                   int src, int dst, double* tape = nullptr) {
     if (a->physics != COSMOS_BLACKHOLE)
         return adjoint_sweep_dev(a, order, op.axis, a->dx, op.dt, bias, limiter, w, o, src, dst);
     if (!a->gbuf) ADJCHK(a, hipMalloc((void**)&a->gbuf, sizeof(double) * 5 * a->plane));
@@ -477,10 +507,20 @@ int adjoint_op_dev(cosmos_adjoint* a, const Op& op, int order, double bias, int 
     A.gbuf = planes_at(a, a->gbuf);
     hipStream_t st = stream_of(a);
     if (op.forcing) {
+        if (!tape && a->tape_mode != COSMOS_TAPE_NONE) {
+            if (!a->tape) ADJCHK(a, hipMalloc((void**)&a->tape, tape_bytes(a)));
+            tape = a->tape;
+            A.tape = tape;
+            A.tape_plane = (long)a->plane;
+            ADJCHK(a, cfv::launch_gr_forcing_tape(A, st));
+        }
+        A.tape = tape;
+        A.tape_plane = (long)a->plane;
         ADJCHK(a, hipMemsetAsync(a->lam[dst], 0, sizeof(double) * 5 * a->plane, st));
         ADJCHK(a, cfv::launch_gr_adjoint_forcing(A, st));
     } else {
-        ADJCHK(a, cfv::launch_gr_sweep(A.L, op.axis, st));
+        if (tape) point_sweep_tape(a, tape, A.L);
+        else ADJCHK(a, cfv::launch_gr_sweep(A.L, op.axis, st));
         ADJCHK(a, cfv::launch_gr_adjoint_sweep(A, op.axis, st));
     }
     return COSMOS_OK;
@@ -555,6 +595,9 @@ int cosmos_adjoint_destroy(cosmos_adjoint* a) {
     hipFree(a->lam[1]);
     hipFree(a->stage);
     hipFree(a->gbuf);
+    hipFree(a->tape);
+    free_list(a->tapes);
+    free_list(a->stapes);
     free_list(a->checkpoints);
     free_list(a->seg);
     cosmos_destroy(a->s);
@@ -610,6 +653,20 @@ int cosmos_adjoint_set_subcycles(cosmos_adjoint* a, int subcycles) {
         if (subcycles < 0 || subcycles > 8) return adj_fail(a, COSMOS_ERR_ARGUMENT, "subcycles must be in [0, 8]");
         a->subcycles = subcycles;
         a->dts.clear();
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_adjoint_set_tape(cosmos_adjoint* a, int mode, long long reserve_bytes) {
+    return adj_guarded(a, [&] {
+        if (mode < COSMOS_TAPE_NONE || mode > COSMOS_TAPE_SEGMENT || reserve_bytes < 0)
+            return adj_fail(a, COSMOS_ERR_ARGUMENT, "tape mode must be 0, 1 or 2 and the reserve >= 0");
+        a->tape_mode = mode;
+        a->tape_reserve = reserve_bytes;
+        if (mode != COSMOS_TAPE_SEGMENT) {
+            free_list(a->tapes);
+            free_list(a->stapes);
+        }
         return COSMOS_OK;
     });
 }
@@ -684,6 +741,46 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
             if (rc) return rc;
             a->seg.push_back(p);
         }
+        // black hole: a tape per forcing step and per sweep of one segment,
+        // where they leave tape_reserve bytes of device memory free
+        const bool bh = a->physics == COSMOS_BLACKHOLE;
+        int forcings = 0;
+        for (int k = 0; k < ns; ++k) forcings += ops[k].forcing;
+        const int sweeps = ns - forcings;
+        const size_t nft = (size_t)a->every * forcings, nst = (size_t)a->every * sweeps;
+        bool seg_tapes = bh && a->tape_mode == COSMOS_TAPE_SEGMENT;
+        if (seg_tapes && (a->tapes.size() < nft || a->stapes.size() < nst)) {
+            cfv::GRLaunch L;  // the flux layout (allocates the solver's flux planes)
+            const Op sw{0, 0, 1.0};
+            int rc0 = gr_launch_of(a, sw, a->bias, a->limiter, L);
+            if (rc0) return rc0;
+            a->fplane = (long)(a->ny + 1) * L.fpitch;
+            size_t free_b = 0, total_b = 0;
+            ADJCHK(a, hipMemGetInfo(&free_b, &total_b));
+            const size_t more = (nft - std::min(nft, a->tapes.size())) * tape_bytes(a) +
+                                (nst - std::min(nst, a->stapes.size())) * sweep_tape_bytes(a);
+            if (free_b < more || free_b - more < (size_t)a->tape_reserve) {
+                seg_tapes = false;
+            } else {
+                for (auto* v : {&a->tapes, &a->stapes}) {
+                    const size_t want = v == &a->tapes ? nft : nst;
+                    const size_t bytes = v == &a->tapes ? tape_bytes(a) : sweep_tape_bytes(a);
+                    while (v->size() < want) {
+                        double* p = nullptr;
+                        ADJCHK(a, hipMalloc((void**)&p, bytes));
+                        v->push_back(p);
+                    }
+                }
+            }
+        }
+        // the tape of operation k of the step_in_seg-th step of the segment
+        auto tape_of = [&](int step_in_seg, int k) -> double* {
+            if (!seg_tapes) return nullptr;
+            int f = 0;
+            for (int q = 0; q < k; ++q) f += ops[q].forcing;
+            return ops[k].forcing ? a->tapes[(size_t)step_in_seg * forcings + f]
+                                  : a->stapes[(size_t)step_in_seg * sweeps + (k - f)];
+        };
         int rc = lam_upload(a, grad_out, 0);
         if (rc) return rc;
         int cur = 0;
@@ -695,7 +792,7 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
             for (int it = s0; it < s1; ++it) {
                 step_ops(a, a->dts[it], ops);
                 for (int k = 0; k < ns; ++k, ++q)
-                    if ((rc = forward_op_dev(a, ops[k], state(q), state(q + 1)))) return rc;
+                    if ((rc = forward_op_dev(a, ops[k], state(q), state(q + 1), tape_of(it - s0, k)))) return rc;
             }
             // reverse through the segment
             for (int it = s1 - 1; it >= s0; --it) {
@@ -703,7 +800,7 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
                 for (int k = ns - 1; k >= 0; --k) {
                     const int qi = (it - s0) * ns + k;
                     rc = adjoint_op_dev(a, ops[k], a->order, a->bias, a->limiter, state(qi), state(qi + 1), cur,
-                                        cur ^ 1);
+                                        cur ^ 1, tape_of(it - s0, k));
                     if (rc) return rc;
                     cur ^= 1;
                 }

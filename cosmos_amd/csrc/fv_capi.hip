@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <new>
 #include <string>
 #include <vector>
@@ -644,7 +645,13 @@ int cosmos_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, d
 
 int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
                  double bias, int slope_limiter) {
+    return cosmos_begin_at(s, order, cell_spacing, cfl, final_time, max_steps, bias, slope_limiter, 0.0, 0);
+}
+
+int cosmos_begin_at(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
+                    double bias, int slope_limiter, double start_time, int start_iteration) {
     return guarded(s, [&] {
+        if (start_iteration < 0) return fail(s, COSMOS_ERR_ARGUMENT, "start_iteration must be >= 0");
         int rc = check_order(s, order);
         if (rc) return rc;
         if (s->physics == COSMOS_BLACKHOLE)
@@ -654,12 +661,12 @@ int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, d
         rc = settle(s);
         if (rc) return rc;
         StepState h{};
-        h.t = 0.0;
+        h.t = start_time;
         h.tf = final_time;
         h.cfl = cfl;
         h.dx = cell_spacing;
-        h.iter = 0;
-        h.max_steps = max_steps < 0 ? INT_MAX : max_steps;
+        h.iter = start_iteration;
+        h.max_steps = max_steps < 0 || max_steps > INT_MAX - start_iteration ? INT_MAX : start_iteration + max_steps;
         h.done = 0;
         h.active_steps = 0;
         h.ws_slots = s->st_slots;
@@ -739,15 +746,23 @@ int cosmos_forcing(cosmos_solver* s, double cell_spacing, double time_step, doub
 int cosmos_solve_forced(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time,
                         int max_steps, double bias, int slope_limiter, int subcycles, double* time_out,
                         int* steps_out) {
+    return cosmos_solve_forced_at(s, order, cell_spacing, cfl, final_time, max_steps, bias, slope_limiter, subcycles,
+                                  0.0, 0, time_out, steps_out);
+}
+
+int cosmos_solve_forced_at(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time,
+                           int max_steps, double bias, int slope_limiter, int subcycles, double start_time,
+                           int start_iteration, double* time_out, int* steps_out) {
     return guarded(s, [&] {
         if (s->physics != COSMOS_BLACKHOLE)
             return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_solve_forced needs a COSMOS_BLACKHOLE solver");
         int rc = check_order(s, order);
         if (rc) return rc;
         if (subcycles < 0) return fail(s, COSMOS_ERR_ARGUMENT, "subcycles must be >= 0");
-        double t = 0.0;
-        int it = 0;
-        const int cap = max_steps < 0 ? INT_MAX : max_steps;
+        if (start_iteration < 0) return fail(s, COSMOS_ERR_ARGUMENT, "start_iteration must be >= 0");
+        double t = start_time;
+        int it = start_iteration;
+        const int cap = max_steps < 0 || max_steps > INT_MAX - start_iteration ? INT_MAX : start_iteration + max_steps;
         while (t < final_time && it < cap) {
             double dt = 0.0;
             rc = cosmos_stable_time_step(s, cell_spacing, cfl, t, final_time, it, &dt);
@@ -770,7 +785,7 @@ int cosmos_solve_forced(cosmos_solver* s, int order, double cell_spacing, double
         }
         HIPCHK(s, hipStreamSynchronize(s->stream));
         if (time_out) *time_out = t;
-        if (steps_out) *steps_out = it;
+        if (steps_out) *steps_out = it - start_iteration;
         return COSMOS_OK;
     });
 }
@@ -854,6 +869,156 @@ int cosmos_state_planes(cosmos_solver* s, double** planes5, long* pitch) {
         Planes p = planes_of(s, s->cur);
         for (int k = 0; k < 5; ++k) planes5[k] = p.p[k];
         if (pitch) *pitch = s->pitch;
+        return COSMOS_OK;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// Lossless checkpoint / restart and the reference's .dat text output
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+// Checkpoint file: a 128-byte little-endian header, the state as AoS
+// primitives [ny][nx][5] FP64 (the cosmos_download layout), and the 64-bit
+// FNV-1a hash of those bytes.
+constexpr char kCkMagic[8] = {'C', 'O', 'S', 'M', 'O', 'S', 'C', 'K'};
+constexpr uint32_t kCkVersion = 1;
+
+struct CkHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t physics, dim, nx, ny, iteration;
+    double gamma, time;
+    double hole[5];   // M, a, x, y, z (black hole)
+    double excision;  // explicit excision radius (0: the horizon)
+    uint8_t pad[128 - 8 - 4 - 5 * 4 - 4 - 2 * 8 - 5 * 8 - 8];
+};
+static_assert(sizeof(CkHeader) == 128, "checkpoint header is 128 bytes");
+
+uint64_t fnv1a(const unsigned char* p, size_t n) {
+    uint64_t h = 1469598103934665603ULL;
+    for (size_t k = 0; k < n; ++k) {
+        h ^= p[k];
+        h *= 1099511628211ULL;
+    }
+    return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cosmos_checkpoint_save(cosmos_solver* s, const char* path, double time, int iteration) {
+    return guarded(s, [&] {
+        if (!path) return fail(s, COSMOS_ERR_ARGUMENT, "path is null");
+        const size_t n = (size_t)s->nx * s->ny * 5;
+        std::vector<double> state(n);
+        int rc = download_impl(s, state.data(), hipMemcpyDeviceToHost);
+        if (rc) return rc;
+        CkHeader h{};
+        std::memcpy(h.magic, kCkMagic, 8);
+        h.version = kCkVersion;
+        h.physics = s->physics;
+        h.dim = s->dim;
+        h.nx = s->nx;
+        h.ny = s->ny;
+        h.iteration = iteration;
+        h.gamma = s->P.gamma;
+        h.time = time;
+        const double hole[5] = {s->hole.M, s->hole.a, s->hole.x, s->hole.y, s->hole.z};
+        std::memcpy(h.hole, hole, sizeof hole);
+        h.excision = s->excision;
+        const uint64_t sum = fnv1a(reinterpret_cast<const unsigned char*>(state.data()), n * sizeof(double));
+        const std::string tmp = std::string(path) + ".tmp";
+        FILE* f = std::fopen(tmp.c_str(), "wb");
+        if (!f) return fail(s, COSMOS_ERR_ARGUMENT, std::string("cannot open ") + tmp);
+        bool ok = std::fwrite(&h, sizeof h, 1, f) == 1 && std::fwrite(state.data(), sizeof(double), n, f) == n &&
+                  std::fwrite(&sum, sizeof sum, 1, f) == 1;
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok || std::rename(tmp.c_str(), path) != 0) {
+            std::remove(tmp.c_str());
+            return fail(s, COSMOS_ERR_STATE, std::string("writing the checkpoint ") + path + " failed");
+        }
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_checkpoint_load(cosmos_solver* s, const char* path, double* time, int* iteration) {
+    return guarded(s, [&] {
+        if (!path) return fail(s, COSMOS_ERR_ARGUMENT, "path is null");
+        FILE* f = std::fopen(path, "rb");
+        if (!f) return fail(s, COSMOS_ERR_ARGUMENT, std::string("cannot open ") + path);
+        CkHeader h{};
+        const size_t n = (size_t)s->nx * s->ny * 5;
+        std::vector<double> state(n);
+        uint64_t sum = 0;
+        const bool read_ok = std::fread(&h, sizeof h, 1, f) == 1;
+        std::string why;
+        if (!read_ok || std::memcmp(h.magic, kCkMagic, 8) != 0) why = "not a checkpoint file";
+        else if (h.version != kCkVersion) why = "unsupported checkpoint version " + std::to_string(h.version);
+        else if (h.physics != s->physics || h.dim != s->dim || h.nx != s->nx || h.ny != s->ny)
+            why = "checkpoint grid or physics differs from the solver's";
+        else if (h.gamma != s->P.gamma) why = "checkpoint adiabatic index differs from the solver's";
+        else if (std::fread(state.data(), sizeof(double), n, f) != n || std::fread(&sum, sizeof sum, 1, f) != 1)
+            why = "checkpoint file is truncated";
+        else if (sum != fnv1a(reinterpret_cast<const unsigned char*>(state.data()), n * sizeof(double)))
+            why = "checkpoint checksum mismatch";
+        std::fclose(f);
+        if (!why.empty()) return fail(s, COSMOS_ERR_STATE, why + " (" + path + ")");
+        int rc = settle(s);
+        if (rc) return rc;
+        if (s->physics == COSMOS_BLACKHOLE) {
+            s->excision = h.excision;
+            s->hole = make_hole(h.hole[0], h.hole[1], h.hole[2], h.hole[3], h.hole[4], h.excision);
+        }
+        rc = upload_impl(s, state.data(), hipMemcpyHostToDevice);
+        if (rc) return rc;
+        if (time) *time = h.time;
+        if (iteration) *iteration = h.iteration;
+        return COSMOS_OK;
+    });
+}
+
+// EulerTests::outputSolution{,2D} (Tests/EulerTests.cpp:189-252) and
+// BlackHoleEulerTests::outputSolution{,2D} (Tests/BlackHoleEulerTests.cpp:201-286):
+// one file per primitive, "x value" (1D) or "y x value" rows with a blank line
+// after each grid row (2D), coordinates on the unit square (1 / cell count), the
+// stream's default formatting; black-hole cells inside the excision region
+// (tested at (j / nx, i / ny)) written as 0.
+int cosmos_write_dat(cosmos_solver* s, const char* directory) {
+    return guarded(s, [&] {
+        if (!directory) return fail(s, COSMOS_ERR_ARGUMENT, "directory is null");
+        const size_t n = (size_t)s->nx * s->ny * 5;
+        std::vector<double> W(n);
+        int rc = download_impl(s, W.data(), hipMemcpyDeviceToHost);
+        if (rc) return rc;
+        static const char* names[5] = {"density", "xVelocity", "yVelocity", "zVelocity", "pressure"};
+        const bool bh = s->physics == COSMOS_BLACKHOLE;
+        for (int k = 0; k < 5; ++k) {
+            const std::string fn = std::string(directory) + "/" + names[k] + ".dat";
+            std::ofstream out(fn);
+            if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "cannot open " + fn);
+            if (s->dim == 1) {
+                const double h = 1.0 / s->nx;
+                for (int i = 0; i < s->nx; ++i) {
+                    const bool cut = bh && cfv::excised(s->hole, i * h, 0.0, 0.0);
+                    out << (h * i) << " " << (cut ? 0.0 : W[5 * (size_t)i + k]) << std::endl;
+                }
+            } else {
+                const double hr = 1.0 / s->ny, hc = 1.0 / s->nx;
+                for (int i = 0; i < s->ny; ++i) {
+                    for (int j = 0; j < s->nx; ++j) {
+                        const bool cut = bh && cfv::excised(s->hole, j * hc, i * hr, 0.0);
+                        out << (hr * i) << " " << (hc * j) << " "
+                            << (cut ? 0.0 : W[5 * ((size_t)i * s->nx + j) + k]) << std::endl;
+                    }
+                    out << std::endl;
+                }
+            }
+            if (!out) return fail(s, COSMOS_ERR_STATE, "writing " + fn + " failed");
+        }
         return COSMOS_OK;
     });
 }

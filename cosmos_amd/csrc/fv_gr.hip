@@ -215,6 +215,14 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
         if (AXIS == 0) gr_half_step<AXIS>(W1, W2, W0, x + hh, y, 0, L, pl);
         else gr_half_step<AXIS>(W1, W2, W0, x, y + hh, 0, L, pl);
         gr_force<AXIS>(pr, pl, x, y, L, F);
+        if (L.half) {
+            const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                L.half[k * L.half_plane + f] = pr[k];
+                L.half[(5 + k) * L.half_plane + f] = pl[k];
+            }
+        }
     }
     long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
 #pragma unroll

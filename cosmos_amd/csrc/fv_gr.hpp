@@ -242,10 +242,14 @@ CFV_HD void gr_c2p(const double* U, double sdet, const PhysParams& P, double* W)
 // compute{X,Y}FluxVector(cons) (:159-257 x, tolerance 1e-10; :259-357 y, 1e-8):
 // alpha sqrt(gamma) (rho W vt, rho h W^2 v_i vt + p delta_i, (rho h W^2 - p - rho W) vt + p v)
 // with vt = v^axis - beta^axis / alpha, from the floored density and pressure.
+// gr_flux_of forms the flux from a con2prim record of U (so that the reverse
+// pass can reuse the record of its forward evaluation).
+CFV_HD double gr_flux_tol(const PhysParams& P, int axis) { return gr_tol(P, axis == 0 ? 1e-10 : 1e-8); }
+
 template <int AXIS>
-CFV_HD void gr_flux(const double* U, const Geo& G, const PhysParams& P, double* F) {
-    double R[5], lf, h;
-    gr_con2prim(U, G.sdet, gr_tol(P, AXIS == 0 ? 1e-10 : 1e-8), P, R, lf, h);
+CFV_HD void gr_flux_of(const C2PRec& c, const Geo& G, double* F) {
+    const double* R = c.R;
+    const double lf = c.lf, h = c.h;
     double rho = R[0], p = R[4];
     double as = G.alpha * G.sdet;
     double vn = R[1 + AXIS];
@@ -259,6 +263,13 @@ CFV_HD void gr_flux(const double* U, const Geo& G, const PhysParams& P, double* 
         F[1 + k] = as * t;
     }
     F[4] = as * ((((rhw - p) - (rho * lf)) * vt) + (p * vn));
+}
+
+template <int AXIS>
+CFV_HD void gr_flux(const double* U, const Geo& G, const PhysParams& P, double* F) {
+    C2PRec c;
+    gr_con2prim_rec(U, G.sdet, gr_flux_tol(P, AXIS), P, c);
+    gr_flux_of<AXIS>(c, G, F);
 }
 
 // ---------------------------------------------------------------------------

@@ -5,7 +5,9 @@
 // (Solvers/BlackHoleEulerForcingSolver.cpp:3-250).
 //
 // Sweep: the forward fluxes are in the flux planes (the caller re-runs the
-// forward sweep first).  k_gra_cells gives each cell's adjoint through its
+// forward sweep first, or points them at the ones a forward sweep kept, with
+// its SLIC face states: then the interface adjoint runs as two launches, see
+// k_gra_iface_force).  k_gra_cells gives each cell's adjoint through its
 // update (c2p, then p2c back to the input primitives; excised stencils pass
 // their adjoint through unchanged) and stores the adjoint of the updated
 // conserved vector; k_gra_ifaces forms each interface flux's adjoint from its
@@ -23,6 +25,16 @@ namespace cfv {
 namespace {
 
 constexpr int kGAT = 128;
+// Minimum waves per SIMD the register allocation must allow (more waves hide
+// more FP64 latency; fewer registers per wave spill more) for the split
+// interface kernels and the taped forcing reverse.
+#ifndef COSMOS_GRA_IFACE_WAVES
+#define COSMOS_GRA_IFACE_WAVES 2
+#endif
+#ifndef COSMOS_GRA_REV_WAVES
+#define COSMOS_GRA_REV_WAVES 2
+#endif
+constexpr int kIfaceWaves = COSMOS_GRA_IFACE_WAVES, kRevWaves = COSMOS_GRA_REV_WAVES;
 
 __device__ __forceinline__ int mirror2_a(int c, int n) {
     if (c < 0) c = -c - 1;
@@ -181,6 +193,133 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     add5(A.lam_in, at(c + 1), g3);
 }
 
+// The SLIC interface adjoint in two launches when the forward sweep kept its
+// face states: k_gra_iface_force takes each interface flux's adjoint back
+// through FORCE to the adjoints of its two face states, written over those
+// states in the tape (zeros where the flux has no adjoint);
+// k_gra_iface_half takes each face state's adjoint back through its half step
+// (one thread per interface and side) onto the three input cells.
+template <int AXIS>
+__global__ __launch_bounds__(kGAT) __attribute__((amdgpu_waves_per_eu(kIfaceWaves))) void k_gra_iface_force(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    const int n = AXIS == 0 ? L.nx : L.ny;
+    const long lines = AXIS == 0 ? L.ny : L.nx;
+    const long per = AXIS == 0 ? (long)(n + 1) : lines;
+    const long total = (long)(n + 1) * lines;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= total) return;
+    int c, i, j;
+    if (AXIS == 0) {
+        i = (int)(idx / per);
+        c = (int)(idx - (long)i * per);
+        j = c;
+    } else {
+        c = (int)(idx / per);
+        j = (int)(idx - (long)c * per);
+        i = c;
+    }
+    auto cell_at = [&](int cc) -> long { return AXIS == 0 ? (long)i * L.pitch + cc : (long)cc * L.pitch + j; };
+    double gF[5];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double gr = c < n ? A.gbuf.p[k][cell_at(c)] : 0.0;
+        const double gl = c > 0 ? A.gbuf.p[k][cell_at(c - 1)] : 0.0;
+        gF[k] = L.coef.dt_dx * (gr - gl);
+        any |= gF[k] != 0.0;
+    }
+    const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
+    double* const hp = L.half + f;
+    const long hpl = L.half_plane;
+    double gpr[5], gpl[5];
+    zero5(gpr);
+    zero5(gpl);
+    if (any) {
+        double pr[5], pl[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            pr[k] = hp[k * hpl];
+            pl[k] = hp[(5 + k) * hpl];
+        }
+        double x, y;
+        if (AXIS == 0) {
+            x = (c - 0.5) * L.h;
+            y = L.dim == 2 ? i * L.h : 0.0;
+        } else {
+            x = j * L.h;
+            y = (c - 0.5) * L.h;
+        }
+        adjgr::vjp_force_g<AXIS>(pr, pl, adjgr::force_geo<AXIS>(L.hole, x, y, L.h), L.coef, L.P, gF, gpr, gpl);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        hp[k * hpl] = gpr[k];
+        hp[(5 + k) * hpl] = gpl[k];
+    }
+}
+
+template <int AXIS>
+__global__ __launch_bounds__(kGAT) __attribute__((amdgpu_waves_per_eu(kIfaceWaves))) void k_gra_iface_half(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    const int n = AXIS == 0 ? L.nx : L.ny;
+    const long lines = AXIS == 0 ? L.ny : L.nx;
+    const long per = AXIS == 0 ? (long)(n + 1) : lines;
+    const long total = (long)(n + 1) * lines;
+    long t = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (t >= 2 * total) return;
+    // neighbouring threads take the two sides of one interface
+    const int side = (int)(t & 1);
+    const long idx = t >> 1;
+    int c, i, j;
+    if (AXIS == 0) {
+        i = (int)(idx / per);
+        c = (int)(idx - (long)i * per);
+        j = c;
+    } else {
+        c = (int)(idx / per);
+        j = (int)(idx - (long)c * per);
+        i = c;
+    }
+    const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
+    const double* const hp = L.half + f + (long)(side == 0 ? 0 : 5) * L.half_plane;
+    double g[5];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        g[k] = hp[k * L.half_plane];
+        any |= g[k] != 0.0;
+    }
+    if (!any) return;
+    auto at = [&](int cc) -> long {
+        const int m = mirror2_a(cc, n);
+        return AXIS == 0 ? (long)i * L.pitch + m : (long)m * L.pitch + j;
+    };
+    // side 0: the left cell c - 1's right face, evolved about the interface - h/2;
+    // side 1: the right cell c's left face, about the interface + h/2
+    const int c0 = side == 0 ? c - 2 : c - 1;
+    const double hh = L.h * 0.5;
+    double x, y;
+    if (AXIS == 0) {
+        x = (c - 0.5) * L.h + (side == 0 ? -hh : hh);
+        y = L.dim == 2 ? i * L.h : 0.0;
+    } else {
+        x = j * L.h;
+        y = (c - 0.5) * L.h + (side == 0 ? -hh : hh);
+    }
+    double W0[5], W1[5], W2[5], g0[5], g1[5], g2[5];
+    ld5(L.in, at(c0), W0);
+    ld5(L.in, at(c0 + 1), W1);
+    ld5(L.in, at(c0 + 2), W2);
+    zero5(g0);
+    zero5(g1);
+    zero5(g2);
+    adjgr::vjp_half_g<AXIS>(W0, W1, W2, side == 0 ? 1 : 0, adjgr::half_geo<AXIS>(L.hole, x, y, L.h), L.coef, L.bias,
+                            L.limiter, L.P, g, g0, g1, g2);
+    add5(A.lam_in, at(c0), g0);
+    add5(A.lam_in, at(c0 + 1), g1);
+    add5(A.lam_in, at(c0 + 2), g2);
+}
+
 // Per cell: the adjoint of the RK4 forcing (gr forcing kernel), into a zeroed lam_in.
 template <int DIM>
 __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
@@ -316,6 +455,244 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
     }
 }
 
+// Taped forcing adjoint.  k_gra_forcing_tape runs the forward RK4 chain of
+// every cell (bitwise the forward forcing kernel; it writes the step's output
+// too) and keeps the values its reverse needs in the tape (one SoA plane per
+// value; layout in fv_gr_adjoint_launch.hpp); k_gra_forcing_rev walks the chain
+// back from the tape, so the reverse pass evaluates no forward stage again.
+template <int DIM>
+__global__ __launch_bounds__(kGAT) void k_gra_forcing_tape(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= (long)L.nx * L.ny) return;
+    int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
+    long cell = (long)i * L.pitch + j;
+    double* const tp = A.tape + cell;
+    const long pl = A.tape_plane;
+    auto put = [&](int slot, const double* v) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) tp[(long)(slot + k) * pl] = v[k];
+    };
+    double W[5];
+    ld5(L.in, cell, W);
+    const Hole& b = L.hole;
+    const double h = L.h;
+    if (!clear_of(b, DIM, i, j, 1, h)) {
+        st5(L.out, cell, W);
+        return;
+    }
+    const PhysParams& P = L.P;
+    const double x = j * h, y = DIM == 2 ? i * h : 0.0;
+    const Geo gc = geo_at(b, x, y, 0.0);
+    const Geo gxm = geo_at(b, x - h, y, 0.0), gxp = geo_at(b, x + h, y, 0.0);
+    Geo gym, gyp;
+    if (DIM == 2) {
+        gym = geo_at(b, x, y - h, 0.0);
+        gyp = geo_at(b, x, y + h, 0.0);
+    }
+    // neighbours: p2c at their own centres, c2p at x -/+ h (y -/+ h)
+    double Wn[4][5];
+    {
+        const long nb[4] = {(long)i * L.pitch + clamp1_a(j - 1, L.nx), (long)i * L.pitch + clamp1_a(j + 1, L.nx),
+                            DIM == 2 ? (long)clamp1_a(i - 1, L.ny) * L.pitch + j : cell,
+                            DIM == 2 ? (long)clamp1_a(i + 1, L.ny) * L.pitch + j : cell};
+        const double nbx[4] = {(j - 1) * h, (j + 1) * h, x, x};
+        const double nby[4] = {y, y, DIM == 2 ? (i - 1) * h : y, DIM == 2 ? (i + 1) * h : y};
+        const int nn = DIM == 2 ? 4 : 2;
+        for (int q = 0; q < nn; ++q) {
+            double Wq[5], uq[5];
+            ld5(L.in, nb[q], Wq);
+            gr_p2c(Wq, geo_at(b, nbx[q], nby[q], 0.0), uq, P);
+            gr_c2p(uq, (q == 0 ? gxm : q == 1 ? gxp : q == 2 ? gym : gyp).sdet, P, Wn[q]);
+            put(kTapeWn + 5 * q, Wn[q]);
+        }
+    }
+    double uM[5];
+    gr_p2c(W, gc, uM, P);
+    SourceGeo sg;
+    source_geo(b, x, y, 0.0, h, sg);
+    double kk[4][5], src[5];
+    gr_source(uM, sg, P, src);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
+    for (int st = 0; st < 3; ++st) {
+        const int base = kTapeStage + kTapePerStage * st;
+        double tmp[5], Wm[5], e1[5], um[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
+        gr_c2p(tmp, gc.sdet, P, Wm);
+        put(base + kTapeTmp, tmp);
+        put(base + kTapeWm, Wm);
+        if (DIM == 1) {
+            adjgr::frac_fwd_g<0>(L.frac_dt_h[st], Wn[0], Wm, Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, e1);
+        } else {
+            double e2[5], e3[5];
+            adjgr::frac_fwd_g<0>(L.frac_dt_h[st], Wn[0], Wm, Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, e2);
+            adjgr::frac_fwd_g<1>(L.frac_dt_h_y[st], Wn[2], e2, Wn[3], gym, gc, gyp, L.bias, L.limiter, P, e3);
+            adjgr::frac_fwd_g<0>(L.frac_dt_h[st], Wn[0], e3, Wn[1], gxm, gc, gxp, L.bias, L.limiter, P, e1);
+            put(base + kTapeE2, e2);
+            put(base + kTapeE3, e3);
+        }
+        put(base + kTapeE1, e1);
+        gr_p2c(e1, gc, um, P);
+        gr_source(um, sg, P, src);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
+    }
+    double un[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
+    put(kTapeUn, un);
+    gr_c2p(un, gc.sdet, P, W);
+    st5(L.out, cell, W);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kGAT) __attribute__((amdgpu_waves_per_eu(kRevWaves))) void k_gra_forcing_rev(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= (long)L.nx * L.ny) return;
+    int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
+    long cell = (long)i * L.pitch + j;
+    const double* const tp = A.tape + cell;
+    const long pl = A.tape_plane;
+    auto get = [&](int slot, double* v) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) v[k] = tp[(long)(slot + k) * pl];
+    };
+    double lam[5];
+    ld5(A.lam, cell, lam);
+    const Hole& b = L.hole;
+    const double h = L.h;
+    if (!clear_of(b, DIM, i, j, 1, h)) {
+        add5(A.lam_in, cell, lam);
+        return;
+    }
+    const PhysParams& P = L.P;
+    const double tol = gr_tol(P, 1e-8);
+    const double x = j * h, y = DIM == 2 ? i * h : 0.0;
+    const Geo gc = geo_at(b, x, y, 0.0);
+    double gun[5], gk[5], guM[5], gWn[4][5];
+    {
+        double un[5];
+        get(kTapeUn, un);
+        zero5(gun);
+        adjgr::vjp_c2p(un, gc.sdet, tol, P, lam, gun);
+    }
+    // gk = adjoint of the stage increment k_{st+1} (starting with k4)
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        guM[k] = gun[k];
+        gk[k] = (1.0 / 6.0) * gun[k];
+    }
+    for (int q = 0; q < 4; ++q) zero5(gWn[q]);
+    for (int st = 2; st >= 0; --st) {
+        const int base = kTapeStage + kTapePerStage * st;
+        double gsrc[5], gum[5], ge1[5], gWm[5], gtmp[5];
+        {
+            SourceGeo sg;
+            source_geo(b, x, y, 0.0, h, sg);
+            double e1[5], um[5];
+            get(base + kTapeE1, e1);
+            gr_p2c(e1, gc, um, P);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gk[k];
+            zero5(gum);
+            adjgr::vjp_source(um, sg, P, gsrc, gum);
+            zero5(ge1);
+            adjgr::vjp_p2c(e1, gc, P, gum, ge1);
+        }
+        zero5(gWm);
+        const Geo gxm = geo_at(b, x - h, y, 0.0), gxp = geo_at(b, x + h, y, 0.0);
+        if (DIM == 1) {
+            double Wm[5], Wa[5], Wb[5];
+            get(base + kTapeWm, Wm);
+            get(kTapeWn, Wa);
+            get(kTapeWn + 5, Wb);
+            adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wa, Wm, Wb, gxm, gc, gxp, L.bias, L.limiter, P, ge1, gWn[0], gWm,
+                                 gWn[1]);
+        } else {
+            double ge3[5], ge2[5];
+            zero5(ge3);
+            zero5(ge2);
+            {
+                double e3[5], Wa[5], Wb[5];
+                get(base + kTapeE3, e3);
+                get(kTapeWn, Wa);
+                get(kTapeWn + 5, Wb);
+                adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wa, e3, Wb, gxm, gc, gxp, L.bias, L.limiter, P, ge1, gWn[0],
+                                     ge3, gWn[1]);
+            }
+            {
+                const Geo gym = geo_at(b, x, y - h, 0.0), gyp = geo_at(b, x, y + h, 0.0);
+                double e2[5], Wa[5], Wb[5];
+                get(base + kTapeE2, e2);
+                get(kTapeWn + 10, Wa);
+                get(kTapeWn + 15, Wb);
+                adjgr::vjp_frac_g<1>(L.frac_dt_h_y[st], Wa, e2, Wb, gym, gc, gyp, L.bias, L.limiter, P, ge3, gWn[2],
+                                     ge2, gWn[3]);
+            }
+            {
+                double Wm[5], Wa[5], Wb[5];
+                get(base + kTapeWm, Wm);
+                get(kTapeWn, Wa);
+                get(kTapeWn + 5, Wb);
+                adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wa, Wm, Wb, gxm, gc, gxp, L.bias, L.limiter, P, ge2, gWn[0],
+                                     gWm, gWn[1]);
+            }
+        }
+        {
+            double tmp[5];
+            get(base + kTapeTmp, tmp);
+            zero5(gtmp);
+            adjgr::vjp_c2p(tmp, gc.sdet, tol, P, gWm, gtmp);
+        }
+        // k_{st} feeds the stage's own state (by 1/2 or 1) and the final sum
+        const double f = st < 2 ? 0.5 : 1.0;
+        const double c = st == 0 ? (1.0 / 6.0) : (1.0 / 6.0) * 2.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            guM[k] += gtmp[k];
+            gk[k] = c * gun[k] + f * gtmp[k];
+        }
+    }
+    {
+        SourceGeo sg;
+        source_geo(b, x, y, 0.0, h, sg);
+        double W[5], uM[5], gsrc[5];
+        ld5(L.in, cell, W);
+        gr_p2c(W, gc, uM, P);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gk[k];
+        adjgr::vjp_source(uM, sg, P, gsrc, guM);
+        double gW[5];
+        zero5(gW);
+        adjgr::vjp_p2c(W, gc, P, guM, gW);
+        add5(A.lam_in, cell, gW);
+    }
+    const int nn = DIM == 2 ? 4 : 2;
+    for (int q = 0; q < nn; ++q) {
+        const long nbq = q == 0 ? (long)i * L.pitch + clamp1_a(j - 1, L.nx)
+                         : q == 1 ? (long)i * L.pitch + clamp1_a(j + 1, L.nx)
+                         : q == 2 ? (long)clamp1_a(i - 1, L.ny) * L.pitch + j
+                                  : (long)clamp1_a(i + 1, L.ny) * L.pitch + j;
+        const double px = q == 0 ? (j - 1) * h : q == 1 ? (j + 1) * h : x;
+        const double py = q == 2 ? (i - 1) * h : q == 3 ? (i + 1) * h : y;
+        const double sx = q == 0 ? x - h : q == 1 ? x + h : x;
+        const double sy = q == 2 ? y - h : q == 3 ? y + h : y;
+        double Wq[5], uq[5], gu[5], gw[5];
+        ld5(L.in, nbq, Wq);
+        const Geo gq = geo_at(b, px, py, 0.0);
+        gr_p2c(Wq, gq, uq, P);
+        zero5(gu);
+        zero5(gw);
+        adjgr::vjp_c2p(uq, geo_at(b, sx, sy, 0.0).sdet, tol, P, gWn[q], gu);
+        adjgr::vjp_p2c(Wq, gq, P, gu, gw);
+        add5(A.lam_in, nbq, gw);
+    }
+}
+
 dim3 gat_blocks(long work) { return dim3((unsigned)((work + kGAT - 1) / kGAT)); }
 
 }  // namespace
@@ -325,7 +702,17 @@ hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s
     const long cells = (long)L.nx * L.ny;
     const long ifaces = axis == 0 ? (long)(L.nx + 1) * L.ny : (long)(L.ny + 1) * L.nx;
     const dim3 blk(kGAT);
-    if (L.order == 2) {
+    if (L.order == 2 && L.half) {
+        if (axis == 0) {
+            hipLaunchKernelGGL((k_gra_cells<2, 0>), gat_blocks(cells), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_iface_force<0>), gat_blocks(ifaces), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_iface_half<0>), gat_blocks(2 * ifaces), blk, 0, s, A);
+        } else {
+            hipLaunchKernelGGL((k_gra_cells<2, 1>), gat_blocks(cells), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_iface_force<1>), gat_blocks(ifaces), blk, 0, s, A);
+            hipLaunchKernelGGL((k_gra_iface_half<1>), gat_blocks(2 * ifaces), blk, 0, s, A);
+        }
+    } else if (L.order == 2) {
         if (axis == 0) {
             hipLaunchKernelGGL((k_gra_cells<2, 0>), gat_blocks(cells), blk, 0, s, A);
             hipLaunchKernelGGL((k_gra_ifaces<2, 0>), gat_blocks(ifaces), blk, 0, s, A);
@@ -347,8 +734,20 @@ hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s
 
 hipError_t launch_gr_adjoint_forcing(const GRAdjLaunch& A, hipStream_t s) {
     const long cells = (long)A.L.nx * A.L.ny;
-    if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
-    else hipLaunchKernelGGL((k_gra_forcing<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+    if (A.tape) {
+        if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing_rev<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+        else hipLaunchKernelGGL((k_gra_forcing_rev<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+    } else {
+        if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+        else hipLaunchKernelGGL((k_gra_forcing<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gr_forcing_tape(const GRAdjLaunch& A, hipStream_t s) {
+    const long cells = (long)A.L.nx * A.L.ny;
+    if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing_tape<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+    else hipLaunchKernelGGL((k_gra_forcing_tape<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);
     return hipGetLastError();
 }
 

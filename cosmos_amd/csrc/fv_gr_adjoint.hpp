@@ -130,13 +130,9 @@ CFV_HD void vjp_c2p(const double* U, double sdet, double tol, const PhysParams& 
     vjp_con2prim(c, sdet, P, gW, 0.0, 0.0, gU);
 }
 
-CFV_HD double flux_tol(const PhysParams& P, int axis) { return gr_tol(P, axis == 0 ? 1e-10 : 1e-8); }
-
-// gU += (dF/dU)^T gF for F = gr_flux<AXIS>(U, G)
+// gU += (dF/dU)^T gF for F = gr_flux<AXIS>(U, G), from the con2prim record c of U
 template <int AXIS>
-CFV_HD void vjp_flux(const double* U, const Geo& G, const PhysParams& P, const double* gF, double* gU) {
-    C2PRec c;
-    gr_con2prim_rec(U, G.sdet, flux_tol(P, AXIS), P, c);
+CFV_HD void vjp_flux_rec(const C2PRec& c, const Geo& G, const PhysParams& P, const double* gF, double* gU) {
     const double rho = c.R[0], p = c.R[4], lf = c.lf, h = c.h;
     const double as = G.alpha * G.sdet;
     const double vn = c.R[1 + AXIS];
@@ -159,6 +155,20 @@ CFV_HD void vjp_flux(const double* U, const Geo& G, const PhysParams& P, const d
     const double glf = as * (gF[0] - gF[4]) * rho * vt + grhw * 2.0 * rho * h * lf;
     const double gh = grhw * rho * lf * lf;
     vjp_con2prim(c, G.sdet, P, gR, glf, gh, gU);
+}
+
+template <int AXIS>
+CFV_HD void vjp_flux(const double* U, const Geo& G, const PhysParams& P, const double* gF, double* gU) {
+    C2PRec c;
+    gr_con2prim_rec(U, G.sdet, gr_flux_tol(P, AXIS), P, c);
+    vjp_flux_rec<AXIS>(c, G, P, gF, gU);
+}
+
+// The flux of U with its con2prim record kept for the reverse pass.
+template <int AXIS>
+CFV_HD void flux_keep(const double* U, const Geo& G, const PhysParams& P, C2PRec& c, double* F) {
+    gr_con2prim_rec(U, G.sdet, gr_flux_tol(P, AXIS), P, c);
+    gr_flux_of<AXIS>(c, G, F);
 }
 
 // gU += (dS/dU)^T gS for S = gr_source(U, SG)
@@ -311,10 +321,11 @@ template <int AXIS>
 CFV_HD void vjp_force_g(const double* WL, const double* WR, const ForceGeo& G, const SweepCoef& cf,
                         const PhysParams& P, const double* gF, double* gWL, double* gWR) {
     double ul[5], ur[5], fl[5], fr[5], us[5];
+    C2PRec cl, cr;
     gr_p2c(WL, G.l, ul, P);
-    gr_flux<AXIS>(ul, G.l, P, fl);
+    flux_keep<AXIS>(ul, G.l, P, cl, fl);
     gr_p2c(WR, G.r, ur, P);
-    gr_flux<AXIS>(ur, G.r, P, fr);
+    flux_keep<AXIS>(ur, G.r, P, cr, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (cf.dt_dx * (fl[k] - fr[k])));
     double gfs[5], gfl[5], gfr[5], gul[5], gur[5], gus[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -335,8 +346,8 @@ CFV_HD void vjp_force_g(const double* WL, const double* WR, const ForceGeo& G, c
         gfl[k] += 0.5 * cf.dt_dx * gus[k];
         gfr[k] -= 0.5 * cf.dt_dx * gus[k];
     }
-    vjp_flux<AXIS>(ul, G.l, P, gfl, gul);
-    vjp_flux<AXIS>(ur, G.r, P, gfr, gur);
+    vjp_flux_rec<AXIS>(cl, G.l, P, gfl, gul);
+    vjp_flux_rec<AXIS>(cr, G.r, P, gfr, gur);
     vjp_p2c(WL, G.l, P, gul, gWL);
     vjp_p2c(WR, G.r, P, gur, gWR);
 }
@@ -390,8 +401,9 @@ CFV_HD void vjp_half_g(const double* WL, const double* WM, const double* WR, int
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    gr_flux<AXIS>(xl, G.fl, P, fl);
-    gr_flux<AXIS>(xr, G.fr, P, fr);
+    C2PRec cl, cr;
+    flux_keep<AXIS>(xl, G.fl, P, cl, fl);
+    flux_keep<AXIS>(xr, G.fr, P, cr, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (cf.hdt_dx * (fl[k] - fr[k]));
     double ge[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -404,8 +416,8 @@ CFV_HD void vjp_half_g(const double* WL, const double* WM, const double* WR, int
         gfl[k] = cf.hdt_dx * ge[k];
         gfr[k] = -cf.hdt_dx * ge[k];
     }
-    vjp_flux<AXIS>(xl, G.fl, P, gfl, gxl);
-    vjp_flux<AXIS>(xr, G.fr, P, gfr, gxr);
+    vjp_flux_rec<AXIS>(cl, G.fl, P, gfl, gxl);
+    vjp_flux_rec<AXIS>(cr, G.fr, P, gfr, gxr);
     double gul[5], gum[5], gur[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -472,8 +484,9 @@ CFV_HD void vjp_frac_g(double fdt_h, const double* WL, const double* WM, const d
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    gr_flux<AXIS>(xl, gl, P, fl);
-    gr_flux<AXIS>(xr, gr, P, fr);
+    C2PRec cl, cr;
+    flux_keep<AXIS>(xl, gl, P, cl, fl);
+    flux_keep<AXIS>(xr, gr, P, cr, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = um[k] + (fdt_h * (fl[k] - fr[k]));
     double ge[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -484,8 +497,8 @@ CFV_HD void vjp_frac_g(double fdt_h, const double* WL, const double* WM, const d
         gfl[k] = fdt_h * ge[k];
         gfr[k] = -fdt_h * ge[k];
     }
-    vjp_flux<AXIS>(xl, gl, P, gfl, gxl);
-    vjp_flux<AXIS>(xr, gr, P, gfr, gxr);
+    vjp_flux_rec<AXIS>(cl, gl, P, gfl, gxl);
+    vjp_flux_rec<AXIS>(cr, gr, P, gfr, gxr);
     double gul[5], gum[5], gur[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {

@@ -13,11 +13,25 @@ struct GRAdjLaunch {
     ConstPlanes lam;  // adjoint of the operation's output
     Planes lam_in;    // adjoint of its input (forcing: accumulated, so zero it first)
     Planes gbuf;      // sweeps: per-cell adjoint of the updated conserved vector (state layout)
+    double* tape;     // forcing: kTapePlanes planes of tape_plane doubles (state layout), or null
+    long tape_plane;
 };
+
+// Forcing tape, one plane per value: the final conserved vector un, the four
+// neighbour states Wn (x-, x+, y-, y+; 1D uses the first two), then per RK
+// stage the stage's conserved state tmp, its primitives Wm and the fractional
+// sweep outputs e2 (x), e3 (y), e1 (last x; the only one in 1D).
+constexpr int kTapeUn = 0, kTapeWn = 5, kTapeStage = 25, kTapePerStage = 25;
+constexpr int kTapeTmp = 0, kTapeWm = 5, kTapeE2 = 10, kTapeE3 = 15, kTapeE1 = 20;
+constexpr int kTapePlanes = kTapeStage + 3 * kTapePerStage;
 
 // Sweep: the flux planes must hold the forward sweep's interface fluxes.
 hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s);
+// Forcing: with A.tape set, the tape must hold the operation's chain
+// (launch_gr_forcing_tape on the same input); otherwise the kernel re-runs it.
 hipError_t launch_gr_adjoint_forcing(const GRAdjLaunch& A, hipStream_t s);
+// The forward forcing (output bitwise that of launch_gr_forcing) writing the tape.
+hipError_t launch_gr_forcing_tape(const GRAdjLaunch& A, hipStream_t s);
 
 // The GR launch that cosmos_sweep / cosmos_forcing of a black-hole solver
 // would issue (coefficients, spacetime, physics parameters, flux planes),

@@ -153,6 +153,9 @@ struct GRLaunch {
     ConstPlanes in;
     Planes out;
     Planes flux;                // interface fluxes: x sweep [ny][fpitch], y sweep [ny + 1][fpitch]
+    double* half;               // SLIC sweeps, optional: the two evolved face states of each interface
+                                // (10 planes of half_plane doubles in the flux layout), kept for the adjoint
+    long half_plane;
     int nx, ny;
     long pitch;                 // doubles per state row
     long fpitch;                // doubles per flux row (>= nx + 1)

@@ -55,6 +55,8 @@ SIGNATURES = {
                                                 ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_begin": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_int]),
+    "cosmos_begin_at": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int]),
     "cosmos_advance": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_progress": (ctypes.c_int, [_handle, _c_double_p, _c_int_p, _c_int_p]),
     "cosmos_solve": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -64,6 +66,12 @@ SIGNATURES = {
     "cosmos_solve_forced": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _c_double_p,
                                             _c_int_p]),
+    "cosmos_solve_forced_at": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                               ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_double, ctypes.c_int, _c_double_p, _c_int_p]),
+    "cosmos_checkpoint_save": (ctypes.c_int, [_handle, ctypes.c_char_p, ctypes.c_double, ctypes.c_int]),
+    "cosmos_checkpoint_load": (ctypes.c_int, [_handle, ctypes.c_char_p, _c_double_p, _c_int_p]),
+    "cosmos_write_dat": (ctypes.c_int, [_handle, ctypes.c_char_p]),
     "cosmos_batch_default_schedule": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(BatchSchedule)]),
     "cosmos_batch_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.POINTER(BatchProblem), ctypes.c_int,
                                             ctypes.c_int, ctypes.c_double, ctypes.c_int]),
@@ -98,6 +106,7 @@ SIGNATURES = {
     "cosmos_adjoint_destroy": (ctypes.c_int, [_handle]),
     "cosmos_adjoint_forcing": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
     "cosmos_adjoint_set_subcycles": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cosmos_adjoint_set_tape": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_longlong]),
     "cosmos_adjoint_last_error": (ctypes.c_char_p, [_handle]),
     "cosmos_adjoint_solver": (_handle, [_handle]),
     "cosmos_adjoint_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

@@ -24,6 +24,7 @@ RELATIVISTIC = 1
 BLACKHOLE = 2
 SUPERBEE, VANLEER, MINBEE = 0, 1, 2
 EXACT, FAST = 0, 1
+TAPE_NONE, TAPE_RERUN, TAPE_SEGMENT = 0, 1, 2
 
 
 def _dp(a: np.ndarray):
@@ -129,9 +130,15 @@ class Solver:
         return v.value
 
     def begin(self, order: int, cell_spacing: float, cfl: float, final_time: float = math.inf,
-              max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER) -> None:
-        self._check(self._lib.cosmos_begin(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
-                                           slope_limiter))
+              max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER, start_time: float = 0.0,
+              start_iteration: int = 0) -> None:
+        """cosmos_begin, or cosmos_begin_at for a run resumed at (start_time, start_iteration)."""
+        if start_time == 0.0 and start_iteration == 0:
+            self._check(self._lib.cosmos_begin(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
+                                               slope_limiter))
+        else:
+            self._check(self._lib.cosmos_begin_at(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
+                                                  slope_limiter, start_time, start_iteration))
 
     def advance(self, n_steps: int) -> None:
         self._check(self._lib.cosmos_advance(self._h, n_steps))
@@ -158,12 +165,28 @@ class Solver:
         self._check(self._lib.cosmos_forcing(self._h, cell_spacing, time_step, bias, slope_limiter))
 
     def solve_forced(self, order: int, cell_spacing: float, cfl: float, final_time: float = math.inf,
-                     max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER, subcycles: int = 1):
-        """Returns (time reached, steps taken)."""
+                     max_steps: int = -1, bias: float = 0.0, slope_limiter: int = VANLEER, subcycles: int = 1,
+                     start_time: float = 0.0, start_iteration: int = 0):
+        """Returns (time reached, steps taken); resumes at (start_time, start_iteration) when given."""
         t, n = ctypes.c_double(), ctypes.c_int()
-        self._check(self._lib.cosmos_solve_forced(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
-                                                  slope_limiter, subcycles, ctypes.byref(t), ctypes.byref(n)))
+        self._check(self._lib.cosmos_solve_forced_at(self._h, order, cell_spacing, cfl, final_time, max_steps, bias,
+                                                     slope_limiter, subcycles, start_time, start_iteration,
+                                                     ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
+
+    # -- checkpoint / restart and text output --------------------------------
+    def checkpoint_save(self, path: str, time: float, iteration: int) -> None:
+        self._check(self._lib.cosmos_checkpoint_save(self._h, str(path).encode(), time, iteration))
+
+    def checkpoint_load(self, path: str):
+        """Restores the state (and a black hole's spacetime); returns (time, iteration)."""
+        t, it = ctypes.c_double(), ctypes.c_int()
+        self._check(self._lib.cosmos_checkpoint_load(self._h, str(path).encode(), ctypes.byref(t), ctypes.byref(it)))
+        return t.value, it.value
+
+    def write_dat(self, directory: str) -> None:
+        """density.dat ... pressure.dat in the reference's text layout."""
+        self._check(self._lib.cosmos_write_dat(self._h, str(directory).encode()))
 
     def synchronize(self) -> None:
         self._check(self._lib.cosmos_synchronize(self._h))
@@ -290,6 +313,10 @@ class Adjoint:
 
     def set_subcycles(self, subcycles: int) -> None:
         self._check(self._lib.cosmos_adjoint_set_subcycles(self._h, subcycles))
+
+    def set_tape(self, mode: int, reserve_bytes: int = 8 << 30) -> None:
+        """Black hole: TAPE_NONE, TAPE_RERUN or TAPE_SEGMENT (cosmos_adjoint_set_tape)."""
+        self._check(self._lib.cosmos_adjoint_set_tape(self._h, mode, reserve_bytes))
 
     def forcing(self, cell_spacing: float, time_step: float, lam_out: np.ndarray) -> np.ndarray:
         """Advance the state by one RK4 forcing step; return (d out / d in)^T lam_out."""

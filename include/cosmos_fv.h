@@ -168,6 +168,14 @@ int cosmos_advance(cosmos_solver* s, int n_steps);
 int cosmos_progress(cosmos_solver* s, double* time, int* steps, int* stopped);
 int cosmos_solve(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
                  double bias, int slope_limiter, double* time_out, int* steps_out);
+/* cosmos_begin for a resumed run (no counterpart in the reference, whose loops
+ * always start at t = 0, iteration 0): the loop starts at start_time with the
+ * iteration counter at start_iteration (the first six iterations' 0.2x dt
+ * damping keys on it, Solvers/EulerSolvers.cpp:134-137), and max_steps caps the
+ * steps taken from there.  Resuming from a checkpoint of step k with the same
+ * final time continues the run bitwise. */
+int cosmos_begin_at(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
+                    double bias, int slope_limiter, double start_time, int start_iteration);
 
 /* COSMOS_BLACKHOLE only.  One RK4 source-term step in place:
  * BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep{,2D}
@@ -184,6 +192,31 @@ int cosmos_forcing(cosmos_solver* s, double cell_spacing, double time_step, doub
 int cosmos_solve_forced(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time,
                         int max_steps, double bias, int slope_limiter, int subcycles, double* time_out,
                         int* steps_out);
+/* The same loop resumed at start_time / start_iteration (as cosmos_begin_at);
+ * steps_out counts the steps taken by this call. */
+int cosmos_solve_forced_at(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time,
+                           int max_steps, double bias, int slope_limiter, int subcycles, double start_time,
+                           int start_iteration, double* time_out, int* steps_out);
+
+/* ------------------------------------------------------------------------
+ * Checkpoint / restart and text output.
+ * cosmos_checkpoint_save writes the solver's state losslessly (FP64 bits) with
+ * the run's time and iteration counter, the adiabatic index and, for a black
+ * hole, the spacetime and excision radius; cosmos_checkpoint_load checks the
+ * file (magic, version, physics, grid, adiabatic index, checksum) against the
+ * solver, restores the state (and spacetime) and returns the time and
+ * iteration to pass to cosmos_begin_at / cosmos_solve_forced_at.  No
+ * counterpart in the reference, which keeps no restart state.
+ * cosmos_write_dat writes density.dat, xVelocity.dat, yVelocity.dat,
+ * zVelocity.dat and pressure.dat into `directory` in the layout of
+ * EulerTests::outputSolution{,2D} (Tests/EulerTests.cpp:189-252), shared by the
+ * relativistic tests, and, for a black hole, of
+ * BlackHoleEulerTests::outputSolution{,2D} (Tests/BlackHoleEulerTests.cpp:201-286):
+ * excised cells written as 0.
+ * ------------------------------------------------------------------------ */
+int cosmos_checkpoint_save(cosmos_solver* s, const char* path, double time, int iteration);
+int cosmos_checkpoint_load(cosmos_solver* s, const char* path, double* time, int* iteration);
+int cosmos_write_dat(cosmos_solver* s, const char* directory);
 
 /* ------------------------------------------------------------------------
  * Two-level adaptive mesh refinement (AMR/EulerAMR.cpp, AMR/RelativisticEulerAMR.cpp).
@@ -282,6 +315,19 @@ int cosmos_adjoint_forcing(cosmos_adjoint* a, double cell_spacing, double time_s
 /* Forcing sub-steps on each side of a step's sweeps (default 1; 0 to 8), as
  * the subcyclingIterations argument of the black-hole solve loops. */
 int cosmos_adjoint_set_subcycles(cosmos_adjoint* a, int subcycles);
+/* Black hole: how the reverse pass gets the values of each forcing step's RK4
+ * chain.  COSMOS_TAPE_NONE: the adjoint kernel re-runs the chain in
+ * registers; COSMOS_TAPE_RERUN: a forward kernel writes the chain to a device
+ * tape (about 100 doubles per cell) just before the step's adjoint;
+ * COSMOS_TAPE_SEGMENT (default): the segment recompute writes one tape per
+ * forcing step of the segment, and one per sweep holding its interface
+ * fluxes and SLIC face states (15 doubles per interface), and the reverse
+ * reads them instead of re-running the steps; it falls back to
+ * COSMOS_TAPE_RERUN when those tapes would leave less than reserve_bytes of
+ * device memory free (default 8 GiB).  The gradients agree to rounding (the
+ * adjoint kernels accumulate with atomic adds) across the modes. */
+enum { COSMOS_TAPE_NONE = 0, COSMOS_TAPE_RERUN = 1, COSMOS_TAPE_SEGMENT = 2 };
+int cosmos_adjoint_set_tape(cosmos_adjoint* a, int mode, long long reserve_bytes);
 
 /* Batched point-wise limiter functions on host arrays (evaluated on the device):
  *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)

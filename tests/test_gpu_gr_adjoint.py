@@ -15,7 +15,7 @@ in all (a wrong adjoint is off by O(1))."""
 import numpy as np
 import pytest
 
-from cosmos_amd import BLACKHOLE, Adjoint
+from cosmos_amd import BLACKHOLE, TAPE_NONE, TAPE_RERUN, TAPE_SEGMENT, Adjoint
 
 pytestmark = pytest.mark.gpu
 
@@ -130,3 +130,48 @@ def test_gr_trajectory_gradient(order, every):
         errs = np.array(errs)
         assert np.abs(grad).max() > 1e-6
         assert (errs < TOL).sum() >= 3 and errs.max() < 1e-3, errs
+
+
+def close(a, b, tol=1e-11):
+    """Agreement to rounding: the adjoint kernels accumulate with atomic adds."""
+    scale = max(np.abs(a).max(), np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() <= tol * scale
+
+
+@pytest.mark.parametrize("spin,excision", [(0.0, 2.5), (0.5, 0.0)])
+def test_gr_forcing_tape_modes(spin, excision):
+    """The taped forcing adjoint (the chain written by a forward kernel, then
+    walked back) against the adjoint kernel that re-runs the chain in
+    registers; the tape kernel's rewrite of the step's output is bitwise the
+    forward forcing."""
+    W, h = bh_state(20, 21, ny=22)
+    lam = np.random.default_rng(22).standard_normal(W.shape)
+    dt = 0.2 * h
+    out = {}
+    for mode in (TAPE_NONE, TAPE_RERUN):
+        with make(20, 22, spin, excision) as a:
+            a.set_tape(mode)
+            a.upload(W)
+            out[mode] = (a.forcing(h, dt, lam), a.download())
+    assert np.array_equal(out[TAPE_NONE][1], out[TAPE_RERUN][1])
+    assert np.abs(out[TAPE_NONE][0]).max() > 1e-3
+    assert close(out[TAPE_NONE][0], out[TAPE_RERUN][0])
+
+
+def test_gr_trajectory_tape_modes():
+    """The trajectory gradient is the same whether the forcing chains come from
+    per-segment tapes, a re-run tape or the in-register re-run."""
+    W0, h = bh_state(24, 31)
+    target = bh_state(24, 32)[0][..., 0]
+    grads = {}
+    for mode in (TAPE_NONE, TAPE_RERUN, TAPE_SEGMENT):
+        with make(24, excision=2.5) as a:
+            a.set_tape(mode, 0)
+            a.upload(W0)
+            a.forward(2, h, 0.4, 5, 0.0, 1, 2)
+            WN = a.download()
+            grads[mode] = a.backward(loss_and_grad(WN, target)[1])
+            assert np.array_equal(a.download(), WN)
+    assert np.abs(grads[TAPE_NONE]).max() > 1e-6
+    assert close(grads[TAPE_NONE], grads[TAPE_RERUN])
+    assert close(grads[TAPE_NONE], grads[TAPE_SEGMENT])
