@@ -394,8 +394,9 @@ def gr_adjoint_leg(args):
     p = 0.01, v = 0; SLIC + van Leer with one RK4 forcing sub-step on each side
     of the sweeps, CFL 0.4, args.gra_steps steps, a checkpoint every
     args.gra_every.  forward = the solve loop (cosmos_solve_forced) over the
-    same steps; adjoint = cosmos_adjoint_backward (segment recompute + reverse)
-    for L = sum (rho_N - rho*)^2, rho* from seed + 1.  The finite-difference
+    same steps; adjoint = cosmos_adjoint_backward (the reverse over the segment
+    the forward pass taped; then again with every segment recomputed) for
+    L = sum (rho_N - rho*)^2, rho* from seed + 1.  The finite-difference
     check runs the same workload at args.gra_fd_size with a tight con2prim
     (1e-13), central differences with h = 1e-6 and 1e-8 along 32 seeded
     directions of the initial density field."""
@@ -425,7 +426,13 @@ def gr_adjoint_leg(args):
         t0 = time.perf_counter()
         grad = a.backward(g)
         adj = time.perf_counter() - t0
+        # the same with nothing left by the forward pass: every segment recomputed
+        a.release()
+        t0 = time.perf_counter()
+        grad2 = a.backward(g)
+        adj_rc = time.perf_counter() - t0
         finite = bool(np.all(np.isfinite(WN)) and np.all(np.isfinite(grad)))
+        same_grad = float(np.abs(grad2 - grad).max() / max(np.abs(grad).max(), 1e-300))
 
     m, fsteps = args.gra_fd_size, args.gra_fd_steps
     W, h, hole = workloads.gr_random_field_2d(m, seed=args.seed + 7)
@@ -463,6 +470,13 @@ def gr_adjoint_leg(args):
                         f"r<2.5M, rho = 1 + 0.05 GRF (seed {args.seed}), p = 0.01, SLIC+van Leer, RK4 forcing "
                         f"(1 subcycle), CFL 0.4, {steps} steps, checkpoint every {args.gra_every}, dt schedule frozen",
             "forward_s": fwd, "forward_recording_s": rec, "adjoint_s": adj, "adjoint_forward_ratio": adj / fwd,
+            "adjoint_recomputed_s": adj_rc, "adjoint_recomputed_forward_ratio": adj_rc / fwd,
+            "recording_plus_adjoint_forward_ratio": (rec + adj) / fwd,
+            "recomputed_vs_first_gradient_max_rel_diff": same_grad,
+            "note": "adjoint_s: the reverse pass right after the recording forward pass, which leaves its last "
+                    "segment (here the only one) taped in HBM; adjoint_recomputed_s: the reverse pass that "
+                    "recomputes every segment from its checkpoint (cosmos_adjoint_release first), as every "
+                    "segment but the last does in a longer run",
             "state_and_gradient_finite": finite,
             "fd_check": {"grid": [m, m], "steps": fsteps, "con2prim_tolerance": 1e-13, "directions": 32,
                          "direction": "seeded normal perturbations of the initial density field",
@@ -476,8 +490,9 @@ def adjoint_leg(args):
     """Adjoint of the Newtonian SLIC update: the 4-quadrant workload at
     n x n (CFL 0.9, van Leer), args.adj_steps steps with a checkpoint every
     20.  forward = the device solve loop (exact arithmetic) over the same
-    steps; adjoint = cosmos_adjoint_backward (segment recompute + reverse
-    sweeps).  Gradient agreement of L = sum (rho_N - rho*)^2 against central
+    steps; adjoint = cosmos_adjoint_backward (segment recompute, except the
+    segment the forward pass left in the buffers, + reverse sweeps), and the
+    same with every segment recomputed.  Gradient agreement of L = sum (rho_N - rho*)^2 against central
     differences along 32 seeded directions, dt schedule held fixed."""
     from cosmos_amd import NEWTONIAN, VANLEER, Adjoint, Solver, workloads
 
@@ -505,6 +520,10 @@ def adjoint_leg(args):
         t0 = time.perf_counter()
         a.backward(g)
         bwd_s = time.perf_counter() - t0
+        a.release()
+        t0 = time.perf_counter()
+        a.backward(g)
+        bwd_rc_s = time.perf_counter() - t0
     # Gradient agreement: central differences with the dt schedule frozen, along
     # 32 seeded directions.  The SLIC limiters are piecewise smooth (kinks at
     # r = 0 and r = 1 for van Leer); a finite-difference stencil that straddles
@@ -552,6 +571,10 @@ def adjoint_leg(args):
         "forward_recording_s": rec_s,
         "adjoint_s": bwd_s,
         "adjoint_forward_ratio": bwd_s * 1e3 / fwd_ms,
+        "adjoint_recomputed_s": bwd_rc_s,
+        "adjoint_recomputed_forward_ratio": bwd_rc_s * 1e3 / fwd_ms,
+        "note": "adjoint_s: the reverse pass after the recording forward pass (which leaves its last segment in "
+                "the segment buffers); adjoint_recomputed_s: every segment recomputed (cosmos_adjoint_release first)",
         "fd_check": fd,
     }
 
@@ -645,6 +668,8 @@ def main():
     ap.add_argument("--batch-tune-steps", type=int, default=10)
     ap.add_argument("--no-batch-baseline", action="store_true", help="skip the per-problem launch baselines")
     ap.add_argument("--no-extra", action="store_true", help="skip the SR, black-hole, adjoint and batched legs")
+    ap.add_argument("--legs", default="relativistic,blackhole,adjoint,adjoint_blackhole,batched",
+                    help="comma-separated further legs to run (default: all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -707,11 +732,12 @@ def main():
                   "every launch bracketed by events",
     }
     if not args.no_extra:
-        line["relativistic"] = sr_leg(args)
-        line["blackhole"] = gr_leg(args)
-        line["adjoint"] = adjoint_leg(args)
-        line["adjoint_blackhole"] = gr_adjoint_leg(args)
-        line["batched"] = batch_leg(args)
+        legs = {"relativistic": sr_leg, "blackhole": gr_leg, "adjoint": adjoint_leg,
+                "adjoint_blackhole": gr_adjoint_leg, "batched": batch_leg}
+        for name in args.legs.split(","):
+            if name not in legs:
+                ap.error(f"unknown leg {name!r} (choose from {', '.join(legs)})")
+            line[name] = legs[name](args)
     if not args.no_cpu:
         line.update(cpu_leg(args))
     print(json.dumps(line), flush=True)

@@ -272,9 +272,15 @@ struct cosmos_adjoint {
     int tape_mode = COSMOS_TAPE_SEGMENT;
     long long tape_reserve = 8LL << 30;
     double* tape = nullptr;            // the re-run tape
+    double* carry = nullptr;           // running adjoints of the taped forcing reverse
     std::vector<double*> tapes;        // one per forcing step of a segment
     std::vector<double*> stapes;       // one per sweep of a segment: fluxes (5 planes) and face states (10)
     long fplane = 0;                   // doubles per flux plane
+    // the segment whose states (and, with recorded_tapes, tapes) the segment
+    // buffers hold: the forward pass leaves the last one there, so that the
+    // reverse pass need not recompute it; -1: none
+    int recorded_seg = -1;
+    bool recorded_tapes = false;
     std::string err;
 };
 
@@ -516,6 +522,9 @@ int adjoint_op_dev(cosmos_adjoint* a, const Op& op, int order, double bias, int 
         }
         A.tape = tape;
         A.tape_plane = (long)a->plane;
+        if (tape && !a->carry)
+            ADJCHK(a, hipMalloc((void**)&a->carry, sizeof(double) * cfv::kCarryPlanes * a->plane));
+        A.carry = a->carry;
         ADJCHK(a, hipMemsetAsync(a->lam[dst], 0, sizeof(double) * 5 * a->plane, st));
         ADJCHK(a, cfv::launch_gr_adjoint_forcing(A, st));
     } else {
@@ -541,6 +550,62 @@ int adj_guarded(cosmos_adjoint* a, F&& f) {
 void free_list(std::vector<double*>& v) {
     for (double* p : v) hipFree(p);
     v.clear();
+}
+
+// The buffers of one segment: the output state of every operation (the
+// segment's first input is its checkpoint) and, for a black hole in
+// COSMOS_TAPE_SEGMENT mode, a tape per forcing step and per sweep where they
+// leave tape_reserve bytes of device memory free (seg_tapes says whether).
+int segment_buffers(cosmos_adjoint* a, bool& seg_tapes) {
+    Op ops[kMaxOps];
+    const int ns = step_ops(a, 1.0, ops);
+    const size_t need = (size_t)a->every * ns;
+    while (a->seg.size() < need) {
+        double* p = nullptr;
+        int rc = alloc_state(a, p);
+        if (rc) return rc;
+        a->seg.push_back(p);
+    }
+    int forcings = 0;
+    for (int k = 0; k < ns; ++k) forcings += ops[k].forcing;
+    const size_t nft = (size_t)a->every * forcings, nst = (size_t)a->every * (ns - forcings);
+    seg_tapes = a->physics == COSMOS_BLACKHOLE && a->tape_mode == COSMOS_TAPE_SEGMENT;
+    if (seg_tapes && (a->tapes.size() < nft || a->stapes.size() < nst)) {
+        cfv::GRLaunch L;  // the flux layout (allocates the solver's flux planes)
+        const Op sw{0, 0, 1.0};
+        int rc = gr_launch_of(a, sw, a->bias, a->limiter, L);
+        if (rc) return rc;
+        a->fplane = (long)(a->ny + 1) * L.fpitch;
+        size_t free_b = 0, total_b = 0;
+        ADJCHK(a, hipMemGetInfo(&free_b, &total_b));
+        const size_t more = (nft - std::min(nft, a->tapes.size())) * tape_bytes(a) +
+                            (nst - std::min(nst, a->stapes.size())) * sweep_tape_bytes(a);
+        if (free_b < more || free_b - more < (size_t)a->tape_reserve) {
+            seg_tapes = false;
+        } else {
+            for (auto* v : {&a->tapes, &a->stapes}) {
+                const size_t want = v == &a->tapes ? nft : nst;
+                const size_t bytes = v == &a->tapes ? tape_bytes(a) : sweep_tape_bytes(a);
+                while (v->size() < want) {
+                    double* p = nullptr;
+                    ADJCHK(a, hipMalloc((void**)&p, bytes));
+                    v->push_back(p);
+                }
+            }
+        }
+    }
+    return COSMOS_OK;
+}
+
+// The tape of operation k of the step_in_seg-th step of a segment (null
+// without segment tapes).
+double* seg_tape(const cosmos_adjoint* a, const Op* ops, int ns, int step_in_seg, int k, bool seg_tapes) {
+    if (!seg_tapes) return nullptr;
+    int forcings = 0, f = 0;
+    for (int q = 0; q < ns; ++q) forcings += ops[q].forcing;
+    for (int q = 0; q < k; ++q) f += ops[q].forcing;
+    return ops[k].forcing ? a->tapes[(size_t)step_in_seg * forcings + f]
+                          : a->stapes[(size_t)step_in_seg * (ns - forcings) + (k - f)];
 }
 
 }  // namespace
@@ -596,6 +661,7 @@ int cosmos_adjoint_destroy(cosmos_adjoint* a) {
     hipFree(a->stage);
     hipFree(a->gbuf);
     hipFree(a->tape);
+    hipFree(a->carry);
     free_list(a->tapes);
     free_list(a->stapes);
     free_list(a->checkpoints);
@@ -657,6 +723,20 @@ int cosmos_adjoint_set_subcycles(cosmos_adjoint* a, int subcycles) {
     });
 }
 
+int cosmos_adjoint_release(cosmos_adjoint* a) {
+    return adj_guarded(a, [&] {
+        ADJCHK(a, hipStreamSynchronize(stream_of(a)));
+        free_list(a->seg);
+        free_list(a->tapes);
+        free_list(a->stapes);
+        hipFree(a->tape);
+        hipFree(a->carry);
+        a->tape = a->carry = nullptr;
+        a->recorded_seg = -1;
+        return COSMOS_OK;
+    });
+}
+
 int cosmos_adjoint_set_tape(cosmos_adjoint* a, int mode, long long reserve_bytes) {
     return adj_guarded(a, [&] {
         if (mode < COSMOS_TAPE_NONE || mode > COSMOS_TAPE_SEGMENT || reserve_bytes < 0)
@@ -666,6 +746,7 @@ int cosmos_adjoint_set_tape(cosmos_adjoint* a, int mode, long long reserve_bytes
         if (mode != COSMOS_TAPE_SEGMENT) {
             free_list(a->tapes);
             free_list(a->stapes);
+            if (a->recorded_tapes) a->recorded_seg = -1;
         }
         return COSMOS_OK;
     });
@@ -683,10 +764,15 @@ int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, do
         a->every = checkpoint_every;
         a->dts.clear();
         free_list(a->checkpoints);
-        free_list(a->seg);
+        a->recorded_seg = -1;
         // the reverse pass recomputes segments with the exact kernels
         int rc0 = cosmos_set_mode(a->s, COSMOS_MODE_EXACT);
         if (rc0) return from_solver(a, rc0);
+        // the last segment runs through the segment buffers (and tapes), where
+        // the reverse pass finds it without recomputing it
+        bool seg_tapes = false;
+        if ((rc0 = segment_buffers(a, seg_tapes))) return rc0;
+        const int last = (n_steps - 1) / a->every;
         Op ops[kMaxOps];
         for (int it = 0; it < n_steps; ++it) {
             if (it % a->every == 0) {
@@ -701,10 +787,27 @@ int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, do
             if (rc) return from_solver(a, rc);
             a->dts.push_back(dt);
             const int ns = step_ops(a, dt, ops);
-            for (int k = 0; k < ns; ++k)
-                if ((rc = apply_op(a, ops[k]))) return rc;
+            if (it / a->every != last) {
+                for (int k = 0; k < ns; ++k)
+                    if ((rc = apply_op(a, ops[k]))) return rc;
+                continue;
+            }
+            const int sis = it - last * a->every;
+            auto state = [&](int q) -> double* { return q == 0 ? a->checkpoints[last] : a->seg[q - 1]; };
+            for (int k = 0; k < ns; ++k) {
+                const int q = sis * ns + k;
+                if ((rc = forward_op_dev(a, ops[k], state(q), state(q + 1), seg_tape(a, ops, ns, sis, k, seg_tapes))))
+                    return rc;
+            }
+            // the step's output becomes the solver's state (the next dt reads it)
+            double* cur = nullptr;
+            if ((rc = current_base(a, cur))) return rc;
+            ADJCHK(a, hipMemcpyAsync(cur, state((sis + 1) * ns), sizeof(double) * 5 * a->plane,
+                                     hipMemcpyDeviceToDevice, stream_of(a)));
         }
         ADJCHK(a, hipStreamSynchronize(stream_of(a)));
+        a->recorded_seg = last;
+        a->recorded_tapes = seg_tapes;
         return COSMOS_OK;
     });
 }
@@ -732,67 +835,27 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
         Op ops[kMaxOps];
         const int ns = step_ops(a, 1.0, ops);  // operations per step (the count does not depend on dt)
         const int nsteps = (int)a->dts.size();
-        // segment buffers: the output of every sweep of a segment (its first
-        // input is the checkpoint itself)
-        const size_t need = (size_t)a->every * ns;
-        while (a->seg.size() < need) {
-            double* p = nullptr;
-            int rc = alloc_state(a, p);
-            if (rc) return rc;
-            a->seg.push_back(p);
-        }
-        // black hole: a tape per forcing step and per sweep of one segment,
-        // where they leave tape_reserve bytes of device memory free
-        const bool bh = a->physics == COSMOS_BLACKHOLE;
-        int forcings = 0;
-        for (int k = 0; k < ns; ++k) forcings += ops[k].forcing;
-        const int sweeps = ns - forcings;
-        const size_t nft = (size_t)a->every * forcings, nst = (size_t)a->every * sweeps;
-        bool seg_tapes = bh && a->tape_mode == COSMOS_TAPE_SEGMENT;
-        if (seg_tapes && (a->tapes.size() < nft || a->stapes.size() < nst)) {
-            cfv::GRLaunch L;  // the flux layout (allocates the solver's flux planes)
-            const Op sw{0, 0, 1.0};
-            int rc0 = gr_launch_of(a, sw, a->bias, a->limiter, L);
-            if (rc0) return rc0;
-            a->fplane = (long)(a->ny + 1) * L.fpitch;
-            size_t free_b = 0, total_b = 0;
-            ADJCHK(a, hipMemGetInfo(&free_b, &total_b));
-            const size_t more = (nft - std::min(nft, a->tapes.size())) * tape_bytes(a) +
-                                (nst - std::min(nst, a->stapes.size())) * sweep_tape_bytes(a);
-            if (free_b < more || free_b - more < (size_t)a->tape_reserve) {
-                seg_tapes = false;
-            } else {
-                for (auto* v : {&a->tapes, &a->stapes}) {
-                    const size_t want = v == &a->tapes ? nft : nst;
-                    const size_t bytes = v == &a->tapes ? tape_bytes(a) : sweep_tape_bytes(a);
-                    while (v->size() < want) {
-                        double* p = nullptr;
-                        ADJCHK(a, hipMalloc((void**)&p, bytes));
-                        v->push_back(p);
-                    }
-                }
-            }
-        }
-        // the tape of operation k of the step_in_seg-th step of the segment
-        auto tape_of = [&](int step_in_seg, int k) -> double* {
-            if (!seg_tapes) return nullptr;
-            int f = 0;
-            for (int q = 0; q < k; ++q) f += ops[q].forcing;
-            return ops[k].forcing ? a->tapes[(size_t)step_in_seg * forcings + f]
-                                  : a->stapes[(size_t)step_in_seg * sweeps + (k - f)];
-        };
-        int rc = lam_upload(a, grad_out, 0);
+        bool seg_tapes = false;
+        int rc = segment_buffers(a, seg_tapes);
         if (rc) return rc;
+        auto tape_of = [&](int step_in_seg, int k) { return seg_tape(a, ops, ns, step_in_seg, k, seg_tapes); };
+        if ((rc = lam_upload(a, grad_out, 0))) return rc;
         int cur = 0;
         for (int sgi = (int)a->checkpoints.size() - 1; sgi >= 0; --sgi) {
             const int s0 = sgi * a->every, s1 = std::min(nsteps, s0 + a->every);
-            // recompute the segment from its checkpoint: state q + 1 = sweep q of state q
+            // recompute the segment from its checkpoint (state q + 1 = operation q
+            // of state q), unless the forward pass left it in the buffers
             auto state = [&](int q) -> double* { return q == 0 ? a->checkpoints[sgi] : a->seg[q - 1]; };
+            const bool kept = sgi == a->recorded_seg && a->recorded_tapes == seg_tapes;
             int q = 0;
-            for (int it = s0; it < s1; ++it) {
+            for (int it = s0; it < s1 && !kept; ++it) {
                 step_ops(a, a->dts[it], ops);
                 for (int k = 0; k < ns; ++k, ++q)
                     if ((rc = forward_op_dev(a, ops[k], state(q), state(q + 1), tape_of(it - s0, k)))) return rc;
+            }
+            if (!kept) {  // the buffers now hold segment sgi
+                a->recorded_seg = sgi;
+                a->recorded_tapes = seg_tapes;
             }
             // reverse through the segment
             for (int it = s1 - 1; it >= s0; --it) {

@@ -6,8 +6,8 @@
 //
 // Sweep: the forward fluxes are in the flux planes (the caller re-runs the
 // forward sweep first, or points them at the ones a forward sweep kept, with
-// its SLIC face states: then the interface adjoint runs as two launches, see
-// k_gra_iface_force).  k_gra_cells gives each cell's adjoint through its
+// its SLIC face states, which the interface adjoint then reads instead of
+// re-running the half steps).  k_gra_cells gives each cell's adjoint through its
 // update (c2p, then p2c back to the input primitives; excised stencils pass
 // their adjoint through unchanged) and stores the adjoint of the updated
 // conserved vector; k_gra_ifaces forms each interface flux's adjoint from its
@@ -17,7 +17,9 @@
 //
 // Forcing: one thread per cell re-runs the cell's RK4 chain, keeping every
 // stage's states, then walks it back; its own and its four neighbours'
-// adjoints are accumulated with atomic adds into a zeroed output.
+// adjoints are accumulated with atomic adds into a zeroed output.  With a
+// tape (written by k_gra_forcing_tape) the walk back runs as a chain of
+// smaller kernels instead (k_grr_*).
 #include "fv_gr_adjoint.hpp"
 #include "fv_gr_adjoint_launch.hpp"
 
@@ -25,16 +27,6 @@ namespace cfv {
 namespace {
 
 constexpr int kGAT = 128;
-// Minimum waves per SIMD the register allocation must allow (more waves hide
-// more FP64 latency; fewer registers per wave spill more) for the split
-// interface kernels and the taped forcing reverse.
-#ifndef COSMOS_GRA_IFACE_WAVES
-#define COSMOS_GRA_IFACE_WAVES 2
-#endif
-#ifndef COSMOS_GRA_REV_WAVES
-#define COSMOS_GRA_REV_WAVES 2
-#endif
-constexpr int kIfaceWaves = COSMOS_GRA_IFACE_WAVES, kRevWaves = COSMOS_GRA_REV_WAVES;
 
 __device__ __forceinline__ int mirror2_a(int c, int n) {
     if (c < 0) c = -c - 1;
@@ -174,8 +166,18 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     ld5(L.in, at(c + 1), W3);
     // each metric point evaluated once for the forward re-run and the adjoint
     const adjgr::HalfGeo HL = adjgr::half_geo<AXIS>(b, xl, yl, L.h), HR = adjgr::half_geo<AXIS>(b, xr, yr, L.h);
-    adjgr::half_fwd_g<AXIS>(W0, W1, W2, 1, HL, L.coef, L.bias, L.limiter, L.P, pr);
-    adjgr::half_fwd_g<AXIS>(W1, W2, W3, 0, HR, L.coef, L.bias, L.limiter, L.P, pl);
+    if (L.half) {
+        // the face states the forward sweep kept
+        const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            pr[k] = L.half[k * L.half_plane + f];
+            pl[k] = L.half[(5 + k) * L.half_plane + f];
+        }
+    } else {
+        adjgr::half_fwd_g<AXIS>(W0, W1, W2, 1, HL, L.coef, L.bias, L.limiter, L.P, pr);
+        adjgr::half_fwd_g<AXIS>(W1, W2, W3, 0, HR, L.coef, L.bias, L.limiter, L.P, pl);
+    }
     double gpr[5], gpl[5];
     zero5(gpr);
     zero5(gpl);
@@ -191,133 +193,6 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     add5(A.lam_in, at(c - 1), g1);
     add5(A.lam_in, at(c), g2);
     add5(A.lam_in, at(c + 1), g3);
-}
-
-// The SLIC interface adjoint in two launches when the forward sweep kept its
-// face states: k_gra_iface_force takes each interface flux's adjoint back
-// through FORCE to the adjoints of its two face states, written over those
-// states in the tape (zeros where the flux has no adjoint);
-// k_gra_iface_half takes each face state's adjoint back through its half step
-// (one thread per interface and side) onto the three input cells.
-template <int AXIS>
-__global__ __launch_bounds__(kGAT) __attribute__((amdgpu_waves_per_eu(kIfaceWaves))) void k_gra_iface_force(GRAdjLaunch A) {
-    const GRLaunch& L = A.L;
-    const int n = AXIS == 0 ? L.nx : L.ny;
-    const long lines = AXIS == 0 ? L.ny : L.nx;
-    const long per = AXIS == 0 ? (long)(n + 1) : lines;
-    const long total = (long)(n + 1) * lines;
-    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
-    if (idx >= total) return;
-    int c, i, j;
-    if (AXIS == 0) {
-        i = (int)(idx / per);
-        c = (int)(idx - (long)i * per);
-        j = c;
-    } else {
-        c = (int)(idx / per);
-        j = (int)(idx - (long)c * per);
-        i = c;
-    }
-    auto cell_at = [&](int cc) -> long { return AXIS == 0 ? (long)i * L.pitch + cc : (long)cc * L.pitch + j; };
-    double gF[5];
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const double gr = c < n ? A.gbuf.p[k][cell_at(c)] : 0.0;
-        const double gl = c > 0 ? A.gbuf.p[k][cell_at(c - 1)] : 0.0;
-        gF[k] = L.coef.dt_dx * (gr - gl);
-        any |= gF[k] != 0.0;
-    }
-    const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
-    double* const hp = L.half + f;
-    const long hpl = L.half_plane;
-    double gpr[5], gpl[5];
-    zero5(gpr);
-    zero5(gpl);
-    if (any) {
-        double pr[5], pl[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            pr[k] = hp[k * hpl];
-            pl[k] = hp[(5 + k) * hpl];
-        }
-        double x, y;
-        if (AXIS == 0) {
-            x = (c - 0.5) * L.h;
-            y = L.dim == 2 ? i * L.h : 0.0;
-        } else {
-            x = j * L.h;
-            y = (c - 0.5) * L.h;
-        }
-        adjgr::vjp_force_g<AXIS>(pr, pl, adjgr::force_geo<AXIS>(L.hole, x, y, L.h), L.coef, L.P, gF, gpr, gpl);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        hp[k * hpl] = gpr[k];
-        hp[(5 + k) * hpl] = gpl[k];
-    }
-}
-
-template <int AXIS>
-__global__ __launch_bounds__(kGAT) __attribute__((amdgpu_waves_per_eu(kIfaceWaves))) void k_gra_iface_half(GRAdjLaunch A) {
-    const GRLaunch& L = A.L;
-    const int n = AXIS == 0 ? L.nx : L.ny;
-    const long lines = AXIS == 0 ? L.ny : L.nx;
-    const long per = AXIS == 0 ? (long)(n + 1) : lines;
-    const long total = (long)(n + 1) * lines;
-    long t = (long)blockIdx.x * kGAT + threadIdx.x;
-    if (t >= 2 * total) return;
-    // neighbouring threads take the two sides of one interface
-    const int side = (int)(t & 1);
-    const long idx = t >> 1;
-    int c, i, j;
-    if (AXIS == 0) {
-        i = (int)(idx / per);
-        c = (int)(idx - (long)i * per);
-        j = c;
-    } else {
-        c = (int)(idx / per);
-        j = (int)(idx - (long)c * per);
-        i = c;
-    }
-    const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
-    const double* const hp = L.half + f + (long)(side == 0 ? 0 : 5) * L.half_plane;
-    double g[5];
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        g[k] = hp[k * L.half_plane];
-        any |= g[k] != 0.0;
-    }
-    if (!any) return;
-    auto at = [&](int cc) -> long {
-        const int m = mirror2_a(cc, n);
-        return AXIS == 0 ? (long)i * L.pitch + m : (long)m * L.pitch + j;
-    };
-    // side 0: the left cell c - 1's right face, evolved about the interface - h/2;
-    // side 1: the right cell c's left face, about the interface + h/2
-    const int c0 = side == 0 ? c - 2 : c - 1;
-    const double hh = L.h * 0.5;
-    double x, y;
-    if (AXIS == 0) {
-        x = (c - 0.5) * L.h + (side == 0 ? -hh : hh);
-        y = L.dim == 2 ? i * L.h : 0.0;
-    } else {
-        x = j * L.h;
-        y = (c - 0.5) * L.h + (side == 0 ? -hh : hh);
-    }
-    double W0[5], W1[5], W2[5], g0[5], g1[5], g2[5];
-    ld5(L.in, at(c0), W0);
-    ld5(L.in, at(c0 + 1), W1);
-    ld5(L.in, at(c0 + 2), W2);
-    zero5(g0);
-    zero5(g1);
-    zero5(g2);
-    adjgr::vjp_half_g<AXIS>(W0, W1, W2, side == 0 ? 1 : 0, adjgr::half_geo<AXIS>(L.hole, x, y, L.h), L.coef, L.bias,
-                            L.limiter, L.P, g, g0, g1, g2);
-    add5(A.lam_in, at(c0), g0);
-    add5(A.lam_in, at(c0 + 1), g1);
-    add5(A.lam_in, at(c0 + 2), g2);
 }
 
 // Per cell: the adjoint of the RK4 forcing (gr forcing kernel), into a zeroed lam_in.
@@ -458,7 +333,7 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
 // Taped forcing adjoint.  k_gra_forcing_tape runs the forward RK4 chain of
 // every cell (bitwise the forward forcing kernel; it writes the step's output
 // too) and keeps the values its reverse needs in the tape (one SoA plane per
-// value; layout in fv_gr_adjoint_launch.hpp); k_gra_forcing_rev walks the chain
+// value; layout in fv_gr_adjoint_launch.hpp); the k_grr_* kernels walk the chain
 // back from the tape, so the reverse pass evaluates no forward stage again.
 template <int DIM>
 __global__ __launch_bounds__(kGAT) void k_gra_forcing_tape(GRAdjLaunch A) {
@@ -548,147 +423,186 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing_tape(GRAdjLaunch A) {
     st5(L.out, cell, W);
 }
 
+// The reverse of a taped forcing step as a chain of small kernels, one per
+// piece of the RK4 chain, so that no kernel holds more than one fractional
+// sweep's adjoint in registers; the running adjoints pass through the carry
+// planes (layout in fv_gr_adjoint_launch.hpp).  Per stage st = 2, 1, 0:
+//   src: gk -> the source's adjoint at the stage's state -> ge (= adjoint of e1)
+//   fx1: (2D) the last x fraction back to e3; fy: the y fraction back to e2
+//   fx2: the first x fraction back to Wm, then through c2p to tmp: adds to
+//        guM, and sets gk for the stage before.
+// The neighbour-state adjoints gWn accumulate over all stages; head starts
+// the chain from the output adjoint, tail ends it at the input state and the
+// neighbours (atomic adds into a zeroed lam_in).
+struct RevCell {
+    int i, j;
+    long cell;
+    double x, y;
+};
+
 template <int DIM>
-__global__ __launch_bounds__(kGAT) __attribute__((amdgpu_waves_per_eu(kRevWaves))) void k_gra_forcing_rev(GRAdjLaunch A) {
+__device__ __forceinline__ bool rev_cell(const GRAdjLaunch& A, RevCell& c) {
+    const GRLaunch& L = A.L;
+    long idx = (long)blockIdx.x * kGAT + threadIdx.x;
+    if (idx >= (long)L.nx * L.ny) return false;
+    c.i = (int)(idx / L.nx);
+    c.j = (int)(idx - (long)c.i * L.nx);
+    c.cell = (long)c.i * L.pitch + c.j;
+    c.x = c.j * L.h;
+    c.y = DIM == 2 ? c.i * L.h : 0.0;
+    return clear_of(L.hole, DIM, c.i, c.j, 1, L.h);
+}
+
+__device__ __forceinline__ void tget(const GRAdjLaunch& A, int slot, long cell, double* v) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = A.tape[(long)(slot + k) * A.tape_plane + cell];
+}
+__device__ __forceinline__ void cget(const GRAdjLaunch& A, int slot, long cell, double* v) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) v[k] = A.carry[(long)(slot + k) * A.tape_plane + cell];
+}
+__device__ __forceinline__ void cput(const GRAdjLaunch& A, int slot, long cell, const double* v) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) A.carry[(long)(slot + k) * A.tape_plane + cell] = v[k];
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kGAT) void k_grr_head(GRAdjLaunch A) {
     const GRLaunch& L = A.L;
     long idx = (long)blockIdx.x * kGAT + threadIdx.x;
     if (idx >= (long)L.nx * L.ny) return;
-    int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
-    long cell = (long)i * L.pitch + j;
-    const double* const tp = A.tape + cell;
-    const long pl = A.tape_plane;
-    auto get = [&](int slot, double* v) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) v[k] = tp[(long)(slot + k) * pl];
-    };
+    RevCell c;
+    const bool clear = rev_cell<DIM>(A, c);
     double lam[5];
-    ld5(A.lam, cell, lam);
-    const Hole& b = L.hole;
-    const double h = L.h;
-    if (!clear_of(b, DIM, i, j, 1, h)) {
-        add5(A.lam_in, cell, lam);
+    ld5(A.lam, c.cell, lam);
+    if (!clear) {
+        add5(A.lam_in, c.cell, lam);
         return;
     }
-    const PhysParams& P = L.P;
-    const double tol = gr_tol(P, 1e-8);
-    const double x = j * h, y = DIM == 2 ? i * h : 0.0;
-    const Geo gc = geo_at(b, x, y, 0.0);
-    double gun[5], gk[5], guM[5], gWn[4][5];
-    {
-        double un[5];
-        get(kTapeUn, un);
-        zero5(gun);
-        adjgr::vjp_c2p(un, gc.sdet, tol, P, lam, gun);
+    const Geo gc = geo_at(L.hole, c.x, c.y, 0.0);
+    double un[5], gun[5], gk[5], z[5];
+    tget(A, kTapeUn, c.cell, un);
+    zero5(gun);
+    adjgr::vjp_c2p(un, gc.sdet, gr_tol(L.P, 1e-8), L.P, lam, gun);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) gk[k] = (1.0 / 6.0) * gun[k];
+    zero5(z);
+    cput(A, kCarryGun, c.cell, gun);
+    cput(A, kCarryGuM, c.cell, gun);
+    cput(A, kCarryGk, c.cell, gk);
+    for (int q = 0; q < 2 * DIM; ++q) cput(A, kCarryGWn + 5 * q, c.cell, z);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kGAT) void k_grr_src(GRAdjLaunch A, int st) {
+    const GRLaunch& L = A.L;
+    RevCell c;
+    if (!rev_cell<DIM>(A, c)) return;
+    const Geo gc = geo_at(L.hole, c.x, c.y, 0.0);
+    SourceGeo sg;
+    source_geo(L.hole, c.x, c.y, 0.0, L.h, sg);
+    double gk[5], e1[5], um[5], gsrc[5], gum[5], ge1[5];
+    cget(A, kCarryGk, c.cell, gk);
+    tget(A, kTapeStage + kTapePerStage * st + kTapeE1, c.cell, e1);
+    gr_p2c(e1, gc, um, L.P);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gk[k];
+    zero5(gum);
+    adjgr::vjp_source(um, sg, L.P, gsrc, gum);
+    zero5(ge1);
+    adjgr::vjp_p2c(e1, gc, L.P, gum, ge1);
+    cput(A, kCarryGe, c.cell, ge1);
+}
+
+// One fractional sweep's adjoint: carry ge (the adjoint of its output) back
+// to its middle input (slot `mid` of the stage) and the neighbour pair of its
+// axis; last == true: the stage's first x fraction, continued through c2p.
+template <int DIM, int AXIS, bool LAST>
+__global__ __launch_bounds__(kGAT) void k_grr_frac(GRAdjLaunch A, int st) {
+    const GRLaunch& L = A.L;
+    RevCell c;
+    if (!rev_cell<DIM>(A, c)) return;
+    const int base = kTapeStage + kTapePerStage * st;
+    const int mid = LAST ? kTapeWm : (AXIS == 0 ? kTapeE3 : kTapeE2);
+    const Geo gc = geo_at(L.hole, c.x, c.y, 0.0);
+    const Geo ga = adjgr::geo_shift<AXIS>(L.hole, c.x, c.y, -L.h), gb = adjgr::geo_shift<AXIS>(L.hole, c.x, c.y, L.h);
+    double ge[5], Wa[5], Wm[5], Wb[5], gWa[5], gWb[5], gm[5];
+    cget(A, kCarryGe, c.cell, ge);
+    tget(A, base + mid, c.cell, Wm);
+    tget(A, kTapeWn + 10 * AXIS, c.cell, Wa);
+    tget(A, kTapeWn + 10 * AXIS + 5, c.cell, Wb);
+    cget(A, kCarryGWn + 10 * AXIS, c.cell, gWa);
+    cget(A, kCarryGWn + 10 * AXIS + 5, c.cell, gWb);
+    zero5(gm);
+    const double fdt_h = AXIS == 0 ? L.frac_dt_h[st] : L.frac_dt_h_y[st];
+    adjgr::vjp_frac_g<AXIS>(fdt_h, Wa, Wm, Wb, ga, gc, gb, L.bias, L.limiter, L.P, ge, gWa, gm, gWb);
+    cput(A, kCarryGWn + 10 * AXIS, c.cell, gWa);
+    cput(A, kCarryGWn + 10 * AXIS + 5, c.cell, gWb);
+    if (!LAST) {
+        cput(A, kCarryGe, c.cell, gm);
+        return;
     }
-    // gk = adjoint of the stage increment k_{st+1} (starting with k4)
+    // Wm = c2p(tmp), tmp = uM + f k_st: the stage's adjoint goes to uM and to k_st
+    double tmp[5], gtmp[5], guM[5], gun[5], gk[5];
+    tget(A, base + kTapeTmp, c.cell, tmp);
+    zero5(gtmp);
+    adjgr::vjp_c2p(tmp, gc.sdet, gr_tol(L.P, 1e-8), L.P, gm, gtmp);
+    cget(A, kCarryGuM, c.cell, guM);
+    cget(A, kCarryGun, c.cell, gun);
+    const double f = st < 2 ? 0.5 : 1.0;
+    const double cf = st == 0 ? (1.0 / 6.0) : (1.0 / 6.0) * 2.0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        guM[k] = gun[k];
-        gk[k] = (1.0 / 6.0) * gun[k];
+        guM[k] += gtmp[k];
+        gk[k] = cf * gun[k] + f * gtmp[k];
     }
-    for (int q = 0; q < 4; ++q) zero5(gWn[q]);
-    for (int st = 2; st >= 0; --st) {
-        const int base = kTapeStage + kTapePerStage * st;
-        double gsrc[5], gum[5], ge1[5], gWm[5], gtmp[5];
-        {
-            SourceGeo sg;
-            source_geo(b, x, y, 0.0, h, sg);
-            double e1[5], um[5];
-            get(base + kTapeE1, e1);
-            gr_p2c(e1, gc, um, P);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gk[k];
-            zero5(gum);
-            adjgr::vjp_source(um, sg, P, gsrc, gum);
-            zero5(ge1);
-            adjgr::vjp_p2c(e1, gc, P, gum, ge1);
-        }
-        zero5(gWm);
-        const Geo gxm = geo_at(b, x - h, y, 0.0), gxp = geo_at(b, x + h, y, 0.0);
-        if (DIM == 1) {
-            double Wm[5], Wa[5], Wb[5];
-            get(base + kTapeWm, Wm);
-            get(kTapeWn, Wa);
-            get(kTapeWn + 5, Wb);
-            adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wa, Wm, Wb, gxm, gc, gxp, L.bias, L.limiter, P, ge1, gWn[0], gWm,
-                                 gWn[1]);
-        } else {
-            double ge3[5], ge2[5];
-            zero5(ge3);
-            zero5(ge2);
-            {
-                double e3[5], Wa[5], Wb[5];
-                get(base + kTapeE3, e3);
-                get(kTapeWn, Wa);
-                get(kTapeWn + 5, Wb);
-                adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wa, e3, Wb, gxm, gc, gxp, L.bias, L.limiter, P, ge1, gWn[0],
-                                     ge3, gWn[1]);
-            }
-            {
-                const Geo gym = geo_at(b, x, y - h, 0.0), gyp = geo_at(b, x, y + h, 0.0);
-                double e2[5], Wa[5], Wb[5];
-                get(base + kTapeE2, e2);
-                get(kTapeWn + 10, Wa);
-                get(kTapeWn + 15, Wb);
-                adjgr::vjp_frac_g<1>(L.frac_dt_h_y[st], Wa, e2, Wb, gym, gc, gyp, L.bias, L.limiter, P, ge3, gWn[2],
-                                     ge2, gWn[3]);
-            }
-            {
-                double Wm[5], Wa[5], Wb[5];
-                get(base + kTapeWm, Wm);
-                get(kTapeWn, Wa);
-                get(kTapeWn + 5, Wb);
-                adjgr::vjp_frac_g<0>(L.frac_dt_h[st], Wa, Wm, Wb, gxm, gc, gxp, L.bias, L.limiter, P, ge2, gWn[0],
-                                     gWm, gWn[1]);
-            }
-        }
-        {
-            double tmp[5];
-            get(base + kTapeTmp, tmp);
-            zero5(gtmp);
-            adjgr::vjp_c2p(tmp, gc.sdet, tol, P, gWm, gtmp);
-        }
-        // k_{st} feeds the stage's own state (by 1/2 or 1) and the final sum
-        const double f = st < 2 ? 0.5 : 1.0;
-        const double c = st == 0 ? (1.0 / 6.0) : (1.0 / 6.0) * 2.0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            guM[k] += gtmp[k];
-            gk[k] = c * gun[k] + f * gtmp[k];
-        }
-    }
+    cput(A, kCarryGuM, c.cell, guM);
+    cput(A, kCarryGk, c.cell, gk);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kGAT) void k_grr_tail(GRAdjLaunch A) {
+    const GRLaunch& L = A.L;
+    RevCell c;
+    if (!rev_cell<DIM>(A, c)) return;
+    const Hole& b = L.hole;
+    const double h = L.h;
     {
+        const Geo gc = geo_at(b, c.x, c.y, 0.0);
         SourceGeo sg;
-        source_geo(b, x, y, 0.0, h, sg);
-        double W[5], uM[5], gsrc[5];
-        ld5(L.in, cell, W);
-        gr_p2c(W, gc, uM, P);
+        source_geo(b, c.x, c.y, 0.0, h, sg);
+        double W[5], uM[5], gk[5], gsrc[5], guM[5], gW[5];
+        ld5(L.in, c.cell, W);
+        gr_p2c(W, gc, uM, L.P);
+        cget(A, kCarryGk, c.cell, gk);
+        cget(A, kCarryGuM, c.cell, guM);
 #pragma unroll
         for (int k = 0; k < 5; ++k) gsrc[k] = L.dt * gk[k];
-        adjgr::vjp_source(uM, sg, P, gsrc, guM);
-        double gW[5];
+        adjgr::vjp_source(uM, sg, L.P, gsrc, guM);
         zero5(gW);
-        adjgr::vjp_p2c(W, gc, P, guM, gW);
-        add5(A.lam_in, cell, gW);
+        adjgr::vjp_p2c(W, gc, L.P, guM, gW);
+        add5(A.lam_in, c.cell, gW);
     }
-    const int nn = DIM == 2 ? 4 : 2;
-    for (int q = 0; q < nn; ++q) {
+    const double tol = gr_tol(L.P, 1e-8);
+    for (int q = 0; q < 2 * DIM; ++q) {
+        const int i = c.i, j = c.j;
         const long nbq = q == 0 ? (long)i * L.pitch + clamp1_a(j - 1, L.nx)
                          : q == 1 ? (long)i * L.pitch + clamp1_a(j + 1, L.nx)
                          : q == 2 ? (long)clamp1_a(i - 1, L.ny) * L.pitch + j
                                   : (long)clamp1_a(i + 1, L.ny) * L.pitch + j;
-        const double px = q == 0 ? (j - 1) * h : q == 1 ? (j + 1) * h : x;
-        const double py = q == 2 ? (i - 1) * h : q == 3 ? (i + 1) * h : y;
-        const double sx = q == 0 ? x - h : q == 1 ? x + h : x;
-        const double sy = q == 2 ? y - h : q == 3 ? y + h : y;
-        double Wq[5], uq[5], gu[5], gw[5];
+        const double px = q == 0 ? (j - 1) * h : q == 1 ? (j + 1) * h : c.x;
+        const double py = q == 2 ? (i - 1) * h : q == 3 ? (i + 1) * h : c.y;
+        const double sx = q == 0 ? c.x - h : q == 1 ? c.x + h : c.x;
+        const double sy = q == 2 ? c.y - h : q == 3 ? c.y + h : c.y;
+        double Wq[5], uq[5], gq[5], gu[5], gw[5];
+        cget(A, kCarryGWn + 5 * q, c.cell, gq);
         ld5(L.in, nbq, Wq);
-        const Geo gq = geo_at(b, px, py, 0.0);
-        gr_p2c(Wq, gq, uq, P);
+        const Geo gp = geo_at(b, px, py, 0.0);
+        gr_p2c(Wq, gp, uq, L.P);
         zero5(gu);
         zero5(gw);
-        adjgr::vjp_c2p(uq, geo_at(b, sx, sy, 0.0).sdet, tol, P, gWn[q], gu);
-        adjgr::vjp_p2c(Wq, gq, P, gu, gw);
+        adjgr::vjp_c2p(uq, geo_at(b, sx, sy, 0.0).sdet, tol, L.P, gq, gu);
+        adjgr::vjp_p2c(Wq, gp, L.P, gu, gw);
         add5(A.lam_in, nbq, gw);
     }
 }
@@ -702,17 +616,7 @@ hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s
     const long cells = (long)L.nx * L.ny;
     const long ifaces = axis == 0 ? (long)(L.nx + 1) * L.ny : (long)(L.ny + 1) * L.nx;
     const dim3 blk(kGAT);
-    if (L.order == 2 && L.half) {
-        if (axis == 0) {
-            hipLaunchKernelGGL((k_gra_cells<2, 0>), gat_blocks(cells), blk, 0, s, A);
-            hipLaunchKernelGGL((k_gra_iface_force<0>), gat_blocks(ifaces), blk, 0, s, A);
-            hipLaunchKernelGGL((k_gra_iface_half<0>), gat_blocks(2 * ifaces), blk, 0, s, A);
-        } else {
-            hipLaunchKernelGGL((k_gra_cells<2, 1>), gat_blocks(cells), blk, 0, s, A);
-            hipLaunchKernelGGL((k_gra_iface_force<1>), gat_blocks(ifaces), blk, 0, s, A);
-            hipLaunchKernelGGL((k_gra_iface_half<1>), gat_blocks(2 * ifaces), blk, 0, s, A);
-        }
-    } else if (L.order == 2) {
+    if (L.order == 2) {
         if (axis == 0) {
             hipLaunchKernelGGL((k_gra_cells<2, 0>), gat_blocks(cells), blk, 0, s, A);
             hipLaunchKernelGGL((k_gra_ifaces<2, 0>), gat_blocks(ifaces), blk, 0, s, A);
@@ -735,8 +639,24 @@ hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s
 hipError_t launch_gr_adjoint_forcing(const GRAdjLaunch& A, hipStream_t s) {
     const long cells = (long)A.L.nx * A.L.ny;
     if (A.tape) {
-        if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing_rev<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
-        else hipLaunchKernelGGL((k_gra_forcing_rev<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);
+        const dim3 g = gat_blocks(cells), blk(kGAT);
+        if (A.L.dim == 1) {
+            hipLaunchKernelGGL((k_grr_head<1>), g, blk, 0, s, A);
+            for (int st = 2; st >= 0; --st) {
+                hipLaunchKernelGGL((k_grr_src<1>), g, blk, 0, s, A, st);
+                hipLaunchKernelGGL((k_grr_frac<1, 0, true>), g, blk, 0, s, A, st);
+            }
+            hipLaunchKernelGGL((k_grr_tail<1>), g, blk, 0, s, A);
+        } else {
+            hipLaunchKernelGGL((k_grr_head<2>), g, blk, 0, s, A);
+            for (int st = 2; st >= 0; --st) {
+                hipLaunchKernelGGL((k_grr_src<2>), g, blk, 0, s, A, st);
+                hipLaunchKernelGGL((k_grr_frac<2, 0, false>), g, blk, 0, s, A, st);
+                hipLaunchKernelGGL((k_grr_frac<2, 1, false>), g, blk, 0, s, A, st);
+                hipLaunchKernelGGL((k_grr_frac<2, 0, true>), g, blk, 0, s, A, st);
+            }
+            hipLaunchKernelGGL((k_grr_tail<2>), g, blk, 0, s, A);
+        }
     } else {
         if (A.L.dim == 1) hipLaunchKernelGGL((k_gra_forcing<1>), gat_blocks(cells), dim3(kGAT), 0, s, A);
         else hipLaunchKernelGGL((k_gra_forcing<2>), gat_blocks(cells), dim3(kGAT), 0, s, A);

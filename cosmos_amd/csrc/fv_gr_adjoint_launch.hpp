@@ -15,6 +15,7 @@ struct GRAdjLaunch {
     Planes gbuf;      // sweeps: per-cell adjoint of the updated conserved vector (state layout)
     double* tape;     // forcing: kTapePlanes planes of tape_plane doubles (state layout), or null
     long tape_plane;
+    double* carry;    // forcing with a tape: kCarryPlanes planes of tape_plane doubles
 };
 
 // Forcing tape, one plane per value: the final conserved vector un, the four
@@ -24,6 +25,13 @@ struct GRAdjLaunch {
 constexpr int kTapeUn = 0, kTapeWn = 5, kTapeStage = 25, kTapePerStage = 25;
 constexpr int kTapeTmp = 0, kTapeWm = 5, kTapeE2 = 10, kTapeE3 = 15, kTapeE1 = 20;
 constexpr int kTapePlanes = kTapeStage + 3 * kTapePerStage;
+
+// Running adjoints of the taped forcing reverse: of the output's conserved
+// vector (gun), of the current stage increment (gk), of the step-start
+// conserved state (guM, accumulated), of the fractional sweep output being
+// walked back (ge), and of the four neighbour states (gWn, accumulated).
+constexpr int kCarryGun = 0, kCarryGk = 5, kCarryGuM = 10, kCarryGe = 15, kCarryGWn = 20;
+constexpr int kCarryPlanes = 40;
 
 // Sweep: the flux planes must hold the forward sweep's interface fluxes.
 hipError_t launch_gr_adjoint_sweep(const GRAdjLaunch& A, int axis, hipStream_t s);

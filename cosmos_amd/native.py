@@ -107,6 +107,7 @@ SIGNATURES = {
     "cosmos_adjoint_forcing": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
     "cosmos_adjoint_set_subcycles": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_adjoint_set_tape": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_longlong]),
+    "cosmos_adjoint_release": (ctypes.c_int, [_handle]),
     "cosmos_adjoint_last_error": (ctypes.c_char_p, [_handle]),
     "cosmos_adjoint_solver": (_handle, [_handle]),
     "cosmos_adjoint_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,

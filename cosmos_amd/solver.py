@@ -318,6 +318,10 @@ class Adjoint:
         """Black hole: TAPE_NONE, TAPE_RERUN or TAPE_SEGMENT (cosmos_adjoint_set_tape)."""
         self._check(self._lib.cosmos_adjoint_set_tape(self._h, mode, reserve_bytes))
 
+    def release(self) -> None:
+        """Free the segment buffers and tapes (cosmos_adjoint_release)."""
+        self._check(self._lib.cosmos_adjoint_release(self._h))
+
     def forcing(self, cell_spacing: float, time_step: float, lam_out: np.ndarray) -> np.ndarray:
         """Advance the state by one RK4 forcing step; return (d out / d in)^T lam_out."""
         lo = self._arr(lam_out)

@@ -295,13 +295,17 @@ int cosmos_adjoint_sweep(cosmos_adjoint* a, int order, int axis, double cell_spa
  * condition with the 0.2x damping of the first six steps, no final time),
  * recording every dt and a device checkpoint of the state every
  * checkpoint_every steps.  The solver is set to COSMOS_MODE_EXACT: the
- * reverse pass recomputes segments with the exact kernels. */
+ * reverse pass recomputes segments with the exact kernels.  The last segment
+ * runs through the context's segment buffers (and, for a black hole, its
+ * tapes, cosmos_adjoint_set_tape), where the reverse pass uses it without
+ * recomputing it. */
 int cosmos_adjoint_forward(cosmos_adjoint* a, int order, double cell_spacing, double cfl, int n_steps, double bias,
                            int slope_limiter, int checkpoint_every);
 /* grad_in = (d W_N / d W_0)^T grad_out for the recorded trajectory: each
- * segment between checkpoints is recomputed forward, then traversed in
- * reverse one sweep at a time, in buffers of the context: the solver's
- * state is not touched and stays at W_N. */
+ * segment between checkpoints is recomputed forward (except the one the
+ * segment buffers still hold: after cosmos_adjoint_forward the last one),
+ * then traversed in reverse one operation at a time, in buffers of the
+ * context: the solver's state is not touched and stays at W_N. */
 int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* grad_in);
 /* The recorded dt schedule applied to another initial state (for finite-
  * difference checks); the solver's state is left at the result. */
@@ -328,6 +332,10 @@ int cosmos_adjoint_set_subcycles(cosmos_adjoint* a, int subcycles);
  * adjoint kernels accumulate with atomic adds) across the modes. */
 enum { COSMOS_TAPE_NONE = 0, COSMOS_TAPE_RERUN = 1, COSMOS_TAPE_SEGMENT = 2 };
 int cosmos_adjoint_set_tape(cosmos_adjoint* a, int mode, long long reserve_bytes);
+/* Free the segment buffers and tapes (the recorded trajectory stays): the
+ * next cosmos_adjoint_backward allocates them again and recomputes every
+ * segment. */
+int cosmos_adjoint_release(cosmos_adjoint* a);
 
 /* Batched point-wise limiter functions on host arrays (evaluated on the device):
  *   cosmos_slope_limiter: out[i] = SlopeLimiters::computeSlopeLimiter(steepness[i], bias, limiter)

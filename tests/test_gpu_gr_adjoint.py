@@ -175,3 +175,29 @@ def test_gr_trajectory_tape_modes():
     assert np.abs(grads[TAPE_NONE]).max() > 1e-6
     assert close(grads[TAPE_NONE], grads[TAPE_RERUN])
     assert close(grads[TAPE_NONE], grads[TAPE_SEGMENT])
+
+
+@pytest.mark.parametrize("steps,every", [(6, 2), (5, 5)])
+def test_gr_backward_repeats_and_recorded_segment(steps, every):
+    """The forward pass leaves the last segment (states and tapes) in the
+    segment buffers and the first backward uses it without recomputing it; a
+    second backward (which recomputes every segment the buffers no longer
+    hold) and a context that recomputes everything (TAPE_RERUN: the forward
+    recorded no tapes) give the same gradient."""
+    W0, h = bh_state(20, 41)
+    target = bh_state(20, 42)[0][..., 0]
+    grads = []
+    for mode in (TAPE_SEGMENT, TAPE_RERUN):
+        with make(20, excision=2.5) as a:
+            a.set_tape(mode, 0)
+            a.upload(W0)
+            a.forward(2, h, 0.4, steps, 0.0, 1, every)
+            WN = a.download()
+            assert np.array_equal(a.replay(W0), WN)
+            a.upload(WN)
+            gout = loss_and_grad(WN, target)[1]
+            grads.append(a.backward(gout))
+            grads.append(a.backward(gout))
+    assert np.abs(grads[0]).max() > 1e-6
+    for g in grads[1:]:
+        assert close(grads[0], g)
