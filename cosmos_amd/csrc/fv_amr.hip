@@ -237,6 +237,126 @@ __global__ __launch_bounds__(kAmrThreads) void k_inject_all(Grid c, Grid f, int 
         for (int b = 0; b < 2; ++b) store(f, dim == 1 ? 0 : 2 * i + a, 2 * j + b, W);
 }
 
+// Three-level AMR (EulerAMR::solve{,2D}Level2AMR, AMR/EulerAMR.cpp:593-1381): the
+// flag, exchange and derefinement passes between a parent level and its child
+// level (twice the cells along each axis), one thread per parent cell; each
+// thread writes only its own parent cell and children.
+
+__device__ __forceinline__ long child0(const Grid& c, int dim, int i, int j) {
+    return dim == 1 ? 2L * j : (long)(2 * i) * c.nx + 2 * j;
+}
+
+__device__ __forceinline__ bool children_all(const u8* cflags, const Grid& c, int dim, int i, int j) {
+    const long a = child0(c, dim, i, j);
+    if (dim == 1) return cflags[a] && cflags[a + 1];
+    return cflags[a] && cflags[a + 1] && cflags[a + c.nx] && cflags[a + c.nx + 1];
+}
+
+__device__ __forceinline__ void children_set(u8* cflags, const Grid& c, int dim, int i, int j, u8 v) {
+    const long a = child0(c, dim, i, j);
+    cflags[a] = v;
+    cflags[a + 1] = v;
+    if (dim == 2) {
+        cflags[a + c.nx] = v;
+        cflags[a + c.nx + 1] = v;
+    }
+}
+
+// Refinement flags of the parent from its own states (copy boundary, its two
+// or four neighbours): a jump >= tol over dx clears the parent flag and sets
+// its children's; only_flagged: parents whose flag is already clear are not
+// tested (coarse level :648-666, :982-1010; intermediate level with
+// tol * 2 * order :704-723, :1052-1082).  Level 2 reads the true bottom
+// neighbour for both physics.
+template <class Phys>
+__global__ __launch_bounds__(kAmrThreads) void k_l2_refine(Grid p, Grid c, int dim, u8* pflags, u8* cflags, double dx,
+                                                           double tol, int only_flagged, PhysParams P) {
+    long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
+    if (idx >= (long)p.nx * p.ny) return;
+    int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
+    if (only_flagged && !pflags[idx]) return;
+    double um[5], un[5];
+    cons_at<Phys>(p, i, j, P, um);
+    cons_at<Phys>(p, i, clampc(j - 1, p.nx), P, un);
+    bool hit = jump(un, um, dx, tol);
+    cons_at<Phys>(p, i, clampc(j + 1, p.nx), P, un);
+    hit |= jump(um, un, dx, tol);
+    if (dim == 2) {
+        cons_at<Phys>(p, clampc(i - 1, p.ny), j, P, un);
+        hit |= jump(un, um, dx, tol);
+        cons_at<Phys>(p, clampc(i + 1, p.ny), j, P, un);
+        hit |= jump(um, un, dx, tol);
+    }
+    if (hit) {
+        pflags[idx] = 0;
+        children_set(cflags, c, dim, i, j, 1);
+    }
+}
+
+// Restriction of refined parents and injection into unflagged children.
+// strict (intermediate <-> fine, :780-799, :1137-1172): restrict only where
+// every child is flagged, inject only under a flagged parent; otherwise
+// (coarse <-> intermediate, :856-875, :1255-1290) restrict every refined
+// parent and inject every unflagged child.
+template <class Phys>
+__global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, int dim, const u8* pflags,
+                                                             const u8* cflags, int strict, PhysParams P) {
+    long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
+    if (idx >= (long)p.nx * p.ny) return;
+    int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
+    double W[5];
+    if (!pflags[idx] && (!strict || children_all(cflags, c, dim, i, j))) {
+        restrict_children<Phys>(c, dim, i, j, P, W);
+        store(p, i, j, W);
+    } else {
+        load(p, i, j, W);
+    }
+    for (int a = 0; a < (dim == 1 ? 1 : 2); ++a)
+        for (int b = 0; b < 2; ++b) {
+            const int ci = dim == 1 ? 0 : 2 * i + a, cj = 2 * j + b;
+            if (!cflags[(long)ci * c.nx + cj] && (!strict || pflags[idx])) store(c, ci, cj, W);
+        }
+}
+
+// Derefinement: child (2i+1, 2j+1) (1D: 2j+1) against its neighbours (copy
+// boundary) over dx at tol; quiet parents with every child flagged take their
+// restriction back (fine at tol :802-834, :1175-1220; intermediate at
+// tol * 0.5 * (1 / order) :878-911, :1293-1337).
+template <class Phys>
+__global__ __launch_bounds__(kAmrThreads) void k_l2_derefine(Grid p, Grid c, int dim, u8* pflags, u8* cflags,
+                                                             double dx, double tol, PhysParams P) {
+    long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
+    if (idx >= (long)p.nx * p.ny) return;
+    int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
+    double um[5], un[5];
+    bool marked;
+    if (dim == 1) {
+        cons_at<Phys>(c, 0, 2 * j + 1, P, um);
+        cons_at<Phys>(c, 0, 2 * j, P, un);
+        marked = jump(un, um, dx, tol);
+        cons_at<Phys>(c, 0, clampc(2 * j + 2, c.nx), P, un);
+        marked |= jump(um, un, dx, tol);
+    } else {
+        const int fi = 2 * i + 1, fj = 2 * j + 1;
+        cons_at<Phys>(c, fi, fj, P, um);
+        cons_at<Phys>(c, fi - 1, fj, P, un);
+        marked = jump(un, um, dx, tol);
+        cons_at<Phys>(c, clampc(fi + 1, c.ny), fj, P, un);
+        marked |= jump(um, un, dx, tol);
+        cons_at<Phys>(c, fi, fj - 1, P, un);
+        marked |= jump(un, um, dx, tol);
+        cons_at<Phys>(c, fi, clampc(fj + 1, c.nx), P, un);
+        marked |= jump(um, un, dx, tol);
+    }
+    if (!marked && children_all(cflags, c, dim, i, j)) {
+        children_set(cflags, c, dim, i, j, 0);
+        pflags[idx] = 1;
+        double W[5];
+        restrict_children<Phys>(c, dim, i, j, P, W);
+        store(p, i, j, W);
+    }
+}
+
 dim3 blocks_of(long n) { return dim3((unsigned)((n + kAmrThreads - 1) / kAmrThreads)); }
 
 }  // namespace
@@ -248,8 +368,10 @@ struct cosmos_amr {
     int physics = 0, dim = 2, nx = 0, ny = 0;
     cosmos_solver* coarse = nullptr;
     cosmos_solver* fine = nullptr;
+    cosmos_solver* fine2 = nullptr;   // 1D three-level runs: the 4nx-cell level (lazily created)
     unsigned char* cflags = nullptr;  // device, [ny][nx]
     unsigned char* fflags = nullptr;  // device, [2ny][2nx] (1D: [2nx])
+    unsigned char* f2flags = nullptr; // device, [4nx]
     int* any = nullptr;               // device scratch flag
     cfv::PhysParams P{};
     std::string err;
@@ -285,8 +407,9 @@ int grid_of(cosmos_amr* a, cosmos_solver* s, Grid& g) {
     if (rc) return from_solver(a, s, rc);
     for (int k = 0; k < 5; ++k) g.p[k] = planes[k];
     g.pitch = pitch;
-    g.nx = s == a->coarse ? a->nx : 2 * a->nx;
-    g.ny = s == a->coarse ? a->ny : (a->dim == 1 ? 1 : 2 * a->ny);
+    const int m = s == a->coarse ? 1 : s == a->fine ? 2 : 4;
+    g.nx = m * a->nx;
+    g.ny = a->dim == 1 ? 1 : m * a->ny;
     return COSMOS_OK;
 }
 
@@ -298,6 +421,7 @@ long fine_cells(const cosmos_amr* a) { return (long)(2 * a->nx) * (a->dim == 1 ?
 // either level likewise start only after that stream is idle, so every
 // access to a level's state or flags is ordered.
 hipStream_t order_streams(cosmos_amr* a) {
+    if (a->fine2) cosmos_synchronize(a->fine2);
     cosmos_synchronize(a->fine);
     cosmos_synchronize(a->coarse);
     return (hipStream_t)cosmos_stream(a->coarse);
@@ -334,6 +458,18 @@ template <class Phys>
 struct DerefineK {
     static constexpr auto fn = cfv::k_amr_derefine<Phys>;
 };
+template <class Phys>
+struct L2RefineK {
+    static constexpr auto fn = cfv::k_l2_refine<Phys>;
+};
+template <class Phys>
+struct L2ExchangeK {
+    static constexpr auto fn = cfv::k_l2_exchange<Phys>;
+};
+template <class Phys>
+struct L2DerefineK {
+    static constexpr auto fn = cfv::k_l2_derefine<Phys>;
+};
 
 int amr_refine(cosmos_amr* a, double dx, double tol) {
     Grid c;
@@ -367,9 +503,16 @@ int amr_derefine(cosmos_amr* a, double dx, double tol) {
     return COSMOS_OK;
 }
 
+cosmos_solver* level_solver(cosmos_amr* a, int level) {
+    return level == 0 ? a->coarse : level == 1 ? a->fine : a->fine2;
+}
+unsigned char* level_flags(cosmos_amr* a, int level) {
+    return level == 0 ? a->cflags : level == 1 ? a->fflags : a->f2flags;
+}
+
 // One sweep of a level restricted to that level's flags.
 int amr_sweep(cosmos_amr* a, int level, int order, int axis, double dx, double dt, double bias, int lim) {
-    cosmos_solver* s = level == 0 ? a->coarse : a->fine;
+    cosmos_solver* s = level_solver(a, level);
     order_streams(a);
     Grid before, after;
     int rc = grid_of(a, s, before);
@@ -381,7 +524,7 @@ int amr_sweep(cosmos_amr* a, int level, int order, int axis, double dx, double d
     hipStream_t st = (hipStream_t)cosmos_stream(s);
     long n = (long)after.nx * after.ny;
     hipLaunchKernelGGL(cfv::k_mask_merge, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, st, before, after,
-                       (const unsigned char*)(level == 0 ? a->cflags : a->fflags));
+                       (const unsigned char*)level_flags(a, level));
     AMRCHK(a, hipGetLastError());
     return COSMOS_OK;
 }
@@ -396,16 +539,52 @@ int amr_flux_step(cosmos_amr* a, int level, int order, double dx, double dt) {
     return rc;
 }
 
-int amr_any_refined(cosmos_amr* a, bool& out) {
+int amr_any_set(cosmos_amr* a, const unsigned char* flags, long n, bool& out) {
     hipStream_t s = order_streams(a);
     int zero = 0, v = 0;
     AMRCHK(a, hipMemcpyAsync(a->any, &zero, sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(cfv::k_any, cfv::blocks_of(fine_cells(a)), dim3(cfv::kAmrThreads), 0, s,
-                       (const unsigned char*)a->fflags, fine_cells(a), a->any);
+    hipLaunchKernelGGL(cfv::k_any, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, s, flags, n, a->any);
     AMRCHK(a, hipGetLastError());
     AMRCHK(a, hipMemcpyAsync(&v, a->any, sizeof(int), hipMemcpyDeviceToHost, s));
     AMRCHK(a, hipStreamSynchronize(s));
     out = v != 0;
+    return COSMOS_OK;
+}
+
+int amr_any_refined(cosmos_amr* a, bool& out) { return amr_any_set(a, a->fflags, fine_cells(a), out); }
+
+// The 1D parent / child passes of the three-level driver (parent level lp).
+int l2_refine(cosmos_amr* a, int lp, double dx, double tol, int only_flagged) {
+    Grid p, c;
+    int rc = grid_of(a, level_solver(a, lp), p);
+    if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
+    if (rc) return rc;
+    hipStream_t s = order_streams(a);
+    AMRCHK(a, (launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c, a->dim,
+                                         level_flags(a, lp), level_flags(a, lp + 1), dx, tol, only_flagged, a->P)));
+    return COSMOS_OK;
+}
+
+int l2_exchange(cosmos_amr* a, int lp, int strict) {
+    Grid p, c;
+    int rc = grid_of(a, level_solver(a, lp), p);
+    if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
+    if (rc) return rc;
+    hipStream_t s = order_streams(a);
+    AMRCHK(a, (launch_physics<L2ExchangeK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c, a->dim,
+                                           (const unsigned char*)level_flags(a, lp),
+                                           (const unsigned char*)level_flags(a, lp + 1), strict, a->P)));
+    return COSMOS_OK;
+}
+
+int l2_derefine(cosmos_amr* a, int lp, double dx, double tol) {
+    Grid p, c;
+    int rc = grid_of(a, level_solver(a, lp), p);
+    if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
+    if (rc) return rc;
+    hipStream_t s = order_streams(a);
+    AMRCHK(a, (launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c, a->dim,
+                                           level_flags(a, lp), level_flags(a, lp + 1), dx, tol, a->P)));
     return COSMOS_OK;
 }
 
@@ -448,11 +627,14 @@ int cosmos_amr_destroy(cosmos_amr* a) {
     if (!a) return COSMOS_OK;
     if (a->coarse) cosmos_synchronize(a->coarse);
     if (a->fine) cosmos_synchronize(a->fine);
+    if (a->fine2) cosmos_synchronize(a->fine2);
     hipFree(a->cflags);
     hipFree(a->fflags);
+    hipFree(a->f2flags);
     hipFree(a->any);
     cosmos_destroy(a->coarse);
     cosmos_destroy(a->fine);
+    cosmos_destroy(a->fine2);
     delete a;
     return COSMOS_OK;
 }
@@ -465,6 +647,8 @@ int cosmos_amr_set_con2prim(cosmos_amr* a, double tolerance, int max_iterations)
         if (rc) return from_solver(a, a->coarse, rc);
         rc = cosmos_set_con2prim(a->fine, tolerance, max_iterations);
         if (rc) return from_solver(a, a->fine, rc);
+        if (a->fine2 && (rc = cosmos_set_con2prim(a->fine2, tolerance, max_iterations)))
+            return from_solver(a, a->fine2, rc);
         a->P.c2p_tol = tolerance;
         a->P.c2p_maxit = max_iterations;
         return COSMOS_OK;
@@ -567,6 +751,121 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
         order_streams(a);
         if (time_out) *time_out = t;
         if (steps_out) *steps_out = it;
+        return COSMOS_OK;
+    });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
+                            double tolerance, int max_coarse_steps, double* time_out, int* steps_out) {
+    return amr_guarded(a, [&] {
+        if (order != 1 && order != 2) return amr_fail(a, COSMOS_ERR_ARGUMENT, "order must be 1 or 2");
+        int rc = COSMOS_OK;
+        const long nf2 = 4L * a->nx * (a->dim == 1 ? 1 : 4L * a->ny);
+        if (!a->fine2) {
+            rc = cosmos_create(&a->fine2, a->physics, a->dim, 4 * a->nx, a->dim == 1 ? 1 : 4 * a->ny, a->P.gamma);
+            if (rc) return amr_fail(a, rc, std::string("AMR setup: ") + cosmos_last_error(nullptr));
+            if (a->P.c2p_tol != cfv::make_params(a->P.gamma).c2p_tol ||
+                a->P.c2p_maxit != cfv::make_params(a->P.gamma).c2p_maxit) {
+                rc = cosmos_set_con2prim(a->fine2, a->P.c2p_tol, a->P.c2p_maxit);
+                if (rc) return from_solver(a, a->fine2, rc);
+            }
+            AMRCHK(a, hipMalloc((void**)&a->f2flags, (size_t)nf2));
+        }
+        // every level starts from the coarse state: intermediate and fine cells
+        // copies of their coarse cell, coarse flags set, the others clear (:597-636)
+        Grid c, m, f;
+        rc = grid_of(a, a->coarse, c);
+        if (!rc) rc = grid_of(a, a->fine, m);
+        if (!rc) rc = grid_of(a, a->fine2, f);
+        if (rc) return rc;
+        {
+            hipStream_t s = order_streams(a);
+            const long n = coarse_cells(a), nm = fine_cells(a);
+            hipLaunchKernelGGL(cfv::k_inject_all, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, s, c, m, a->dim);
+            hipLaunchKernelGGL(cfv::k_inject_all, cfv::blocks_of(nm), dim3(cfv::kAmrThreads), 0, s, m, f, a->dim);
+            hipLaunchKernelGGL(cfv::k_fill, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, s, a->cflags, n,
+                               (unsigned char)1);
+            hipLaunchKernelGGL(cfv::k_fill, cfv::blocks_of(nm), dim3(cfv::kAmrThreads), 0, s, a->fflags, nm,
+                               (unsigned char)0);
+            hipLaunchKernelGGL(cfv::k_fill, cfv::blocks_of(nf2), dim3(cfv::kAmrThreads), 0, s, a->f2flags, nf2,
+                               (unsigned char)0);
+            AMRCHK(a, hipGetLastError());
+        }
+        const double itol = (tolerance * 2.0) * order;          // AMRTolerance * 2.0 * order
+        const double dtol = (tolerance * 0.5) * (1.0 / order);  // AMRTolerance * 0.5 * (1.0 / order)
+        const int cap = max_coarse_steps < 0 ? INT_MAX : max_coarse_steps;
+        double t = 0.0;
+        int it = 0, fine_total = 0;
+        while (t < final_time && it < cap) {
+            double cdt = 0.0;
+            rc = cosmos_stable_time_step(a->coarse, cell_spacing, cfl, t, final_time, fine_total, &cdt);
+            if (rc) return from_solver(a, a->coarse, rc);
+            if ((rc = l2_refine(a, 0, cell_spacing, tolerance, 0))) return rc;
+            if ((rc = amr_flux_step(a, 0, order, cell_spacing, cdt))) return rc;
+            bool refined = false;
+            if ((rc = amr_any_set(a, a->fflags, fine_cells(a), refined))) return rc;
+            if (refined) {
+                // intermediate sub-steps to the coarse time, less 1e-4
+                double mt = t;
+                while (mt < ((t + cdt) - 1e-4)) {
+                    double mdt = 0.0;
+                    rc = cosmos_stable_time_step(a->fine, cell_spacing * 0.5, cfl, mt, t + cdt, fine_total, &mdt);
+                    if (rc) return from_solver(a, a->fine, rc);
+                    if ((rc = l2_refine(a, 1, cell_spacing, itol, 1))) return rc;
+                    if ((rc = amr_flux_step(a, 1, order, cell_spacing * 0.5, mdt))) return rc;
+                    bool refined2 = false;
+                    if ((rc = amr_any_set(a, a->f2flags, nf2, refined2))) return rc;
+                    if (refined2) {
+                        double ft = mt;
+                        while (ft < ((mt + mdt) - 1e-4)) {
+                            double fdt = 0.0;
+                            rc = cosmos_stable_time_step(a->fine2, cell_spacing * 0.25, cfl, ft, mt + mdt, fine_total,
+                                                         &fdt);
+                            if (rc) return from_solver(a, a->fine2, rc);
+                            if ((rc = amr_flux_step(a, 2, order, cell_spacing * 0.25, fdt))) return rc;
+                            ft += fdt;
+                            fine_total += 1;
+                        }
+                    } else {
+                        fine_total += 1;
+                    }
+                    mt += mdt;
+                    fine_total += 1;
+                    if ((rc = l2_exchange(a, 1, 1))) return rc;
+                    if ((rc = l2_derefine(a, 1, cell_spacing, tolerance))) return rc;
+                }
+            } else {
+                fine_total += 1;
+            }
+            t += cdt;
+            it += 1;
+            if ((rc = l2_exchange(a, 0, 0))) return rc;
+            if ((rc = l2_derefine(a, 0, cell_spacing, dtol))) return rc;
+        }
+        order_streams(a);
+        if (time_out) *time_out = t;
+        if (steps_out) *steps_out = it;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_amr_download_level2(cosmos_amr* a, double* coarse, double* intermediate, double* fine,
+                               unsigned char* coarse_flags, unsigned char* intermediate_flags,
+                               unsigned char* fine_flags) {
+    return amr_guarded(a, [&] {
+        if (!a->fine2) return amr_fail(a, COSMOS_ERR_STATE, "no three-level run (cosmos_amr_solve_level2)");
+        int rc = cosmos_amr_download(a, coarse, intermediate, coarse_flags, intermediate_flags);
+        if (rc) return rc;
+        rc = fine ? cosmos_download(a->fine2, fine) : COSMOS_OK;
+        if (rc) return from_solver(a, a->fine2, rc);
+        hipStream_t s = order_streams(a);
+        const size_t nf2 = (size_t)(4 * a->nx) * (a->dim == 1 ? 1 : 4 * a->ny);
+        if (fine_flags) AMRCHK(a, hipMemcpyAsync(fine_flags, a->f2flags, nf2, hipMemcpyDeviceToHost, s));
+        AMRCHK(a, hipStreamSynchronize(s));
         return COSMOS_OK;
     });
 }

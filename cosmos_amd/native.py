@@ -101,6 +101,11 @@ SIGNATURES = {
     "cosmos_amr_solve_level1": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,
                                                 _c_int_p]),
+    "cosmos_amr_solve_level2": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,
+                                                _c_int_p]),
+    "cosmos_amr_download_level2": (ctypes.c_int, [_handle, _c_double_p, _c_double_p, _c_double_p, _c_u8_p, _c_u8_p,
+                                                   _c_u8_p]),
     "cosmos_adjoint_create": (ctypes.c_int, [ctypes.POINTER(_handle), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                               ctypes.c_int, ctypes.c_double]),
     "cosmos_adjoint_destroy": (ctypes.c_int, [_handle]),

@@ -279,6 +279,24 @@ class Amr:
                                                       max_coarse_steps, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
 
+    def solve_level2(self, order: int, cell_spacing: float, cfl: float, final_time: float, tolerance: float,
+                     max_coarse_steps: int = -1):
+        """Three-level run from the uploaded coarse state; returns (time reached, coarse steps taken)."""
+        t, n = ctypes.c_double(), ctypes.c_int()
+        self._check(self._lib.cosmos_amr_solve_level2(self._h, order, cell_spacing, cfl, final_time, tolerance,
+                                                      max_coarse_steps, ctypes.byref(t), ctypes.byref(n)))
+        return t.value, n.value
+
+    def download_level2(self):
+        """(coarse, intermediate, fine, coarse_flags, intermediate_flags, fine_flags) of a three-level run."""
+        shape = (lambda k: (k * self.nx, 5)) if self.dim == 1 else (lambda k: (k * self.ny, k * self.nx, 5))
+        c, m, f = np.empty(shape(1)), np.empty(shape(2)), np.empty(shape(4))
+        cf, mf, ff = (np.empty(shape(k)[:-1], dtype=np.uint8) for k in (1, 2, 4))
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        self._check(self._lib.cosmos_amr_download_level2(self._h, _dp(c), _dp(m), _dp(f), cf.ctypes.data_as(u8),
+                                                         mf.ctypes.data_as(u8), ff.ctypes.data_as(u8)))
+        return c, m, f, cf, mf, ff
+
 
 class Adjoint:
     """Reverse mode of the Newtonian or black-hole update (cosmos_adjoint_*):

@@ -266,6 +266,24 @@ int cosmos_amr_derefine(cosmos_amr* a, double cell_spacing, double tolerance);
  * max_coarse_steps < 0 means no cap. */
 int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
+/* EulerAMR::solve{,2D}Level2AMR and the special-relativistic copies
+ * (AMR/EulerAMR.cpp:593-1381, AMR/RelativisticEulerAMR.cpp:605-1394): three
+ * levels of nx x ny, 2nx x 2ny and 4nx x 4ny cells (1D: nx, 2nx, 4nx) started
+ * from the uploaded coarse state (finer levels copies of it, only the coarse
+ * flags set; the 2D tests read the true bottom neighbour); per coarse step the
+ * coarse flags and masked step, intermediate sub-steps (flags of flagged cells
+ * at tolerance * 2 * order, masked step, fine sub-steps, restriction /
+ * injection, fine derefinement) to the coarse time less 1e-4, then coarse
+ * restriction / injection and intermediate derefinement at
+ * tolerance * 0.5 * (1 / order); every test over the coarse spacing, every dt
+ * damped by the reference's fine-step counter. */
+int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
+                            double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
+/* The three levels after cosmos_amr_solve_level2 (any pointer may be null),
+ * row-major, flags as 0/1 bytes. */
+int cosmos_amr_download_level2(cosmos_amr* a, double* coarse, double* intermediate, double* fine,
+                               unsigned char* coarse_flags, unsigned char* intermediate_flags,
+                               unsigned char* fine_flags);
 
 /* ------------------------------------------------------------------------
  * Adjoint (reverse mode) of the Newtonian and black-hole updates (no

@@ -75,6 +75,13 @@ class CpuPath:
             fn.restype = res
             fn.argtypes = args
         u8 = ctypes.POINTER(ctypes.c_ubyte)
+        if prefix == "ref_" and hasattr(self.lib, "ref_amr_level2"):
+            self.lib.ref_amr_level2.restype = _I
+            self.lib.ref_amr_level2.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _P, _P, _P, u8, u8, u8]
+        if prefix == "orc_":
+            self.lib.orc_amr_level2.restype = _I
+            self.lib.orc_amr_level2.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P, _P, _P, u8, u8,
+                                                u8, ctypes.POINTER(_D)]
         if prefix == "ref_" and hasattr(self.lib, "ref_amr_level1"):
             self.lib.ref_amr_level1.restype = _I
             self.lib.ref_amr_level1.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _P, _P, u8, u8]
@@ -225,6 +232,24 @@ class CpuPath:
         n = self.lib.orc_amr_level1(*args, max_steps, _dp(coarse), _dp(fine), cf.ctypes.data_as(u8),
                                     ff.ctypes.data_as(u8), ctypes.byref(t))
         return coarse, fine, cf, ff, t.value, n
+
+    def amr_level2(self, W, physics, order, dx, cfl, tf, tol, gamma=1.4, max_steps=-1):
+        """(coarse, inter, fine, coarse_flags, inter_flags, fine_flags[, t, steps]) of solve{,2D}Level2AMR."""
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        shape = (lambda m: (m * nx, 5)) if dim == 1 else (lambda m: (m * ny, m * nx, 5))
+        coarse, inter, fine = np.empty_like(a), np.empty(shape(2)), np.empty(shape(4))
+        cf, jf, ff = (np.empty(shape(m)[:-1], dtype=np.uint8) for m in (1, 2, 4))
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        args = [physics, dim, order, _dp(a), nx, ny, dx, cfl, tf, tol, gamma]
+        outs = [_dp(coarse), _dp(inter), _dp(fine), cf.ctypes.data_as(u8), jf.ctypes.data_as(u8),
+                ff.ctypes.data_as(u8)]
+        if self.prefix == "ref_":
+            self.lib.ref_amr_level2(*args, *outs)
+            return coarse, inter, fine, cf, jf, ff
+        t = _D()
+        k = self.lib.orc_amr_level2(*args, max_steps, *outs, ctypes.byref(t))
+        return coarse, inter, fine, cf, jf, ff, t.value, k
 
     def amr_refine_flags(self, W, physics, dx, tol, cflags, fflags, gamma=1.4):
         """One sticky flag pass; returns updated copies of the flag arrays."""

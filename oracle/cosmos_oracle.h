@@ -59,6 +59,13 @@ int orc_amr_level1(int physics, int dim, int order, const double* prim, int nx, 
                    double tf, double tol, double gamma, int max_coarse_steps, double* coarse, double* fine,
                    unsigned char* cflags, unsigned char* fflags, double* t_out);
 
+/* Three-level AMR (EulerAMR::solve{,2D}Level2AMR and the special-relativistic copies):
+ * coarse nx x ny, intermediate 2nx x 2ny, fine 4nx x 4ny (1D: nx, 2nx, 4nx), flags as
+ * 0/1 bytes.  Returns the coarse steps taken. */
+int orc_amr_level2(int physics, int dim, int order, const double* prim, int nx, int ny, double dx, double cfl,
+                   double tf, double tol, double gamma, int max_coarse_steps, double* coarse, double* inter,
+                   double* fine, unsigned char* cflags, unsigned char* iflags, unsigned char* fflags, double* t_out);
+
 /* One sticky refinement-flag pass over a coarse state (flags in/out). */
 void orc_amr_refine_flags(int physics, int dim, const double* coarse, int nx, int ny, double dx, double tol,
                           double gamma, unsigned char* cflags, unsigned char* fflags);

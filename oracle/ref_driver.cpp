@@ -543,3 +543,59 @@ extern "C" int ref_amr_level1(int physics, int dim, int order, const double* pri
                                                                        gamma, coarse, fine, cflags, fflags);
     return 0;
 }
+
+// AMR level 2 (AMR/EulerAMR.cpp:593-1381, AMR/RelativisticEulerAMR.cpp:605-1394): the
+// reference's own solve{,2D}Level2AMR; coarse [ny][nx], intermediate [2ny][2nx],
+// fine [4ny][4nx] (1D: n, 2n, 4n cells).
+namespace {
+template <class V>
+void flags_out(const std::vector<V>& f, unsigned char* out) {
+    for (size_t i = 0; i < f.size(); ++i) out[i] = f[i] ? 1 : 0;
+}
+template <class V>
+void flags_out(const std::vector<std::vector<V> >& f, unsigned char* out) {
+    size_t k = 0;
+    for (const auto& row : f)
+        for (bool v : row) out[k++] = v ? 1 : 0;
+}
+
+template <class AMR, class S>
+void amr_level2(int dim, int order, const double* prim, int nx, int ny, double dx, double cfl, double tf, double tol,
+                double gamma, double* coarse, double* inter, double* fine, unsigned char* cflags,
+                unsigned char* iflags, unsigned char* fflags) {
+    QuietCout quiet;
+    EulerMaterialParameters mp(gamma);
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        auto r = AMR::solveLevel2AMR(c, dx, cfl, tf, tol, order, mp);
+        from_1d(std::get<0>(r), coarse);
+        from_1d(std::get<1>(r), inter);
+        from_1d(std::get<2>(r), fine);
+        flags_out(std::get<3>(r), cflags);
+        flags_out(std::get<4>(r), iflags);
+        flags_out(std::get<5>(r), fflags);
+        return;
+    }
+    std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+    auto r = AMR::solve2DLevel2AMR(c, dx, cfl, tf, tol, order, mp);
+    from_2d(std::get<0>(r), coarse);
+    from_2d(std::get<1>(r), inter);
+    from_2d(std::get<2>(r), fine);
+    flags_out(std::get<3>(r), cflags);
+    flags_out(std::get<4>(r), iflags);
+    flags_out(std::get<5>(r), fflags);
+}
+}  // namespace
+
+extern "C" int ref_amr_level2(int physics, int dim, int order, const double* prim, int nx, int ny, double dx,
+                              double cfl, double tf, double tol, double gamma, double* coarse, double* inter,
+                              double* fine, unsigned char* cflags, unsigned char* iflags, unsigned char* fflags) {
+    if (physics == 0)
+        amr_level2<EulerAMR, EulerStateVector>(dim, order, prim, nx, ny, dx, cfl, tf, tol, gamma, coarse, inter, fine,
+                                               cflags, iflags, fflags);
+    else
+        amr_level2<RelativisticEulerAMR, RelativisticEulerStateVector>(dim, order, prim, nx, ny, dx, cfl, tf, tol,
+                                                                       gamma, coarse, inter, fine, cflags, iflags,
+                                                                       fflags);
+    return 0;
+}

@@ -4,6 +4,7 @@ sources, compiled by oracle/Makefile into oracle/_ref/libcosmos_ref.so.
 Run in a container that has the reference checked out (it is not needed
 afterwards; the fixtures are committed):
     make -C oracle ref && python tests/golden/make_golden.py
+    (python tests/golden/make_golden.py amr_level2 writes only amr_level2.npz)
 
 Every fixture stores inputs and expected outputs as raw float64 arrays.
 """
@@ -38,10 +39,42 @@ def amr_cases():
             ("sr2_slic", 1, 2, spiky, 3.0, 5.0 / 3.0)]
 
 
+def amr_level2_cases():
+    """(tag, physics, order, input, tolerance, gamma, final time) of the three-level AMR fixtures."""
+    from cosmos_amd import scenarios
+
+    spiky = workloads.quadrants_2d(10, seed=6, ny=8)
+    spiky[..., 1:3] *= 0.5
+    return [("n_slic", 0, 2, scenarios.toro1(64).W, 1.0, 1.4, 0.05),
+            ("n_force", 0, 1, workloads.sod_1d(48), 0.5, 1.4, 0.08),
+            ("sr_slic", 1, 2, scenarios.sr_blast(64).W, 1.0, 5.0 / 3.0, 0.1),
+            ("sr_force", 1, 1, workloads.sr_riemann_1d(40, seed=3), 2.0, 5.0 / 3.0, 0.1),
+            ("n2_slic", 0, 2, workloads.quadrants_2d(12, seed=1, ny=10), 3.0, 1.4, 0.02),
+            ("n2_force", 0, 1, scenarios.toro1_2d(16).W, 1.0, 1.4, 0.02),
+            ("sr2_slic", 1, 2, spiky, 3.0, 5.0 / 3.0, 0.02)]
+
+
+def amr_level2_fixtures(ref) -> None:
+    """Three-level AMR: the reference's solve{,2D}Level2AMR (Newtonian and special
+    relativistic, FORCE and SLIC, 1D and 2D): all three levels and their flags."""
+    fx = {}
+    for tag, physics, order, W, tol, gamma, tf in amr_level2_cases():
+        n = W.shape[-2]
+        out = ref.amr_level2(W, physics, order, 1.0 / n, 0.8, tf, tol, gamma)
+        fx[f"{tag}_input"] = W
+        fx[f"{tag}_meta"] = np.array([physics, order, tol, gamma, tf])
+        for name, v in zip(("coarse", "inter", "fine", "cflags", "iflags", "fflags"), out):
+            fx[f"{tag}_{name}"] = v
+    np.savez_compressed(os.path.join(OUT, "amr_level2.npz"), **fx)
+
+
 def main() -> None:
     ref = load_ref()
     if ref is None:
         raise SystemExit("oracle/_ref/libcosmos_ref.so is not built (make -C oracle ref)")
+    if sys.argv[1:] == ["amr_level2"]:
+        amr_level2_fixtures(ref)
+        return
 
     # 1. Toro test 1 (Sod), 400 cells, SLIC + van Leer, CFL 0.8, to t = 0.25 through the
     #    reference's own EulerSecondOrderSolver::solve, and the same with FORCE.
@@ -163,6 +196,7 @@ def main() -> None:
         fx.update({f"{tag}_input": W, f"{tag}_coarse": c, f"{tag}_fine": f, f"{tag}_cflags": cf,
                    f"{tag}_fflags": ff, f"{tag}_meta": np.array([physics, order, tol, gamma])})
     np.savez_compressed(os.path.join(OUT, "amr.npz"), **fx)
+    amr_level2_fixtures(ref)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -87,3 +87,42 @@ def test_amr_2d_larger_newtonian(orc):
     for k in range(4):
         assert same(got[k], exp[k]), k
     assert (t, n) == (exp[4], exp[5]) and n == 6
+
+
+# ---- three-level AMR ----
+
+from test_oracle_amr import LEVEL2_NAMES, level2_cases  # noqa: E402
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_amr_level2_bitwise(orc, case):
+    tag, physics, order, W, tol, g, tf = level2_cases()[case]
+    n = W.shape[-2]
+    exp = orc.amr_level2(W, physics, order, 1.0 / n, 0.8, tf, tol, g)
+    dim = 1 if W.ndim == 2 else 2
+    with Amr(physics, dim, n, W.shape[0] if dim == 2 else 1, g) as a:
+        a.upload(W)
+        t, k = a.solve_level2(order, 1.0 / n, 0.8, tf, tol)
+        got = a.download_level2()
+        # a second run from the coarse state it left restarts every level from it
+        a.upload(W)
+        a.solve_level2(order, 1.0 / n, 0.8, tf, tol)
+        again = a.download_level2()
+    for k2, name in enumerate(LEVEL2_NAMES):
+        assert same(got[k2], exp[k2]), (tag, name)
+        assert same(again[k2], exp[k2]), (tag, name)
+    assert (t, k) == (exp[6], exp[7])
+
+
+def test_amr_level2_matches_golden():
+    g = golden("amr_level2.npz")
+    for tag, *_ in level2_cases():
+        physics, order, tol, gamma, tf = g[f"{tag}_meta"]
+        W = g[f"{tag}_input"]
+        dim = 1 if W.ndim == 2 else 2
+        with Amr(int(physics), dim, W.shape[-2], W.shape[0] if dim == 2 else 1, gamma) as a:
+            a.upload(W)
+            a.solve_level2(int(order), 1.0 / W.shape[-2], 0.8, tf, tol)
+            got = a.download_level2()
+        for k, name in enumerate(LEVEL2_NAMES):
+            assert same(got[k], g[f"{tag}_{name}"]), (tag, name)

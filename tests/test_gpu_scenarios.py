@@ -5,7 +5,8 @@ Toro 1-3 in 1D and 2D, the special-relativistic shock, blast and perturbed
 density in 1D and 2D, and black-hole accretion (spherical, spinning,
 spheroidal) in 1D and 2D, at both orders.  Grids are reduced so that the
 oracle finishes in seconds; the Newtonian and special-relativistic runs go to
-the drivers' own final times, the black-hole ones stop after a few steps."""
+the drivers' own final times, the black-hole ones stop after a few steps (or
+at the final time, whichever comes first)."""
 import math
 
 import numpy as np
@@ -54,5 +55,5 @@ def test_blackhole_scenario_matches_oracle(orc, make, order):
         out = s.download()
     exp, te, ke = orc.gr_run(sc.W, order, sc.cell_spacing, sc.cfl, sc.hole, tf=sc.final_time, max_steps=steps,
                              subcycles=1, gamma=sc.gamma)
-    assert (k, t) == (ke, te) and k == steps
+    assert (k, t) == (ke, te) and (k == steps or t == sc.final_time) and k > 3
     assert same(out, exp), sc.name

@@ -128,3 +128,58 @@ def test_uniform_state_never_refines(orc):
     c, f, cf, ff, t, n = orc.amr_level1(W, 0, 2, 1.0 / 8, 0.8, 0.05, 0.5, G_N)
     assert cf.all() and not ff.any()
     assert same(c, W) and same(f, np.kron(W, np.ones((2, 2, 1))))
+
+
+# ---- three-level AMR (EulerAMR::solve{,2D}Level2AMR and the special-relativistic copies) ----
+
+def level2_cases():
+    import sys, os
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    from make_golden import amr_level2_cases
+
+    return amr_level2_cases()
+
+
+LEVEL2_NAMES = ("coarse", "inter", "fine", "cflags", "iflags", "fflags")
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_level2_oracle_matches_reference(orc, ref, case):
+    tag, physics, order, W, tol, g, tf = level2_cases()[case]
+    n = W.shape[-2]
+    exp = ref.amr_level2(W, physics, order, 1.0 / n, 0.8, tf, tol, g)
+    got = orc.amr_level2(W, physics, order, 1.0 / n, 0.8, tf, tol, g)
+    for k, name in enumerate(LEVEL2_NAMES):
+        assert same(got[k], exp[k]), (tag, name)
+    assert got[5].sum() > 0 and got[4].sum() > 0  # both finer levels were used
+
+
+def test_level2_oracle_matches_golden(orc):
+    g = golden("amr_level2.npz")
+    for tag, *_ in level2_cases():
+        physics, order, tol, gamma, tf = g[f"{tag}_meta"]
+        W = g[f"{tag}_input"]
+        got = orc.amr_level2(W, int(physics), int(order), 1.0 / W.shape[-2], 0.8, tf, tol, gamma)
+        for k, name in enumerate(LEVEL2_NAMES):
+            assert same(got[k], g[f"{tag}_{name}"]), (tag, name)
+
+
+@pytest.mark.parametrize("physics,g", [(0, G_N), (1, G_SR)])
+def test_level2_without_refinement_is_the_plain_solve(orc, physics, g):
+    """A tolerance no jump reaches: no flag is ever set below the coarse level,
+    every coarse step advances the step counter once, so the coarse level is the
+    plain solve loop, the intermediate level its copy (injected after every
+    coarse step) and the fine level the initial state (it is only ever written
+    in intermediate sub-steps)."""
+    W = workloads.sod_1d(50) if physics == 0 else workloads.sr_riemann_1d(50, seed=4)
+    c, m, f, cf, mf, ff, t, k = orc.amr_level2(W, physics, 2, 1.0 / 50, 0.8, 0.05, 1e30, g)
+    exp, te, ke = orc.run(W, physics, 2, 1.0 / 50, 0.8, tf=0.05, gamma=g)
+    assert same(c, exp) and (t, k) == (te, ke)
+    assert cf.all() and not mf.any() and not ff.any()
+    assert same(m, np.repeat(c, 2, axis=0)) and same(f, np.repeat(W, 4, axis=0))
+
+
+def test_level2_uniform_state_never_refines(orc):
+    W = np.tile([0.7, 0.1, 0.0, 0.0, 0.9], (30, 1))
+    c, m, f, cf, mf, ff, t, k = orc.amr_level2(W, 0, 2, 1.0 / 30, 0.8, 0.1, 1e-9, G_N)
+    assert k > 3 and cf.all() and not mf.any() and not ff.any()
