@@ -694,6 +694,18 @@ int cosmos_amr_download(cosmos_amr* a, double* coarse, double* fine, unsigned ch
     });
 }
 
+int cosmos_amr_set_flags(cosmos_amr* a, const unsigned char* coarse_flags, const unsigned char* fine_flags) {
+    return amr_guarded(a, [&] {
+        hipStream_t s = order_streams(a);
+        if (coarse_flags)
+            AMRCHK(a, hipMemcpyAsync(a->cflags, coarse_flags, (size_t)coarse_cells(a), hipMemcpyHostToDevice, s));
+        if (fine_flags)
+            AMRCHK(a, hipMemcpyAsync(a->fflags, fine_flags, (size_t)fine_cells(a), hipMemcpyHostToDevice, s));
+        AMRCHK(a, hipStreamSynchronize(s));
+        return COSMOS_OK;
+    });
+}
+
 int cosmos_amr_refine_flags(cosmos_amr* a, double cell_spacing, double tolerance) {
     return amr_guarded(a, [&] { return amr_refine(a, cell_spacing, tolerance); });
 }

@@ -489,3 +489,249 @@ GRGrid BlackHoleEulerSecondOrderSolver::solve2D(GRGrid c, double dx, double cfl,
     store_2d(s, c, buf);
     return c;
 }
+
+// --- AMR (EulerAMR, RelativisticEulerAMR) --------------------------------------
+namespace {
+
+struct AmrHandle {
+    cosmos_amr* a = nullptr;
+    AmrHandle(int physics, int dim, int nx, int ny, double gamma) {
+        if (cosmos_amr_create(&a, physics, dim, nx, ny, gamma) != COSMOS_OK)
+            throw std::runtime_error(std::string("cosmos: ") + cosmos_amr_last_error(nullptr));
+    }
+    ~AmrHandle() { cosmos_amr_destroy(a); }
+    void check(int rc) const {
+        if (rc != COSMOS_OK) throw std::runtime_error(std::string("cosmos: ") + cosmos_amr_last_error(a));
+    }
+};
+
+template <class S>
+void pack_grid(vector<vector<S> >& cells, int& nx, int& ny, vector<double>& buf) {
+    ny = (int)cells.size();
+    nx = ny ? (int)cells[0].size() : 0;
+    for (auto& row : cells)
+        if ((int)row.size() != nx) throw std::invalid_argument("cosmos: ragged grid rows");
+    buf.resize(5 * (size_t)nx * ny);
+    for (int i = 0; i < ny; ++i) pack(cells[i], buf.data() + 5 * (size_t)i * nx);
+}
+
+template <class S>
+vector<vector<S> > unpack_grid(const vector<double>& buf, int nx, int ny) {
+    vector<vector<S> > out(ny, vector<S>(nx));
+    for (int i = 0; i < ny; ++i) unpack(buf.data() + 5 * (size_t)i * nx, out[i]);
+    return out;
+}
+
+vector<bool> bools(const vector<unsigned char>& f) { return vector<bool>(f.begin(), f.end()); }
+
+vector<vector<bool> > bools(const vector<unsigned char>& f, int nx, int ny) {
+    vector<vector<bool> > out(ny, vector<bool>(nx));
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j) out[i][j] = f[(size_t)i * nx + j] != 0;
+    return out;
+}
+
+// A masked sweep: the cells as the coarse level of an AMR context, the mask
+// as its flags (AMR/EulerAMR.cpp:3-167).
+template <class S>
+void masked_sweep_1d(int physics, int order, vector<S>& cells, double dx, double dt, double bias, int lim,
+                     const vector<bool>& mask, double gamma) {
+    const int n = (int)cells.size();
+    if ((int)mask.size() != n) throw std::invalid_argument("cosmos: AMRStructure size differs from the cells'");
+    AmrHandle h(physics, 1, n, 1, gamma);
+    vector<double> buf(5 * (size_t)n);
+    pack(cells, buf.data());
+    h.check(cosmos_amr_upload(h.a, buf.data()));
+    vector<unsigned char> f(mask.begin(), mask.end());
+    h.check(cosmos_amr_set_flags(h.a, f.data(), nullptr));
+    h.check(cosmos_amr_sweep(h.a, 0, order, 0, dx, dt, bias, lim));
+    h.check(cosmos_amr_download(h.a, buf.data(), nullptr, nullptr, nullptr));
+    unpack(buf.data(), cells);
+}
+
+template <class S>
+void masked_sweep_2d(int physics, int order, int axis, vector<vector<S> >& cells, double dx, double dt, double bias,
+                     int lim, const vector<vector<bool> >& mask, double gamma) {
+    int nx, ny;
+    vector<double> buf;
+    pack_grid(cells, nx, ny, buf);
+    if ((int)mask.size() != ny) throw std::invalid_argument("cosmos: AMRStructure size differs from the cells'");
+    vector<unsigned char> f((size_t)nx * ny);
+    for (int i = 0; i < ny; ++i) {
+        if ((int)mask[i].size() != nx) throw std::invalid_argument("cosmos: AMRStructure size differs from the cells'");
+        for (int j = 0; j < nx; ++j) f[(size_t)i * nx + j] = mask[i][j] ? 1 : 0;
+    }
+    AmrHandle h(physics, 2, nx, ny, gamma);
+    h.check(cosmos_amr_upload(h.a, buf.data()));
+    h.check(cosmos_amr_set_flags(h.a, f.data(), nullptr));
+    h.check(cosmos_amr_sweep(h.a, 0, order, axis, dx, dt, bias, lim));
+    h.check(cosmos_amr_download(h.a, buf.data(), nullptr, nullptr, nullptr));
+    cells = unpack_grid<S>(buf, nx, ny);
+}
+
+template <class S>
+tuple<vector<S>, vector<S>, vector<bool>, vector<bool> > level1_1d(int physics, vector<S>& cells, double dx,
+                                                                    double cfl, double tf, double tol, int order,
+                                                                    double gamma) {
+    const int n = (int)cells.size();
+    AmrHandle h(physics, 1, n, 1, gamma);
+    vector<double> c(5 * (size_t)n), f(10 * (size_t)n);
+    pack(cells, c.data());
+    h.check(cosmos_amr_upload(h.a, c.data()));
+    h.check(cosmos_amr_solve_level1(h.a, order, dx, cfl, tf, tol, -1, nullptr, nullptr));
+    vector<unsigned char> cf(n), ff(2 * (size_t)n);
+    h.check(cosmos_amr_download(h.a, c.data(), f.data(), cf.data(), ff.data()));
+    vector<S> coarse(n), fine(2 * (size_t)n);
+    unpack(c.data(), coarse);
+    unpack(f.data(), fine);
+    return tuple<vector<S>, vector<S>, vector<bool>, vector<bool> >(coarse, fine, bools(cf), bools(ff));
+}
+
+template <class S>
+tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >, vector<vector<bool> > > level1_2d(
+    int physics, vector<vector<S> >& cells, double dx, double cfl, double tf, double tol, int order, double gamma) {
+    int nx, ny;
+    vector<double> c;
+    pack_grid(cells, nx, ny, c);
+    AmrHandle h(physics, 2, nx, ny, gamma);
+    h.check(cosmos_amr_upload(h.a, c.data()));
+    h.check(cosmos_amr_solve_level1(h.a, order, dx, cfl, tf, tol, -1, nullptr, nullptr));
+    vector<double> f(5 * (size_t)(4 * nx) * ny);
+    vector<unsigned char> cf((size_t)nx * ny), ff((size_t)4 * nx * ny);
+    h.check(cosmos_amr_download(h.a, c.data(), f.data(), cf.data(), ff.data()));
+    return tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >, vector<vector<bool> > >(
+        unpack_grid<S>(c, nx, ny), unpack_grid<S>(f, 2 * nx, 2 * ny), bools(cf, nx, ny), bools(ff, 2 * nx, 2 * ny));
+}
+
+template <class S>
+tuple<vector<S>, vector<S>, vector<S>, vector<bool>, vector<bool>, vector<bool> > level2_1d(
+    int physics, vector<S>& cells, double dx, double cfl, double tf, double tol, int order, double gamma) {
+    const int n = (int)cells.size();
+    AmrHandle h(physics, 1, n, 1, gamma);
+    vector<double> c(5 * (size_t)n), m(10 * (size_t)n), f(20 * (size_t)n);
+    pack(cells, c.data());
+    h.check(cosmos_amr_upload(h.a, c.data()));
+    h.check(cosmos_amr_solve_level2(h.a, order, dx, cfl, tf, tol, -1, nullptr, nullptr));
+    vector<unsigned char> cf(n), mf(2 * (size_t)n), ff(4 * (size_t)n);
+    h.check(cosmos_amr_download_level2(h.a, c.data(), m.data(), f.data(), cf.data(), mf.data(), ff.data()));
+    vector<S> coarse(n), mid(2 * (size_t)n), fine(4 * (size_t)n);
+    unpack(c.data(), coarse);
+    unpack(m.data(), mid);
+    unpack(f.data(), fine);
+    return tuple<vector<S>, vector<S>, vector<S>, vector<bool>, vector<bool>, vector<bool> >(
+        coarse, mid, fine, bools(cf), bools(mf), bools(ff));
+}
+
+template <class S>
+tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >, vector<vector<bool> >,
+      vector<vector<bool> > >
+level2_2d(int physics, vector<vector<S> >& cells, double dx, double cfl, double tf, double tol, int order,
+          double gamma) {
+    int nx, ny;
+    vector<double> c;
+    pack_grid(cells, nx, ny, c);
+    AmrHandle h(physics, 2, nx, ny, gamma);
+    h.check(cosmos_amr_upload(h.a, c.data()));
+    h.check(cosmos_amr_solve_level2(h.a, order, dx, cfl, tf, tol, -1, nullptr, nullptr));
+    const size_t n = (size_t)nx * ny;
+    vector<double> m(5 * 4 * n), f(5 * 16 * n);
+    vector<unsigned char> cf(n), mf(4 * n), ff(16 * n);
+    h.check(cosmos_amr_download_level2(h.a, c.data(), m.data(), f.data(), cf.data(), mf.data(), ff.data()));
+    return tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >,
+                 vector<vector<bool> >, vector<vector<bool> > >(
+        unpack_grid<S>(c, nx, ny), unpack_grid<S>(m, 2 * nx, 2 * ny), unpack_grid<S>(f, 4 * nx, 4 * ny),
+        bools(cf, nx, ny), bools(mf, 2 * nx, 2 * ny), bools(ff, 4 * nx, 4 * ny));
+}
+
+typedef EulerStateVector NS;
+typedef RelativisticEulerStateVector RS;
+
+}  // namespace
+
+void EulerAMR::computeFORCETimeStepAMR(vector<NS>& c, double dx, double dt, vector<bool> m, EulerMaterialParameters mp) {
+    masked_sweep_1d(COSMOS_NEWTONIAN, 1, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex());
+}
+void EulerAMR::computeXFORCETimeStep2DAMR(vector<vector<NS> >& c, double dx, double dt, vector<vector<bool> > m,
+                                          EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_NEWTONIAN, 1, 0, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex());
+}
+void EulerAMR::computeYFORCETimeStep2DAMR(vector<vector<NS> >& c, double dx, double dt, vector<vector<bool> > m,
+                                          EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_NEWTONIAN, 1, 1, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex());
+}
+void EulerAMR::computeSLICTimeStepAMR(vector<NS>& c, double dx, double dt, double bias, int lim, vector<bool> m,
+                                      EulerMaterialParameters mp) {
+    masked_sweep_1d(COSMOS_NEWTONIAN, 2, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex());
+}
+void EulerAMR::computeXSLICTimeStep2DAMR(vector<vector<NS> >& c, double dx, double dt, double bias, int lim,
+                                         vector<vector<bool> > m, EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_NEWTONIAN, 2, 0, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex());
+}
+void EulerAMR::computeYSLICTimeStep2DAMR(vector<vector<NS> >& c, double dx, double dt, double bias, int lim,
+                                         vector<vector<bool> > m, EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_NEWTONIAN, 2, 1, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex());
+}
+tuple<vector<NS>, vector<NS>, vector<bool>, vector<bool> > EulerAMR::solveLevel1AMR(
+    vector<NS> c, double dx, double cfl, double tf, double tol, int order, EulerMaterialParameters mp) {
+    return level1_1d(COSMOS_NEWTONIAN, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+tuple<vector<vector<NS> >, vector<vector<NS> >, vector<vector<bool> >, vector<vector<bool> > >
+EulerAMR::solve2DLevel1AMR(vector<vector<NS> > c, double dx, double cfl, double tf, double tol, int order,
+                           EulerMaterialParameters mp) {
+    return level1_2d(COSMOS_NEWTONIAN, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+tuple<vector<NS>, vector<NS>, vector<NS>, vector<bool>, vector<bool>, vector<bool> > EulerAMR::solveLevel2AMR(
+    vector<NS> c, double dx, double cfl, double tf, double tol, int order, EulerMaterialParameters mp) {
+    return level2_1d(COSMOS_NEWTONIAN, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+tuple<vector<vector<NS> >, vector<vector<NS> >, vector<vector<NS> >, vector<vector<bool> >, vector<vector<bool> >,
+      vector<vector<bool> > >
+EulerAMR::solve2DLevel2AMR(vector<vector<NS> > c, double dx, double cfl, double tf, double tol, int order,
+                           EulerMaterialParameters mp) {
+    return level2_2d(COSMOS_NEWTONIAN, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+
+void RelativisticEulerAMR::computeFORCETimeStepAMR(vector<RS>& c, double dx, double dt, vector<bool> m,
+                                                   EulerMaterialParameters mp) {
+    masked_sweep_1d(COSMOS_RELATIVISTIC, 1, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex());
+}
+void RelativisticEulerAMR::computeXFORCETimeStep2DAMR(vector<vector<RS> >& c, double dx, double dt,
+                                                      vector<vector<bool> > m, EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_RELATIVISTIC, 1, 0, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex());
+}
+void RelativisticEulerAMR::computeYFORCETimeStep2DAMR(vector<vector<RS> >& c, double dx, double dt,
+                                                      vector<vector<bool> > m, EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_RELATIVISTIC, 1, 1, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex());
+}
+void RelativisticEulerAMR::computeSLICTimeStepAMR(vector<RS>& c, double dx, double dt, double bias, int lim,
+                                                  vector<bool> m, EulerMaterialParameters mp) {
+    masked_sweep_1d(COSMOS_RELATIVISTIC, 2, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex());
+}
+void RelativisticEulerAMR::computeXSLICTimeStep2DAMR(vector<vector<RS> >& c, double dx, double dt, double bias,
+                                                     int lim, vector<vector<bool> > m, EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_RELATIVISTIC, 2, 0, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex());
+}
+void RelativisticEulerAMR::computeYSLICTimeStep2DAMR(vector<vector<RS> >& c, double dx, double dt, double bias,
+                                                     int lim, vector<vector<bool> > m, EulerMaterialParameters mp) {
+    masked_sweep_2d(COSMOS_RELATIVISTIC, 2, 1, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex());
+}
+tuple<vector<RS>, vector<RS>, vector<bool>, vector<bool> > RelativisticEulerAMR::solveLevel1AMR(
+    vector<RS> c, double dx, double cfl, double tf, double tol, int order, EulerMaterialParameters mp) {
+    return level1_1d(COSMOS_RELATIVISTIC, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+tuple<vector<vector<RS> >, vector<vector<RS> >, vector<vector<bool> >, vector<vector<bool> > >
+RelativisticEulerAMR::solve2DLevel1AMR(vector<vector<RS> > c, double dx, double cfl, double tf, double tol, int order,
+                                       EulerMaterialParameters mp) {
+    return level1_2d(COSMOS_RELATIVISTIC, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+tuple<vector<RS>, vector<RS>, vector<RS>, vector<bool>, vector<bool>, vector<bool> >
+RelativisticEulerAMR::solveLevel2AMR(vector<RS> c, double dx, double cfl, double tf, double tol, int order,
+                                     EulerMaterialParameters mp) {
+    return level2_1d(COSMOS_RELATIVISTIC, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+tuple<vector<vector<RS> >, vector<vector<RS> >, vector<vector<RS> >, vector<vector<bool> >, vector<vector<bool> >,
+      vector<vector<bool> > >
+RelativisticEulerAMR::solve2DLevel2AMR(vector<vector<RS> > c, double dx, double cfl, double tf, double tol, int order,
+                                       EulerMaterialParameters mp) {
+    return level2_2d(COSMOS_RELATIVISTIC, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}

@@ -11,6 +11,8 @@
 //   BlackHoleEulerSolvers / ...FirstOrderSolver / ...SecondOrderSolver / ...ForcingSolver
 //     (Solvers/BlackHoleEuler{Solvers,FirstOrderSolver,SecondOrderSolver,ForcingSolver}.hpp)
 //   SlopeLimiters (Solvers/SlopeLimiters.hpp:10-35)         limiter functions and slope vectors
+//   EulerAMR, RelativisticEulerAMR (AMR/EulerAMR.hpp:8-44,
+//     AMR/RelativisticEulerAMR.hpp)                         masked sweeps, level-1 and level-2 drivers
 //
 // The state types come from the host application's headers
 // (EulerStateVector.hpp, RelativisticEulerStateVector.hpp,
@@ -25,6 +27,7 @@
 #ifndef COSMOS_DROPIN_HPP
 #define COSMOS_DROPIN_HPP
 
+#include <tuple>
 #include <vector>
 
 #include "BlackHoleEulerStateVector.hpp"
@@ -33,6 +36,7 @@
 #include "EulerStateVector.hpp"
 #include "RelativisticEulerStateVector.hpp"
 
+using std::tuple;
 using std::vector;
 
 class SlopeLimiters {
@@ -218,6 +222,85 @@ public:
                                                               int subcyclingIterations,
                                                               EulerMaterialParameters materialParameters,
                                                               BlackHoleSpacetime blackHole);
+};
+
+// Two- and three-level AMR (the drivers run on the device; the status lines
+// of AMRHelper are not printed).  Masked sweeps update only the cells whose
+// AMRStructure entry is true.
+class EulerAMR {
+public:
+    static void computeFORCETimeStepAMR(vector<EulerStateVector>& currentCells, double cellSpacing, double timeStep,
+                                        vector<bool> AMRStructure, EulerMaterialParameters materialParameters);
+    static void computeXFORCETimeStep2DAMR(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                           double timeStep, vector<vector<bool> > AMRStructure,
+                                           EulerMaterialParameters materialParameters);
+    static void computeYFORCETimeStep2DAMR(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                           double timeStep, vector<vector<bool> > AMRStructure,
+                                           EulerMaterialParameters materialParameters);
+    static void computeSLICTimeStepAMR(vector<EulerStateVector>& currentCells, double cellSpacing, double timeStep,
+                                       double bias, int slopeLimiter, vector<bool> AMRStructure,
+                                       EulerMaterialParameters materialParameters);
+    static void computeXSLICTimeStep2DAMR(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                          double timeStep, double bias, int slopeLimiter,
+                                          vector<vector<bool> > AMRStructure, EulerMaterialParameters materialParameters);
+    static void computeYSLICTimeStep2DAMR(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
+                                          double timeStep, double bias, int slopeLimiter,
+                                          vector<vector<bool> > AMRStructure, EulerMaterialParameters materialParameters);
+    static tuple<vector<EulerStateVector>, vector<EulerStateVector>, vector<bool>, vector<bool> > solveLevel1AMR(
+        vector<EulerStateVector> initialCells, double cellSpacing, double CFLCoefficient, double finalTime,
+        double AMRTolerance, int order, EulerMaterialParameters materialParameters);
+    static tuple<vector<vector<EulerStateVector> >, vector<vector<EulerStateVector> >, vector<vector<bool> >,
+                 vector<vector<bool> > >
+    solve2DLevel1AMR(vector<vector<EulerStateVector> > initialCells, double cellSpacing, double CFLCoefficient,
+                     double finalTime, double AMRTolerance, int order, EulerMaterialParameters materialParameters);
+    static tuple<vector<EulerStateVector>, vector<EulerStateVector>, vector<EulerStateVector>, vector<bool>,
+                 vector<bool>, vector<bool> >
+    solveLevel2AMR(vector<EulerStateVector> initialCells, double cellSpacing, double CFLCoefficient, double finalTime,
+                   double AMRTolerance, int order, EulerMaterialParameters materialParameters);
+    static tuple<vector<vector<EulerStateVector> >, vector<vector<EulerStateVector> >, vector<vector<EulerStateVector> >,
+                 vector<vector<bool> >, vector<vector<bool> >, vector<vector<bool> > >
+    solve2DLevel2AMR(vector<vector<EulerStateVector> > initialCells, double cellSpacing, double CFLCoefficient,
+                     double finalTime, double AMRTolerance, int order, EulerMaterialParameters materialParameters);
+};
+
+class RelativisticEulerAMR {
+public:
+    static void computeFORCETimeStepAMR(vector<RelativisticEulerStateVector>& currentCells, double cellSpacing,
+                                        double timeStep, vector<bool> AMRStructure,
+                                        EulerMaterialParameters materialParameters);
+    static void computeXFORCETimeStep2DAMR(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                           double cellSpacing, double timeStep, vector<vector<bool> > AMRStructure,
+                                           EulerMaterialParameters materialParameters);
+    static void computeYFORCETimeStep2DAMR(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                           double cellSpacing, double timeStep, vector<vector<bool> > AMRStructure,
+                                           EulerMaterialParameters materialParameters);
+    static void computeSLICTimeStepAMR(vector<RelativisticEulerStateVector>& currentCells, double cellSpacing,
+                                       double timeStep, double bias, int slopeLimiter, vector<bool> AMRStructure,
+                                       EulerMaterialParameters materialParameters);
+    static void computeXSLICTimeStep2DAMR(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                          double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                          vector<vector<bool> > AMRStructure, EulerMaterialParameters materialParameters);
+    static void computeYSLICTimeStep2DAMR(vector<vector<RelativisticEulerStateVector> >& currentCells,
+                                          double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                          vector<vector<bool> > AMRStructure, EulerMaterialParameters materialParameters);
+    static tuple<vector<RelativisticEulerStateVector>, vector<RelativisticEulerStateVector>, vector<bool>, vector<bool> >
+    solveLevel1AMR(vector<RelativisticEulerStateVector> initialCells, double cellSpacing, double CFLCoefficient,
+                   double finalTime, double AMRTolerance, int order, EulerMaterialParameters materialParameters);
+    static tuple<vector<vector<RelativisticEulerStateVector> >, vector<vector<RelativisticEulerStateVector> >,
+                 vector<vector<bool> >, vector<vector<bool> > >
+    solve2DLevel1AMR(vector<vector<RelativisticEulerStateVector> > initialCells, double cellSpacing,
+                     double CFLCoefficient, double finalTime, double AMRTolerance, int order,
+                     EulerMaterialParameters materialParameters);
+    static tuple<vector<RelativisticEulerStateVector>, vector<RelativisticEulerStateVector>,
+                 vector<RelativisticEulerStateVector>, vector<bool>, vector<bool>, vector<bool> >
+    solveLevel2AMR(vector<RelativisticEulerStateVector> initialCells, double cellSpacing, double CFLCoefficient,
+                   double finalTime, double AMRTolerance, int order, EulerMaterialParameters materialParameters);
+    static tuple<vector<vector<RelativisticEulerStateVector> >, vector<vector<RelativisticEulerStateVector> >,
+                 vector<vector<RelativisticEulerStateVector> >, vector<vector<bool> >, vector<vector<bool> >,
+                 vector<vector<bool> > >
+    solve2DLevel2AMR(vector<vector<RelativisticEulerStateVector> > initialCells, double cellSpacing,
+                     double CFLCoefficient, double finalTime, double AMRTolerance, int order,
+                     EulerMaterialParameters materialParameters);
 };
 
 #endif

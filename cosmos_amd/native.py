@@ -101,6 +101,7 @@ SIGNATURES = {
     "cosmos_amr_solve_level1": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,
                                                 _c_int_p]),
+    "cosmos_amr_set_flags": (ctypes.c_int, [_handle, _c_u8_p, _c_u8_p]),
     "cosmos_amr_solve_level2": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,
                                                 _c_int_p]),

@@ -239,6 +239,10 @@ int cosmos_amr_upload(cosmos_amr* a, const double* prim);
 /* Any of the outputs may be NULL. */
 int cosmos_amr_download(cosmos_amr* a, double* coarse, double* fine, unsigned char* coarse_flags,
                         unsigned char* fine_flags);
+/* Replace the coarse and / or fine flags (0/1 bytes, row-major; a null pointer
+ * leaves that level's flags): the AMRStructure argument of the reference's
+ * masked sweeps (computeFORCETimeStepAMR and siblings, AMR/EulerAMR.cpp:3-167). */
+int cosmos_amr_set_flags(cosmos_amr* a, const unsigned char* coarse_flags, const unsigned char* fine_flags);
 /* Refinement flags from the coarse state: a coarse cell with any conserved
  * component k jumping by |dU_k| / cell_spacing >= tolerance to a neighbour
  * (left, right; 2D also top and bottom, 1-cell copy boundary) clears its own

@@ -17,8 +17,16 @@
 //      t = 0.02, one subcycle                                               (60 x 5)
 //  10. one forcing step + X/Y/X SLIC per-call on the 2D input (dt 0.002)   (16 x 16 x 5)
 //  11. BlackHoleEulerSolvers::computeMaximumWaveSpeed2D of the 2D input    (1)
+//  12. EulerAMR::solveLevel1AMR, Sod at 40 cells, tol 5, CFL 0.8, t = 0.02,
+//      SLIC: coarse, fine, coarse flags, fine flags                         (40x5, 80x5, 40, 80)
+//  13. RelativisticEulerAMR::solve2DLevel2AMR, the quadrant states of 3 at
+//      10 x 8, tol 3, t = 0.02, SLIC: three levels and their flags
+//      (8x10x5, 16x20x5, 32x40x5, 80, 320, 1280)
+//  14. EulerAMR::computeXSLICTimeStep2DAMR on the input of 3 with a
+//      checkerboard AMRStructure, dt 0.001                                   (20 x 24 x 5)
 #include <cmath>
 #include <cstdio>
+#include <tuple>
 #include <vector>
 
 #include "cosmos_dropin.hpp"
@@ -117,6 +125,51 @@ int main(int argc, char** argv) {
         BlackHoleEulerSecondOrderSolver::computeXSLICTimeStep2D(g3, 1.0 / n, 0.001, 0.0, 1, mps, bh);
         for (auto& row : g3) put_cells(f, row);
         put(f, BlackHoleEulerSolvers::computeMaximumWaveSpeed2D(g, mps, bh));
+    }
+    {
+        EulerMaterialParameters mp(1.4);
+        const int n = 40;
+        vector<EulerStateVector> sod(n);
+        for (int i = 0; i < n; ++i)
+            sod[i] = i < n / 2 ? EulerStateVector(1.0, 0.0, 0.0, 0.0, 1.0) : EulerStateVector(0.125, 0.0, 0.0, 0.0, 0.1);
+        auto r = EulerAMR::solveLevel1AMR(sod, 1.0 / n, 0.8, 0.02, 5.0, 2, mp);
+        put_cells(f, std::get<0>(r));
+        put_cells(f, std::get<1>(r));
+        for (bool b : std::get<2>(r)) put(f, b ? 1.0 : 0.0);
+        for (bool b : std::get<3>(r)) put(f, b ? 1.0 : 0.0);
+
+        EulerMaterialParameters mps(5.0 / 3.0);
+        const double q[4][5] = {{1.2, 0.3, -0.4, 0.0, 1.5}, {0.4, -0.2, 0.1, 0.0, 0.6}, {0.9, 0.5, 0.5, 0.0, 0.3},
+                                {1.7, -0.6, -0.1, 0.0, 1.1}};
+        const int nx = 10, ny = 8;
+        vector<vector<RelativisticEulerStateVector> > sr(ny, vector<RelativisticEulerStateVector>(nx));
+        for (int i = 0; i < ny; ++i)
+            for (int j = 0; j < nx; ++j) {
+                const double* w = q[2 * (i >= ny / 2) + (j >= nx / 2)];
+                sr[i][j] = RelativisticEulerStateVector(w[0], w[1], w[2], w[3], w[4]);
+            }
+        auto r2 = RelativisticEulerAMR::solve2DLevel2AMR(sr, 1.0 / nx, 0.8, 0.02, 3.0, 2, mps);
+        for (auto& row : std::get<0>(r2)) put_cells(f, row);
+        for (auto& row : std::get<1>(r2)) put_cells(f, row);
+        for (auto& row : std::get<2>(r2)) put_cells(f, row);
+        for (auto& row : std::get<3>(r2))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+        for (auto& row : std::get<4>(r2))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+        for (auto& row : std::get<5>(r2))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+
+        const int mx = 24, my = 20;
+        vector<vector<EulerStateVector> > w(my, vector<EulerStateVector>(mx));
+        vector<vector<bool> > mask(my, vector<bool>(mx));
+        for (int i = 0; i < my; ++i)
+            for (int j = 0; j < mx; ++j) {
+                const double* s4 = q[2 * (i >= 10) + (j >= 12)];
+                w[i][j] = EulerStateVector(s4[0], s4[1], s4[2], s4[3], s4[4]);
+                mask[i][j] = ((i + j) % 2) == 0;
+            }
+        EulerAMR::computeXSLICTimeStep2DAMR(w, 1.0 / mx, 0.001, 0.0, 1, mask, mp);
+        for (auto& row : w) put_cells(f, row);
     }
     fclose(f);
     return 0;

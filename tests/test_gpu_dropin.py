@@ -75,4 +75,26 @@ def test_dropin_classes_match_oracle(orc, tmp_path):
     e = orc.gr_sweep(e, 2, 0, h2, 0.001, hole2, 0.0, 1, g)
     assert same(take((16, 16, 5)), e)
     assert take((1,))[0] == orc.gr_max_wavespeed(W2, hole2, g)
+
+    # AMR drop-in classes
+    S = np.zeros((40, 5))
+    S[:20] = (1.0, 0.0, 0.0, 0.0, 1.0)
+    S[20:] = (0.125, 0.0, 0.0, 0.0, 0.1)
+    c, fn, cf, ff, _, _ = orc.amr_level1(S, 0, 2, 1 / 40, 0.8, 0.02, 5.0, 1.4)
+    assert same(take((40, 5)), c) and same(take((80, 5)), fn)
+    assert same(take((40,)), cf.astype(float)) and same(take((80,)), ff.astype(float))
+    assert cf.sum() < 40  # the shock was refined
+    Q = np.empty((8, 10, 5))
+    for i in range(8):
+        for j in range(10):
+            Q[i, j] = q[2 * (i >= 4) + (j >= 5)]
+    exp = orc.amr_level2(Q, 1, 2, 1 / 10, 0.8, 0.02, 3.0, 5 / 3)
+    for k, shape in enumerate(((8, 10, 5), (16, 20, 5), (32, 40, 5))):
+        assert same(take(shape), exp[k]), ("level", k)
+    for k, shape in enumerate(((8, 10), (16, 20), (32, 40))):
+        assert same(take(shape), exp[3 + k].astype(float)), ("flags", k)
+    assert exp[5].sum() > 0
+    mask = (np.add.outer(np.arange(20), np.arange(24)) % 2) == 0
+    e = orc.sweep(W, 0, 2, 0, 1 / 24, 0.001)
+    assert same(take((20, 24, 5)), np.where(mask[..., None], e, W))
     assert pos == v.size
