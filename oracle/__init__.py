@@ -82,6 +82,13 @@ class CpuPath:
             self.lib.orc_amr_level2.restype = _I
             self.lib.orc_amr_level2.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P, _P, _P, u8, u8,
                                                 u8, ctypes.POINTER(_D)]
+        if prefix == "ref_" and hasattr(self.lib, "ref_gr_amr_level1"):
+            self.lib.ref_gr_amr_level1.restype = _I
+            self.lib.ref_gr_amr_level1.argtypes = [_I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _P,
+                                                   _P, u8, u8]
+            self.lib.ref_gr_amr_level2.restype = _I
+            self.lib.ref_gr_amr_level2.argtypes = [_I, _P, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _P, _P, _P,
+                                                   u8, u8, u8]
         if prefix == "ref_" and hasattr(self.lib, "ref_amr_level1"):
             self.lib.ref_amr_level1.restype = _I
             self.lib.ref_amr_level1.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _P, _P, u8, u8]
@@ -94,6 +101,12 @@ class CpuPath:
             self.lib.orc_set_c2p.argtypes = [_D, _I]
             self.lib.orc_set_threads.argtypes = [_I]
             self.lib.orc_gr_set_excision.argtypes = [_D]
+            self.lib.orc_gr_amr_level1.restype = _I
+            self.lib.orc_gr_amr_level1.argtypes = [_I, _I, _P, _I, _I, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D,
+                                                   _P, _P, u8, u8, ctypes.POINTER(_D)]
+            self.lib.orc_gr_amr_level2.restype = _I
+            self.lib.orc_gr_amr_level2.argtypes = [_I, _P, _I, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D, _P, _P,
+                                                   _P, u8, u8, u8, ctypes.POINTER(_D)]
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -249,6 +262,44 @@ class CpuPath:
             return coarse, inter, fine, cf, jf, ff
         t = _D()
         k = self.lib.orc_amr_level2(*args, max_steps, *outs, ctypes.byref(t))
+        return coarse, inter, fine, cf, jf, ff, t.value, k
+
+    def gr_amr_level1(self, W, order, dx, cfl, tf, tol, hole, gamma=4.0 / 3.0, max_steps=-1):
+        """(coarse, fine, coarse_flags, fine_flags[, t, steps]) of BlackHoleEulerAMR::solve{,2D}Level1AMR."""
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        fshape = (2 * nx, 5) if dim == 1 else (2 * ny, 2 * nx, 5)
+        coarse, fine = np.empty_like(a), np.empty(fshape)
+        cf = np.empty(a.shape[:-1], dtype=np.uint8)
+        ff = np.empty(fshape[:-1], dtype=np.uint8)
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        if self.prefix == "ref_":
+            self.lib.ref_gr_amr_level1(order, dim, _dp(a), nx, ny, dx, cfl, tf, tol, gamma, *hole, _dp(coarse),
+                                       _dp(fine), cf.ctypes.data_as(u8), ff.ctypes.data_as(u8))
+            return coarse, fine, cf, ff
+        t = _D()
+        n = self.lib.orc_gr_amr_level1(order, dim, _dp(a), nx, ny, dx, cfl, tf, tol, max_steps, gamma, *hole,
+                                       _dp(coarse), _dp(fine), cf.ctypes.data_as(u8), ff.ctypes.data_as(u8),
+                                       ctypes.byref(t))
+        return coarse, fine, cf, ff, t.value, n
+
+    def gr_amr_level2(self, W, order, dx, cfl, tf, tol, hole, gamma=4.0 / 3.0, max_steps=-1):
+        """(coarse, inter, fine, flags x3[, t, steps]) of the 1D BlackHoleEulerAMR::solveLevel2AMR."""
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        if a.ndim != 2:
+            raise ValueError("the three-level black-hole driver is 1D")
+        nx = a.shape[0]
+        coarse, inter, fine = np.empty_like(a), np.empty((2 * nx, 5)), np.empty((4 * nx, 5))
+        cf, jf, ff = (np.empty(m * nx, dtype=np.uint8) for m in (1, 2, 4))
+        u8 = ctypes.POINTER(ctypes.c_ubyte)
+        outs = [_dp(coarse), _dp(inter), _dp(fine), cf.ctypes.data_as(u8), jf.ctypes.data_as(u8),
+                ff.ctypes.data_as(u8)]
+        if self.prefix == "ref_":
+            self.lib.ref_gr_amr_level2(order, _dp(a), nx, dx, cfl, tf, tol, gamma, *hole, *outs)
+            return coarse, inter, fine, cf, jf, ff
+        t = _D()
+        k = self.lib.orc_gr_amr_level2(order, _dp(a), nx, dx, cfl, tf, tol, max_steps, gamma, *hole, *outs,
+                                       ctypes.byref(t))
         return coarse, inter, fine, cf, jf, ff, t.value, k
 
     def amr_refine_flags(self, W, physics, dx, tol, cflags, fflags, gamma=1.4):

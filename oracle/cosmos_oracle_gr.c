@@ -793,3 +793,332 @@ void orc_gr_point(int what, const double* in, double x, double y, double z, doub
         default: break;
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* AMR on the black-hole spacetime (AMR/BlackHoleEulerAMR.cpp)              */
+/* ------------------------------------------------------------------------ */
+/* BlackHoleEulerAMR::solveLevel1AMR / solve2DLevel1AMR (:239-695) and the 1D
+ * solveLevel2AMR (:696-1058); the 2D level-2 driver is truncated in the
+ * reference (:1319-1325) and not restated.  The loops are those of the
+ * Newtonian drivers with: p2c / c2p of the GR state at each cell's coordinate
+ * (the expressions of each call site below), masked sweeps that are the GR
+ * sweeps restricted to the flags (:3-237), the GR stable time step, and no
+ * forcing.  Quirks kept: the 2D level-1 refinement test reads the diagonal
+ * cell (i + 1, j + 1) as "bottom" (:494; the boundary-padded grid's corner is
+ * a default state (1, 0, 0, 0, 1)), and the 1D level-1 derefinement test
+ * evaluates fine cells 2i, 2i+1, 2i+2 at the coordinates of 2i-1, 2i, 2i+1
+ * (:597-602), and the 2D driver always sub-steps its fine level (:533). */
+
+static int clampi_gr(int c, int n) { return c < 0 ? 0 : (c >= n ? n - 1 : c); }
+
+static int jump_gr(const double* a, const double* b, double dx, double tol) {
+    for (int k = 0; k < 5; ++k)
+        if (fabs((b[k] - a[k]) / dx) >= tol) return 1;
+    return 0;
+}
+
+static void gr_sweep_masked(const bh_t* b, int order, int axis, int dim, double* prim, int nx, int ny, double h,
+                            double dt, double gamma, const unsigned char* mask) {
+    const long n = (long)nx * (dim == 1 ? 1 : ny);
+    double* upd = (double*)malloc(sizeof(double) * 5 * (size_t)n);
+    if (!upd) return;
+    memcpy(upd, prim, sizeof(double) * 5 * (size_t)n);
+    orc_gr_sweep(order, axis, dim, upd, nx, ny, h, dt, 0.0, 1, gamma, b->M, b->a, b->x, b->y, b->z);
+    for (long c = 0; c < n; ++c)
+        if (mask[c]) memcpy(prim + 5 * c, upd + 5 * c, 40);
+    free(upd);
+}
+
+static void gr_flux_step_masked(const bh_t* b, int order, int dim, double* prim, int nx, int ny, double h, double dt,
+                                double gamma, const unsigned char* mask) {
+    if (dim == 1) {
+        gr_sweep_masked(b, order, 0, 1, prim, nx, 1, h, dt, gamma, mask);
+        return;
+    }
+    gr_sweep_masked(b, order, 0, 2, prim, nx, ny, h, dt * 0.5, gamma, mask);
+    gr_sweep_masked(b, order, 1, 2, prim, nx, ny, h, dt, gamma, mask);
+    gr_sweep_masked(b, order, 0, 2, prim, nx, ny, h, dt * 0.5, gamma, mask);
+}
+
+static double gr_amr_dt(const bh_t* b, int dim, const double* prim, int nx, int ny, double h, double cfl, double t,
+                        double tf, int iteration, double gamma) {
+    const double ws = orc_gr_max_wavespeed(dim, prim, nx, ny, gamma, b->M, b->a, b->x, b->y, b->z);
+    return orc_stable_dt_scalar(cfl * (h / ws), t, tf, iteration);
+}
+
+/* c2p at (x, y) of 0.5 (U_a + U_b) (1D) */
+static void gr_restrict2(const bh_t* b, const double* Wa, double xa, const double* Wb, double xb, double xp,
+                         double gamma, double* out) {
+    double ua[5], ub[5], m[5];
+    gr_p2c(b, Wa, xa, 0.0, 0.0, gamma, ua);
+    gr_p2c(b, Wb, xb, 0.0, 0.0, gamma, ub);
+    for (int k = 0; k < 5; ++k) m[k] = 0.5 * (ua[k] + ub[k]);
+    gr_c2p(b, m, xp, 0.0, 0.0, gamma, out);
+}
+
+/* c2p at the parent of 0.25 (((TL + TR) + BL) + BR), children at their coordinates (2D) */
+static void gr_restrict4(const bh_t* b, const double* fine, int fnx, int i, int j, double dx, double gamma,
+                         double* out) {
+    double u[4][5], m[5];
+    const double hf = dx * 0.5;
+    gr_p2c(b, fine + 5 * ((long)(2 * i) * fnx + 2 * j), (j * 2) * hf, (i * 2) * hf, 0.0, gamma, u[0]);
+    gr_p2c(b, fine + 5 * ((long)(2 * i) * fnx + 2 * j + 1), ((j * 2) + 1) * hf, (i * 2) * hf, 0.0, gamma, u[1]);
+    gr_p2c(b, fine + 5 * ((long)(2 * i + 1) * fnx + 2 * j), (j * 2) * hf, ((i * 2) + 1) * hf, 0.0, gamma, u[2]);
+    gr_p2c(b, fine + 5 * ((long)(2 * i + 1) * fnx + 2 * j + 1), ((j * 2) + 1) * hf, ((i * 2) + 1) * hf, 0.0, gamma,
+           u[3]);
+    for (int k = 0; k < 5; ++k) m[k] = 0.25 * (((u[0][k] + u[1][k]) + u[2][k]) + u[3][k]);
+    gr_c2p(b, m, j * dx, i * dx, 0.0, gamma, out);
+}
+
+/* 1D refinement flags of a level of n cells at spacing hs (coordinate c * hs),
+ * tests over dx at tol; only_flagged: untested where the flag is clear. */
+static void gr_refine_1d(const bh_t* b, const double* W, int n, double hs, double dx, double tol, int only_flagged,
+                         double gamma, unsigned char* pf, unsigned char* cf) {
+    for (int i = 0; i < n; ++i) {
+        if (only_flagged && !pf[i]) continue;
+        double ul[5], um[5], ur[5];
+        gr_p2c(b, W + 5L * clampi_gr(i - 1, n), (i - 1) * hs, 0.0, 0.0, gamma, ul);
+        gr_p2c(b, W + 5L * i, i * hs, 0.0, 0.0, gamma, um);
+        gr_p2c(b, W + 5L * clampi_gr(i + 1, n), (i + 1) * hs, 0.0, 0.0, gamma, ur);
+        if (jump_gr(ul, um, dx, tol) || jump_gr(um, ur, dx, tol)) {
+            pf[i] = 0;
+            cf[2 * i] = cf[2 * i + 1] = 1;
+        }
+    }
+}
+
+int orc_gr_amr_level1(int order, int dim, const double* prim, int nx, int ny, double dx, double cfl, double tf,
+                      double tol, int max_coarse_steps, double gamma, double M, double a, double bx, double by,
+                      double bz, double* coarse, double* fine, unsigned char* cflags, unsigned char* fflags,
+                      double* t_out) {
+    const bh_t B = {M, a, bx, by, bz}, *b = &B;
+    if (dim == 1) ny = 1;
+    const int fnx = 2 * nx, fny = dim == 1 ? 1 : 2 * ny, rows = dim == 1 ? 1 : 2;
+    const long nc = (long)nx * ny, nf = (long)fnx * fny;
+    memcpy(coarse, prim, sizeof(double) * 5 * (size_t)nc);
+    for (long c = 0; c < nc; ++c) cflags[c] = 1;
+    memset(fflags, 0, (size_t)nf);
+    for (int i = 0; i < ny; ++i)
+        for (int j = 0; j < nx; ++j)
+            for (int r = 0; r < rows; ++r)
+                for (int s = 0; s < 2; ++s)
+                    memcpy(fine + 5 * ((long)(rows * i + r) * fnx + 2 * j + s), coarse + 5 * ((long)i * nx + j), 40);
+    const double Wdef[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+    double t = 0.0;
+    int it = 0, fine_total = 0;
+    const int cap = max_coarse_steps < 0 ? 0x7fffffff : max_coarse_steps;
+    while (t < tf && it < cap) {
+        const double cdt = gr_amr_dt(b, dim, coarse, nx, ny, dx, cfl, t, tf, fine_total, gamma);
+        if (dim == 1) {
+            gr_refine_1d(b, coarse, nx, dx, dx, tol, 0, gamma, cflags, fflags);
+        } else {
+            for (int i = 0; i < ny; ++i)
+                for (int j = 0; j < nx; ++j) {
+                    double ut[5], ul[5], um[5], ur[5], ub[5];
+                    gr_p2c(b, coarse + 5 * ((long)clampi_gr(i - 1, ny) * nx + j), j * dx, (i - 1) * dx, 0.0, gamma,
+                           ut);
+                    gr_p2c(b, coarse + 5 * ((long)i * nx + clampi_gr(j - 1, nx)), (j - 1) * dx, i * dx, 0.0, gamma,
+                           ul);
+                    gr_p2c(b, coarse + 5 * ((long)i * nx + j), j * dx, i * dx, 0.0, gamma, um);
+                    gr_p2c(b, coarse + 5 * ((long)i * nx + clampi_gr(j + 1, nx)), (j + 1) * dx, i * dx, 0.0, gamma,
+                           ur);
+                    const double* wb = (i + 1 >= ny && j + 1 >= nx)
+                                           ? Wdef
+                                           : coarse + 5 * ((long)clampi_gr(i + 1, ny) * nx + clampi_gr(j + 1, nx));
+                    gr_p2c(b, wb, (j + 1) * dx, (i + 1) * dx, 0.0, gamma, ub);
+                    if (jump_gr(ut, um, dx, tol) || jump_gr(um, ub, dx, tol) || jump_gr(ul, um, dx, tol) ||
+                        jump_gr(um, ur, dx, tol)) {
+                        cflags[(long)i * nx + j] = 0;
+                        for (int r = 0; r < 2; ++r)
+                            for (int s = 0; s < 2; ++s) fflags[(long)(2 * i + r) * fnx + 2 * j + s] = 1;
+                    }
+                }
+        }
+        gr_flux_step_masked(b, order, dim, coarse, nx, ny, dx, cdt, gamma, cflags);
+        /* the 2D driver starts its "refined" test at true (:533), so its fine
+         * level always sub-steps */
+        int refined = dim == 2;
+        for (long f = 0; f < nf; ++f) refined |= fflags[f];
+        if (refined) {
+            double ft = t;
+            while (ft < ((t + cdt) - pow(10.0, -4.0))) {
+                const double fdt = gr_amr_dt(b, dim, fine, fnx, fny, dx * 0.5, cfl, ft, t + cdt, fine_total, gamma);
+                gr_flux_step_masked(b, order, dim, fine, fnx, fny, dx * 0.5, fdt, gamma, fflags);
+                ft += fdt;
+                fine_total += 1;
+            }
+        } else {
+            fine_total += 1;
+        }
+        t += cdt;
+        it += 1;
+        /* restriction of refined coarse cells (children at their coordinates), injection */
+        for (int i = 0; i < ny; ++i)
+            for (int j = 0; j < nx; ++j) {
+                double* cc = coarse + 5 * ((long)i * nx + j);
+                if (!cflags[(long)i * nx + j]) {
+                    if (dim == 1)
+                        gr_restrict2(b, fine + 5 * (2L * j), (j * 2) * (dx * 0.5), fine + 5 * (2L * j + 1),
+                                     ((j * 2) + 1) * (dx * 0.5), j * dx, gamma, cc);
+                    else
+                        gr_restrict4(b, fine, fnx, i, j, dx, gamma, cc);
+                }
+                for (int r = 0; r < rows; ++r)
+                    for (int s = 0; s < 2; ++s) {
+                        const long f = (long)(rows * i + r) * fnx + 2 * j + s;
+                        if (!fflags[f]) memcpy(fine + 5 * f, cc, 40);
+                    }
+            }
+        /* derefinement over the coarse spacing at tol * 0.5, child 0 flagged */
+        for (int i = 0; i < ny; ++i)
+            for (int j = 0; j < nx; ++j) {
+                int marked;
+                long first;
+                const double hf = dx * 0.5;
+                if (dim == 1) {
+                    double ul[5], um[5], ur[5];
+                    gr_p2c(b, fine + 5 * (2L * j), ((j * 2) - 1) * hf, 0.0, 0.0, gamma, ul);
+                    gr_p2c(b, fine + 5 * (2L * j + 1), (j * 2) * hf, 0.0, 0.0, gamma, um);
+                    gr_p2c(b, fine + 5L * clampi_gr(2 * j + 2, fnx), ((j * 2) + 1) * hf, 0.0, 0.0, gamma, ur);
+                    marked = jump_gr(ul, um, dx, tol * 0.5) || jump_gr(um, ur, dx, tol * 0.5);
+                    first = 2L * j;
+                } else {
+                    double ut[5], ul[5], um[5], ur[5], ub[5];
+                    const int fi = 2 * i + 1, fj = 2 * j + 1;
+                    gr_p2c(b, fine + 5 * ((long)(fi - 1) * fnx + fj), ((j * 2) + 1) * hf, (i * 2) * hf, 0.0, gamma,
+                           ut);
+                    gr_p2c(b, fine + 5 * ((long)fi * fnx + fj - 1), (j * 2) * hf, ((i * 2) + 1) * hf, 0.0, gamma, ul);
+                    gr_p2c(b, fine + 5 * ((long)fi * fnx + fj), ((j * 2) + 1) * hf, ((i * 2) + 1) * hf, 0.0, gamma,
+                           um);
+                    gr_p2c(b, fine + 5 * ((long)fi * fnx + clampi_gr(fj + 1, fnx)), ((j * 2) + 2) * hf,
+                           ((i * 2) + 1) * hf, 0.0, gamma, ur);
+                    gr_p2c(b, fine + 5 * ((long)clampi_gr(fi + 1, fny) * fnx + fj), ((j * 2) + 1) * hf,
+                           ((i * 2) + 2) * hf, 0.0, gamma, ub);
+                    marked = jump_gr(ut, um, dx, tol * 0.5) || jump_gr(um, ub, dx, tol * 0.5) ||
+                             jump_gr(ul, um, dx, tol * 0.5) || jump_gr(um, ur, dx, tol * 0.5);
+                    first = (2L * i) * fnx + 2 * j;
+                }
+                if (!marked && fflags[first]) {
+                    for (int r = 0; r < rows; ++r)
+                        for (int s = 0; s < 2; ++s) fflags[(long)(rows * i + r) * fnx + 2 * j + s] = 0;
+                    cflags[(long)i * nx + j] = 1;
+                    double* cc = coarse + 5 * ((long)i * nx + j);
+                    if (dim == 1)
+                        gr_restrict2(b, fine + 5 * (2L * j), (j * 2) * (dx * 0.5), fine + 5 * (2L * j + 1),
+                                     ((j * 2) + 1) * (dx * 0.5), j * dx, gamma, cc);
+                    else
+                        gr_restrict4(b, fine, fnx, i, j, dx, gamma, cc);
+                }
+            }
+    }
+    if (t_out) *t_out = t;
+    return it;
+}
+
+/* Restriction of a refined parent p (children 2p, 2p+1 at spacing hc, parent
+ * at spacing hp) and injection of the parent into unflagged children.  strict
+ * (the intermediate/fine exchange, :889-917): restrict only where both
+ * children are flagged and inject only into children of flagged parents;
+ * otherwise (the coarse/intermediate exchange, :961-985) restrict every
+ * unflagged parent and inject into every unflagged child. */
+static void gr_exchange_1d(const bh_t* b, double* P, unsigned char* pf, int np, double hp, double* C,
+                           const unsigned char* cf, double hc, int strict, double gamma) {
+    for (int i = 0; i < np; ++i) {
+        const int restrict_here = strict ? (!pf[i] && (cf[2 * i] && cf[2 * i + 1])) : !pf[i];
+        if (restrict_here)
+            gr_restrict2(b, C + 5 * (2L * i), (i * 2) * hc, C + 5 * (2L * i + 1), ((i * 2) + 1) * hc, i * hp, gamma,
+                         P + 5L * i);
+        for (int s = 0; s < 2; ++s)
+            if (!cf[2 * i + s] && (!strict || pf[i])) memcpy(C + 5 * (2L * i + s), P + 5L * i, 40);
+    }
+}
+
+/* Derefinement of parents whose children 2p, 2p+1 are both flagged and whose
+ * children 2p, 2p+1, 2p+2 (at their own coordinates) jump by less than tol
+ * over dx (:919-959, :987-1025). */
+static void gr_derefine_1d(const bh_t* b, double* P, unsigned char* pf, int np, double hp, const double* C,
+                           unsigned char* cf, double hc, double dx, double tol, double gamma) {
+    const int nc = 2 * np;
+    for (int i = 0; i < np; ++i) {
+        double ul[5], um[5], ur[5];
+        gr_p2c(b, C + 5 * (2L * i), (i * 2) * hc, 0.0, 0.0, gamma, ul);
+        gr_p2c(b, C + 5 * (2L * i + 1), ((i * 2) + 1) * hc, 0.0, 0.0, gamma, um);
+        gr_p2c(b, C + 5L * clampi_gr(2 * i + 2, nc), ((i * 2) + 2) * hc, 0.0, 0.0, gamma, ur);
+        const int marked = jump_gr(ul, um, dx, tol) || jump_gr(um, ur, dx, tol);
+        if (!marked && (cf[2 * i] && cf[2 * i + 1])) {
+            cf[2 * i] = cf[2 * i + 1] = 0;
+            pf[i] = 1;
+            gr_restrict2(b, C + 5 * (2L * i), (i * 2) * hc, C + 5 * (2L * i + 1), ((i * 2) + 1) * hc, i * hp, gamma,
+                         P + 5L * i);
+        }
+    }
+}
+
+static int any_gr(const unsigned char* f, long n) {
+    for (long i = 0; i < n; ++i)
+        if (f[i]) return 1;
+    return 0;
+}
+
+/* BlackHoleEulerAMR::solveLevel2AMR (:696-1058), 1D: coarse (nx cells, dx),
+ * intermediate (2 nx, dx * 0.5) and fine (4 nx, dx * 0.25) levels.  Refinement
+ * tests divide by dx on every level with tolerances tol (coarse), tol * 2 *
+ * order (intermediate, flagged cells only) and derefinement tolerances tol
+ * (fine into intermediate) and tol * 0.5 / order (intermediate into coarse);
+ * fineTotalIteration counts every fine and every intermediate step. */
+int orc_gr_amr_level2(int order, const double* prim, int nx, double dx, double cfl, double tf, double tol,
+                      int max_coarse_steps, double gamma, double M, double a, double bx, double by, double bz,
+                      double* coarse, double* inter, double* fine, unsigned char* cflags, unsigned char* iflags,
+                      unsigned char* fflags, double* t_out) {
+    const bh_t B = {M, a, bx, by, bz}, *b = &B;
+    const int mnx = 2 * nx, fnx = 4 * nx;
+    memcpy(coarse, prim, sizeof(double) * 5 * (size_t)nx);
+    for (int i = 0; i < nx; ++i) cflags[i] = 1;
+    memset(iflags, 0, (size_t)mnx);
+    memset(fflags, 0, (size_t)fnx);
+    for (int i = 0; i < mnx; ++i) memcpy(inter + 5L * i, coarse + 5L * (i / 2), 40);
+    for (int i = 0; i < fnx; ++i) memcpy(fine + 5L * i, coarse + 5L * (i / 4), 40);
+    const double itol = (tol * 2.0) * order;
+    const double dtol = (tol * 0.5) * (1.0 / order);
+    const double hm = 0.5 * dx; /* the intermediate tests' (0.5 * cellSpacing) */
+    double t = 0.0;
+    int it = 0, fine_total = 0;
+    const int cap = max_coarse_steps < 0 ? 0x7fffffff : max_coarse_steps;
+    while (t < tf && it < cap) {
+        const double cdt = gr_amr_dt(b, 1, coarse, nx, 1, dx, cfl, t, tf, fine_total, gamma);
+        gr_refine_1d(b, coarse, nx, dx, dx, tol, 0, gamma, cflags, iflags);
+        gr_flux_step_masked(b, order, 1, coarse, nx, 1, dx, cdt, gamma, cflags);
+        if (any_gr(iflags, mnx)) {
+            double mt = t;
+            while (mt < ((t + cdt) - pow(10.0, -4.0))) {
+                const double mdt = gr_amr_dt(b, 1, inter, mnx, 1, dx * 0.5, cfl, mt, t + cdt, fine_total, gamma);
+                gr_refine_1d(b, inter, mnx, hm, dx, itol, 1, gamma, iflags, fflags);
+                gr_flux_step_masked(b, order, 1, inter, mnx, 1, dx * 0.5, mdt, gamma, iflags);
+                if (any_gr(fflags, fnx)) {
+                    double ft = mt;
+                    while (ft < ((mt + mdt) - pow(10.0, -4.0))) {
+                        const double fdt =
+                            gr_amr_dt(b, 1, fine, fnx, 1, dx * 0.25, cfl, ft, mt + mdt, fine_total, gamma);
+                        gr_flux_step_masked(b, order, 1, fine, fnx, 1, dx * 0.25, fdt, gamma, fflags);
+                        ft += fdt;
+                        fine_total += 1;
+                    }
+                } else {
+                    fine_total += 1;
+                }
+                mt += mdt;
+                fine_total += 1;
+                gr_exchange_1d(b, inter, iflags, mnx, dx * 0.5, fine, fflags, dx * 0.25, 1, gamma);
+                gr_derefine_1d(b, inter, iflags, mnx, dx * 0.5, fine, fflags, dx * 0.25, dx, tol, gamma);
+            }
+        } else {
+            fine_total += 1;
+        }
+        t += cdt;
+        it += 1;
+        gr_exchange_1d(b, coarse, cflags, nx, dx, inter, iflags, dx * 0.5, 0, gamma);
+        gr_derefine_1d(b, coarse, cflags, nx, dx, inter, iflags, dx * 0.5, dx, dtol, gamma);
+    }
+    if (t_out) *t_out = t;
+    return it;
+}

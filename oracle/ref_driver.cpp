@@ -599,3 +599,54 @@ extern "C" int ref_amr_level2(int physics, int dim, int order, const double* pri
                                                                        fflags);
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Black-hole AMR (AMR/BlackHoleEulerAMR.cpp:238-1055): the reference's own
+// solveLevel1AMR, solve2DLevel1AMR and 1D solveLevel2AMR, run to final_time,
+// on BlackHoleSpacetime(M, a, x, y, z); layouts as above.
+// ---------------------------------------------------------------------------
+#include "AMR/BlackHoleEulerAMR.hpp"
+
+extern "C" int ref_gr_amr_level1(int order, int dim, const double* prim, int nx, int ny, double dx, double cfl,
+                                 double tf, double tol, double gamma, double M, double a, double bx, double by,
+                                 double bz, double* coarse, double* fine, unsigned char* cflags,
+                                 unsigned char* fflags) {
+    QuietCout quiet;
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    typedef BlackHoleEulerStateVector S;
+    if (dim == 1) {
+        std::vector<S> c = to_1d<S>(prim, nx);
+        auto r = BlackHoleEulerAMR::solveLevel1AMR(c, dx, cfl, tf, tol, order, mp, bh);
+        from_1d(std::get<0>(r), coarse);
+        from_1d(std::get<1>(r), fine);
+        flags_out(std::get<2>(r), cflags);
+        flags_out(std::get<3>(r), fflags);
+        return 0;
+    }
+    std::vector<std::vector<S> > c = to_2d<S>(prim, nx, ny);
+    auto r = BlackHoleEulerAMR::solve2DLevel1AMR(c, dx, cfl, tf, tol, order, mp, bh);
+    from_2d(std::get<0>(r), coarse);
+    from_2d(std::get<1>(r), fine);
+    flags_out(std::get<2>(r), cflags);
+    flags_out(std::get<3>(r), fflags);
+    return 0;
+}
+
+extern "C" int ref_gr_amr_level2(int order, const double* prim, int nx, double dx, double cfl, double tf, double tol,
+                                 double gamma, double M, double a, double bx, double by, double bz, double* coarse,
+                                 double* inter, double* fine, unsigned char* cflags, unsigned char* iflags,
+                                 unsigned char* fflags) {
+    QuietCout quiet;
+    EulerMaterialParameters mp(gamma);
+    BlackHoleSpacetime bh(M, a, bx, by, bz);
+    std::vector<BlackHoleEulerStateVector> c = to_1d<BlackHoleEulerStateVector>(prim, nx);
+    auto r = BlackHoleEulerAMR::solveLevel2AMR(c, dx, cfl, tf, tol, order, mp, bh);
+    from_1d(std::get<0>(r), coarse);
+    from_1d(std::get<1>(r), inter);
+    from_1d(std::get<2>(r), fine);
+    flags_out(std::get<3>(r), cflags);
+    flags_out(std::get<4>(r), iflags);
+    flags_out(std::get<5>(r), fflags);
+    return 0;
+}
