@@ -1,7 +1,9 @@
-// Two-level adaptive mesh refinement: the refinement flags, masked sweeps,
-// restriction / injection and derefinement of EulerAMR::solve{,2D}Level1AMR
-// (AMR/EulerAMR.cpp:169-590) and the special-relativistic copies
-// (AMR/RelativisticEulerAMR.cpp), on gfx950.
+// Adaptive mesh refinement: the refinement flags, masked sweeps, restriction /
+// injection and derefinement of EulerAMR::solve{,2D}Level{1,2}AMR
+// (AMR/EulerAMR.cpp:169-1381), the special-relativistic copies
+// (AMR/RelativisticEulerAMR.cpp) and the black-hole drivers
+// (AMR/BlackHoleEulerAMR.cpp:238-1055: both 1D and 2D level 1, 1D level 2,
+// states converted at each cell's Kerr-Schild coordinate), on gfx950.
 //
 // A cosmos_amr context holds two solver handles of the public C-ABI (the
 // coarse nx x ny grid and the fine 2nx x 2ny grid) and the two flag grids as
@@ -19,6 +21,7 @@
 #include <vector>
 
 #include "../../include/cosmos_fv.h"
+#include "fv_gr.hpp"
 #include "fv_kernels.hpp"
 
 namespace cfv {
@@ -54,60 +57,120 @@ __device__ __forceinline__ bool jump(const double* a, const double* b, double dx
     return hit;
 }
 
+// Black-hole physics tag of the AMR kernels.
+struct BlackHoleAmr {};
+
+// Where a cell's state is converted: column xi and row yi (1D: row ignored) of
+// a level at spacing h, the coordinate (xi * h, yi * h, 0) of the reference's
+// black-hole call sites ((j - 1) * cellSpacing, ((i * 2) + 1) * (cellSpacing
+// * 0.5), ...: integer index times the level's spacing).  The index is the
+// one the reference writes, which is not always the cell read (clamped
+// neighbours at the edge keep the unclamped coordinate).
+struct At {
+    int xi, yi;
+};
+
+struct Frame {
+    int dim;
+    double h;
+    Hole hole;
+    __device__ __forceinline__ double x(const At& q) const { return (double)q.xi * h; }
+    __device__ __forceinline__ double y(const At& q) const { return dim == 1 ? 0.0 : (double)q.yi * h; }
+};
+
+// State conversions for the flag, restriction and derefinement kernels:
+// Newtonian and special-relativistic states do not depend on where the cell
+// is; black-hole states use the Kerr-Schild geometry at the cell coordinate
+// (BlackHoleEuler/BlackHoleEulerStateVector.cpp:30-157).
 template <class Phys>
-__device__ __forceinline__ void cons_at(const Grid& g, int i, int j, const PhysParams& P, double* U) {
+struct Convert {
+    static __device__ __forceinline__ void p2c(const double* W, const Frame&, const At&, const PhysParams& P,
+                                               double* U) {
+        Phys::p2c(W, U, P);
+    }
+    static __device__ __forceinline__ void c2p(const double* U, const Frame&, const At&, const PhysParams& P,
+                                               double* W) {
+        Phys::c2p(U, W, Phys::inv_density(U), P);
+    }
+};
+
+template <>
+struct Convert<BlackHoleAmr> {
+    static __device__ __forceinline__ void p2c(const double* W, const Frame& F, const At& q, const PhysParams& P,
+                                               double* U) {
+        const Geo G = geo_at(F.hole, F.x(q), F.y(q), 0.0);
+        gr_p2c(W, G, U, P);
+    }
+    static __device__ __forceinline__ void c2p(const double* U, const Frame& F, const At& q, const PhysParams& P,
+                                               double* W) {
+        const Geo G = geo_at(F.hole, F.x(q), F.y(q), 0.0);
+        gr_c2p(U, G.sdet, P, W);
+    }
+};
+
+// Conserved vector of cell (i, j) of g, converted at q of frame F.
+template <class Phys>
+__device__ __forceinline__ void cons_at(const Grid& g, int i, int j, const Frame& F, At q, const PhysParams& P,
+                                        double* U) {
     double W[5];
     load(g, i, j, W);
-    Phys::p2c(W, U, P);
+    Convert<Phys>::p2c(W, F, q, P, U);
 }
 
-// Mean of the children's conserved vectors, back to primitives: 1D 0.5 (L + R),
-// 2D 0.25 (((TL + TR) + BL) + BR) (AMR/EulerAMR.cpp:290-296, :500-509).
+// Mean of the children's conserved vectors (children of parent (i, j) in f,
+// each at its own coordinate in frame Fc), back to primitives at the parent's
+// coordinate in frame Fp: 1D 0.5 (L + R), 2D 0.25 (((TL + TR) + BL) + BR)
+// (AMR/EulerAMR.cpp:290-296, :500-509; AMR/BlackHoleEulerAMR.cpp:355-369,
+// :571-597).
 template <class Phys>
-__device__ __forceinline__ void restrict_children(const Grid& f, int dim, int i, int j, const PhysParams& P,
-                                                  double* W) {
+__device__ __forceinline__ void restrict_children(const Grid& f, int dim, int i, int j, const Frame& Fc,
+                                                  const Frame& Fp, const PhysParams& P, double* W) {
     double a[5], b[5], m[5];
     if (dim == 1) {
-        cons_at<Phys>(f, 0, 2 * j, P, a);
-        cons_at<Phys>(f, 0, 2 * j + 1, P, b);
+        cons_at<Phys>(f, 0, 2 * j, Fc, At{2 * j, 0}, P, a);
+        cons_at<Phys>(f, 0, 2 * j + 1, Fc, At{2 * j + 1, 0}, P, b);
 #pragma unroll
         for (int k = 0; k < 5; ++k) m[k] = 0.5 * (a[k] + b[k]);
     } else {
         double c[5], d[5];
-        cons_at<Phys>(f, 2 * i, 2 * j, P, a);
-        cons_at<Phys>(f, 2 * i, 2 * j + 1, P, b);
-        cons_at<Phys>(f, 2 * i + 1, 2 * j, P, c);
-        cons_at<Phys>(f, 2 * i + 1, 2 * j + 1, P, d);
+        cons_at<Phys>(f, 2 * i, 2 * j, Fc, At{2 * j, 2 * i}, P, a);
+        cons_at<Phys>(f, 2 * i, 2 * j + 1, Fc, At{2 * j + 1, 2 * i}, P, b);
+        cons_at<Phys>(f, 2 * i + 1, 2 * j, Fc, At{2 * j, 2 * i + 1}, P, c);
+        cons_at<Phys>(f, 2 * i + 1, 2 * j + 1, Fc, At{2 * j + 1, 2 * i + 1}, P, d);
 #pragma unroll
         for (int k = 0; k < 5; ++k) m[k] = 0.25 * (((a[k] + b[k]) + c[k]) + d[k]);
     }
-    Phys::c2p(m, W, Phys::inv_density(m), P);
+    Convert<Phys>::c2p(m, Fp, At{j, i}, P, W);
 }
 
-// A1: sticky refinement flags from the coarse state, 1-cell copy boundary.
-// diag: the 2D "bottom" neighbour is the diagonal cell of the boundary-padded
-// grid (AMR/RelativisticEulerAMR.cpp:422), whose bottom-right corner is a
-// default state (1, 0, 0, 0, 1).
+// A1: sticky refinement flags from the coarse state, 1-cell copy boundary,
+// neighbours converted at their unclamped coordinates.  diag: the 2D "bottom"
+// neighbour is the diagonal cell of the boundary-padded grid
+// (AMR/RelativisticEulerAMR.cpp:422, AMR/BlackHoleEulerAMR.cpp:493-495), whose
+// bottom-right corner is a default state (1, 0, 0, 0, 1).
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_amr_refine(Grid c, int dim, double dx, double tol, int diag,
+__global__ __launch_bounds__(kAmrThreads) void k_amr_refine(Grid c, Frame F, double dx, double tol, int diag,
                                                             PhysParams P, u8* cflags, u8* fflags) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)c.nx * c.ny) return;
     int i = (int)(idx / c.nx), j = (int)(idx - (long)i * c.nx);
+    const int dim = F.dim;
     double um[5], un[5];
-    cons_at<Phys>(c, i, j, P, um);
-    cons_at<Phys>(c, i, clampc(j - 1, c.nx), P, un);
+    cons_at<Phys>(c, i, j, F, At{j, i}, P, um);
+    cons_at<Phys>(c, i, clampc(j - 1, c.nx), F, At{j - 1, i}, P, un);
     bool hit = jump(un, um, dx, tol);
-    cons_at<Phys>(c, i, clampc(j + 1, c.nx), P, un);
+    cons_at<Phys>(c, i, clampc(j + 1, c.nx), F, At{j + 1, i}, P, un);
     hit |= jump(um, un, dx, tol);
     if (dim == 2) {
-        cons_at<Phys>(c, clampc(i - 1, c.ny), j, P, un);
+        cons_at<Phys>(c, clampc(i - 1, c.ny), j, F, At{j, i - 1}, P, un);
         hit |= jump(un, um, dx, tol);
         if (diag && i + 1 >= c.ny && j + 1 >= c.nx) {
             const double Wd[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
-            Phys::p2c(Wd, un, P);
+            Convert<Phys>::p2c(Wd, F, At{j + 1, i + 1}, P, un);
+        } else if (diag) {
+            cons_at<Phys>(c, clampc(i + 1, c.ny), clampc(j + 1, c.nx), F, At{j + 1, i + 1}, P, un);
         } else {
-            cons_at<Phys>(c, clampc(i + 1, c.ny), diag ? clampc(j + 1, c.nx) : j, P, un);
+            cons_at<Phys>(c, clampc(i + 1, c.ny), j, F, At{j, i + 1}, P, un);
         }
         hit |= jump(um, un, dx, tol);
     }
@@ -128,16 +191,17 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_refine(Grid c, int dim, dou
 
 // A5 after the fine sub-steps: refined coarse cells take the restriction of
 // their children; children whose flag is clear take a copy of the coarse cell
-// (AMR/EulerAMR.cpp:282-308, :494-536).
+// (AMR/EulerAMR.cpp:282-308, :494-536).  Fc, Fp: fine and coarse frames.
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_amr_restrict(Grid c, Grid f, int dim, PhysParams P,
+__global__ __launch_bounds__(kAmrThreads) void k_amr_restrict(Grid c, Grid f, Frame Fc, Frame Fp, PhysParams P,
                                                               const u8* cflags, const u8* fflags) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)c.nx * c.ny) return;
     int i = (int)(idx / c.nx), j = (int)(idx - (long)i * c.nx);
+    const int dim = Fp.dim;
     double W[5];
     if (!cflags[idx]) {
-        restrict_children<Phys>(f, dim, i, j, P, W);
+        restrict_children<Phys>(f, dim, i, j, Fc, Fp, P, W);
         store(c, i, j, W);
     } else {
         load(c, i, j, W);
@@ -153,33 +217,37 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_restrict(Grid c, Grid f, in
 // A2: derefinement of quiet refined cells, tested on fine cell 2j+1
 // ((2i+1, 2j+1) in 2D) against its fine neighbours (1-cell copy boundary),
 // over the coarse spacing at tol / 2 (AMR/EulerAMR.cpp:312-345, :540-586).
+// The 1D black-hole driver converts fine cells 2j, 2j+1, 2j+2 at the
+// coordinates of 2j-1, 2j, 2j+1 (AMR/BlackHoleEulerAMR.cpp:378-383); the 2D
+// driver at their own (:633-642).  Neither matters to the other physics.
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, int dim, double dx, double tol,
-                                                              PhysParams P, u8* cflags, u8* fflags) {
+__global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, Frame Fc, Frame Fp, double dx,
+                                                              double tol, PhysParams P, u8* cflags, u8* fflags) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)c.nx * c.ny) return;
     int i = (int)(idx / c.nx), j = (int)(idx - (long)i * c.nx);
+    const int dim = Fp.dim;
     const double t2 = tol * 0.5;
     double um[5], un[5];
     bool marked;
     long first;
     if (dim == 1) {
-        cons_at<Phys>(f, 0, 2 * j + 1, P, um);
-        cons_at<Phys>(f, 0, 2 * j, P, un);
+        cons_at<Phys>(f, 0, 2 * j + 1, Fc, At{2 * j, 0}, P, um);
+        cons_at<Phys>(f, 0, 2 * j, Fc, At{2 * j - 1, 0}, P, un);
         marked = jump(un, um, dx, t2);
-        cons_at<Phys>(f, 0, clampc(2 * j + 2, f.nx), P, un);
+        cons_at<Phys>(f, 0, clampc(2 * j + 2, f.nx), Fc, At{2 * j + 1, 0}, P, un);
         marked |= jump(um, un, dx, t2);
         first = 2 * j;
     } else {
         const int fi = 2 * i + 1, fj = 2 * j + 1;
-        cons_at<Phys>(f, fi, fj, P, um);
-        cons_at<Phys>(f, fi - 1, fj, P, un);
+        cons_at<Phys>(f, fi, fj, Fc, At{fj, fi}, P, um);
+        cons_at<Phys>(f, fi - 1, fj, Fc, At{fj, fi - 1}, P, un);
         marked = jump(un, um, dx, t2);
-        cons_at<Phys>(f, clampc(fi + 1, f.ny), fj, P, un);
+        cons_at<Phys>(f, clampc(fi + 1, f.ny), fj, Fc, At{fj, fi + 1}, P, un);
         marked |= jump(um, un, dx, t2);
-        cons_at<Phys>(f, fi, fj - 1, P, un);
+        cons_at<Phys>(f, fi, fj - 1, Fc, At{fj - 1, fi}, P, un);
         marked |= jump(un, um, dx, t2);
-        cons_at<Phys>(f, fi, clampc(fj + 1, f.nx), P, un);
+        cons_at<Phys>(f, fi, clampc(fj + 1, f.nx), Fc, At{fj + 1, fi}, P, un);
         marked |= jump(um, un, dx, t2);
         first = (long)(2 * i) * f.nx + 2 * j;
     }
@@ -195,7 +263,7 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, in
         }
         cflags[idx] = 1;
         double W[5];
-        restrict_children<Phys>(f, dim, i, j, P, W);
+        restrict_children<Phys>(f, dim, i, j, Fc, Fp, P, W);
         store(c, i, j, W);
     }
 }
@@ -237,10 +305,12 @@ __global__ __launch_bounds__(kAmrThreads) void k_inject_all(Grid c, Grid f, int 
         for (int b = 0; b < 2; ++b) store(f, dim == 1 ? 0 : 2 * i + a, 2 * j + b, W);
 }
 
-// Three-level AMR (EulerAMR::solve{,2D}Level2AMR, AMR/EulerAMR.cpp:593-1381): the
-// flag, exchange and derefinement passes between a parent level and its child
-// level (twice the cells along each axis), one thread per parent cell; each
-// thread writes only its own parent cell and children.
+// Three-level AMR (EulerAMR::solve{,2D}Level2AMR, AMR/EulerAMR.cpp:593-1381;
+// the 1D BlackHoleEulerAMR::solveLevel2AMR, AMR/BlackHoleEulerAMR.cpp:695-1055):
+// the flag, exchange and derefinement passes between a parent level and its
+// child level (twice the cells along each axis), one thread per parent cell;
+// each thread writes only its own parent cell and children.  Every conversion
+// uses the natural coordinate of the cell read (unclamped at the edges).
 
 __device__ __forceinline__ long child0(const Grid& c, int dim, int i, int j) {
     return dim == 1 ? 2L * j : (long)(2 * i) * c.nx + 2 * j;
@@ -263,28 +333,29 @@ __device__ __forceinline__ void children_set(u8* cflags, const Grid& c, int dim,
 }
 
 // Refinement flags of the parent from its own states (copy boundary, its two
-// or four neighbours): a jump >= tol over dx clears the parent flag and sets
-// its children's; only_flagged: parents whose flag is already clear are not
-// tested (coarse level :648-666, :982-1010; intermediate level with
-// tol * 2 * order :704-723, :1052-1082).  Level 2 reads the true bottom
-// neighbour for both physics.
+// or four neighbours; Fp the parent frame): a jump >= tol over dx clears the
+// parent flag and sets its children's; only_flagged: parents whose flag is
+// already clear are not tested (coarse level :648-666, :982-1010; intermediate
+// level with tol * 2 * order :704-723, :1052-1082).  Level 2 reads the true
+// bottom neighbour for every physics.
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_l2_refine(Grid p, Grid c, int dim, u8* pflags, u8* cflags, double dx,
-                                                           double tol, int only_flagged, PhysParams P) {
+__global__ __launch_bounds__(kAmrThreads) void k_l2_refine(Grid p, Grid c, Frame Fp, u8* pflags, u8* cflags,
+                                                           double dx, double tol, int only_flagged, PhysParams P) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
     if (only_flagged && !pflags[idx]) return;
+    const int dim = Fp.dim;
     double um[5], un[5];
-    cons_at<Phys>(p, i, j, P, um);
-    cons_at<Phys>(p, i, clampc(j - 1, p.nx), P, un);
+    cons_at<Phys>(p, i, j, Fp, At{j, i}, P, um);
+    cons_at<Phys>(p, i, clampc(j - 1, p.nx), Fp, At{j - 1, i}, P, un);
     bool hit = jump(un, um, dx, tol);
-    cons_at<Phys>(p, i, clampc(j + 1, p.nx), P, un);
+    cons_at<Phys>(p, i, clampc(j + 1, p.nx), Fp, At{j + 1, i}, P, un);
     hit |= jump(um, un, dx, tol);
     if (dim == 2) {
-        cons_at<Phys>(p, clampc(i - 1, p.ny), j, P, un);
+        cons_at<Phys>(p, clampc(i - 1, p.ny), j, Fp, At{j, i - 1}, P, un);
         hit |= jump(un, um, dx, tol);
-        cons_at<Phys>(p, clampc(i + 1, p.ny), j, P, un);
+        cons_at<Phys>(p, clampc(i + 1, p.ny), j, Fp, At{j, i + 1}, P, un);
         hit |= jump(um, un, dx, tol);
     }
     if (hit) {
@@ -293,20 +364,21 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_refine(Grid p, Grid c, int d
     }
 }
 
-// Restriction of refined parents and injection into unflagged children.
-// strict (intermediate <-> fine, :780-799, :1137-1172): restrict only where
-// every child is flagged, inject only under a flagged parent; otherwise
-// (coarse <-> intermediate, :856-875, :1255-1290) restrict every refined
-// parent and inject every unflagged child.
+// Restriction of refined parents and injection into unflagged children
+// (Fc, Fp: child and parent frames).  strict (intermediate <-> fine,
+// :780-799, :1137-1172): restrict only where every child is flagged, inject
+// only under a flagged parent; otherwise (coarse <-> intermediate, :856-875,
+// :1255-1290) restrict every refined parent and inject every unflagged child.
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, int dim, const u8* pflags,
+__global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, Frame Fc, Frame Fp, const u8* pflags,
                                                              const u8* cflags, int strict, PhysParams P) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
+    const int dim = Fp.dim;
     double W[5];
     if (!pflags[idx] && (!strict || children_all(cflags, c, dim, i, j))) {
-        restrict_children<Phys>(c, dim, i, j, P, W);
+        restrict_children<Phys>(c, dim, i, j, Fc, Fp, P, W);
         store(p, i, j, W);
     } else {
         load(p, i, j, W);
@@ -323,36 +395,37 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, int
 // restriction back (fine at tol :802-834, :1175-1220; intermediate at
 // tol * 0.5 * (1 / order) :878-911, :1293-1337).
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_l2_derefine(Grid p, Grid c, int dim, u8* pflags, u8* cflags,
-                                                             double dx, double tol, PhysParams P) {
+__global__ __launch_bounds__(kAmrThreads) void k_l2_derefine(Grid p, Grid c, Frame Fc, Frame Fp, u8* pflags,
+                                                             u8* cflags, double dx, double tol, PhysParams P) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
+    const int dim = Fp.dim;
     double um[5], un[5];
     bool marked;
     if (dim == 1) {
-        cons_at<Phys>(c, 0, 2 * j + 1, P, um);
-        cons_at<Phys>(c, 0, 2 * j, P, un);
+        cons_at<Phys>(c, 0, 2 * j + 1, Fc, At{2 * j + 1, 0}, P, um);
+        cons_at<Phys>(c, 0, 2 * j, Fc, At{2 * j, 0}, P, un);
         marked = jump(un, um, dx, tol);
-        cons_at<Phys>(c, 0, clampc(2 * j + 2, c.nx), P, un);
+        cons_at<Phys>(c, 0, clampc(2 * j + 2, c.nx), Fc, At{2 * j + 2, 0}, P, un);
         marked |= jump(um, un, dx, tol);
     } else {
         const int fi = 2 * i + 1, fj = 2 * j + 1;
-        cons_at<Phys>(c, fi, fj, P, um);
-        cons_at<Phys>(c, fi - 1, fj, P, un);
+        cons_at<Phys>(c, fi, fj, Fc, At{fj, fi}, P, um);
+        cons_at<Phys>(c, fi - 1, fj, Fc, At{fj, fi - 1}, P, un);
         marked = jump(un, um, dx, tol);
-        cons_at<Phys>(c, clampc(fi + 1, c.ny), fj, P, un);
+        cons_at<Phys>(c, clampc(fi + 1, c.ny), fj, Fc, At{fj, fi + 1}, P, un);
         marked |= jump(um, un, dx, tol);
-        cons_at<Phys>(c, fi, fj - 1, P, un);
+        cons_at<Phys>(c, fi, fj - 1, Fc, At{fj - 1, fi}, P, un);
         marked |= jump(un, um, dx, tol);
-        cons_at<Phys>(c, fi, clampc(fj + 1, c.nx), P, un);
+        cons_at<Phys>(c, fi, clampc(fj + 1, c.nx), Fc, At{fj + 1, fi}, P, un);
         marked |= jump(um, un, dx, tol);
     }
     if (!marked && children_all(cflags, c, dim, i, j)) {
         children_set(cflags, c, dim, i, j, 0);
         pflags[idx] = 1;
         double W[5];
-        restrict_children<Phys>(c, dim, i, j, P, W);
+        restrict_children<Phys>(c, dim, i, j, Fc, Fp, P, W);
         store(p, i, j, W);
     }
 }
@@ -374,6 +447,9 @@ struct cosmos_amr {
     unsigned char* f2flags = nullptr; // device, [4nx]
     int* any = nullptr;               // device scratch flag
     cfv::PhysParams P{};
+    cfv::Hole hole = cfv::make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);  // COSMOS_BLACKHOLE (the solvers' default)
+    double excision = 0.0;
+    double dx = 0.0;  // the coarse cell spacing last passed in (black-hole coordinates of cosmos_amr_restrict)
     std::string err;
 };
 
@@ -442,9 +518,18 @@ int amr_guarded(cosmos_amr* a, F&& f) {
 template <template <class> class K, class... Args>
 hipError_t launch_physics(int physics, dim3 g, hipStream_t s, Args... args) {
     if (physics == COSMOS_NEWTONIAN) hipLaunchKernelGGL(K<cfv::Newtonian>::fn, g, dim3(cfv::kAmrThreads), 0, s, args...);
-    else hipLaunchKernelGGL(K<cfv::Relativistic>::fn, g, dim3(cfv::kAmrThreads), 0, s, args...);
+    else if (physics == COSMOS_RELATIVISTIC)
+        hipLaunchKernelGGL(K<cfv::Relativistic>::fn, g, dim3(cfv::kAmrThreads), 0, s, args...);
+    else hipLaunchKernelGGL(K<cfv::BlackHoleAmr>::fn, g, dim3(cfv::kAmrThreads), 0, s, args...);
     return hipGetLastError();
 }
+
+// The coordinate frame of a level at spacing h.
+cfv::Frame frame(const cosmos_amr* a, double h) { return cfv::Frame{a->dim, h, a->hole}; }
+
+// Spacing of level 0 (coarse), 1 or 2 of a run with coarse spacing dx, as the
+// reference writes it (cellSpacing, cellSpacing * 0.5, cellSpacing * 0.25).
+double level_spacing(double dx, int level) { return level == 0 ? dx : level == 1 ? dx * 0.5 : dx * 0.25; }
 
 template <class Phys>
 struct RefineK {
@@ -476,19 +561,22 @@ int amr_refine(cosmos_amr* a, double dx, double tol) {
     int rc = grid_of(a, a->coarse, c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<RefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, a->dim, dx, tol,
-                                       a->physics == COSMOS_RELATIVISTIC ? 1 : 0, a->P, a->cflags, a->fflags)));
+    // the special-relativistic and black-hole 2D drivers read the diagonal cell as "bottom"
+    const int diag = a->physics == COSMOS_NEWTONIAN ? 0 : 1;
+    AMRCHK(a, (launch_physics<RefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, frame(a, dx), dx, tol,
+                                       diag, a->P, a->cflags, a->fflags)));
     return COSMOS_OK;
 }
 
-int amr_restrict(cosmos_amr* a) {
+int amr_restrict(cosmos_amr* a, double dx) {
     Grid c, f;
     int rc = grid_of(a, a->coarse, c);
     if (!rc) rc = grid_of(a, a->fine, f);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<RestrictK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f, a->dim, a->P,
-                                         (const unsigned char*)a->cflags, (const unsigned char*)a->fflags)));
+    AMRCHK(a, (launch_physics<RestrictK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f, frame(a, dx * 0.5),
+                                         frame(a, dx), a->P, (const unsigned char*)a->cflags,
+                                         (const unsigned char*)a->fflags)));
     return COSMOS_OK;
 }
 
@@ -498,8 +586,8 @@ int amr_derefine(cosmos_amr* a, double dx, double tol) {
     if (!rc) rc = grid_of(a, a->fine, f);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<DerefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f, a->dim, dx, tol,
-                                         a->P, a->cflags, a->fflags)));
+    AMRCHK(a, (launch_physics<DerefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f, frame(a, dx * 0.5),
+                                         frame(a, dx), dx, tol, a->P, a->cflags, a->fflags)));
     return COSMOS_OK;
 }
 
@@ -560,18 +648,20 @@ int l2_refine(cosmos_amr* a, int lp, double dx, double tol, int only_flagged) {
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c, a->dim,
-                                         level_flags(a, lp), level_flags(a, lp + 1), dx, tol, only_flagged, a->P)));
+    AMRCHK(a, (launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
+                                         frame(a, level_spacing(dx, lp)), level_flags(a, lp), level_flags(a, lp + 1),
+                                         dx, tol, only_flagged, a->P)));
     return COSMOS_OK;
 }
 
-int l2_exchange(cosmos_amr* a, int lp, int strict) {
+int l2_exchange(cosmos_amr* a, int lp, double dx, int strict) {
     Grid p, c;
     int rc = grid_of(a, level_solver(a, lp), p);
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<L2ExchangeK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c, a->dim,
+    AMRCHK(a, (launch_physics<L2ExchangeK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
+                                           frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
                                            (const unsigned char*)level_flags(a, lp),
                                            (const unsigned char*)level_flags(a, lp + 1), strict, a->P)));
     return COSMOS_OK;
@@ -583,7 +673,8 @@ int l2_derefine(cosmos_amr* a, int lp, double dx, double tol) {
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c, a->dim,
+    AMRCHK(a, (launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
+                                           frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
                                            level_flags(a, lp), level_flags(a, lp + 1), dx, tol, a->P)));
     return COSMOS_OK;
 }
@@ -595,8 +686,9 @@ extern "C" {
 int cosmos_amr_create(cosmos_amr** out, int physics, int dim, int nx, int ny, double adiabatic_index) {
     if (!out) return amr_fail(nullptr, COSMOS_ERR_ARGUMENT, "out is null");
     *out = nullptr;
-    if (physics != COSMOS_NEWTONIAN && physics != COSMOS_RELATIVISTIC)
-        return amr_fail(nullptr, COSMOS_ERR_ARGUMENT, "AMR physics must be COSMOS_NEWTONIAN or COSMOS_RELATIVISTIC");
+    if (physics != COSMOS_NEWTONIAN && physics != COSMOS_RELATIVISTIC && physics != COSMOS_BLACKHOLE)
+        return amr_fail(nullptr, COSMOS_ERR_ARGUMENT,
+                        "AMR physics must be COSMOS_NEWTONIAN, COSMOS_RELATIVISTIC or COSMOS_BLACKHOLE");
     if (dim != 1 && dim != 2) return amr_fail(nullptr, COSMOS_ERR_ARGUMENT, "dim must be 1 or 2");
     if (dim == 1) ny = 1;
     if (nx < 2 || ny < 1 || (dim == 2 && ny < 2)) return amr_fail(nullptr, COSMOS_ERR_ARGUMENT, "grid too small");
@@ -651,6 +743,34 @@ int cosmos_amr_set_con2prim(cosmos_amr* a, double tolerance, int max_iterations)
             return from_solver(a, a->fine2, rc);
         a->P.c2p_tol = tolerance;
         a->P.c2p_maxit = max_iterations;
+        if (a->physics == COSMOS_BLACKHOLE) a->P.gr_tol = tolerance;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_amr_set_spacetime(cosmos_amr* a, double mass, double spin, double x, double y, double z) {
+    return amr_guarded(a, [&] {
+        if (a->physics != COSMOS_BLACKHOLE)
+            return amr_fail(a, COSMOS_ERR_ARGUMENT, "cosmos_amr_set_spacetime needs a COSMOS_BLACKHOLE context");
+        for (cosmos_solver* s : {a->coarse, a->fine, a->fine2}) {
+            int rc = s ? cosmos_set_spacetime(s, mass, spin, x, y, z) : COSMOS_OK;
+            if (rc) return from_solver(a, s, rc);
+        }
+        a->hole = cfv::make_hole(mass, spin, x, y, z, a->excision);
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_amr_set_excision(cosmos_amr* a, double radius) {
+    return amr_guarded(a, [&] {
+        if (a->physics != COSMOS_BLACKHOLE)
+            return amr_fail(a, COSMOS_ERR_ARGUMENT, "cosmos_amr_set_excision needs a COSMOS_BLACKHOLE context");
+        for (cosmos_solver* s : {a->coarse, a->fine, a->fine2}) {
+            int rc = s ? cosmos_set_excision(s, radius) : COSMOS_OK;
+            if (rc) return from_solver(a, s, rc);
+        }
+        a->excision = radius > 0.0 ? radius : 0.0;
+        a->hole = cfv::make_hole(a->hole.M, a->hole.a, a->hole.x, a->hole.y, a->hole.z, a->excision);
         return COSMOS_OK;
     });
 }
@@ -707,7 +827,10 @@ int cosmos_amr_set_flags(cosmos_amr* a, const unsigned char* coarse_flags, const
 }
 
 int cosmos_amr_refine_flags(cosmos_amr* a, double cell_spacing, double tolerance) {
-    return amr_guarded(a, [&] { return amr_refine(a, cell_spacing, tolerance); });
+    return amr_guarded(a, [&] {
+        a->dx = cell_spacing;
+        return amr_refine(a, cell_spacing, tolerance);
+    });
 }
 
 int cosmos_amr_sweep(cosmos_amr* a, int level, int order, int axis, double cell_spacing, double time_step,
@@ -719,11 +842,19 @@ int cosmos_amr_sweep(cosmos_amr* a, int level, int order, int axis, double cell_
 }
 
 int cosmos_amr_restrict(cosmos_amr* a) {
-    return amr_guarded(a, [&] { return amr_restrict(a); });
+    return amr_guarded(a, [&] {
+        if (a->physics == COSMOS_BLACKHOLE && !(a->dx > 0.0))
+            return amr_fail(a, COSMOS_ERR_STATE,
+                            "black-hole restriction needs the coarse cell spacing: call cosmos_amr_refine_flags first");
+        return amr_restrict(a, a->dx);
+    });
 }
 
 int cosmos_amr_derefine(cosmos_amr* a, double cell_spacing, double tolerance) {
-    return amr_guarded(a, [&] { return amr_derefine(a, cell_spacing, tolerance); });
+    return amr_guarded(a, [&] {
+        a->dx = cell_spacing;
+        return amr_derefine(a, cell_spacing, tolerance);
+    });
 }
 
 int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
@@ -731,6 +862,7 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
     return amr_guarded(a, [&] {
         if (order != 1 && order != 2) return amr_fail(a, COSMOS_ERR_ARGUMENT, "order must be 1 or 2");
         const int cap = max_coarse_steps < 0 ? INT_MAX : max_coarse_steps;
+        a->dx = cell_spacing;
         double t = 0.0;
         int it = 0, fine_total = 0, rc = COSMOS_OK;
         while (t < final_time && it < cap) {
@@ -739,8 +871,10 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
             if (rc) return from_solver(a, a->coarse, rc);
             if ((rc = amr_refine(a, cell_spacing, tolerance))) return rc;
             if ((rc = amr_flux_step(a, 0, order, cell_spacing, cdt))) return rc;
-            bool refined = false;
-            if ((rc = amr_any_refined(a, refined))) return rc;
+            // the 2D black-hole driver starts its "refined" test at true
+            // (AMR/BlackHoleEulerAMR.cpp:533): its fine level always sub-steps
+            bool refined = a->physics == COSMOS_BLACKHOLE && a->dim == 2;
+            if (!refined && (rc = amr_any_refined(a, refined))) return rc;
             if (refined) {
                 // fine sub-steps to the coarse time, less 1e-4 (pow(10.0, -4.0) in the reference)
                 double ft = t;
@@ -757,7 +891,7 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
             }
             t += cdt;
             it += 1;
-            if ((rc = amr_restrict(a))) return rc;
+            if ((rc = amr_restrict(a, cell_spacing))) return rc;
             if ((rc = amr_derefine(a, cell_spacing, tolerance))) return rc;
         }
         order_streams(a);
@@ -775,14 +909,24 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out) {
     return amr_guarded(a, [&] {
         if (order != 1 && order != 2) return amr_fail(a, COSMOS_ERR_ARGUMENT, "order must be 1 or 2");
+        if (a->physics == COSMOS_BLACKHOLE && a->dim == 2)
+            return amr_fail(a, COSMOS_ERR_ARGUMENT,
+                            "no 2D black-hole three-level driver (the reference's solve2DLevel2AMR is truncated, "
+                            "AMR/BlackHoleEulerAMR.cpp:1319-1325)");
         int rc = COSMOS_OK;
+        a->dx = cell_spacing;
         const long nf2 = 4L * a->nx * (a->dim == 1 ? 1 : 4L * a->ny);
         if (!a->fine2) {
             rc = cosmos_create(&a->fine2, a->physics, a->dim, 4 * a->nx, a->dim == 1 ? 1 : 4 * a->ny, a->P.gamma);
             if (rc) return amr_fail(a, rc, std::string("AMR setup: ") + cosmos_last_error(nullptr));
             if (a->P.c2p_tol != cfv::make_params(a->P.gamma).c2p_tol ||
-                a->P.c2p_maxit != cfv::make_params(a->P.gamma).c2p_maxit) {
+                a->P.c2p_maxit != cfv::make_params(a->P.gamma).c2p_maxit || a->P.gr_tol > 0.0) {
                 rc = cosmos_set_con2prim(a->fine2, a->P.c2p_tol, a->P.c2p_maxit);
+                if (rc) return from_solver(a, a->fine2, rc);
+            }
+            if (a->physics == COSMOS_BLACKHOLE) {
+                rc = cosmos_set_spacetime(a->fine2, a->hole.M, a->hole.a, a->hole.x, a->hole.y, a->hole.z);
+                if (!rc) rc = cosmos_set_excision(a->fine2, a->excision);
                 if (rc) return from_solver(a, a->fine2, rc);
             }
             AMRCHK(a, hipMalloc((void**)&a->f2flags, (size_t)nf2));
@@ -847,7 +991,7 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
                     }
                     mt += mdt;
                     fine_total += 1;
-                    if ((rc = l2_exchange(a, 1, 1))) return rc;
+                    if ((rc = l2_exchange(a, 1, cell_spacing, 1))) return rc;
                     if ((rc = l2_derefine(a, 1, cell_spacing, tolerance))) return rc;
                 }
             } else {
@@ -855,7 +999,7 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
             }
             t += cdt;
             it += 1;
-            if ((rc = l2_exchange(a, 0, 0))) return rc;
+            if ((rc = l2_exchange(a, 0, cell_spacing, 0))) return rc;
             if ((rc = l2_derefine(a, 0, cell_spacing, dtol))) return rc;
         }
         order_streams(a);

@@ -86,20 +86,7 @@ int fail(cosmos_solver* s, int code, const std::string& msg) {
     } while (0)
 
 
-// BlackHoleSpacetime(M, a, x, y, z) and the constants its accessors form
-// (BlackHoleEuler/BlackHoleSpacetime.cpp:54-113, 372-386).
-cfv::Hole make_hole(double M, double a, double x, double y, double z, double excision) {
-    cfv::Hole b{};
-    b.M = M;
-    b.a = a;
-    b.x = x;
-    b.y = y;
-    b.z = z;
-    b.ma = (M * M) * (a * a);
-    b.aM = a * M;
-    b.r_exc = excision > 0.0 ? excision : M * (1.0 + std::sqrt(1.0 - (a * a)));
-    return b;
-}
+using cfv::make_hole;
 
 Planes planes_of(cosmos_solver* s, int b) {
     Planes p;

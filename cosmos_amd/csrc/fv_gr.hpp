@@ -32,6 +32,22 @@ struct Hole {
     double r_exc;  // M * (1.0 + sqrt(1.0 - (a * a))), the excision radius (:372-386)
 };
 
+// BlackHoleSpacetime(M, a, x, y, z) and the constants its accessors form
+// (BlackHoleEuler/BlackHoleSpacetime.cpp:54-113, 372-386); a positive
+// excision radius replaces the horizon radius.
+inline Hole make_hole(double M, double a, double x, double y, double z, double excision) {
+    Hole b{};
+    b.M = M;
+    b.a = a;
+    b.x = x;
+    b.y = y;
+    b.z = z;
+    b.ma = (M * M) * (a * a);
+    b.aM = a * M;
+    b.r_exc = excision > 0.0 ? excision : M * (1.0 + __builtin_sqrt(1.0 - (a * a)));
+    return b;
+}
+
 // Kerr-Schild radius, scalar H and spatial null vector l at one point
 // (computeKerrSchildScalar :54-68, computeKerrSchildVector :70-90).
 struct KSPoint {

@@ -490,14 +490,19 @@ GRGrid BlackHoleEulerSecondOrderSolver::solve2D(GRGrid c, double dx, double cfl,
     return c;
 }
 
-// --- AMR (EulerAMR, RelativisticEulerAMR) --------------------------------------
+// --- AMR (EulerAMR, RelativisticEulerAMR, BlackHoleEulerAMR) --------------------
 namespace {
 
 struct AmrHandle {
     cosmos_amr* a = nullptr;
-    AmrHandle(int physics, int dim, int nx, int ny, double gamma) {
+    // bh: the black hole of a COSMOS_BLACKHOLE context
+    AmrHandle(int physics, int dim, int nx, int ny, double gamma, BlackHoleSpacetime* bh = nullptr) {
         if (cosmos_amr_create(&a, physics, dim, nx, ny, gamma) != COSMOS_OK)
             throw std::runtime_error(std::string("cosmos: ") + cosmos_amr_last_error(nullptr));
+        if (bh)
+            check(cosmos_amr_set_spacetime(a, bh->getBlackHoleMass(), bh->getBlackHoleSpin(),
+                                           bh->getBlackHoleXPosition(), bh->getBlackHoleYPosition(),
+                                           bh->getBlackHoleZPosition()));
     }
     ~AmrHandle() { cosmos_amr_destroy(a); }
     void check(int rc) const {
@@ -535,10 +540,10 @@ vector<vector<bool> > bools(const vector<unsigned char>& f, int nx, int ny) {
 // as its flags (AMR/EulerAMR.cpp:3-167).
 template <class S>
 void masked_sweep_1d(int physics, int order, vector<S>& cells, double dx, double dt, double bias, int lim,
-                     const vector<bool>& mask, double gamma) {
+                     const vector<bool>& mask, double gamma, BlackHoleSpacetime* bh = nullptr) {
     const int n = (int)cells.size();
     if ((int)mask.size() != n) throw std::invalid_argument("cosmos: AMRStructure size differs from the cells'");
-    AmrHandle h(physics, 1, n, 1, gamma);
+    AmrHandle h(physics, 1, n, 1, gamma, bh);
     vector<double> buf(5 * (size_t)n);
     pack(cells, buf.data());
     h.check(cosmos_amr_upload(h.a, buf.data()));
@@ -551,7 +556,7 @@ void masked_sweep_1d(int physics, int order, vector<S>& cells, double dx, double
 
 template <class S>
 void masked_sweep_2d(int physics, int order, int axis, vector<vector<S> >& cells, double dx, double dt, double bias,
-                     int lim, const vector<vector<bool> >& mask, double gamma) {
+                     int lim, const vector<vector<bool> >& mask, double gamma, BlackHoleSpacetime* bh = nullptr) {
     int nx, ny;
     vector<double> buf;
     pack_grid(cells, nx, ny, buf);
@@ -561,7 +566,7 @@ void masked_sweep_2d(int physics, int order, int axis, vector<vector<S> >& cells
         if ((int)mask[i].size() != nx) throw std::invalid_argument("cosmos: AMRStructure size differs from the cells'");
         for (int j = 0; j < nx; ++j) f[(size_t)i * nx + j] = mask[i][j] ? 1 : 0;
     }
-    AmrHandle h(physics, 2, nx, ny, gamma);
+    AmrHandle h(physics, 2, nx, ny, gamma, bh);
     h.check(cosmos_amr_upload(h.a, buf.data()));
     h.check(cosmos_amr_set_flags(h.a, f.data(), nullptr));
     h.check(cosmos_amr_sweep(h.a, 0, order, axis, dx, dt, bias, lim));
@@ -572,9 +577,9 @@ void masked_sweep_2d(int physics, int order, int axis, vector<vector<S> >& cells
 template <class S>
 tuple<vector<S>, vector<S>, vector<bool>, vector<bool> > level1_1d(int physics, vector<S>& cells, double dx,
                                                                     double cfl, double tf, double tol, int order,
-                                                                    double gamma) {
+                                                                    double gamma, BlackHoleSpacetime* bh = nullptr) {
     const int n = (int)cells.size();
-    AmrHandle h(physics, 1, n, 1, gamma);
+    AmrHandle h(physics, 1, n, 1, gamma, bh);
     vector<double> c(5 * (size_t)n), f(10 * (size_t)n);
     pack(cells, c.data());
     h.check(cosmos_amr_upload(h.a, c.data()));
@@ -589,11 +594,12 @@ tuple<vector<S>, vector<S>, vector<bool>, vector<bool> > level1_1d(int physics, 
 
 template <class S>
 tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >, vector<vector<bool> > > level1_2d(
-    int physics, vector<vector<S> >& cells, double dx, double cfl, double tf, double tol, int order, double gamma) {
+    int physics, vector<vector<S> >& cells, double dx, double cfl, double tf, double tol, int order, double gamma,
+    BlackHoleSpacetime* bh = nullptr) {
     int nx, ny;
     vector<double> c;
     pack_grid(cells, nx, ny, c);
-    AmrHandle h(physics, 2, nx, ny, gamma);
+    AmrHandle h(physics, 2, nx, ny, gamma, bh);
     h.check(cosmos_amr_upload(h.a, c.data()));
     h.check(cosmos_amr_solve_level1(h.a, order, dx, cfl, tf, tol, -1, nullptr, nullptr));
     vector<double> f(5 * (size_t)(4 * nx) * ny);
@@ -605,9 +611,10 @@ tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >, vector<vect
 
 template <class S>
 tuple<vector<S>, vector<S>, vector<S>, vector<bool>, vector<bool>, vector<bool> > level2_1d(
-    int physics, vector<S>& cells, double dx, double cfl, double tf, double tol, int order, double gamma) {
+    int physics, vector<S>& cells, double dx, double cfl, double tf, double tol, int order, double gamma,
+    BlackHoleSpacetime* bh = nullptr) {
     const int n = (int)cells.size();
-    AmrHandle h(physics, 1, n, 1, gamma);
+    AmrHandle h(physics, 1, n, 1, gamma, bh);
     vector<double> c(5 * (size_t)n), m(10 * (size_t)n), f(20 * (size_t)n);
     pack(cells, c.data());
     h.check(cosmos_amr_upload(h.a, c.data()));
@@ -626,11 +633,11 @@ template <class S>
 tuple<vector<vector<S> >, vector<vector<S> >, vector<vector<S> >, vector<vector<bool> >, vector<vector<bool> >,
       vector<vector<bool> > >
 level2_2d(int physics, vector<vector<S> >& cells, double dx, double cfl, double tf, double tol, int order,
-          double gamma) {
+          double gamma, BlackHoleSpacetime* bh = nullptr) {
     int nx, ny;
     vector<double> c;
     pack_grid(cells, nx, ny, c);
-    AmrHandle h(physics, 2, nx, ny, gamma);
+    AmrHandle h(physics, 2, nx, ny, gamma, bh);
     h.check(cosmos_amr_upload(h.a, c.data()));
     h.check(cosmos_amr_solve_level2(h.a, order, dx, cfl, tf, tol, -1, nullptr, nullptr));
     const size_t n = (size_t)nx * ny;
@@ -645,6 +652,7 @@ level2_2d(int physics, vector<vector<S> >& cells, double dx, double cfl, double 
 
 typedef EulerStateVector NS;
 typedef RelativisticEulerStateVector RS;
+typedef BlackHoleEulerStateVector BS;
 
 }  // namespace
 
@@ -734,4 +742,55 @@ tuple<vector<vector<RS> >, vector<vector<RS> >, vector<vector<RS> >, vector<vect
 RelativisticEulerAMR::solve2DLevel2AMR(vector<vector<RS> > c, double dx, double cfl, double tf, double tol, int order,
                                        EulerMaterialParameters mp) {
     return level2_2d(COSMOS_RELATIVISTIC, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex());
+}
+
+void BlackHoleEulerAMR::computeFORCETimeStepAMR(vector<BS>& c, double dx, double dt, vector<bool> m,
+                                               EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    masked_sweep_1d(COSMOS_BLACKHOLE, 1, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex(), &bh);
+}
+void BlackHoleEulerAMR::computeXFORCETimeStep2DAMR(vector<vector<BS> >& c, double dx, double dt,
+                                                  vector<vector<bool> > m, EulerMaterialParameters mp,
+                                                  BlackHoleSpacetime bh) {
+    masked_sweep_2d(COSMOS_BLACKHOLE, 1, 0, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex(), &bh);
+}
+void BlackHoleEulerAMR::computeYFORCETimeStep2DAMR(vector<vector<BS> >& c, double dx, double dt,
+                                                  vector<vector<bool> > m, EulerMaterialParameters mp,
+                                                  BlackHoleSpacetime bh) {
+    masked_sweep_2d(COSMOS_BLACKHOLE, 1, 1, c, dx, dt, 0.0, 1, m, mp.getAdiabaticIndex(), &bh);
+}
+void BlackHoleEulerAMR::computeSLICTimeStepAMR(vector<BS>& c, double dx, double dt, double bias, int lim,
+                                              vector<bool> m, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    masked_sweep_1d(COSMOS_BLACKHOLE, 2, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex(), &bh);
+}
+void BlackHoleEulerAMR::computeXSLICTimeStep2DAMR(vector<vector<BS> >& c, double dx, double dt, double bias, int lim,
+                                                 vector<vector<bool> > m, EulerMaterialParameters mp,
+                                                 BlackHoleSpacetime bh) {
+    masked_sweep_2d(COSMOS_BLACKHOLE, 2, 0, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex(), &bh);
+}
+void BlackHoleEulerAMR::computeYSLICTimeStep2DAMR(vector<vector<BS> >& c, double dx, double dt, double bias, int lim,
+                                                 vector<vector<bool> > m, EulerMaterialParameters mp,
+                                                 BlackHoleSpacetime bh) {
+    masked_sweep_2d(COSMOS_BLACKHOLE, 2, 1, c, dx, dt, bias, lim, m, mp.getAdiabaticIndex(), &bh);
+}
+tuple<vector<BS>, vector<BS>, vector<bool>, vector<bool> > BlackHoleEulerAMR::solveLevel1AMR(
+    vector<BS> c, double dx, double cfl, double tf, double tol, int order, EulerMaterialParameters mp,
+    BlackHoleSpacetime bh) {
+    return level1_1d(COSMOS_BLACKHOLE, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex(), &bh);
+}
+tuple<vector<vector<BS> >, vector<vector<BS> >, vector<vector<bool> >, vector<vector<bool> > >
+BlackHoleEulerAMR::solve2DLevel1AMR(vector<vector<BS> > c, double dx, double cfl, double tf, double tol, int order,
+                                    EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    return level1_2d(COSMOS_BLACKHOLE, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex(), &bh);
+}
+tuple<vector<BS>, vector<BS>, vector<BS>, vector<bool>, vector<bool>, vector<bool> > BlackHoleEulerAMR::solveLevel2AMR(
+    vector<BS> c, double dx, double cfl, double tf, double tol, int order, EulerMaterialParameters mp,
+    BlackHoleSpacetime bh) {
+    return level2_1d(COSMOS_BLACKHOLE, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex(), &bh);
+}
+tuple<vector<vector<BS> >, vector<vector<BS> >, vector<vector<BS> >, vector<vector<bool> >, vector<vector<bool> >,
+      vector<vector<bool> > >
+BlackHoleEulerAMR::solve2DLevel2AMR(vector<vector<BS> > c, double dx, double cfl, double tf, double tol, int order,
+                                    EulerMaterialParameters mp, BlackHoleSpacetime bh) {
+    // the library refuses it (std::runtime_error): the reference's definition is truncated
+    return level2_2d(COSMOS_BLACKHOLE, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex(), &bh);
 }

@@ -13,6 +13,9 @@
 //   SlopeLimiters (Solvers/SlopeLimiters.hpp:10-35)         limiter functions and slope vectors
 //   EulerAMR, RelativisticEulerAMR (AMR/EulerAMR.hpp:8-44,
 //     AMR/RelativisticEulerAMR.hpp)                         masked sweeps, level-1 and level-2 drivers
+//   BlackHoleEulerAMR (AMR/BlackHoleEulerAMR.hpp:8-53)      masked sweeps, level-1 drivers, 1D level 2
+//                                                           (solve2DLevel2AMR throws: the reference's
+//                                                           definition is truncated)
 //
 // The state types come from the host application's headers
 // (EulerStateVector.hpp, RelativisticEulerStateVector.hpp,
@@ -301,6 +304,50 @@ public:
     solve2DLevel2AMR(vector<vector<RelativisticEulerStateVector> > initialCells, double cellSpacing,
                      double CFLCoefficient, double finalTime, double AMRTolerance, int order,
                      EulerMaterialParameters materialParameters);
+};
+
+class BlackHoleEulerAMR {
+public:
+    static void computeFORCETimeStepAMR(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
+                                        double timeStep, vector<bool> AMRStructure,
+                                        EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeXFORCETimeStep2DAMR(vector<vector<BlackHoleEulerStateVector> >& currentCells,
+                                           double cellSpacing, double timeStep, vector<vector<bool> > AMRStructure,
+                                           EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeYFORCETimeStep2DAMR(vector<vector<BlackHoleEulerStateVector> >& currentCells,
+                                           double cellSpacing, double timeStep, vector<vector<bool> > AMRStructure,
+                                           EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeSLICTimeStepAMR(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
+                                       double timeStep, double bias, int slopeLimiter, vector<bool> AMRStructure,
+                                       EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeXSLICTimeStep2DAMR(vector<vector<BlackHoleEulerStateVector> >& currentCells,
+                                          double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                          vector<vector<bool> > AMRStructure,
+                                          EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static void computeYSLICTimeStep2DAMR(vector<vector<BlackHoleEulerStateVector> >& currentCells,
+                                          double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                          vector<vector<bool> > AMRStructure,
+                                          EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static tuple<vector<BlackHoleEulerStateVector>, vector<BlackHoleEulerStateVector>, vector<bool>, vector<bool> >
+    solveLevel1AMR(vector<BlackHoleEulerStateVector> initialCells, double cellSpacing, double CFLCoefficient,
+                   double finalTime, double AMRTolerance, int order, EulerMaterialParameters materialParameters,
+                   BlackHoleSpacetime blackHole);
+    static tuple<vector<vector<BlackHoleEulerStateVector> >, vector<vector<BlackHoleEulerStateVector> >,
+                 vector<vector<bool> >, vector<vector<bool> > >
+    solve2DLevel1AMR(vector<vector<BlackHoleEulerStateVector> > initialCells, double cellSpacing,
+                     double CFLCoefficient, double finalTime, double AMRTolerance, int order,
+                     EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static tuple<vector<BlackHoleEulerStateVector>, vector<BlackHoleEulerStateVector>, vector<BlackHoleEulerStateVector>,
+                 vector<bool>, vector<bool>, vector<bool> >
+    solveLevel2AMR(vector<BlackHoleEulerStateVector> initialCells, double cellSpacing, double CFLCoefficient,
+                   double finalTime, double AMRTolerance, int order, EulerMaterialParameters materialParameters,
+                   BlackHoleSpacetime blackHole);
+    static tuple<vector<vector<BlackHoleEulerStateVector> >, vector<vector<BlackHoleEulerStateVector> >,
+                 vector<vector<BlackHoleEulerStateVector> >, vector<vector<bool> >, vector<vector<bool> >,
+                 vector<vector<bool> > >
+    solve2DLevel2AMR(vector<vector<BlackHoleEulerStateVector> > initialCells, double cellSpacing,
+                     double CFLCoefficient, double finalTime, double AMRTolerance, int order,
+                     EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
 };
 
 #endif

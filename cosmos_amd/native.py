@@ -97,6 +97,8 @@ SIGNATURES = {
     "cosmos_amr_sweep": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int]),
     "cosmos_amr_restrict": (ctypes.c_int, [_handle]),
+    "cosmos_amr_set_spacetime": (ctypes.c_int, [_handle] + [ctypes.c_double] * 5),
+    "cosmos_amr_set_excision": (ctypes.c_int, [_handle, ctypes.c_double]),
     "cosmos_amr_derefine": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double]),
     "cosmos_amr_solve_level1": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,

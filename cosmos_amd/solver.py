@@ -203,8 +203,9 @@ class Solver:
 
 
 class Amr:
-    """Two-level AMR context (cosmos_amr_*): a coarse grid, a fine grid of twice
-    the resolution and one refinement flag per cell of each."""
+    """AMR context (cosmos_amr_*): a coarse grid, a fine grid of twice the
+    resolution (and, after solve_level2, one of four times) and one refinement
+    flag per cell of each; Newtonian, special-relativistic or black-hole."""
 
     def __init__(self, physics: int, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4):
         self._lib = native.load()
@@ -237,6 +238,13 @@ class Amr:
 
     def set_con2prim(self, tolerance: float, max_iterations: int) -> None:
         self._check(self._lib.cosmos_amr_set_con2prim(self._h, tolerance, max_iterations))
+
+    def set_spacetime(self, mass: float, spin: float, x: float, y: float, z: float = 0.0) -> None:
+        """Black-hole contexts: the Kerr-Schild hole of every level."""
+        self._check(self._lib.cosmos_amr_set_spacetime(self._h, mass, spin, x, y, z))
+
+    def set_excision(self, radius: float) -> None:
+        self._check(self._lib.cosmos_amr_set_excision(self._h, radius))
 
     def upload(self, prim: np.ndarray) -> None:
         a = np.ascontiguousarray(prim, dtype=np.float64)

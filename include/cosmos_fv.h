@@ -232,6 +232,13 @@ int cosmos_amr_destroy(cosmos_amr* a);
 const char* cosmos_amr_last_error(const cosmos_amr* a);
 /* as cosmos_set_con2prim, for both levels */
 int cosmos_amr_set_con2prim(cosmos_amr* a, double tolerance, int max_iterations);
+/* COSMOS_BLACKHOLE contexts: as cosmos_set_spacetime / cosmos_set_excision,
+ * for every level (the blackHole argument of BlackHoleEulerAMR's drivers,
+ * AMR/BlackHoleEulerAMR.cpp:239-240).  The flag, restriction and
+ * derefinement passes convert each state at the cell coordinate the
+ * reference's call site names (index times the level's spacing, 1D y = 0). */
+int cosmos_amr_set_spacetime(cosmos_amr* a, double mass, double spin, double x, double y, double z);
+int cosmos_amr_set_excision(cosmos_amr* a, double radius);
 /* The coarse state (AoS primitives); the fine grid becomes a copy of each coarse
  * cell in its children, coarse flags all set, fine flags all clear
  * (the set-up of solve{,2D}Level1AMR, AMR/EulerAMR.cpp:174-200, 356-392). */
@@ -256,17 +263,22 @@ int cosmos_amr_sweep(cosmos_amr* a, int level, int order, int axis, double cell_
                      double bias, int slope_limiter);
 /* Coarse cells whose flag is clear take the mean of their children's conserved
  * vectors; fine cells whose flag is clear take a copy of their parent
- * (AMR/EulerAMR.cpp:282-308, 494-536). */
+ * (AMR/EulerAMR.cpp:282-308, 494-536).  Black-hole contexts place the cells
+ * with the coarse spacing last passed to cosmos_amr_refine_flags,
+ * cosmos_amr_derefine or a solve call (COSMOS_ERR_STATE before any). */
 int cosmos_amr_restrict(cosmos_amr* a);
 /* Derefinement: a refined coarse cell whose centre child (1D: child 2j+1; 2D:
  * (2i+1, 2j+1)) jumps by less than tolerance / 2 over cell_spacing to each fine
  * neighbour is made coarse again and restricted (AMR/EulerAMR.cpp:310-345, 538-586). */
 int cosmos_amr_derefine(cosmos_amr* a, double cell_spacing, double tolerance);
-/* EulerAMR::solveLevel1AMR / solve2DLevel1AMR and the RelativisticEulerAMR copies
- * (AMR/EulerAMR.cpp:169-590): per coarse step the CFL step (damped by the
- * fine-step counter), the refinement flags, the masked coarse flux step, masked
- * fine sub-steps to the coarse time (less 1e-4), restriction and
- * derefinement; bias 0 and van Leer, as the reference fixes them.
+/* EulerAMR::solveLevel1AMR / solve2DLevel1AMR and the RelativisticEulerAMR and
+ * BlackHoleEulerAMR copies (AMR/EulerAMR.cpp:169-590,
+ * AMR/BlackHoleEulerAMR.cpp:238-694): per coarse step the CFL step (damped by
+ * the fine-step counter), the refinement flags, the masked coarse flux step,
+ * masked fine sub-steps to the coarse time (less 1e-4; the 2D black-hole
+ * driver sub-steps even with no fine flag set, :533), restriction and
+ * derefinement; bias 0 and van Leer, as the reference fixes them; no
+ * black-hole forcing (the reference's AMR drivers call none).
  * max_coarse_steps < 0 means no cap. */
 int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
@@ -280,7 +292,10 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
  * injection, fine derefinement) to the coarse time less 1e-4, then coarse
  * restriction / injection and intermediate derefinement at
  * tolerance * 0.5 * (1 / order); every test over the coarse spacing, every dt
- * damped by the reference's fine-step counter. */
+ * damped by the reference's fine-step counter.  Black hole: 1D only
+ * (BlackHoleEulerAMR::solveLevel2AMR, AMR/BlackHoleEulerAMR.cpp:695-1055; its
+ * 2D driver is truncated in the reference, :1319-1325), COSMOS_ERR_ARGUMENT
+ * in 2D. */
 int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
 /* The three levels after cosmos_amr_solve_level2 (any pointer may be null),

@@ -24,8 +24,17 @@
 //      (8x10x5, 16x20x5, 32x40x5, 80, 320, 1280)
 //  14. EulerAMR::computeXSLICTimeStep2DAMR on the input of 3 with a
 //      checkerboard AMRStructure, dt 0.001                                   (20 x 24 x 5)
+//  15. BlackHoleEulerAMR::solveLevel1AMR, the 1D accretion set-up of
+//      Tests/BlackHoleEulerAMRTests.cpp at 40 cells, tol 0.5, CFL 0.95,
+//      t = 0.05, SLIC: coarse, fine, both flag grids                        (40x5, 80x5, 40, 80)
+//  16. BlackHoleEulerAMR::solve2DLevel1AMR, the spinning spheroidal set-up at
+//      12 x 12, tol 0.5, t = 0.02, FORCE                                     (12x12x5, 24x24x5, 144, 576)
+//  17. BlackHoleEulerAMR::solveLevel2AMR, the spinning 1D set-up at 32 cells,
+//      tol 0.5, t = 0.03, FORCE: three levels and their flags               (32x5, 64x5, 128x5, 32, 64, 128)
+//  18. BlackHoleEulerAMR::solve2DLevel2AMR throws (1.0 written if it did)   (1)
 #include <cmath>
 #include <cstdio>
+#include <stdexcept>
 #include <tuple>
 #include <vector>
 
@@ -170,6 +179,56 @@ int main(int argc, char** argv) {
             }
         EulerAMR::computeXSLICTimeStep2DAMR(w, 1.0 / mx, 0.001, 0.0, 1, mask, mp);
         for (auto& row : w) put_cells(f, row);
+    }
+    {
+        EulerMaterialParameters mpg(5.0 / 3.0);
+        const BlackHoleEulerStateVector dense(10.0, 0.0, 0.0, 0.0, 0.1), thin(0.1, 0.0, 0.0, 0.0, 0.1);
+        int n = 40;
+        vector<BlackHoleEulerStateVector> line(n);
+        for (int i = 0; i < n; ++i) line[i] = (i >= (0.1 * n) && i <= (0.9 * n)) ? dense : thin;
+        auto r = BlackHoleEulerAMR::solveLevel1AMR(line, 1.0 / n, 0.95, 0.05, 0.5, 2, mpg,
+                                                   BlackHoleSpacetime(0.05, 0.5, 0.0, 0.0));
+        put_cells(f, std::get<0>(r));
+        put_cells(f, std::get<1>(r));
+        for (bool b : std::get<2>(r)) put(f, b ? 1.0 : 0.0);
+        for (bool b : std::get<3>(r)) put(f, b ? 1.0 : 0.0);
+
+        const int m = 12;
+        vector<vector<BlackHoleEulerStateVector> > ball(m, vector<BlackHoleEulerStateVector>(m));
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j)
+                ball[i][j] = std::sqrt(((i - (0.5 * m)) * (i - (0.5 * m)) * 0.8) +
+                                       ((j - (0.5 * m)) * (j - (0.5 * m)) * 1.2)) <= (0.4 * m)
+                                 ? dense
+                                 : thin;
+        BlackHoleSpacetime spinning(0.1, 0.99, 0.5, 0.5, 0.0);
+        auto r2 = BlackHoleEulerAMR::solve2DLevel1AMR(ball, 1.0 / m, 0.95, 0.02, 0.5, 1, mpg, spinning);
+        for (auto& row : std::get<0>(r2)) put_cells(f, row);
+        for (auto& row : std::get<1>(r2)) put_cells(f, row);
+        for (auto& row : std::get<2>(r2))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+        for (auto& row : std::get<3>(r2))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+
+        n = 32;
+        line.assign(n, thin);
+        for (int i = 0; i < n; ++i) line[i] = (i >= (0.1 * n) && i <= (0.9 * n)) ? dense : thin;
+        auto r3 = BlackHoleEulerAMR::solveLevel2AMR(line, 1.0 / n, 0.95, 0.03, 0.5, 1, mpg,
+                                                    BlackHoleSpacetime(0.1, 0.99, 0.5, 0.0, 0.0));
+        put_cells(f, std::get<0>(r3));
+        put_cells(f, std::get<1>(r3));
+        put_cells(f, std::get<2>(r3));
+        for (bool b : std::get<3>(r3)) put(f, b ? 1.0 : 0.0);
+        for (bool b : std::get<4>(r3)) put(f, b ? 1.0 : 0.0);
+        for (bool b : std::get<5>(r3)) put(f, b ? 1.0 : 0.0);
+
+        double threw = 0.0;
+        try {
+            BlackHoleEulerAMR::solve2DLevel2AMR(ball, 1.0 / m, 0.95, 0.02, 0.5, 1, mpg, spinning);
+        } catch (const std::runtime_error&) {
+            threw = 1.0;
+        }
+        put(f, threw);
     }
     fclose(f);
     return 0;

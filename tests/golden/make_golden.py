@@ -4,7 +4,8 @@ sources, compiled by oracle/Makefile into oracle/_ref/libcosmos_ref.so.
 Run in a container that has the reference checked out (it is not needed
 afterwards; the fixtures are committed):
     make -C oracle ref && python tests/golden/make_golden.py
-    (python tests/golden/make_golden.py amr_level2 writes only amr_level2.npz)
+    (python tests/golden/make_golden.py amr_level2 writes only amr_level2.npz,
+     ... gr_amr only gr_amr.npz)
 
 Every fixture stores inputs and expected outputs as raw float64 arrays.
 """
@@ -68,12 +69,47 @@ def amr_level2_fixtures(ref) -> None:
     np.savez_compressed(os.path.join(OUT, "amr_level2.npz"), **fx)
 
 
+def gr_amr_cases():
+    """(tag, levels, order, scenario, tolerance, final time) of the black-hole AMR
+    fixtures: the set-ups of Tests/BlackHoleEulerAMRTests.cpp (the black-hole
+    scenario states, gamma 5/3, CFL 0.95) at small sizes and short times."""
+    from cosmos_amd import scenarios as sc
+
+    return [("l1_1d_slic", 1, 2, sc.bh_spherical_1d(40), 0.5, 0.05),
+            ("l1_1d_force_spin", 1, 1, sc.bh_spinning_1d(40), 0.5, 0.05),
+            ("l1_2d_slic", 1, 2, sc.bh_spherical_2d(16), 0.5, 0.03),
+            ("l1_2d_force_spin", 1, 1, sc.bh_spinning_spheroidal_2d(16), 0.5, 0.03),
+            ("l2_1d_slic", 2, 2, sc.bh_spherical_1d(64), 0.2, 0.08),
+            ("l2_1d_force_spin", 2, 1, sc.bh_spinning_1d(40), 0.5, 0.05)]
+
+
+GR_AMR_NAMES = {1: ("coarse", "fine", "cflags", "fflags"),
+                2: ("coarse", "inter", "fine", "cflags", "iflags", "fflags")}
+
+
+def gr_amr_fixtures(ref) -> None:
+    """Black-hole AMR: the reference's BlackHoleEulerAMR::solveLevel1AMR,
+    solve2DLevel1AMR and 1D solveLevel2AMR, every level and its flags."""
+    fx = {}
+    for tag, levels, order, S, tol, tf in gr_amr_cases():
+        run = ref.gr_amr_level1 if levels == 1 else ref.gr_amr_level2
+        out = run(S.W, order, S.cell_spacing, S.cfl, tf, tol, S.hole, S.gamma)
+        fx[f"{tag}_input"] = S.W
+        fx[f"{tag}_meta"] = np.array([levels, order, tol, tf, S.cfl, S.gamma, *S.hole])
+        for name, v in zip(GR_AMR_NAMES[levels], out):
+            fx[f"{tag}_{name}"] = v
+    np.savez_compressed(os.path.join(OUT, "gr_amr.npz"), **fx)
+
+
 def main() -> None:
     ref = load_ref()
     if ref is None:
         raise SystemExit("oracle/_ref/libcosmos_ref.so is not built (make -C oracle ref)")
     if sys.argv[1:] == ["amr_level2"]:
         amr_level2_fixtures(ref)
+        return
+    if sys.argv[1:] == ["gr_amr"]:
+        gr_amr_fixtures(ref)
         return
 
     # 1. Toro test 1 (Sod), 400 cells, SLIC + van Leer, CFL 0.8, to t = 0.25 through the
@@ -197,6 +233,7 @@ def main() -> None:
                    f"{tag}_fflags": ff, f"{tag}_meta": np.array([physics, order, tol, gamma])})
     np.savez_compressed(os.path.join(OUT, "amr.npz"), **fx)
     amr_level2_fixtures(ref)
+    gr_amr_fixtures(ref)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

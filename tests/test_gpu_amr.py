@@ -126,3 +126,78 @@ def test_amr_level2_matches_golden():
             got = a.download_level2()
         for k, name in enumerate(LEVEL2_NAMES):
             assert same(got[k], g[f"{tag}_{name}"]), (tag, name)
+
+
+# ---- black-hole AMR (BlackHoleEulerAMR::solve{,2D}Level1AMR, 1D solveLevel2AMR) ----
+
+from cosmos_amd import BLACKHOLE  # noqa: E402
+from test_oracle_gr_amr import GR_AMR_NAMES, gr_amr_cases  # noqa: E402
+
+
+def gr_amr_gpu(W, levels, order, dx, cfl, tf, tol, hole, gamma, excision=0.0):
+    dim = 1 if W.ndim == 2 else 2
+    with Amr(BLACKHOLE, dim, W.shape[-2], W.shape[0] if dim == 2 else 1, gamma) as a:
+        a.set_spacetime(*hole)
+        if excision:
+            a.set_excision(excision)
+        a.upload(W)
+        if levels == 1:
+            t, k = a.solve_level1(order, dx, cfl, tf, tol)
+            return a.download(), t, k
+        t, k = a.solve_level2(order, dx, cfl, tf, tol)
+        return a.download_level2(), t, k
+
+
+@pytest.mark.parametrize("case", range(len(gr_amr_cases())))
+def test_gr_amr_bitwise(orc, case):
+    tag, levels, order, S, tol, tf = gr_amr_cases()[case]
+    f = orc.gr_amr_level1 if levels == 1 else orc.gr_amr_level2
+    exp = f(S.W, order, S.cell_spacing, S.cfl, tf, tol, S.hole, S.gamma)
+    print(f"black-hole AMR {tag}: levels {levels} order {order} shape {S.W.shape}", flush=True)
+    got, t, k = gr_amr_gpu(S.W, levels, order, S.cell_spacing, S.cfl, tf, tol, S.hole, S.gamma)
+    for i, name in enumerate(GR_AMR_NAMES[levels]):
+        assert same(got[i], exp[i]), (tag, name)
+    assert (t, k) == (exp[-2], exp[-1])
+
+
+def test_gr_amr_matches_golden():
+    g = golden("gr_amr.npz")
+    for tag, *_ in gr_amr_cases():
+        lv, order, tol, tf, cfl, gamma, *hole = g[f"{tag}_meta"]
+        W = g[f"{tag}_input"]
+        got, _, _ = gr_amr_gpu(W, int(lv), int(order), 1.0 / W.shape[-2], cfl, tf, tol, tuple(hole), gamma)
+        for i, name in enumerate(GR_AMR_NAMES[int(lv)]):
+            assert same(got[i], g[f"{tag}_{name}"]), (tag, name)
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_gr_amr_excision_radius(orc, levels):
+    """A set excision radius reaches every level's sweeps (the oracle's global
+    radius, reset afterwards)."""
+    from cosmos_amd import scenarios
+
+    S = scenarios.bh_spherical_1d(48)
+    f = orc.gr_amr_level1 if levels == 1 else orc.gr_amr_level2
+    try:
+        orc.gr_set_excision(0.12)
+        exp = f(S.W, 2, S.cell_spacing, S.cfl, 0.05, 0.3, S.hole, S.gamma)
+    finally:
+        orc.gr_set_excision(0.0)
+    plain = f(S.W, 2, S.cell_spacing, S.cfl, 0.05, 0.3, S.hole, S.gamma)
+    got, t, k = gr_amr_gpu(S.W, levels, 2, S.cell_spacing, S.cfl, 0.05, 0.3, S.hole, S.gamma, excision=0.12)
+    for i, name in enumerate(GR_AMR_NAMES[levels]):
+        assert same(got[i], exp[i]), name
+    assert not same(plain[0], exp[0])  # the radius changed the run
+
+
+def test_gr_amr_argument_errors():
+    with Amr(BLACKHOLE, 2, 8, 8, 5.0 / 3.0) as a:
+        a.upload(np.tile([1.0, 0.0, 0.0, 0.0, 1.0], (8, 8, 1)))
+        with pytest.raises(Exception, match="truncated"):
+            a.solve_level2(2, 1.0 / 8, 0.9, 0.01, 0.5)
+    with Amr(BLACKHOLE, 1, 16, 1, 5.0 / 3.0) as a:
+        with pytest.raises(Exception, match="cell spacing"):
+            a.restrict()
+    with Amr(0, 1, 16, 1, 1.4) as a:
+        with pytest.raises(Exception, match="COSMOS_BLACKHOLE"):
+            a.set_spacetime(1.0, 0.0, 0.5, 0.0)

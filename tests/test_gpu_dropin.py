@@ -97,4 +97,23 @@ def test_dropin_classes_match_oracle(orc, tmp_path):
     mask = (np.add.outer(np.arange(20), np.arange(24)) % 2) == 0
     e = orc.sweep(W, 0, 2, 0, 1 / 24, 0.001)
     assert same(take((20, 24, 5)), np.where(mask[..., None], e, W))
+
+    # black-hole AMR drop-in class
+    from cosmos_amd import scenarios
+
+    S = scenarios.bh_spherical_1d(40)
+    c, fn, cf, ff, _, _ = orc.gr_amr_level1(S.W, 2, 1 / 40, 0.95, 0.05, 0.5, S.hole, g)
+    assert same(take((40, 5)), c) and same(take((80, 5)), fn)
+    assert same(take((40,)), cf.astype(float)) and same(take((80,)), ff.astype(float))
+    S = scenarios.bh_spinning_spheroidal_2d(12)
+    c, fn, cf, ff, _, _ = orc.gr_amr_level1(S.W, 1, 1 / 12, 0.95, 0.02, 0.5, S.hole, g)
+    assert same(take((12, 12, 5)), c) and same(take((24, 24, 5)), fn)
+    assert same(take((12, 12)), cf.astype(float)) and same(take((24, 24)), ff.astype(float))
+    S = scenarios.bh_spinning_1d(32)
+    exp = orc.gr_amr_level2(S.W, 1, 1 / 32, 0.95, 0.03, 0.5, S.hole, g)
+    for k, n in enumerate((32, 64, 128)):
+        assert same(take((n, 5)), exp[k]), ("level", k)
+    for k, n in enumerate((32, 64, 128)):
+        assert same(take((n,)), exp[3 + k].astype(float)), ("flags", k)
+    assert take((1,))[0] == 1.0  # solve2DLevel2AMR refused
     assert pos == v.size
