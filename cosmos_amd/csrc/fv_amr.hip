@@ -2,8 +2,10 @@
 // injection and derefinement of EulerAMR::solve{,2D}Level{1,2}AMR
 // (AMR/EulerAMR.cpp:169-1381), the special-relativistic copies
 // (AMR/RelativisticEulerAMR.cpp) and the black-hole drivers
-// (AMR/BlackHoleEulerAMR.cpp:238-1055: both 1D and 2D level 1, 1D level 2,
-// states converted at each cell's Kerr-Schild coordinate), on gfx950.
+// (AMR/BlackHoleEulerAMR.cpp:238-1334: states converted at each cell's
+// Kerr-Schild coordinate; the 2D level-2 driver, truncated in the reference at
+// :1319-1325, completed as its 1D and special-relativistic counterparts
+// continue), on gfx950.
 //
 // A cosmos_amr context holds two solver handles of the public C-ABI (the
 // coarse nx x ny grid and the fine 2nx x 2ny grid) and the two flag grids as
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(kAmrThreads) void k_inject_all(Grid c, Grid f, int 
 }
 
 // Three-level AMR (EulerAMR::solve{,2D}Level2AMR, AMR/EulerAMR.cpp:593-1381;
-// the 1D BlackHoleEulerAMR::solveLevel2AMR, AMR/BlackHoleEulerAMR.cpp:695-1055):
+// BlackHoleEulerAMR::solve{,2D}Level2AMR, AMR/BlackHoleEulerAMR.cpp:695-1334):
 // the flag, exchange and derefinement passes between a parent level and its
 // child level (twice the cells along each axis), one thread per parent cell;
 // each thread writes only its own parent cell and children.  Every conversion
@@ -909,10 +911,6 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out) {
     return amr_guarded(a, [&] {
         if (order != 1 && order != 2) return amr_fail(a, COSMOS_ERR_ARGUMENT, "order must be 1 or 2");
-        if (a->physics == COSMOS_BLACKHOLE && a->dim == 2)
-            return amr_fail(a, COSMOS_ERR_ARGUMENT,
-                            "no 2D black-hole three-level driver (the reference's solve2DLevel2AMR is truncated, "
-                            "AMR/BlackHoleEulerAMR.cpp:1319-1325)");
         int rc = COSMOS_OK;
         a->dx = cell_spacing;
         const long nf2 = 4L * a->nx * (a->dim == 1 ? 1 : 4L * a->ny);

@@ -791,6 +791,7 @@ tuple<vector<vector<BS> >, vector<vector<BS> >, vector<vector<BS> >, vector<vect
       vector<vector<bool> > >
 BlackHoleEulerAMR::solve2DLevel2AMR(vector<vector<BS> > c, double dx, double cfl, double tf, double tol, int order,
                                     EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    // the library refuses it (std::runtime_error): the reference's definition is truncated
+    // the reference's definition is truncated (AMR/BlackHoleEulerAMR.cpp:1319-1325);
+    // the library completes it as the 1D and special-relativistic drivers continue
     return level2_2d(COSMOS_BLACKHOLE, c, dx, cfl, tf, tol, order, mp.getAdiabaticIndex(), &bh);
 }

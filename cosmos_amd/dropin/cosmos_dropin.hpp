@@ -13,9 +13,9 @@
 //   SlopeLimiters (Solvers/SlopeLimiters.hpp:10-35)         limiter functions and slope vectors
 //   EulerAMR, RelativisticEulerAMR (AMR/EulerAMR.hpp:8-44,
 //     AMR/RelativisticEulerAMR.hpp)                         masked sweeps, level-1 and level-2 drivers
-//   BlackHoleEulerAMR (AMR/BlackHoleEulerAMR.hpp:8-53)      masked sweeps, level-1 drivers, 1D level 2
-//                                                           (solve2DLevel2AMR throws: the reference's
-//                                                           definition is truncated)
+//   BlackHoleEulerAMR (AMR/BlackHoleEulerAMR.hpp:8-53)      masked sweeps, level-1 and level-2 drivers
+//                                                           (solve2DLevel2AMR: the reference's definition
+//                                                           is truncated; completed as the 1D one goes on)
 //
 // The state types come from the host application's headers
 // (EulerStateVector.hpp, RelativisticEulerStateVector.hpp,

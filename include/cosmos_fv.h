@@ -292,10 +292,13 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
  * injection, fine derefinement) to the coarse time less 1e-4, then coarse
  * restriction / injection and intermediate derefinement at
  * tolerance * 0.5 * (1 / order); every test over the coarse spacing, every dt
- * damped by the reference's fine-step counter.  Black hole: 1D only
- * (BlackHoleEulerAMR::solveLevel2AMR, AMR/BlackHoleEulerAMR.cpp:695-1055; its
- * 2D driver is truncated in the reference, :1319-1325), COSMOS_ERR_ARGUMENT
- * in 2D. */
+ * damped by the reference's fine-step counter.  Black hole:
+ * BlackHoleEulerAMR::solveLevel2AMR (AMR/BlackHoleEulerAMR.cpp:695-1055) and
+ * solve2DLevel2AMR (:1057-1334), whose source is truncated at :1319-1325
+ * inside the intermediate/fine exchange; past that point the 2D run follows
+ * the 1D and special-relativistic drivers (strict exchange, fine
+ * derefinement, coarse exchange, intermediate derefinement, natural cell
+ * coordinates), a completion of this library's, not reference behaviour. */
 int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, double cfl, double final_time,
                             double tolerance, int max_coarse_steps, double* time_out, int* steps_out);
 /* The three levels after cosmos_amr_solve_level2 (any pointer may be null),

@@ -105,8 +105,8 @@ class CpuPath:
             self.lib.orc_gr_amr_level1.argtypes = [_I, _I, _P, _I, _I, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D,
                                                    _P, _P, u8, u8, ctypes.POINTER(_D)]
             self.lib.orc_gr_amr_level2.restype = _I
-            self.lib.orc_gr_amr_level2.argtypes = [_I, _P, _I, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D, _P, _P,
-                                                   _P, u8, u8, u8, ctypes.POINTER(_D)]
+            self.lib.orc_gr_amr_level2.argtypes = [_I, _I, _P, _I, _I, _D, _D, _D, _D, _I, _D, _D, _D, _D, _D, _D,
+                                                   _P, _P, _P, u8, u8, u8, ctypes.POINTER(_D)]
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
@@ -284,21 +284,23 @@ class CpuPath:
         return coarse, fine, cf, ff, t.value, n
 
     def gr_amr_level2(self, W, order, dx, cfl, tf, tol, hole, gamma=4.0 / 3.0, max_steps=-1):
-        """(coarse, inter, fine, flags x3[, t, steps]) of the 1D BlackHoleEulerAMR::solveLevel2AMR."""
+        """(coarse, inter, fine, flags x3[, t, steps]) of BlackHoleEulerAMR::solve{,2D}Level2AMR
+        (the reference build: 1D only, its 2D driver being truncated)."""
         a = np.ascontiguousarray(W, dtype=np.float64)
-        if a.ndim != 2:
-            raise ValueError("the three-level black-hole driver is 1D")
-        nx = a.shape[0]
-        coarse, inter, fine = np.empty_like(a), np.empty((2 * nx, 5)), np.empty((4 * nx, 5))
-        cf, jf, ff = (np.empty(m * nx, dtype=np.uint8) for m in (1, 2, 4))
+        dim, nx, ny = self._dims(a)
+        shape = (lambda m: (m * nx, 5)) if dim == 1 else (lambda m: (m * ny, m * nx, 5))
+        coarse, inter, fine = np.empty_like(a), np.empty(shape(2)), np.empty(shape(4))
+        cf, jf, ff = (np.empty(shape(m)[:-1], dtype=np.uint8) for m in (1, 2, 4))
         u8 = ctypes.POINTER(ctypes.c_ubyte)
         outs = [_dp(coarse), _dp(inter), _dp(fine), cf.ctypes.data_as(u8), jf.ctypes.data_as(u8),
                 ff.ctypes.data_as(u8)]
         if self.prefix == "ref_":
+            if dim != 1:
+                raise ValueError("the reference's 2D black-hole level-2 driver does not compile")
             self.lib.ref_gr_amr_level2(order, _dp(a), nx, dx, cfl, tf, tol, gamma, *hole, *outs)
             return coarse, inter, fine, cf, jf, ff
         t = _D()
-        k = self.lib.orc_gr_amr_level2(order, _dp(a), nx, dx, cfl, tf, tol, max_steps, gamma, *hole, *outs,
+        k = self.lib.orc_gr_amr_level2(order, dim, _dp(a), nx, ny, dx, cfl, tf, tol, max_steps, gamma, *hole, *outs,
                                        ctypes.byref(t))
         return coarse, inter, fine, cf, jf, ff, t.value, k
 

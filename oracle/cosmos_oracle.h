@@ -48,16 +48,17 @@ double orc_gr_max_wavespeed(int dim, const double* prim, int nx, int ny, double 
 int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, double cfl, double tf, int max_steps,
                double bias, int limiter, int subcycles, double gamma, double M, double a, double bx, double by,
                double bz, double* t_out);
-/* BlackHoleEulerAMR::solve{,2D}Level1AMR (two levels, 1D/2D) and the 1D
- * solveLevel2AMR (three levels); outputs as orc_amr_level1 / orc_amr_level2. */
+/* BlackHoleEulerAMR::solve{,2D}Level1AMR (two levels, 1D/2D) and
+ * solve{,2D}Level2AMR (three levels; the 2D one completed past the
+ * reference's truncation); outputs as orc_amr_level1 / orc_amr_level2. */
 int orc_gr_amr_level1(int order, int dim, const double* prim, int nx, int ny, double dx, double cfl, double tf,
                       double tol, int max_coarse_steps, double gamma, double M, double a, double bx, double by,
                       double bz, double* coarse, double* fine, unsigned char* cflags, unsigned char* fflags,
                       double* t_out);
-int orc_gr_amr_level2(int order, const double* prim, int nx, double dx, double cfl, double tf, double tol,
-                      int max_coarse_steps, double gamma, double M, double a, double bx, double by, double bz,
-                      double* coarse, double* inter, double* fine, unsigned char* cflags, unsigned char* iflags,
-                      unsigned char* fflags, double* t_out);
+int orc_gr_amr_level2(int order, int dim, const double* prim, int nx, int ny, double dx, double cfl, double tf,
+                      double tol, int max_coarse_steps, double gamma, double M, double a, double bx, double by,
+                      double bz, double* coarse, double* inter, double* fine, unsigned char* cflags,
+                      unsigned char* iflags, unsigned char* fflags, double* t_out);
 /* what: 0 p2c, 1 c2p, 2 x flux, 3 y flux, 4 source term, 5 sound speed, 6 metric pieces (44 values) */
 void orc_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M,
                   double a, double bx, double by, double bz, double* out);

@@ -1015,43 +1015,131 @@ int orc_gr_amr_level1(int order, int dim, const double* prim, int nx, int ny, do
     return it;
 }
 
-/* Restriction of a refined parent p (children 2p, 2p+1 at spacing hc, parent
- * at spacing hp) and injection of the parent into unflagged children.  strict
- * (the intermediate/fine exchange, :889-917): restrict only where both
- * children are flagged and inject only into children of flagged parents;
- * otherwise (the coarse/intermediate exchange, :961-985) restrict every
- * unflagged parent and inject into every unflagged child. */
-static void gr_exchange_1d(const bh_t* b, double* P, unsigned char* pf, int np, double hp, double* C,
-                           const unsigned char* cf, double hc, int strict, double gamma) {
-    for (int i = 0; i < np; ++i) {
-        const int restrict_here = strict ? (!pf[i] && (cf[2 * i] && cf[2 * i + 1])) : !pf[i];
-        if (restrict_here)
-            gr_restrict2(b, C + 5 * (2L * i), (i * 2) * hc, C + 5 * (2L * i + 1), ((i * 2) + 1) * hc, i * hp, gamma,
-                         P + 5L * i);
-        for (int s = 0; s < 2; ++s)
-            if (!cf[2 * i + s] && (!strict || pf[i])) memcpy(C + 5 * (2L * i + s), P + 5L * i, 40);
-    }
+/* One level of the three-level driver: states, flags, size and the spacing
+ * its cells' coordinates use (cellSpacing, cellSpacing * 0.5 or * 0.25). */
+typedef struct {
+    double* W;
+    unsigned char* f;
+    int nx, ny;
+    double h;
+} GLevel;
+
+/* p2c of cell (i, j) of L converted at the coordinate of (yi, xi) (1D: y = 0) */
+static void gl_cons(const bh_t* b, int dim, const GLevel* L, int i, int j, int yi, int xi, double g, double* U) {
+    gr_p2c(b, L->W + 5 * ((long)i * L->nx + j), xi * L->h, dim == 2 ? yi * L->h : 0.0, 0.0, g, U);
 }
 
-/* Derefinement of parents whose children 2p, 2p+1 are both flagged and whose
- * children 2p, 2p+1, 2p+2 (at their own coordinates) jump by less than tol
- * over dx (:919-959, :987-1025). */
-static void gr_derefine_1d(const bh_t* b, double* P, unsigned char* pf, int np, double hp, const double* C,
-                           unsigned char* cf, double hc, double dx, double tol, double gamma) {
-    const int nc = 2 * np;
-    for (int i = 0; i < np; ++i) {
-        double ul[5], um[5], ur[5];
-        gr_p2c(b, C + 5 * (2L * i), (i * 2) * hc, 0.0, 0.0, gamma, ul);
-        gr_p2c(b, C + 5 * (2L * i + 1), ((i * 2) + 1) * hc, 0.0, 0.0, gamma, um);
-        gr_p2c(b, C + 5L * clampi_gr(2 * i + 2, nc), ((i * 2) + 2) * hc, 0.0, 0.0, gamma, ur);
-        const int marked = jump_gr(ul, um, dx, tol) || jump_gr(um, ur, dx, tol);
-        if (!marked && (cf[2 * i] && cf[2 * i + 1])) {
-            cf[2 * i] = cf[2 * i + 1] = 0;
-            pf[i] = 1;
-            gr_restrict2(b, C + 5 * (2L * i), (i * 2) * hc, C + 5 * (2L * i + 1), ((i * 2) + 1) * hc, i * hp, gamma,
-                         P + 5L * i);
-        }
+/* parent (i, j) of P <- c2p at its coordinate of the mean of its children's
+ * conserved vectors, each at its own coordinate (1D 0.5 (L + R), 2D
+ * 0.25 (((TL + TR) + BL) + BR)) */
+static void gl_restrict(const bh_t* b, int dim, const GLevel* P, const GLevel* C, int i, int j, double g) {
+    double* out = P->W + 5 * ((long)i * P->nx + j);
+    if (dim == 1) {
+        gr_restrict2(b, C->W + 5 * (2L * j), (j * 2) * C->h, C->W + 5 * (2L * j + 1), ((j * 2) + 1) * C->h,
+                     j * P->h, g, out);
+        return;
     }
+    double u[4][5], m[5];
+    gl_cons(b, 2, C, 2 * i, 2 * j, i * 2, j * 2, g, u[0]);
+    gl_cons(b, 2, C, 2 * i, 2 * j + 1, i * 2, (j * 2) + 1, g, u[1]);
+    gl_cons(b, 2, C, 2 * i + 1, 2 * j, (i * 2) + 1, j * 2, g, u[2]);
+    gl_cons(b, 2, C, 2 * i + 1, 2 * j + 1, (i * 2) + 1, (j * 2) + 1, g, u[3]);
+    for (int k = 0; k < 5; ++k) m[k] = 0.25 * (((u[0][k] + u[1][k]) + u[2][k]) + u[3][k]);
+    gr_c2p(b, m, j * P->h, i * P->h, 0.0, g, out);
+}
+
+static int gl_children_all(int dim, const GLevel* C, int i, int j) {
+    if (dim == 1) return C->f[2 * j] && C->f[2 * j + 1];
+    const long a = (long)(2 * i) * C->nx + 2 * j;
+    return C->f[a] && C->f[a + 1] && C->f[a + C->nx] && C->f[a + C->nx + 1];
+}
+
+static void gl_children_set(int dim, const GLevel* C, int i, int j, unsigned char v) {
+    for (int a = 0; a < (dim == 1 ? 1 : 2); ++a)
+        for (int c = 0; c < 2; ++c) C->f[(long)(2 * i + a) * C->nx + 2 * j + c] = v;
+}
+
+/* Refinement flags of P from its own states (copy boundary; left, right and,
+ * in 2D, the true top and bottom neighbours at their unclamped coordinates),
+ * jumps over dx against tol; only_flagged: parents whose flag is clear are not
+ * tested (:740-768 coarse, :799-826 intermediate; 2D :1119-1150, :1184-1214). */
+static void gl_refine(const bh_t* b, int dim, const GLevel* P, const GLevel* C, double dx, double tol,
+                      int only_flagged, double g) {
+    for (int i = 0; i < P->ny; ++i)
+        for (int j = 0; j < P->nx; ++j) {
+            const long c = (long)i * P->nx + j;
+            if (only_flagged && !P->f[c]) continue;
+            double um[5], un[5];
+            gl_cons(b, dim, P, i, j, i, j, g, um);
+            gl_cons(b, dim, P, i, clampi_gr(j - 1, P->nx), i, j - 1, g, un);
+            int hit = jump_gr(un, um, dx, tol);
+            gl_cons(b, dim, P, i, clampi_gr(j + 1, P->nx), i, j + 1, g, un);
+            hit |= jump_gr(um, un, dx, tol);
+            if (dim == 2) {
+                gl_cons(b, dim, P, clampi_gr(i - 1, P->ny), j, i - 1, j, g, un);
+                hit |= jump_gr(un, um, dx, tol);
+                gl_cons(b, dim, P, clampi_gr(i + 1, P->ny), j, i + 1, j, g, un);
+                hit |= jump_gr(um, un, dx, tol);
+            }
+            if (hit) {
+                P->f[c] = 0;
+                gl_children_set(dim, C, i, j, 1);
+            }
+        }
+}
+
+/* Restriction of refined parents and injection into unflagged children.
+ * strict (the intermediate/fine exchange, :889-917): restrict only where every
+ * child is flagged and inject only under flagged parents; otherwise (the
+ * coarse/intermediate exchange, :961-985) restrict every unflagged parent and
+ * inject into every unflagged child. */
+static void gl_exchange(const bh_t* b, int dim, const GLevel* P, const GLevel* C, int strict, double g) {
+    for (int i = 0; i < P->ny; ++i)
+        for (int j = 0; j < P->nx; ++j) {
+            const long c = (long)i * P->nx + j;
+            if (!P->f[c] && (!strict || gl_children_all(dim, C, i, j))) gl_restrict(b, dim, P, C, i, j, g);
+            for (int a = 0; a < (dim == 1 ? 1 : 2); ++a)
+                for (int d = 0; d < 2; ++d) {
+                    const long f = (long)(2 * i + a) * C->nx + 2 * j + d;
+                    if (!C->f[f] && (!strict || P->f[c])) memcpy(C->W + 5 * f, P->W + 5 * c, 40);
+                }
+        }
+}
+
+/* Derefinement: child 2j+1 against 2j and 2j+2 (2D: child (2i+1, 2j+1)
+ * against its four neighbours), copy boundary, each at its own coordinate,
+ * jumps over dx against tol; quiet parents whose children are all flagged
+ * take their restriction back (:919-959, :987-1025). */
+static void gl_derefine(const bh_t* b, int dim, const GLevel* P, const GLevel* C, double dx, double tol,
+                        double g) {
+    for (int i = 0; i < P->ny; ++i)
+        for (int j = 0; j < P->nx; ++j) {
+            double um[5], un[5];
+            int marked;
+            if (dim == 1) {
+                gl_cons(b, 1, C, 0, 2 * j + 1, 0, (j * 2) + 1, g, um);
+                gl_cons(b, 1, C, 0, 2 * j, 0, j * 2, g, un);
+                marked = jump_gr(un, um, dx, tol);
+                gl_cons(b, 1, C, 0, clampi_gr(2 * j + 2, C->nx), 0, (j * 2) + 2, g, un);
+                marked |= jump_gr(um, un, dx, tol);
+            } else {
+                const int fi = 2 * i + 1, fj = 2 * j + 1;
+                gl_cons(b, 2, C, fi, fj, fi, fj, g, um);
+                gl_cons(b, 2, C, fi - 1, fj, fi - 1, fj, g, un);
+                marked = jump_gr(un, um, dx, tol);
+                gl_cons(b, 2, C, clampi_gr(fi + 1, C->ny), fj, fi + 1, fj, g, un);
+                marked |= jump_gr(um, un, dx, tol);
+                gl_cons(b, 2, C, fi, fj - 1, fi, fj - 1, g, un);
+                marked |= jump_gr(un, um, dx, tol);
+                gl_cons(b, 2, C, fi, clampi_gr(fj + 1, C->nx), fi, fj + 1, g, un);
+                marked |= jump_gr(um, un, dx, tol);
+            }
+            if (!marked && gl_children_all(dim, C, i, j)) {
+                gl_children_set(dim, C, i, j, 0);
+                P->f[(long)i * P->nx + j] = 1;
+                gl_restrict(b, dim, P, C, i, j, g);
+            }
+        }
 }
 
 static int any_gr(const unsigned char* f, long n) {
@@ -1060,46 +1148,62 @@ static int any_gr(const unsigned char* f, long n) {
     return 0;
 }
 
-/* BlackHoleEulerAMR::solveLevel2AMR (:696-1058), 1D: coarse (nx cells, dx),
- * intermediate (2 nx, dx * 0.5) and fine (4 nx, dx * 0.25) levels.  Refinement
- * tests divide by dx on every level with tolerances tol (coarse), tol * 2 *
- * order (intermediate, flagged cells only) and derefinement tolerances tol
- * (fine into intermediate) and tol * 0.5 / order (intermediate into coarse);
- * fineTotalIteration counts every fine and every intermediate step. */
-int orc_gr_amr_level2(int order, const double* prim, int nx, double dx, double cfl, double tf, double tol,
-                      int max_coarse_steps, double gamma, double M, double a, double bx, double by, double bz,
-                      double* coarse, double* inter, double* fine, unsigned char* cflags, unsigned char* iflags,
-                      unsigned char* fflags, double* t_out) {
+/* BlackHoleEulerAMR::solveLevel2AMR (:695-1055, 1D) and solve2DLevel2AMR
+ * (:1057-1334, 2D): coarse (nx x ny cells, dx), intermediate (2nx x 2ny,
+ * dx * 0.5) and fine (4nx x 4ny, dx * 0.25) levels.  Refinement tests divide
+ * by dx on every level with tolerances tol (coarse) and tol * 2 * order
+ * (intermediate, flagged cells only); derefinement tolerances tol (fine into
+ * intermediate) and tol * 0.5 / order (intermediate into coarse);
+ * fineTotalIteration counts every fine and every intermediate step.
+ * The 2D driver is truncated in the reference at :1319-1325, inside the
+ * intermediate/fine exchange; from there it is completed as the 1D driver
+ * and the special-relativistic 2D driver (AMR/RelativisticEulerAMR.cpp:
+ * 1180-1394) continue: strict exchange, fine derefinement at tol, coarse
+ * exchange and intermediate derefinement, every conversion at the natural
+ * coordinate of the cell read.  That completion is this restatement's
+ * choice, not reference behaviour (none exists). */
+int orc_gr_amr_level2(int order, int dim, const double* prim, int nx, int ny, double dx, double cfl, double tf,
+                      double tol, int max_coarse_steps, double gamma, double M, double a, double bx, double by,
+                      double bz, double* coarse, double* inter, double* fine, unsigned char* cflags,
+                      unsigned char* iflags, unsigned char* fflags, double* t_out) {
     const bh_t B = {M, a, bx, by, bz}, *b = &B;
-    const int mnx = 2 * nx, fnx = 4 * nx;
-    memcpy(coarse, prim, sizeof(double) * 5 * (size_t)nx);
-    for (int i = 0; i < nx; ++i) cflags[i] = 1;
-    memset(iflags, 0, (size_t)mnx);
-    memset(fflags, 0, (size_t)fnx);
-    for (int i = 0; i < mnx; ++i) memcpy(inter + 5L * i, coarse + 5L * (i / 2), 40);
-    for (int i = 0; i < fnx; ++i) memcpy(fine + 5L * i, coarse + 5L * (i / 4), 40);
+    if (dim == 1) ny = 1;
+    const int s = dim == 1 ? 1 : 2;
+    GLevel C = {coarse, cflags, nx, ny, dx}, Mi = {inter, iflags, 2 * nx, s * ny, dx * 0.5},
+           F = {fine, fflags, 4 * nx, s * s * ny, dx * 0.25};
+    const long nc = (long)nx * ny, nm = (long)Mi.nx * Mi.ny, nf = (long)F.nx * F.ny;
+    memcpy(coarse, prim, sizeof(double) * 5 * (size_t)nc);
+    for (long c = 0; c < nc; ++c) cflags[c] = 1;
+    memset(iflags, 0, (size_t)nm);
+    memset(fflags, 0, (size_t)nf);
+    for (int i = 0; i < Mi.ny; ++i)
+        for (int j = 0; j < Mi.nx; ++j)
+            memcpy(inter + 5 * ((long)i * Mi.nx + j), coarse + 5 * ((long)(i / s) * nx + j / 2), 40);
+    for (int i = 0; i < F.ny; ++i)
+        for (int j = 0; j < F.nx; ++j)
+            memcpy(fine + 5 * ((long)i * F.nx + j), coarse + 5 * ((long)(i / (s * s)) * nx + j / 4), 40);
     const double itol = (tol * 2.0) * order;
     const double dtol = (tol * 0.5) * (1.0 / order);
-    const double hm = 0.5 * dx; /* the intermediate tests' (0.5 * cellSpacing) */
     double t = 0.0;
     int it = 0, fine_total = 0;
     const int cap = max_coarse_steps < 0 ? 0x7fffffff : max_coarse_steps;
     while (t < tf && it < cap) {
-        const double cdt = gr_amr_dt(b, 1, coarse, nx, 1, dx, cfl, t, tf, fine_total, gamma);
-        gr_refine_1d(b, coarse, nx, dx, dx, tol, 0, gamma, cflags, iflags);
-        gr_flux_step_masked(b, order, 1, coarse, nx, 1, dx, cdt, gamma, cflags);
-        if (any_gr(iflags, mnx)) {
+        const double cdt = gr_amr_dt(b, dim, coarse, nx, ny, dx, cfl, t, tf, fine_total, gamma);
+        gl_refine(b, dim, &C, &Mi, dx, tol, 0, gamma);
+        gr_flux_step_masked(b, order, dim, coarse, nx, ny, dx, cdt, gamma, cflags);
+        if (any_gr(iflags, nm)) {
             double mt = t;
             while (mt < ((t + cdt) - pow(10.0, -4.0))) {
-                const double mdt = gr_amr_dt(b, 1, inter, mnx, 1, dx * 0.5, cfl, mt, t + cdt, fine_total, gamma);
-                gr_refine_1d(b, inter, mnx, hm, dx, itol, 1, gamma, iflags, fflags);
-                gr_flux_step_masked(b, order, 1, inter, mnx, 1, dx * 0.5, mdt, gamma, iflags);
-                if (any_gr(fflags, fnx)) {
+                const double mdt =
+                    gr_amr_dt(b, dim, inter, Mi.nx, Mi.ny, dx * 0.5, cfl, mt, t + cdt, fine_total, gamma);
+                gl_refine(b, dim, &Mi, &F, dx, itol, 1, gamma);
+                gr_flux_step_masked(b, order, dim, inter, Mi.nx, Mi.ny, dx * 0.5, mdt, gamma, iflags);
+                if (any_gr(fflags, nf)) {
                     double ft = mt;
                     while (ft < ((mt + mdt) - pow(10.0, -4.0))) {
                         const double fdt =
-                            gr_amr_dt(b, 1, fine, fnx, 1, dx * 0.25, cfl, ft, mt + mdt, fine_total, gamma);
-                        gr_flux_step_masked(b, order, 1, fine, fnx, 1, dx * 0.25, fdt, gamma, fflags);
+                            gr_amr_dt(b, dim, fine, F.nx, F.ny, dx * 0.25, cfl, ft, mt + mdt, fine_total, gamma);
+                        gr_flux_step_masked(b, order, dim, fine, F.nx, F.ny, dx * 0.25, fdt, gamma, fflags);
                         ft += fdt;
                         fine_total += 1;
                     }
@@ -1108,16 +1212,16 @@ int orc_gr_amr_level2(int order, const double* prim, int nx, double dx, double c
                 }
                 mt += mdt;
                 fine_total += 1;
-                gr_exchange_1d(b, inter, iflags, mnx, dx * 0.5, fine, fflags, dx * 0.25, 1, gamma);
-                gr_derefine_1d(b, inter, iflags, mnx, dx * 0.5, fine, fflags, dx * 0.25, dx, tol, gamma);
+                gl_exchange(b, dim, &Mi, &F, 1, gamma);
+                gl_derefine(b, dim, &Mi, &F, dx, tol, gamma);
             }
         } else {
             fine_total += 1;
         }
         t += cdt;
         it += 1;
-        gr_exchange_1d(b, coarse, cflags, nx, dx, inter, iflags, dx * 0.5, 0, gamma);
-        gr_derefine_1d(b, coarse, cflags, nx, dx, inter, iflags, dx * 0.5, dx, dtol, gamma);
+        gl_exchange(b, dim, &C, &Mi, 0, gamma);
+        gl_derefine(b, dim, &C, &Mi, dx, dtol, gamma);
     }
     if (t_out) *t_out = t;
     return it;

@@ -31,7 +31,9 @@
 //      12 x 12, tol 0.5, t = 0.02, FORCE                                     (12x12x5, 24x24x5, 144, 576)
 //  17. BlackHoleEulerAMR::solveLevel2AMR, the spinning 1D set-up at 32 cells,
 //      tol 0.5, t = 0.03, FORCE: three levels and their flags               (32x5, 64x5, 128x5, 32, 64, 128)
-//  18. BlackHoleEulerAMR::solve2DLevel2AMR throws (1.0 written if it did)   (1)
+//  18. BlackHoleEulerAMR::solve2DLevel2AMR on the input of 16 (the completion
+//      of the reference's truncated driver): three levels and their flags
+//      (12x12x5, 24x24x5, 48x48x5, 144, 576, 2304)
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
@@ -222,13 +224,16 @@ int main(int argc, char** argv) {
         for (bool b : std::get<4>(r3)) put(f, b ? 1.0 : 0.0);
         for (bool b : std::get<5>(r3)) put(f, b ? 1.0 : 0.0);
 
-        double threw = 0.0;
-        try {
-            BlackHoleEulerAMR::solve2DLevel2AMR(ball, 1.0 / m, 0.95, 0.02, 0.5, 1, mpg, spinning);
-        } catch (const std::runtime_error&) {
-            threw = 1.0;
-        }
-        put(f, threw);
+        auto r4 = BlackHoleEulerAMR::solve2DLevel2AMR(ball, 1.0 / m, 0.95, 0.02, 0.5, 1, mpg, spinning);
+        for (auto& row : std::get<0>(r4)) put_cells(f, row);
+        for (auto& row : std::get<1>(r4)) put_cells(f, row);
+        for (auto& row : std::get<2>(r4)) put_cells(f, row);
+        for (auto& row : std::get<3>(r4))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+        for (auto& row : std::get<4>(r4))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
+        for (auto& row : std::get<5>(r4))
+            for (bool b : row) put(f, b ? 1.0 : 0.0);
     }
     fclose(f);
     return 0;

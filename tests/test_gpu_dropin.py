@@ -115,5 +115,10 @@ def test_dropin_classes_match_oracle(orc, tmp_path):
         assert same(take((n, 5)), exp[k]), ("level", k)
     for k, n in enumerate((32, 64, 128)):
         assert same(take((n,)), exp[3 + k].astype(float)), ("flags", k)
-    assert take((1,))[0] == 1.0  # solve2DLevel2AMR refused
+    S = scenarios.bh_spinning_spheroidal_2d(12)
+    exp = orc.gr_amr_level2(S.W, 1, 1 / 12, 0.95, 0.02, 0.5, S.hole, g)
+    for k, n in enumerate((12, 24, 48)):
+        assert same(take((n, n, 5)), exp[k]), ("2D level", k)
+    for k, n in enumerate((12, 24, 48)):
+        assert same(take((n, n)), exp[3 + k].astype(float)), ("2D flags", k)
     assert pos == v.size

@@ -101,3 +101,27 @@ def test_gr_amr_full_refinement_one_step(orc):
     a = orc.gr_point(GR_P2C, f[20], 20 * (dx * 0.5), 0.0, 0.0, 0.0, hole, G)
     b = orc.gr_point(GR_P2C, f[21], 21 * (dx * 0.5), 0.0, 0.0, 0.0, hole, G)
     assert not same(orc.gr_point(GR_C2P, 0.5 * (a + b), 11 * dx, 0.0, 0.0, 0.0, hole, G), c[10])
+
+
+def test_gr_amr_level2_2d_without_refinement_is_the_plain_solve(orc):
+    """The 2D three-level driver (completed past the reference's truncation):
+    with no flag below the coarse level it is the plain 2D solve without
+    forcing, the intermediate level its copy and the fine level the initial
+    state."""
+    from cosmos_amd import scenarios
+
+    S = scenarios.bh_spherical_2d(12)
+    c, m, f, cf, mf, ff, t, k = orc.gr_amr_level2(S.W, 2, S.cell_spacing, S.cfl, 0.05, 1e30, S.hole, S.gamma)
+    exp, te, ke = orc.gr_run(S.W, 2, S.cell_spacing, S.cfl, S.hole, tf=0.05, subcycles=0, gamma=S.gamma)
+    assert same(c, exp) and (t, k) == (te, ke)
+    assert cf.all() and not mf.any() and not ff.any()
+    assert same(m, np.kron(c, np.ones((2, 2, 1)))) and same(f, np.kron(S.W, np.ones((4, 4, 1))))
+
+
+def test_gr_amr_level2_2d_uses_every_level(orc):
+    from cosmos_amd import scenarios
+
+    S = scenarios.bh_spherical_2d(12)
+    c, m, f, cf, mf, ff, t, k = orc.gr_amr_level2(S.W, 2, S.cell_spacing, S.cfl, 0.03, 0.5, S.hole, S.gamma)
+    assert k >= 2 and not cf.all() and mf.any() and ff.any()
+    assert t == pytest.approx(0.03) and np.isfinite(f).all()
