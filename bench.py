@@ -96,6 +96,38 @@ def issue_roofline(size, mode, kernel, avg_ms):
                       "the live average launch time"}
 
 
+LEG_VALU_FILE = os.path.join(ROOT, "profiles", "relativistic_blackhole_valu.json")
+
+
+def leg_valu(leg, size, seed, steps=None, mode=None):
+    """(VALU wave instructions, FP64 FLOPs) of a special-relativistic run (all
+    of its steps, for `mode`) or of one black-hole step, from the committed
+    rocprofv3 SQ counter pass, or None when the profiled workload differs."""
+    try:
+        with open(LEG_VALU_FILE) as f:
+            d = json.load(f)[leg]
+        if d["grid"] != size or d["seed"] != seed:
+            return None
+        if leg == "relativistic":
+            if d["steps"] != steps:
+                return None
+            m = d["modes"][mode]
+            return float(m["valu_wave_instructions"]), float(m["fp64_flop"])
+        return float(d["per_step"]["valu_wave_instructions"]), float(d["per_step"]["fp64_flop"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def leg_issue(valu, flop, ms):
+    """VALU issue and FP64 rates of a whole leg over its live time."""
+    rate = valu / (ms * 1e-3)
+    tflops = flop / (ms * 1e-3) / 1e12
+    return {"valu_instr_per_s": rate, "valu_frac": rate / VALU_PEAK_INSTR_PER_S, "fp64_tflops": tflops,
+            "fp64_frac": tflops / FP64_PEAK_TFLOPS,
+            "source": "rocprofv3 SQ_INSTS_VALU and SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of the same run (committed), "
+                      "over the live time"}
+
+
 def measured_step_traffic(size, mode, kernels, steps):
     """HBM bytes per step of the step's sweep kernels (the dt head's few bytes
     aside) from the committed counter passes, or None if one is missing."""
@@ -275,6 +307,8 @@ def gr_leg(args):
     names = {0: "forcing", 1: "sweep_x", 2: "sweep_y"}
     kernels = {names[k]: {"launches": cnt[k], "avg_ms": tot[k] / cnt[k]} for k in names if cnt[k]}
     ms_per_step = ms_total / steps
+    v = leg_valu("blackhole", n, args.seed)
+    issue = leg_issue(v[0] * steps, v[1] * steps, ms_total) if v else None
     return {
         "workload": f"2D black-hole accretion, {n}x{n} FP64 on [-20M,20M]^2, M=1, a=0, gamma 4/3, "
                     f"excision r<=2.5M, SLIC+van Leer, RK4 forcing (1 subcycle), CFL 0.4, {steps} steps, "
@@ -286,6 +320,7 @@ def gr_leg(args):
         "state_finite": finite,
         "kernels": kernels,
         "bound": "fp64",
+        "issue": issue,
     }
 
 
@@ -314,6 +349,8 @@ def sr_leg(args):
             ms = ev.elapsed_ms(e0, e1)
             _, steps, _ = s.progress()
         res[tag] = {"cell_updates_per_s": n * n * steps / (ms * 1e-3), "ms_per_step": ms / steps, "steps": steps}
+        v = leg_valu("relativistic", n, args.seed, steps, tag)
+        res[tag]["issue"] = leg_issue(v[0], v[1], ms) if v else None
     return {"workload": f"2D special-relativistic blast, {n}x{n} FP64, gamma 5/3, SLIC+van Leer, CFL 0.5, "
                         f"{args.sr_steps} steps (seed {args.seed})", "bound": "fp64", **res}
 
