@@ -355,6 +355,59 @@ def sr_leg(args):
                         f"{args.sr_steps} steps (seed {args.seed})", "bound": "fp64", **res}
 
 
+AMR_FLAG_BYTES_PER_CELL = 41.0   # 5 FP64 read + 1 flag byte per coarse cell
+
+
+def amr_leg(args):
+    """The AMR refinement-flag pass (EulerAMR's level-1 flag loop) on the
+    headline state, Newtonian 2D, coarse args.amr_size^2 (seeded 4-quadrant,
+    tolerance 5 over dx), timed launch by launch with HIP events, against
+    41 B of algorithmic traffic per coarse cell; then the level-1 driver for a
+    few coarse steps (SLIC, CFL 0.9) with its kernel split."""
+    from cosmos_amd import NEWTONIAN, Amr, native, workloads
+
+    n = args.amr_size
+    W = workloads.quadrants_2d(n, seed=args.seed)
+    dx, tol, reps = 1.0 / n, 5.0, 20
+    names = ("refine", "exchange", "derefine", "merge")
+    lib = native.load()
+
+    def read(a):
+        tot = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int * 4)()
+        native.check(lib.cosmos_amr_profile_read(a._h, tot, cnt), a._h, amr=True)
+        return {names[k]: {"launches": cnt[k], "avg_ms": tot[k] / cnt[k], "total_ms": tot[k]} for k in range(4)
+                if cnt[k]}
+
+    with Amr(NEWTONIAN, 2, n, n, 1.4) as a:
+        a.upload(W)
+        a.refine_flags(dx, tol)   # warm-up; the flags are sticky, so every pass reads and tests the same state
+        native.check(lib.cosmos_amr_profile_enable(a._h, 1), a._h, amr=True)
+        for _ in range(reps):
+            a.refine_flags(dx, tol)
+        flags = read(a)["refine"]
+        refined = int((a.download()[2] == 0).sum())
+        a.upload(W)
+        t0 = time.perf_counter()
+        t, k = a.solve_level1(2, dx, 0.9, math.inf, tol, args.amr_steps)
+        wall = time.perf_counter() - t0
+        driver = read(a)
+        native.check(lib.cosmos_amr_profile_enable(a._h, 0), a._h, amr=True)
+    alg = AMR_FLAG_BYTES_PER_CELL * n * n
+    achieved = alg / (flags["avg_ms"] * 1e-3) / 1e9
+    return {
+        "workload": f"AMR level-1 refinement flags, Newtonian 2D, coarse {n}x{n} (fine {2 * n}x{2 * n}) FP64, "
+                    f"seeded 4-quadrant (seed {args.seed}), tolerance {tol:g} over dx; level-1 driver "
+                    f"{args.amr_steps} coarse steps, SLIC, CFL 0.9",
+        "flag_pass": {"launches": flags["launches"], "avg_ms": flags["avg_ms"], "refined_coarse_cells": refined,
+                      "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": alg}},
+        "level1_driver": {"coarse_steps": k, "t_reached": t, "wall_s": wall, "kernels": driver,
+                          "note": "masked sweeps run the level's full sweep and restore the unflagged cells "
+                                  "(merge); their sweep kernels are the solver's and are not counted here"},
+    }
+
+
 def batch_leg(args):
     """The batched form: args.batch_count independent Riemann problems
     (70 % Newtonian / 30 % special relativistic, half 1D in [64, 4096] cells,
@@ -705,7 +758,9 @@ def main():
     ap.add_argument("--batch-tune-steps", type=int, default=10)
     ap.add_argument("--no-batch-baseline", action="store_true", help="skip the per-problem launch baselines")
     ap.add_argument("--no-extra", action="store_true", help="skip the SR, black-hole, adjoint and batched legs")
-    ap.add_argument("--legs", default="relativistic,blackhole,adjoint,adjoint_blackhole,batched",
+    ap.add_argument("--amr-size", type=int, default=4096)
+    ap.add_argument("--amr-steps", type=int, default=3)
+    ap.add_argument("--legs", default="relativistic,blackhole,adjoint,adjoint_blackhole,batched,amr",
                     help="comma-separated further legs to run (default: all)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -770,7 +825,7 @@ def main():
     }
     if not args.no_extra:
         legs = {"relativistic": sr_leg, "blackhole": gr_leg, "adjoint": adjoint_leg,
-                "adjoint_blackhole": gr_adjoint_leg, "batched": batch_leg}
+                "adjoint_blackhole": gr_adjoint_leg, "batched": batch_leg, "amr": amr_leg}
         for name in args.legs.split(","):
             if name not in legs:
                 ap.error(f"unknown leg {name!r} (choose from {', '.join(legs)})")

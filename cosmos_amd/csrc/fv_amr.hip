@@ -16,6 +16,7 @@
 // thread per coarse cell; each thread writes only its own coarse flag, its
 // own children's fine flags and its own coarse state, so no launch races.
 #include <climits>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -188,6 +189,107 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_refine(Grid c, Frame F, dou
             fflags[(long)(2 * i + 1) * fnx + 2 * j] = 1;
             fflags[(long)(2 * i + 1) * fnx + 2 * j + 1] = 1;
         }
+    }
+}
+
+// A1 for states that do not depend on position (Newtonian, special
+// relativistic): the flags of k_amr_refine with every coarse cell converted
+// once, into an LDS tile of the block's cells and a one-cell ring (edge-
+// clamped as the copy boundary), instead of once per neighbour that reads it;
+// and, when 1 / dx is an exact power of two, |(b - a) * (1 / dx)| in place of
+// |(b - a) / dx| (both are then the correctly rounded value of the same real
+// number, so the flags are bitwise those of the reference's test).  2D tiles
+// are 64 x 8 cells (two rows per thread), 1D tiles 256 cells (one per thread).
+constexpr int kRefTX2 = 64, kRefTY2 = 8, kRefTX1 = kAmrThreads;
+constexpr int kRefLds = 5 * (kRefTY2 + 2) * (kRefTX2 + 2) > 5 * (kRefTX1 + 2) ? 5 * (kRefTY2 + 2) * (kRefTX2 + 2)
+                                                                               : 5 * (kRefTX1 + 2);
+
+__device__ __forceinline__ bool jump_scaled(const double* a, const double* b, double dx, double inv_dx,
+                                            int use_inv, double tol) {
+    bool hit = false;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const double d = b[k] - a[k];
+        hit |= __builtin_fabs(use_inv ? d * inv_dx : d / dx) >= tol;
+    }
+    return hit;
+}
+
+template <class Phys>
+__global__ __launch_bounds__(kAmrThreads) void k_amr_refine_tile(Grid c, int dim, double dx, double inv_dx,
+                                                                 int use_inv, double tol, int diag, PhysParams P,
+                                                                 u8* cflags, u8* fflags) {
+    __shared__ double U[kRefLds];
+    const int TX = dim == 1 ? kRefTX1 : kRefTX2, TY = dim == 1 ? 1 : kRefTY2;
+    const int W = TX + 2, H = dim == 1 ? 1 : TY + 2;
+    const int tiles_x = (c.nx + TX - 1) / TX;
+    const int j0 = (int)(blockIdx.x % tiles_x) * TX, i0 = (int)(blockIdx.x / tiles_x) * TY;
+    // at most three ring-tile cells per thread: (8 + 2) x (64 + 2) = 660 <= 3 x 256; every load issued first
+    constexpr int kLoads = 3;
+    static_assert((kRefTY2 + 2) * (kRefTX2 + 2) <= kLoads * kAmrThreads && kRefTX1 + 2 <= kLoads * kAmrThreads,
+                  "ring tile larger than the load slots");
+    double Wl[kLoads][5];
+#pragma unroll
+    for (int m = 0; m < kLoads; ++m) {
+        const int t = threadIdx.x + m * kAmrThreads;
+        if (t < W * H) {
+            const int r = t / W, q = t - r * W;
+            load(c, dim == 1 ? 0 : clampc(i0 - 1 + r, c.ny), clampc(j0 - 1 + q, c.nx), Wl[m]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kLoads; ++m) {
+        const int t = threadIdx.x + m * kAmrThreads;
+        if (t < W * H) {
+            const int r = t / W, q = t - r * W;
+            double Uc[5];
+            Phys::p2c(Wl[m], Uc, P);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) U[(k * H + r) * W + q] = Uc[k];
+        }
+    }
+    __syncthreads();
+    const int rows_per_pass = kAmrThreads / TX;  // 4 in 2D, 1 in 1D
+    for (int ti = threadIdx.x / TX; ti < TY; ti += rows_per_pass) {
+    const int tj = threadIdx.x % TX;
+    const int i = i0 + ti, j = j0 + tj;
+    if (i >= c.ny || j >= c.nx) continue;
+    const int r = dim == 1 ? 0 : ti + 1, q = tj + 1;
+    auto at = [&](int rr, int qq, double* out) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) out[k] = U[(k * H + rr) * W + qq];
+    };
+    double um[5], un[5];
+    at(r, q, um);
+    at(r, q - 1, un);
+    bool hit = jump_scaled(un, um, dx, inv_dx, use_inv, tol);
+    at(r, q + 1, un);
+    hit |= jump_scaled(um, un, dx, inv_dx, use_inv, tol);
+    if (dim == 2) {
+        at(r - 1, q, un);
+        hit |= jump_scaled(un, um, dx, inv_dx, use_inv, tol);
+        if (diag && i + 1 >= c.ny && j + 1 >= c.nx) {
+            const double Wd[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+            Phys::p2c(Wd, un, P);
+        } else {
+            at(r + 1, diag ? q + 1 : q, un);
+        }
+        hit |= jump_scaled(um, un, dx, inv_dx, use_inv, tol);
+    }
+    if (hit) {
+        const long idx = (long)i * c.nx + j;
+        const int fnx = 2 * c.nx;
+        cflags[idx] = 0;
+        if (dim == 1) {
+            fflags[2 * j] = 1;
+            fflags[2 * j + 1] = 1;
+        } else {
+            fflags[(long)(2 * i) * fnx + 2 * j] = 1;
+            fflags[(long)(2 * i) * fnx + 2 * j + 1] = 1;
+            fflags[(long)(2 * i + 1) * fnx + 2 * j] = 1;
+            fflags[(long)(2 * i + 1) * fnx + 2 * j + 1] = 1;
+        }
+    }
     }
 }
 
@@ -452,6 +554,8 @@ struct cosmos_amr {
     cfv::Hole hole = cfv::make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);  // COSMOS_BLACKHOLE (the solvers' default)
     double excision = 0.0;
     double dx = 0.0;  // the coarse cell spacing last passed in (black-hole coordinates of cosmos_amr_restrict)
+    bool profiling = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof;  // (kind, events) per timed launch
     std::string err;
 };
 
@@ -517,6 +621,28 @@ int amr_guarded(cosmos_amr* a, F&& f) {
     }
 }
 
+// Kernel kinds reported by cosmos_amr_profile_read.
+enum { kAmrProfRefine = 0, kAmrProfExchange = 1, kAmrProfDerefine = 2, kAmrProfMerge = 3, kAmrProfKinds = 4 };
+
+// Runs launch() (kernels on stream st), bracketed by HIP events when profiling.
+template <class F>
+hipError_t amr_timed(cosmos_amr* a, int kind, hipStream_t st, F&& launch) {
+    if (!a->profiling) return launch();
+    hipEvent_t b = nullptr, e = nullptr;
+    hipError_t r = hipEventCreate(&b);
+    if (r == hipSuccess) r = hipEventCreate(&e);
+    if (r == hipSuccess) r = hipEventRecord(b, st);
+    if (r == hipSuccess) r = launch();
+    if (r == hipSuccess) r = hipEventRecord(e, st);
+    if (r != hipSuccess) {
+        if (b) hipEventDestroy(b);
+        if (e) hipEventDestroy(e);
+        return r;
+    }
+    a->prof.push_back({kind, {b, e}});
+    return hipSuccess;
+}
+
 template <template <class> class K, class... Args>
 hipError_t launch_physics(int physics, dim3 g, hipStream_t s, Args... args) {
     if (physics == COSMOS_NEWTONIAN) hipLaunchKernelGGL(K<cfv::Newtonian>::fn, g, dim3(cfv::kAmrThreads), 0, s, args...);
@@ -565,8 +691,28 @@ int amr_refine(cosmos_amr* a, double dx, double tol) {
     hipStream_t s = order_streams(a);
     // the special-relativistic and black-hole 2D drivers read the diagonal cell as "bottom"
     const int diag = a->physics == COSMOS_NEWTONIAN ? 0 : 1;
-    AMRCHK(a, (launch_physics<RefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, frame(a, dx), dx, tol,
-                                       diag, a->P, a->cflags, a->fflags)));
+    if (a->physics == COSMOS_BLACKHOLE) {
+        AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
+                   return launch_physics<RefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, frame(a, dx),
+                                                  dx, tol, diag, a->P, a->cflags, a->fflags);
+               }));
+        return COSMOS_OK;
+    }
+    // 1 / dx exact (dx a power of two with a normal reciprocal): multiply instead of divide
+    int e2 = 0;
+    const double inv = 1.0 / dx;
+    const int use_inv = std::frexp(dx, &e2) == 0.5 && std::isnormal(inv) && std::isnormal(dx) ? 1 : 0;
+    const int tx = a->dim == 1 ? cfv::kRefTX1 : cfv::kRefTX2, ty = a->dim == 1 ? 1 : cfv::kRefTY2;
+    const dim3 grid((unsigned)(((a->nx + tx - 1) / tx) * (long)((a->ny + ty - 1) / ty)));
+    AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
+               if (a->physics == COSMOS_NEWTONIAN)
+                   hipLaunchKernelGGL(cfv::k_amr_refine_tile<cfv::Newtonian>, grid, dim3(cfv::kAmrThreads), 0, s, c,
+                                      a->dim, dx, inv, use_inv, tol, diag, a->P, a->cflags, a->fflags);
+               else
+                   hipLaunchKernelGGL(cfv::k_amr_refine_tile<cfv::Relativistic>, grid, dim3(cfv::kAmrThreads), 0, s,
+                                      c, a->dim, dx, inv, use_inv, tol, diag, a->P, a->cflags, a->fflags);
+               return hipGetLastError();
+           }));
     return COSMOS_OK;
 }
 
@@ -576,9 +722,11 @@ int amr_restrict(cosmos_amr* a, double dx) {
     if (!rc) rc = grid_of(a, a->fine, f);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<RestrictK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f, frame(a, dx * 0.5),
-                                         frame(a, dx), a->P, (const unsigned char*)a->cflags,
-                                         (const unsigned char*)a->fflags)));
+    AMRCHK(a, amr_timed(a, kAmrProfExchange, s, [&] {
+               return launch_physics<RestrictK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f,
+                                                frame(a, dx * 0.5), frame(a, dx), a->P,
+                                                (const unsigned char*)a->cflags, (const unsigned char*)a->fflags);
+           }));
     return COSMOS_OK;
 }
 
@@ -588,8 +736,11 @@ int amr_derefine(cosmos_amr* a, double dx, double tol) {
     if (!rc) rc = grid_of(a, a->fine, f);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<DerefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f, frame(a, dx * 0.5),
-                                         frame(a, dx), dx, tol, a->P, a->cflags, a->fflags)));
+    AMRCHK(a, amr_timed(a, kAmrProfDerefine, s, [&] {
+               return launch_physics<DerefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f,
+                                                frame(a, dx * 0.5), frame(a, dx), dx, tol, a->P, a->cflags,
+                                                a->fflags);
+           }));
     return COSMOS_OK;
 }
 
@@ -613,9 +764,11 @@ int amr_sweep(cosmos_amr* a, int level, int order, int axis, double dx, double d
     if (rc) return rc;
     hipStream_t st = (hipStream_t)cosmos_stream(s);
     long n = (long)after.nx * after.ny;
-    hipLaunchKernelGGL(cfv::k_mask_merge, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, st, before, after,
-                       (const unsigned char*)level_flags(a, level));
-    AMRCHK(a, hipGetLastError());
+    AMRCHK(a, amr_timed(a, kAmrProfMerge, st, [&] {
+               hipLaunchKernelGGL(cfv::k_mask_merge, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, st, before, after,
+                                  (const unsigned char*)level_flags(a, level));
+               return hipGetLastError();
+           }));
     return COSMOS_OK;
 }
 
@@ -650,9 +803,11 @@ int l2_refine(cosmos_amr* a, int lp, double dx, double tol, int only_flagged) {
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
-                                         frame(a, level_spacing(dx, lp)), level_flags(a, lp), level_flags(a, lp + 1),
-                                         dx, tol, only_flagged, a->P)));
+    AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
+               return launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
+                                                frame(a, level_spacing(dx, lp)), level_flags(a, lp),
+                                                level_flags(a, lp + 1), dx, tol, only_flagged, a->P);
+           }));
     return COSMOS_OK;
 }
 
@@ -662,10 +817,12 @@ int l2_exchange(cosmos_amr* a, int lp, double dx, int strict) {
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<L2ExchangeK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
-                                           frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
-                                           (const unsigned char*)level_flags(a, lp),
-                                           (const unsigned char*)level_flags(a, lp + 1), strict, a->P)));
+    AMRCHK(a, amr_timed(a, kAmrProfExchange, s, [&] {
+               return launch_physics<L2ExchangeK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
+                                                  frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
+                                                  (const unsigned char*)level_flags(a, lp),
+                                                  (const unsigned char*)level_flags(a, lp + 1), strict, a->P);
+           }));
     return COSMOS_OK;
 }
 
@@ -675,9 +832,11 @@ int l2_derefine(cosmos_amr* a, int lp, double dx, double tol) {
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
-    AMRCHK(a, (launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
-                                           frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
-                                           level_flags(a, lp), level_flags(a, lp + 1), dx, tol, a->P)));
+    AMRCHK(a, amr_timed(a, kAmrProfDerefine, s, [&] {
+               return launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
+                                                  frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
+                                                  level_flags(a, lp), level_flags(a, lp + 1), dx, tol, a->P);
+           }));
     return COSMOS_OK;
 }
 
@@ -722,6 +881,10 @@ int cosmos_amr_destroy(cosmos_amr* a) {
     if (a->coarse) cosmos_synchronize(a->coarse);
     if (a->fine) cosmos_synchronize(a->fine);
     if (a->fine2) cosmos_synchronize(a->fine2);
+    for (auto& p : a->prof) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
     hipFree(a->cflags);
     hipFree(a->fflags);
     hipFree(a->f2flags);
@@ -734,6 +897,42 @@ int cosmos_amr_destroy(cosmos_amr* a) {
 }
 
 const char* cosmos_amr_last_error(const cosmos_amr* a) { return a ? a->err.c_str() : g_amr_create_error.c_str(); }
+
+int cosmos_amr_profile_enable(cosmos_amr* a, int on) {
+    return amr_guarded(a, [&] {
+        a->profiling = on != 0;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_amr_profile_read(cosmos_amr* a, double* total_ms, int* launches) {
+    return amr_guarded(a, [&] {
+        if (!total_ms || !launches) return amr_fail(a, COSMOS_ERR_ARGUMENT, "null output");
+        order_streams(a);
+        for (int k = 0; k < kAmrProfKinds; ++k) {
+            total_ms[k] = 0.0;
+            launches[k] = 0;
+        }
+        int rc = COSMOS_OK;
+        for (auto& p : a->prof) {
+            float ms = 0.0f;
+            hipError_t e = hipEventElapsedTime(&ms, p.second.first, p.second.second);
+            if (e != hipSuccess && rc == COSMOS_OK)
+                rc = amr_fail(a, COSMOS_ERR_DEVICE, std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+            total_ms[p.first] += ms;
+            launches[p.first] += 1;
+            hipEventDestroy(p.second.first);
+            hipEventDestroy(p.second.second);
+        }
+        a->prof.clear();
+        return rc;
+    });
+}
+
+cosmos_solver* cosmos_amr_level_solver(cosmos_amr* a, int level) {
+    if (!a || level < 0 || level > 2) return nullptr;
+    return level_solver(a, level);
+}
 
 int cosmos_amr_set_con2prim(cosmos_amr* a, double tolerance, int max_iterations) {
     return amr_guarded(a, [&] {

@@ -99,6 +99,9 @@ SIGNATURES = {
     "cosmos_amr_restrict": (ctypes.c_int, [_handle]),
     "cosmos_amr_set_spacetime": (ctypes.c_int, [_handle] + [ctypes.c_double] * 5),
     "cosmos_amr_set_excision": (ctypes.c_int, [_handle, ctypes.c_double]),
+    "cosmos_amr_profile_enable": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cosmos_amr_profile_read": (ctypes.c_int, [_handle, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
+    "cosmos_amr_level_solver": (ctypes.c_void_p, [_handle, ctypes.c_int]),
     "cosmos_amr_derefine": (ctypes.c_int, [_handle, ctypes.c_double, ctypes.c_double]),
     "cosmos_amr_solve_level1": (ctypes.c_int, [_handle, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p,

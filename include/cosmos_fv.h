@@ -232,6 +232,19 @@ int cosmos_amr_destroy(cosmos_amr* a);
 const char* cosmos_amr_last_error(const cosmos_amr* a);
 /* as cosmos_set_con2prim, for both levels */
 int cosmos_amr_set_con2prim(cosmos_amr* a, double tolerance, int max_iterations);
+/* No counterpart in the reference (its drivers print status lines instead).
+ * While enabled, every flag, restriction / injection, derefinement and masked-
+ * sweep merge kernel is bracketed by HIP events on the stream it runs on;
+ * cosmos_amr_profile_read synchronises and returns, per kind (0 refinement
+ * flags, 1 restriction / injection, 2 derefinement, 3 masked-sweep merge), the
+ * summed kernel milliseconds and launch counts since the last read. */
+int cosmos_amr_profile_enable(cosmos_amr* a, int on);
+int cosmos_amr_profile_read(cosmos_amr* a, double* total_ms, int* launches);
+/* The solver handle holding level 0 (coarse), 1 or 2 (the three-level run's
+ * finest; null before cosmos_amr_solve_level2) of the context, for its stream,
+ * state planes and the solver API; owned by the context.  No counterpart in the
+ * reference (its drivers return the levels as vectors). */
+cosmos_solver* cosmos_amr_level_solver(cosmos_amr* a, int level);
 /* COSMOS_BLACKHOLE contexts: as cosmos_set_spacetime / cosmos_set_excision,
  * for every level (the blackHole argument of BlackHoleEulerAMR's drivers,
  * AMR/BlackHoleEulerAMR.cpp:239-240).  The flag, restriction and

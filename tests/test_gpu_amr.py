@@ -4,7 +4,7 @@ states and flags."""
 import numpy as np
 import pytest
 
-from conftest import golden, rel_linf, same
+from conftest import conserved, golden, rel_linf, same
 from cosmos_amd import Amr, workloads
 from test_oracle_amr import cases
 
@@ -76,6 +76,35 @@ def test_amr_flags_and_masked_sweep(orc, physics, gamma, shape):
         full = orc.sweep(exp, physics, 2, axis, dx, 0.1 * dx, 0.0, 1, gamma)
         exp = np.where(ecf[..., None] == 1, full, exp)
     assert same(c2, exp), rel_linf(c2, exp)
+
+
+@pytest.mark.parametrize("physics,gamma", [(0, G_N), (1, G_SR)])
+@pytest.mark.parametrize("shape,dx", [((300,), 1.0 / 256), ((300,), 1.0 / 300), ((9, 70), 1.0 / 64),
+                                      ((9, 70), 0.013), ((130, 67), 2.0 ** -7), ((130, 67), 1.0 / 67)])
+def test_refine_flags_tiles_and_exact_reciprocal(orc, physics, gamma, shape, dx):
+    """The tiled flag kernel (partial tiles at the right and bottom edges, the
+    copy boundary from the tile ring, the SR diagonal "bottom" and corner) and
+    its multiply-by-1/dx form for power-of-two spacings against the oracle's
+    divide; a second pass over already-set flags changes nothing (sticky)."""
+    rng = np.random.default_rng(sum(shape) + physics)
+    W = np.empty(tuple(shape) + (5,))
+    W[..., 0] = rng.uniform(0.5, 2.0, shape)
+    W[..., 1:3] = rng.uniform(-0.4, 0.4, tuple(shape) + (2,))
+    W[..., 3] = 0.0
+    W[..., 4] = rng.uniform(0.5, 2.0, shape)
+    U = conserved(W, gamma, physics).reshape(W.shape)
+    tol = float(np.quantile(np.abs(np.diff(U, axis=-2)).max(axis=-1), 0.8)) / dx
+    fshape = (2 * shape[0],) if len(shape) == 1 else (2 * shape[0], 2 * shape[1])
+    ecf, eff = orc.amr_refine_flags(W, physics, dx, tol, np.ones(shape, np.uint8), np.zeros(fshape, np.uint8),
+                                    gamma)
+    assert 0 < (ecf == 0).sum() < ecf.size
+    with amr_for(W, physics, gamma) as a:
+        a.refine_flags(dx, tol)
+        _, _, cf, ff = a.download()
+        a.refine_flags(dx, tol)
+        _, _, cf2, ff2 = a.download()
+    assert same(cf, ecf) and same(ff, eff)
+    assert same(cf2, ecf) and same(ff2, eff)
 
 
 def test_amr_2d_larger_newtonian(orc):
