@@ -246,6 +246,26 @@ class Amr:
     def set_excision(self, radius: float) -> None:
         self._check(self._lib.cosmos_amr_set_excision(self._h, radius))
 
+    AMR_KERNEL_KINDS = ("refine", "exchange", "derefine", "merge")
+
+    def profile(self, on: bool = True) -> None:
+        """Bracket every AMR pass kernel with HIP events while on."""
+        self._check(self._lib.cosmos_amr_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        """{kind: (launches, total_ms)} of the timed AMR kernels since the last read."""
+        tot = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int * 4)()
+        self._check(self._lib.cosmos_amr_profile_read(self._h, tot, cnt))
+        return {name: (cnt[k], tot[k]) for k, name in enumerate(self.AMR_KERNEL_KINDS)}
+
+    def level_stream(self, level: int = 0) -> int:
+        """The HIP stream of a level's solver handle (0 coarse, 1 fine, 2 finest of a three-level run)."""
+        s = self._lib.cosmos_amr_level_solver(self._h, level)
+        if not s:
+            raise native.CosmosError(f"AMR level {level} has no solver handle")
+        return self._lib.cosmos_stream(s)
+
     def upload(self, prim: np.ndarray) -> None:
         a = np.ascontiguousarray(prim, dtype=np.float64)
         if a.size != self.nx * self.ny * 5:

@@ -241,3 +241,24 @@ def test_gr_amr_argument_errors():
     with Amr(0, 1, 16, 1, 1.4) as a:
         with pytest.raises(Exception, match="COSMOS_BLACKHOLE"):
             a.set_spacetime(1.0, 0.0, 0.5, 0.0)
+
+
+def test_amr_profile_counts_each_pass():
+    """cosmos_amr_profile_*: one timed launch per pass kind, positive times,
+    reset by the read; the level solver handles expose their streams."""
+    W = workloads.quadrants_2d(40, seed=3, ny=24)
+    with amr_for(W, 0, G_N) as a:
+        assert a.level_stream(0) and a.level_stream(1)
+        with pytest.raises(Exception):
+            a.level_stream(2)  # no three-level run yet
+        a.profile(True)
+        a.refine_flags(1.0 / 40, 5.0)
+        a.sweep(0, 2, 0, 1.0 / 40, 1e-4)
+        a.restrict()
+        a.derefine(1.0 / 40, 5.0)
+        got = a.profile_read()
+        assert {k: n for k, (n, _) in got.items()} == {"refine": 1, "exchange": 1, "derefine": 1, "merge": 1}
+        assert all(ms > 0.0 for _, ms in got.values())
+        a.profile(False)
+        a.refine_flags(1.0 / 40, 5.0)
+        assert all(n == 0 for n, _ in a.profile_read().values())
