@@ -52,11 +52,22 @@ __device__ __forceinline__ void store(const Grid& g, int i, int j, const double*
 }
 
 // |(b - a) / dx| >= tol for any component, the test of the reference's flag
-// loops (AMR/EulerAMR.cpp:215-216, :412-415).
-__device__ __forceinline__ bool jump(const double* a, const double* b, double dx, double tol) {
+// loops (AMR/EulerAMR.cpp:215-216, :412-415).  Spacing carries dx and, when
+// it is exact (dx a power of two with a normal reciprocal), 1 / dx: then
+// (b - a) * (1 / dx) is the correctly rounded value of the same real number
+// as (b - a) / dx, so the tests are bitwise the reference's.
+struct Spacing {
+    double dx, inv;
+    int use_inv;
+};
+
+__device__ __forceinline__ bool jump(const double* a, const double* b, const Spacing& s, double tol) {
     bool hit = false;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) hit |= __builtin_fabs((b[k] - a[k]) / dx) >= tol;
+    for (int k = 0; k < 5; ++k) {
+        const double d = b[k] - a[k];
+        hit |= __builtin_fabs(s.use_inv ? d * s.inv : d / s.dx) >= tol;
+    }
     return hit;
 }
 
@@ -152,7 +163,7 @@ __device__ __forceinline__ void restrict_children(const Grid& f, int dim, int i,
 // (AMR/RelativisticEulerAMR.cpp:422, AMR/BlackHoleEulerAMR.cpp:493-495), whose
 // bottom-right corner is a default state (1, 0, 0, 0, 1).
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_amr_refine(Grid c, Frame F, double dx, double tol, int diag,
+__global__ __launch_bounds__(kAmrThreads) void k_amr_refine(Grid c, Frame F, Spacing dx, double tol, int diag,
                                                             PhysParams P, u8* cflags, u8* fflags) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)c.nx * c.ny) return;
@@ -206,13 +217,7 @@ constexpr int kRefLds = 5 * (kRefTY2 + 2) * (kRefTX2 + 2) > 5 * (kRefTX1 + 2) ? 
 
 __device__ __forceinline__ bool jump_scaled(const double* a, const double* b, double dx, double inv_dx,
                                             int use_inv, double tol) {
-    bool hit = false;
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const double d = b[k] - a[k];
-        hit |= __builtin_fabs(use_inv ? d * inv_dx : d / dx) >= tol;
-    }
-    return hit;
+    return jump(a, b, Spacing{dx, inv_dx, use_inv}, tol);
 }
 
 template <class Phys>
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_restrict(Grid c, Grid f, Fr
 // coordinates of 2j-1, 2j, 2j+1 (AMR/BlackHoleEulerAMR.cpp:378-383); the 2D
 // driver at their own (:633-642).  Neither matters to the other physics.
 template <class Phys>
-__global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, Frame Fc, Frame Fp, double dx,
+__global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, Frame Fc, Frame Fp, Spacing dx,
                                                               double tol, PhysParams P, u8* cflags, u8* fflags) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)c.nx * c.ny) return;
@@ -444,7 +449,7 @@ __device__ __forceinline__ void children_set(u8* cflags, const Grid& c, int dim,
 // bottom neighbour for every physics.
 template <class Phys>
 __global__ __launch_bounds__(kAmrThreads) void k_l2_refine(Grid p, Grid c, Frame Fp, u8* pflags, u8* cflags,
-                                                           double dx, double tol, int only_flagged, PhysParams P) {
+                                                           Spacing dx, double tol, int only_flagged, PhysParams P) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, Fra
 // tol * 0.5 * (1 / order) :878-911, :1293-1337).
 template <class Phys>
 __global__ __launch_bounds__(kAmrThreads) void k_l2_derefine(Grid p, Grid c, Frame Fc, Frame Fp, u8* pflags,
-                                                             u8* cflags, double dx, double tol, PhysParams P) {
+                                                             u8* cflags, Spacing dx, double tol, PhysParams P) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
@@ -659,6 +664,14 @@ cfv::Frame frame(const cosmos_amr* a, double h) { return cfv::Frame{a->dim, h, a
 // reference writes it (cellSpacing, cellSpacing * 0.5, cellSpacing * 0.25).
 double level_spacing(double dx, int level) { return level == 0 ? dx : level == 1 ? dx * 0.5 : dx * 0.25; }
 
+// dx for the flag tests, with its reciprocal when that is exact
+cfv::Spacing spacing_of(double dx) {
+    int e2 = 0;
+    const double inv = 1.0 / dx;
+    const int use_inv = std::frexp(dx, &e2) == 0.5 && std::isnormal(inv) && std::isnormal(dx) ? 1 : 0;
+    return cfv::Spacing{dx, inv, use_inv};
+}
+
 template <class Phys>
 struct RefineK {
     static constexpr auto fn = cfv::k_amr_refine<Phys>;
@@ -694,14 +707,14 @@ int amr_refine(cosmos_amr* a, double dx, double tol) {
     if (a->physics == COSMOS_BLACKHOLE) {
         AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
                    return launch_physics<RefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, frame(a, dx),
-                                                  dx, tol, diag, a->P, a->cflags, a->fflags);
+                                                  spacing_of(dx), tol, diag, a->P, a->cflags, a->fflags);
                }));
         return COSMOS_OK;
     }
     // 1 / dx exact (dx a power of two with a normal reciprocal): multiply instead of divide
-    int e2 = 0;
-    const double inv = 1.0 / dx;
-    const int use_inv = std::frexp(dx, &e2) == 0.5 && std::isnormal(inv) && std::isnormal(dx) ? 1 : 0;
+    const cfv::Spacing sp = spacing_of(dx);
+    const double inv = sp.inv;
+    const int use_inv = sp.use_inv;
     const int tx = a->dim == 1 ? cfv::kRefTX1 : cfv::kRefTX2, ty = a->dim == 1 ? 1 : cfv::kRefTY2;
     const dim3 grid((unsigned)(((a->nx + tx - 1) / tx) * (long)((a->ny + ty - 1) / ty)));
     AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
@@ -738,8 +751,8 @@ int amr_derefine(cosmos_amr* a, double dx, double tol) {
     hipStream_t s = order_streams(a);
     AMRCHK(a, amr_timed(a, kAmrProfDerefine, s, [&] {
                return launch_physics<DerefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f,
-                                                frame(a, dx * 0.5), frame(a, dx), dx, tol, a->P, a->cflags,
-                                                a->fflags);
+                                                frame(a, dx * 0.5), frame(a, dx), spacing_of(dx), tol, a->P,
+                                                a->cflags, a->fflags);
            }));
     return COSMOS_OK;
 }
@@ -806,7 +819,7 @@ int l2_refine(cosmos_amr* a, int lp, double dx, double tol, int only_flagged) {
     AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
                return launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
                                                 frame(a, level_spacing(dx, lp)), level_flags(a, lp),
-                                                level_flags(a, lp + 1), dx, tol, only_flagged, a->P);
+                                                level_flags(a, lp + 1), spacing_of(dx), tol, only_flagged, a->P);
            }));
     return COSMOS_OK;
 }
@@ -835,7 +848,8 @@ int l2_derefine(cosmos_amr* a, int lp, double dx, double tol) {
     AMRCHK(a, amr_timed(a, kAmrProfDerefine, s, [&] {
                return launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
                                                   frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
-                                                  level_flags(a, lp), level_flags(a, lp + 1), dx, tol, a->P);
+                                                  level_flags(a, lp), level_flags(a, lp + 1), spacing_of(dx), tol,
+                                                  a->P);
            }));
     return COSMOS_OK;
 }
