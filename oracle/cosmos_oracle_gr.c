@@ -10,7 +10,12 @@
  * evaluation order), including its quirks: the (r + (y - y_BH)) term of the
  * Kerr-Schild 4-vector, tau with W rather than W^2 in p2c, flat |S|^2 in
  * con2prim, the x-flux tolerance 1e-10, the unit-domain wave speed with
- * swapped coordinates, and the per-call coordinate expressions.
+ * swapped coordinates, and the per-call coordinate expressions.  The AMR
+ * drivers at the end follow AMR/BlackHoleEulerAMR.cpp (:1-1055 pinned bitwise
+ * against the reference's own compiled functions).  Parity unpinned: the 2D
+ * three-level driver past the reference's truncation (:1319-1325), a
+ * completion of this restatement's (no reference behaviour exists); only its
+ * no-refinement invariant is checked.
  */
 #include "cosmos_oracle.h"
 
