@@ -186,3 +186,93 @@ def bh_spinning_spheroidal_2d(n: int) -> Scenario:  # :167-199
 ALL = [toro1, toro2, toro3, toro1_2d, toro2_2d, toro3_2d, sr_shock, sr_blast, sr_perturbed, sr_shock_2d,
        sr_blast_2d, sr_perturbed_2d, bh_spherical_1d, bh_spinning_1d, bh_spherical_2d, bh_spinning_2d,
        bh_spheroidal_2d, bh_spinning_spheroidal_2d]
+
+
+# ---------------------------------------------------------------------------
+# The AMR drivers of Tests/EulerAMRTests.cpp, Tests/RelativisticEulerAMRTests.cpp
+# and Tests/BlackHoleEulerAMRTests.cpp: their set-ups (CFL 0.95 throughout),
+# with the tolerance, order and level main.cpp:20-46 passes to each.
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class AmrScenario:
+    scenario: Scenario
+    tolerance: float
+    order: int
+    levels: int
+
+
+def _amr(name, physics, dim, gamma, final_time, W, tol, order, levels, hole=(1.0, 0.0, 0.5, 0.5, 0.0)):
+    return AmrScenario(Scenario(name, physics, dim, gamma, 0.95, final_time, W, hole), tol, order, levels)
+
+
+def euler_toro1_amr(n: int) -> AmrScenario:  # Tests/EulerAMRTests.cpp:3-30
+    W = _line(n, _const(1.0, 0.0, 0.0, 0.0, 1.0), _const(0.125, 0.0, 0.0, 0.0, 0.1), _left(n))
+    return _amr("euler_toro1_amr", NEWTONIAN, 1, 1.4, 0.25, W, 1.0, 2, 2)
+
+
+def euler_toro1_2d_amr(n: int) -> AmrScenario:  # :32-62
+    W = _disc(n, _const(1.0, 0.0, 0.0, 0.0, 1.0), _const(0.125, 0.0, 0.0, 0.0, 0.1), _centre(n, 0.2))
+    return _amr("euler_toro1_2d_amr", NEWTONIAN, 2, 1.4, 0.125, W, 1.0, 2, 2)
+
+
+def sr_mild_shock_amr(n: int) -> AmrScenario:  # Tests/RelativisticEulerAMRTests.cpp:3-30
+    W = _line(n, _const(10.0, 0.0, 0.0, 0.0, 13.3), _const(1.0, 0.0, 0.0, 0.0, 0.0), _left(n))
+    return _amr("sr_mild_shock_amr", RELATIVISTIC, 1, 5.0 / 3.0, 0.4, W, 5.0, 2, 2)
+
+
+def sr_strong_blast_amr(n: int) -> AmrScenario:  # :32-59
+    W = _line(n, _const(1.0, 0.0, 0.0, 0.0, 1000.0), _const(1.0, 0.0, 0.0, 0.0, 0.01), _left(n))
+    return _amr("sr_strong_blast_amr", RELATIVISTIC, 1, 5.0 / 3.0, 0.4, W, 100.0, 2, 2)
+
+
+def sr_perturbed_amr(n: int) -> AmrScenario:  # :61-88
+    h = 1.0 / n
+    W = _line(n, _const(5.0, 0.0, 0.0, 0.0, 50.0),
+              lambda i: (2.0 + 0.3 * math.sin(50 * (h * i)), 0.0, 0.0, 0.0, 5.0), _left(n))
+    return _amr("sr_perturbed_amr", RELATIVISTIC, 1, 5.0 / 3.0, 0.35, W, 10.0, 2, 2)
+
+
+def sr_mild_shock_2d_amr(n: int) -> AmrScenario:  # :90-120
+    W = _disc(n, _const(10.0, 0.0, 0.0, 0.0, 13.3), _const(1.0, 0.0, 0.0, 0.0, 0.0), _centre(n, 0.2))
+    return _amr("sr_mild_shock_2d_amr", RELATIVISTIC, 2, 5.0 / 3.0, 0.2, W, 10.0, 2, 2)
+
+
+def sr_strong_blast_2d_amr(n: int) -> AmrScenario:  # :122-152
+    W = _disc(n, _const(1.0, 0.0, 0.0, 0.0, 1000.0), _const(1.0, 0.0, 0.0, 0.0, 0.01), _centre(n, 0.2))
+    return _amr("sr_strong_blast_2d_amr", RELATIVISTIC, 2, 5.0 / 3.0, 0.2, W, 1000.0, 1, 2)
+
+
+def sr_perturbed_2d_amr(n: int) -> AmrScenario:  # :154-187
+    h = 1.0 / n
+
+    def outside(i, j):
+        a, b = h * (i - (0.5 * n)), h * (j - (0.5 * n))
+        return (2.0 + 0.3 * math.sin(50.0 * math.sqrt((a * a) + (b * b))), 0.0, 0.0, 0.0, 5.0)
+
+    W = _disc(n, _const(5.0, 0.0, 0.0, 0.0, 50.0), outside, _centre(n, 0.2))
+    return _amr("sr_perturbed_2d_amr", RELATIVISTIC, 2, 5.0 / 3.0, 0.175, W, 20.0, 2, 2)
+
+
+def _bh_amr(make, levels):
+    def f(n: int) -> AmrScenario:
+        S = make(n)
+        return AmrScenario(S, 5.0, 2, levels)
+
+    f.__name__ = make.__name__ + "_amr"
+    return f
+
+
+# Tests/BlackHoleEulerAMRTests.cpp:3-131: the black-hole scenario states; main.cpp runs the 1D ones at
+# level 2 and the 2D ones at level 1 (its 2D level-2 driver is truncated)
+bh_spherical_1d_amr = _bh_amr(bh_spherical_1d, 2)
+bh_spinning_1d_amr = _bh_amr(bh_spinning_1d, 2)
+bh_spherical_2d_amr = _bh_amr(bh_spherical_2d, 1)
+bh_spinning_2d_amr = _bh_amr(bh_spinning_2d, 1)
+bh_spinning_spheroidal_2d_amr = _bh_amr(bh_spinning_spheroidal_2d, 1)
+
+AMR_ALL = [euler_toro1_amr, euler_toro1_2d_amr, sr_mild_shock_amr, sr_strong_blast_amr, sr_perturbed_amr,
+           sr_mild_shock_2d_amr, sr_strong_blast_2d_amr, sr_perturbed_2d_amr, bh_spherical_1d_amr,
+           bh_spinning_1d_amr, bh_spherical_2d_amr, bh_spinning_2d_amr, bh_spinning_spheroidal_2d_amr]
+
