@@ -256,45 +256,45 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_refine_tile(Grid c, int dim
     __syncthreads();
     const int rows_per_pass = kAmrThreads / TX;  // 4 in 2D, 1 in 1D
     for (int ti = threadIdx.x / TX; ti < TY; ti += rows_per_pass) {
-    const int tj = threadIdx.x % TX;
-    const int i = i0 + ti, j = j0 + tj;
-    if (i >= c.ny || j >= c.nx) continue;
-    const int r = dim == 1 ? 0 : ti + 1, q = tj + 1;
-    auto at = [&](int rr, int qq, double* out) {
+        const int tj = threadIdx.x % TX;
+        const int i = i0 + ti, j = j0 + tj;
+        if (i >= c.ny || j >= c.nx) continue;
+        const int r = dim == 1 ? 0 : ti + 1, q = tj + 1;
+        auto at = [&](int rr, int qq, double* out) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) out[k] = U[(k * H + rr) * W + qq];
-    };
-    double um[5], un[5];
-    at(r, q, um);
-    at(r, q - 1, un);
-    bool hit = jump_scaled(un, um, dx, inv_dx, use_inv, tol);
-    at(r, q + 1, un);
-    hit |= jump_scaled(um, un, dx, inv_dx, use_inv, tol);
-    if (dim == 2) {
-        at(r - 1, q, un);
-        hit |= jump_scaled(un, um, dx, inv_dx, use_inv, tol);
-        if (diag && i + 1 >= c.ny && j + 1 >= c.nx) {
-            const double Wd[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
-            Phys::p2c(Wd, un, P);
-        } else {
-            at(r + 1, diag ? q + 1 : q, un);
-        }
+            for (int k = 0; k < 5; ++k) out[k] = U[(k * H + rr) * W + qq];
+        };
+        double um[5], un[5];
+        at(r, q, um);
+        at(r, q - 1, un);
+        bool hit = jump_scaled(un, um, dx, inv_dx, use_inv, tol);
+        at(r, q + 1, un);
         hit |= jump_scaled(um, un, dx, inv_dx, use_inv, tol);
-    }
-    if (hit) {
-        const long idx = (long)i * c.nx + j;
-        const int fnx = 2 * c.nx;
-        cflags[idx] = 0;
-        if (dim == 1) {
-            fflags[2 * j] = 1;
-            fflags[2 * j + 1] = 1;
-        } else {
-            fflags[(long)(2 * i) * fnx + 2 * j] = 1;
-            fflags[(long)(2 * i) * fnx + 2 * j + 1] = 1;
-            fflags[(long)(2 * i + 1) * fnx + 2 * j] = 1;
-            fflags[(long)(2 * i + 1) * fnx + 2 * j + 1] = 1;
+        if (dim == 2) {
+            at(r - 1, q, un);
+            hit |= jump_scaled(un, um, dx, inv_dx, use_inv, tol);
+            if (diag && i + 1 >= c.ny && j + 1 >= c.nx) {
+                const double Wd[5] = {1.0, 0.0, 0.0, 0.0, 1.0};
+                Phys::p2c(Wd, un, P);
+            } else {
+                at(r + 1, diag ? q + 1 : q, un);
+            }
+            hit |= jump_scaled(um, un, dx, inv_dx, use_inv, tol);
         }
-    }
+        if (hit) {
+            const long idx = (long)i * c.nx + j;
+            const int fnx = 2 * c.nx;
+            cflags[idx] = 0;
+            if (dim == 1) {
+                fflags[2 * j] = 1;
+                fflags[2 * j + 1] = 1;
+            } else {
+                fflags[(long)(2 * i) * fnx + 2 * j] = 1;
+                fflags[(long)(2 * i) * fnx + 2 * j + 1] = 1;
+                fflags[(long)(2 * i + 1) * fnx + 2 * j] = 1;
+                fflags[(long)(2 * i + 1) * fnx + 2 * j + 1] = 1;
+            }
+        }
     }
 }
 
