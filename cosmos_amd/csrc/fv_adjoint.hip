@@ -828,6 +828,16 @@ int cosmos_adjoint_replay(cosmos_adjoint* a, const double* prim_in, double* prim
     });
 }
 
+int cosmos_adjoint_time_steps(cosmos_adjoint* a, double* out, int capacity, int* count) {
+    return adj_guarded(a, [&] {
+        if (!count) return adj_fail(a, COSMOS_ERR_ARGUMENT, "null count");
+        *count = (int)a->dts.size();
+        if (out)
+            for (int k = 0; k < capacity && k < (int)a->dts.size(); ++k) out[k] = a->dts[k];
+        return COSMOS_OK;
+    });
+}
+
 int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* grad_in) {
     return adj_guarded(a, [&] {
         if (a->dts.empty()) return adj_fail(a, COSMOS_ERR_STATE, "no recorded trajectory (cosmos_adjoint_forward)");

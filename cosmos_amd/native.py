@@ -127,6 +127,7 @@ SIGNATURES = {
                                                ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "cosmos_adjoint_backward": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
     "cosmos_adjoint_replay": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
+    "cosmos_adjoint_time_steps": (ctypes.c_int, [_handle, _c_double_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
                                             ctypes.c_int, _c_double_p]),

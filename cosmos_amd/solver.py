@@ -436,6 +436,14 @@ class Adjoint:
         self._check(self._lib.cosmos_adjoint_backward(self._h, _dp(go), _dp(gi)))
         return gi
 
+    def time_steps(self) -> np.ndarray:
+        """The dt schedule the last forward pass recorded."""
+        n = ctypes.c_int()
+        self._check(self._lib.cosmos_adjoint_time_steps(self._h, None, 0, ctypes.byref(n)))
+        out = np.empty(n.value)
+        self._check(self._lib.cosmos_adjoint_time_steps(self._h, _dp(out), n.value, ctypes.byref(n)))
+        return out
+
     def replay(self, prim_in: np.ndarray) -> np.ndarray:
         a = self._arr(prim_in)
         out = np.empty(self.shape)

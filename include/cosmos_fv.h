@@ -363,6 +363,10 @@ int cosmos_adjoint_backward(cosmos_adjoint* a, const double* grad_out, double* g
 /* The recorded dt schedule applied to another initial state (for finite-
  * difference checks); the solver's state is left at the result. */
 int cosmos_adjoint_replay(cosmos_adjoint* a, const double* prim_in, double* prim_out);
+/* The recorded dt schedule: *count = steps recorded, out[0 .. min(capacity,
+ * count)) = their time steps (out may be NULL to query the count).  No
+ * counterpart in the reference (its solve loops keep dt local). */
+int cosmos_adjoint_time_steps(cosmos_adjoint* a, double* out, int capacity, int* count);
 /* Black hole (COSMOS_BLACKHOLE contexts; set the spacetime and excision on
  * the context's solver).  One RK4 forcing step of the solver's state
  * (bias 0 and van Leer, as the solve loop fixes them) and its adjoint, as

@@ -99,6 +99,13 @@ class CpuPath:
             self.lib.orc_amr_level1.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _I, _P, _P, u8, u8,
                                                 ctypes.POINTER(_D)]
             self.lib.orc_set_c2p.argtypes = [_D, _I]
+            self.lib.orc_gr_set_c2p.argtypes = [_D, _I]
+            self.lib.orc_gr_run_dts.restype = None
+            self.lib.orc_gr_run_dts.argtypes = [_I, _I, _P, _I, _I, _D, _P, _I, _D, _I, _I, _D, _D, _D, _D, _D, _D]
+            self.lib.orc_trace_begin.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_long]
+            self.lib.orc_trace_end.restype = ctypes.c_long
+            self.lib.orc_trace_replay.argtypes = [ctypes.POINTER(ctypes.c_ubyte), ctypes.c_long]
+            self.lib.orc_events_read.argtypes = [ctypes.POINTER(ctypes.c_longlong)]
             self.lib.orc_set_threads.argtypes = [_I]
             self.lib.orc_gr_set_excision.argtypes = [_D]
             self.lib.orc_gr_amr_level1.restype = _I
@@ -219,6 +226,43 @@ class CpuPath:
                               *hole, ctypes.byref(t))
         return a, t.value, n
 
+    def gr_run_dts(self, W, order, h, dts, hole, bias=0.0, limiter=1, subcycles=1, gamma=1.4):
+        """The black-hole solve loop's steps with the time steps dts held fixed."""
+        a = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dim, nx, ny = self._dims(a)
+        d = np.ascontiguousarray(dts, dtype=np.float64)
+        self.lib.orc_gr_run_dts(order, dim, _dp(a), nx, ny, h, _dp(d), len(d), bias, limiter, subcycles, gamma, *hole)
+        return a
+
+    def traced(self, fn, capacity=1 << 26):
+        """(fn(), trace bytes): the slope limiter's branch trace of fn, run on one thread."""
+        buf = np.zeros(capacity, dtype=np.uint8)
+        self.set_threads(1)
+        self.lib.orc_trace_begin(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), capacity)
+        try:
+            res = fn()
+        finally:
+            n = self.lib.orc_trace_end()
+            self.set_threads(0)
+        if n > capacity:
+            raise ValueError(f"trace of {n} bytes exceeds the {capacity}-byte buffer")
+        return res, buf[:n]
+
+    def on_pieces(self, trace, fn):
+        """fn() run on one thread with every slope component held to the
+        clamps and limiter piece `trace` (from traced()) recorded for it."""
+        buf = np.ascontiguousarray(trace, dtype=np.uint8)
+        self.set_threads(1)
+        self.lib.orc_trace_replay(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)), len(buf))
+        try:
+            res = fn()
+        finally:
+            n = self.lib.orc_trace_end()
+            self.set_threads(0)
+        if n != len(buf):
+            raise ValueError(f"replayed run evaluated {n} slope components, the trace holds {len(buf)}")
+        return res
+
     def gr_point(self, what, v, x, y, z, h, hole, gamma=1.4):
         out = np.full(44, np.nan)
         a = np.ascontiguousarray(v if v is not None else np.zeros(5), dtype=np.float64)
@@ -318,6 +362,32 @@ class CpuPath:
     # oracle-only controls
     def set_con2prim(self, tol, max_iterations):
         self.lib.orc_set_c2p(tol, max_iterations)
+
+    def gr_set_con2prim(self, tol, max_iterations=1000):
+        """Black-hole con2prim: tol > 0 replaces every per-call tolerance; 0 restores the reference's."""
+        self.lib.orc_gr_set_c2p(tol, max_iterations)
+
+    def events_reset(self):
+        self.lib.orc_events_reset()
+
+    def events(self):
+        """Special-relativistic con2prim events since the last reset:
+        (cap hits, q floors, density floors, pressure floors)."""
+        out = (ctypes.c_longlong * 4)()
+        self.lib.orc_events_read(out)
+        return tuple(int(v) for v in out)
+
+    def gr_run_nan_counts(self, W, order, h, cfl, hole, steps, subcycles=1, gamma=1.4):
+        """(state, per-step counts of cells holding a NaN) of a black-hole solve of `steps` steps."""
+        buf = np.full(steps, -1, dtype=np.intc)
+        fn = self._f("set_step_nan_counts")
+        fn.argtypes = [ctypes.POINTER(ctypes.c_int), _I]
+        fn(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), steps)
+        try:
+            out = self.gr_run(W, order, h, cfl, hole, max_steps=steps, subcycles=subcycles, gamma=gamma)
+        finally:
+            fn(None, 0)
+        return out[0], buf
 
     def set_threads(self, n):
         self.lib.orc_set_threads(n)

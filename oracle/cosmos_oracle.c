@@ -36,6 +36,33 @@ void orc_set_c2p(double tol, int max_iterations) {
 
 void orc_set_threads(int n) { g_threads = n; }
 
+int orc_thread_count(void) {
+#ifdef _OPENMP
+    return g_threads > 0 ? g_threads : omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Con2prim event counters (a diagnostic with no counterpart in the reference,
+ * which accepts these states silently): [0] solves that ran to the iteration
+ * cap, [1] solves whose (tau + D)^2 - |S|^2 fell below 1e-8 (root taken at
+ * sqrt(1e-8)), [2] solves whose density and [3] whose pressure were raised to
+ * 1e-8 (RelativisticEuler/RelativisticEulerStateVector.cpp:45-50, 59-80,
+ * 93-100).  Each distinct solve of a sweep is counted once: every half step,
+ * interface flux and cell update of a line is evaluated once here (the
+ * reference repeats some with identical inputs). */
+static long long g_events[4];
+
+void orc_events_reset(void) { memset(g_events, 0, sizeof g_events); }
+
+void orc_events_read(long long* out) { memcpy(out, g_events, sizeof g_events); }
+
+static void count_event(int k) {
+#pragma omp atomic
+    g_events[k] += 1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Newtonian physics                                                        */
 /* ------------------------------------------------------------------------ */
@@ -130,6 +157,7 @@ static void sr_con2prim(const double* U, double g, double* out_rho, double* out_
     if (q < 1e-8) {
         dc = D / sqrt(1e-8);
         ec = (D + tau) / sqrt(1e-8);
+        count_event(1);
     }
     double gm1g = (g - 1.0) / g;
     double c0 = -1.0 / (g * g);
@@ -138,7 +166,7 @@ static void sr_con2prim(const double* U, double g, double* out_rho, double* out_
     double c4 = (ec * ec) - 1.0;
     double eta = (2.0 * dc) * gm1g;
     double x = 1.0;
-    int it = 0;
+    int it = 0, converged = 0;
     while (it < g_c2p_maxit) {
         double x2 = x * x;
         double x3 = x2 * x;
@@ -147,11 +175,13 @@ static void sr_con2prim(const double* U, double g, double* out_rho, double* out_
         double xn = x - (f / fd);
         if (fabs(x - xn) < g_c2p_tol) {
             it = g_c2p_maxit;
+            converged = 1;
         } else {
             it += 1;
             x = xn;
         }
     }
+    if (!converged) count_event(0);
     double lf = ((0.5 * ec) * x) * (1.0 + sqrt(1.0 + ((4.0 * gm1g) * ((1.0 - (dc * x)) / ((ec * ec) * (x * x))))));
     double h = 1.0 / (dc * x);
     double rho = D / lf;
@@ -160,8 +190,14 @@ static void sr_con2prim(const double* U, double g, double* out_rho, double* out_
     out_v[1] = Sy / rhw;
     out_v[2] = Sz / rhw;
     double p = (rhw - D) - tau;
-    if (rho < 1e-8) rho = 1e-8;
-    if (p < 1e-8) p = 1e-8;
+    if (rho < 1e-8) {
+        rho = 1e-8;
+        count_event(2);
+    }
+    if (p < 1e-8) {
+        p = 1e-8;
+        count_event(3);
+    }
     *out_rho = rho;
     *out_p = p;
     *out_lf = lf;
@@ -244,6 +280,68 @@ double orc_limiter(double r, double bias, int limiter) {
     return std_min(1.0, gradient_ratio(r, bias));
 }
 
+/* Which piece of computeSlopeLimiter's definition (:8-69) gives phi(r): 0 for
+ * r < 0 (or NaN), then the pieces in the order the reference tests them, the
+ * std::min candidates last (1 the first, 2 the gradient ratio, 3 the constant). */
+static int limiter_piece(double r, double bias, int limiter) {
+    if (!(r >= 0.0)) return 0;
+    double xi = gradient_ratio(r, bias);
+    if (limiter == 0) {
+        if (r < 0.5) return 4;
+        if (r < 1.0) return 5;
+        double m = std_min(r, xi);
+        return (2.0 < m) ? 3 : ((xi < r) ? 2 : 1);
+    }
+    if (limiter == 1) return (xi < ((2.0 * r) / (1.0 + r))) ? 2 : 1;
+    if (r < 1.0) return 4;
+    return (xi < 1.0) ? 2 : 3;
+}
+
+/* Branch trace (a diagnostic for finite-difference checks): while a buffer is
+ * set, each slope component appends one byte, bit 0 = |dl| clamped, bit 1 =
+ * |dr| clamped, bits 2-4 = limiter_piece.  The byte order is the call order,
+ * so it is only meaningful with one thread (orc_set_threads(1)). */
+static unsigned char* g_trace;
+static long g_trace_cap, g_trace_len;
+static int g_trace_replay;
+
+void orc_trace_begin(unsigned char* buf, long capacity) {
+    g_trace = buf;
+    g_trace_cap = capacity;
+    g_trace_len = 0;
+    g_trace_replay = 0;
+}
+
+/* Replay a recorded trace instead: each slope component takes the clamps and
+ * the limiter piece the trace recorded for it, whatever its own differences
+ * are.  The forward map then follows the pieces of the recorded run: the map
+ * a reverse pass that differentiates the branches taken (subgradients at the
+ * kinks) is the derivative of. */
+void orc_trace_replay(unsigned char* buf, long length) {
+    g_trace = buf;
+    g_trace_cap = length;
+    g_trace_len = 0;
+    g_trace_replay = 1;
+}
+
+long orc_trace_end(void) {
+    g_trace = NULL;
+    g_trace_replay = 0;
+    return g_trace_len;
+}
+
+/* phi(r) on a given piece of limiter_piece's numbering. */
+static double limiter_on_piece(double r, double bias, int limiter, int piece) {
+    switch (piece) {
+        case 0: return 0.0;
+        case 1: return limiter == 1 ? (2.0 * r) / (1.0 + r) : r;
+        case 2: return gradient_ratio(r, bias);
+        case 3: return limiter == 0 ? 2.0 : 1.0;
+        case 4: return limiter == 0 ? 2.0 * r : r;
+        default: return 1.0;
+    }
+}
+
 /* computeSlopeVector(cons L, M, R) (:71-101): the ratio clamps |d| < 1e-5 to
  * +1e-5, discarding the sign. */
 void orc_slope(const double* ul, const double* um, const double* ur, double bias, int limiter, double* out) {
@@ -252,9 +350,23 @@ void orc_slope(const double* ul, const double* um, const double* ur, double bias
         double dr = ur[k] - um[k];
         double s = ((0.5 * (1.0 + bias)) * dl) + ((0.5 * (1.0 - bias)) * dr);
         double num = dl, den = dr;
-        if (fabs(num) < 1e-5) num = 1e-5;
-        if (fabs(den) < 1e-5) den = 1e-5;
+        int bits = 0;
+        if (g_trace && g_trace_replay) {
+            int rec = g_trace_len < g_trace_cap ? g_trace[g_trace_len] : 0;
+            g_trace_len += 1;
+            if (rec & 1) num = 1e-5;
+            if (rec & 2) den = 1e-5;
+            out[k] = s * limiter_on_piece(num / den, bias, limiter, rec >> 2);
+            continue;
+        }
+        if (fabs(num) < 1e-5) num = 1e-5, bits |= 1;
+        if (fabs(den) < 1e-5) den = 1e-5, bits |= 2;
         out[k] = s * orc_limiter(num / den, bias, limiter);
+        if (g_trace) {
+            if (g_trace_len < g_trace_cap)
+                g_trace[g_trace_len] = (unsigned char)(bits | (limiter_piece(num / den, bias, limiter) << 2));
+            g_trace_len += 1;
+        }
     }
 }
 
@@ -291,19 +403,25 @@ static void half_step_cons(int phys, int axis, const double* ul, const double* u
  * computeFORCEFlux (:21-24).  LF and Richtmyer both evaluate p2c and the
  * instance flux (p2c, then the cons flux) of each state; the values are the
  * same, so they are evaluated once. */
-static void force_flux_states(int phys, int axis, const double* wl, const double* wr, double dt_dx,
-                              double dx_dt, double g, double* out) {
-    double ul[5], ur[5], fl[5], fr[5], us[5], fs[5];
-    p2c(phys, wl, ul, g);
-    p2c(phys, wr, ur, g);
-    flux(phys, axis, ul, fl, g);
-    flux(phys, axis, ur, fr, g);
+static void force_flux_cons(int phys, int axis, const double* ul, const double* fl, const double* ur,
+                            const double* fr, double dt_dx, double dx_dt, double g, double* out) {
+    double us[5], fs[5];
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (dt_dx * (fl[k] - fr[k])));
     flux(phys, axis, us, fs, g);
     for (int k = 0; k < 5; ++k) {
         double lf = 0.5 * ((fl[k] + fr[k]) + (dx_dt * (ul[k] - ur[k])));
         out[k] = 0.5 * (lf + fs[k]);
     }
+}
+
+static void force_flux_states(int phys, int axis, const double* wl, const double* wr, double dt_dx,
+                              double dx_dt, double g, double* out) {
+    double ul[5], ur[5], fl[5], fr[5];
+    p2c(phys, wl, ul, g);
+    p2c(phys, wr, ur, g);
+    flux(phys, axis, ul, fl, g);
+    flux(phys, axis, ur, fr, g);
+    force_flux_cons(phys, axis, ul, fl, ur, fr, dt_dx, dx_dt, g, out);
 }
 
 void orc_force_flux(int physics, int axis, const double* l, const double* r, double dx, double dt,
@@ -378,8 +496,11 @@ static void update_line(int phys, int order, int axis, double* line, long stride
         for (int k = 1; k < m - 2; ++k)
             force_flux_states(phys, axis, p1 + 5 * k, p0 + 5 * (k + 1), dt_dx, dx_dt, g, fx + 5 * k);
     } else {
+        /* the flux of each cell's conserved state once (p1 is free in FORCE) */
+        for (int k = 0; k < m; ++k) flux(phys, axis, cons + 5 * k, p1 + 5 * k, g);
         for (int k = 0; k < m - 1; ++k)
-            force_flux_states(phys, axis, ext + 5 * k, ext + 5 * (k + 1), dt_dx, dx_dt, g, fx + 5 * k);
+            force_flux_cons(phys, axis, cons + 5 * k, p1 + 5 * k, cons + 5 * (k + 1), p1 + 5 * (k + 1), dt_dx,
+                            dx_dt, g, fx + 5 * k);
     }
     for (int j = 0; j < n; ++j) {
         int k = j + ng; /* ext index of cell j; left interface k - 1, right interface k */

@@ -89,8 +89,34 @@ void orc_gr_set_excision(double radius);
  * (RelativisticEuler/RelativisticEulerStateVector.cpp:59-80); other values
  * select the tighter "config mode".  Returns the previous settings. */
 void orc_set_c2p(double tol, int max_iterations);
+/* Con2prim event counters of the special-relativistic solves since the last
+ * reset: out[0] iteration-cap hits, out[1] q floors, out[2]
+ * density floors, out[3] pressure floors (each distinct solve counted once). */
+void orc_events_reset(void);
+void orc_events_read(long long* out);
+/* Black-hole con2prim controls: tolerance > 0 replaces every tolerance of the
+ * reference's (1e-10 in the x flux, 1e-8 elsewhere); 0 restores them.  The
+ * iteration cap (reference: 1000) applies to every black-hole solve. */
+void orc_gr_set_c2p(double tolerance, int max_iterations);
 /* Number of OpenMP threads used by sweeps (0 = runtime default). */
 void orc_set_threads(int n);
+int orc_thread_count(void);
+/* Per-step NaN cell counts of orc_gr_run (buf[k] = cells holding a NaN after
+ * step k; NULL stops recording). */
+void orc_set_step_nan_counts(int* buf, int capacity);
+/* Black-hole solve loop steps with a given time-step schedule dts[0 .. n). */
+void orc_gr_run_dts(int order, int dim, double* prim, int nx, int ny, double h, const double* dts, int n, double bias,
+                    int limiter, int subcycles, double gamma, double M, double a, double bx, double by, double bz);
+/* Branch trace of the slope limiter: while on, each slope component evaluated
+ * appends one byte to buf (bit 0: |dl| < 1e-5 clamped, bit 1: |dr| clamped,
+ * bits 2-4: the limiter's piece); meaningful with one thread.  orc_trace_end
+ * returns the number of components evaluated (bytes beyond capacity are
+ * counted, not stored). */
+void orc_trace_begin(unsigned char* buf, long capacity);
+/* Replay a recorded trace: each slope component takes the clamps and limiter
+ * piece recorded for it (the forward map on the recorded run's pieces). */
+void orc_trace_replay(unsigned char* buf, long length);
+long orc_trace_end(void);
 
 #ifdef __cplusplus
 }

@@ -283,10 +283,22 @@ typedef struct {
     double rho, v[3], p, lf, h;
 } c2p_t;
 
+/* Con2prim controls (not in the reference): a positive g_gr_tol replaces the
+ * per-call tolerance of every solve; g_gr_maxit caps the Newton iterations
+ * (the reference's 1000 by default). */
+static double g_gr_tol = 0.0;
+static int g_gr_maxit = 1000;
+
+void orc_gr_set_c2p(double tolerance, int max_iterations) {
+    g_gr_tol = tolerance;
+    g_gr_maxit = max_iterations;
+}
+
 /* The con2prim shared by setConservedVariableVector (:30-103), the fluxes and
  * the source term: divide by sqrt(det gamma), then the special-relativistic
  * quartic Newton with flat |S|^2. */
 static void gr_con2prim(const double* U, double sdet, double g, double tol, c2p_t* o) {
+    if (g_gr_tol > 0.0) tol = g_gr_tol;
     double D = U[0] / sdet, Sx = U[1] / sdet, Sy = U[2] / sdet, Sz = U[3] / sdet, tau = U[4] / sdet;
     double q = ((tau + D) * (tau + D)) - (((Sx * Sx) + (Sy * Sy)) + (Sz * Sz));
     double dc = D / sqrt(q);
@@ -302,7 +314,7 @@ static void gr_con2prim(const double* U, double sdet, double g, double tol, c2p_
     double c4 = (ec * ec) - 1.0;
     double eta = (2.0 * dc) * gm1g;
     double x = 1.0;
-    for (int it = 0; it < 1000; ++it) {
+    for (int it = 0; it < g_gr_maxit; ++it) {
         double x2 = x * x, x3 = x2 * x;
         double f = ((((c4 * x3) * (x - eta)) + (c2 * x2)) + (c1 * x)) + c0;
         double fd = ((c1 + ((2.0 * c2) * x)) + ((4.0 * c4) * x3)) - (((3.0 * eta) * c4) * x2);
@@ -599,7 +611,7 @@ int orc_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, dou
     double* out = (double*)malloc(sizeof(double) * 5 * (size_t)nx * ny);
     if (!out) return -1;
     memcpy(out, prim, sizeof(double) * 5 * (size_t)nx * ny);
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_thread_count())
     for (int i = 0; i < ny; ++i) {
         for (int j = 0; j < nx; ++j) {
             if (!stencil_clear(&b, dim, i, j, gh, h)) continue;
@@ -685,7 +697,7 @@ int orc_gr_forcing(int dim, double* prim, int nx, int ny, double h, double dt, d
     double* out = (double*)malloc(sizeof(double) * 5 * (size_t)nx * ny);
     if (!out) return -1;
     memcpy(out, prim, sizeof(double) * 5 * (size_t)nx * ny);
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(orc_thread_count())
     for (int i = 0; i < ny; ++i) {
         for (int j = 0; j < nx; ++j) {
             if (!stencil_clear(&b, dim, i, j, 1, h)) continue;
@@ -735,6 +747,43 @@ double orc_gr_max_wavespeed(int dim, const double* prim, int nx, int ny, double 
 /* BlackHoleEuler{First,Second}OrderSolver::solve{,2D} loops (..FirstOrderSolver.cpp:183-253,
  * ..SecondOrderSolver.cpp:145-218), capped at max_steps: forcing(dt/2N) x N, the flux
  * step (Strang in 2D), forcing(dt/2N) x N.  The forcing uses bias 0 and van Leer. */
+/* One step of the solve loops: forcing, the flux step, forcing. */
+static void gr_step(int order, int dim, double* prim, int nx, int ny, double h, double dt, double bias, int lim,
+                    int subcycles, double gamma, double M, double a, double bx, double by, double bz) {
+    for (int s = 0; s < subcycles; ++s)
+        orc_gr_forcing(dim, prim, nx, ny, h, 0.5 * (dt / subcycles), 0.0, 1, gamma, M, a, bx, by, bz);
+    if (dim == 1) {
+        orc_gr_sweep(order, 0, 1, prim, nx, 1, h, dt, bias, lim, gamma, M, a, bx, by, bz);
+    } else {
+        orc_gr_sweep(order, 0, 2, prim, nx, ny, h, dt * 0.5, bias, lim, gamma, M, a, bx, by, bz);
+        orc_gr_sweep(order, 1, 2, prim, nx, ny, h, dt, bias, lim, gamma, M, a, bx, by, bz);
+        orc_gr_sweep(order, 0, 2, prim, nx, ny, h, dt * 0.5, bias, lim, gamma, M, a, bx, by, bz);
+    }
+    for (int s = 0; s < subcycles; ++s)
+        orc_gr_forcing(dim, prim, nx, ny, h, 0.5 * (dt / subcycles), 0.0, 1, gamma, M, a, bx, by, bz);
+}
+
+/* Per-step NaN cell counts of orc_gr_run: while a buffer is set, step k's
+ * count of cells holding a NaN goes to buf[k]. */
+static int* g_step_nans;
+static int g_step_nans_cap;
+
+void orc_set_step_nan_counts(int* buf, int capacity) {
+    g_step_nans = buf;
+    g_step_nans_cap = buf ? capacity : 0;
+}
+
+static void record_step_nans(const double* prim, long cells, int step) {
+    if (!g_step_nans || step >= g_step_nans_cap) return;
+    int n = 0;
+    for (long c = 0; c < cells; ++c) {
+        int bad = 0;
+        for (int k = 0; k < 5; ++k) bad |= isnan(prim[5 * c + k]);
+        n += bad;
+    }
+    g_step_nans[step] = n;
+}
+
 int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, double cfl, double tf, int max_steps,
                double bias, int lim, int subcycles, double gamma, double M, double a, double bx, double by, double bz,
                double* t_out) {
@@ -744,22 +793,21 @@ int orc_gr_run(int order, int dim, double* prim, int nx, int ny, double h, doubl
     while (t < tf && it < max_steps) {
         double ws = orc_gr_max_wavespeed(dim, prim, nx, ny, gamma, M, a, bx, by, bz);
         double dt = orc_stable_dt_scalar(cfl * (h / ws), t, tf, it);
-        for (int s = 0; s < subcycles; ++s)
-            orc_gr_forcing(dim, prim, nx, ny, h, 0.5 * (dt / subcycles), 0.0, 1, gamma, M, a, bx, by, bz);
-        if (dim == 1) {
-            orc_gr_sweep(order, 0, 1, prim, nx, 1, h, dt, bias, lim, gamma, M, a, bx, by, bz);
-        } else {
-            orc_gr_sweep(order, 0, 2, prim, nx, ny, h, dt * 0.5, bias, lim, gamma, M, a, bx, by, bz);
-            orc_gr_sweep(order, 1, 2, prim, nx, ny, h, dt, bias, lim, gamma, M, a, bx, by, bz);
-            orc_gr_sweep(order, 0, 2, prim, nx, ny, h, dt * 0.5, bias, lim, gamma, M, a, bx, by, bz);
-        }
-        for (int s = 0; s < subcycles; ++s)
-            orc_gr_forcing(dim, prim, nx, ny, h, 0.5 * (dt / subcycles), 0.0, 1, gamma, M, a, bx, by, bz);
+        gr_step(order, dim, prim, nx, ny, h, dt, bias, lim, subcycles, gamma, M, a, bx, by, bz);
         t += dt;
+        record_step_nans(prim, (long)nx * ny, it);
         it += 1;
     }
     if (t_out) *t_out = t;
     return it;
+}
+
+/* The same steps with a given time-step schedule dts[0 .. n) (the schedule a
+ * recorded forward pass chose, held fixed for finite differences). */
+void orc_gr_run_dts(int order, int dim, double* prim, int nx, int ny, double h, const double* dts, int n, double bias,
+                    int lim, int subcycles, double gamma, double M, double a, double bx, double by, double bz) {
+    if (dim == 1) ny = 1;
+    for (int k = 0; k < n; ++k) gr_step(order, dim, prim, nx, ny, h, dts[k], bias, lim, subcycles, gamma, M, a, bx, by, bz);
 }
 
 /* ------------------------------------------------------------------------ */

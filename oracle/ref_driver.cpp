@@ -347,7 +347,43 @@ void ref_slope(const double* ul, const double* um, const double* ur, double bias
 #include "Solvers/BlackHoleEulerForcingSolver.hpp"
 #include "Solvers/BlackHoleEulerSecondOrderSolver.hpp"
 
+// Per-step NaN cell counts of ref_gr_run (a diagnostic of this driver): while a
+// buffer is set, step k's count of cells holding a NaN goes to buf[k].
+static int* g_step_nans = nullptr;
+static int g_step_nans_cap = 0;
+
+template <class S>
+static int count_nan_cells(std::vector<S>& cells) {
+    int n = 0;
+    for (size_t j = 0; j < cells.size(); ++j) {
+        double w[5];
+        store_state(cells[j], w);
+        bool bad = false;
+        for (int k = 0; k < 5; ++k) bad = bad || (w[k] != w[k]);
+        n += bad ? 1 : 0;
+    }
+    return n;
+}
+
+template <class S>
+static void record_step_nans(std::vector<S>& cells, int step) {
+    if (g_step_nans && step < g_step_nans_cap) g_step_nans[step] = count_nan_cells(cells);
+}
+
+template <class S>
+static void record_step_nans(std::vector<std::vector<S> >& cells, int step) {
+    if (!g_step_nans || step >= g_step_nans_cap) return;
+    int n = 0;
+    for (size_t i = 0; i < cells.size(); ++i) n += count_nan_cells(cells[i]);
+    g_step_nans[step] = n;
+}
+
 extern "C" {
+
+void ref_set_step_nan_counts(int* buf, int capacity) {
+    g_step_nans = buf;
+    g_step_nans_cap = buf ? capacity : 0;
+}
 
 int ref_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, double h, double dt, double bias,
                  int lim, double gamma, double M, double a, double bx, double by, double bz) {
@@ -424,6 +460,7 @@ int ref_gr_run(int order, int dim, double* prim, int nx, int ny, double h, doubl
             for (int s = 0; s < subcycles; ++s)
                 BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep(c, h, 0.5 * (dt / subcycles), 0.0, 1, mp, bh);
             t += dt;
+            record_step_nans(c, it);
             it += 1;
         }
         from_1d(c, prim);
@@ -445,6 +482,7 @@ int ref_gr_run(int order, int dim, double* prim, int nx, int ny, double h, doubl
             for (int s = 0; s < subcycles; ++s)
                 BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(c, h, 0.5 * (dt / subcycles), 0.0, 1, mp, bh);
             t += dt;
+            record_step_nans(c, it);
             it += 1;
         }
         from_2d(c, prim);

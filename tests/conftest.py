@@ -36,6 +36,36 @@ def golden(name: str):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
 
 
+# Special-relativistic primitive states that drive con2prim off its regular
+# path (RelativisticEuler/RelativisticEulerStateVector.cpp:40-100, 118-123):
+# |v| > 1 (p2c clamps W, the conserved state has (tau + D)^2 < |S|^2 and no
+# root: Newton runs to its iteration cap), |v| just below 1 (the W clamp), a
+# density and a pressure below the 1e-8 floors, a negative pressure, and
+# |v| ~ 0.99.
+SR_EXTREME_STATES = (
+    (1.0, 1.2, 0.3, 0.0, 1.0),
+    (1.0, 0.9999999999, 0.0, 0.0, 1.0),
+    (1e-9, 0.1, 0.0, 0.0, 1.0),
+    (1.0, 0.0, 0.2, 0.0, 1e-12),
+    (1.0, 0.0, 0.0, 0.0, -0.5),
+    (1.0, -0.7, 0.7, 0.0, 1.0),
+)
+
+
+def sr_extreme_2d(nx: int, ny: int, seed: int) -> np.ndarray:
+    """The seeded SR blast cut to ny x nx with each state of SR_EXTREME_STATES
+    planted (three times each where the grid is large enough) at seeded cells."""
+    from cosmos_amd import workloads
+
+    W = workloads.sr_blast_2d(max(nx, ny), seed=seed)[:ny, :nx].copy()
+    rng = np.random.default_rng(seed)
+    flat = W.reshape(-1, 5)
+    idx = rng.choice(nx * ny, size=min(nx * ny, 3 * len(SR_EXTREME_STATES)), replace=False)
+    for n, k in enumerate(idx):
+        flat[k] = SR_EXTREME_STATES[n % len(SR_EXTREME_STATES)]
+    return W
+
+
 def same(a, b) -> bool:
     """Bitwise-equal values (NaN at the same places counts as equal)."""
     a = np.asarray(a)

@@ -5,7 +5,7 @@ Run in a container that has the reference checked out (it is not needed
 afterwards; the fixtures are committed):
     make -C oracle ref && python tests/golden/make_golden.py
     (python tests/golden/make_golden.py amr_level2 writes only amr_level2.npz,
-     ... gr_amr only gr_amr.npz)
+     ... gr_amr only gr_amr.npz, ... gr_nan_onset only gr_nan_onset.npz)
 
 Every fixture stores inputs and expected outputs as raw float64 arrays.
 """
@@ -101,6 +101,20 @@ def gr_amr_fixtures(ref) -> None:
     np.savez_compressed(os.path.join(OUT, "gr_amr.npz"), **fx)
 
 
+GR_NAN_ONSET_STEPS = 40
+
+
+def gr_nan_onset_fixture(ref) -> None:
+    """The perturbed black-hole medium (2D, 64^2 on [-20M, 20M]^2, seed 1,
+    gamma 4/3, SLIC + RK4 forcing, CFL 0.4, the reference's horizon excision)
+    through the reference's solve loop for 40 steps: the count of cells holding
+    a NaN after each step, and the final state (its NaN cells included)."""
+    W, h, hole = workloads.gr_random_field_2d(64, seed=1)
+    out, counts = ref.gr_run_nan_counts(W, 2, h, 0.4, hole, GR_NAN_ONSET_STEPS, gamma=4.0 / 3.0)
+    np.savez_compressed(os.path.join(OUT, "gr_nan_onset.npz"), nan_counts=counts.astype(np.int32), output=out,
+                        meta=np.array([64, 1, h, 0.4, 4.0 / 3.0, GR_NAN_ONSET_STEPS]))
+
+
 def main() -> None:
     ref = load_ref()
     if ref is None:
@@ -110,6 +124,9 @@ def main() -> None:
         return
     if sys.argv[1:] == ["gr_amr"]:
         gr_amr_fixtures(ref)
+        return
+    if sys.argv[1:] == ["gr_nan_onset"]:
+        gr_nan_onset_fixture(ref)
         return
 
     # 1. Toro test 1 (Sod), 400 cells, SLIC + van Leer, CFL 0.8, to t = 0.25 through the
@@ -234,6 +251,7 @@ def main() -> None:
     np.savez_compressed(os.path.join(OUT, "amr.npz"), **fx)
     amr_level2_fixtures(ref)
     gr_amr_fixtures(ref)
+    gr_nan_onset_fixture(ref)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -165,3 +165,30 @@ def test_batch_full_workload_against_solver():
             W = b.download(k)
             assert same(W, E), (k, p, rel_linf(W, E))
             assert t[k] == te and steps[k] == ne
+
+
+def test_batch_relativistic_extreme_states_bitwise(orc):
+    """Special-relativistic problems holding rootless, clamped and floored
+    states (1D and 2D, whole-block and tiled) in one batch with regular
+    Newtonian ones, each bitwise the oracle's solve loop."""
+    from conftest import sr_extreme_2d
+
+    shapes = [(17, 9), (40, 33), (300, 70), (5, 4)]
+    problems, states = [], []
+    for k, (nx, ny) in enumerate(shapes):
+        W = sr_extreme_2d(nx, ny, seed=3 + k)
+        for dim, st in ((2, W), (1, W[ny // 2].copy())):
+            states.append(st)
+            problems.append({"physics": 1, "dim": dim, "nx": nx, "ny": ny if dim == 2 else 1, "cell_spacing": 1 / 40,
+                             "cfl": 0.5, "final_time": math.inf, "max_steps": 4 + k, "adiabatic_index": G_SR})
+    rng = np.random.default_rng(9)
+    states.append(_state((24, 20), rng, 0))
+    problems.append({"physics": 0, "dim": 2, "nx": 20, "ny": 24, "cell_spacing": 1 / 24, "cfl": 0.9,
+                     "final_time": math.inf, "max_steps": 6, "adiabatic_index": G_N})
+    exp = oracle_runs(orc, problems, states, 2, 0.0, 1)
+    for sched in (None, {1: {"whole_max_cells": 0}}):
+        outs, t, steps, _ = run_batch(problems, states, schedules=sched)
+        for k, (W, te, ne) in enumerate(exp):
+            assert steps[k] == ne and t[k] == te, (k, sched)
+            assert same(outs[k], W), (k, sched, rel_linf(outs[k], W))
+

@@ -156,3 +156,40 @@ def test_gr_excision_override(orc):
         out = s.download()
     assert same(out, exp), rel_linf(out, exp)
     assert (t, n) == (te, ne)
+
+
+@pytest.mark.parametrize("hole", HOLES[:2])
+def test_gr_tight_con2prim_bitwise(orc, hole):
+    """The tight con2prim mode (every black-hole con2prim at tolerance 1e-13,
+    at most 200 iterations: cosmos_set_con2prim on a black-hole solver, the
+    mode of the gradient checks) bitwise against the oracle's same mode:
+    sweeps of both orders and axes, the RK4 forcing and a forced solve."""
+    W = random_state((12, 20), seed=31)
+    h = 1.0 / 20
+    orc.gr_set_con2prim(1e-13, 200)
+    try:
+        for order in (1, 2):
+            for axis in (0, 1):
+                with solver_for(W, hole) as s:
+                    s.set_con2prim(1e-13, 200)
+                    s.sweep(order, axis, h, 0.2 * h, 0.0, 1)
+                    out = s.download()
+                exp = orc.gr_sweep(W, order, axis, h, 0.2 * h, hole, 0.0, 1, G_GR)
+                assert same(out, exp), (order, axis, rel_linf(out, exp))
+        with solver_for(W, hole) as s:
+            s.set_con2prim(1e-13, 200)
+            s.forcing(h, 0.1 * h, 0.0, 1)
+            assert same(s.download(), orc.gr_forcing(W, h, 0.1 * h, hole, 0.0, 1, G_GR))
+        W2, h2, hole2 = workloads.gr_random_field_2d(64, seed=3)
+        orc.gr_set_excision(2.5)
+        exp, te, ne = orc.gr_run(W2, 2, h2, 0.4, hole2, max_steps=4, subcycles=1, gamma=4.0 / 3.0)
+        with solver_for(W2, hole2, 4.0 / 3.0) as s:
+            s.set_con2prim(1e-13, 200)
+            s.set_excision(2.5)
+            t, n = s.solve_forced(2, h2, 0.4, max_steps=4, subcycles=1)
+            out = s.download()
+        assert same(out, exp), rel_linf(out, exp)
+        assert (t, n) == (te, ne)
+    finally:
+        orc.gr_set_con2prim(0.0, 1000)
+        orc.gr_set_excision(0.0)

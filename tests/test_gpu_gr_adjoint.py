@@ -201,3 +201,123 @@ def test_gr_backward_repeats_and_recorded_segment(steps, every):
     assert np.abs(grads[0]).max() > 1e-6
     for g in grads[1:]:
         assert close(grads[0], g)
+
+
+# ---------------------------------------------------------------------------
+# The config-5 gradient against the oracle's finite differences.  The
+# perturbed medium at 64^2 (seed 7, target seed 8, excision 2.5M, tight
+# con2prim), 4 steps, L = sum (rho_N - rho*)^2, 32 seeded directions of the
+# initial density field.  L(W + he) - L(W - he) is summed cell by cell as
+# (rho+ - rho-)(rho+ + rho- - 2 rho*), which keeps the rounding of L itself out
+# of the quotient.
+# ---------------------------------------------------------------------------
+C5_N, C5_STEPS, C5_DIRS = 64, 4, 32
+
+
+@pytest.fixture(scope="module")
+def c5(orc):
+    from cosmos_amd import workloads
+
+    W, h, hole = workloads.gr_random_field_2d(C5_N, seed=7)
+    target = workloads.gr_random_field_2d(C5_N, seed=8)[0][..., 0]
+    with Adjoint(2, C5_N, C5_N, G, physics=BLACKHOLE) as a:
+        a.set_spacetime(*hole)
+        a.set_excision(2.5)
+        a.set_con2prim(1e-13, 200)
+        a.upload(W)
+        a.forward(2, h, 0.4, C5_STEPS, 0.0, 1, 20)
+        WN = a.download()
+        dts = a.time_steps()
+        g = np.zeros_like(WN)
+        g[..., 0] = 2.0 * (WN[..., 0] - target)
+        grad = a.backward(g)
+    rng = np.random.default_rng(9)
+    dirs = []
+    for _ in range(C5_DIRS):
+        e = np.zeros_like(W)
+        e[..., 0] = rng.standard_normal(W.shape[:-1])
+        dirs.append(e)
+    return dict(W=W, h=h, hole=hole, target=target, WN=WN, dts=dts, grad=grad, dirs=dirs)
+
+
+class _oracle_c5_mode:
+    """The oracle in the workload's mode: excision 2.5M, con2prim 1e-13 / 200."""
+
+    def __init__(self, orc):
+        self.orc = orc
+
+    def __enter__(self):
+        self.orc.gr_set_excision(2.5)
+        self.orc.gr_set_con2prim(1e-13, 200)
+        return self.orc
+
+    def __exit__(self, *exc):
+        self.orc.gr_set_excision(0.0)
+        self.orc.gr_set_con2prim(0.0, 1000)
+
+
+def _fd(p, q, target, hh):
+    return float(np.sum((p[..., 0] - q[..., 0]) * (p[..., 0] + q[..., 0] - 2.0 * target))) / (2.0 * hh)
+
+
+def test_c5_tight_forward_is_the_oracle(orc, c5):
+    """The adjoint's recorded forward pass (GPU, tight con2prim) is bitwise the
+    oracle's solve loop: the same dt schedule and the same final state."""
+    with _oracle_c5_mode(orc) as o:
+        exp, te, ne = o.gr_run(c5["W"], 2, c5["h"], 0.4, c5["hole"], max_steps=C5_STEPS, subcycles=1, gamma=G)
+        t = 0.0
+        for dt in c5["dts"]:
+            t += float(dt)
+        assert ne == C5_STEPS and te == t
+        assert np.array_equal(exp, c5["WN"])
+        assert np.array_equal(o.gr_run_dts(c5["W"], 2, c5["h"], c5["dts"], c5["hole"], gamma=G), c5["WN"])
+
+
+@pytest.mark.parametrize("half", [0, 1])
+def test_c5_gradient_matches_oracle_fd_on_the_forward_pieces(orc, c5, half):
+    """h = 1e-6 central differences of the oracle's forward map held to the
+    branches the unperturbed run took (each slope component keeps its recorded
+    clamps and limiter piece: the map whose derivative the adjoint's
+    subgradients are) agree with the GPU gradient to 1e-6 in every direction."""
+    hh = 1e-6
+    with _oracle_c5_mode(orc) as o:
+        run = lambda W0: o.gr_run_dts(W0, 2, c5["h"], c5["dts"], c5["hole"], gamma=G)  # noqa: E731
+        base, trace = o.traced(lambda: run(c5["W"]))
+        assert np.array_equal(base, c5["WN"])
+        errs = []
+        for e in c5["dirs"][16 * half:16 * half + 16]:
+            p = o.on_pieces(trace, lambda: run(c5["W"] + hh * e))
+            q = o.on_pieces(trace, lambda: run(c5["W"] - hh * e))
+            fd, exact = _fd(p, q, c5["target"], hh), float(np.sum(c5["grad"] * e))
+            errs.append(abs(fd - exact) / max(abs(fd), abs(exact)))
+    print("piece-held FD h=1e-6 rel errors:", " ".join(f"{x:.1e}" for x in errs))
+    assert max(errs) <= TOL, errs
+
+
+def test_c5_gradient_matches_oracle_fd_and_branch_flips(orc, c5):
+    """Free central differences of the oracle's forward map: at h = 1e-8 no
+    slope component changes its clamps or limiter piece between W + he and
+    W - he, and every direction agrees with the GPU gradient to 1e-6; at
+    h = 1e-6 every direction flips tens of them (|delta| < 1e-5 -> +1e-5,
+    Solvers/SlopeLimiters.cpp:71-101, sits where the p = 0.01 medium's tau
+    differences do), which is what moves h = 1e-6 differences off the
+    gradient."""
+    with _oracle_c5_mode(orc) as o:
+        run = lambda W0: o.gr_run_dts(W0, 2, c5["h"], c5["dts"], c5["hole"], gamma=G)  # noqa: E731
+        errs = []
+        for e in c5["dirs"]:
+            p, q = run(c5["W"] + 1e-8 * e), run(c5["W"] - 1e-8 * e)
+            fd, exact = _fd(p, q, c5["target"], 1e-8), float(np.sum(c5["grad"] * e))
+            errs.append(abs(fd - exact) / max(abs(fd), abs(exact)))
+        flips = {}
+        for hh in (1e-8, 1e-6):
+            flips[hh] = []
+            for e in c5["dirs"][:4]:
+                _, tp = o.traced(lambda: run(c5["W"] + hh * e))
+                _, tq = o.traced(lambda: run(c5["W"] - hh * e))
+                flips[hh].append(int(np.count_nonzero(tp != tq)))
+    print("free FD h=1e-8 rel errors:", " ".join(f"{x:.1e}" for x in errs))
+    print("branch flips per direction:", flips)
+    assert max(errs) <= TOL, errs
+    assert all(f == 0 for f in flips[1e-8]), flips
+    assert all(f > 0 for f in flips[1e-6]), flips

@@ -172,6 +172,54 @@ def test_config3_relativistic_reduced_bitwise(orc):
     assert t == te and same(out, exp), rel_linf(out, exp)
 
 
+@pytest.mark.parametrize("nx,ny", [(17, 9), (40, 33), (5, 4), (300, 70)])
+def test_relativistic_extreme_states_bitwise(orc, nx, ny):
+    """Rootless (|v| > 1: Newton to the 1000-iteration cap), W-clamped,
+    density- and pressure-floored and negative-pressure states planted among
+    the blast's cells: every sweep (orders 1 and 2, both axes, every y-sweep
+    layout), the 1D sweep of a row and a short solve, bitwise against the
+    oracle (RelativisticEuler/RelativisticEulerStateVector.cpp:30-136)."""
+    from conftest import sr_extreme_2d
+
+    W = sr_extreme_2d(nx, ny, seed=3)
+    dx, dt = 1.0 / 40, 0.1 / 40
+    for order in (1, 2):
+        for axis in (0, 1):
+            exp = orc.sweep(W, 1, order, axis, dx, dt, 0.0, 1, G_SR)
+            for columns in ((0, 8, 16) if axis == 1 else (-1,)):
+                with Solver(RELATIVISTIC, 2, nx, ny, G_SR) as s:
+                    if columns >= 0:
+                        s.set_y_tile(columns)
+                    s.upload(W)
+                    s.sweep(order, axis, dx, dt, 0.0, 1)
+                    out = s.download()
+                assert same(out, exp), (order, axis, columns, rel_linf(out, exp))
+    row = W[ny // 2].copy()
+    for order in (1, 2):
+        with Solver(RELATIVISTIC, 1, nx, 1, G_SR) as s:
+            s.upload(row)
+            s.sweep(order, 0, dx, dt, 0.0, 1)
+            assert same(s.download(), orc.sweep(row, 1, order, 0, dx, dt, 0.0, 1, G_SR)), order
+    out, t, k = gpu_run(W, RELATIVISTIC, 2, dx, 0.5, max_steps=4, gamma=G_SR)
+    exp, te, ke = orc.run(W, 1, 2, dx, 0.5, max_steps=4, gamma=G_SR)
+    assert (t, k) == (te, ke) and same(out, exp), rel_linf(out, exp)
+
+
+def test_config3_2048_past_the_cap_hitting_steps_bitwise(orc):
+    """The seeded SR blast at its full 2048^2 for 20 steps: from step 13 on,
+    interface states with no con2prim root appear and their Newton solves run
+    to the reference's 1000-iteration cap; the GPU stays bitwise the oracle."""
+    W = workloads.sr_blast_2d(2048, seed=0)
+    out, t, k = gpu_run(W, RELATIVISTIC, 2, 1.0 / 2048, 0.5, max_steps=20, gamma=G_SR)
+    orc.events_reset()
+    exp, te, ke = orc.run(W, 1, 2, 1.0 / 2048, 0.5, max_steps=20, gamma=G_SR)
+    caps = orc.events()[0]
+    print(f"SR blast 2048^2 x 20 steps: {caps} con2prim solves at the iteration cap")
+    assert caps > 0
+    assert k == ke == 20 and t == te
+    assert same(out, exp), rel_linf(out, exp)
+
+
 def test_relativistic_config_mode_con2prim(orc):
     """Tighter con2prim (tol 1e-12, <= 50 iterations) on both sides."""
     W = workloads.sr_blast_2d(96, seed=6)

@@ -97,6 +97,48 @@ def test_sr_unphysical_state_matches_reference(ref, orc):
         assert same(ref.flux(1, 0, u, G_SR), orc.flux(1, 0, u, G_SR))
 
 
+@pytest.mark.parametrize("nx,ny", [(17, 9), (40, 33), (5, 4)])
+def test_sr_extreme_grid_matches_reference(ref, orc, nx, ny):
+    """Grids holding rootless, clamped and floored states among regular cells:
+    every sweep and a short solve bitwise equal to the reference build, and
+    every con2prim event kind (iteration cap, q floor, density and pressure
+    floors) occurs in the second-order sweeps."""
+    from conftest import sr_extreme_2d
+
+    W = sr_extreme_2d(nx, ny, seed=3)
+    seen = np.zeros(4, dtype=np.int64)
+    for order in (1, 2):
+        for axis in (0, 1):
+            orc.events_reset()
+            a = orc.sweep(W, 1, order, axis, 1 / 40, 0.1 / 40, 0.0, 1, G_SR)
+            if order == 2:
+                seen += np.array(orc.events())
+            assert same(a, ref.sweep(W, 1, order, axis, 1 / 40, 0.1 / 40, 0.0, 1, G_SR)), (order, axis)
+    assert (seen[1:] > 0).all(), seen
+    a = ref.run(W, 1, 2, 1 / 40, 0.5, max_steps=3, gamma=G_SR)
+    b = orc.run(W, 1, 2, 1 / 40, 0.5, max_steps=3, gamma=G_SR)
+    assert same(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+    row = W[ny // 2]
+    for order in (1, 2):
+        assert same(ref.sweep(row, 1, order, 0, 1 / 40, 0.1 / 40, 0.0, 1, G_SR),
+                    orc.sweep(row, 1, order, 0, 1 / 40, 0.1 / 40, 0.0, 1, G_SR))
+
+
+def test_sr_extreme_grid_reaches_the_iteration_cap(orc):
+    """The rootless |v| > 1 states make second-order sweeps run Newton to the
+    reference's 1000-iteration cap."""
+    from conftest import sr_extreme_2d
+
+    caps = 0
+    for nx, ny in ((17, 9), (40, 33)):
+        W = sr_extreme_2d(nx, ny, seed=3)
+        for axis in (0, 1):
+            orc.events_reset()
+            orc.sweep(W, 1, 2, axis, 1 / 40, 0.1 / 40, 0.0, 1, G_SR)
+            caps += orc.events()[0]
+    assert caps > 0
+
+
 # ---------------------------------------------------------------------------
 # 2. oracle == golden fixtures (these need no reference build)
 # ---------------------------------------------------------------------------

@@ -218,3 +218,59 @@ def test_excision_radius_override(orc):
         orc.gr_set_excision(0.0)
     assert metric(orc, 0.5 + 0.2 * (1.0 - 1e-9), 0.5, 0.0, hole)["excised"] == 1.0
     assert metric(orc, 0.5 + 0.3, 0.5, 0.0, hole)["excised"] == 0.0
+
+
+def _sr_cons(W, sd, g):
+    """Flat special-relativistic p2c scaled by sqrt(det gamma): the conserved
+    state whose con2prim root is W (the GR con2prim inverts this map, with the
+    flat |S|^2; BlackHoleEuler/BlackHoleEulerStateVector.cpp:30-103)."""
+    rho, v, p = W[0], W[1:4], W[4]
+    lf = 1.0 / math.sqrt(1.0 - float(np.dot(v, v)))
+    h = 1.0 + (p / rho) * (g / (g - 1.0))
+    return sd * np.array([rho * lf, *(rho * h * lf * lf * v), rho * h * lf * lf - p - rho * lf])
+
+
+def test_gr_tight_con2prim_reaches_the_root(orc):
+    """With the tolerance control set (1e-13, the gradient-check mode), c2p
+    returns primitives whose conserved state reproduces the input to rounding;
+    at the reference's tolerance the kept Newton iterate leaves residuals
+    orders of magnitude larger (measured: 1.5e-13 against 1.9e-8).  A control
+    that is not applied fails the first bound.  The other call sites (fluxes,
+    source) are compared bitwise with the GPU's tight mode in the GPU tests."""
+    hole = (1.0, 0.0, 20.0, 20.0, 0.0)
+    rng = np.random.default_rng(21)
+    Ws = states(200, 22)
+    worst = {}
+    for tol in (1e-13, 0.0):
+        orc.gr_set_con2prim(tol, 200 if tol else 1000)
+        try:
+            res = 0.0
+            for W in Ws:
+                x, y = rng.uniform(0.0, 40.0, 2)
+                if math.hypot(x - 20.0, y - 20.0) < 3.0:
+                    continue
+                sd = math.sqrt(orc.gr_point(GR_METRIC, None, x, y, 0.0, 0.04, hole, G)[8])
+                U = _sr_cons(W, sd, G)
+                R = orc.gr_point(GR_C2P, U, x, y, 0.0, 0.04, hole, G)
+                res = max(res, float(np.max(np.abs(_sr_cons(R, sd, G) - U) / np.max(np.abs(U)))))
+            worst[tol] = res
+        finally:
+            orc.gr_set_con2prim(0.0, 1000)
+    assert worst[1e-13] <= 1e-12, worst
+    assert worst[0.0] > 1e3 * worst[1e-13], worst
+
+
+def test_gr_nan_onset_matches_reference_fixture(orc):
+    """The perturbed black-hole medium at 64^2 (gamma 4/3, p = 0.01, CFL 0.4,
+    SLIC + RK4 forcing, the reference's excision) goes NaN in the reference's
+    own arithmetic: no NaN for 33 steps, 24 cells after step 34, then spreading.
+    The oracle reproduces the per-step counts and the state after 40 steps
+    (tests/golden/gr_nan_onset.npz, written from the reference build).  This is
+    why the black-hole workloads run 20 steps, not 200."""
+    g = golden("gr_nan_onset.npz")
+    W, h, hole = workloads.gr_random_field_2d(64, seed=1)
+    out, counts = orc.gr_run_nan_counts(W, 2, h, 0.4, hole, len(g["nan_counts"]), gamma=4.0 / 3.0)
+    assert np.array_equal(counts, g["nan_counts"])
+    assert same(out, g["output"])
+    first = int(np.argmax(counts > 0))
+    assert first == 33 and counts[first] == 24 and (counts[:first] == 0).all()
