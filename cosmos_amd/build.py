@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("COSMOS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["fv_kernels.hip", "fv_gr.hip", "fv_capi.hip", "fv_amr.hip", "fv_adjoint.hip", "fv_gr_adjoint.hip",
-           "fv_batch.hip", "fv_pointwise.hip"]
+           "fv_batch.hip", "fv_pointwise.hip", "fv_noalloc.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",

@@ -127,6 +127,13 @@ SIGNATURES = {
                                                ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "cosmos_adjoint_backward": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
     "cosmos_adjoint_replay": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
+    "cosmos_step_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
+                                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_void_p,
+                                            ctypes.c_void_p]),
+    "cosmos_noalloc_grid_limit": (ctypes.c_size_t, []),
+    "cosmos_noalloc_last_error": (ctypes.c_char_p, []),
     "cosmos_adjoint_time_steps": (ctypes.c_int, [_handle, _c_double_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,

@@ -30,6 +30,8 @@
 #ifndef COSMOS_FV_H
 #define COSMOS_FV_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -404,6 +406,29 @@ int cosmos_slope_vector(const double* left, const double* middle, const double* 
                         int slope_limiter, double* out);
 
 /* Wait for all queued work on the handle's stream. */
+/* One step of the solve loop on caller-owned device memory that allocates
+ * nothing: no counterpart in the reference; for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).  The step is the one
+ * cosmos_solve takes (computeStableTimeStep{,2D}'s dt with the six-step
+ * damping and the final-time clamp, then X(dt/2) Y(dt) X(dt/2) in 2D or one
+ * sweep in 1D; exact arithmetic, Newtonian or special relativistic).
+ * in / out: five SoA FP64 planes of primitives (rho, u, v, w, p), plane k at
+ * base + k * pitch * ny, row i at + i * pitch (pitch >= nx doubles); distinct
+ * buffers; in is not written.  time_step_out: a device double receiving dt
+ * (may be NULL).  stream: a hipStream_t (NULL: the null stream) on the
+ * current device.  Work is queued, not waited for.  Device working memory is
+ * a fixed 32 MiB array per GPU held until the process ends: a 2D grid needs
+ * 5 * pitch * ny <= cosmos_noalloc_grid_limit() doubles (about 838k cells at
+ * pitch = nx), larger grids are rejected with COSMOS_ERR_ARGUMENT.  Calls
+ * share that array: calls that overlap on one GPU race.  Errors:
+ * cosmos_noalloc_last_error() (this thread). */
+int cosmos_step_noalloc(int physics, int dim, int nx, int ny, long pitch, const double* in, double* out, int order,
+                        double cell_spacing, double cfl, double current_time, double final_time,
+                        int current_iteration, double bias, int slope_limiter, double adiabatic_index,
+                        double* time_step_out, void* stream);
+size_t cosmos_noalloc_grid_limit(void);
+const char* cosmos_noalloc_last_error(void);
+
 int cosmos_synchronize(cosmos_solver* s);
 /* The handle's HIP stream (hipStream_t), for callers that time or order work. */
 void* cosmos_stream(cosmos_solver* s);
