@@ -49,6 +49,16 @@ struct cosmos_solver {
     double excision = 0.0;                     // > 0: excision radius override
     double* gr_flux = nullptr;                 // device, interface fluxes of the GR sweeps (lazily allocated)
     long gr_fpitch = 0;
+    // COSMOS_BLACKHOLE geometry cache (lazily allocated): the tables of one
+    // (h, hole); geo_exact when h's half lattice is exact (Geo and neighbour
+    // tables used), the source table otherwise alone
+    int geo_cache = 1;
+    double* geo_tab = nullptr;
+    double* src_tab = nullptr;
+    double* q_tab = nullptr;
+    cfv::GeoTab geo{};
+    bool geo_built = false, geo_exact = false;
+    double geo_key[6] = {0, 0, 0, 0, 0, 0};
     hipStream_t stream = nullptr;
     int device = 0;
     // solve-loop parameters set by cosmos_begin.  Steps are queued without
@@ -276,6 +286,64 @@ cfv::GRLaunch gr_launch(cosmos_solver* s, double h) {
     return L;
 }
 
+// Whether every half-lattice position a h / 2, |a| <= 2 max(nx, ny) + 8, and
+// every sum or difference of two of them is exact: h = m 2^e with m odd and
+// a m < 2^53.
+bool lattice_exact(double h, int nx, int ny) {
+    if (!(h > 1e-290) || !std::isfinite(h)) return false;
+    int e = 0;
+    double f = std::frexp(h, &e);
+    uint64_t m = (uint64_t)std::ldexp(f, 53);
+    while (m && !(m & 1)) m >>= 1;
+    const uint64_t amax = 2ull * (uint64_t)(nx > ny ? nx : ny) + 8ull;
+    return m != 0 && m < (1ull << 53) / amax;
+}
+
+// The geometry cache for spacing h and the solver's hole (built on first use
+// and whenever either changes).  With the cache off or for other physics the
+// launch keeps an empty table and the kernels evaluate at the call sites.
+int ensure_geo(cosmos_solver* s, double h, cfv::GeoTab& tab) {
+    tab = cfv::GeoTab{};
+    if (!s->geo_cache || s->physics != COSMOS_BLACKHOLE) return COSMOS_OK;
+    const double key[6] = {h, s->hole.M, s->hole.a, s->hole.x, s->hole.y, s->hole.z};
+    const bool same = s->geo_built && std::memcmp(key, s->geo_key, sizeof key) == 0;
+    if (!same) {
+        const long rows = s->dim == 2 ? s->ny + 4 : 1;
+        const long pitch = ((long)s->nx + 4 + 15) / 16 * 16;
+        const long spitch = s->pitch;
+        const long qrows = s->dim == 2 ? s->ny + 2 : 1;
+        const long qpitch = ((long)s->nx + 2 + 15) / 16 * 16;
+        if (!s->src_tab) {
+            HIPCHK(s, hipMalloc((void**)&s->src_tab, sizeof(double) * cfv::kSrcVals * (size_t)s->ny * spitch));
+        }
+        const bool exact = lattice_exact(h, s->nx, s->ny);
+        if (exact && !s->geo_tab) {
+            HIPCHK(s, hipMalloc((void**)&s->geo_tab, sizeof(double) * 3 * cfv::kGeoVals * (size_t)rows * pitch));
+            HIPCHK(s, hipMalloc((void**)&s->q_tab, sizeof(double) * 5 * (size_t)qrows * qpitch));
+        }
+        cfv::GeoTab t{};
+        t.pitch = pitch;
+        t.plane = rows * pitch;
+        t.spitch = spitch;
+        t.splane = (long)s->ny * spitch;
+        t.qpitch = qpitch;
+        t.qplane = qrows * qpitch;
+        cfv::GRLaunch L = gr_launch(s, h);
+        L.tab = t;
+        hipError_t e = cfv::launch_gr_geo_build(L, exact ? s->geo_tab : nullptr, s->src_tab, s->stream);
+        if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("geometry cache: ") + hipGetErrorString(e));
+        t.geo = exact ? s->geo_tab : nullptr;
+        t.src = s->src_tab;
+        t.q = exact ? s->q_tab : nullptr;
+        s->geo = t;
+        s->geo_exact = exact;
+        s->geo_built = true;
+        std::memcpy(s->geo_key, key, sizeof key);
+    }
+    tab = s->geo;
+    return COSMOS_OK;
+}
+
 int ensure_gr_flux(cosmos_solver* s) {
     if (s->gr_flux) return COSMOS_OK;
     // x sweeps: ny rows of nx + 1 interfaces; y sweeps: ny + 1 rows of nx
@@ -288,6 +356,7 @@ int gr_sweep_setup(cosmos_solver* s, int order, double h, double dt, double bias
     int rc = ensure_gr_flux(s);
     if (rc) return rc;
     L = gr_launch(s, h);
+    if ((rc = ensure_geo(s, h, L.tab))) return rc;
     L.order = order;
     L.bias = bias;
     L.limiter = limiter;
@@ -310,8 +379,10 @@ int gr_sweep(cosmos_solver* s, int order, int axis, double h, double dt, double 
 // The stage fractions of evolveConservedVariableVector{,2D}
 // (Solvers/BlackHoleEulerForcingSolver.cpp:21-53, 98-167), each times dt / h as
 // computeFractionalEvolutionVector forms it (Solvers/EulerSolvers.cpp:163-167).
-cfv::GRLaunch gr_forcing_setup(cosmos_solver* s, double h, double dt, double bias, int limiter) {
-    cfv::GRLaunch L = gr_launch(s, h);
+int gr_forcing_setup(cosmos_solver* s, double h, double dt, double bias, int limiter, cfv::GRLaunch& L) {
+    L = gr_launch(s, h);
+    int rc = ensure_geo(s, h, L.tab);
+    if (rc) return rc;
     L.bias = bias;
     L.limiter = limiter;
     L.dt = dt;
@@ -320,11 +391,13 @@ cfv::GRLaunch gr_forcing_setup(cosmos_solver* s, double h, double dt, double bia
         L.frac_dt_h[k] = (s->dim == 1 ? f1[k] : fx[k]) * (dt / h);
         L.frac_dt_h_y[k] = fy[k] * (dt / h);
     }
-    return L;
+    return COSMOS_OK;
 }
 
 int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) {
-    cfv::GRLaunch L = gr_forcing_setup(s, h, dt, bias, limiter);
+    cfv::GRLaunch L;
+    int rc = gr_forcing_setup(s, h, dt, bias, limiter, L);
+    if (rc) return rc;
     hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_gr_forcing(L, s->stream); });
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("GR forcing launch: ") + hipGetErrorString(e));
     s->cur ^= 1;
@@ -342,8 +415,7 @@ int gr_sweep_launch(cosmos_solver* s, int order, double h, double dt, double bia
 
 int gr_forcing_launch(cosmos_solver* s, double h, double dt, double bias, int limiter, GRLaunch* L) {
     if (!s || s->physics != COSMOS_BLACKHOLE) return COSMOS_ERR_ARGUMENT;
-    *L = gr_forcing_setup(s, h, dt, bias, limiter);
-    return COSMOS_OK;
+    return gr_forcing_setup(s, h, dt, bias, limiter, *L);
 }
 
 }  // namespace cfv
@@ -417,6 +489,9 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->flag);
     hipFree(s->stage);
     hipFree(s->gr_flux);
+    hipFree(s->geo_tab);
+    hipFree(s->src_tab);
+    hipFree(s->q_tab);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
     return COSMOS_OK;
@@ -485,6 +560,13 @@ int cosmos_set_spacetime(cosmos_solver* s, double mass, double spin, double x, d
         if (s->physics != COSMOS_BLACKHOLE)
             return fail(s, COSMOS_ERR_ARGUMENT, "cosmos_set_spacetime needs a COSMOS_BLACKHOLE solver");
         s->hole = make_hole(mass, spin, x, y, z, s->excision);
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_set_geometry_cache(cosmos_solver* s, int enable) {
+    return guarded(s, [&] {
+        s->geo_cache = enable ? 1 : 0;
         return COSMOS_OK;
     });
 }

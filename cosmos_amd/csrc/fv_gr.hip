@@ -17,6 +17,17 @@
 // per interface is thousands of FP64 operations (metric evaluations and
 // Newton con2prim solves), so the flux round trip through HBM (80 B per
 // interface) is negligible and the kernels are FP64-issue-bound.
+//
+// Geometry cache (GEO = 1, where the host found the half lattice of h exact;
+// the position-only values then come from the tables GeoTab holds instead of
+// being re-derived at each call site, bitwise the same): every Geo of the
+// sweeps and the forcing, each forcing cell's SourceGeo, and the forcing's
+// neighbour states.  A forcing cell converts each neighbour to conserved
+// variables at the neighbour's centre, back to primitives at its own x -/+ h
+// (y -/+ h), and to conserved again there in every fractional sweep; on the
+// exact lattice those positions are the neighbour's centre, so that chain is
+// a function of the neighbour alone, formed once per cell by k_gr_roundtrip
+// (tab.q) instead of four times per neighbour per step.
 #include "fv_kernels.hpp"
 
 namespace cfv {
@@ -61,10 +72,18 @@ __device__ __forceinline__ bool stencil_clear(const Hole& b, int dim, int i, int
     return true;
 }
 
-// position (x, y) moved by off along the sweep axis
-template <int AXIS>
-__device__ __forceinline__ Geo geo_off(const Hole& b, double x, double y, double off) {
-    return AXIS == 0 ? geo_at(b, x + off, y, 0.0) : geo_at(b, x, y + off, 0.0);
+// The Geo at (x, y), half-lattice point (ax, ay): from the table (GEO) or
+// evaluated at the call site's position.
+template <int GEO>
+__device__ __forceinline__ Geo geo_pick(const GRLaunch& L, double x, double y, int ax, int ay) {
+    if constexpr (GEO == 1) return geo_from_table(L.tab, L.dim, ax, ay);
+    else return geo_at(L.hole, x, y, 0.0);
+}
+
+// position (x, y) moved by off (d half-lattice units) along the sweep axis
+template <int AXIS, int GEO>
+__device__ __forceinline__ Geo geo_off(const GRLaunch& L, double x, double y, double off, int ax, int ay, int d) {
+    return AXIS == 0 ? geo_pick<GEO>(L, x + off, y, ax + d, ay) : geo_pick<GEO>(L, x, y + off, ax, ay + d);
 }
 
 // Limited slope of the conserved vectors (SlopeLimiters::computeSlopeVector{X,Y},
@@ -84,22 +103,21 @@ __device__ __forceinline__ void slope5(const double* ul, const double* um, const
 // middle cell centred at (x, y): slope from p2c at x - h, x, x + h; face fluxes
 // at x -/+ h/2; the selected face evolved by 0.5 (dt/h) (F_L - F_R) and
 // converted back at that face.
-template <int AXIS>
-__device__ void gr_half_step(const double* WL, const double* WM, const double* WR, double x, double y, int side,
-                             const GRLaunch& L, double* out) {
-    const Hole& b = L.hole;
+template <int AXIS, int GEO>
+__device__ void gr_half_step(const double* WL, const double* WM, const double* WR, double x, double y, int ax,
+                             int ay, int side, const GRLaunch& L, double* out) {
     const double h = L.h;
     double ul[5], um[5], ur[5], s[5];
     {
-        Geo g = geo_off<AXIS>(b, x, y, -h);
+        Geo g = geo_off<AXIS, GEO>(L, x, y, -h, ax, ay, -2);
         gr_p2c(WL, g, ul, L.P);
     }
     {
-        Geo g = geo_at(b, x, y, 0.0);
+        Geo g = geo_pick<GEO>(L, x, y, ax, ay);
         gr_p2c(WM, g, um, L.P);
     }
     {
-        Geo g = geo_off<AXIS>(b, x, y, h);
+        Geo g = geo_off<AXIS, GEO>(L, x, y, h, ax, ay, 2);
         gr_p2c(WR, g, ur, L.P);
     }
     slope5(ul, um, ur, L.bias, L.limiter, s);
@@ -110,9 +128,9 @@ __device__ void gr_half_step(const double* WL, const double* WM, const double* W
         xr[k] = um[k] + (0.5 * s[k]);
     }
     const double hh = h * 0.5;
-    Geo gl = geo_off<AXIS>(b, x, y, -hh);
+    Geo gl = geo_off<AXIS, GEO>(L, x, y, -hh, ax, ay, -1);
+    Geo gr = geo_off<AXIS, GEO>(L, x, y, hh, ax, ay, 1);
     gr_flux<AXIS>(xl, gl, L.P, fl);
-    Geo gr = geo_off<AXIS>(b, x, y, hh);
     gr_flux<AXIS>(xr, gr, L.P, fr);
     double e[5];
 #pragma unroll
@@ -123,25 +141,23 @@ __device__ void gr_half_step(const double* WL, const double* WM, const double* W
 // compute{X,Y}FORCEFlux (Solvers/BlackHoleEulerFirstOrderSolver.cpp:3-99) at the
 // interface (x, y): the states' p2c and fluxes at the interface -/+ h/2, the
 // Richtmyer state's flux at the interface.
-template <int AXIS>
-__device__ void gr_force(const double* WL, const double* WR, double x, double y, const GRLaunch& L, double* F) {
-    const Hole& b = L.hole;
+template <int AXIS, int GEO>
+__device__ void gr_force(const double* WL, const double* WR, double x, double y, int ax, int ay, const GRLaunch& L,
+                         double* F) {
     const double hh = L.h * 0.5;
     double ul[5], ur[5], fl[5], fr[5], us[5], fs[5];
     {
-        Geo g = geo_off<AXIS>(b, x, y, -hh);
-        gr_p2c(WL, g, ul, L.P);
-        gr_flux<AXIS>(ul, g, L.P, fl);
-    }
-    {
-        Geo g = geo_off<AXIS>(b, x, y, hh);
-        gr_p2c(WR, g, ur, L.P);
-        gr_flux<AXIS>(ur, g, L.P, fr);
+        Geo gl = geo_off<AXIS, GEO>(L, x, y, -hh, ax, ay, -1);
+        Geo gr = geo_off<AXIS, GEO>(L, x, y, hh, ax, ay, 1);
+        gr_p2c(WL, gl, ul, L.P);
+        gr_p2c(WR, gr, ur, L.P);
+        gr_flux<AXIS>(ul, gl, L.P, fl);
+        gr_flux<AXIS>(ur, gr, L.P, fr);
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (L.coef.dt_dx * (fl[k] - fr[k])));
     {
-        Geo g = geo_at(b, x, y, 0.0);
+        Geo g = geo_pick<GEO>(L, x, y, ax, ay);
         gr_flux<AXIS>(us, g, L.P, fs);
     }
 #pragma unroll
@@ -154,7 +170,7 @@ __device__ void gr_force(const double* WL, const double* WR, double x, double y,
 // One flux per interface.  x sweep: thread -> (row i, interface c), c in [0, nx];
 // y sweep: thread -> (interface c, column j), c in [0, ny].  Interface c lies
 // between cells c - 1 and c at (c - 0.5) * h.
-template <int ORDER, int AXIS>
+template <int ORDER, int AXIS, int GEO>
 __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
     const int n = AXIS == 0 ? L.nx : L.ny;
     const long lines = AXIS == 0 ? L.ny : L.nx;
@@ -187,19 +203,24 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
         return AXIS == 0 ? (long)i * L.pitch + m : (long)m * L.pitch + j;
     };
     double x, y;
+    int ax, ay;  // half-lattice point of the interface
     if (AXIS == 0) {
         x = (c - 0.5) * L.h;
         y = L.dim == 2 ? i * L.h : 0.0;
+        ax = 2 * c - 1;
+        ay = L.dim == 2 ? 2 * i : 0;
     } else {
         x = j * L.h;
         y = (c - 0.5) * L.h;
+        ax = 2 * j;
+        ay = 2 * c - 1;
     }
     double F[5];
     if (ORDER == 1) {
         double WL[5], WR[5];
         load5(L.in, at(c - 1), WL);
         load5(L.in, at(c), WR);
-        gr_force<AXIS>(WL, WR, x, y, L, F);
+        gr_force<AXIS, GEO>(WL, WR, x, y, ax, ay, L, F);
     } else {
         // computeXSLICFlux (Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-35): the left
         // cell's right-face state evolved about the interface - h/2, the right cell's
@@ -209,12 +230,12 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
         load5(L.in, at(c - 2), W0);
         load5(L.in, at(c - 1), W1);
         load5(L.in, at(c), W2);
-        if (AXIS == 0) gr_half_step<AXIS>(W0, W1, W2, x - hh, y, 1, L, pr);
-        else gr_half_step<AXIS>(W0, W1, W2, x, y - hh, 1, L, pr);
+        if (AXIS == 0) gr_half_step<AXIS, GEO>(W0, W1, W2, x - hh, y, ax - 1, ay, 1, L, pr);
+        else gr_half_step<AXIS, GEO>(W0, W1, W2, x, y - hh, ax, ay - 1, 1, L, pr);
         load5(L.in, at(c + 1), W0);
-        if (AXIS == 0) gr_half_step<AXIS>(W1, W2, W0, x + hh, y, 0, L, pl);
-        else gr_half_step<AXIS>(W1, W2, W0, x, y + hh, 0, L, pl);
-        gr_force<AXIS>(pr, pl, x, y, L, F);
+        if (AXIS == 0) gr_half_step<AXIS, GEO>(W1, W2, W0, x + hh, y, ax + 1, ay, 0, L, pl);
+        else gr_half_step<AXIS, GEO>(W1, W2, W0, x, y + hh, ax, ay + 1, 0, L, pl);
+        gr_force<AXIS, GEO>(pr, pl, x, y, ax, ay, L, F);
         if (L.half) {
             const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
 #pragma unroll
@@ -231,7 +252,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
 
 // The conservative update of each cell (computeFORCEUpdate with p2c / c2p at
 // the cell centre) where its stencil is clear; a copy elsewhere.
-template <int ORDER, int AXIS>
+template <int ORDER, int AXIS, int GEO>
 __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
     long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
     if (idx >= (long)L.nx * L.ny) return;
@@ -240,7 +261,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
     double W[5];
     load5(L.in, cell, W);
     if (stencil_clear(L.hole, L.dim, i, j, ORDER == 2 ? 2 : 1, L.h)) {
-        Geo g = geo_at(L.hole, j * L.h, L.dim == 2 ? i * L.h : 0.0, 0.0);
+        Geo g = geo_pick<GEO>(L, j * L.h, L.dim == 2 ? i * L.h : 0.0, 2 * j, L.dim == 2 ? 2 * i : 0);
         double U[5], un[5];
         gr_p2c(W, g, U, L.P);
         // x: interfaces j, j + 1 of row i; y: interfaces i, i + 1 of column j
@@ -256,19 +277,13 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
 // evolveStateByFractional{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:171-217):
 // slope from p2c at x - h, x, x + h; face fluxes at x -/+ h (a whole cell,
 // as in the reference); U_M + frac (dt/h) (F_L - F_R) converted back at x.
+// ul, ur: the neighbours' conserved vectors at x -/+ h (formed by the caller).
 template <int AXIS>
-__device__ void gr_fractional(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
-                              const GRLaunch& L, double* out) {
-    const Hole& b = L.hole;
-    const double h = L.h;
-    double ul[5], um[5], ur[5], s[5];
-    // the neighbours' p2c and the face fluxes share their positions x -/+ h
-    const Geo gl = geo_off<AXIS>(b, x, y, -h);
-    const Geo gc = geo_at(b, x, y, 0.0);
-    const Geo gr = geo_off<AXIS>(b, x, y, h);
-    gr_p2c(WL, gl, ul, L.P);
+__device__ __forceinline__ void gr_fractional_core(double fdt_h, const double* ul, const double* WM, const double* ur,
+                                                   const Geo& gl, const Geo& gc, const Geo& gr, const GRLaunch& L,
+                                                   double* out) {
+    double um[5], s[5];
     gr_p2c(WM, gc, um, L.P);
-    gr_p2c(WR, gr, ur, L.P);
     slope5(ul, um, ur, L.bias, L.limiter, s);
     double xl[5], xr[5], fl[5], fr[5];
 #pragma unroll
@@ -284,12 +299,51 @@ __device__ void gr_fractional(double fdt_h, const double* WL, const double* WM, 
     gr_c2p(e, gc.sdet, L.P, out);
 }
 
+template <int AXIS>
+__device__ void gr_fractional(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
+                              const GRLaunch& L, double* out) {
+    const Hole& b = L.hole;
+    const double h = L.h;
+    double ul[5], ur[5];
+    // the neighbours' p2c and the face fluxes share their positions x -/+ h
+    const Geo gl = AXIS == 0 ? geo_at(b, x - h, y, 0.0) : geo_at(b, x, y - h, 0.0);
+    const Geo gc = geo_at(b, x, y, 0.0);
+    const Geo gr = AXIS == 0 ? geo_at(b, x + h, y, 0.0) : geo_at(b, x, y + h, 0.0);
+    gr_p2c(WL, gl, ul, L.P);
+    gr_p2c(WR, gr, ur, L.P);
+    gr_fractional_core<AXIS>(fdt_h, ul, WM, ur, gl, gc, gr, L, out);
+}
+
+// The neighbour chain of a forcing cell on the exact lattice (GEO): for every
+// cell c of [-1, nx] x [-1, ny] (its data clamped into the grid, its geometry
+// at its own centre), p2c(c2p(p2c(W_c))) with the Geo of that centre: what the
+// fractional sweeps of a neighbouring cell convert it to
+// (computeRungeKuttaTimeStep{,2D} p2c at the neighbour's centre, c2p at x -/+ h
+// inside evolveConservedVariableVector{,2D}, p2c at x -/+ h in each sweep).
+template <int DIM>
+__global__ __launch_bounds__(kGRThreads) void k_gr_roundtrip(GRLaunch L, double* q) {
+    const int qw = L.nx + 2, qh = DIM == 2 ? L.ny + 2 : 1;
+    long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
+    if (idx >= (long)qw * qh) return;
+    const int r = (int)(idx / qw), cc = (int)(idx - (long)r * qw);
+    const int i = DIM == 2 ? r - 1 : 0, j = cc - 1;
+    double W[5], U[5], R[5];
+    load5(L.in, (long)clamp1_gr(i, L.ny) * L.pitch + clamp1_gr(j, L.nx), W);
+    const Geo g = geo_from_table(L.tab, DIM, 2 * j, 2 * i);
+    gr_p2c(W, g, U, L.P);
+    gr_c2p(U, g.sdet, L.P, R);
+    gr_p2c(R, g, U, L.P);
+    const long o = (long)r * L.tab.qpitch + cc;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) q[k * L.tab.qplane + o] = U[k];
+}
+
 // computeRungeKuttaTimeStep{,2D} (Solvers/BlackHoleEulerForcingSolver.cpp:187-250) with
 // evolveConservedVariableVector{,2D} (:3-185): per cell, RK4 on the source term,
 // each stage's middle state first evolved by fractional flux sweeps against
 // the step-start neighbours (1D: x by 1/2, 1/2, 1; 2D: x 1/4 y 1/2 x 1/4,
 // twice, then x 1/2 y 1 x 1/2), U + (k1 + 2 k2 + 2 k3 + k4) / 6.
-template <int DIM>
+template <int DIM, int GEO>
 __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
     long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
     if (idx >= (long)L.nx * L.ny) return;
@@ -304,9 +358,71 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
         return;
     }
     const double x = j * h, y = DIM == 2 ? i * h : 0.0;
+    double uM[5];
+    double kk[4][5];
+    double src[5];
+    SourceGeo sg;
+    if constexpr (GEO == 1) {
+        // every position-only value and the neighbours' chain read once, at the
+        // start, from the tables (tab.geo, tab.src, tab.q); the RK4 chain then
+        // runs on registers only
+        const int ax = 2 * j, ay = DIM == 2 ? 2 * i : 0;
+        const Geo gc = geo_from_table(L.tab, DIM, ax, ay);
+        Geo gn[4];  // x-, x+, y-, y+ neighbours' centres (flux geometry)
+        gn[0] = geo_from_table(L.tab, DIM, ax - 2, ay);
+        gn[1] = geo_from_table(L.tab, DIM, ax + 2, ay);
+        if (DIM == 2) {
+            gn[2] = geo_from_table(L.tab, DIM, ax, ay - 2);
+            gn[3] = geo_from_table(L.tab, DIM, ax, ay + 2);
+        }
+        double qn[4][5];
+        {
+            const long qo = (long)(DIM == 2 ? i + 1 : 0) * L.tab.qpitch + (j + 1);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                const double* q = L.tab.q + k * L.tab.qplane + qo;
+                qn[0][k] = q[-1];
+                qn[1][k] = q[1];
+                if (DIM == 2) {
+                    qn[2][k] = q[-L.tab.qpitch];
+                    qn[3][k] = q[L.tab.qpitch];
+                }
+            }
+        }
+        source_from_table(L.tab.src + (long)i * L.tab.spitch + j, L.tab.splane, sg);
+        gr_p2c(W, gc, uM, L.P);
+        gr_source(uM, sg, L.P, src);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
+        for (int st = 0; st < 3; ++st) {
+            double tmp[5], Wm[5], e1[5], um[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
+            gr_c2p(tmp, gc.sdet, L.P, Wm);
+            if (DIM == 1) {
+                gr_fractional_core<0>(L.frac_dt_h[st], qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e1);
+            } else {
+                double e2[5], e3[5];
+                gr_fractional_core<0>(L.frac_dt_h[st], qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e2);
+                gr_fractional_core<1>(L.frac_dt_h_y[st], qn[2], e2, qn[3], gn[2], gc, gn[3], L, e3);
+                gr_fractional_core<0>(L.frac_dt_h[st], qn[0], e3, qn[1], gn[0], gc, gn[1], L, e1);
+            }
+            gr_p2c(e1, gc, um, L.P);
+            gr_source(um, sg, L.P, src);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
+        }
+        double un[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
+        gr_c2p(un, gc.sdet, L.P, W);
+        store5(L.out, cell, W);
+        return;
+    }
     // neighbours' conserved vectors, each formed at its own centre
     // (the grid loop of computeRungeKuttaTimeStep{,2D}) ...
-    double uL[5], uM[5], uR[5], uT[5], uB[5];
+    double uL[5], uR[5], uT[5], uB[5];
     {
         double Wn[5];
         load5(L.in, (long)i * L.pitch + clamp1_gr(j - 1, L.nx), Wn);
@@ -330,11 +446,9 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
         gr_c2p(uT, geo_at(b, x, y - h, 0.0).sdet, L.P, WTn);
         gr_c2p(uB, geo_at(b, x, y + h, 0.0).sdet, L.P, WBn);
     }
-    SourceGeo sg;
-    source_geo(b, x, y, 0.0, h, sg);
-    double kk[4][5];
-    double src[5];
-    gr_source(uM, sg, L.P, src);
+    if (L.tab.src) source_from_table(L.tab.src + (long)i * L.tab.spitch + j, L.tab.splane, sg);
+    else source_geo(b, x, y, 0.0, h, sg);
+    gr_source_of(uM, SrcRef{sg}, L.P, src);
 #pragma unroll
     for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
     for (int st = 0; st < 3; ++st) {
@@ -351,7 +465,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
             gr_fractional<0>(L.frac_dt_h[st], WLn, e3, WRn, x, y, L, e1);
         }
         gr_p2c(e1, gc, um, L.P);
-        gr_source(um, sg, L.P, src);
+        gr_source_of(um, SrcRef{sg}, L.P, src);
 #pragma unroll
         for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
     }
@@ -361,6 +475,36 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
         un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
     gr_c2p(un, gc.sdet, L.P, W);
     store5(L.out, cell, W);
+}
+
+// The geometry tables: geo_at at every half-lattice point of the three
+// classes (centres, x faces, y faces) over the grid plus a two-cell halo, at
+// the exact position (ax h / 2, ay h / 2); source_geo at every cell centre
+// with the forcing's own expressions x = j h, y = i h.
+__global__ __launch_bounds__(kGRThreads) void k_gr_geo_build(GRLaunch L, double* geo, long pitch, long plane,
+                                                             int rows) {
+    const long per = (long)rows * pitch;
+    long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
+    if (idx >= 3 * per) return;
+    const int cls = (int)(idx / per);
+    const long rem = idx - (long)cls * per;
+    const int row = (int)(rem / pitch), col = (int)(rem - (long)row * pitch);
+    const int ax = 2 * col - 4 + (cls & 1);
+    const int ay = L.dim == 2 ? 2 * row - 4 + (cls >> 1) : 0;
+    if (L.dim == 1 && cls == 2) return;
+    const double hh = 0.5 * L.h;
+    const Geo G = geo_at(L.hole, ax * hh, ay * hh, 0.0);
+    geo_to_table(G, geo + (long)cls * kGeoVals * plane + (long)row * pitch + col, plane);
+}
+
+__global__ __launch_bounds__(kGRThreads) void k_gr_src_build(GRLaunch L, double* src, long spitch, long splane) {
+    long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
+    if (idx >= (long)L.nx * L.ny) return;
+    const int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
+    const double x = j * L.h, y = L.dim == 2 ? i * L.h : 0.0;
+    SourceGeo S;
+    source_geo(L.hole, x, y, 0.0, L.h, S);
+    source_to_table(S, src + (long)i * spitch + j, splane);
 }
 
 // computeMaximumWaveSpeed{,2D} (Solvers/BlackHoleEulerSolvers.cpp:88-140): the
@@ -395,34 +539,71 @@ dim3 blocks_for(long work) { return dim3((unsigned)((work + kGRThreads - 1) / kG
 
 }  // namespace
 
-hipError_t launch_gr_sweep(const GRLaunch& L, int axis, hipStream_t s) {
+namespace {
+
+template <int GEO>
+hipError_t launch_gr_sweep_t(const GRLaunch& L, int axis, hipStream_t s) {
     const dim3 block(kGRThreads);
     const long ifaces = axis == 0 ? (long)(L.nx + 1) * L.ny : (long)(L.ny + 1) * L.nx;
     if (L.order == 2) {
-        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<2, 0>), blocks_for(ifaces), block, 0, s, L);
-        else hipLaunchKernelGGL((k_gr_iface<2, 1>), blocks_for(ifaces), block, 0, s, L);
+        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<2, 0, GEO>), blocks_for(ifaces), block, 0, s, L);
+        else hipLaunchKernelGGL((k_gr_iface<2, 1, GEO>), blocks_for(ifaces), block, 0, s, L);
     } else {
-        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<1, 0>), blocks_for(ifaces), block, 0, s, L);
-        else hipLaunchKernelGGL((k_gr_iface<1, 1>), blocks_for(ifaces), block, 0, s, L);
+        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<1, 0, GEO>), blocks_for(ifaces), block, 0, s, L);
+        else hipLaunchKernelGGL((k_gr_iface<1, 1, GEO>), blocks_for(ifaces), block, 0, s, L);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const long cells = (long)L.nx * L.ny;
     if (L.order == 2) {
-        if (axis == 0) hipLaunchKernelGGL((k_gr_update<2, 0>), blocks_for(cells), block, 0, s, L);
-        else hipLaunchKernelGGL((k_gr_update<2, 1>), blocks_for(cells), block, 0, s, L);
+        if (axis == 0) hipLaunchKernelGGL((k_gr_update<2, 0, GEO>), blocks_for(cells), block, 0, s, L);
+        else hipLaunchKernelGGL((k_gr_update<2, 1, GEO>), blocks_for(cells), block, 0, s, L);
     } else {
-        if (axis == 0) hipLaunchKernelGGL((k_gr_update<1, 0>), blocks_for(cells), block, 0, s, L);
-        else hipLaunchKernelGGL((k_gr_update<1, 1>), blocks_for(cells), block, 0, s, L);
+        if (axis == 0) hipLaunchKernelGGL((k_gr_update<1, 0, GEO>), blocks_for(cells), block, 0, s, L);
+        else hipLaunchKernelGGL((k_gr_update<1, 1, GEO>), blocks_for(cells), block, 0, s, L);
     }
     return hipGetLastError();
 }
 
+}  // namespace
+
+hipError_t launch_gr_sweep(const GRLaunch& L, int axis, hipStream_t s) {
+    return L.tab.geo ? launch_gr_sweep_t<1>(L, axis, s) : launch_gr_sweep_t<0>(L, axis, s);
+}
+
 hipError_t launch_gr_forcing(const GRLaunch& L, hipStream_t s) {
     const long cells = (long)L.nx * L.ny;
-    if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
-    else hipLaunchKernelGGL((k_gr_forcing<2>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
+    if (L.tab.geo && L.tab.src && L.tab.q) {
+        // the neighbour chain once per cell, then the forcing
+        const long qn = (long)(L.nx + 2) * (L.dim == 2 ? L.ny + 2 : 1);
+        double* q = const_cast<double*>(L.tab.q);
+        if (L.dim == 1) hipLaunchKernelGGL((k_gr_roundtrip<1>), blocks_for(qn), dim3(kGRThreads), 0, s, L, q);
+        else hipLaunchKernelGGL((k_gr_roundtrip<2>), blocks_for(qn), dim3(kGRThreads), 0, s, L, q);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1, 1>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
+        else hipLaunchKernelGGL((k_gr_forcing<2, 1>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
+        return hipGetLastError();
+    }
+    if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1, 0>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
+    else hipLaunchKernelGGL((k_gr_forcing<2, 0>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
     return hipGetLastError();
+}
+
+hipError_t launch_gr_geo_build(const GRLaunch& L, double* geo, double* src, hipStream_t s) {
+    if (geo) {
+        const long rows = L.dim == 2 ? L.ny + 4 : 1;
+        hipLaunchKernelGGL(k_gr_geo_build, blocks_for(3 * rows * L.tab.pitch), dim3(kGRThreads), 0, s, L, geo,
+                           L.tab.pitch, L.tab.plane, (int)rows);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (src) {
+        hipLaunchKernelGGL(k_gr_src_build, blocks_for((long)L.nx * L.ny), dim3(kGRThreads), 0, s, L, src,
+                           L.tab.spitch, L.tab.splane);
+        return hipGetLastError();
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_gr_wavespeed(const GRLaunch& L, unsigned long long* slots, hipStream_t s) {

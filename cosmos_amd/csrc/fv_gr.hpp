@@ -177,57 +177,58 @@ struct C2PRec {
     double R[5];           // (rho, v^x, v^y, v^z, p), rho and p floored
 };
 
-CFV_HD void gr_con2prim_rec(const double* U, double sdet, double tol, const PhysParams& P, C2PRec& c) {
+// The three parts of the solve: the undensitised state and the quartic's
+// coefficients (setup), the Newton iteration, and the primitives from its
+// kept iterate (finish).
+struct C2PCoef {
+    double c1, c2x2, c4x4, e3c4;
+};
+
+CFV_HD void c2p_setup(const double* U, double sdet, const PhysParams& P, C2PRec& c, C2PCoef& k) {
     double is = 1.0 / sdet;
-    double D = div_by(U[0], sdet, is);
-    double Sx = div_by(U[1], sdet, is);
-    double Sy = div_by(U[2], sdet, is);
-    double Sz = div_by(U[3], sdet, is);
-    double tau = div_by(U[4], sdet, is);
-    double q = ((tau + D) * (tau + D)) - (((Sx * Sx) + (Sy * Sy)) + (Sz * Sz));
-    double sq = (q < 1e-8) ? __builtin_sqrt(1e-8) : __builtin_sqrt(q);
-    double isq = 1.0 / sq;
-    double dc = div_by(D, sq, isq);
-    double ec = div_by(D + tau, sq, isq);
-    double c1 = (-2.0 * dc) * P.sr_c1f;
-    double c2 = ((P.sr_gm2g * ((ec * ec) - 1.0)) + 1.0) - (((dc * dc) * P.sr_gm1g) * P.sr_gm1g);
-    double c4 = (ec * ec) - 1.0;
-    double eta = (2.0 * dc) * P.sr_gm1g;
-    double c0 = P.sr_c0;
-    double c2x2 = 2.0 * c2, c4x4 = 4.0 * c4, e3c4 = (3.0 * eta) * c4;
-    double x = 1.0, fd = 1.0;
-    for (int it = 0; it < P.c2p_maxit; ++it) {
-        double x2 = x * x;
-        double x3 = x2 * x;
-        double f = ((((c4 * x3) * (x - eta)) + (c2 * x2)) + (c1 * x)) + c0;
-        fd = ((c1 + (c2x2 * x)) + (c4x4 * x3)) - (e3c4 * x2);
-        double xn = x - (f / fd);
-        if (__builtin_fabs(x - xn) < tol) break;
-        x = xn;
-    }
+    c.D = div_by(U[0], sdet, is);
+    c.S[0] = div_by(U[1], sdet, is);
+    c.S[1] = div_by(U[2], sdet, is);
+    c.S[2] = div_by(U[3], sdet, is);
+    c.tau = div_by(U[4], sdet, is);
+    const double D = c.D, Sx = c.S[0], Sy = c.S[1], Sz = c.S[2], tau = c.tau;
+    c.q = ((tau + D) * (tau + D)) - (((Sx * Sx) + (Sy * Sy)) + (Sz * Sz));
+    c.sq = (c.q < 1e-8) ? __builtin_sqrt(1e-8) : __builtin_sqrt(c.q);
+    double isq = 1.0 / c.sq;
+    c.dc = div_by(D, c.sq, isq);
+    c.ec = div_by(D + tau, c.sq, isq);
+    const double dc = c.dc, ec = c.ec;
+    k.c1 = (-2.0 * dc) * P.sr_c1f;
+    c.c2 = ((P.sr_gm2g * ((ec * ec) - 1.0)) + 1.0) - (((dc * dc) * P.sr_gm1g) * P.sr_gm1g);
+    c.c4 = (ec * ec) - 1.0;
+    c.eta = (2.0 * dc) * P.sr_gm1g;
+    k.c2x2 = 2.0 * c.c2;
+    k.c4x4 = 4.0 * c.c4;
+    k.e3c4 = (3.0 * c.eta) * c.c4;
+}
+
+// One Newton step of the quartic at x: returns x_new, fd = f'(x).
+CFV_HD double c2p_newton(const C2PRec& c, const C2PCoef& k, double c0, double x, double& fd) {
+    double x2 = x * x;
+    double x3 = x2 * x;
+    double f = ((((c.c4 * x3) * (x - c.eta)) + (c.c2 * x2)) + (k.c1 * x)) + c0;
+    fd = ((k.c1 + (k.c2x2 * x)) + (k.c4x4 * x3)) - (k.e3c4 * x2);
+    return x - (f / fd);
+}
+
+CFV_HD void c2p_finish(C2PRec& c, double x, double fd, const PhysParams& P) {
+    const double dc = c.dc, ec = c.ec, D = c.D, tau = c.tau;
     double lf = ((0.5 * ec) * x) * (1.0 + __builtin_sqrt(1.0 + (P.sr_4gm1g * ((1.0 - (dc * x)) / ((ec * ec) * (x * x))))));
     double h = 1.0 / (dc * x);
     double rho = D / lf;
     double rhw = (rho * h) * (lf * lf);
     double irhw = 1.0 / rhw;
-    c.R[1] = div_by(Sx, rhw, irhw);
-    c.R[2] = div_by(Sy, rhw, irhw);
-    c.R[3] = div_by(Sz, rhw, irhw);
+    c.R[1] = div_by(c.S[0], rhw, irhw);
+    c.R[2] = div_by(c.S[1], rhw, irhw);
+    c.R[3] = div_by(c.S[2], rhw, irhw);
     double p = (rhw - D) - tau;
     c.R[0] = (rho < 1e-8) ? 1e-8 : rho;
     c.R[4] = (p < 1e-8) ? 1e-8 : p;
-    c.D = D;
-    c.S[0] = Sx;
-    c.S[1] = Sy;
-    c.S[2] = Sz;
-    c.tau = tau;
-    c.q = q;
-    c.sq = sq;
-    c.dc = dc;
-    c.ec = ec;
-    c.c2 = c2;
-    c.c4 = c4;
-    c.eta = eta;
     c.x = x;
     c.fd = fd;
     c.lf = lf;
@@ -235,6 +236,18 @@ CFV_HD void gr_con2prim_rec(const double* U, double sdet, double tol, const Phys
     c.rho = rho;
     c.rhw = rhw;
     c.p = p;
+}
+
+CFV_HD void gr_con2prim_rec(const double* U, double sdet, double tol, const PhysParams& P, C2PRec& c) {
+    C2PCoef k;
+    c2p_setup(U, sdet, P, c, k);
+    double x = 1.0, fd = 1.0;
+    for (int it = 0; it < P.c2p_maxit; ++it) {
+        double xn = c2p_newton(c, k, P.sr_c0, x, fd);
+        if (__builtin_fabs(x - xn) < tol) break;
+        x = xn;
+    }
+    c2p_finish(c, x, fd, P);
 }
 
 // R = (rho, v^x, v^y, v^z, p); lf = W; h = specific enthalpy.
@@ -432,28 +445,130 @@ CFV_HD void source_geo(const Hole& b, double x, double y, double z, double h, So
         }
 }
 
+// ---------------------------------------------------------------------------
+// Geometry cache.  Every metric quantity depends on position only, and the
+// positions the kernels evaluate form the half lattice (ax h / 2, ay h / 2):
+// cell centres (ax, ay even), x interfaces (ax odd) and y interfaces (ay odd).
+// Where h's significand is short enough that every such position and every
+// sum or difference of two of them is exact (h = 40 / 1024 = 5 * 2^-7, for
+// one), the expressions the reference forms at its call sites ((c - 0.5) * h,
+// x - h, (j - 1) * h, x + 0.5 * h ...) all land on the exact lattice value,
+// so a table of geo_at over the lattice, built with the same function, gives
+// the same bits as evaluating at the call site.  The source term's position
+// part (SourceGeo) is formed at x = j h, y = i h with the step's h, the
+// expressions of its only call site, so its table is exact for any h.
+// ---------------------------------------------------------------------------
+constexpr int kGeoVals = 14;  // g[9], sdet, alpha, beta[3]
+constexpr int kSrcVals = 78;  // alpha, sdet, beta[3], gm[9], gi4[16], da[3], db[9], dg[27], K[9]
+
+struct GeoTab {
+    const double* geo;  // three classes (centres, x faces, y faces) of kGeoVals planes, rows x pitch each
+    long pitch;         // doubles per row: nx + 4 points (x index (ax + 4) / 2, ax from -4 / -3)
+    long plane;         // rows * pitch; rows = ny + 4 (2D, y index (ay + 4) / 2) or 1 (1D)
+    const double* src;  // kSrcVals planes of ny x spitch: SourceGeo at the cell centres
+    long spitch, splane;
+    const double* q;    // roundtrip planes (5) over cells [-1, nx] x [-1, ny], pitch qpitch
+    long qpitch, qplane;
+};
+
+CFV_HD long geo_index(const GeoTab& T, int dim, int ax, int ay) {
+    const int cls = (ax & 1) | ((ay & 1) << 1);
+    const long row = dim == 2 ? (long)((ay + 4) >> 1) : 0;
+    return cls * kGeoVals * T.plane + row * T.pitch + ((ax + 4) >> 1);
+}
+
+CFV_HD Geo geo_from_table(const GeoTab& T, int dim, int ax, int ay) {
+    const double* p = T.geo + geo_index(T, dim, ax, ay);
+    Geo G;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) G.g[k] = p[k * T.plane];
+    G.sdet = p[9 * T.plane];
+    G.alpha = p[10 * T.plane];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) G.beta[k] = p[(11 + k) * T.plane];
+    return G;
+}
+
+CFV_HD void geo_to_table(const Geo& G, double* p, long plane) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) p[k * plane] = G.g[k];
+    p[9 * plane] = G.sdet;
+    p[10 * plane] = G.alpha;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p[(11 + k) * plane] = G.beta[k];
+}
+
+CFV_HD void source_to_table(const SourceGeo& S, double* p, long plane) {
+    int k = 0;
+    p[(k++) * plane] = S.alpha;
+    p[(k++) * plane] = S.sdet;
+    for (int i = 0; i < 3; ++i) p[(k++) * plane] = S.beta[i];
+    for (int i = 0; i < 9; ++i) p[(k++) * plane] = S.gm[i];
+    for (int i = 0; i < 16; ++i) p[(k++) * plane] = S.gi4[i];
+    for (int i = 0; i < 3; ++i) p[(k++) * plane] = S.da[i];
+    for (int i = 0; i < 9; ++i) p[(k++) * plane] = S.db[i];
+    for (int i = 0; i < 27; ++i) p[(k++) * plane] = S.dg[i];
+    for (int i = 0; i < 9; ++i) p[(k++) * plane] = S.K[i];
+}
+
+CFV_HD void source_from_table(const double* p, long plane, SourceGeo& S) {
+    int k = 0;
+    S.alpha = p[(k++) * plane];
+    S.sdet = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) S.beta[i] = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) S.gm[i] = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) S.gi4[i] = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) S.da[i] = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) S.db[i] = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) S.dg[i] = p[(k++) * plane];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) S.K[i] = p[(k++) * plane];
+}
+
+// Read access to a position's SourceGeo (the source term is written against
+// this accessor).
+struct SrcRef {
+    const SourceGeo& s;
+    CFV_HD double alpha() const { return s.alpha; }
+    CFV_HD double sdet() const { return s.sdet; }
+    CFV_HD double beta(int k) const { return s.beta[k]; }
+    CFV_HD double gm(int k) const { return s.gm[k]; }
+    CFV_HD double gi4(int k) const { return s.gi4[k]; }
+    CFV_HD double da(int k) const { return s.da[k]; }
+    CFV_HD double db(int k) const { return s.db[k]; }
+    CFV_HD double dg(int k) const { return s.dg[k]; }
+    CFV_HD double K(int k) const { return s.K[k]; }
+};
+
 // computeSourceTermVector from the fluid state and the position's SourceGeo:
 // u^mu (:438-442), T^{mu nu} = rho h u^mu u^nu + p g^{mu nu} (:444-453), the
 // momentum source from d gamma, beta, T, d beta and d alpha (:478-494), the
 // energy source from K and d alpha (:496-507), all times alpha sqrt(gamma) (:509).
-CFV_HD void gr_source(const double* U, const SourceGeo& S, const PhysParams& P, double* out) {
+template <class S>
+CFV_HD void gr_source_of(const double* U, const S& sg, const PhysParams& P, double* out) {
     double R[5], lf, hh;
-    gr_con2prim(U, S.sdet, gr_tol(P, 1e-8), P, R, lf, hh);
+    gr_con2prim(U, sg.sdet(), gr_tol(P, 1e-8), P, R, lf, hh);
     double rho = R[0], p = R[4];
-    double al = S.alpha;
+    double al = sg.alpha();
     double u[4];
     u[0] = lf / al;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) u[1 + k] = ((al * u[0]) * R[1 + k]) - (S.beta[k] * u[0]);
+    for (int k = 0; k < 3; ++k) u[1 + k] = ((al * u[0]) * R[1 + k]) - (sg.beta(k) * u[0]);
     double T[16];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) T[i * 4 + j] = (((rho * hh) * u[i]) * u[j]) + (p * S.gi4[i * 4 + j]);
+        for (int j = 0; j < 4; ++j) T[i * 4 + j] = (((rho * hh) * u[i]) * u[j]) + (p * sg.gi4(i * 4 + j));
     double vc[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        vc[i] = ((S.gm[i * 3 + 0] * R[1]) + (S.gm[i * 3 + 1] * R[2])) + (S.gm[i * 3 + 2] * R[3]);
+        vc[i] = ((sg.gm(i * 3 + 0) * R[1]) + (sg.gm(i * 3 + 1) * R[2])) + (sg.gm(i * 3 + 2) * R[3]);
     double rhw = (rho * hh) * (lf * lf);
     double s[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -462,30 +577,34 @@ CFV_HD void gr_source(const double* U, const SourceGeo& S, const PhysParams& P, 
         for (int j = 0; j < 3; ++j) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                double d = S.dg[i * 9 + j * 3 + k];
-                s[i + 1] += T[0] * (((0.5 * S.beta[j]) * S.beta[k]) * d);
-                s[i + 1] += (T[j + 1] * S.beta[k]) * d;
+                double d = sg.dg(i * 9 + j * 3 + k);
+                s[i + 1] += T[0] * (((0.5 * sg.beta(j)) * sg.beta(k)) * d);
+                s[i + 1] += (T[j + 1] * sg.beta(k)) * d;
                 s[i + 1] += 0.5 * (T[(j + 1) * 4 + (k + 1)] * d);
             }
-            s[i + 1] += ((rhw * vc[j]) / al) * S.db[i * 3 + j];
+            s[i + 1] += ((rhw * vc[j]) / al) * sg.db(i * 3 + j);
         }
-        s[i + 1] -= (T[0] * al) * S.da[i];
+        s[i + 1] -= (T[0] * al) * sg.da(i);
     }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            double K = S.K[i * 3 + j];
-            s[4] += T[0] * ((S.beta[i] * S.beta[j]) * K);
-            s[4] += T[i + 1] * ((2.0 * S.beta[j]) * K);
+            double K = sg.K(i * 3 + j);
+            s[4] += T[0] * ((sg.beta(i) * sg.beta(j)) * K);
+            s[4] += T[i + 1] * ((2.0 * sg.beta(j)) * K);
             s[4] += T[(i + 1) * 4 + (j + 1)] * K;
         }
-        s[4] -= T[0] * (S.beta[i] * S.da[i]);
-        s[4] -= T[i + 1] * S.da[i];
+        s[4] -= T[0] * (sg.beta(i) * sg.da(i));
+        s[4] -= T[i + 1] * sg.da(i);
     }
-    double f = al * S.sdet;
+    double f = al * sg.sdet();
 #pragma unroll
     for (int k = 0; k < 5; ++k) out[k] = f * s[k];
+}
+
+CFV_HD void gr_source(const double* U, const SourceGeo& S, const PhysParams& P, double* out) {
+    gr_source_of(U, SrcRef{S}, P, out);
 }
 
 // computeSoundSpeed (:519-549) with BlackHoleEulerEigenvalues (:3-96): the

@@ -170,9 +170,13 @@ struct GRLaunch {
     double frac_dt_h_y[3];      // 2D y: 1/2 1/2 1
     PhysParams P;
     Hole hole;
+    GeoTab tab;                 // geometry cache of this h (tab.geo null: evaluate at the call sites)
 };
 
 hipError_t launch_gr_sweep(const GRLaunch& L, int axis, hipStream_t s);
+// Fill the geometry cache of L.h, L.hole: geo_at over the half lattice of the
+// grid plus halo (tab.geo) and source_geo at the cell centres (tab.src).
+hipError_t launch_gr_geo_build(const GRLaunch& L, double* geo, double* src, hipStream_t s);
 hipError_t launch_gr_forcing(const GRLaunch& L, hipStream_t s);
 hipError_t launch_gr_wavespeed(const GRLaunch& L, unsigned long long* slots, hipStream_t s);
 

@@ -127,6 +127,7 @@ SIGNATURES = {
                                                ctypes.c_double, ctypes.c_int, ctypes.c_int]),
     "cosmos_adjoint_backward": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
     "cosmos_adjoint_replay": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
+    "cosmos_set_geometry_cache": (ctypes.c_int, [_handle, ctypes.c_int]),
     "cosmos_step_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,

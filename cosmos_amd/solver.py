@@ -156,6 +156,10 @@ class Solver:
                                            slope_limiter, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
 
+    def set_geometry_cache(self, enable: bool) -> None:
+        """Black hole: tabulate the position-only metric values (default on)."""
+        self._check(self._lib.cosmos_set_geometry_cache(self._h, 1 if enable else 0))
+
     def set_excision(self, radius: float) -> None:
         """Excision radius override of a BLACKHOLE solver (<= 0: the reference's horizon)."""
         self._check(self._lib.cosmos_set_excision(self._h, radius))

@@ -116,6 +116,16 @@ int cosmos_set_spacetime(cosmos_solver* s, double mass, double spin, double x, d
  * BlackHoleEuler/BlackHoleSpacetime.cpp:372-386).  radius > 0 replaces that
  * radius (same <= test); radius <= 0 restores it. */
 int cosmos_set_excision(cosmos_solver* s, double radius);
+/* Black-hole geometry cache (no counterpart in the reference; on by default):
+ * the position-only metric values (lapse, shift, spatial metric and its
+ * determinant at every cell centre and interface, and the source term's
+ * derivatives and extrinsic curvature at every centre) tabulated once per
+ * spacing and spacetime in device memory, about 1 KB per cell, instead of
+ * re-derived at every use.  Results are bitwise those of the uncached
+ * evaluation: the interface and neighbour tables are used only when every
+ * position the kernels form is exact (a spacing with a short binary
+ * significand, e.g. 40 / 1024), the source table always.  0 disables it. */
+int cosmos_set_geometry_cache(cosmos_solver* s, int enable);
 
 /* State transfer.  *_device variants take device pointers to the same AoS
  * layout (for callers that already hold the grid in GPU memory). */

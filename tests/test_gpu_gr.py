@@ -193,3 +193,64 @@ def test_gr_tight_con2prim_bitwise(orc, hole):
     finally:
         orc.gr_set_con2prim(0.0, 1000)
         orc.gr_set_excision(0.0)
+
+
+# Holes placed on the half lattice of dyadic spacings (1/32, 1/64) and off it.
+CACHE_HOLES = [(0.05, 0.0, 0.5, 0.5, 0.0), (0.1, 0.9, 0.453125, 0.5625, 0.0), (0.08, 0.5, 0.31, 0.62, 0.02)]
+
+
+def _run_ops(W, hole, h, cache, gamma=G_GR):
+    """Every black-hole operation in turn on one solver: sweeps of both
+    orders and axes (1D: x), the forcing, and a forced solve."""
+    outs = []
+    dim = 1 if W.ndim == 2 else 2
+    for order in (1, 2):
+        for axis in ((0,) if dim == 1 else (0, 1)):
+            for lim, bias in (((1, 0.0), (0, 0.0), (2, 0.3)) if order == 2 else ((1, 0.0),)):
+                with solver_for(W, hole, gamma) as s:
+                    s.set_geometry_cache(cache)
+                    s.sweep(order, axis, h, 0.2 * h, bias, lim)
+                    outs.append(s.download())
+    with solver_for(W, hole, gamma) as s:
+        s.set_geometry_cache(cache)
+        s.forcing(h, 0.1 * h, 0.0, 1)
+        outs.append(s.download())
+    with solver_for(W, hole, gamma) as s:
+        s.set_geometry_cache(cache)
+        t, n = s.solve_forced(2, h, 0.95, max_steps=3, subcycles=1)
+        outs.append(s.download())
+    return outs, (t, n)
+
+
+@pytest.mark.parametrize("hole", CACHE_HOLES)
+@pytest.mark.parametrize("shape,h", [((64,), 1.0 / 64), ((32, 32), 1.0 / 32), ((20, 40), 1.0 / 64),
+                                     ((18, 24), 1.0 / 24)])
+def test_gr_geometry_cache_is_bitwise(orc, hole, shape, h):
+    """The geometry cache (tables of the position-only metric values) gives the
+    bitwise results of evaluating at every call site: with dyadic spacings
+    (1/32, 1/64: every lattice position exact, all tables used) and a
+    non-dyadic one (1/24: the source table alone), for every operation, a
+    Schwarzschild and two Kerr holes; the cached forced solve is also checked
+    against the oracle."""
+    W = random_state(shape, seed=41 + len(shape))
+    on, ton = _run_ops(W, hole, h, True)
+    off, toff = _run_ops(W, hole, h, False)
+    assert ton == toff
+    for k, (a, b) in enumerate(zip(on, off)):
+        assert same(a, b), (k, rel_linf(a, b))
+    exp, te, ne = orc.gr_run(W, 2, h, 0.95, hole, max_steps=3, subcycles=1, gamma=G_GR)
+    assert same(on[-1], exp) and ton == (te, ne)
+
+
+def test_gr_geometry_cache_follows_spacing_and_hole(orc):
+    """The tables are rebuilt when the spacing or the spacetime changes."""
+    W = random_state((16, 16), seed=7)
+    with solver_for(W, HOLES[0]) as s:
+        for h, hole in ((1.0 / 16, HOLES[0]), (1.0 / 32, HOLES[0]), (1.0 / 32, HOLES[1]), (0.05, HOLES[1])):
+            s.set_spacetime(*hole)
+            s.upload(W)
+            s.forcing(h, 0.1 * h, 0.0, 1)
+            assert same(s.download(), orc.gr_forcing(W, h, 0.1 * h, hole, 0.0, 1, G_GR)), (h, hole)
+            s.upload(W)
+            s.sweep(2, 1, h, 0.2 * h)
+            assert same(s.download(), orc.gr_sweep(W, 2, 1, h, 0.2 * h, hole, 0.0, 1, G_GR)), (h, hole)
