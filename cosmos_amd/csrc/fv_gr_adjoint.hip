@@ -61,6 +61,39 @@ __device__ __forceinline__ void zero5(double* g) {
     for (int k = 0; k < 5; ++k) g[k] = 0.0;
 }
 
+// Position-only values from the launch's geometry cache where it has one
+// (bitwise what evaluating at the call site gives), else evaluated there.
+// (ax, ay): the half-lattice point of (x, y).
+__device__ __forceinline__ Geo geo_l(const GRLaunch& L, double x, double y, int ax, int ay) {
+    return L.tab.geo ? geo_from_table(L.tab, L.dim, ax, ay) : geo_at(L.hole, x, y, 0.0);
+}
+
+template <int AXIS>
+__device__ __forceinline__ Geo geo_shift_l(const GRLaunch& L, double x, double y, double off, int ax, int ay, int d) {
+    return AXIS == 0 ? geo_l(L, x + off, y, ax + d, ay) : geo_l(L, x, y + off, ax, ay + d);
+}
+
+template <int AXIS>
+__device__ __forceinline__ adjgr::ForceGeo force_geo_l(const GRLaunch& L, double x, double y, int ax, int ay) {
+    const double hh = L.h * 0.5;
+    return adjgr::ForceGeo{geo_shift_l<AXIS>(L, x, y, -hh, ax, ay, -1), geo_shift_l<AXIS>(L, x, y, hh, ax, ay, 1),
+                           geo_l(L, x, y, ax, ay)};
+}
+
+template <int AXIS>
+__device__ __forceinline__ adjgr::HalfGeo half_geo_l(const GRLaunch& L, double x, double y, int ax, int ay) {
+    const double h = L.h;
+    return adjgr::HalfGeo{geo_shift_l<AXIS>(L, x, y, -h, ax, ay, -2), geo_l(L, x, y, ax, ay),
+                          geo_shift_l<AXIS>(L, x, y, h, ax, ay, 2), geo_shift_l<AXIS>(L, x, y, -h * 0.5, ax, ay, -1),
+                          geo_shift_l<AXIS>(L, x, y, h * 0.5, ax, ay, 1)};
+}
+
+// The source term's position part at cell (i, j) = (x, y).
+__device__ __forceinline__ void source_l(const GRLaunch& L, int i, int j, double x, double y, SourceGeo& sg) {
+    if (L.tab.src) source_from_table(L.tab.src + (long)i * L.tab.spitch + j, L.tab.splane, sg);
+    else source_geo(L.hole, x, y, 0.0, L.h, sg);
+}
+
 // The forward's excision test (stencil_clear in the forward kernels).
 __device__ __forceinline__ bool clear_of(const Hole& b, int dim, int i, int j, int w, double h) {
     double y = dim == 2 ? i * h : 0.0;
@@ -86,7 +119,7 @@ __global__ __launch_bounds__(kGAT) void k_gra_cells(GRAdjLaunch A) {
     if (clear_of(L.hole, L.dim, i, j, ORDER == 2 ? 2 : 1, L.h)) {
         double W[5], U[5], un[5];
         ld5(L.in, cell, W);
-        const Geo g = geo_at(L.hole, j * L.h, L.dim == 2 ? i * L.h : 0.0, 0.0);
+        const Geo g = geo_l(L, j * L.h, L.dim == 2 ? i * L.h : 0.0, 2 * j, L.dim == 2 ? 2 * i : 0);
         gr_p2c(W, g, U, L.P);
         long fl = (long)i * L.fpitch + j;
         long fr = AXIS == 0 ? fl + 1 : fl + L.fpitch;
@@ -137,21 +170,25 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     if (!any) return;
     auto at = [&](int cc) -> long { return cell_at(ORDER == 2 ? mirror2_a(cc, n) : clamp1_a(cc, n)); };
     double x, y;
+    int ax, ay;
     if (AXIS == 0) {
         x = (c - 0.5) * L.h;
         y = L.dim == 2 ? i * L.h : 0.0;
+        ax = 2 * c - 1;
+        ay = L.dim == 2 ? 2 * i : 0;
     } else {
         x = j * L.h;
         y = (c - 0.5) * L.h;
+        ax = 2 * j;
+        ay = 2 * c - 1;
     }
-    const Hole& b = L.hole;
     if (ORDER == 1) {
         double WL[5], WR[5], gL[5], gR[5];
         ld5(L.in, at(c - 1), WL);
         ld5(L.in, at(c), WR);
         zero5(gL);
         zero5(gR);
-        adjgr::vjp_force_g<AXIS>(WL, WR, adjgr::force_geo<AXIS>(b, x, y, L.h), L.coef, L.P, gF, gL, gR);
+        adjgr::vjp_force_g<AXIS>(WL, WR, force_geo_l<AXIS>(L, x, y, ax, ay), L.coef, L.P, gF, gL, gR);
         add5(A.lam_in, at(c - 1), gL);
         add5(A.lam_in, at(c), gR);
         return;
@@ -165,7 +202,8 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     ld5(L.in, at(c), W2);
     ld5(L.in, at(c + 1), W3);
     // each metric point evaluated once for the forward re-run and the adjoint
-    const adjgr::HalfGeo HL = adjgr::half_geo<AXIS>(b, xl, yl, L.h), HR = adjgr::half_geo<AXIS>(b, xr, yr, L.h);
+    const adjgr::HalfGeo HL = half_geo_l<AXIS>(L, xl, yl, AXIS == 0 ? ax - 1 : ax, AXIS == 0 ? ay : ay - 1),
+                         HR = half_geo_l<AXIS>(L, xr, yr, AXIS == 0 ? ax + 1 : ax, AXIS == 0 ? ay : ay + 1);
     if (L.half) {
         // the face states the forward sweep kept
         const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
@@ -181,7 +219,7 @@ __global__ __launch_bounds__(kGAT) void k_gra_ifaces(GRAdjLaunch A) {
     double gpr[5], gpl[5];
     zero5(gpr);
     zero5(gpl);
-    adjgr::vjp_force_g<AXIS>(pr, pl, adjgr::force_geo<AXIS>(b, x, y, L.h), L.coef, L.P, gF, gpr, gpl);
+    adjgr::vjp_force_g<AXIS>(pr, pl, force_geo_l<AXIS>(L, x, y, ax, ay), L.coef, L.P, gF, gpr, gpl);
     double g0[5], g1[5], g2[5], g3[5];
     zero5(g0);
     zero5(g1);
@@ -222,25 +260,27 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
     const double nby[4] = {y, y, DIM == 2 ? (i - 1) * h : y, DIM == 2 ? (i + 1) * h : y};
     // the metric at the centre and one cell either side, shared by every
     // fractional sweep of the four stages (the positions the forward forms)
-    const Geo gc = geo_at(b, x, y, 0.0);
-    const Geo gxm = geo_at(b, x - h, y, 0.0), gxp = geo_at(b, x + h, y, 0.0);
+    const int ax = 2 * j, ay = DIM == 2 ? 2 * i : 0;
+    const Geo gc = geo_l(L, x, y, ax, ay);
+    const Geo gxm = geo_l(L, x - h, y, ax - 2, ay), gxp = geo_l(L, x + h, y, ax + 2, ay);
     Geo gym, gyp;
     if (DIM == 2) {
-        gym = geo_at(b, x, y - h, 0.0);
-        gyp = geo_at(b, x, y + h, 0.0);
+        gym = geo_l(L, x, y - h, ax, ay - 2);
+        gyp = geo_l(L, x, y + h, ax, ay + 2);
     }
     const int nn = DIM == 2 ? 4 : 2;
     double Wnb[4][5], unb[4][5], Wn[4][5];
     for (int q = 0; q < nn; ++q) {
         ld5(L.in, nb[q], Wnb[q]);
-        gr_p2c(Wnb[q], geo_at(b, nbx[q], nby[q], 0.0), unb[q], P);
+        gr_p2c(Wnb[q], geo_l(L, nbx[q], nby[q], q == 0 ? ax - 2 : q == 1 ? ax + 2 : ax,
+                             q == 2 ? ay - 2 : q == 3 ? ay + 2 : ay), unb[q], P);
         gr_c2p(unb[q], (q == 0 ? gxm : q == 1 ? gxp : q == 2 ? gym : gyp).sdet, P, Wn[q]);
     }
     double W[5], uM[5];
     ld5(L.in, cell, W);
     gr_p2c(W, gc, uM, P);
     SourceGeo sg;
-    source_geo(b, x, y, 0.0, h, sg);
+    source_l(L, i, j, x, y, sg);
     double kk[4][5], tmp[3][5], Wm[3][5], e2[3][5], e3[3][5], e1[3][5], um[3][5], src[5];
     gr_source(uM, sg, P, src);
 #pragma unroll
@@ -325,7 +365,8 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing(GRAdjLaunch A) {
         zero5(gu);
         zero5(gw);
         adjgr::vjp_c2p(unb[q], (q == 0 ? gxm : q == 1 ? gxp : q == 2 ? gym : gyp).sdet, tol, P, gWn[q], gu);
-        adjgr::vjp_p2c(Wnb[q], geo_at(b, nbx[q], nby[q], 0.0), P, gu, gw);
+        adjgr::vjp_p2c(Wnb[q], geo_l(L, nbx[q], nby[q], q == 0 ? ax - 2 : q == 1 ? ax + 2 : ax,
+                                     q == 2 ? ay - 2 : q == 3 ? ay + 2 : ay), P, gu, gw);
         add5(A.lam_in, nb[q], gw);
     }
 }
@@ -358,12 +399,13 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing_tape(GRAdjLaunch A) {
     }
     const PhysParams& P = L.P;
     const double x = j * h, y = DIM == 2 ? i * h : 0.0;
-    const Geo gc = geo_at(b, x, y, 0.0);
-    const Geo gxm = geo_at(b, x - h, y, 0.0), gxp = geo_at(b, x + h, y, 0.0);
+    const int ax = 2 * j, ay = DIM == 2 ? 2 * i : 0;
+    const Geo gc = geo_l(L, x, y, ax, ay);
+    const Geo gxm = geo_l(L, x - h, y, ax - 2, ay), gxp = geo_l(L, x + h, y, ax + 2, ay);
     Geo gym, gyp;
     if (DIM == 2) {
-        gym = geo_at(b, x, y - h, 0.0);
-        gyp = geo_at(b, x, y + h, 0.0);
+        gym = geo_l(L, x, y - h, ax, ay - 2);
+        gyp = geo_l(L, x, y + h, ax, ay + 2);
     }
     // neighbours: p2c at their own centres, c2p at x -/+ h (y -/+ h)
     double Wn[4][5];
@@ -377,7 +419,8 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing_tape(GRAdjLaunch A) {
         for (int q = 0; q < nn; ++q) {
             double Wq[5], uq[5];
             ld5(L.in, nb[q], Wq);
-            gr_p2c(Wq, geo_at(b, nbx[q], nby[q], 0.0), uq, P);
+            gr_p2c(Wq, geo_l(L, nbx[q], nby[q], q == 0 ? ax - 2 : q == 1 ? ax + 2 : ax,
+                             q == 2 ? ay - 2 : q == 3 ? ay + 2 : ay), uq, P);
             gr_c2p(uq, (q == 0 ? gxm : q == 1 ? gxp : q == 2 ? gym : gyp).sdet, P, Wn[q]);
             put(kTapeWn + 5 * q, Wn[q]);
         }
@@ -385,7 +428,7 @@ __global__ __launch_bounds__(kGAT) void k_gra_forcing_tape(GRAdjLaunch A) {
     double uM[5];
     gr_p2c(W, gc, uM, P);
     SourceGeo sg;
-    source_geo(b, x, y, 0.0, h, sg);
+    source_l(L, i, j, x, y, sg);
     double kk[4][5], src[5];
     gr_source(uM, sg, P, src);
 #pragma unroll
@@ -479,7 +522,7 @@ __global__ __launch_bounds__(kGAT) void k_grr_head(GRAdjLaunch A) {
         add5(A.lam_in, c.cell, lam);
         return;
     }
-    const Geo gc = geo_at(L.hole, c.x, c.y, 0.0);
+    const Geo gc = geo_l(L, c.x, c.y, 2 * c.j, DIM == 2 ? 2 * c.i : 0);
     double un[5], gun[5], gk[5], z[5];
     tget(A, kTapeUn, c.cell, un);
     zero5(gun);
@@ -498,9 +541,9 @@ __global__ __launch_bounds__(kGAT) void k_grr_src(GRAdjLaunch A, int st) {
     const GRLaunch& L = A.L;
     RevCell c;
     if (!rev_cell<DIM>(A, c)) return;
-    const Geo gc = geo_at(L.hole, c.x, c.y, 0.0);
+    const Geo gc = geo_l(L, c.x, c.y, 2 * c.j, DIM == 2 ? 2 * c.i : 0);
     SourceGeo sg;
-    source_geo(L.hole, c.x, c.y, 0.0, L.h, sg);
+    source_l(L, c.i, c.j, c.x, c.y, sg);
     double gk[5], e1[5], um[5], gsrc[5], gum[5], ge1[5];
     cget(A, kCarryGk, c.cell, gk);
     tget(A, kTapeStage + kTapePerStage * st + kTapeE1, c.cell, e1);
@@ -524,8 +567,9 @@ __global__ __launch_bounds__(kGAT) void k_grr_frac(GRAdjLaunch A, int st) {
     if (!rev_cell<DIM>(A, c)) return;
     const int base = kTapeStage + kTapePerStage * st;
     const int mid = LAST ? kTapeWm : (AXIS == 0 ? kTapeE3 : kTapeE2);
-    const Geo gc = geo_at(L.hole, c.x, c.y, 0.0);
-    const Geo ga = adjgr::geo_shift<AXIS>(L.hole, c.x, c.y, -L.h), gb = adjgr::geo_shift<AXIS>(L.hole, c.x, c.y, L.h);
+    const int ax = 2 * c.j, ay = DIM == 2 ? 2 * c.i : 0;
+    const Geo gc = geo_l(L, c.x, c.y, ax, ay);
+    const Geo ga = geo_shift_l<AXIS>(L, c.x, c.y, -L.h, ax, ay, -2), gb = geo_shift_l<AXIS>(L, c.x, c.y, L.h, ax, ay, 2);
     double ge[5], Wa[5], Wm[5], Wb[5], gWa[5], gWb[5], gm[5];
     cget(A, kCarryGe, c.cell, ge);
     tget(A, base + mid, c.cell, Wm);
@@ -568,9 +612,9 @@ __global__ __launch_bounds__(kGAT) void k_grr_tail(GRAdjLaunch A) {
     const Hole& b = L.hole;
     const double h = L.h;
     {
-        const Geo gc = geo_at(b, c.x, c.y, 0.0);
+        const Geo gc = geo_l(L, c.x, c.y, 2 * c.j, DIM == 2 ? 2 * c.i : 0);
         SourceGeo sg;
-        source_geo(b, c.x, c.y, 0.0, h, sg);
+        source_l(L, c.i, c.j, c.x, c.y, sg);
         double W[5], uM[5], gk[5], gsrc[5], guM[5], gW[5];
         ld5(L.in, c.cell, W);
         gr_p2c(W, gc, uM, L.P);
@@ -597,11 +641,12 @@ __global__ __launch_bounds__(kGAT) void k_grr_tail(GRAdjLaunch A) {
         double Wq[5], uq[5], gq[5], gu[5], gw[5];
         cget(A, kCarryGWn + 5 * q, c.cell, gq);
         ld5(L.in, nbq, Wq);
-        const Geo gp = geo_at(b, px, py, 0.0);
+        const int qax = 2 * j + (q == 0 ? -2 : q == 1 ? 2 : 0), qay = DIM == 2 ? 2 * i + (q == 2 ? -2 : q == 3 ? 2 : 0) : 0;
+        const Geo gp = geo_l(L, px, py, qax, qay);
         gr_p2c(Wq, gp, uq, L.P);
         zero5(gu);
         zero5(gw);
-        adjgr::vjp_c2p(uq, geo_at(b, sx, sy, 0.0).sdet, tol, L.P, gq, gu);
+        adjgr::vjp_c2p(uq, geo_l(L, sx, sy, qax, qay).sdet, tol, L.P, gq, gu);
         adjgr::vjp_p2c(Wq, gp, L.P, gu, gw);
         add5(A.lam_in, nbq, gw);
     }
