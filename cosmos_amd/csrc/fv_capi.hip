@@ -24,6 +24,12 @@ using cfv::Planes;
 using cfv::StepState;
 using cfv::SweepLaunch;
 
+// The device status block (the cosmos_status_t layout): [0] con2prim solves
+// at the iteration cap, [1] q floors, [2] density floors, [3] pressure floors,
+// [4] NaN cells of the state (formed when read), [5] a solve loop took a NaN
+// time step, [6] every cell excised from a black-hole wave speed, [7] unused.
+constexpr int kStatusWords = 8;
+
 struct cosmos_solver {
     int physics = 0;
     int dim = 2;
@@ -37,6 +43,10 @@ struct cosmos_solver {
     double* st_part = nullptr;                 // device, per-block maxima of the solve loop's last sweep
     unsigned long long* ws_scratch = nullptr;  // device, slots for cosmos_max_wave_speed
     int* flag = nullptr;                       // device, scratch flag
+    unsigned long long* status = nullptr;      // device, the status block (cosmos_status_t layout, 8 words)
+    int excised_known = 0;                     // black hole: excised_all computed for excised_key
+    bool excised_all = false;
+    double excised_key[6] = {0, 0, 0, 0, 0, 0};
     double* stage = nullptr;                   // device, AoS staging for transfers (lazily allocated)
     bool w_zero = false;                       // z-velocity plane identically zero (both buffers)
     int mode = COSMOS_MODE_EXACT;
@@ -156,6 +166,7 @@ SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, in
     L.y_rows = s->y_rows;
     L.y_tile = s->y_tile;
     L.fuse_rows = s->fuse_rows;
+    L.status = s->status;
     return L;
 }
 
@@ -454,6 +465,7 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     if (e == hipSuccess) e = hipMalloc((void**)&s->st_slots, sizeof(unsigned long long) * cfv::kSlotWords);
     if (e == hipSuccess) e = hipMalloc((void**)&s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords);
     if (e == hipSuccess) e = hipMalloc((void**)&s->flag, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc((void**)&s->status, sizeof(unsigned long long) * kStatusWords);
     if (e == hipSuccess)
         e = hipMalloc((void**)&s->st_part, sizeof(double) * (size_t)cfv::sweep_x_blocks(s->nx, s->ny, 2));
     // Zero on the handle's own stream and wait: the stream is non-blocking, so
@@ -462,6 +474,7 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     if (e == hipSuccess) e = hipMemsetAsync(s->buf[0], 0, bytes, s->stream);
     if (e == hipSuccess) e = hipMemsetAsync(s->buf[1], 0, bytes, s->stream);
     if (e == hipSuccess) e = hipMemsetAsync(s->st, 0, sizeof(StepState), s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(s->status, 0, sizeof(unsigned long long) * kStatusWords, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         std::string msg = std::string("device setup: ") + hipGetErrorString(e);
@@ -487,6 +500,7 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->st_part);
     hipFree(s->ws_scratch);
     hipFree(s->flag);
+    hipFree(s->status);
     hipFree(s->stage);
     hipFree(s->gr_flux);
     hipFree(s->geo_tab);
@@ -671,6 +685,23 @@ static int max_ws_impl(cosmos_solver* s, unsigned long long* slots_dev) {
     int rc = settle(s);
     if (rc) return rc;
     hipError_t e = cfv::launch_zero_slots(slots_dev, s->stream);
+    if (e == hipSuccess && s->physics == COSMOS_BLACKHOLE) {
+        // computeMaximumWaveSpeed{,2D} skips excised cells at the unit-square
+        // positions (j / nx, i / ny); whether that is every cell depends on the
+        // spacetime and the grid only, so it is found once per spacetime
+        const double key[6] = {s->hole.M, s->hole.a, s->hole.x, s->hole.y, s->hole.z, s->hole.r_exc};
+        if (!s->excised_known || std::memcmp(key, s->excised_key, sizeof key) != 0) {
+            const double dr = 1.0 / (double)s->ny, dc = 1.0 / (double)s->nx;
+            bool all = true;
+            for (int i = 0; i < s->ny && all; ++i)
+                for (int j = 0; j < s->nx && all; ++j)
+                    all = cfv::excised(s->hole, j * dc, s->dim == 2 ? i * dr : 0.0, 0.0);
+            s->excised_all = all;
+            s->excised_known = 1;
+            std::memcpy(s->excised_key, key, sizeof key);
+        }
+        if (s->excised_all) e = cfv::launch_fill_u64(s->status + 6, 1, 1ull, s->stream);
+    }
     if (e == hipSuccess) {
         if (s->physics == COSMOS_BLACKHOLE)
             e = cfv::launch_gr_wavespeed(gr_launch(s, 0.0), slots_dev, s->stream);
@@ -742,6 +773,7 @@ int cosmos_begin_at(cosmos_solver* s, int order, double cell_spacing, double cfl
         h.ws_part = s->st_part;
         h.ws_nparts = 0;
         h.ws_slot_atomics = s->ws_mode != 1;
+        h.status = s->status;
         h.ws_nparts_sweep = h.ws_slot_atomics ? 0 : cfv::sweep_x_blocks(s->nx, s->ny, order);
         HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
         rc = max_ws_impl(s, s->st_slots);
@@ -836,6 +868,7 @@ int cosmos_solve_forced_at(cosmos_solver* s, int order, double cell_spacing, dou
             double dt = 0.0;
             rc = cosmos_stable_time_step(s, cell_spacing, cfl, t, final_time, it, &dt);
             if (rc) return rc;
+            if (dt != dt) HIPCHK(s, cfv::launch_fill_u64(s->status + 5, 1, 1ull, s->stream));
             // forcing(dt / 2N) x N, flux step, forcing(dt / 2N) x N, with the forcing's
             // bias 0 and van Leer fixed as in the reference
             for (int k = 0; k < subcycles; ++k)
@@ -855,6 +888,35 @@ int cosmos_solve_forced_at(cosmos_solver* s, int order, double cell_spacing, dou
         HIPCHK(s, hipStreamSynchronize(s->stream));
         if (time_out) *time_out = t;
         if (steps_out) *steps_out = it - start_iteration;
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_status_async(cosmos_solver* s, cosmos_status_t* out) {
+    return guarded(s, [&] {
+        if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "out is null");
+        static_assert(sizeof(cosmos_status_t) == sizeof(unsigned long long) * kStatusWords, "status layout");
+        int rc = settle(s);
+        if (rc) return rc;
+        HIPCHK(s, cfv::launch_fill_u64(s->status + 4, 1, 0ull, s->stream));
+        HIPCHK(s, cfv::launch_nan_cells(cplanes_of(s, s->cur), s->nx, s->ny, s->pitch, s->status + 4, s->stream));
+        HIPCHK(s, hipMemcpyAsync(out, s->status, sizeof(cosmos_status_t), hipMemcpyDeviceToHost, s->stream));
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_status(cosmos_solver* s, cosmos_status_t* out) {
+    return guarded(s, [&] {
+        int rc = cosmos_status_async(s, out);
+        if (rc) return rc;
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return COSMOS_OK;
+    });
+}
+
+int cosmos_status_reset(cosmos_solver* s) {
+    return guarded(s, [&] {
+        HIPCHK(s, cfv::launch_fill_u64(s->status, kStatusWords, 0ull, s->stream));
         return COSMOS_OK;
     });
 }

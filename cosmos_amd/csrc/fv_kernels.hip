@@ -89,6 +89,16 @@ __device__ __forceinline__ void wave_max_to_slot(double m, unsigned long long* s
         atomicMax(slots + (wave_id % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(m));
 }
 
+// A thread's con2prim event counts into the solver's status counters (events
+// are rare: most threads add nothing).
+__device__ __forceinline__ void status_add(unsigned long long* st, const Ev& e) {
+    if (!st) return;
+    if (e.cap) atomicAdd(st + 0, (unsigned long long)e.cap);
+    if (e.q) atomicAdd(st + 1, (unsigned long long)e.q);
+    if (e.rho) atomicAdd(st + 2, (unsigned long long)e.rho);
+    if (e.p) atomicAdd(st + 3, (unsigned long long)e.p);
+}
+
 __device__ __forceinline__ bool resolve_coef(const SweepLaunch& L, SweepCoef& cf) {
     cf = L.coef;
     if (L.step) {
@@ -117,8 +127,11 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const int blk = xcd_tile(blockIdx.x, gridDim.x);
     const int row = blk / tiles;
     const int tile = blk - row * tiles;
+    Ev acc{};
+    Ev* const pacc = (Phys::kPhysics == 1 && L.status) ? &acc : nullptr;
     const double m = x_segment<Phys, ORDER, LIMMODE, DIM, WS, W0, NT>(L.in, L.out, nx, (long)row * L.pitch, tile, cf,
-                                                                     L.bias, L.limiter, P, lds);
+                                                                     L.bias, L.limiter, P, lds, pacc);
+    if (Phys::kPhysics == 1) status_add(L.status, acc);
     if constexpr (WS) {
         if (L.step->ws_slot_atomics) {
             wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
@@ -157,8 +170,11 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min
     const int jj = colok ? j : nx - 1;
     const int i0 = by * ROWS;
     const int i1 = min(i0 + ROWS, ny);
+    Ev acc{};
+    Ev* const pacc = (Phys::kPhysics == 1 && L.status) ? &acc : nullptr;
     const double wsmax = y_march<Phys, ORDER, LIMMODE, W0>(L.in, L.out, ny, L.pitch, j, colok, jj, i0, i1, WS, cf,
-                                                          L.bias, L.limiter, P);
+                                                          L.bias, L.limiter, P, pacc);
+    if (Phys::kPhysics == 1) status_add(L.status, acc);
     if constexpr (WS) {
         if (L.step->ws_slot_atomics) wave_max_to_slot(wsmax, L.step->ws_slots, blockIdx.x);
         else block_max_store<kYThreads>(wsmax, L.step->ws_part, nullptr);
@@ -183,8 +199,12 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
     const int blk = xcd_tile(blockIdx.x, gridDim.x);
     const int br = blk / tc;
     const int bc = blk - br * tc;
+    Ev acc{};
+    Ev* const pacc = (Phys::kPhysics == 1 && L.status) ? &acc : nullptr;
     const double m = y_tile_cells<Phys, ORDER, LIMMODE, W0, NT, TC>(L.in, L.out, L.nx, L.ny, L.pitch, bc * TC, br * TRV,
-                                                                     WS ? DIM : 0, cf, L.bias, L.limiter, L.P, lds);
+                                                                     WS ? DIM : 0, cf, L.bias, L.limiter, L.P, lds,
+                                                                     pacc);
+    if (Phys::kPhysics == 1) status_add(L.status, acc);
     if constexpr (WS) {
         if (L.step->ws_slot_atomics) wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
         else block_max_store<NT>(m, L.step->ws_part, &lds[ORDER == 2 ? 5 : 0][0]);
@@ -387,6 +407,8 @@ __global__ __launch_bounds__(kHeadThreads) void k_step_begin(StepState* st) {
     if (st->iter <= 5) dt *= 0.2;
     if ((st->t + dt) > st->tf) dt = st->tf - st->t;
     st->dt = dt;
+    // a NaN step ends the loop after it (t becomes NaN), as in the reference
+    if (dt != dt && st->status) st->status[5] = 1ull;
     double dth = 0.5 * dt;
     st->half.dt_dx = dth / st->dx;
     st->half.dx_dt = st->dx / dth;
@@ -397,6 +419,30 @@ __global__ __launch_bounds__(kHeadThreads) void k_step_begin(StepState* st) {
     st->t = st->t + dt;
     st->iter = st->iter + 1;
     st->active_steps = st->active_steps + 1;
+}
+
+// Cells of an nx-by-ny state holding a NaN in any of the five planes, added to *count.
+__global__ __launch_bounds__(256) void k_nan_cells(ConstPlanes in, int nx, int ny, long pitch,
+                                                   unsigned long long* count) {
+    long total = (long)nx * ny;
+    unsigned n = 0;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+        long i = idx / nx, j = idx - i * nx;
+        bool bad = false;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            double v = in.p[k][i * pitch + j];
+            bad = bad || (v != v);
+        }
+        n += bad ? 1u : 0u;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if ((threadIdx.x & 63) == 0 && n) atomicAdd(count, (unsigned long long)n);
+}
+
+__global__ __launch_bounds__(64) void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+    for (int k = threadIdx.x; k < n; k += blockDim.x) p[k] = v;
 }
 
 __global__ __launch_bounds__(256) void k_zero_slots(unsigned long long* slots) {
@@ -578,6 +624,20 @@ hipError_t launch_step_begin(StepState* st, hipStream_t s) {
 int sweep_x_blocks(int nx, int ny, int order) {
     const int TW = kXThreads - 2 * (order == 2 ? 2 : 1);
     return (nx + TW - 1) / TW * ny;
+}
+
+hipError_t launch_nan_cells(const ConstPlanes& in, int nx, int ny, long pitch, unsigned long long* count,
+                            hipStream_t s) {
+    long total = (long)nx * ny;
+    long blocks = (total + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_nan_cells, dim3((unsigned)blocks), dim3(256), 0, s, in, nx, ny, pitch, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, p, n, v);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s) {

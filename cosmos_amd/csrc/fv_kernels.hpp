@@ -94,6 +94,7 @@ struct StepState {
     int max_steps;
     int done;
     int active_steps;            // steps actually taken since the loop began
+    unsigned long long* status;  // the solver's status block (null: none); [5] set on a NaN time step
     SweepCoef half;              // the outer x sweeps of a 2D step: 0.5 * dt
     SweepCoef full;              // the y sweep of a 2D step and the 1D sweep: dt
 };
@@ -126,6 +127,7 @@ struct SweepLaunch {
     int y_rows;                 // rows per y-sweep thread (16, 32, 64; 0: by physics)
     int fuse_rows;  // fused step: output rows per block (0: chosen from the occupancy)
     int y_tile;                 // y-sweep layout: 0 column march, 8 or 16 columns per LDS tile; -1: by physics
+    unsigned long long* status; // special relativity: con2prim event counters (cap, q, rho, p floors), or null
 };
 
 // All return hipSuccess or the launch error.
@@ -141,6 +143,11 @@ hipError_t launch_step_fused(const SweepLaunch& L, hipStream_t s);
 // Blocks of an x-sweep launch (the number of per-block maxima it writes).
 int sweep_x_blocks(int nx, int ny, int order);
 hipError_t launch_zero_slots(unsigned long long* slots, hipStream_t s);
+// p[0 .. n) = v (a fill kernel, no memset node).
+hipError_t launch_fill_u64(unsigned long long* p, int n, unsigned long long v, hipStream_t s);
+// *count += cells of the state holding a NaN (count zeroed by the caller).
+hipError_t launch_nan_cells(const ConstPlanes& in, int nx, int ny, long pitch, unsigned long long* count,
+                            hipStream_t s);
 // flag[0] = 1 if any value of the nx-by-ny plane (row pitch `pitch`) is nonzero, else 0.
 hipError_t launch_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag, hipStream_t s);
 

@@ -38,6 +38,25 @@ struct PhysParams {
 // Device functions that host-side checks also compile.
 #define CFV_HD __host__ __device__ __forceinline__
 
+// Con2prim events of one thread's special-relativistic solves (a diagnostic;
+// the reference accepts these states silently,
+// RelativisticEuler/RelativisticEulerStateVector.cpp:45-50, 59-80, 93-100):
+// solves that ran to the iteration cap, that took the q floor (root at
+// sqrt(1e-8)), and whose density or pressure was raised to 1e-8.  Physics
+// functions take an optional Ev* and count into it; null counts nothing.
+struct Ev {
+    unsigned cap, q, rho, p;
+};
+
+CFV_HD void ev_add(Ev& a, const Ev& b, bool on) {
+    if (on) {
+        a.cap += b.cap;
+        a.q += b.q;
+        a.rho += b.rho;
+        a.p += b.p;
+    }
+}
+
 // libstdc++'s std::min / std::max: min(a, b) = (b < a) ? b : a.
 __device__ __forceinline__ double min_std(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double max_std(double a, double b) { return (a < b) ? b : a; }
@@ -113,14 +132,16 @@ struct NewtonianT {
     }
 
     // c2p: primitives of U.
-    __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P) {
+    __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P,
+                                               Ev* = nullptr) {
         double E;
         recover(U, W, E, irho, P);
     }
 
     // F(U) (the static cons flux).
     template <int AXIS>
-    __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P) {
+    __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P,
+                                                Ev* = nullptr) {
         double R[5], E;
         recover(U, R, E, irho, P);
         flux_of<AXIS>(R, E, F);
@@ -169,7 +190,7 @@ struct RelativisticT {
     // the old iterate, floors 1e-8 on rho and p (after the velocities are formed).
     // R = (rho, vx, vy, vz, p) floored; lf = W; h = specific enthalpy.
     __device__ static __forceinline__ void con2prim(const double* U, double* R, double& lf_out, double& h_out,
-                                                    const PhysParams& P) {
+                                                    const PhysParams& P, Ev* ev = nullptr) {
         double D = U[0], Sx = U[1], Sy = U[2], Sz = U[3], tau = U[4];
         double ssq = (Sx * Sx) + (Sy * Sy);
         if (!W0) ssq = ssq + (Sz * Sz);
@@ -186,7 +207,8 @@ struct RelativisticT {
         double c2x2 = 2.0 * c2, c4x4 = 4.0 * c4, e3c4 = (3.0 * eta) * c4;
         (void)g;
         double x = 1.0;
-        for (int it = 0; it < P.c2p_maxit; ++it) {
+        int it = 0;
+        for (; it < P.c2p_maxit; ++it) {
             double x2 = x * x;
             double x3 = x2 * x;
             double f = ((((c4 * x3) * (x - eta)) + (c2 * x2)) + (c1 * x)) + c0;
@@ -208,19 +230,27 @@ struct RelativisticT {
         R[4] = (p < 1e-8) ? 1e-8 : p;
         lf_out = lf;
         h_out = h;
+        if (ev) {
+            ev->cap += it == P.c2p_maxit ? 1u : 0u;
+            ev->q += q < 1e-8 ? 1u : 0u;
+            ev->rho += rho < 1e-8 ? 1u : 0u;
+            ev->p += p < 1e-8 ? 1u : 0u;
+        }
     }
 
-    __device__ static __forceinline__ void c2p(const double* U, double* W, double, const PhysParams& P) {
+    __device__ static __forceinline__ void c2p(const double* U, double* W, double, const PhysParams& P,
+                                               Ev* ev = nullptr) {
         double lf, h;
-        con2prim(U, W, lf, h, P);
+        con2prim(U, W, lf, h, P, ev);
     }
 
     // compute{X,Y}FluxVector(cons) (RelativisticEuler/RelativisticEulerStateVector.cpp:210-219, 299-308),
     // using the floored density and pressure.
     template <int AXIS>
-    __device__ static __forceinline__ void flux(const double* U, double* F, double, const PhysParams& P) {
+    __device__ static __forceinline__ void flux(const double* U, double* F, double, const PhysParams& P,
+                                                Ev* ev = nullptr) {
         double R[5], lf, h;
-        con2prim(U, R, lf, h, P);
+        con2prim(U, R, lf, h, P, ev);
         double rho = R[0], p = R[4];
         double rhw = (rho * h) * (lf * lf);
         double vn = R[1 + AXIS];
@@ -352,7 +382,8 @@ struct NewtonianFastT {
         F[4] = un * (U[4] + p);
     }
 
-    __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P) {
+    __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P,
+                                               Ev* = nullptr) {
         W[0] = U[0];
         W[1] = U[1] * irho;
         W[2] = U[2] * irho;
@@ -361,7 +392,8 @@ struct NewtonianFastT {
     }
 
     template <int AXIS>
-    __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P) {
+    __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P,
+                                                Ev* = nullptr) {
         flux_cons<AXIS>(U, irho, pressure(U, irho, P), F);
     }
 

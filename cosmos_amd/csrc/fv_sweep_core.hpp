@@ -46,10 +46,13 @@ __device__ __forceinline__ void store_cell(const Planes& out, long idx, const do
 // (EulerSolvers::evolveStateByHalf{X,Y}TimeStep, Solvers/EulerSolvers.cpp:191-223),
 // then the p2c and flux of both evolved face states as the FORCE flux of
 // the neighbouring interfaces will need them (Solvers/EulerFirstOrderSolver.cpp:52-59).
+// Con2prim events are counted into eh (the face fluxes and face states), eg0
+// and eg1 (the fluxes of the two face states' conserved vectors) when given.
 template <class Phys, int AXIS, int LIMMODE, bool W0>
 __device__ __forceinline__ void half_step(const double* ul, const double* um, const double* ur, const SweepCoef& cf,
                                           double bias, int limiter, const PhysParams& P, double* Q0, double* G0,
-                                          double* Q1, double* G1) {
+                                          double* Q1, double* G1, Ev* eh = nullptr, Ev* eg0 = nullptr,
+                                          Ev* eg1 = nullptr) {
     double xl[5], xr[5], fl[5], fr[5], a[5], b[5], p0[5], p1[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -67,8 +70,8 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
             xr[k] = um[k] + (0.5 * s);
         }
     }
-    Phys::template flux<AXIS>(xl, fl, Phys::inv_density(xl), P);
-    Phys::template flux<AXIS>(xr, fr, Phys::inv_density(xr), P);
+    Phys::template flux<AXIS>(xl, fl, Phys::inv_density(xl), P, eh);
+    Phys::template flux<AXIS>(xr, fr, Phys::inv_density(xr), P, eh);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         double ev = (W0 && k == 3) ? 0.0 : cf.hdt_dx * (fl[k] - fr[k]);
@@ -88,12 +91,12 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
             Q1[k] = b[k];
         }
     } else {
-        Phys::c2p(a, p0, ia, P);
-        Phys::c2p(b, p1, ib, P);
+        Phys::c2p(a, p0, ia, P, eh);
+        Phys::c2p(b, p1, ib, P, eh);
         Phys::p2c(p0, Q0, P);
         Phys::p2c(p1, Q1, P);
-        Phys::template flux<AXIS>(Q0, G0, ia, P);
-        Phys::template flux<AXIS>(Q1, G1, ib, P);
+        Phys::template flux<AXIS>(Q0, G0, ia, P, eg0);
+        Phys::template flux<AXIS>(Q1, G1, ib, P, eg1);
     }
 }
 
@@ -102,7 +105,7 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
 // Solvers/EulerFirstOrderSolver.cpp:3-24).
 template <class Phys, int AXIS, bool W0>
 __device__ __forceinline__ void force_flux(const double* UL, const double* FL, const double* UR, const double* FR,
-                                           const SweepCoef& cf, const PhysParams& P, double* F) {
+                                           const SweepCoef& cf, const PhysParams& P, double* F, Ev* ei = nullptr) {
     double us[5], fs[5];
     if constexpr (Phys::kFast) {
         // U* = (UL + UR) / 2 + (dt / 2dx)(FL - FR);  F = (FL + FR) / 4 + (dx / 4dt)(UL - UR) + F(U*) / 2
@@ -124,7 +127,7 @@ __device__ __forceinline__ void force_flux(const double* UL, const double* FL, c
 #pragma unroll
     for (int k = 0; k < 5; ++k)
         us[k] = (W0 && k == 3) ? 0.0 : 0.5 * ((UL[k] + UR[k]) + (cf.dt_dx * (FL[k] - FR[k])));
-    Phys::template flux<AXIS>(us, fs, Phys::inv_density(us), P);
+    Phys::template flux<AXIS>(us, fs, Phys::inv_density(us), P, ei);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (W0 && k == 3) {
@@ -139,7 +142,7 @@ __device__ __forceinline__ void force_flux(const double* UL, const double* FL, c
 // computeFORCEUpdate (Solvers/EulerFirstOrderSolver.cpp:96-101) then c2p.
 template <class Phys, bool W0>
 __device__ __forceinline__ void update_cell(const double* U, const double* Fl, const double* Fr, const SweepCoef& cf,
-                                            const PhysParams& P, double* W) {
+                                            const PhysParams& P, double* W, Ev* eu = nullptr) {
     double un[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -147,7 +150,28 @@ __device__ __forceinline__ void update_cell(const double* U, const double* Fl, c
         else if constexpr (Phys::kFast) un[k] = __builtin_fma(cf.dt_dx, Fl[k] - Fr[k], U[k]);
         else un[k] = U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
     }
-    Phys::c2p(un, W, Phys::inv_density(un), P);
+    Phys::c2p(un, W, Phys::inv_density(un), P, eu);
+}
+
+// Which of a thread's con2prim solves it owns for the event counts: every
+// distinct solve of a line is counted once, in the sites the oracle
+// evaluates (cells -1 .. n of the line for the half steps and, FORCE, the cell
+// fluxes; interfaces (-1|0) .. (n-1|n); updates 0 .. n-1), though tiles
+// recompute halo sites of their neighbours.  c: the thread's position along
+// the line of n cells; valid: it is an output cell of this block; first,
+// last: this block holds the line's first / last cells.
+struct Own {
+    bool half, g0, g1, iface, upd;
+};
+
+__device__ __forceinline__ Own own_of(int c, int n, bool valid, bool first, bool last) {
+    Own o;
+    o.half = valid || (c == -1 && first) || (c == n && last);
+    o.g0 = o.half && c != -1;
+    o.g1 = o.half && c != n;
+    o.iface = valid || (c == -1 && first);
+    o.upd = valid;
+    return o;
 }
 
 // One sweep pass of a block of NT threads over cells laid out in LDS so that
@@ -166,7 +190,14 @@ __device__ __forceinline__ void update_cell(const double* U, const double* Fl, c
 // thread's cell has its full stencil in the block).
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>
 __device__ __forceinline__ void line_step(const double* W, double* Wn, const SweepCoef& cf, double bias, int limiter,
-                                          const PhysParams& P, double (*lds)[NT]) {
+                                          const PhysParams& P, double (*lds)[NT], const Own* own = nullptr,
+                                          Ev* acc = nullptr) {
+    Ev eh{}, eg0{}, eg1{}, ei{}, eu{};
+    Ev* const ph = acc ? &eh : nullptr;
+    Ev* const pg0 = acc ? &eg0 : nullptr;
+    Ev* const pg1 = acc ? &eg1 : nullptr;
+    Ev* const pi = acc ? &ei : nullptr;
+    Ev* const pu = acc ? &eu : nullptr;
     //   SLIC:  sA = rows 0-4   stage A U, then the interface fluxes
     //          sB = rows 5-14  stage B Q0, G0
     //   FORCE: sA = rows 0-9   U and F(U);  sB = rows 10-14 the interface fluxes
@@ -190,7 +221,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
             ul[k] = (W0 && k == 3) ? 0.0 : sA[k][sl];
             ur[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
         }
-        half_step<Phys, AXIS, LIMMODE, W0>(ul, U, ur, cf, bias, limiter, P, Q0, G0, Q1, G1);
+        half_step<Phys, AXIS, LIMMODE, W0>(ul, U, ur, cf, bias, limiter, P, Q0, G0, Q1, G1, ph, pg0, pg1);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             if (W0 && k == 3) continue;
@@ -204,7 +235,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
             UR[k] = (W0 && k == 3) ? 0.0 : sB[k][sr];
             FR[k] = (W0 && k == 3) ? 0.0 : sB[5 + k][sr];
         }
-        force_flux<Phys, AXIS, W0>(Q1, G1, UR, FR, cf, P, F);
+        force_flux<Phys, AXIS, W0>(Q1, G1, UR, FR, cf, P, F, pi);
         // interface (s, s+1) -> sA (stage A values are no longer read)
 #pragma unroll
         for (int k = 0; k < 5; ++k)
@@ -212,7 +243,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
         __syncthreads();
     } else {
         double FU[5];
-        Phys::template flux<AXIS>(U, FU, Phys::inv_density(U), P);
+        Phys::template flux<AXIS>(U, FU, Phys::inv_density(U), P, ph);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             if (W0 && k == 3) continue;
@@ -226,7 +257,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
             UR[k] = (W0 && k == 3) ? 0.0 : sA[k][sr];
             FR[k] = (W0 && k == 3) ? 0.0 : sA[5 + k][sr];
         }
-        force_flux<Phys, AXIS, W0>(U, FU, UR, FR, cf, P, F);
+        force_flux<Phys, AXIS, W0>(U, FU, UR, FR, cf, P, F, pi);
 #pragma unroll
         for (int k = 0; k < 5; ++k)
             if (!(W0 && k == 3)) sB[k][s] = F[k];
@@ -235,16 +266,25 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
     double Fl[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : (ORDER == 2 ? sA[k][sl] : sB[k][sl]);
-    update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
+    update_cell<Phys, W0>(U, Fl, F, cf, P, Wn, pu);
+    if (acc) {
+        // FORCE: the cell's own flux counts where SLIC counts its half step
+        ev_add(*acc, eh, own->half);
+        ev_add(*acc, eg0, own->g0);
+        ev_add(*acc, eg1, own->g1);
+        ev_add(*acc, ei, own->iface);
+        ev_add(*acc, eu, own->upd);
+    }
 }
 
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>
 __device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
                                           bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
-                                          const PhysParams& P, double (*lds)[NT]) {
+                                          const PhysParams& P, double (*lds)[NT], const Own* own = nullptr,
+                                          Ev* acc = nullptr) {
     double W[5], Wn[5];
     load_cell<W0>(in, ld_idx, W);
-    line_step<Phys, ORDER, LIMMODE, W0, NT, AXIS, STRIDE>(W, Wn, cf, bias, limiter, P, lds);
+    line_step<Phys, ORDER, LIMMODE, W0, NT, AXIS, STRIDE>(W, Wn, cf, bias, limiter, P, lds, own, acc);
     if (valid) store_cell<W0>(out, st_idx, Wn);
     double m = 0.0;
     if (ws_dim != 0 && valid) {
@@ -257,9 +297,10 @@ __device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT>
 __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
                                           bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
-                                          const PhysParams& P, double (*lds)[NT]) {
+                                          const PhysParams& P, double (*lds)[NT], const Own* own = nullptr,
+                                          Ev* acc = nullptr) {
     return line_cells<Phys, ORDER, LIMMODE, W0, NT, 0, 1>(in, out, ld_idx, st_idx, valid, ws_dim, cf, bias, limiter, P,
-                                                           lds);
+                                                           lds, own, acc);
 }
 
 // One y-sweep tile by a block of NT threads: TC columns by NT / TC rows,
@@ -269,7 +310,7 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int TC>
 __device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Planes& out, int nx, int ny, long pitch,
                                                int c0, int r0, int ws_dim, const SweepCoef& cf, double bias,
-                                               int limiter, const PhysParams& P, double (*lds)[NT]) {
+                                               int limiter, const PhysParams& P, double (*lds)[NT], Ev* acc = nullptr) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TR = NT / TC;
     const int s = threadIdx.x;
@@ -279,8 +320,10 @@ __device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Plan
     const int jj = j < nx ? j : nx - 1;
     const int ii = ORDER == 2 ? mirror2(i, ny) : clamp1(i, ny);
     const bool valid = (r >= G) && (r < TR - G) && (i < ny) && (j < nx);
+    Own own = own_of(i, ny, valid, r0 == 0, r0 + (TR - 2 * G) >= ny);
+    if (j >= nx) own = Own{false, false, false, false, false};
     return line_cells<Phys, ORDER, LIMMODE, W0, NT, 1, TC>(in, out, (long)ii * pitch + jj, (long)i * pitch + j, valid,
-                                                            ws_dim, cf, bias, limiter, P, lds);
+                                                            ws_dim, cf, bias, limiter, P, lds, &own, acc);
 }
 
 // One row segment of an x sweep: thread s holds cell tile * (NT - 2G) - G + s
@@ -288,26 +331,38 @@ __device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Plan
 template <class Phys, int ORDER, int LIMMODE, int DIM, bool WS, bool W0, int NT>
 __device__ __forceinline__ double x_segment(const ConstPlanes& in, const Planes& out, int nx, long rowbase, int tile,
                                             const SweepCoef& cf, double bias, int limiter, const PhysParams& P,
-                                            double (*lds)[NT]) {
+                                            double (*lds)[NT], Ev* acc = nullptr) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TW = NT - 2 * G;
     const int s = threadIdx.x;
     const int c = tile * TW - G + s;
     const int cc = ORDER == 2 ? mirror2(c, nx) : clamp1(c, nx);
     const bool valid = (s >= G) && (s < NT - G) && (c < nx);
+    const Own own = own_of(c, nx, valid, tile == 0, (tile + 1) * TW >= nx);
     return x_cells<Phys, ORDER, LIMMODE, W0, NT>(in, out, rowbase + cc, rowbase + c, valid, WS ? DIM : 0, cf, bias,
-                                                 limiter, P, lds);
+                                                 limiter, P, lds, &own, acc);
 }
 
 // One y-sweep column march: thread's column j (jj = j clamped into the grid,
 // colok = j < nx) over output rows [i0, i1), reading rows i0 - G .. i1 + G - 1
 // through the boundary map and keeping a three-cell register window.  Returns
 // the largest wave speed of the written cells when ws.
+// acc: con2prim events of the solves this march owns (rows [i0, i1) and, at
+// the grid's edges, the ghost sites the oracle evaluates; see Own).
 template <class Phys, int ORDER, int LIMMODE, bool W0>
 __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& out, int ny, long pitch, int j,
                                           bool colok, int jj, int i0, int i1, bool ws, const SweepCoef& cf,
-                                          double bias, int limiter, const PhysParams& P) {
+                                          double bias, int limiter, const PhysParams& P, Ev* acc = nullptr) {
     double wsmax = 0.0;
+    Ev eh{}, eg0{}, eg1{}, ei{}, eu{};
+    Ev* const ph = acc ? &eh : nullptr;
+    Ev* const pg0 = acc ? &eg0 : nullptr;
+    Ev* const pg1 = acc ? &eg1 : nullptr;
+    Ev* const pi = acc ? &ei : nullptr;
+    Ev* const pu = acc ? &eu : nullptr;
+    const bool first = i0 == 0, last = i1 >= ny;
+    auto owned = [&](int q) { return colok && ((q >= i0 && q < i1) || (q == -1 && first) || (q == ny && last)); };
+    auto owned_iface = [&](int q) { return colok && ((q >= i0 && q < i1) || (q == -1 && first)); };
 
     if constexpr (ORDER == 2) {
         double Ua[5], Ub[5], Q1p[5], G1p[5], Fp[5], Wn[5];
@@ -321,13 +376,30 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
             Phys::p2c(W, U, P);
             if (r >= i0) {
                 double Q0[5], G0[5], Q1[5], G1[5];
-                half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cf, bias, limiter, P, Q0, G0, Q1, G1);
+                half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cf, bias, limiter, P, Q0, G0, Q1, G1, ph, pg0, pg1);
+                if (acc) {
+                    // the half step of row r - 1
+                    const bool o = owned(r - 1);
+                    ev_add(*acc, eh, o);
+                    ev_add(*acc, eg0, o && r - 1 != -1);
+                    ev_add(*acc, eg1, o && r - 1 != ny);
+                    eh = eg0 = eg1 = Ev{};
+                }
                 if (r >= i0 + 1) {
                     double F[5];
-                    force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cf, P, F);
+                    force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cf, P, F, pi);
+                    if (acc) {
+                        // interface (r - 2 | r - 1)
+                        ev_add(*acc, ei, owned_iface(r - 2));
+                        ei = Ev{};
+                    }
                     if (r >= i0 + 2) {
                         double Wo[5];
-                        update_cell<Phys, W0>(Ua, Fp, F, cf, P, Wo);
+                        update_cell<Phys, W0>(Ua, Fp, F, cf, P, Wo, pu);
+                        if (acc) {
+                            ev_add(*acc, eu, colok);
+                            eu = Ev{};
+                        }
                         if (colok) store_cell<W0>(out, (long)(r - 2) * pitch + j, Wo);
                         if (ws) {
                             double w = Phys::template wavespeed<2>(Wo, P);
@@ -359,13 +431,27 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
             for (int k = 0; k < 5; ++k) W[k] = Wn[k];
             if (r < i1) load_cell<W0>(in, (long)clamp1(r + 1, ny) * pitch + jj, Wn);
             Phys::p2c(W, U, P);
-            Phys::template flux<1>(U, FU, Phys::inv_density(U), P);
+            Phys::template flux<1>(U, FU, Phys::inv_density(U), P, ph);
+            if (acc) {
+                // the flux of row r's conserved vector
+                ev_add(*acc, eh, owned(r));
+                eh = Ev{};
+            }
             if (r >= i0) {
                 double F[5];
-                force_flux<Phys, 1, W0>(Up, FUp, U, FU, cf, P, F);
+                force_flux<Phys, 1, W0>(Up, FUp, U, FU, cf, P, F, pi);
+                if (acc) {
+                    // interface (r - 1 | r)
+                    ev_add(*acc, ei, owned_iface(r - 1));
+                    ei = Ev{};
+                }
                 if (r >= i0 + 1) {
                     double Wo[5];
-                    update_cell<Phys, W0>(Up, Fp, F, cf, P, Wo);
+                    update_cell<Phys, W0>(Up, Fp, F, cf, P, Wo, pu);
+                    if (acc) {
+                        ev_add(*acc, eu, colok);
+                        eu = Ev{};
+                    }
                     if (colok) store_cell<W0>(out, (long)(r - 1) * pitch + j, Wo);
                     if (ws) {
                         double w = Phys::template wavespeed<2>(Wo, P);

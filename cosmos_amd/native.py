@@ -25,6 +25,13 @@ class BatchProblem(ctypes.Structure):
                 ("max_steps", ctypes.c_int), ("adiabatic_index", ctypes.c_double)]
 
 
+class Status(ctypes.Structure):
+    """cosmos_status_t"""
+    _fields_ = [(n, ctypes.c_longlong) for n in ("c2p_cap", "c2p_q_floor", "c2p_density_floor",
+                                                 "c2p_pressure_floor", "nan_cells", "nan_time_step",
+                                                 "excised_grid", "reserved")]
+
+
 class BatchSchedule(ctypes.Structure):
     """cosmos_batch_schedule"""
     _fields_ = [("whole_max_cells", ctypes.c_int), ("x_segments_per_task", ctypes.c_int), ("y_rows", ctypes.c_int),
@@ -128,6 +135,9 @@ SIGNATURES = {
     "cosmos_adjoint_backward": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
     "cosmos_adjoint_replay": (ctypes.c_int, [_handle, _c_double_p, _c_double_p]),
     "cosmos_set_geometry_cache": (ctypes.c_int, [_handle, ctypes.c_int]),
+    "cosmos_status": (ctypes.c_int, [_handle, ctypes.c_void_p]),
+    "cosmos_status_async": (ctypes.c_int, [_handle, ctypes.c_void_p]),
+    "cosmos_status_reset": (ctypes.c_int, [_handle]),
     "cosmos_step_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_long,
                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,

@@ -156,6 +156,15 @@ class Solver:
                                            slope_limiter, ctypes.byref(t), ctypes.byref(n)))
         return t.value, n.value
 
+    def status(self) -> dict:
+        """cosmos_status: the numerical-fault counters and flags."""
+        st = native.Status()
+        self._check(self._lib.cosmos_status(self._h, ctypes.byref(st)))
+        return {name: int(getattr(st, name)) for name, _ in native.Status._fields_ if name != "reserved"}
+
+    def status_reset(self) -> None:
+        self._check(self._lib.cosmos_status_reset(self._h))
+
     def set_geometry_cache(self, enable: bool) -> None:
         """Black hole: tabulate the position-only metric values (default on)."""
         self._check(self._lib.cosmos_set_geometry_cache(self._h, 1 if enable else 0))

@@ -439,6 +439,40 @@ int cosmos_step_noalloc(int physics, int dim, int nx, int ny, long pitch, const 
 size_t cosmos_noalloc_grid_limit(void);
 const char* cosmos_noalloc_last_error(void);
 
+/* Numerical-fault status (no counterpart in the reference, which accepts
+ * these silently: RelativisticEuler/RelativisticEulerStateVector.cpp:45-50,
+ * 59-80, 93-100; Solvers/EulerSecondOrderSolver.cpp:109;
+ * Solvers/BlackHoleEulerSolvers.cpp:88-140).  Counters accumulate on the
+ * device from the solver's creation or last cosmos_status_reset:
+ *   c2p_cap .. c2p_pressure_floor: special-relativistic con2prim solves of
+ *     the per-solver sweeps (cosmos_sweep and the solve loops; not the batched
+ *     or AMR kernels) that ran to the iteration cap, took the q floor, or had
+ *     their density / pressure raised to 1e-8; each distinct solve of a sweep
+ *     counted once (every half step, interface flux and cell update of a line
+ *     once, though the reference evaluates some twice and the kernels
+ *     recompute tile halos);
+ *   nan_cells: cells of the current state holding a NaN, formed when read;
+ *   nan_time_step: 1 once a solve loop took a NaN time step (after which the
+ *     loop stops, as the reference's does);
+ *   excised_grid: 1 once a black-hole wave speed found every cell excised
+ *     (its maximum is then 0 and dt infinite, clamped to the final time). */
+typedef struct {
+    long long c2p_cap;
+    long long c2p_q_floor;
+    long long c2p_density_floor;
+    long long c2p_pressure_floor;
+    long long nan_cells;
+    long long nan_time_step;
+    long long excised_grid;
+    long long reserved;
+} cosmos_status_t;
+/* Queue the NaN count and a copy of the status into out (host memory that
+ * must stay valid until the stream reaches the copy: pinned memory makes the
+ * copy asynchronous); cosmos_status does the same and waits. */
+int cosmos_status_async(cosmos_solver* s, cosmos_status_t* out);
+int cosmos_status(cosmos_solver* s, cosmos_status_t* out);
+int cosmos_status_reset(cosmos_solver* s);
+
 int cosmos_synchronize(cosmos_solver* s);
 /* The handle's HIP stream (hipStream_t), for callers that time or order work. */
 void* cosmos_stream(cosmos_solver* s);

@@ -210,14 +210,19 @@ def test_config3_2048_past_the_cap_hitting_steps_bitwise(orc):
     interface states with no con2prim root appear and their Newton solves run
     to the reference's 1000-iteration cap; the GPU stays bitwise the oracle."""
     W = workloads.sr_blast_2d(2048, seed=0)
-    out, t, k = gpu_run(W, RELATIVISTIC, 2, 1.0 / 2048, 0.5, max_steps=20, gamma=G_SR)
+    with Solver(RELATIVISTIC, 2, 2048, 2048, G_SR) as s:
+        s.upload(W)
+        t, k = s.solve(2, 1.0 / 2048, 0.5, math.inf, 20)
+        out = s.download()
+        st = s.status()
     orc.events_reset()
     exp, te, ke = orc.run(W, 1, 2, 1.0 / 2048, 0.5, max_steps=20, gamma=G_SR)
-    caps = orc.events()[0]
-    print(f"SR blast 2048^2 x 20 steps: {caps} con2prim solves at the iteration cap")
-    assert caps > 0
+    ev = orc.events()
+    print(f"SR blast 2048^2 x 20 steps: oracle con2prim events (cap, q, rho, p floors) {ev}, device {st}")
+    assert ev[0] > 0
     assert k == ke == 20 and t == te
     assert same(out, exp), rel_linf(out, exp)
+    assert (st["c2p_cap"], st["c2p_q_floor"], st["c2p_density_floor"], st["c2p_pressure_floor"]) == ev
 
 
 def test_relativistic_config_mode_con2prim(orc):
