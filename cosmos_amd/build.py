@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("COSMOS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["fv_kernels.hip", "fv_gr.hip", "fv_capi.hip", "fv_amr.hip", "fv_adjoint.hip", "fv_gr_adjoint.hip",
-           "fv_batch.hip", "fv_pointwise.hip", "fv_noalloc.hip"]
+           "fv_batch.hip", "fv_pointwise.hip", "fv_noalloc.hip", "fv_point.hip"]
 
 FLAGS = [
     f"--offload-arch={ARCH}",
@@ -61,8 +61,9 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, *FLAGS, "-o", tmp, *srcs], verbose)
     os.replace(tmp, LIB)
     inc = ["-I" + os.path.join(ROOT, "include"), "-I" + DROPIN, "-I" + os.path.join(DROPIN, "compat")]
-    _run([CXX, "-std=c++14", "-O2", "-fPIC", "-shared", "-Wall", *inc, "-o", DROPIN_LIB,
-          os.path.join(DROPIN, "cosmos_dropin.cpp"), "-L" + HERE, "-lcosmos_fv", "-Wl,-rpath,$ORIGIN"], verbose)
+    _run([CXX, "-std=c++14", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", *inc, "-o", DROPIN_LIB,
+          os.path.join(DROPIN, "cosmos_dropin.cpp"), os.path.join(DROPIN, "cosmos_dropin_point.cpp"), "-L" + HERE,
+          "-lcosmos_fv", "-Wl,-rpath,$ORIGIN"], verbose)
     _run([CXX, "-std=c++14", "-O2", "-Wall", *inc, "-o", DROPIN_CHECK, DROPIN_CHECK + ".cpp", "-L" + HERE,
           "-lcosmos_dropin", "-lcosmos_fv", "-Wl,-rpath," + HERE], verbose)
     return LIB

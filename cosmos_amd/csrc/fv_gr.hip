@@ -28,7 +28,7 @@
 // exact lattice those positions are the neighbour's centre, so that chain is
 // a function of the neighbour alone, formed once per cell by k_gr_roundtrip
 // (tab.q) instead of four times per neighbour per step.
-#include "fv_kernels.hpp"
+#include "fv_gr_ops.hpp"
 
 namespace cfv {
 namespace {
@@ -70,101 +70,6 @@ __device__ __forceinline__ bool stencil_clear(const Hole& b, int dim, int i, int
         if (dim == 2 && (excised(b, j * h, (i - d) * h, 0.0) || excised(b, j * h, (i + d) * h, 0.0))) return false;
     }
     return true;
-}
-
-// The Geo at (x, y), half-lattice point (ax, ay): from the table (GEO) or
-// evaluated at the call site's position.
-template <int GEO>
-__device__ __forceinline__ Geo geo_pick(const GRLaunch& L, double x, double y, int ax, int ay) {
-    if constexpr (GEO == 1) return geo_from_table(L.tab, L.dim, ax, ay);
-    else return geo_at(L.hole, x, y, 0.0);
-}
-
-// position (x, y) moved by off (d half-lattice units) along the sweep axis
-template <int AXIS, int GEO>
-__device__ __forceinline__ Geo geo_off(const GRLaunch& L, double x, double y, double off, int ax, int ay, int d) {
-    return AXIS == 0 ? geo_pick<GEO>(L, x + off, y, ax + d, ay) : geo_pick<GEO>(L, x, y + off, ax, ay + d);
-}
-
-// Limited slope of the conserved vectors (SlopeLimiters::computeSlopeVector{X,Y},
-// Solvers/SlopeLimiters.cpp:123-147, on top of computeSlopeVector :71-101).
-__device__ __forceinline__ void slope5(const double* ul, const double* um, const double* ur, double bias, int limiter,
-                                       double* s) {
-    if (bias == 0.0 && limiter == 1) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) s[k] = slope_component<1>(ul[k], um[k], ur[k], bias, limiter);
-    } else {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) s[k] = slope_component<0>(ul[k], um[k], ur[k], bias, limiter);
-    }
-}
-
-// evolveStateByHalf{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:219-307) of the
-// middle cell centred at (x, y): slope from p2c at x - h, x, x + h; face fluxes
-// at x -/+ h/2; the selected face evolved by 0.5 (dt/h) (F_L - F_R) and
-// converted back at that face.
-template <int AXIS, int GEO>
-__device__ void gr_half_step(const double* WL, const double* WM, const double* WR, double x, double y, int ax,
-                             int ay, int side, const GRLaunch& L, double* out) {
-    const double h = L.h;
-    double ul[5], um[5], ur[5], s[5];
-    {
-        Geo g = geo_off<AXIS, GEO>(L, x, y, -h, ax, ay, -2);
-        gr_p2c(WL, g, ul, L.P);
-    }
-    {
-        Geo g = geo_pick<GEO>(L, x, y, ax, ay);
-        gr_p2c(WM, g, um, L.P);
-    }
-    {
-        Geo g = geo_off<AXIS, GEO>(L, x, y, h, ax, ay, 2);
-        gr_p2c(WR, g, ur, L.P);
-    }
-    slope5(ul, um, ur, L.bias, L.limiter, s);
-    double xl[5], xr[5], fl[5], fr[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        xl[k] = um[k] - (0.5 * s[k]);
-        xr[k] = um[k] + (0.5 * s[k]);
-    }
-    const double hh = h * 0.5;
-    Geo gl = geo_off<AXIS, GEO>(L, x, y, -hh, ax, ay, -1);
-    Geo gr = geo_off<AXIS, GEO>(L, x, y, hh, ax, ay, 1);
-    gr_flux<AXIS>(xl, gl, L.P, fl);
-    gr_flux<AXIS>(xr, gr, L.P, fr);
-    double e[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (L.coef.hdt_dx * (fl[k] - fr[k]));
-    gr_c2p(e, side == 0 ? gl.sdet : gr.sdet, L.P, out);
-}
-
-// compute{X,Y}FORCEFlux (Solvers/BlackHoleEulerFirstOrderSolver.cpp:3-99) at the
-// interface (x, y): the states' p2c and fluxes at the interface -/+ h/2, the
-// Richtmyer state's flux at the interface.
-template <int AXIS, int GEO>
-__device__ void gr_force(const double* WL, const double* WR, double x, double y, int ax, int ay, const GRLaunch& L,
-                         double* F) {
-    const double hh = L.h * 0.5;
-    double ul[5], ur[5], fl[5], fr[5], us[5], fs[5];
-    {
-        Geo gl = geo_off<AXIS, GEO>(L, x, y, -hh, ax, ay, -1);
-        Geo gr = geo_off<AXIS, GEO>(L, x, y, hh, ax, ay, 1);
-        gr_p2c(WL, gl, ul, L.P);
-        gr_p2c(WR, gr, ur, L.P);
-        gr_flux<AXIS>(ul, gl, L.P, fl);
-        gr_flux<AXIS>(ur, gr, L.P, fr);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (L.coef.dt_dx * (fl[k] - fr[k])));
-    {
-        Geo g = geo_pick<GEO>(L, x, y, ax, ay);
-        gr_flux<AXIS>(us, g, L.P, fs);
-    }
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        double lf = 0.5 * ((fl[k] + fr[k]) + (L.coef.dx_dt * (ul[k] - ur[k])));
-        F[k] = 0.5 * (lf + fs[k]);
-    }
 }
 
 // One flux per interface.  x sweep: thread -> (row i, interface c), c in [0, nx];
@@ -274,46 +179,6 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
     store5(L.out, cell, W);
 }
 
-// evolveStateByFractional{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:171-217):
-// slope from p2c at x - h, x, x + h; face fluxes at x -/+ h (a whole cell,
-// as in the reference); U_M + frac (dt/h) (F_L - F_R) converted back at x.
-// ul, ur: the neighbours' conserved vectors at x -/+ h (formed by the caller).
-template <int AXIS>
-__device__ __forceinline__ void gr_fractional_core(double fdt_h, const double* ul, const double* WM, const double* ur,
-                                                   const Geo& gl, const Geo& gc, const Geo& gr, const GRLaunch& L,
-                                                   double* out) {
-    double um[5], s[5];
-    gr_p2c(WM, gc, um, L.P);
-    slope5(ul, um, ur, L.bias, L.limiter, s);
-    double xl[5], xr[5], fl[5], fr[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        xl[k] = um[k] - (0.5 * s[k]);
-        xr[k] = um[k] + (0.5 * s[k]);
-    }
-    gr_flux<AXIS>(xl, gl, L.P, fl);
-    gr_flux<AXIS>(xr, gr, L.P, fr);
-    double e[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) e[k] = um[k] + (fdt_h * (fl[k] - fr[k]));
-    gr_c2p(e, gc.sdet, L.P, out);
-}
-
-template <int AXIS>
-__device__ void gr_fractional(double fdt_h, const double* WL, const double* WM, const double* WR, double x, double y,
-                              const GRLaunch& L, double* out) {
-    const Hole& b = L.hole;
-    const double h = L.h;
-    double ul[5], ur[5];
-    // the neighbours' p2c and the face fluxes share their positions x -/+ h
-    const Geo gl = AXIS == 0 ? geo_at(b, x - h, y, 0.0) : geo_at(b, x, y - h, 0.0);
-    const Geo gc = geo_at(b, x, y, 0.0);
-    const Geo gr = AXIS == 0 ? geo_at(b, x + h, y, 0.0) : geo_at(b, x, y + h, 0.0);
-    gr_p2c(WL, gl, ul, L.P);
-    gr_p2c(WR, gr, ur, L.P);
-    gr_fractional_core<AXIS>(fdt_h, ul, WM, ur, gl, gc, gr, L, out);
-}
-
 // The neighbour chain of a forcing cell on the exact lattice (GEO): for every
 // cell c of [-1, nx] x [-1, ny] (its data clamped into the grid, its geometry
 // at its own centre), p2c(c2p(p2c(W_c))) with the Geo of that centre: what the
@@ -359,10 +224,10 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
     }
     const double x = j * h, y = DIM == 2 ? i * h : 0.0;
     double uM[5];
-    double kk[4][5];
-    double src[5];
     SourceGeo sg;
     if constexpr (GEO == 1) {
+        double kk[4][5];
+        double src[5];
         // every position-only value and the neighbours' chain read once, at the
         // start, from the tables (tab.geo, tab.src, tab.q); the RK4 chain then
         // runs on registers only
@@ -421,7 +286,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
         return;
     }
     // neighbours' conserved vectors, each formed at its own centre
-    // (the grid loop of computeRungeKuttaTimeStep{,2D}) ...
+    // (the grid loop of computeRungeKuttaTimeStep{,2D})
     double uL[5], uR[5], uT[5], uB[5];
     {
         double Wn[5];
@@ -438,41 +303,10 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
     }
     const Geo gc = geo_at(b, x, y, 0.0);
     gr_p2c(W, gc, uM, L.P);
-    // ... and back to primitives at x -/+ h (y -/+ h) inside evolveConservedVariableVector.
-    double WLn[5], WRn[5], WTn[5], WBn[5];
-    gr_c2p(uL, geo_at(b, x - h, y, 0.0).sdet, L.P, WLn);
-    gr_c2p(uR, geo_at(b, x + h, y, 0.0).sdet, L.P, WRn);
-    if (DIM == 2) {
-        gr_c2p(uT, geo_at(b, x, y - h, 0.0).sdet, L.P, WTn);
-        gr_c2p(uB, geo_at(b, x, y + h, 0.0).sdet, L.P, WBn);
-    }
     if (L.tab.src) source_from_table(L.tab.src + (long)i * L.tab.spitch + j, L.tab.splane, sg);
     else source_geo(b, x, y, 0.0, h, sg);
-    gr_source_of(uM, SrcRef{sg}, L.P, src);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
-    for (int st = 0; st < 3; ++st) {
-        double tmp[5], Wm[5], e1[5], um[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
-        gr_c2p(tmp, gc.sdet, L.P, Wm);
-        if (DIM == 1) {
-            gr_fractional<0>(L.frac_dt_h[st], WLn, Wm, WRn, x, y, L, e1);
-        } else {
-            double e2[5], e3[5];
-            gr_fractional<0>(L.frac_dt_h[st], WLn, Wm, WRn, x, y, L, e2);
-            gr_fractional<1>(L.frac_dt_h_y[st], WTn, e2, WBn, x, y, L, e3);
-            gr_fractional<0>(L.frac_dt_h[st], WLn, e3, WRn, x, y, L, e1);
-        }
-        gr_p2c(e1, gc, um, L.P);
-        gr_source_of(um, SrcRef{sg}, L.P, src);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
-    }
     double un[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
+    gr_rk4_cons<DIM, 0>(uL, uM, uR, uT, uB, x, y, gc, sg, L, un);
     gr_c2p(un, gc.sdet, L.P, W);
     store5(L.out, cell, W);
 }

@@ -178,6 +178,8 @@ struct GRLaunch {
     PhysParams P;
     Hole hole;
     GeoTab tab;                 // geometry cache of this h (tab.geo null: evaluate at the call sites)
+    double z;                   // z of the evaluation positions of the point-level functions (the
+                                // sweeps and the forcing work in the z = 0 plane)
 };
 
 hipError_t launch_gr_sweep(const GRLaunch& L, int axis, hipStream_t s);

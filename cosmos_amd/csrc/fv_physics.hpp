@@ -58,8 +58,8 @@ CFV_HD void ev_add(Ev& a, const Ev& b, bool on) {
 }
 
 // libstdc++'s std::min / std::max: min(a, b) = (b < a) ? b : a.
-__device__ __forceinline__ double min_std(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double max_std(double a, double b) { return (a < b) ? b : a; }
+CFV_HD double min_std(double a, double b) { return (b < a) ? b : a; }
+CFV_HD double max_std(double a, double b) { return (a < b) ? b : a; }
 
 // RN(a / b) from y = RN(1 / b).
 CFV_HD double div_by(double a, double b, double y) {
@@ -81,7 +81,7 @@ struct NewtonianT {
     static constexpr bool kFast = false;
 
     // EulerStateVector::computeConservedVariableVector (Euler/EulerStateVector.cpp:55-66, 129-134).
-    __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
+    __host__ __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
         double vsq = (W[1] * W[1]) + (W[2] * W[2]);
         if (!W0) vsq = vsq + (W[3] * W[3]);
         double e = W[4] / (P.gm1 * W[0]);
@@ -95,7 +95,7 @@ struct NewtonianT {
     // Primitive recovery shared by setConservedVariableVector (Euler/EulerStateVector.cpp:30-40)
     // and compute{X,Y}FluxVector(cons) (:68-117).  R = (rho, u, v, w, p), E = U4 / rho.
     // irho = RN(1 / U0) may be supplied when already known.
-    __device__ static __forceinline__ void recover(const double* U, double* R, double& E, double irho,
+    __host__ __device__ static __forceinline__ void recover(const double* U, double* R, double& E, double irho,
                                                    const PhysParams& P) {
         double rho = U[0];
         double u = div_by(U[1], rho, irho);
@@ -114,7 +114,7 @@ struct NewtonianT {
 
     // Flux from recovered primitives (Euler/EulerStateVector.cpp:82-86 x, :110-114 y).
     template <int AXIS>
-    __device__ static __forceinline__ void flux_of(const double* R, double E, double* F) {
+    __host__ __device__ static __forceinline__ void flux_of(const double* R, double E, double* F) {
         double rho = R[0], u = R[1], v = R[2], w = R[3], p = R[4];
         if (AXIS == 0) {
             F[0] = rho * u;
@@ -132,7 +132,7 @@ struct NewtonianT {
     }
 
     // c2p: primitives of U.
-    __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P,
+    __host__ __device__ static __forceinline__ void c2p(const double* U, double* W, double irho, const PhysParams& P,
                                                Ev* = nullptr) {
         double E;
         recover(U, W, E, irho, P);
@@ -140,20 +140,20 @@ struct NewtonianT {
 
     // F(U) (the static cons flux).
     template <int AXIS>
-    __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P,
+    __host__ __device__ static __forceinline__ void flux(const double* U, double* F, double irho, const PhysParams& P,
                                                 Ev* = nullptr) {
         double R[5], E;
         recover(U, R, E, irho, P);
         flux_of<AXIS>(R, E, F);
     }
 
-    __device__ static __forceinline__ double inv_density(const double* U) { return 1.0 / U[0]; }
+    __host__ __device__ static __forceinline__ double inv_density(const double* U) { return 1.0 / U[0]; }
 
     // EulerEquationOfState::computeSoundSpeed (Euler/EulerEquationOfState.cpp:17-22) and the
     // per-cell wave speed of computeMaximumWaveSpeed{,2D} (Solvers/EulerSolvers.cpp:87-128):
     // 1D |u| + |c|; 2D max(|u|, |v| + |c|).
     template <int DIM>
-    __device__ static __forceinline__ double wavespeed(const double* W, const PhysParams& P) {
+    __host__ __device__ static __forceinline__ double wavespeed(const double* W, const PhysParams& P) {
         double c = __builtin_sqrt((P.gamma * W[4]) / W[0]);
         if (DIM == 1) return __builtin_fabs(W[1]) + __builtin_fabs(c);
         return max_std(__builtin_fabs(W[1]), __builtin_fabs(W[2]) + __builtin_fabs(c));
@@ -170,7 +170,7 @@ struct RelativisticT {
 
     // RelativisticEulerStateVector::computeConservedVariableVector
     // (RelativisticEuler/RelativisticEulerStateVector.cpp:116-136).
-    __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
+    __host__ __device__ static __forceinline__ void p2c(const double* W, double* U, const PhysParams& P) {
         double rho = W[0], u = W[1], v = W[2], w = W[3], p = W[4];
         double vsq = (u * u) + (v * v);
         if (!W0) vsq = vsq + (w * w);
@@ -189,7 +189,7 @@ struct RelativisticT {
     // flux functions :138-309): quartic Newton from 1.0, stop when |x - x_new| < tol keeping
     // the old iterate, floors 1e-8 on rho and p (after the velocities are formed).
     // R = (rho, vx, vy, vz, p) floored; lf = W; h = specific enthalpy.
-    __device__ static __forceinline__ void con2prim(const double* U, double* R, double& lf_out, double& h_out,
+    __host__ __device__ static __forceinline__ void con2prim(const double* U, double* R, double& lf_out, double& h_out,
                                                     const PhysParams& P, Ev* ev = nullptr) {
         double D = U[0], Sx = U[1], Sy = U[2], Sz = U[3], tau = U[4];
         double ssq = (Sx * Sx) + (Sy * Sy);
@@ -238,7 +238,7 @@ struct RelativisticT {
         }
     }
 
-    __device__ static __forceinline__ void c2p(const double* U, double* W, double, const PhysParams& P,
+    __host__ __device__ static __forceinline__ void c2p(const double* U, double* W, double, const PhysParams& P,
                                                Ev* ev = nullptr) {
         double lf, h;
         con2prim(U, W, lf, h, P, ev);
@@ -247,7 +247,7 @@ struct RelativisticT {
     // compute{X,Y}FluxVector(cons) (RelativisticEuler/RelativisticEulerStateVector.cpp:210-219, 299-308),
     // using the floored density and pressure.
     template <int AXIS>
-    __device__ static __forceinline__ void flux(const double* U, double* F, double, const PhysParams& P,
+    __host__ __device__ static __forceinline__ void flux(const double* U, double* F, double, const PhysParams& P,
                                                 Ev* ev = nullptr) {
         double R[5], lf, h;
         con2prim(U, R, lf, h, P, ev);
@@ -262,14 +262,14 @@ struct RelativisticT {
         F[4] = (((rhw - p) - (rho * lf)) * vn) + (p * vn);
     }
 
-    __device__ static __forceinline__ double inv_density(const double*) { return 0.0; }
+    __host__ __device__ static __forceinline__ double inv_density(const double*) { return 0.0; }
 
     // RelativisticEulerEquationOfState::computeSoundSpeed
     // (RelativisticEuler/RelativisticEulerEquationOfState.cpp:10-18) and the wave speed of
     // RelativisticEulerSolvers::computeMaximumWaveSpeed{,2D} (Solvers/RelativisticEulerSolvers.cpp:89-130):
     // 1D |u| + |c|; 2D max(|u|, |v|) + |c|.
     template <int DIM>
-    __device__ static __forceinline__ double wavespeed(const double* W, const PhysParams& P) {
+    __host__ __device__ static __forceinline__ double wavespeed(const double* W, const PhysParams& P) {
         double num = (P.gamma * W[4]) / W[0];
         double den = 1.0 + ((W[4] / W[0]) * P.sr_gg1);
         double c = __builtin_sqrt(num / den);
@@ -283,7 +283,7 @@ struct RelativisticT {
 // ---------------------------------------------------------------------------
 
 // computeSlopeLimiter (:3-69) for a general bias.
-__device__ __forceinline__ double limiter_general(double r, double bias, int limiter) {
+CFV_HD double limiter_general(double r, double bias, int limiter) {
     double xi = 2.0 / ((1.0 - bias) + ((1.0 + bias) * r));
     if (limiter == 0) {
         if (r < 0.0) return 0.0;
@@ -305,7 +305,7 @@ __device__ __forceinline__ double limiter_general(double r, double bias, int lim
 // smaller (RN is monotone), else the second, so one division of the selected
 // numerator gives std::min's result; r = +inf gives inf/inf = NaN in the
 // reference's first operand, which std::min then returns.
-__device__ __forceinline__ double vanleer_bias0(double r) {
+CFV_HD double vanleer_bias0(double r) {
     if (r < 0.0) return 0.0;
     double num = (r <= 1.0) ? (2.0 * r) : 2.0;
     if (r == __builtin_inf()) num = __builtin_nan("");
@@ -314,7 +314,7 @@ __device__ __forceinline__ double vanleer_bias0(double r) {
 
 // computeSlopeVector (:71-101) for one component.
 template <int LIMITER_MODE>  // 0: general (runtime limiter and bias), 1: van Leer at bias 0
-__device__ __forceinline__ double slope_component(double ul, double um, double ur, double bias, int limiter) {
+CFV_HD double slope_component(double ul, double um, double ur, double bias, int limiter) {
     double dl = um - ul;
     double dr = ur - um;
     double s = ((0.5 * (1.0 + bias)) * dl) + ((0.5 * (1.0 - bias)) * dr);
@@ -324,6 +324,31 @@ __device__ __forceinline__ double slope_component(double ul, double um, double u
     double phi = (LIMITER_MODE == 1) ? vanleer_bias0(r) : limiter_general(r, bias, limiter);
     return s * phi;
 }
+
+// ---------------------------------------------------------------------------
+// Per-component arithmetic of the flux schemes, the ratios dt/dx, dx/dt and
+// f (dt/dx) formed once by the caller as the reference forms them
+// (Solvers/EulerFirstOrderSolver.cpp:3-24, 96-101; Solvers/EulerSolvers.cpp:163-172).
+// ---------------------------------------------------------------------------
+
+// computeLaxFriedrichsFlux: 0.5 ((F_L + F_R) + (dx/dt) (U_L - U_R))
+CFV_HD double lax_friedrichs_k(double ul, double ur, double fl, double fr, double dx_dt) {
+    return 0.5 * ((fl + fr) + (dx_dt * (ul - ur)));
+}
+
+// computeRichtmyerFlux (the intermediate state): 0.5 ((U_L + U_R) + (dt/dx) (F_L - F_R))
+CFV_HD double richtmyer_k(double ul, double ur, double fl, double fr, double dt_dx) {
+    return 0.5 * ((ul + ur) + (dt_dx * (fl - fr)));
+}
+
+// computeFORCEFlux: 0.5 (F_LF + F_Richtmyer)
+CFV_HD double force_k(double lf, double ri) { return 0.5 * (lf + ri); }
+
+// computeFORCEUpdate: U + (dt/dx) (F_L - F_R)
+CFV_HD double force_update_k(double u, double fl, double fr, double dt_dx) { return u + (dt_dx * (fl - fr)); }
+
+// computeFractionalEvolutionVector: (f (dt/dx)) (F_L - F_R)
+CFV_HD double evolution_k(double fdt_dx, double fl, double fr) { return fdt_dx * (fl - fr); }
 
 // ---------------------------------------------------------------------------
 // Fast mode (Newtonian, opt-in).  The same equations, re-associated for

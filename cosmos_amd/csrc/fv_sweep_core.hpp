@@ -42,26 +42,24 @@ __device__ __forceinline__ void store_cell(const Planes& out, long idx, const do
         if (!(W0 && k == 3)) out.p[k][idx] = W[k];
 }
 
-// Half-step evolution of one cell from its conserved neighbours
-// (EulerSolvers::evolveStateByHalf{X,Y}TimeStep, Solvers/EulerSolvers.cpp:191-223),
-// then the p2c and flux of both evolved face states as the FORCE flux of
-// the neighbouring interfaces will need them (Solvers/EulerFirstOrderSolver.cpp:52-59).
-// Con2prim events are counted into eh (the face fluxes and face states), eg0
-// and eg1 (the fluxes of the two face states' conserved vectors) when given.
+// Half-step evolution of one cell from its conserved neighbours up to the
+// evolved face states' conserved vectors a (side 0) and b (side 1)
+// (EulerSolvers::evolveStateByHalf{X,Y}TimeStep, Solvers/EulerSolvers.cpp:191-223,
+// before its final setConservedVariableVector).  Con2prim events of the face
+// fluxes are counted into eh when given.
 template <class Phys, int AXIS, int LIMMODE, bool W0>
-__device__ __forceinline__ void half_step(const double* ul, const double* um, const double* ur, const SweepCoef& cf,
-                                          double bias, int limiter, const PhysParams& P, double* Q0, double* G0,
-                                          double* Q1, double* G1, Ev* eh = nullptr, Ev* eg0 = nullptr,
-                                          Ev* eg1 = nullptr) {
-    double xl[5], xr[5], fl[5], fr[5], a[5], b[5], p0[5], p1[5];
+CFV_HD void half_faces(const double* ul, const double* um, const double* ur, const SweepCoef& cf, double bias,
+                       int limiter, const PhysParams& P, double* a, double* b, Ev* eh = nullptr) {
+    double xl[5], xr[5], fl[5], fr[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (W0 && k == 3) {
             xl[k] = xr[k] = 0.0;
             continue;
         }
-        double s = LIMMODE == 2 ? slope_component_fast_vl(ul[k], um[k], ur[k])
-                                : slope_component<LIMMODE>(ul[k], um[k], ur[k], bias, limiter);
+        double s;
+        if constexpr (LIMMODE == 2) s = slope_component_fast_vl(ul[k], um[k], ur[k]);
+        else s = slope_component<LIMMODE>(ul[k], um[k], ur[k], bias, limiter);
         if constexpr (Phys::kFast) {
             xl[k] = __builtin_fma(-0.5, s, um[k]);
             xr[k] = __builtin_fma(0.5, s, um[k]);
@@ -74,10 +72,23 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
     Phys::template flux<AXIS>(xr, fr, Phys::inv_density(xr), P, eh);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        double ev = (W0 && k == 3) ? 0.0 : cf.hdt_dx * (fl[k] - fr[k]);
+        double ev = (W0 && k == 3) ? 0.0 : evolution_k(cf.hdt_dx, fl[k], fr[k]);
         a[k] = (W0 && k == 3) ? 0.0 : xl[k] + ev;
         b[k] = (W0 && k == 3) ? 0.0 : xr[k] + ev;
     }
+}
+
+// The half-step evolution of one cell (half_faces), then the p2c and flux of
+// both evolved face states as the FORCE flux of the neighbouring interfaces
+// will need them (Solvers/EulerFirstOrderSolver.cpp:52-59).
+// Con2prim events are counted into eh (the face fluxes and face states), eg0
+// and eg1 (the fluxes of the two face states' conserved vectors) when given.
+template <class Phys, int AXIS, int LIMMODE, bool W0>
+CFV_HD void half_step(const double* ul, const double* um, const double* ur, const SweepCoef& cf, double bias,
+                      int limiter, const PhysParams& P, double* Q0, double* G0, double* Q1, double* G1,
+                      Ev* eh = nullptr, Ev* eg0 = nullptr, Ev* eg1 = nullptr) {
+    double a[5], b[5], p0[5], p1[5];
+    half_faces<Phys, AXIS, LIMMODE, W0>(ul, um, ur, cf, bias, limiter, P, a, b, eh);
     // P0 = c2p(a) (side 0), P1 = c2p(b) (side 1); p2c(P) has density P[0] = a[0] (resp.
     // b[0]), so the flux of p2c(P) divides by the same density.
     double ia = Phys::inv_density(a), ib = Phys::inv_density(b);
@@ -104,8 +115,8 @@ __device__ __forceinline__ void half_step(const double* ul, const double* um, co
 // (computeLaxFriedrichsFlux, computeRichtmyerFlux, computeFORCEFlux:
 // Solvers/EulerFirstOrderSolver.cpp:3-24).
 template <class Phys, int AXIS, bool W0>
-__device__ __forceinline__ void force_flux(const double* UL, const double* FL, const double* UR, const double* FR,
-                                           const SweepCoef& cf, const PhysParams& P, double* F, Ev* ei = nullptr) {
+CFV_HD void force_flux(const double* UL, const double* FL, const double* UR, const double* FR, const SweepCoef& cf,
+                       const PhysParams& P, double* F, Ev* ei = nullptr) {
     double us[5], fs[5];
     if constexpr (Phys::kFast) {
         // U* = (UL + UR) / 2 + (dt / 2dx)(FL - FR);  F = (FL + FR) / 4 + (dx / 4dt)(UL - UR) + F(U*) / 2
@@ -126,7 +137,7 @@ __device__ __forceinline__ void force_flux(const double* UL, const double* FL, c
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k)
-        us[k] = (W0 && k == 3) ? 0.0 : 0.5 * ((UL[k] + UR[k]) + (cf.dt_dx * (FL[k] - FR[k])));
+        us[k] = (W0 && k == 3) ? 0.0 : richtmyer_k(UL[k], UR[k], FL[k], FR[k], cf.dt_dx);
     Phys::template flux<AXIS>(us, fs, Phys::inv_density(us), P, ei);
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -134,21 +145,20 @@ __device__ __forceinline__ void force_flux(const double* UL, const double* FL, c
             F[k] = 0.0;
             continue;
         }
-        double lf = 0.5 * ((FL[k] + FR[k]) + (cf.dx_dt * (UL[k] - UR[k])));
-        F[k] = 0.5 * (lf + fs[k]);
+        F[k] = force_k(lax_friedrichs_k(UL[k], UR[k], FL[k], FR[k], cf.dx_dt), fs[k]);
     }
 }
 
 // computeFORCEUpdate (Solvers/EulerFirstOrderSolver.cpp:96-101) then c2p.
 template <class Phys, bool W0>
-__device__ __forceinline__ void update_cell(const double* U, const double* Fl, const double* Fr, const SweepCoef& cf,
-                                            const PhysParams& P, double* W, Ev* eu = nullptr) {
+CFV_HD void update_cell(const double* U, const double* Fl, const double* Fr, const SweepCoef& cf,
+                        const PhysParams& P, double* W, Ev* eu = nullptr) {
     double un[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         if (W0 && k == 3) un[k] = 0.0;
         else if constexpr (Phys::kFast) un[k] = __builtin_fma(cf.dt_dx, Fl[k] - Fr[k], U[k]);
-        else un[k] = U[k] + (cf.dt_dx * (Fl[k] - Fr[k]));
+        else un[k] = force_update_k(U[k], Fl[k], Fr[k], cf.dt_dx);
     }
     Phys::c2p(un, W, Phys::inv_density(un), P, eu);
 }

@@ -161,12 +161,7 @@ double wave_speed_2d(int physics, vector<vector<S> >& cells, double gamma) {
 
 }  // namespace
 
-// --- SlopeLimiters ---------------------------------------------------------
-double SlopeLimiters::computeSlopeLimiter(double steepness, double bias, int slopeLimiter) {
-    double out = 0.0;
-    check(cosmos_slope_limiter(&steepness, 1, bias, slopeLimiter, &out), nullptr);
-    return out;
-}
+// --- SlopeLimiters (computeSlopeLimiter and the slope vectors: cosmos_dropin_point.cpp) ---
 double SlopeLimiters::computeSuperBeeSlopeLimiter(double steepness, double bias) {
     return computeSlopeLimiter(steepness, bias, 0);
 }
@@ -180,14 +175,6 @@ double SlopeLimiters::computeGradientRatio(double steepness, double bias) {
     // xi_R(r) = 2 / ((1 - bias) + (1 + bias) r) (Solvers/SlopeLimiters.cpp:3-6): a scalar helper
     // the limiters use; the device evaluates the same expression inside the sweeps.
     return 2.0 / ((1.0 - bias) + ((1.0 + bias) * steepness));
-}
-vector<double> SlopeLimiters::computeSlopeVector(vector<double> left, vector<double> middle, vector<double> right,
-                                                 double bias, int slopeLimiter) {
-    if (left.size() != 5 || middle.size() != 5 || right.size() != 5)
-        throw std::invalid_argument("cosmos: slope vectors have 5 components");
-    vector<double> out(5);
-    check(cosmos_slope_vector(left.data(), middle.data(), right.data(), 1, bias, slopeLimiter, out.data()), nullptr);
-    return out;
 }
 
 // --- Newtonian ---------------------------------------------------------------

@@ -23,10 +23,18 @@
 // EulerMaterialParameters.hpp) when the drop-in is built inside it; the
 // compat/ directory supplies minimal stand-ins for standalone builds.
 //
+// Grid-level methods run on the device; the point-level statics (one state,
+// interface or cell: fluxes, half and fractional steps, slope vectors, the
+// black-hole forcing of one cell, the vector arithmetic) are evaluated on the
+// host by the library's cosmos_point_* / cosmos_vector_* entry points, which
+// compile the kernels' own physics code for the host (same bits, no device
+// round trip per call).
+//
 // Differences, all deliberate: device or argument failures throw
 // std::runtime_error (the reference has no error path); solve/solve2D run
 // the whole loop on the device and do not print the per-iteration status
-// line (EulerSolvers::outputStatus is still provided).
+// line (EulerSolvers::outputStatus is still provided); vector arguments of
+// the point-level statics must have the reference's five components.
 #ifndef COSMOS_DROPIN_HPP
 #define COSMOS_DROPIN_HPP
 
@@ -52,10 +60,51 @@ public:
     static vector<double> computeSlopeVector(vector<double> leftConservedVariableVector,
                                              vector<double> middleConservedVariableVector,
                                              vector<double> rightConservedVariableVector, double bias, int slopeLimiter);
+    static vector<double> computeSlopeVector(EulerStateVector leftStateVector, EulerStateVector middleStateVector,
+                                             EulerStateVector rightStateVector, double bias, int slopeLimiter,
+                                             EulerMaterialParameters materialParameters);
+    static vector<double> computeSlopeVector(RelativisticEulerStateVector leftStateVector,
+                                             RelativisticEulerStateVector middleStateVector,
+                                             RelativisticEulerStateVector rightStateVector, double bias,
+                                             int slopeLimiter, EulerMaterialParameters materialParameters);
+    static vector<double> computeSlopeVectorX(BlackHoleEulerStateVector leftStateVector,
+                                              BlackHoleEulerStateVector middleStateVector,
+                                              BlackHoleEulerStateVector rightStateVector, double cellSpacing,
+                                              double bias, int slopeLimiter, double xCoordinate, double yCoordinate,
+                                              double zCoordinate, EulerMaterialParameters materialParameters,
+                                              BlackHoleSpacetime blackHole);
+    static vector<double> computeSlopeVectorY(BlackHoleEulerStateVector topStateVector,
+                                              BlackHoleEulerStateVector middleStateVector,
+                                              BlackHoleEulerStateVector bottomStateVector, double cellSpacing,
+                                              double bias, int slopeLimiter, double xCoordinate, double yCoordinate,
+                                              double zCoordinate, EulerMaterialParameters materialParameters,
+                                              BlackHoleSpacetime blackHole);
 };
 
 class EulerSolvers {
 public:
+    static vector<EulerStateVector> insertBoundaryCells(vector<EulerStateVector>& currentCells, int boundarySize);
+    static vector<vector<EulerStateVector> > insertBoundaryCells2D(vector<vector<EulerStateVector> >& currentCells,
+                                                                   int boundarySize);
+    static vector<double> computeFractionalEvolutionVector(double stepFraction, vector<double> leftFluxVector,
+                                                           vector<double> rightFluxVector, double cellSpacing,
+                                                           double timeStep);
+    static vector<double> computeEvolutionVector(vector<double> leftFluxVector, vector<double> rightFluxVector,
+                                                 double cellSpacing, double timeStep);
+    static EulerStateVector evolveStateByHalfTimeStep(vector<double> leftExtrapolatedValue,
+                                                      vector<double> rightExtrapolatedValue,
+                                                      vector<double> evolutionVector, int side,
+                                                      EulerMaterialParameters materialParameters);
+    static EulerStateVector evolveStateByHalfXTimeStep(EulerStateVector leftStateVector,
+                                                       EulerStateVector middleStateVector,
+                                                       EulerStateVector rightStateVector, double cellSpacing,
+                                                       double timeStep, double bias, int slopeLimiter, int side,
+                                                       EulerMaterialParameters materialParameters);
+    static EulerStateVector evolveStateByHalfYTimeStep(EulerStateVector topStateVector,
+                                                       EulerStateVector middleStateVector,
+                                                       EulerStateVector bottomStateVector, double cellSpacing,
+                                                       double timeStep, double bias, int slopeLimiter, int side,
+                                                       EulerMaterialParameters materialParameters);
     static double computeMaximumWaveSpeed(vector<EulerStateVector>& currentCells, EulerMaterialParameters materialParameters);
     static double computeMaximumWaveSpeed2D(vector<vector<EulerStateVector> >& currentCells,
                                             EulerMaterialParameters materialParameters);
@@ -71,6 +120,35 @@ public:
 
 class EulerFirstOrderSolver {
 public:
+    static vector<double> computeLaxFriedrichsFlux(vector<double> leftConservedVariableVector,
+                                                   vector<double> rightConservedVariableVector,
+                                                   vector<double> leftFluxVector, vector<double> rightFluxVector,
+                                                   double cellSpacing, double timeStep);
+    static vector<double> computeRichtmyerFlux(vector<double> leftConservedVariableVector,
+                                               vector<double> rightConservedVariableVector,
+                                               vector<double> leftFluxVector, vector<double> rightFluxVector,
+                                               double cellSpacing, double timeStep);
+    static vector<double> computeFORCEFlux(vector<double> laxFriedrichsFlux, vector<double> richtmyerFlux);
+    static vector<double> computeXLaxFriedrichsFlux(EulerStateVector leftStateVector, EulerStateVector rightStateVector,
+                                                    double cellSpacing, double timeStep,
+                                                    EulerMaterialParameters materialParameters);
+    static vector<double> computeXRichtmyerFlux(EulerStateVector leftStateVector, EulerStateVector rightStateVector,
+                                                double cellSpacing, double timeStep,
+                                                EulerMaterialParameters materialParameters);
+    static vector<double> computeXFORCEFlux(EulerStateVector leftStateVector, EulerStateVector rightStateVector,
+                                            double cellSpacing, double timeStep,
+                                            EulerMaterialParameters materialParameters);
+    static vector<double> computeYLaxFriedrichsFlux(EulerStateVector topStateVector, EulerStateVector bottomStateVector,
+                                                    double cellSpacing, double timeStep,
+                                                    EulerMaterialParameters materialParameters);
+    static vector<double> computeYRichtmyerFlux(EulerStateVector topStateVector, EulerStateVector bottomStateVector,
+                                                double cellSpacing, double timeStep,
+                                                EulerMaterialParameters materialParameters);
+    static vector<double> computeYFORCEFlux(EulerStateVector topStateVector, EulerStateVector bottomStateVector,
+                                            double cellSpacing, double timeStep,
+                                            EulerMaterialParameters materialParameters);
+    static vector<double> computeFORCEUpdate(vector<double> conservedVariableVector, vector<double> leftFluxVector,
+                                             vector<double> rightFluxVector, double cellSpacing, double timeStep);
     static void computeFORCETimeStep(vector<EulerStateVector>& currentCells, double cellSpacing, double timeStep,
                                      EulerMaterialParameters materialParameters);
     static void computeXFORCETimeStep2D(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
@@ -87,6 +165,14 @@ public:
 
 class EulerSecondOrderSolver {
 public:
+    static vector<double> computeXSLICFlux(EulerStateVector leftLeftStateVector, EulerStateVector leftStateVector,
+                                           EulerStateVector rightStateVector, EulerStateVector rightRightStateVector,
+                                           double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                           EulerMaterialParameters materialParameters);
+    static vector<double> computeYSLICFlux(EulerStateVector topTopStateVector, EulerStateVector topStateVector,
+                                           EulerStateVector bottomStateVector, EulerStateVector bottomBottomStateVector,
+                                           double cellSpacing, double timeStep, double bias, int slopeLimiter,
+                                           EulerMaterialParameters materialParameters);
     static void computeSLICTimeStep(vector<EulerStateVector>& currentCells, double cellSpacing, double timeStep,
                                     double bias, int slopeLimiter, EulerMaterialParameters materialParameters);
     static void computeXSLICTimeStep2D(vector<vector<EulerStateVector> >& currentCells, double cellSpacing,
@@ -106,6 +192,22 @@ public:
 
 class RelativisticEulerSolvers {
 public:
+    static vector<RelativisticEulerStateVector> insertBoundaryCells(vector<RelativisticEulerStateVector>& currentCells,
+                                                                    int boundarySize);
+    static vector<vector<RelativisticEulerStateVector> > insertBoundaryCells2D(
+        vector<vector<RelativisticEulerStateVector> >& currentCells, int boundarySize);
+    static RelativisticEulerStateVector evolveStateByHalfTimeStep(vector<double> leftExtrapolatedValue,
+                                                                  vector<double> rightExtrapolatedValue,
+                                                                  vector<double> evolutionVector, int side,
+                                                                  EulerMaterialParameters materialParameters);
+    static RelativisticEulerStateVector evolveStateByHalfXTimeStep(
+        RelativisticEulerStateVector leftStateVector, RelativisticEulerStateVector middleStateVector,
+        RelativisticEulerStateVector rightStateVector, double cellSpacing, double timeStep, double bias,
+        int slopeLimiter, int side, EulerMaterialParameters materialParameters);
+    static RelativisticEulerStateVector evolveStateByHalfYTimeStep(
+        RelativisticEulerStateVector topStateVector, RelativisticEulerStateVector middleStateVector,
+        RelativisticEulerStateVector bottomStateVector, double cellSpacing, double timeStep, double bias,
+        int slopeLimiter, int side, EulerMaterialParameters materialParameters);
     static double computeMaximumWaveSpeed(vector<RelativisticEulerStateVector>& currentCells,
                                           EulerMaterialParameters materialParameters);
     static double computeMaximumWaveSpeed2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
@@ -121,6 +223,24 @@ public:
 
 class RelativisticEulerFirstOrderSolver {
 public:
+    static vector<double> computeXLaxFriedrichsFlux(RelativisticEulerStateVector leftStateVector,
+                                                    RelativisticEulerStateVector rightStateVector, double cellSpacing,
+                                                    double timeStep, EulerMaterialParameters materialParameters);
+    static vector<double> computeXRichtmyerFlux(RelativisticEulerStateVector leftStateVector,
+                                                RelativisticEulerStateVector rightStateVector, double cellSpacing,
+                                                double timeStep, EulerMaterialParameters materialParameters);
+    static vector<double> computeXFORCEFlux(RelativisticEulerStateVector leftStateVector,
+                                            RelativisticEulerStateVector rightStateVector, double cellSpacing,
+                                            double timeStep, EulerMaterialParameters materialParameters);
+    static vector<double> computeYLaxFriedrichsFlux(RelativisticEulerStateVector topStateVector,
+                                                    RelativisticEulerStateVector bottomStateVector, double cellSpacing,
+                                                    double timeStep, EulerMaterialParameters materialParameters);
+    static vector<double> computeYRichtmyerFlux(RelativisticEulerStateVector topStateVector,
+                                                RelativisticEulerStateVector bottomStateVector, double cellSpacing,
+                                                double timeStep, EulerMaterialParameters materialParameters);
+    static vector<double> computeYFORCEFlux(RelativisticEulerStateVector topStateVector,
+                                            RelativisticEulerStateVector bottomStateVector, double cellSpacing,
+                                            double timeStep, EulerMaterialParameters materialParameters);
     static void computeFORCETimeStep(vector<RelativisticEulerStateVector>& currentCells, double cellSpacing,
                                      double timeStep, EulerMaterialParameters materialParameters);
     static void computeXFORCETimeStep2D(vector<vector<RelativisticEulerStateVector> >& currentCells,
@@ -137,6 +257,18 @@ public:
 
 class RelativisticEulerSecondOrderSolver {
 public:
+    static vector<double> computeXSLICFlux(RelativisticEulerStateVector leftLeftStateVector,
+                                           RelativisticEulerStateVector leftStateVector,
+                                           RelativisticEulerStateVector rightStateVector,
+                                           RelativisticEulerStateVector rightRightStateVector, double cellSpacing,
+                                           double timeStep, double bias, int slopeLimiter,
+                                           EulerMaterialParameters materialParameters);
+    static vector<double> computeYSLICFlux(RelativisticEulerStateVector topTopStateVector,
+                                           RelativisticEulerStateVector topStateVector,
+                                           RelativisticEulerStateVector bottomStateVector,
+                                           RelativisticEulerStateVector bottomBottomStateVector, double cellSpacing,
+                                           double timeStep, double bias, int slopeLimiter,
+                                           EulerMaterialParameters materialParameters);
     static void computeSLICTimeStep(vector<RelativisticEulerStateVector>& currentCells, double cellSpacing,
                                     double timeStep, double bias, int slopeLimiter,
                                     EulerMaterialParameters materialParameters);
@@ -157,6 +289,50 @@ public:
 
 class BlackHoleEulerSolvers {
 public:
+    static vector<BlackHoleEulerStateVector> insertBoundaryCells(vector<BlackHoleEulerStateVector>& currentCells,
+                                                                 int boundarySize);
+    static vector<vector<BlackHoleEulerStateVector> > insertBoundaryCells2D(
+        vector<vector<BlackHoleEulerStateVector> >& currentCells, int boundarySize);
+    static BlackHoleEulerStateVector evolveStateByFractionalTimeStep(vector<double> middleConservedVariableVector,
+                                                                     vector<double> conservedVariableVectorEvolution,
+                                                                     double xCoordinate, double yCoordinate,
+                                                                     double zCoordinate,
+                                                                     EulerMaterialParameters materialParameters,
+                                                                     BlackHoleSpacetime blackHole);
+    static BlackHoleEulerStateVector evolveStateByFractionalXTimeStep(
+        double stepFraction, BlackHoleEulerStateVector leftStateVector, BlackHoleEulerStateVector middleStateVector,
+        BlackHoleEulerStateVector rightStateVector, double cellSpacing, double timeStep, double bias, int slopeLimiter,
+        double xCoordinate, double yCoordinate, double zCoordinate, EulerMaterialParameters materialParameters,
+        BlackHoleSpacetime blackHole);
+    static BlackHoleEulerStateVector evolveStateByFractionalYTimeStep(
+        double stepFraction, BlackHoleEulerStateVector topStateVector, BlackHoleEulerStateVector middleStateVector,
+        BlackHoleEulerStateVector bottomStateVector, double cellSpacing, double timeStep, double bias, int slopeLimiter,
+        double xCoordinate, double yCoordinate, double zCoordinate, EulerMaterialParameters materialParameters,
+        BlackHoleSpacetime blackHole);
+    static BlackHoleEulerStateVector evolveStateByHalfXTimeStep(vector<double> leftExtrapolatedValue,
+                                                                vector<double> rightExtrapolatedValue,
+                                                                vector<double> evolutionVector, int side,
+                                                                double cellSpacing, double xCoordinate,
+                                                                double yCoordinate, double zCoordinate,
+                                                                EulerMaterialParameters materialParameters,
+                                                                BlackHoleSpacetime blackHole);
+    static BlackHoleEulerStateVector evolveStateByHalfXTimeStep(
+        BlackHoleEulerStateVector leftStateVector, BlackHoleEulerStateVector middleStateVector,
+        BlackHoleEulerStateVector rightStateVector, double cellSpacing, double timeStep, double bias, int slopeLimiter,
+        int side, double xCoordinate, double yCoordinate, double zCoordinate, EulerMaterialParameters materialParameters,
+        BlackHoleSpacetime blackHole);
+    static BlackHoleEulerStateVector evolveStateByHalfYTimeStep(vector<double> topExtrapolatedValue,
+                                                                vector<double> bottomExtrapolatedValue,
+                                                                vector<double> evolutionVector, int side,
+                                                                double cellSpacing, double xCoordinate,
+                                                                double yCoordinate, double zCoordinate,
+                                                                EulerMaterialParameters materialParameters,
+                                                                BlackHoleSpacetime blackHole);
+    static BlackHoleEulerStateVector evolveStateByHalfYTimeStep(
+        BlackHoleEulerStateVector topStateVector, BlackHoleEulerStateVector middleStateVector,
+        BlackHoleEulerStateVector bottomStateVector, double cellSpacing, double timeStep, double bias, int slopeLimiter,
+        int side, double xCoordinate, double yCoordinate, double zCoordinate, EulerMaterialParameters materialParameters,
+        BlackHoleSpacetime blackHole);
     static double computeMaximumWaveSpeed(vector<BlackHoleEulerStateVector>& currentCells,
                                           EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
     static double computeMaximumWaveSpeed2D(vector<vector<BlackHoleEulerStateVector> >& currentCells,
@@ -173,6 +349,19 @@ public:
 
 class BlackHoleEulerForcingSolver {
 public:
+    static vector<double> evolveConservedVariableVector(vector<double> leftConservedVariableVector,
+                                                        vector<double> middleConservedVariableVector,
+                                                        vector<double> rightConservedVaraibleVector,
+                                                        double xCoordinate, double yCoordinate, double zCoordinate,
+                                                        double cellSpacing, double timeStep, double bias,
+                                                        int slopeLimiter, EulerMaterialParameters materialParameters,
+                                                        BlackHoleSpacetime blackHole);
+    static vector<double> evolveConservedVariableVector2D(
+        vector<double> leftConservedVariableVector, vector<double> middleConservedVariableVector,
+        vector<double> rightConservedVariableVector, vector<double> topConservedVariableVector,
+        vector<double> bottomConservedVariableVector, double xCoordinate, double yCoordinate, double zCoordinate,
+        double cellSpacing, double timeStep, double bias, int slopeLimiter, EulerMaterialParameters materialParameters,
+        BlackHoleSpacetime blackHole);
     static void computeRungeKuttaTimeStep(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
                                           double timeStep, double bias, int slopeLimiter,
                                           EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
@@ -183,6 +372,36 @@ public:
 
 class BlackHoleEulerFirstOrderSolver {
 public:
+    static vector<double> computeXLaxFriedrichsFlux(BlackHoleEulerStateVector leftStateVector,
+                                                    BlackHoleEulerStateVector rightStateVector, double xCoordinate,
+                                                    double yCoordinate, double zCoordinate, double cellSpacing,
+                                                    double timeStep, EulerMaterialParameters materialParameters,
+                                                    BlackHoleSpacetime blackHole);
+    static vector<double> computeXRichtmyerFlux(BlackHoleEulerStateVector leftStateVector,
+                                                BlackHoleEulerStateVector rightStateVector, double xCoordinate,
+                                                double yCoordinate, double zCoordinate, double cellSpacing,
+                                                double timeStep, EulerMaterialParameters materialParameters,
+                                                BlackHoleSpacetime blackHole);
+    static vector<double> computeXFORCEFlux(BlackHoleEulerStateVector leftStateVector,
+                                            BlackHoleEulerStateVector rightStateVector, double xCoordinate,
+                                            double yCoordinate, double zCoordinate, double cellSpacing,
+                                            double timeStep, EulerMaterialParameters materialParameters,
+                                            BlackHoleSpacetime blackHole);
+    static vector<double> computeYLaxFriedrichsFlux(BlackHoleEulerStateVector topStateVector,
+                                                    BlackHoleEulerStateVector bottomStateVector, double xCoordinate,
+                                                    double yCoordinate, double zCoordinate, double cellSpacing,
+                                                    double timeStep, EulerMaterialParameters materialParameters,
+                                                    BlackHoleSpacetime blackHole);
+    static vector<double> computeYRichtmyerFlux(BlackHoleEulerStateVector topStateVector,
+                                                BlackHoleEulerStateVector bottomStateVector, double xCoordinate,
+                                                double yCoordinate, double zCoordinate, double cellSpacing,
+                                                double timeStep, EulerMaterialParameters materialParameters,
+                                                BlackHoleSpacetime blackHole);
+    static vector<double> computeYFORCEFlux(BlackHoleEulerStateVector topStateVector,
+                                            BlackHoleEulerStateVector bottomStateVector, double xCoordinate,
+                                            double yCoordinate, double zCoordinate, double cellSpacing,
+                                            double timeStep, EulerMaterialParameters materialParameters,
+                                            BlackHoleSpacetime blackHole);
     static void computeFORCETimeStep(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
                                      double timeStep, EulerMaterialParameters materialParameters,
                                      BlackHoleSpacetime blackHole);
@@ -205,6 +424,20 @@ public:
 
 class BlackHoleEulerSecondOrderSolver {
 public:
+    static vector<double> computeXSLICFlux(BlackHoleEulerStateVector leftLeftStateVector,
+                                           BlackHoleEulerStateVector leftStateVector,
+                                           BlackHoleEulerStateVector rightStateVector,
+                                           BlackHoleEulerStateVector rightRightStateVector, double cellSpacing,
+                                           double timeStep, double bias, int slopeLimiter, double xCoordinate,
+                                           double yCoordinate, double zCoordinate,
+                                           EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
+    static vector<double> computeYSLICFlux(BlackHoleEulerStateVector topTopStateVector,
+                                           BlackHoleEulerStateVector topStateVector,
+                                           BlackHoleEulerStateVector bottomStateVector,
+                                           BlackHoleEulerStateVector bottomBottomStateVector, double cellSpacing,
+                                           double timeStep, double bias, int slopeLimiter, double xCoordinate,
+                                           double yCoordinate, double zCoordinate,
+                                           EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);
     static void computeSLICTimeStep(vector<BlackHoleEulerStateVector>& currentCells, double cellSpacing,
                                     double timeStep, double bias, int slopeLimiter,
                                     EulerMaterialParameters materialParameters, BlackHoleSpacetime blackHole);

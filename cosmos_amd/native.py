@@ -149,6 +149,43 @@ SIGNATURES = {
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
                                             ctypes.c_int, _c_double_p]),
+    "cosmos_point_last_error": (ctypes.c_char_p, []),
+    "cosmos_vector_lax_friedrichs": (ctypes.c_int, [_c_double_p] * 4 + [ctypes.c_long, ctypes.c_double,
+                                                                         ctypes.c_double, _c_double_p]),
+    "cosmos_vector_richtmyer": (ctypes.c_int, [_c_double_p] * 4 + [ctypes.c_long, ctypes.c_double, ctypes.c_double,
+                                                                    _c_double_p]),
+    "cosmos_vector_force": (ctypes.c_int, [_c_double_p, _c_double_p, ctypes.c_long, _c_double_p]),
+    "cosmos_vector_force_update": (ctypes.c_int, [_c_double_p] * 3 + [ctypes.c_long, ctypes.c_double,
+                                                                       ctypes.c_double, _c_double_p]),
+    "cosmos_vector_evolution": (ctypes.c_int, [ctypes.c_double, _c_double_p, _c_double_p, ctypes.c_long,
+                                                ctypes.c_double, ctypes.c_double, _c_double_p]),
+    "cosmos_vector_slope": (ctypes.c_int, [_c_double_p] * 3 + [ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                                                                _c_double_p]),
+    "cosmos_point_slope_limiter": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_double_p]),
+    "cosmos_point_conserved": (ctypes.c_int, [ctypes.c_int, _c_double_p, ctypes.c_double, _c_double_p, _c_double_p,
+                                               _c_double_p]),
+    "cosmos_point_primitive": (ctypes.c_int, [ctypes.c_int, _c_double_p, ctypes.c_double, _c_double_p, _c_double_p,
+                                               _c_double_p]),
+    "cosmos_point_flux": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _c_double_p, ctypes.c_double, _c_double_p,
+                                          _c_double_p, _c_double_p]),
+    "cosmos_point_interface_flux": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_double_p, _c_double_p,
+                                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, _c_double_p,
+                                                    _c_double_p, _c_double_p]),
+    "cosmos_point_slic_flux": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_c_double_p] * 4 +
+                               [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                _c_double_p, _c_double_p, _c_double_p]),
+    "cosmos_point_half_step": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_c_double_p] * 3 +
+                               [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, _c_double_p, _c_double_p, _c_double_p]),
+    "cosmos_point_fractional_step": (ctypes.c_int, [ctypes.c_int, ctypes.c_double] + [_c_double_p] * 3 +
+                                     [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_double, _c_double_p, _c_double_p, _c_double_p]),
+    "cosmos_point_slope_vector": (ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_c_double_p] * 3 +
+                                  [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double, _c_double_p,
+                                   _c_double_p, _c_double_p]),
+    "cosmos_point_forcing": (ctypes.c_int, [ctypes.c_int] + [_c_double_p] * 5 +
+                             [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                              _c_double_p, _c_double_p, _c_double_p]),
     "cosmos_synchronize": (ctypes.c_int, [_handle]),
     "cosmos_stream": (ctypes.c_void_p, [_handle]),
     "cosmos_profile_enable": (ctypes.c_int, [_handle, ctypes.c_int]),

@@ -415,6 +415,100 @@ int cosmos_slope_limiter(const double* steepness, long n, double bias, int slope
 int cosmos_slope_vector(const double* left, const double* middle, const double* right, long n, double bias,
                         int slope_limiter, double* out);
 
+/* ---- Point-level functions, evaluated on the host ----------------------------
+ * The public statics of the reference's solver classes that work on one state,
+ * one interface or one cell, computed by the same physics code the kernels run
+ * (compiled for the host), so their results are the kernels' bits without a
+ * device round trip per call.  Primitive states are (rho, vx, vy, vz, p),
+ * conserved and flux vectors the reference's five components.  For
+ * COSMOS_BLACKHOLE, spacetime = (mass, spin, x, y, z) of
+ * BlackHoleSpacetime(mass, spin, x, y, z) and position = the (x, y, z) the
+ * reference method takes; both are ignored (may be NULL) otherwise.  Every
+ * entry point returns COSMOS_OK or COSMOS_ERR_ARGUMENT (message:
+ * cosmos_point_last_error, per thread). */
+const char* cosmos_point_last_error(void);
+
+/* Vector arithmetic on n components:
+ *   lax_friedrichs  0.5((F_L + F_R) + (dx/dt)(U_L - U_R))  EulerFirstOrderSolver::computeLaxFriedrichsFlux
+ *                                                         (Solvers/EulerFirstOrderSolver.cpp:3-11)
+ *   richtmyer       0.5((U_L + U_R) + (dt/dx)(F_L - F_R))  ::computeRichtmyerFlux, the intermediate state (:13-19)
+ *   force           0.5(F_LF + F_Ri)                       ::computeFORCEFlux (:21-24)
+ *   force_update    U + (dt/dx)(F_L - F_R)                 ::computeFORCEUpdate (:96-101)
+ *   evolution       (f (dt/dx))(F_L - F_R)                 EulerSolvers::computeFractionalEvolutionVector;
+ *                   f = 0.5 is computeEvolutionVector     (Solvers/EulerSolvers.cpp:163-172)
+ *   slope           SlopeLimiters::computeSlopeVector on conserved vectors (Solvers/SlopeLimiters.cpp:71-101) */
+int cosmos_vector_lax_friedrichs(const double* left_cons, const double* right_cons, const double* left_flux,
+                                 const double* right_flux, long n, double cell_spacing, double time_step,
+                                 double* out);
+int cosmos_vector_richtmyer(const double* left_cons, const double* right_cons, const double* left_flux,
+                            const double* right_flux, long n, double cell_spacing, double time_step, double* out);
+int cosmos_vector_force(const double* lax_friedrichs, const double* richtmyer, long n, double* out);
+int cosmos_vector_force_update(const double* cons, const double* left_flux, const double* right_flux, long n,
+                               double cell_spacing, double time_step, double* out);
+int cosmos_vector_evolution(double step_fraction, const double* left_flux, const double* right_flux, long n,
+                            double cell_spacing, double time_step, double* out);
+int cosmos_vector_slope(const double* left, const double* middle, const double* right, long n, double bias,
+                        int slope_limiter, double* out);
+/* SlopeLimiters::computeSlopeLimiter (Solvers/SlopeLimiters.cpp:3-69). */
+int cosmos_point_slope_limiter(double steepness, double bias, int slope_limiter, double* out);
+
+/* State conversions: computeConservedVariableVector, setConservedVariableVector
+ * (the primitives it stores) and the static compute{X,Y}FluxVector(cons)
+ * (Euler/EulerStateVector.cpp:30-117, RelativisticEuler/RelativisticEulerStateVector.cpp:30-309,
+ * BlackHoleEuler/BlackHoleEulerStateVector.cpp:30-357). */
+int cosmos_point_conserved(int physics, const double* prim, double adiabatic_index, const double* spacetime,
+                           const double* position, double* cons);
+int cosmos_point_primitive(int physics, const double* cons, double adiabatic_index, const double* spacetime,
+                           const double* position, double* prim);
+int cosmos_point_flux(int physics, int axis, const double* cons, double adiabatic_index, const double* spacetime,
+                      const double* position, double* flux);
+
+#define COSMOS_FLUX_LAX_FRIEDRICHS 0
+#define COSMOS_FLUX_RICHTMYER 1
+#define COSMOS_FLUX_FORCE 2
+/* compute{X,Y}{LaxFriedrichs,Richtmyer,FORCE}Flux(left, right, ...) of
+ * EulerFirstOrderSolver (Solvers/EulerFirstOrderSolver.cpp:27-94),
+ * RelativisticEulerFirstOrderSolver (Solvers/RelativisticEulerFirstOrderSolver.cpp) and
+ * BlackHoleEulerFirstOrderSolver (Solvers/BlackHoleEulerFirstOrderSolver.cpp:3-99; position =
+ * the interface, the states taken at position -/+ cell_spacing * 0.5 along the axis). */
+int cosmos_point_interface_flux(int physics, int kind, int axis, const double* left, const double* right,
+                                double cell_spacing, double time_step, double adiabatic_index,
+                                const double* spacetime, const double* position, double* out);
+/* compute{X,Y}SLICFlux(leftLeft, left, right, rightRight, ...) of the second-order solvers
+ * (Solvers/EulerSecondOrderSolver.cpp:3-27, Solvers/RelativisticEulerSecondOrderSolver.cpp,
+ * Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-35; position = the interface). */
+int cosmos_point_slic_flux(int physics, int axis, const double* left_left, const double* left, const double* right,
+                           const double* right_right, double cell_spacing, double time_step, double bias,
+                           int slope_limiter, double adiabatic_index, const double* spacetime,
+                           const double* position, double* out);
+/* evolveStateByHalf{X,Y}TimeStep(left, middle, right, ..., side) (Solvers/EulerSolvers.cpp:191-223,
+ * Solvers/RelativisticEulerSolvers.cpp, Solvers/BlackHoleEulerSolvers.cpp:250-307; position = the
+ * middle cell's centre): the primitives of the evolved face state (side 0: left / top,
+ * 1: right / bottom). */
+int cosmos_point_half_step(int physics, int axis, const double* left, const double* middle, const double* right,
+                           double cell_spacing, double time_step, double bias, int slope_limiter, int side,
+                           double adiabatic_index, const double* spacetime, const double* position,
+                           double* prim_out);
+/* BlackHoleEulerSolvers::evolveStateByFractional{X,Y}TimeStep(stepFraction, left, middle, right, ...)
+ * (Solvers/BlackHoleEulerSolvers.cpp:171-217): COSMOS_BLACKHOLE only. */
+int cosmos_point_fractional_step(int axis, double step_fraction, const double* left, const double* middle,
+                                 const double* right, double cell_spacing, double time_step, double bias,
+                                 int slope_limiter, double adiabatic_index, const double* spacetime,
+                                 const double* position, double* prim_out);
+/* SlopeLimiters::computeSlopeVector(left, middle, right state, ...) (Solvers/SlopeLimiters.cpp:103-121;
+ * axis and cell_spacing unused) and computeSlopeVector{X,Y} (:123-147, COSMOS_BLACKHOLE: the states
+ * taken at position -/+ cell_spacing along the axis). */
+int cosmos_point_slope_vector(int physics, int axis, const double* left, const double* middle, const double* right,
+                              double cell_spacing, double bias, int slope_limiter, double adiabatic_index,
+                              const double* spacetime, const double* position, double* out);
+/* BlackHoleEulerForcingSolver::evolveConservedVariableVector (dim 1; top and bottom unused) and
+ * evolveConservedVariableVector2D (dim 2) on conserved vectors (Solvers/BlackHoleEulerForcingSolver.cpp:3-185):
+ * the RK4 source-term update of the middle cell at position. */
+int cosmos_point_forcing(int dim, const double* left, const double* middle, const double* right, const double* top,
+                         const double* bottom, double cell_spacing, double time_step, double bias, int slope_limiter,
+                         double adiabatic_index, const double* spacetime, const double* position,
+                         double* cons_out);
+
 /* Wait for all queued work on the handle's stream. */
 /* One step of the solve loop on caller-owned device memory that allocates
  * nothing: no counterpart in the reference; for callers that cannot allocate

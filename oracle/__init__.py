@@ -51,6 +51,9 @@ _SIGS = {
 
 #: what-codes of gr_point
 GR_P2C, GR_C2P, GR_XFLUX, GR_YFLUX, GR_SOURCE, GR_SOUND, GR_METRIC = range(7)
+# point-level statics (ORC_PT_* of cosmos_oracle.h)
+(PT_P2C, PT_C2P, PT_FLUX, PT_LF, PT_RICHTMYER, PT_FORCE, PT_SLIC, PT_HALF, PT_SLOPE, PT_FRACTIONAL, PT_FORCING,
+ PT_VEC_LF, PT_VEC_RICHTMYER, PT_VEC_FORCE, PT_VEC_UPDATE, PT_VEC_EVOLUTION, PT_VEC_SLOPE, PT_LIMITER) = range(18)
 
 
 def build(ref: bool = True) -> None:
@@ -117,6 +120,36 @@ class CpuPath:
 
     def _f(self, name):
         return getattr(self.lib, self.prefix + name)
+
+    # ---- point-level statics ----
+    def point(self, fn, physics, axis, states, dx=1.0, dt=1.0, bias=0.0, frac=0.0, gamma=1.4, limiter=1, side=0,
+              hole=None, pos=None):
+        """One point-level static (PT_*): states = up to five 5-vectors (primitive
+        states, or conserved / flux vectors where the static takes those)."""
+        s = np.zeros(25)
+        flat = np.ascontiguousarray(states, dtype=np.float64).ravel()
+        s[:flat.size] = flat
+        par = np.array([dx, dt, bias, frac, gamma], dtype=np.float64)
+        st = np.array(hole if hole is not None else (0.0,) * 5, dtype=np.float64)
+        p = np.array(pos if pos is not None else (0.0, 0.0, 0.0), dtype=np.float64)
+        out = np.full(5, np.nan)
+        f = self._f("point")
+        f.restype = None
+        f.argtypes = [_I, _I, _I, _P, _P, _I, _I, _P, _P, _P]
+        f(fn, physics, axis, _dp(s), _dp(par), limiter, side, _dp(st), _dp(p), _dp(out))
+        return out[:1] if fn == PT_LIMITER else out
+
+    def insert_boundary(self, W, physics, size):
+        """insertBoundaryCells{,2D}(cells, size) of the reference build (ref_ only)."""
+        a = np.ascontiguousarray(W, dtype=np.float64)
+        dim, nx, ny = self._dims(a)
+        shape = (nx + 2 * size, 5) if dim == 1 else (ny + 2 * size, nx + 2 * size, 5)
+        out = np.full(shape, np.nan)
+        f = self.lib.ref_insert_boundary
+        f.restype = None
+        f.argtypes = [_I, _I, _I, _P, _I, _I, _P]
+        f(physics, dim, size, _dp(a), nx, ny, _dp(out))
+        return out
 
     @staticmethod
     def _dims(W):

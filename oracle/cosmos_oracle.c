@@ -429,6 +429,31 @@ void orc_force_flux(int physics, int axis, const double* l, const double* r, dou
     force_flux_states(physics, axis, l, r, dt / dx, dx / dt, gamma, out);
 }
 
+/* EulerFirstOrderSolver::compute{X,Y}LaxFriedrichsFlux(states) (Solvers/EulerFirstOrderSolver.cpp:27-37,
+ * 62-72): computeLaxFriedrichsFlux (:3-11) of the states' p2c and instance fluxes. */
+void orc_lf_flux(int physics, int axis, const double* l, const double* r, double dx, double dt, double gamma,
+                 double* out) {
+    double ul[5], ur[5], fl[5], fr[5];
+    p2c(physics, l, ul, gamma);
+    p2c(physics, r, ur, gamma);
+    flux(physics, axis, ul, fl, gamma);
+    flux(physics, axis, ur, fr, gamma);
+    for (int k = 0; k < 5; ++k) out[k] = 0.5 * ((fl[k] + fr[k]) + ((dx / dt) * (ul[k] - ur[k])));
+}
+
+/* EulerFirstOrderSolver::compute{X,Y}RichtmyerFlux(states) (:39-50, 74-85): the cons flux of
+ * computeRichtmyerFlux's intermediate state (:13-19). */
+void orc_richtmyer_flux(int physics, int axis, const double* l, const double* r, double dx, double dt,
+                        double gamma, double* out) {
+    double ul[5], ur[5], fl[5], fr[5], us[5];
+    p2c(physics, l, ul, gamma);
+    p2c(physics, r, ur, gamma);
+    flux(physics, axis, ul, fl, gamma);
+    flux(physics, axis, ur, fr, gamma);
+    for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + ((dt / dx) * (fl[k] - fr[k])));
+    flux(physics, axis, us, out, gamma);
+}
+
 void orc_half_step(int physics, int axis, const double* l, const double* m, const double* r, double dx,
                    double dt, double bias, int limiter, int side, double gamma, double* out) {
     double ul[5], um[5], ur[5], p0[5], p1[5];

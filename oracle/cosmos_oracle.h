@@ -63,6 +63,30 @@ int orc_gr_amr_level2(int order, int dim, const double* prim, int nx, int ny, do
 void orc_gr_point(int what, const double* in, double x, double y, double z, double h, double gamma, double M,
                   double a, double bx, double by, double bz, double* out);
 
+/* Single interface fluxes of the first-order solvers (Solvers/EulerFirstOrderSolver.cpp:27-85). */
+void orc_lf_flux(int physics, int axis, const double* l, const double* r, double dx, double dt, double gamma,
+                 double* out);
+void orc_richtmyer_flux(int physics, int axis, const double* l, const double* r, double dx, double dt,
+                        double gamma, double* out);
+
+/* Point-level statics of the solver classes: s = up to five 5-vectors (primitive
+ * states unless noted), par = {dx, dt, bias, step fraction, gamma}; physics 0, 1
+ * or 2 (black hole: spacetime = {M, a, bx, by, bz}, pos = {x, y, z}).
+ *   P2C, C2P (s0 conserved), FLUX (s0 conserved): the state conversions
+ *   LF, RICHTMYER, FORCE (s0, s1): compute{X,Y}...Flux(left, right) (pos = the interface)
+ *   SLIC (s0..s3), HALF (s0..s2, side; pos = the middle centre), SLOPE (s0..s2)
+ *   FRACTIONAL (s0..s2; black hole), FORCING (s0..s4 conserved L, M, R, T, B; black
+ *   hole, dimension axis + 1)
+ *   VEC_*: the vector arithmetic on s0.. (conserved / flux vectors), LIMITER: phi(s0[0]) */
+enum {
+    ORC_PT_P2C, ORC_PT_C2P, ORC_PT_FLUX, ORC_PT_LF, ORC_PT_RICHTMYER, ORC_PT_FORCE, ORC_PT_SLIC, ORC_PT_HALF,
+    ORC_PT_SLOPE, ORC_PT_FRACTIONAL, ORC_PT_FORCING,
+    ORC_PT_VEC_LF, ORC_PT_VEC_RICHTMYER, ORC_PT_VEC_FORCE, ORC_PT_VEC_UPDATE, ORC_PT_VEC_EVOLUTION,
+    ORC_PT_VEC_SLOPE, ORC_PT_LIMITER
+};
+void orc_point(int fn, int physics, int axis, const double* s, const double* par, int lim, int side,
+               const double* spacetime, const double* pos, double* out);
+
 /* Two-level AMR (EulerAMR::solve{,2D}Level1AMR and the special-relativistic copies):
  * coarse nx x ny, fine 2nx x 2ny, flags as 0/1 bytes; at most max_coarse_steps coarse
  * steps (< 0: none).  Returns the coarse steps taken. */

@@ -477,31 +477,32 @@ static void at(int axis, double x, double y, double off, double* px, double* py)
 
 /* SlopeLimiters::computeSlopeVector{X,Y} (Solvers/SlopeLimiters.cpp:123-147) */
 static void gr_slope(const bh_t* b, int axis, const double* L, const double* M, const double* R, double x, double y,
-                     double h, double bias, int lim, double g, double* s) {
+                     double z, double h, double bias, int lim, double g, double* s) {
     double ul[5], um[5], ur[5], px, py;
     at(axis, x, y, -h, &px, &py);
-    gr_p2c(b, L, px, py, 0.0, g, ul);
-    gr_p2c(b, M, x, y, 0.0, g, um);
+    gr_p2c(b, L, px, py, z, g, ul);
+    gr_p2c(b, M, x, y, z, g, um);
     at(axis, x, y, h, &px, &py);
-    gr_p2c(b, R, px, py, 0.0, g, ur);
+    gr_p2c(b, R, px, py, z, g, ur);
     orc_slope(ul, um, ur, bias, lim, s);
 }
 
 /* evolveStateByHalf{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:219-307): fluxes
  * and the evolved-state c2p at x -/+ h*0.5 */
 static void gr_half_step(const bh_t* b, int axis, const double* L, const double* M, const double* R, double h,
-                         double dt, double bias, int lim, int side, double x, double y, double g, double* out) {
+                         double dt, double bias, int lim, int side, double x, double y, double z, double g,
+                         double* out) {
     double s[5], um[5], xl[5], xr[5], fl[5], fr[5], px, py;
-    gr_slope(b, axis, L, M, R, x, y, h, bias, lim, g, s);
-    gr_p2c(b, M, x, y, 0.0, g, um);
+    gr_slope(b, axis, L, M, R, x, y, z, h, bias, lim, g, s);
+    gr_p2c(b, M, x, y, z, g, um);
     for (int k = 0; k < 5; ++k) {
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
     at(axis, x, y, -(h * 0.5), &px, &py);
-    gr_flux(b, axis, xl, px, py, 0.0, g, fl);
+    gr_flux(b, axis, xl, px, py, z, g, fl);
     at(axis, x, y, h * 0.5, &px, &py);
-    gr_flux(b, axis, xr, px, py, 0.0, g, fr);
+    gr_flux(b, axis, xr, px, py, z, g, fr);
     double f = 0.5 * (dt / h);
     double e[5];
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (f * (fl[k] - fr[k]));
@@ -512,57 +513,68 @@ static void gr_half_step(const bh_t* b, int axis, const double* L, const double*
         px = x;
         py = side == 0 ? y - (h * 0.5) : y + (h * 0.5);
     }
-    gr_c2p(b, e, px, py, 0.0, g, out);
+    gr_c2p(b, e, px, py, z, g, out);
 }
 
 /* evolveStateByFractional{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:159-217):
  * fluxes at x -/+ h, c2p at x */
 static void gr_fractional(const bh_t* b, int axis, double frac, const double* L, const double* M, const double* R,
-                          double h, double dt, double bias, int lim, double x, double y, double g, double* out) {
+                          double h, double dt, double bias, int lim, double x, double y, double z, double g,
+                          double* out) {
     double s[5], um[5], xl[5], xr[5], fl[5], fr[5], px, py, e[5];
-    gr_slope(b, axis, L, M, R, x, y, h, bias, lim, g, s);
-    gr_p2c(b, M, x, y, 0.0, g, um);
+    gr_slope(b, axis, L, M, R, x, y, z, h, bias, lim, g, s);
+    gr_p2c(b, M, x, y, z, g, um);
     for (int k = 0; k < 5; ++k) {
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
     at(axis, x, y, -h, &px, &py);
-    gr_flux(b, axis, xl, px, py, 0.0, g, fl);
+    gr_flux(b, axis, xl, px, py, z, g, fl);
     at(axis, x, y, h, &px, &py);
-    gr_flux(b, axis, xr, px, py, 0.0, g, fr);
+    gr_flux(b, axis, xr, px, py, z, g, fr);
     double f = frac * (dt / h);
     for (int k = 0; k < 5; ++k) e[k] = um[k] + (f * (fl[k] - fr[k]));
-    gr_c2p(b, e, x, y, 0.0, g, out);
+    gr_c2p(b, e, x, y, z, g, out);
 }
 
 /* BlackHoleEulerFirstOrderSolver::compute{X,Y}FORCEFlux (Solvers/BlackHoleEulerFirstOrderSolver.cpp:3-99):
  * states p2c'd at the interface -/+ h*0.5, the Richtmyer state's flux at the interface */
-static void gr_force(const bh_t* b, int axis, const double* L, const double* R, double x, double y, double h, double dt,
-                     double g, double* F) {
+static void gr_force_kind(const bh_t* b, int kind, int axis, const double* L, const double* R, double x, double y,
+                          double z, double h, double dt, double g, double* F) {
     double ul[5], ur[5], fl[5], fr[5], us[5], fs[5], lx, ly, rx, ry;
     at(axis, x, y, -(h * 0.5), &lx, &ly);
     at(axis, x, y, h * 0.5, &rx, &ry);
-    gr_p2c(b, L, lx, ly, 0.0, g, ul);
-    gr_p2c(b, R, rx, ry, 0.0, g, ur);
-    gr_flux(b, axis, ul, lx, ly, 0.0, g, fl);
-    gr_flux(b, axis, ur, rx, ry, 0.0, g, fr);
+    gr_p2c(b, L, lx, ly, z, g, ul);
+    gr_p2c(b, R, rx, ry, z, g, ur);
+    gr_flux(b, axis, ul, lx, ly, z, g, fl);
+    gr_flux(b, axis, ur, rx, ry, z, g, fr);
+    if (kind == 0) { /* computeLaxFriedrichsFlux only */
+        for (int k = 0; k < 5; ++k) F[k] = 0.5 * ((fl[k] + fr[k]) + ((h / dt) * (ul[k] - ur[k])));
+        return;
+    }
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + ((dt / h) * (fl[k] - fr[k])));
-    gr_flux(b, axis, us, x, y, 0.0, g, fs);
+    gr_flux(b, axis, us, x, y, z, g, kind == 1 ? F : fs);
+    if (kind == 1) return; /* the Richtmyer flux */
     for (int k = 0; k < 5; ++k) {
         double lf = 0.5 * ((fl[k] + fr[k]) + ((h / dt) * (ul[k] - ur[k])));
         F[k] = 0.5 * (lf + fs[k]);
     }
 }
 
+static void gr_force(const bh_t* b, int axis, const double* L, const double* R, double x, double y, double z, double h,
+                     double dt, double g, double* F) {
+    gr_force_kind(b, 2, axis, L, R, x, y, z, h, dt, g, F);
+}
+
 /* BlackHoleEulerSecondOrderSolver::compute{X,Y}SLICFlux (Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-35) */
 static void gr_slic(const bh_t* b, int axis, const double* LL, const double* L, const double* R, const double* RR,
-                    double h, double dt, double bias, int lim, double x, double y, double g, double* F) {
+                    double h, double dt, double bias, int lim, double x, double y, double z, double g, double* F) {
     double pr[5], pl[5], px, py;
     at(axis, x, y, -(h * 0.5), &px, &py);
-    gr_half_step(b, axis, LL, L, R, h, dt, bias, lim, 1, px, py, g, pr);
+    gr_half_step(b, axis, LL, L, R, h, dt, bias, lim, 1, px, py, z, g, pr);
     at(axis, x, y, h * 0.5, &px, &py);
-    gr_half_step(b, axis, L, R, RR, h, dt, bias, lim, 0, px, py, g, pl);
-    gr_force(b, axis, pr, pl, x, y, h, dt, g, F);
+    gr_half_step(b, axis, L, R, RR, h, dt, bias, lim, 0, px, py, z, g, pl);
+    gr_force(b, axis, pr, pl, x, y, z, h, dt, g, F);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -628,11 +640,11 @@ int orc_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, dou
                 lx = x; ly = (c - 0.5) * h; rx = x; ry = (c + 0.5) * h;
             }
             if (order == 2) {
-                gr_slic(&b, axis, NB(-2), NB(-1), NB(0), NB(1), h, dt, bias, lim, lx, ly, gamma, Fl);
-                gr_slic(&b, axis, NB(-1), NB(0), NB(1), NB(2), h, dt, bias, lim, rx, ry, gamma, Fr);
+                gr_slic(&b, axis, NB(-2), NB(-1), NB(0), NB(1), h, dt, bias, lim, lx, ly, 0.0, gamma, Fl);
+                gr_slic(&b, axis, NB(-1), NB(0), NB(1), NB(2), h, dt, bias, lim, rx, ry, 0.0, gamma, Fr);
             } else {
-                gr_force(&b, axis, NB(-1), NB(0), lx, ly, h, dt, gamma, Fl);
-                gr_force(&b, axis, NB(0), NB(1), rx, ry, h, dt, gamma, Fr);
+                gr_force(&b, axis, NB(-1), NB(0), lx, ly, 0.0, h, dt, gamma, Fl);
+                gr_force(&b, axis, NB(0), NB(1), rx, ry, 0.0, h, dt, gamma, Fr);
             }
 #undef NB
             for (int k = 0; k < 5; ++k) un[k] = U[k] + ((dt / h) * (Fl[k] - Fr[k]));
@@ -646,39 +658,39 @@ int orc_gr_sweep(int order, int axis, int dim, double* prim, int nx, int ny, dou
 
 /* BlackHoleEulerForcingSolver::evolveConservedVariableVector{,2D} (Solvers/BlackHoleEulerForcingSolver.cpp:3-185) */
 static void gr_rk4(const bh_t* b, int dim, const double* uL, const double* uM, const double* uR, const double* uT,
-                   const double* uB, double x, double y, double h, double dt, double bias, int lim, double g,
-                   double* out) {
+                   const double* uB, double x, double y, double z, double h, double dt, double bias, int lim,
+                   double g, double* out) {
     double WL[5], WR[5], WT[5], WB[5], Wm[5], s1[5], s2[5], s3[5], s4[5], k1[5], k2[5], k3[5], tmp[5], e1[5], e2[5],
         e3[5], um[5];
-    gr_c2p(b, uL, x - h, y, 0.0, g, WL);
-    gr_c2p(b, uR, x + h, y, 0.0, g, WR);
+    gr_c2p(b, uL, x - h, y, z, g, WL);
+    gr_c2p(b, uR, x + h, y, z, g, WR);
     if (dim == 2) {
-        gr_c2p(b, uT, x, y - h, 0.0, g, WT);
-        gr_c2p(b, uB, x, y + h, 0.0, g, WB);
+        gr_c2p(b, uT, x, y - h, z, g, WT);
+        gr_c2p(b, uB, x, y + h, z, g, WB);
     }
     const double fr1[3] = {0.5, 0.5, 1.0};   /* 1D fractions of stages 1..3 */
     const double fx2[3] = {0.25, 0.25, 0.5}; /* 2D x fractions */
     const double fy2[3] = {0.5, 0.5, 1.0};   /* 2D y fractions */
     double* stage_src[3] = {s1, s2, s3};
     double* stage_k[3] = {k1, k2, k3};
-    gr_source(b, uM, x, y, 0.0, h, g, s1);
+    gr_source(b, uM, x, y, z, h, g, s1);
     for (int k = 0; k < 5; ++k) k1[k] = dt * s1[k];
     for (int st = 0; st < 3; ++st) {
         double* kk = stage_k[st];
         /* U + 0.5 k (stages 1, 2) or U + k (stage 3) */
         for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[k]) : uM[k] + kk[k];
-        gr_c2p(b, tmp, x, y, 0.0, g, Wm);
+        gr_c2p(b, tmp, x, y, z, g, Wm);
         if (dim == 1) {
-            gr_fractional(b, 0, fr1[st], WL, Wm, WR, h, dt, bias, lim, x, y, g, e1);
+            gr_fractional(b, 0, fr1[st], WL, Wm, WR, h, dt, bias, lim, x, y, z, g, e1);
         } else {
-            gr_fractional(b, 0, fx2[st], WL, Wm, WR, h, dt, bias, lim, x, y, g, e1);
-            gr_fractional(b, 1, fy2[st], WT, e1, WB, h, dt, bias, lim, x, y, g, e2);
-            gr_fractional(b, 0, fx2[st], WL, e2, WR, h, dt, bias, lim, x, y, g, e3);
+            gr_fractional(b, 0, fx2[st], WL, Wm, WR, h, dt, bias, lim, x, y, z, g, e1);
+            gr_fractional(b, 1, fy2[st], WT, e1, WB, h, dt, bias, lim, x, y, z, g, e2);
+            gr_fractional(b, 0, fx2[st], WL, e2, WR, h, dt, bias, lim, x, y, z, g, e3);
             memcpy(e1, e3, sizeof e1);
         }
-        gr_p2c(b, e1, x, y, 0.0, g, um);
+        gr_p2c(b, e1, x, y, z, g, um);
         double* src = st == 0 ? s2 : (st == 1 ? s3 : s4);
-        gr_source(b, um, x, y, 0.0, h, g, src);
+        gr_source(b, um, x, y, z, h, g, src);
         double* knext = st == 0 ? k2 : (st == 1 ? k3 : NULL);
         if (knext) for (int k = 0; k < 5; ++k) knext[k] = dt * src[k];
     }
@@ -710,7 +722,7 @@ int orc_gr_forcing(int dim, double* prim, int nx, int ny, double h, double dt, d
                 gr_p2c(&b, cell2(prim, nx, ny, i - 1, j, 1), x, (i - 1) * h, 0.0, gamma, uT);
                 gr_p2c(&b, cell2(prim, nx, ny, i + 1, j, 1), x, (i + 1) * h, 0.0, gamma, uB);
             }
-            gr_rk4(&b, dim, uL, uM, uR, uT, uB, x, y, h, dt, bias, lim, gamma, un);
+            gr_rk4(&b, dim, uL, uM, uR, uT, uB, x, y, 0.0, h, dt, bias, lim, gamma, un);
             gr_c2p(&b, un, x, y, 0.0, gamma, out + 5 * ((long)i * nx + j));
         }
     }
@@ -843,6 +855,77 @@ void orc_gr_point(int what, const double* in, double x, double y, double z, doub
             out[43] = excised(&b, x, y, z);
             break;
         }
+        default: break;
+    }
+}
+
+/* Point-level statics of the solver classes, one call each (the numbering of
+ * ORC_PT_* in cosmos_oracle.h).  The Newtonian and special-relativistic ones
+ * come from cosmos_oracle.c; the black-hole ones are the building blocks
+ * above at the given (x, y, z). */
+void orc_point(int fn, int physics, int axis, const double* s, const double* par, int lim, int side,
+               const double* spacetime, const double* pos, double* out) {
+    const double dx = par[0], dt = par[1], bias = par[2], frac = par[3], g = par[4];
+    const double *s0 = s, *s1 = s + 5, *s2 = s + 10, *s3 = s + 15, *s4 = s + 20;
+    if (fn >= ORC_PT_VEC_LF) {
+        switch (fn) {
+            case ORC_PT_VEC_LF: /* computeLaxFriedrichsFlux (Solvers/EulerFirstOrderSolver.cpp:3-11) */
+                for (int k = 0; k < 5; ++k) out[k] = 0.5 * ((s2[k] + s3[k]) + ((dx / dt) * (s0[k] - s1[k])));
+                break;
+            case ORC_PT_VEC_RICHTMYER: /* computeRichtmyerFlux (:13-19) */
+                for (int k = 0; k < 5; ++k) out[k] = 0.5 * ((s0[k] + s1[k]) + ((dt / dx) * (s2[k] - s3[k])));
+                break;
+            case ORC_PT_VEC_FORCE: /* computeFORCEFlux (:21-24) */
+                for (int k = 0; k < 5; ++k) out[k] = 0.5 * (s0[k] + s1[k]);
+                break;
+            case ORC_PT_VEC_UPDATE: /* computeFORCEUpdate (:96-101) */
+                for (int k = 0; k < 5; ++k) out[k] = s0[k] + ((dt / dx) * (s1[k] - s2[k]));
+                break;
+            case ORC_PT_VEC_EVOLUTION: /* computeFractionalEvolutionVector (Solvers/EulerSolvers.cpp:163-167) */
+                for (int k = 0; k < 5; ++k) out[k] = (frac * (dt / dx)) * (s0[k] - s1[k]);
+                break;
+            case ORC_PT_VEC_SLOPE: orc_slope(s0, s1, s2, bias, lim, out); break;
+            case ORC_PT_LIMITER: out[0] = orc_limiter(s0[0], bias, lim); break;
+            default: break;
+        }
+        return;
+    }
+    if (physics != 2) {
+        switch (fn) {
+            case ORC_PT_P2C: orc_p2c(physics, s0, out, g); break;
+            case ORC_PT_C2P: orc_c2p(physics, s0, out, g); break;
+            case ORC_PT_FLUX: orc_flux(physics, axis, s0, out, g); break;
+            case ORC_PT_LF: orc_lf_flux(physics, axis, s0, s1, dx, dt, g, out); break;
+            case ORC_PT_RICHTMYER: orc_richtmyer_flux(physics, axis, s0, s1, dx, dt, g, out); break;
+            case ORC_PT_FORCE: orc_force_flux(physics, axis, s0, s1, dx, dt, g, out); break;
+            case ORC_PT_SLIC: orc_slic_flux(physics, axis, s0, s1, s2, s3, dx, dt, bias, lim, g, out); break;
+            case ORC_PT_HALF: orc_half_step(physics, axis, s0, s1, s2, dx, dt, bias, lim, side, g, out); break;
+            case ORC_PT_SLOPE: { /* computeSlopeVector(states) (Solvers/SlopeLimiters.cpp:103-121) */
+                double ul[5], um[5], ur[5];
+                orc_p2c(physics, s0, ul, g);
+                orc_p2c(physics, s1, um, g);
+                orc_p2c(physics, s2, ur, g);
+                orc_slope(ul, um, ur, bias, lim, out);
+                break;
+            }
+            default: break;
+        }
+        return;
+    }
+    bh_t b = {spacetime[0], spacetime[1], spacetime[2], spacetime[3], spacetime[4]};
+    const double x = pos[0], y = pos[1], z = pos[2];
+    switch (fn) {
+        case ORC_PT_P2C: gr_p2c(&b, s0, x, y, z, g, out); break;
+        case ORC_PT_C2P: gr_c2p(&b, s0, x, y, z, g, out); break;
+        case ORC_PT_FLUX: gr_flux(&b, axis, s0, x, y, z, g, out); break;
+        case ORC_PT_LF: gr_force_kind(&b, 0, axis, s0, s1, x, y, z, dx, dt, g, out); break;
+        case ORC_PT_RICHTMYER: gr_force_kind(&b, 1, axis, s0, s1, x, y, z, dx, dt, g, out); break;
+        case ORC_PT_FORCE: gr_force(&b, axis, s0, s1, x, y, z, dx, dt, g, out); break;
+        case ORC_PT_SLIC: gr_slic(&b, axis, s0, s1, s2, s3, dx, dt, bias, lim, x, y, z, g, out); break;
+        case ORC_PT_HALF: gr_half_step(&b, axis, s0, s1, s2, dx, dt, bias, lim, side, x, y, z, g, out); break;
+        case ORC_PT_SLOPE: gr_slope(&b, axis, s0, s1, s2, x, y, z, dx, bias, lim, g, out); break;
+        case ORC_PT_FRACTIONAL: gr_fractional(&b, axis, frac, s0, s1, s2, dx, dt, bias, lim, x, y, z, g, out); break;
+        case ORC_PT_FORCING: gr_rk4(&b, axis + 1, s0, s1, s2, s3, s4, x, y, z, dx, dt, bias, lim, g, out); break;
         default: break;
     }
 }

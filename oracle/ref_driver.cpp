@@ -688,3 +688,187 @@ extern "C" int ref_gr_amr_level2(int order, const double* prim, int nx, double d
     flags_out(std::get<5>(r), fflags);
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Point-level statics of the solver classes, one call each, numbered as
+// ORC_PT_* in cosmos_oracle.h: s = up to five 5-vectors, par = {dx, dt, bias,
+// step fraction, gamma}, black hole: spacetime = {M, a, bx, by, bz},
+// pos = {x, y, z}.
+// ---------------------------------------------------------------------------
+namespace {
+
+enum {
+    PT_P2C, PT_C2P, PT_FLUX, PT_LF, PT_RICHTMYER, PT_FORCE, PT_SLIC, PT_HALF, PT_SLOPE, PT_FRACTIONAL, PT_FORCING,
+    PT_VEC_LF, PT_VEC_RICHTMYER, PT_VEC_FORCE, PT_VEC_UPDATE, PT_VEC_EVOLUTION, PT_VEC_SLOPE, PT_LIMITER
+};
+
+std::vector<double> vec5(const double* v) { return std::vector<double>(v, v + 5); }
+
+template <class T>
+void point_sr(int fn, int axis, const double* s, const double* par, int lim, int side, double* out) {
+    typedef typename T::State S;
+    EulerMaterialParameters mp(par[4]);
+    const double dx = par[0], dt = par[1], bias = par[2];
+    S a = make_state<S>(s), b = make_state<S>(s + 5), c = make_state<S>(s + 10), d = make_state<S>(s + 15);
+    std::vector<double> r;
+    switch (fn) {
+        case PT_P2C: p2c<S>(s, out, mp); return;
+        case PT_C2P: c2p<S>(s, out, mp); return;
+        case PT_FLUX: flux<S>(axis, s, out, mp); return;
+        case PT_LF:
+            r = axis == 0 ? T::First::computeXLaxFriedrichsFlux(a, b, dx, dt, mp)
+                          : T::First::computeYLaxFriedrichsFlux(a, b, dx, dt, mp);
+            break;
+        case PT_RICHTMYER:
+            r = axis == 0 ? T::First::computeXRichtmyerFlux(a, b, dx, dt, mp)
+                          : T::First::computeYRichtmyerFlux(a, b, dx, dt, mp);
+            break;
+        case PT_FORCE:
+            r = axis == 0 ? T::First::computeXFORCEFlux(a, b, dx, dt, mp) : T::First::computeYFORCEFlux(a, b, dx, dt, mp);
+            break;
+        case PT_SLIC:
+            r = axis == 0 ? T::Second::computeXSLICFlux(a, b, c, d, dx, dt, bias, lim, mp)
+                          : T::Second::computeYSLICFlux(a, b, c, d, dx, dt, bias, lim, mp);
+            break;
+        case PT_HALF: {
+            S e = axis == 0 ? T::Helpers::evolveStateByHalfXTimeStep(a, b, c, dx, dt, bias, lim, side, mp)
+                            : T::Helpers::evolveStateByHalfYTimeStep(a, b, c, dx, dt, bias, lim, side, mp);
+            store_state(e, out);
+            return;
+        }
+        case PT_SLOPE: r = SlopeLimiters::computeSlopeVector(a, b, c, bias, lim, mp); break;
+        default: return;
+    }
+    for (int k = 0; k < 5; ++k) out[k] = r[k];
+}
+
+void point_bh(int fn, int axis, const double* s, const double* par, int lim, int side, const double* st,
+              const double* pos, double* out) {
+    typedef BlackHoleEulerStateVector S;
+    EulerMaterialParameters mp(par[4]);
+    BlackHoleSpacetime bh(st[0], st[1], st[2], st[3], st[4]);
+    const double dx = par[0], dt = par[1], bias = par[2], frac = par[3];
+    const double x = pos[0], y = pos[1], z = pos[2];
+    S a = make_state<S>(s), b = make_state<S>(s + 5), c = make_state<S>(s + 10), d = make_state<S>(s + 15);
+    std::vector<double> r;
+    switch (fn) {
+        case PT_P2C: r = a.computeConservedVariableVector(x, y, z, mp, bh); break;
+        case PT_C2P: {
+            S t;
+            t.setConservedVariableVector(vec5(s), x, y, z, mp, bh);
+            store_state(t, out);
+            return;
+        }
+        case PT_FLUX:
+            r = axis == 0 ? S::computeXFluxVector(vec5(s), x, y, z, mp, bh) : S::computeYFluxVector(vec5(s), x, y, z, mp, bh);
+            break;
+        case PT_LF:
+            r = axis == 0 ? BlackHoleEulerFirstOrderSolver::computeXLaxFriedrichsFlux(a, b, x, y, z, dx, dt, mp, bh)
+                          : BlackHoleEulerFirstOrderSolver::computeYLaxFriedrichsFlux(a, b, x, y, z, dx, dt, mp, bh);
+            break;
+        case PT_RICHTMYER:
+            r = axis == 0 ? BlackHoleEulerFirstOrderSolver::computeXRichtmyerFlux(a, b, x, y, z, dx, dt, mp, bh)
+                          : BlackHoleEulerFirstOrderSolver::computeYRichtmyerFlux(a, b, x, y, z, dx, dt, mp, bh);
+            break;
+        case PT_FORCE:
+            r = axis == 0 ? BlackHoleEulerFirstOrderSolver::computeXFORCEFlux(a, b, x, y, z, dx, dt, mp, bh)
+                          : BlackHoleEulerFirstOrderSolver::computeYFORCEFlux(a, b, x, y, z, dx, dt, mp, bh);
+            break;
+        case PT_SLIC:
+            r = axis == 0
+                    ? BlackHoleEulerSecondOrderSolver::computeXSLICFlux(a, b, c, d, dx, dt, bias, lim, x, y, z, mp, bh)
+                    : BlackHoleEulerSecondOrderSolver::computeYSLICFlux(a, b, c, d, dx, dt, bias, lim, x, y, z, mp, bh);
+            break;
+        case PT_HALF: {
+            S e = axis == 0 ? BlackHoleEulerSolvers::evolveStateByHalfXTimeStep(a, b, c, dx, dt, bias, lim, side, x, y, z,
+                                                                               mp, bh)
+                            : BlackHoleEulerSolvers::evolveStateByHalfYTimeStep(a, b, c, dx, dt, bias, lim, side, x, y, z,
+                                                                               mp, bh);
+            store_state(e, out);
+            return;
+        }
+        case PT_SLOPE:
+            r = axis == 0 ? SlopeLimiters::computeSlopeVectorX(a, b, c, dx, bias, lim, x, y, z, mp, bh)
+                          : SlopeLimiters::computeSlopeVectorY(a, b, c, dx, bias, lim, x, y, z, mp, bh);
+            break;
+        case PT_FRACTIONAL: {
+            S e = axis == 0 ? BlackHoleEulerSolvers::evolveStateByFractionalXTimeStep(frac, a, b, c, dx, dt, bias, lim, x,
+                                                                                     y, z, mp, bh)
+                            : BlackHoleEulerSolvers::evolveStateByFractionalYTimeStep(frac, a, b, c, dx, dt, bias, lim, x,
+                                                                                     y, z, mp, bh);
+            store_state(e, out);
+            return;
+        }
+        case PT_FORCING:
+            r = axis == 0 ? BlackHoleEulerForcingSolver::evolveConservedVariableVector(vec5(s), vec5(s + 5), vec5(s + 10),
+                                                                                      x, y, z, dx, dt, bias, lim, mp, bh)
+                          : BlackHoleEulerForcingSolver::evolveConservedVariableVector2D(
+                                vec5(s), vec5(s + 5), vec5(s + 10), vec5(s + 15), vec5(s + 20), x, y, z, dx, dt, bias,
+                                lim, mp, bh);
+            break;
+        default: return;
+    }
+    for (int k = 0; k < 5; ++k) out[k] = r[k];
+}
+
+}  // namespace
+
+extern "C" void ref_point(int fn, int physics, int axis, const double* s, const double* par, int lim, int side,
+                          const double* spacetime, const double* pos, double* out) {
+    const double dx = par[0], dt = par[1], bias = par[2], frac = par[3];
+    std::vector<double> r;
+    switch (fn) {
+        case PT_VEC_LF:
+            r = EulerFirstOrderSolver::computeLaxFriedrichsFlux(vec5(s), vec5(s + 5), vec5(s + 10), vec5(s + 15), dx, dt);
+            break;
+        case PT_VEC_RICHTMYER:
+            r = EulerFirstOrderSolver::computeRichtmyerFlux(vec5(s), vec5(s + 5), vec5(s + 10), vec5(s + 15), dx, dt);
+            break;
+        case PT_VEC_FORCE: r = EulerFirstOrderSolver::computeFORCEFlux(vec5(s), vec5(s + 5)); break;
+        case PT_VEC_UPDATE: r = EulerFirstOrderSolver::computeFORCEUpdate(vec5(s), vec5(s + 5), vec5(s + 10), dx, dt); break;
+        case PT_VEC_EVOLUTION: r = EulerSolvers::computeFractionalEvolutionVector(frac, vec5(s), vec5(s + 5), dx, dt); break;
+        case PT_VEC_SLOPE: r = SlopeLimiters::computeSlopeVector(vec5(s), vec5(s + 5), vec5(s + 10), bias, lim); break;
+        case PT_LIMITER: out[0] = SlopeLimiters::computeSlopeLimiter(s[0], bias, lim); return;
+        default:
+            if (physics == 0) point_sr<NewtonianTraits>(fn, axis, s, par, lim, side, out);
+            else if (physics == 1) point_sr<RelativisticTraits>(fn, axis, s, par, lim, side, out);
+            else point_bh(fn, axis, s, par, lim, side, spacetime, pos, out);
+            return;
+    }
+    for (int k = 0; k < 5; ++k) out[k] = r[k];
+}
+
+// EulerSolvers::insertBoundaryCells{,2D} (Solvers/EulerSolvers.cpp:3-85) and the
+// relativistic and black-hole copies: the padded grid, (nx + 2 size) x
+// (ny + 2 size) in 2D (its corners are default-constructed states).
+extern "C" void ref_insert_boundary(int physics, int dim, int size, const double* prim, int nx, int ny, double* out) {
+    if (dim == 1) {
+        if (physics == 0) {
+            std::vector<EulerStateVector> c = to_1d<EulerStateVector>(prim, nx);
+            std::vector<EulerStateVector> r = EulerSolvers::insertBoundaryCells(c, size);
+            from_1d(r, out);
+        } else if (physics == 1) {
+            std::vector<RelativisticEulerStateVector> c = to_1d<RelativisticEulerStateVector>(prim, nx);
+            std::vector<RelativisticEulerStateVector> r = RelativisticEulerSolvers::insertBoundaryCells(c, size);
+            from_1d(r, out);
+        } else {
+            std::vector<BlackHoleEulerStateVector> c = to_1d<BlackHoleEulerStateVector>(prim, nx);
+            std::vector<BlackHoleEulerStateVector> r = BlackHoleEulerSolvers::insertBoundaryCells(c, size);
+            from_1d(r, out);
+        }
+        return;
+    }
+    if (physics == 0) {
+        std::vector<std::vector<EulerStateVector> > c = to_2d<EulerStateVector>(prim, nx, ny);
+        std::vector<std::vector<EulerStateVector> > r = EulerSolvers::insertBoundaryCells2D(c, size);
+        from_2d(r, out);
+    } else if (physics == 1) {
+        std::vector<std::vector<RelativisticEulerStateVector> > c = to_2d<RelativisticEulerStateVector>(prim, nx, ny);
+        std::vector<std::vector<RelativisticEulerStateVector> > r = RelativisticEulerSolvers::insertBoundaryCells2D(c, size);
+        from_2d(r, out);
+    } else {
+        std::vector<std::vector<BlackHoleEulerStateVector> > c = to_2d<BlackHoleEulerStateVector>(prim, nx, ny);
+        std::vector<std::vector<BlackHoleEulerStateVector> > r = BlackHoleEulerSolvers::insertBoundaryCells2D(c, size);
+        from_2d(r, out);
+    }
+}

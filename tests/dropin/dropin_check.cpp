@@ -34,13 +34,17 @@
 //  18. BlackHoleEulerAMR::solve2DLevel2AMR on the input of 16 (the completion
 //      of the reference's truncated driver): three levels and their flags
 //      (12x12x5, 24x24x5, 48x48x5, 144, 576, 2304)
+//  19-21. the point-level statics (host-evaluated; also run alone with the
+//      argument "points", which needs no GPU): see point_sections below.
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <string>
 #include <tuple>
 #include <vector>
 
 #include "cosmos_dropin.hpp"
+#include "cosmos_fv.h"
 
 using std::vector;
 
@@ -57,10 +61,181 @@ static void put_cells(FILE* f, vector<S>& c) {
     }
 }
 
+static void put_vec(FILE* f, const vector<double>& v) {
+    for (double x : v) put(f, x);
+}
+
+template <class S>
+static void put_state(FILE* f, S s) {
+    vector<S> one(1, s);
+    put_cells(f, one);
+}
+
+// The inputs of the point sections: five primitive states k = 0..4
+//   (1 + 0.1 k, a (0.2 - 0.05 k), a (-0.1 + 0.03 k), a 0.04 k, 1 + 0.2 k)
+// with velocity scale a, and five vectors k = 0..4 with components
+//   v_k[c] = 0.3 + 0.1 k + 0.05 c, negated for odd c.
+template <class S>
+static vector<S> point_states(double a) {
+    vector<S> out;
+    for (int k = 0; k < 5; ++k)
+        out.push_back(S(1.0 + 0.1 * k, a * (0.2 - 0.05 * k), a * (-0.1 + 0.03 * k), a * (0.04 * k), 1.0 + 0.2 * k));
+    return out;
+}
+
+static vector<vector<double> > point_vectors() {
+    vector<vector<double> > out;
+    for (int k = 0; k < 5; ++k) {
+        vector<double> v(5);
+        for (int c = 0; c < 5; ++c) v[c] = (c % 2 ? -1.0 : 1.0) * (0.3 + 0.1 * k + 0.05 * c);
+        out.push_back(v);
+    }
+    return out;
+}
+
+// 19. Newtonian (gamma 1.4, a = 1; dx 0.01, dt 0.004, bias 0.3, limiter 0):
+//     X LF, X Richtmyer, X FORCE, Y LF, Y Richtmyer, Y FORCE of (s0, s1); X, Y SLIC of s0..s3;
+//     half steps X side 0, X side 1, Y side 0, Y side 1 of s0..s2; slope vector of s0..s2;
+//     on the vectors v: LF(v0, v1, v2, v3), Richtmyer(v0..v3), FORCE(v0, v1), FORCE update
+//     (v0, v1, v2), fractional evolution (0.25, v0, v1), evolution (v0, v1), slope (v0, v1, v2),
+//     half-step state of (xl, xr, e) sides 0, 1 with xl = (1, 0.1, -0.2, 0.05, 2.5),
+//     xr = (1.1, 0.2, -0.1, 0, 2.7), e = 0.01 v0; insertBoundaryCells(s, 2), (s, 1) and
+//     insertBoundaryCells2D of the 3 x 4 grid g[i][j] = s_{(i + j) % 5}, sizes 2 and 1
+//     (24 x 5, 9 x 5, 7 x 5, 7 x 8 x 5, 5 x 6 x 5)
+// 20. Relativistic (gamma 5/3, a = 0.5): the state functions of 19 and the half-step state
+//     of (xl, xr, e); insertBoundaryCells(s, 2)  (15 x 5 + 2 x 5 + 9 x 5)
+// 21. Black hole (gamma 4/3, a = 0.3; hole (0.1, 0.9, 0.5, 0.4, 0), position (1.1, 0.7, 0.05)):
+//     the LF / Richtmyer / FORCE fluxes, SLIC fluxes, half steps and slope vectors (X, Y) as in
+//     19 (slope X, Y); fractional steps X, Y (0.5) of s0..s2; the vector half steps X side 0, 1,
+//     Y side 0, 1 of (xl, xr, e); the fractional-step state of (xl, e); the forcing of the
+//     conserved vectors c_k of s_k at the position (1D: c0, c1, c2; 2D: c0..c4);
+//     insertBoundaryCells2D of g, size 1  (16 x 5 + 2 x 5 + 4 x 5 + 5 + 2 x 5 + 5 x 6 x 5)
+static void point_sections(FILE* f) {
+    const double dx = 0.01, dt = 0.004, bias = 0.3;
+    const int lim = 0;
+    const vector<vector<double> > v = point_vectors();
+    const vector<double> xl = {1.0, 0.1, -0.2, 0.05, 2.5}, xr = {1.1, 0.2, -0.1, 0.0, 2.7};
+    vector<double> e(5);
+    for (int c = 0; c < 5; ++c) e[c] = 0.01 * v[0][c];
+    {
+        EulerMaterialParameters mp(1.4);
+        vector<EulerStateVector> s = point_states<EulerStateVector>(1.0);
+        put_vec(f, EulerFirstOrderSolver::computeXLaxFriedrichsFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, EulerFirstOrderSolver::computeXRichtmyerFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, EulerFirstOrderSolver::computeXFORCEFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, EulerFirstOrderSolver::computeYLaxFriedrichsFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, EulerFirstOrderSolver::computeYRichtmyerFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, EulerFirstOrderSolver::computeYFORCEFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, EulerSecondOrderSolver::computeXSLICFlux(s[0], s[1], s[2], s[3], dx, dt, bias, lim, mp));
+        put_vec(f, EulerSecondOrderSolver::computeYSLICFlux(s[0], s[1], s[2], s[3], dx, dt, bias, lim, mp));
+        put_state(f, EulerSolvers::evolveStateByHalfXTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 0, mp));
+        put_state(f, EulerSolvers::evolveStateByHalfXTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 1, mp));
+        put_state(f, EulerSolvers::evolveStateByHalfYTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 0, mp));
+        put_state(f, EulerSolvers::evolveStateByHalfYTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 1, mp));
+        put_vec(f, SlopeLimiters::computeSlopeVector(s[0], s[1], s[2], bias, lim, mp));
+        put_vec(f, EulerFirstOrderSolver::computeLaxFriedrichsFlux(v[0], v[1], v[2], v[3], dx, dt));
+        put_vec(f, EulerFirstOrderSolver::computeRichtmyerFlux(v[0], v[1], v[2], v[3], dx, dt));
+        put_vec(f, EulerFirstOrderSolver::computeFORCEFlux(v[0], v[1]));
+        put_vec(f, EulerFirstOrderSolver::computeFORCEUpdate(v[0], v[1], v[2], dx, dt));
+        put_vec(f, EulerSolvers::computeFractionalEvolutionVector(0.25, v[0], v[1], dx, dt));
+        put_vec(f, EulerSolvers::computeEvolutionVector(v[0], v[1], dx, dt));
+        put_vec(f, SlopeLimiters::computeSlopeVector(v[0], v[1], v[2], bias, lim));
+        put_state(f, EulerSolvers::evolveStateByHalfTimeStep(xl, xr, e, 0, mp));
+        put_state(f, EulerSolvers::evolveStateByHalfTimeStep(xl, xr, e, 1, mp));
+        vector<EulerStateVector> p2 = EulerSolvers::insertBoundaryCells(s, 2);
+        put_cells(f, p2);
+        vector<EulerStateVector> p1 = EulerSolvers::insertBoundaryCells(s, 1);
+        put_cells(f, p1);
+        vector<vector<EulerStateVector> > g(3, vector<EulerStateVector>(4));
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) g[i][j] = s[(i + j) % 5];
+        for (auto& row : EulerSolvers::insertBoundaryCells2D(g, 2)) put_cells(f, row);
+        for (auto& row : EulerSolvers::insertBoundaryCells2D(g, 1)) put_cells(f, row);
+    }
+    {
+        EulerMaterialParameters mp(5.0 / 3.0);
+        typedef RelativisticEulerStateVector R;
+        vector<R> s = point_states<R>(0.5);
+        put_vec(f, RelativisticEulerFirstOrderSolver::computeXLaxFriedrichsFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, RelativisticEulerFirstOrderSolver::computeXRichtmyerFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, RelativisticEulerFirstOrderSolver::computeXFORCEFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, RelativisticEulerFirstOrderSolver::computeYLaxFriedrichsFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, RelativisticEulerFirstOrderSolver::computeYRichtmyerFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, RelativisticEulerFirstOrderSolver::computeYFORCEFlux(s[0], s[1], dx, dt, mp));
+        put_vec(f, RelativisticEulerSecondOrderSolver::computeXSLICFlux(s[0], s[1], s[2], s[3], dx, dt, bias, lim, mp));
+        put_vec(f, RelativisticEulerSecondOrderSolver::computeYSLICFlux(s[0], s[1], s[2], s[3], dx, dt, bias, lim, mp));
+        put_state(f, RelativisticEulerSolvers::evolveStateByHalfXTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 0, mp));
+        put_state(f, RelativisticEulerSolvers::evolveStateByHalfXTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 1, mp));
+        put_state(f, RelativisticEulerSolvers::evolveStateByHalfYTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 0, mp));
+        put_state(f, RelativisticEulerSolvers::evolveStateByHalfYTimeStep(s[0], s[1], s[2], dx, dt, bias, lim, 1, mp));
+        put_vec(f, SlopeLimiters::computeSlopeVector(s[0], s[1], s[2], bias, lim, mp));
+        put_state(f, RelativisticEulerSolvers::evolveStateByHalfTimeStep(xl, xr, e, 0, mp));
+        put_state(f, RelativisticEulerSolvers::evolveStateByHalfTimeStep(xl, xr, e, 1, mp));
+        vector<R> p2 = RelativisticEulerSolvers::insertBoundaryCells(s, 2);
+        put_cells(f, p2);
+    }
+    {
+        EulerMaterialParameters mp(4.0 / 3.0);
+        BlackHoleSpacetime bh(0.1, 0.9, 0.5, 0.4, 0.0);
+        const double x = 1.1, y = 0.7, z = 0.05;
+        typedef BlackHoleEulerStateVector B;
+        typedef BlackHoleEulerFirstOrderSolver F1;
+        vector<B> s = point_states<B>(0.3);
+        put_vec(f, F1::computeXLaxFriedrichsFlux(s[0], s[1], x, y, z, dx, dt, mp, bh));
+        put_vec(f, F1::computeXRichtmyerFlux(s[0], s[1], x, y, z, dx, dt, mp, bh));
+        put_vec(f, F1::computeXFORCEFlux(s[0], s[1], x, y, z, dx, dt, mp, bh));
+        put_vec(f, F1::computeYLaxFriedrichsFlux(s[0], s[1], x, y, z, dx, dt, mp, bh));
+        put_vec(f, F1::computeYRichtmyerFlux(s[0], s[1], x, y, z, dx, dt, mp, bh));
+        put_vec(f, F1::computeYFORCEFlux(s[0], s[1], x, y, z, dx, dt, mp, bh));
+        put_vec(f, BlackHoleEulerSecondOrderSolver::computeXSLICFlux(s[0], s[1], s[2], s[3], dx, dt, bias, lim, x, y, z,
+                                                                     mp, bh));
+        put_vec(f, BlackHoleEulerSecondOrderSolver::computeYSLICFlux(s[0], s[1], s[2], s[3], dx, dt, bias, lim, x, y, z,
+                                                                     mp, bh));
+        for (int axis = 0; axis < 2; ++axis)
+            for (int side = 0; side < 2; ++side)
+                put_state(f, axis == 0 ? BlackHoleEulerSolvers::evolveStateByHalfXTimeStep(s[0], s[1], s[2], dx, dt, bias,
+                                                                                          lim, side, x, y, z, mp, bh)
+                                       : BlackHoleEulerSolvers::evolveStateByHalfYTimeStep(s[0], s[1], s[2], dx, dt, bias,
+                                                                                          lim, side, x, y, z, mp, bh));
+        put_vec(f, SlopeLimiters::computeSlopeVectorX(s[0], s[1], s[2], dx, bias, lim, x, y, z, mp, bh));
+        put_vec(f, SlopeLimiters::computeSlopeVectorY(s[0], s[1], s[2], dx, bias, lim, x, y, z, mp, bh));
+        put_state(f, BlackHoleEulerSolvers::evolveStateByFractionalXTimeStep(0.5, s[0], s[1], s[2], dx, dt, bias, lim, x,
+                                                                            y, z, mp, bh));
+        put_state(f, BlackHoleEulerSolvers::evolveStateByFractionalYTimeStep(0.5, s[0], s[1], s[2], dx, dt, bias, lim, x,
+                                                                            y, z, mp, bh));
+        for (int side = 0; side < 2; ++side)
+            put_state(f, BlackHoleEulerSolvers::evolveStateByHalfXTimeStep(xl, xr, e, side, dx, x, y, z, mp, bh));
+        for (int side = 0; side < 2; ++side)
+            put_state(f, BlackHoleEulerSolvers::evolveStateByHalfYTimeStep(xl, xr, e, side, dx, x, y, z, mp, bh));
+        put_state(f, BlackHoleEulerSolvers::evolveStateByFractionalTimeStep(xl, e, x, y, z, mp, bh));
+        vector<vector<double> > c(5, vector<double>(5));
+        for (int k = 0; k < 5; ++k) {
+            double w[5] = {s[k].getDensity(), s[k].getXVelocity(), s[k].getYVelocity(), s[k].getZVelocity(),
+                           s[k].getPressure()};
+            const double hole[5] = {0.1, 0.9, 0.5, 0.4, 0.0}, pos[3] = {x, y, z};
+            if (cosmos_point_conserved(COSMOS_BLACKHOLE, w, 4.0 / 3.0, hole, pos, c[k].data()) != COSMOS_OK)
+                throw std::runtime_error("cosmos_point_conserved");
+        }
+        put_vec(f, BlackHoleEulerForcingSolver::evolveConservedVariableVector(c[0], c[1], c[2], x, y, z, dx, dt, bias,
+                                                                              lim, mp, bh));
+        put_vec(f, BlackHoleEulerForcingSolver::evolveConservedVariableVector2D(c[0], c[1], c[2], c[3], c[4], x, y, z,
+                                                                                dx, dt, bias, lim, mp, bh));
+        vector<vector<B> > g(3, vector<B>(4));
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) g[i][j] = s[(i + j) % 5];
+        for (auto& row : BlackHoleEulerSolvers::insertBoundaryCells2D(g, 1)) put_cells(f, row);
+    }
+}
+
 int main(int argc, char** argv) {
     if (argc < 2) return 2;
     FILE* f = fopen(argv[1], "wb");
     if (!f) return 3;
+    if (argc > 2 && std::string(argv[2]) == "points") {
+        point_sections(f);
+        fclose(f);
+        return 0;
+    }
     EulerMaterialParameters mp(1.4);
 
     vector<EulerStateVector> sod;
@@ -235,6 +410,7 @@ int main(int argc, char** argv) {
         for (auto& row : std::get<5>(r4))
             for (bool b : row) put(f, b ? 1.0 : 0.0);
     }
+    point_sections(f);
     fclose(f);
     return 0;
 }

@@ -121,4 +121,7 @@ def test_dropin_classes_match_oracle(orc, tmp_path):
         assert same(take((n, n, 5)), exp[k]), ("2D level", k)
     for k, n in enumerate((12, 24, 48)):
         assert same(take((n, n)), exp[3 + k].astype(float)), ("2D flags", k)
+    # the point-level statics (sections 19-21), after the device work of the same process
+    from test_point import bits, expected_points
+    assert bits(take((1005,)), expected_points(orc))
     assert pos == v.size
