@@ -11,8 +11,10 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cosmos_fv.h"
@@ -601,6 +603,124 @@ static int ensure_stage(cosmos_solver* s) {
     return COSMOS_OK;
 }
 
+// Host transfers.  Page-locked host memory (cosmos_host_alloc, hipHostMalloc)
+// is copied by DMA directly.  Pageable memory goes through two page-locked
+// staging chunks: the CPU copy of one chunk (split over worker threads)
+// overlaps the DMA of the other, instead of the runtime's own pageable path.
+// The chunks belong to the device and are shared by its solvers (one
+// transfer at a time holds them; 64 MiB of page-locked memory per device,
+// allocated on the first pageable transfer and kept until the process ends).
+constexpr size_t kChunkBytes = size_t(32) << 20;
+constexpr int kCopyThreads = 8;
+constexpr int kMaxStageDevices = 64;
+
+struct StagePool {
+    std::mutex mu;
+    char* chunk[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};  // the last DMA of each chunk
+};
+
+StagePool g_stage[kMaxStageDevices];
+
+static bool is_page_locked(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is unknown to the runtime: not an error
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
+static void par_memcpy(void* dst, const void* src, size_t bytes) {
+    if (bytes < (size_t(4) << 20)) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes / kCopyThreads + 4095) & ~size_t(4095);
+    std::thread th[kCopyThreads];
+    int n = 0;
+    for (size_t off = 0; off < bytes && n < kCopyThreads; off += part, ++n) {
+        const size_t len = bytes - off < part ? bytes - off : part;
+        th[n] = std::thread([=] { std::memcpy(static_cast<char*>(dst) + off, static_cast<const char*>(src) + off, len); });
+    }
+    for (int k = 0; k < n; ++k) th[k].join();
+}
+
+// The device's pool, its chunks allocated; called with pool.mu held.
+static int stage_pool(cosmos_solver* s, StagePool** out) {
+    if (s->device < 0 || s->device >= kMaxStageDevices)
+        return fail(s, COSMOS_ERR_DEVICE, "device index beyond the supported 64");
+    StagePool& p = g_stage[s->device];
+    for (int k = 0; k < 2; ++k) {
+        if (!p.chunk[k]) HIPCHK(s, hipHostMalloc((void**)&p.chunk[k], kChunkBytes, hipHostMallocDefault));
+        if (!p.ev[k]) HIPCHK(s, hipEventCreateWithFlags(&p.ev[k], hipEventDisableTiming));
+    }
+    *out = &p;
+    return COSMOS_OK;
+}
+
+// host -> device on the solver's stream, complete on return
+static int h2d(cosmos_solver* s, void* dev, const void* host, size_t bytes) {
+    if (is_page_locked(host)) {
+        HIPCHK(s, hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return COSMOS_OK;
+    }
+    if (s->device < 0 || s->device >= kMaxStageDevices)
+        return fail(s, COSMOS_ERR_DEVICE, "device index beyond the supported 64");
+    std::lock_guard<std::mutex> lock(g_stage[s->device].mu);
+    StagePool* p = nullptr;
+    int rc = stage_pool(s, &p);
+    if (rc) return rc;
+    size_t i = 0;
+    for (size_t off = 0; off < bytes; off += kChunkBytes, ++i) {
+        const int b = (int)(i & 1);
+        const size_t len = bytes - off < kChunkBytes ? bytes - off : kChunkBytes;
+        HIPCHK(s, hipEventSynchronize(p->ev[b]));  // the chunk's previous DMA has drained
+        par_memcpy(p->chunk[b], static_cast<const char*>(host) + off, len);
+        HIPCHK(s, hipMemcpyAsync(static_cast<char*>(dev) + off, p->chunk[b], len, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(s, hipEventRecord(p->ev[b], s->stream));
+    }
+    HIPCHK(s, hipStreamSynchronize(s->stream));  // the chunks are free for the next transfer
+    return COSMOS_OK;
+}
+
+// device -> host, complete on return
+static int d2h(cosmos_solver* s, void* host, const void* dev, size_t bytes) {
+    if (is_page_locked(host)) {
+        HIPCHK(s, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        return COSMOS_OK;
+    }
+    if (s->device < 0 || s->device >= kMaxStageDevices)
+        return fail(s, COSMOS_ERR_DEVICE, "device index beyond the supported 64");
+    std::lock_guard<std::mutex> lock(g_stage[s->device].mu);
+    StagePool* p = nullptr;
+    int rc = stage_pool(s, &p);
+    if (rc) return rc;
+    const size_t n = (bytes + kChunkBytes - 1) / kChunkBytes;
+    auto issue = [&](size_t i) -> int {
+        const int b = (int)(i & 1);
+        const size_t off = i * kChunkBytes;
+        const size_t len = bytes - off < kChunkBytes ? bytes - off : kChunkBytes;
+        HIPCHK(s, hipMemcpyAsync(p->chunk[b], static_cast<const char*>(dev) + off, len, hipMemcpyDeviceToHost,
+                                 s->stream));
+        HIPCHK(s, hipEventRecord(p->ev[b], s->stream));
+        return COSMOS_OK;
+    };
+    for (size_t i = 0; i < n && i < 2; ++i)
+        if ((rc = issue(i))) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        const int b = (int)(i & 1);
+        const size_t off = i * kChunkBytes;
+        const size_t len = bytes - off < kChunkBytes ? bytes - off : kChunkBytes;
+        HIPCHK(s, hipEventSynchronize(p->ev[b]));
+        par_memcpy(static_cast<char*>(host) + off, p->chunk[b], len);
+        if (i + 2 < n && (rc = issue(i + 2))) return rc;
+    }
+    return COSMOS_OK;
+}
+
 static int upload_impl(cosmos_solver* s, const double* prim, hipMemcpyKind kind) {
     if (!prim) return fail(s, COSMOS_ERR_ARGUMENT, "null state pointer");
     int rc = settle(s);
@@ -608,7 +728,11 @@ static int upload_impl(cosmos_solver* s, const double* prim, hipMemcpyKind kind)
     rc = ensure_stage(s);
     if (rc) return rc;
     long cells = (long)s->nx * s->ny;
-    HIPCHK(s, hipMemcpyAsync(s->stage, prim, sizeof(double) * 5 * cells, kind, s->stream));
+    if (kind == hipMemcpyHostToDevice) {
+        if ((rc = h2d(s, s->stage, prim, sizeof(double) * 5 * cells))) return rc;
+    } else {
+        HIPCHK(s, hipMemcpyAsync(s->stage, prim, sizeof(double) * 5 * cells, kind, s->stream));
+    }
     hipLaunchKernelGGL(k_aos_to_soa, grid_for(cells), dim3(256), 0, s->stream, s->stage, planes_of(s, s->cur), s->nx,
                        s->ny, s->pitch);
     HIPCHK(s, hipGetLastError());
@@ -639,9 +763,21 @@ static int download_impl(cosmos_solver* s, double* prim, hipMemcpyKind kind) {
     hipLaunchKernelGGL(k_soa_to_aos, grid_for(cells), dim3(256), 0, s->stream, cplanes_of(s, s->cur), s->stage,
                        s->nx, s->ny, s->pitch);
     HIPCHK(s, hipGetLastError());
+    if (kind == hipMemcpyDeviceToHost) return d2h(s, prim, s->stage, sizeof(double) * 5 * cells);
     HIPCHK(s, hipMemcpyAsync(prim, s->stage, sizeof(double) * 5 * cells, kind, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
     return COSMOS_OK;
+}
+
+int cosmos_host_alloc(size_t bytes, void** out) {
+    if (!out || bytes == 0) return COSMOS_ERR_ARGUMENT;
+    *out = nullptr;
+    return hipHostMalloc(out, bytes, hipHostMallocDefault) == hipSuccess ? COSMOS_OK : COSMOS_ERR_DEVICE;
+}
+
+int cosmos_host_free(void* p) {
+    if (!p) return COSMOS_OK;
+    return hipHostFree(p) == hipSuccess ? COSMOS_OK : COSMOS_ERR_DEVICE;
 }
 
 int cosmos_upload(cosmos_solver* s, const double* prim) {

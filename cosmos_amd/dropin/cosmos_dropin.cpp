@@ -8,6 +8,7 @@
 #include "cosmos_dropin.hpp"
 
 #include <cmath>
+#include <cstring>
 #include <iostream>
 #include <limits>
 #include <stdexcept>
@@ -17,12 +18,20 @@
 
 namespace {
 
+// The thread's cached solver handle and a page-locked AoS copy of the state
+// its device buffer holds.  `held` marks that copy valid: a call whose grid is
+// bitwise the copy (a caller passing back the grid it received, the usual
+// pattern of per-sweep loops) skips the upload; any call that changes the
+// device state clears `held` until its result is downloaded into the copy.
 struct CachedHandle {
     cosmos_solver* s = nullptr;
     int physics = -1, dim = 0, nx = 0, ny = 0;
     double gamma = 0.0;
+    double* host = nullptr;
+    bool held = false;
     ~CachedHandle() {
         if (s) cosmos_destroy(s);
+        if (host) cosmos_host_free(host);
     }
 };
 
@@ -36,8 +45,15 @@ cosmos_solver* handle(int physics, int dim, int nx, int ny, double gamma) {
     CachedHandle& h = g_handle;
     if (h.s && h.physics == physics && h.dim == dim && h.nx == nx && h.ny == ny && h.gamma == gamma) return h.s;
     if (h.s) cosmos_destroy(h.s);
+    if (h.host) cosmos_host_free(h.host);
     h.s = nullptr;
+    h.host = nullptr;
+    h.held = false;
     check(cosmos_create(&h.s, physics, dim, nx, ny, gamma), nullptr);
+    void* p = nullptr;
+    if (cosmos_host_alloc(sizeof(double) * 5 * (size_t)nx * ny, &p) != COSMOS_OK)
+        throw std::runtime_error("cosmos: page-locked host allocation failed");
+    h.host = static_cast<double*>(p);
     h.physics = physics;
     h.dim = dim;
     h.nx = nx;
@@ -68,83 +84,107 @@ void unpack(const double* in, vector<S>& cells) {
     }
 }
 
+// Are the cells bitwise the AoS copy h?
 template <class S>
-cosmos_solver* load_1d(int physics, vector<S>& cells, double gamma, vector<double>& buf) {
+bool holds(vector<S>& cells, const double* h) {
+    for (size_t j = 0; j < cells.size(); ++j) {
+        const double w[5] = {cells[j].getDensity(), cells[j].getXVelocity(), cells[j].getYVelocity(),
+                             cells[j].getZVelocity(), cells[j].getPressure()};
+        if (std::memcmp(w, h + 5 * j, sizeof w) != 0) return false;
+    }
+    return true;
+}
+
+// The device holds `cells` on return; modifies: the caller is about to change
+// the device state (so the host copy stops describing it).
+template <class S>
+cosmos_solver* load_1d(int physics, vector<S>& cells, double gamma, bool modifies = true) {
     int nx = (int)cells.size();
-    buf.resize(5 * (size_t)nx);
-    pack(cells, buf.data());
     cosmos_solver* s = handle(physics, 1, nx, 1, gamma);
-    check(cosmos_upload(s, buf.data()), s);
+    CachedHandle& h = g_handle;
+    if (!(h.held && holds(cells, h.host))) {
+        h.held = false;
+        pack(cells, h.host);
+        check(cosmos_upload(s, h.host), s);
+        h.held = true;
+    }
+    if (modifies) h.held = false;
     return s;
 }
 
 template <class S>
-cosmos_solver* load_2d(int physics, vector<vector<S> >& cells, double gamma, vector<double>& buf) {
+cosmos_solver* load_2d(int physics, vector<vector<S> >& cells, double gamma, bool modifies = true) {
     int ny = (int)cells.size();
     int nx = ny ? (int)cells[0].size() : 0;
     for (auto& row : cells)
         if ((int)row.size() != nx) throw std::invalid_argument("cosmos: ragged grid rows");
-    buf.resize(5 * (size_t)nx * ny);
-    for (int i = 0; i < ny; ++i) pack(cells[i], buf.data() + 5 * (size_t)i * nx);
     cosmos_solver* s = handle(physics, 2, nx, ny, gamma);
-    check(cosmos_upload(s, buf.data()), s);
+    CachedHandle& h = g_handle;
+    bool same = h.held;
+    for (int i = 0; same && i < ny; ++i) same = holds(cells[i], h.host + 5 * (size_t)i * nx);
+    if (!same) {
+        h.held = false;
+        for (int i = 0; i < ny; ++i) pack(cells[i], h.host + 5 * (size_t)i * nx);
+        check(cosmos_upload(s, h.host), s);
+        h.held = true;
+    }
+    if (modifies) h.held = false;
     return s;
 }
 
 template <class S>
-void store_1d(cosmos_solver* s, vector<S>& cells, vector<double>& buf) {
-    check(cosmos_download(s, buf.data()), s);
-    unpack(buf.data(), cells);
+void store_1d(cosmos_solver* s, vector<S>& cells) {
+    CachedHandle& h = g_handle;
+    check(cosmos_download(s, h.host), s);
+    h.held = true;
+    unpack(h.host, cells);
 }
 
 template <class S>
-void store_2d(cosmos_solver* s, vector<vector<S> >& cells, vector<double>& buf) {
-    check(cosmos_download(s, buf.data()), s);
+void store_2d(cosmos_solver* s, vector<vector<S> >& cells) {
+    CachedHandle& h = g_handle;
+    check(cosmos_download(s, h.host), s);
+    h.held = true;
     size_t nx = cells.empty() ? 0 : cells[0].size();
-    for (size_t i = 0; i < cells.size(); ++i) unpack(buf.data() + 5 * i * nx, cells[i]);
+    for (size_t i = 0; i < cells.size(); ++i) unpack(h.host + 5 * i * nx, cells[i]);
 }
 
 template <class S>
 void sweep_1d(int physics, int order, vector<S>& cells, double dx, double dt, double bias, int lim, double gamma) {
-    vector<double> buf;
-    cosmos_solver* s = load_1d(physics, cells, gamma, buf);
+    cosmos_solver* s = load_1d(physics, cells, gamma);
     check(cosmos_sweep(s, order, 0, dx, dt, bias, lim), s);
-    store_1d(s, cells, buf);
+    store_1d(s, cells);
 }
 
 template <class S>
 void sweep_2d(int physics, int order, int axis, vector<vector<S> >& cells, double dx, double dt, double bias, int lim,
               double gamma) {
-    vector<double> buf;
-    cosmos_solver* s = load_2d(physics, cells, gamma, buf);
+    cosmos_solver* s = load_2d(physics, cells, gamma);
     check(cosmos_sweep(s, order, axis, dx, dt, bias, lim), s);
-    store_2d(s, cells, buf);
+    store_2d(s, cells);
 }
 
 template <class S>
 vector<S> solve_1d(int physics, int order, vector<S> cells, double dx, double cfl, double tf, double bias, int lim,
                    double gamma) {
-    vector<double> buf;
-    cosmos_solver* s = load_1d(physics, cells, gamma, buf);
+    cosmos_solver* s = load_1d(physics, cells, gamma);
     check(cosmos_solve(s, order, dx, cfl, tf, -1, bias, lim, nullptr, nullptr), s);
-    store_1d(s, cells, buf);
+    store_1d(s, cells);
     return cells;
 }
 
 template <class S>
 vector<vector<S> > solve_2d(int physics, int order, vector<vector<S> > cells, double dx, double cfl, double tf,
                             double bias, int lim, double gamma) {
-    vector<double> buf;
-    cosmos_solver* s = load_2d(physics, cells, gamma, buf);
+    cosmos_solver* s = load_2d(physics, cells, gamma);
     check(cosmos_solve(s, order, dx, cfl, tf, -1, bias, lim, nullptr, nullptr), s);
-    store_2d(s, cells, buf);
+    store_2d(s, cells);
     return cells;
 }
 
 template <class S>
 double wave_speed_1d(int physics, vector<S>& cells, double gamma) {
-    vector<double> buf;
-    cosmos_solver* s = load_1d(physics, cells, gamma, buf);
+    cosmos_solver* s = load_1d(physics, cells, gamma, false);
     double v = 0.0;
     check(cosmos_max_wave_speed(s, &v), s);
     return v;
@@ -152,8 +192,7 @@ double wave_speed_1d(int physics, vector<S>& cells, double gamma) {
 
 template <class S>
 double wave_speed_2d(int physics, vector<vector<S> >& cells, double gamma) {
-    vector<double> buf;
-    cosmos_solver* s = load_2d(physics, cells, gamma, buf);
+    cosmos_solver* s = load_2d(physics, cells, gamma, false);
     double v = 0.0;
     check(cosmos_max_wave_speed(s, &v), s);
     return v;
@@ -193,16 +232,14 @@ double EulerSolvers::computeStableTimeStep(double timeStep, double currentTime, 
 }
 double EulerSolvers::computeStableTimeStep(vector<EulerStateVector> c, double dx, double cfl, double t, double tf,
                                            int it, EulerMaterialParameters mp) {
-    vector<double> buf;
-    cosmos_solver* s = load_1d(COSMOS_NEWTONIAN, c, mp.getAdiabaticIndex(), buf);
+    cosmos_solver* s = load_1d(COSMOS_NEWTONIAN, c, mp.getAdiabaticIndex(), false);
     double dt = 0.0;
     check(cosmos_stable_time_step(s, dx, cfl, t, tf, it, &dt), s);
     return dt;
 }
 double EulerSolvers::computeStableTimeStep2D(vector<vector<EulerStateVector> > c, double dx, double cfl, double t,
                                              double tf, int it, EulerMaterialParameters mp) {
-    vector<double> buf;
-    cosmos_solver* s = load_2d(COSMOS_NEWTONIAN, c, mp.getAdiabaticIndex(), buf);
+    cosmos_solver* s = load_2d(COSMOS_NEWTONIAN, c, mp.getAdiabaticIndex(), false);
     double dt = 0.0;
     check(cosmos_stable_time_step(s, dx, cfl, t, tf, it, &dt), s);
     return dt;
@@ -266,8 +303,7 @@ double RelativisticEulerSolvers::computeMaximumWaveSpeed2D(vector<vector<Relativ
 }
 double RelativisticEulerSolvers::computeStableTimeStep(vector<RelativisticEulerStateVector> c, double dx, double cfl,
                                                        double t, double tf, int it, EulerMaterialParameters mp) {
-    vector<double> buf;
-    cosmos_solver* s = load_1d(COSMOS_RELATIVISTIC, c, mp.getAdiabaticIndex(), buf);
+    cosmos_solver* s = load_1d(COSMOS_RELATIVISTIC, c, mp.getAdiabaticIndex(), false);
     double dt = 0.0;
     check(cosmos_stable_time_step(s, dx, cfl, t, tf, it, &dt), s);
     return dt;
@@ -275,8 +311,7 @@ double RelativisticEulerSolvers::computeStableTimeStep(vector<RelativisticEulerS
 double RelativisticEulerSolvers::computeStableTimeStep2D(vector<vector<RelativisticEulerStateVector> > c, double dx,
                                                          double cfl, double t, double tf, int it,
                                                          EulerMaterialParameters mp) {
-    vector<double> buf;
-    cosmos_solver* s = load_2d(COSMOS_RELATIVISTIC, c, mp.getAdiabaticIndex(), buf);
+    cosmos_solver* s = load_2d(COSMOS_RELATIVISTIC, c, mp.getAdiabaticIndex(), false);
     double dt = 0.0;
     check(cosmos_stable_time_step(s, dx, cfl, t, tf, it, &dt), s);
     return dt;
@@ -336,9 +371,8 @@ namespace {
 
 // The cached handle with the caller's black hole set (a host-side parameter).
 template <class Load, class Cells>
-cosmos_solver* load_gr(Load load, Cells& cells, EulerMaterialParameters& mp, BlackHoleSpacetime& bh,
-                       vector<double>& buf) {
-    cosmos_solver* s = load(COSMOS_BLACKHOLE, cells, mp.getAdiabaticIndex(), buf);
+cosmos_solver* load_gr(Load load, Cells& cells, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, bool modifies) {
+    cosmos_solver* s = load(COSMOS_BLACKHOLE, cells, mp.getAdiabaticIndex(), modifies);
     check(cosmos_set_spacetime(s, bh.getBlackHoleMass(), bh.getBlackHoleSpin(), bh.getBlackHoleXPosition(),
                                bh.getBlackHoleYPosition(), bh.getBlackHoleZPosition()),
           s);
@@ -348,27 +382,25 @@ cosmos_solver* load_gr(Load load, Cells& cells, EulerMaterialParameters& mp, Bla
 typedef vector<BlackHoleEulerStateVector> GRLine;
 typedef vector<vector<BlackHoleEulerStateVector> > GRGrid;
 
-cosmos_solver* load_gr_1d(GRLine& c, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, vector<double>& buf) {
-    return load_gr(load_1d<BlackHoleEulerStateVector>, c, mp, bh, buf);
+cosmos_solver* load_gr_1d(GRLine& c, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, bool modifies = true) {
+    return load_gr(load_1d<BlackHoleEulerStateVector>, c, mp, bh, modifies);
 }
-cosmos_solver* load_gr_2d(GRGrid& c, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, vector<double>& buf) {
-    return load_gr(load_2d<BlackHoleEulerStateVector>, c, mp, bh, buf);
+cosmos_solver* load_gr_2d(GRGrid& c, EulerMaterialParameters& mp, BlackHoleSpacetime& bh, bool modifies = true) {
+    return load_gr(load_2d<BlackHoleEulerStateVector>, c, mp, bh, modifies);
 }
 
 void gr_sweep_1d(int order, GRLine& c, double dx, double dt, double bias, int lim, EulerMaterialParameters& mp,
                  BlackHoleSpacetime& bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_1d(c, mp, bh);
     check(cosmos_sweep(s, order, 0, dx, dt, bias, lim), s);
-    store_1d(s, c, buf);
+    store_1d(s, c);
 }
 
 void gr_sweep_2d(int order, int axis, GRGrid& c, double dx, double dt, double bias, int lim,
                  EulerMaterialParameters& mp, BlackHoleSpacetime& bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_2d(c, mp, bh);
     check(cosmos_sweep(s, order, axis, dx, dt, bias, lim), s);
-    store_2d(s, c, buf);
+    store_2d(s, c);
 }
 
 double gr_stable_dt(cosmos_solver* s, double dx, double cfl, double t, double tf, int it) {
@@ -380,43 +412,37 @@ double gr_stable_dt(cosmos_solver* s, double dx, double cfl, double t, double tf
 }  // namespace
 
 double BlackHoleEulerSolvers::computeMaximumWaveSpeed(GRLine& c, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_1d(c, mp, bh, false);
     double v = 0.0;
     check(cosmos_max_wave_speed(s, &v), s);
     return v;
 }
 double BlackHoleEulerSolvers::computeMaximumWaveSpeed2D(GRGrid& c, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_2d(c, mp, bh, false);
     double v = 0.0;
     check(cosmos_max_wave_speed(s, &v), s);
     return v;
 }
 double BlackHoleEulerSolvers::computeStableTimeStep(GRLine c, double dx, double cfl, double t, double tf, int it,
                                                     EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    return gr_stable_dt(load_gr_1d(c, mp, bh, buf), dx, cfl, t, tf, it);
+    return gr_stable_dt(load_gr_1d(c, mp, bh, false), dx, cfl, t, tf, it);
 }
 double BlackHoleEulerSolvers::computeStableTimeStep2D(GRGrid c, double dx, double cfl, double t, double tf, int it,
                                                       EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    return gr_stable_dt(load_gr_2d(c, mp, bh, buf), dx, cfl, t, tf, it);
+    return gr_stable_dt(load_gr_2d(c, mp, bh, false), dx, cfl, t, tf, it);
 }
 
 void BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep(GRLine& c, double dx, double dt, double bias, int lim,
                                                             EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_1d(c, mp, bh);
     check(cosmos_forcing(s, dx, dt, bias, lim), s);
-    store_1d(s, c, buf);
+    store_1d(s, c);
 }
 void BlackHoleEulerForcingSolver::computeRungeKuttaTimeStep2D(GRGrid& c, double dx, double dt, double bias, int lim,
                                                               EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_2d(c, mp, bh);
     check(cosmos_forcing(s, dx, dt, bias, lim), s);
-    store_2d(s, c, buf);
+    store_2d(s, c);
 }
 
 void BlackHoleEulerFirstOrderSolver::computeFORCETimeStep(GRLine& c, double dx, double dt, EulerMaterialParameters mp,
@@ -433,18 +459,16 @@ void BlackHoleEulerFirstOrderSolver::computeYFORCETimeStep2D(GRGrid& c, double d
 }
 GRLine BlackHoleEulerFirstOrderSolver::solve(GRLine c, double dx, double cfl, double tf, int sub,
                                              EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_1d(c, mp, bh);
     check(cosmos_solve_forced(s, 1, dx, cfl, tf, -1, 0.0, 1, sub, nullptr, nullptr), s);
-    store_1d(s, c, buf);
+    store_1d(s, c);
     return c;
 }
 GRGrid BlackHoleEulerFirstOrderSolver::solve2D(GRGrid c, double dx, double cfl, double tf, int sub,
                                                EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_2d(c, mp, bh);
     check(cosmos_solve_forced(s, 1, dx, cfl, tf, -1, 0.0, 1, sub, nullptr, nullptr), s);
-    store_2d(s, c, buf);
+    store_2d(s, c);
     return c;
 }
 
@@ -462,18 +486,16 @@ void BlackHoleEulerSecondOrderSolver::computeYSLICTimeStep2D(GRGrid& c, double d
 }
 GRLine BlackHoleEulerSecondOrderSolver::solve(GRLine c, double dx, double cfl, double tf, double bias, int lim,
                                               int sub, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_1d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_1d(c, mp, bh);
     check(cosmos_solve_forced(s, 2, dx, cfl, tf, -1, bias, lim, sub, nullptr, nullptr), s);
-    store_1d(s, c, buf);
+    store_1d(s, c);
     return c;
 }
 GRGrid BlackHoleEulerSecondOrderSolver::solve2D(GRGrid c, double dx, double cfl, double tf, double bias, int lim,
                                                 int sub, EulerMaterialParameters mp, BlackHoleSpacetime bh) {
-    vector<double> buf;
-    cosmos_solver* s = load_gr_2d(c, mp, bh, buf);
+    cosmos_solver* s = load_gr_2d(c, mp, bh);
     check(cosmos_solve_forced(s, 2, dx, cfl, tf, -1, bias, lim, sub, nullptr, nullptr), s);
-    store_2d(s, c, buf);
+    store_2d(s, c);
     return c;
 }
 

@@ -149,6 +149,8 @@ SIGNATURES = {
     "cosmos_slope_limiter": (ctypes.c_int, [_c_double_p, ctypes.c_long, ctypes.c_double, ctypes.c_int, _c_double_p]),
     "cosmos_slope_vector": (ctypes.c_int, [_c_double_p, _c_double_p, _c_double_p, ctypes.c_long, ctypes.c_double,
                                             ctypes.c_int, _c_double_p]),
+    "cosmos_host_alloc": (ctypes.c_int, [ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "cosmos_host_free": (ctypes.c_int, [ctypes.c_void_p]),
     "cosmos_point_last_error": (ctypes.c_char_p, []),
     "cosmos_vector_lax_friedrichs": (ctypes.c_int, [_c_double_p] * 4 + [ctypes.c_long, ctypes.c_double,
                                                                          ctypes.c_double, _c_double_p]),

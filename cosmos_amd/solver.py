@@ -31,6 +31,42 @@ def _dp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+class _HostBlock:
+    """Page-locked host memory from cosmos_host_alloc, freed with the last array over it."""
+
+    def __init__(self, nbytes: int):
+        self._lib = native.load()
+        p = ctypes.c_void_p()
+        if self._lib.cosmos_host_alloc(nbytes, ctypes.byref(p)) != 0 or not p.value:
+            raise native.CosmosError(f"cosmos_host_alloc({nbytes}) failed")
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._lib.cosmos_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class HostArray(np.ndarray):
+    """A numpy array in page-locked host memory: Solver.upload / download copy it
+    by DMA directly instead of staging it."""
+
+    def __array_finalize__(self, obj):
+        self._block = getattr(obj, "_block", None)
+
+
+def host_empty(shape, dtype=np.float64) -> HostArray:
+    """An uninitialised HostArray of the given shape."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    block = _HostBlock(max(nbytes, 1))
+    raw = (ctypes.c_char * block.nbytes).from_address(block.ptr)
+    a = np.frombuffer(raw, dtype=np.uint8)[:nbytes].view(dtype).reshape(shape).view(HostArray)
+    a._block = block
+    return a
+
+
 class Solver:
     def __init__(self, physics: int, dim: int, nx: int, ny: int = 1, adiabatic_index: float = 1.4):
         self._lib = native.load()
@@ -73,8 +109,13 @@ class Solver:
             raise ValueError(f"state has {a.size} values, grid needs {self.nx * self.ny * 5}")
         self._check(self._lib.cosmos_upload(self._h, _dp(a)))
 
-    def download(self) -> np.ndarray:
-        out = np.empty(self.shape, dtype=np.float64)
+    def download(self, out: np.ndarray | None = None) -> np.ndarray:
+        """The state (into ``out`` when given: a C-contiguous float64 array of the
+        grid's size, e.g. a HostArray, which is copied by DMA directly)."""
+        if out is None:
+            out = np.empty(self.shape, dtype=np.float64)
+        elif out.dtype != np.float64 or not out.flags.c_contiguous or out.size != self.nx * self.ny * 5:
+            raise ValueError("out must be a C-contiguous float64 array of the grid's size")
         self._check(self._lib.cosmos_download(self._h, _dp(out)))
         return out
 

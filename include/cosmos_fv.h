@@ -128,11 +128,18 @@ int cosmos_set_excision(cosmos_solver* s, double radius);
 int cosmos_set_geometry_cache(cosmos_solver* s, int enable);
 
 /* State transfer.  *_device variants take device pointers to the same AoS
- * layout (for callers that already hold the grid in GPU memory). */
+ * layout (for callers that already hold the grid in GPU memory).  Host
+ * arrays in page-locked memory (cosmos_host_alloc) are copied by DMA
+ * directly; pageable arrays are staged through two page-locked 32 MiB chunks
+ * per solver, the CPU copy of one overlapping the DMA of the other. */
 int cosmos_upload(cosmos_solver* s, const double* prim);
 int cosmos_download(cosmos_solver* s, double* prim);
 int cosmos_upload_device(cosmos_solver* s, const double* prim_device);
 int cosmos_download_device(cosmos_solver* s, double* prim_device);
+/* Page-locked host memory (hipHostMalloc) for state arrays; no counterpart in
+ * the reference, whose grids are std::vector.  Free with cosmos_host_free. */
+int cosmos_host_alloc(size_t bytes, void** out);
+int cosmos_host_free(void* p);
 
 /* One dimensionally split sweep in place on the device state.
  *   order 2, dim 2: EulerSecondOrderSolver::compute{X,Y}SLICTimeStep2D

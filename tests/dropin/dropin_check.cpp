@@ -34,7 +34,11 @@
 //  18. BlackHoleEulerAMR::solve2DLevel2AMR on the input of 16 (the completion
 //      of the reference's truncated driver): three levels and their flags
 //      (12x12x5, 24x24x5, 48x48x5, 144, 576, 2304)
-//  19-21. the point-level statics (host-evaluated; also run alone with the
+//  19. per-sweep calls through the drop-in's device-resident copy: on the
+//      input of 3, X SLIC (dt 0.001), the caller's edit of cell (5, 7)
+//      (density x 1.5), X SLIC, the maximum wave speed, Y SLIC, X FORCE on
+//      the grid passed back each time                                      (20 x 24 x 5)
+//  20-22. the point-level statics (host-evaluated; also run alone with the
 //      argument "points", which needs no GPU): see point_sections below.
 #include <cmath>
 #include <cstdio>
@@ -93,7 +97,7 @@ static vector<vector<double> > point_vectors() {
     return out;
 }
 
-// 19. Newtonian (gamma 1.4, a = 1; dx 0.01, dt 0.004, bias 0.3, limiter 0):
+// 20. Newtonian (gamma 1.4, a = 1; dx 0.01, dt 0.004, bias 0.3, limiter 0):
 //     X LF, X Richtmyer, X FORCE, Y LF, Y Richtmyer, Y FORCE of (s0, s1); X, Y SLIC of s0..s3;
 //     half steps X side 0, X side 1, Y side 0, Y side 1 of s0..s2; slope vector of s0..s2;
 //     on the vectors v: LF(v0, v1, v2, v3), Richtmyer(v0..v3), FORCE(v0, v1), FORCE update
@@ -102,11 +106,11 @@ static vector<vector<double> > point_vectors() {
 //     xr = (1.1, 0.2, -0.1, 0, 2.7), e = 0.01 v0; insertBoundaryCells(s, 2), (s, 1) and
 //     insertBoundaryCells2D of the 3 x 4 grid g[i][j] = s_{(i + j) % 5}, sizes 2 and 1
 //     (24 x 5, 9 x 5, 7 x 5, 7 x 8 x 5, 5 x 6 x 5)
-// 20. Relativistic (gamma 5/3, a = 0.5): the state functions of 19 and the half-step state
+// 21. Relativistic (gamma 5/3, a = 0.5): the state functions of 20 and the half-step state
 //     of (xl, xr, e); insertBoundaryCells(s, 2)  (15 x 5 + 2 x 5 + 9 x 5)
-// 21. Black hole (gamma 4/3, a = 0.3; hole (0.1, 0.9, 0.5, 0.4, 0), position (1.1, 0.7, 0.05)):
+// 22. Black hole (gamma 4/3, a = 0.3; hole (0.1, 0.9, 0.5, 0.4, 0), position (1.1, 0.7, 0.05)):
 //     the LF / Richtmyer / FORCE fluxes, SLIC fluxes, half steps and slope vectors (X, Y) as in
-//     19 (slope X, Y); fractional steps X, Y (0.5) of s0..s2; the vector half steps X side 0, 1,
+//     20 (slope X, Y); fractional steps X, Y (0.5) of s0..s2; the vector half steps X side 0, 1,
 //     Y side 0, 1 of (xl, xr, e); the fractional-step state of (xl, e); the forcing of the
 //     conserved vectors c_k of s_k at the position (1D: c0, c1, c2; 2D: c0..c4);
 //     insertBoundaryCells2D of g, size 1  (16 x 5 + 2 x 5 + 4 x 5 + 5 + 2 x 5 + 5 x 6 x 5)
@@ -409,6 +413,16 @@ int main(int argc, char** argv) {
             for (bool b : row) put(f, b ? 1.0 : 0.0);
         for (auto& row : std::get<5>(r4))
             for (bool b : row) put(f, b ? 1.0 : 0.0);
+    }
+    {
+        vector<vector<EulerStateVector> > g4 = grid;
+        EulerSecondOrderSolver::computeXSLICTimeStep2D(g4, 1.0 / 24, 0.001, 0.0, 1, mp);
+        g4[5][7].setDensity(1.5 * g4[5][7].getDensity());
+        EulerSecondOrderSolver::computeXSLICTimeStep2D(g4, 1.0 / 24, 0.001, 0.0, 1, mp);
+        (void)EulerSolvers::computeMaximumWaveSpeed2D(g4, mp);
+        EulerSecondOrderSolver::computeYSLICTimeStep2D(g4, 1.0 / 24, 0.001, 0.0, 1, mp);
+        EulerFirstOrderSolver::computeXFORCETimeStep2D(g4, 1.0 / 24, 0.001, mp);
+        for (auto& row : g4) put_cells(f, row);
     }
     point_sections(f);
     fclose(f);

@@ -121,7 +121,14 @@ def test_dropin_classes_match_oracle(orc, tmp_path):
         assert same(take((n, n, 5)), exp[k]), ("2D level", k)
     for k, n in enumerate((12, 24, 48)):
         assert same(take((n, n)), exp[3 + k].astype(float)), ("2D flags", k)
-    # the point-level statics (sections 19-21), after the device work of the same process
+    # per-sweep calls through the device-resident copy, with an edit by the caller between them
+    e = orc.sweep(W, 0, 2, 0, 1 / 24, 0.001)
+    e[5, 7, 0] = 1.5 * e[5, 7, 0]
+    e = orc.sweep(e, 0, 2, 0, 1 / 24, 0.001)
+    e = orc.sweep(e, 0, 2, 1, 1 / 24, 0.001)
+    e = orc.sweep(e, 0, 1, 0, 1 / 24, 0.001)
+    assert same(take((20, 24, 5)), e)
+    # the point-level statics (sections 20-22), after the device work of the same process
     from test_point import bits, expected_points
     assert bits(take((1005,)), expected_points(orc))
     assert pos == v.size

@@ -260,7 +260,7 @@ def test_point_uniform_identities(orc, physics):
 
 # ---------------------------------------------------------------------------
 # The drop-in classes' point-level statics (tests/dropin/dropin_check.cpp,
-# sections 19-21), through the C++ drop-in library against the oracle
+# sections 20-22), through the C++ drop-in library against the oracle
 # ---------------------------------------------------------------------------
 def point_states(a):
     return np.array([(1.0 + 0.1 * k, a * (0.2 - 0.05 * k), a * (-0.1 + 0.03 * k), a * (0.04 * k), 1.0 + 0.2 * k)
@@ -300,7 +300,7 @@ def padded_2d(g, size):
 
 
 def expected_points(orc, ref=None):
-    """The values sections 19-21 of the drop-in check write, from the oracle
+    """The values sections 20-22 of the drop-in check write, from the oracle
     (and the boundary layouts from the reference build where it is present)."""
     dx, dt, bias, lim = 0.01, 0.004, 0.3, 0
     v = point_vectors()
