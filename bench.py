@@ -358,7 +358,7 @@ def gr_leg(args):
         cnt = (ctypes.c_int * 5)()
         native.check(lib.cosmos_profile_read(s._h, tot, cnt), s._h)
         lib.cosmos_profile_enable(s._h, 0)
-    names = {0: "forcing", 1: "sweep_x", 2: "sweep_y"}
+    names = {0: "forcing", 1: "sweep_x", 2: "sweep_y", 3: "wavespeed_and_step_head"}
     kernels = {names[k]: {"launches": cnt[k], "avg_ms": tot[k] / cnt[k]} for k in names if cnt[k]}
     ms_per_step = ms_total / steps
     v = leg_valu("blackhole", n, args.seed)

@@ -83,6 +83,7 @@ struct cosmos_solver {
     bool run_ready = false;
     int cur_begin = 0;
     long queued = 0;
+    int run_flips = 0;                         // buffer flips per step of a black-hole loop (0: sweeps_per_step)
     bool unsettled = false;
     // optional per-kernel timing of queued steps (HIP events around each launch)
     bool profiling = false;
@@ -258,7 +259,8 @@ int settle(cosmos_solver* s) {
     StepState h{};
     HIPCHK(s, hipMemcpyAsync(&h, s->st, sizeof h, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    s->cur = (int)((s->cur_begin + (long)sweeps_per_step(s) * h.active_steps) & 1);
+    const int flips = s->run_flips ? s->run_flips : sweeps_per_step(s);
+    s->cur = (int)((s->cur_begin + (long)flips * h.active_steps) & 1);
     s->unsettled = false;
     return COSMOS_OK;
 }
@@ -414,6 +416,81 @@ int gr_forcing(cosmos_solver* s, double h, double dt, double bias, int limiter) 
     hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_gr_forcing(L, s->stream); });
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("GR forcing launch: ") + hipGetErrorString(e));
     s->cur ^= 1;
+    return COSMOS_OK;
+}
+
+// computeMaximumWaveSpeed{,2D} skips excised cells at the unit-square
+// positions (j / nx, i / ny); whether that is every cell depends on the
+// spacetime and the grid only, so it is found once per spacetime and marked
+// in status word 6 (a fully excised grid).
+int gr_excised_status(cosmos_solver* s) {
+    const double key[6] = {s->hole.M, s->hole.a, s->hole.x, s->hole.y, s->hole.z, s->hole.r_exc};
+    if (!s->excised_known || std::memcmp(key, s->excised_key, sizeof key) != 0) {
+        const double dr = 1.0 / (double)s->ny, dc = 1.0 / (double)s->nx;
+        bool all = true;
+        for (int i = 0; i < s->ny && all; ++i)
+            for (int j = 0; j < s->nx && all; ++j) all = cfv::excised(s->hole, j * dc, s->dim == 2 ? i * dr : 0.0, 0.0);
+        s->excised_all = all;
+        s->excised_known = 1;
+        std::memcpy(s->excised_key, key, sizeof key);
+    }
+    if (s->excised_all) HIPCHK(s, cfv::launch_fill_u64(s->status + 6, 1, 1ull, s->stream));
+    return COSMOS_OK;
+}
+
+// One step of the device-resident black-hole loop (the solve loops of
+// Solvers/BlackHoleEulerSecondOrderSolver.cpp:145-218 and ..FirstOrderSolver.cpp:183-253):
+// the state's wave speed into the step state's slots, the time step and
+// every coefficient formed on the device (k_step_begin), forcing(dt / 2N) x N
+// with the forcing's bias 0 and van Leer as in the reference, the flux step,
+// forcing(dt / 2N) x N.  Each launch takes its buffers from the step's place
+// in the loop and does nothing once the loop has stopped.
+int queue_gr_step(cosmos_solver* s, int order, double h, double bias, int limiter, int subcycles,
+                  const cfv::GeoTab& tab) {
+    int b = (int)((s->cur_begin + (long)s->run_flips * s->queued) & 1);
+    auto at = [&](int buf) {
+        cfv::GRLaunch L = gr_launch(s, h);
+        L.in = cplanes_of(s, buf);
+        L.out = planes_of(s, buf ^ 1);
+        L.tab = tab;
+        L.step = s->st;
+        return L;
+    };
+    hipError_t e = timed(s, kProfSweepXWave, [&] {
+        hipError_t r = cfv::launch_gr_wavespeed(at(b), s->st_slots, s->stream);
+        return r == hipSuccess ? cfv::launch_step_begin(s->st, s->stream) : r;
+    });
+    auto forcing = [&]() {
+        for (int k = 0; k < subcycles && e == hipSuccess; ++k) {
+            cfv::GRLaunch L = at(b);
+            L.bias = 0.0;
+            L.limiter = 1;
+            e = timed(s, kProfStep, [&] { return cfv::launch_gr_forcing(L, s->stream); });
+            b ^= 1;
+        }
+    };
+    auto sweep = [&](int axis, int full) {
+        if (e != hipSuccess) return;
+        cfv::GRLaunch L = at(b);
+        L.order = order;
+        L.bias = bias;
+        L.limiter = limiter;
+        L.use_full = full;
+        e = timed(s, axis == 0 ? kProfSweepX : kProfSweepY, [&] { return cfv::launch_gr_sweep(L, axis, s->stream); });
+        b ^= 1;
+    };
+    forcing();
+    if (s->dim == 1) {
+        sweep(0, 1);
+    } else {
+        sweep(0, 0);
+        sweep(1, 1);
+        sweep(0, 0);
+    }
+    forcing();
+    if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("black-hole step launch: ") + hipGetErrorString(e));
+    s->queued += 1;
+    s->unsettled = true;
     return COSMOS_OK;
 }
 
@@ -821,23 +898,7 @@ static int max_ws_impl(cosmos_solver* s, unsigned long long* slots_dev) {
     int rc = settle(s);
     if (rc) return rc;
     hipError_t e = cfv::launch_zero_slots(slots_dev, s->stream);
-    if (e == hipSuccess && s->physics == COSMOS_BLACKHOLE) {
-        // computeMaximumWaveSpeed{,2D} skips excised cells at the unit-square
-        // positions (j / nx, i / ny); whether that is every cell depends on the
-        // spacetime and the grid only, so it is found once per spacetime
-        const double key[6] = {s->hole.M, s->hole.a, s->hole.x, s->hole.y, s->hole.z, s->hole.r_exc};
-        if (!s->excised_known || std::memcmp(key, s->excised_key, sizeof key) != 0) {
-            const double dr = 1.0 / (double)s->ny, dc = 1.0 / (double)s->nx;
-            bool all = true;
-            for (int i = 0; i < s->ny && all; ++i)
-                for (int j = 0; j < s->nx && all; ++j)
-                    all = cfv::excised(s->hole, j * dc, s->dim == 2 ? i * dr : 0.0, 0.0);
-            s->excised_all = all;
-            s->excised_known = 1;
-            std::memcpy(s->excised_key, key, sizeof key);
-        }
-        if (s->excised_all) e = cfv::launch_fill_u64(s->status + 6, 1, 1ull, s->stream);
-    }
+    if (e == hipSuccess && s->physics == COSMOS_BLACKHOLE && (rc = gr_excised_status(s))) return rc;
     if (e == hipSuccess) {
         if (s->physics == COSMOS_BLACKHOLE)
             e = cfv::launch_gr_wavespeed(gr_launch(s, 0.0), slots_dev, s->stream);
@@ -920,6 +981,7 @@ int cosmos_begin_at(cosmos_solver* s, int order, double cell_spacing, double cfl
         s->run_ready = true;
         s->cur_begin = s->cur;
         s->queued = 0;
+        s->run_flips = 0;
         HIPCHK(s, hipStreamSynchronize(s->stream));
         return COSMOS_OK;
     });
@@ -997,33 +1059,56 @@ int cosmos_solve_forced_at(cosmos_solver* s, int order, double cell_spacing, dou
         if (rc) return rc;
         if (subcycles < 0) return fail(s, COSMOS_ERR_ARGUMENT, "subcycles must be >= 0");
         if (start_iteration < 0) return fail(s, COSMOS_ERR_ARGUMENT, "start_iteration must be >= 0");
-        double t = start_time;
-        int it = start_iteration;
         const int cap = max_steps < 0 || max_steps > INT_MAX - start_iteration ? INT_MAX : start_iteration + max_steps;
-        while (t < final_time && it < cap) {
-            double dt = 0.0;
-            rc = cosmos_stable_time_step(s, cell_spacing, cfl, t, final_time, it, &dt);
-            if (rc) return rc;
-            if (dt != dt) HIPCHK(s, cfv::launch_fill_u64(s->status + 5, 1, 1ull, s->stream));
-            // forcing(dt / 2N) x N, flux step, forcing(dt / 2N) x N, with the forcing's
-            // bias 0 and van Leer fixed as in the reference
-            for (int k = 0; k < subcycles; ++k)
-                if ((rc = gr_forcing(s, cell_spacing, 0.5 * (dt / subcycles), 0.0, 1))) return rc;
-            if (s->dim == 1) {
-                if ((rc = gr_sweep(s, order, 0, cell_spacing, dt, bias, slope_limiter))) return rc;
-            } else {
-                if ((rc = gr_sweep(s, order, 0, cell_spacing, dt * 0.5, bias, slope_limiter))) return rc;
-                if ((rc = gr_sweep(s, order, 1, cell_spacing, dt, bias, slope_limiter))) return rc;
-                if ((rc = gr_sweep(s, order, 0, cell_spacing, dt * 0.5, bias, slope_limiter))) return rc;
-            }
-            for (int k = 0; k < subcycles; ++k)
-                if ((rc = gr_forcing(s, cell_spacing, 0.5 * (dt / subcycles), 0.0, 1))) return rc;
-            t += dt;
-            it += 1;
+        if (!(start_time < final_time) || start_iteration >= cap) {
+            if (time_out) *time_out = start_time;
+            if (steps_out) *steps_out = 0;
+            return COSMOS_OK;
         }
-        HIPCHK(s, hipStreamSynchronize(s->stream));
-        if (time_out) *time_out = t;
-        if (steps_out) *steps_out = it - start_iteration;
+        if ((rc = settle(s))) return rc;
+        if ((rc = ensure_gr_flux(s))) return rc;
+        cfv::GeoTab tab;
+        if ((rc = ensure_geo(s, cell_spacing, tab))) return rc;
+        // the unit-square wave speed positions of every step are the same: a
+        // fully excised grid is found once (status word 6)
+        if ((rc = gr_excised_status(s))) return rc;
+        StepState h{};
+        h.t = start_time;
+        h.tf = final_time;
+        h.cfl = cfl;
+        h.dx = cell_spacing;
+        h.iter = start_iteration;
+        h.max_steps = cap;
+        h.ws_slots = s->st_slots;
+        h.ws_slot_atomics = 1;
+        h.status = s->status;
+        h.gr = 1;
+        h.gr_subcycles = subcycles;
+        h.gr_dim = s->dim;
+        HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
+        HIPCHK(s, cfv::launch_zero_slots(s->st_slots, s->stream));
+        s->run_ready = false;
+        s->cur_begin = s->cur;
+        s->queued = 0;
+        s->run_flips = 2 * subcycles + sweeps_per_step(s);
+        HIPCHK(s, hipStreamSynchronize(s->stream));  // h is read before it leaves scope
+        // steps are queued in chunks of at most 16 (never past the step cap); the
+        // loop's own stop test (t >= final_time, a NaN time step) turns the rest
+        // of a chunk into launches that return at once
+        long left = (long)cap - start_iteration;
+        StepState now{};
+        while (true) {
+            const long chunk = left < 16 ? left : 16;
+            for (long k = 0; k < chunk; ++k)
+                if ((rc = queue_gr_step(s, order, cell_spacing, bias, slope_limiter, subcycles, tab))) return rc;
+            left -= chunk;
+            HIPCHK(s, hipMemcpyAsync(&now, s->st, sizeof now, hipMemcpyDeviceToHost, s->stream));
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            if (now.done || !(now.t < now.tf) || now.iter >= now.max_steps || left <= 0) break;
+        }
+        if ((rc = settle(s))) return rc;
+        if (time_out) *time_out = now.t;
+        if (steps_out) *steps_out = now.active_steps;
         return COSMOS_OK;
     });
 }

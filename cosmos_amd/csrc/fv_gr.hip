@@ -77,6 +77,8 @@ __device__ __forceinline__ bool stencil_clear(const Hole& b, int dim, int i, int
 // between cells c - 1 and c at (c - 0.5) * h.
 template <int ORDER, int AXIS, int GEO>
 __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
+    if (gr_stopped(L)) return;
+    const SweepCoef cf = gr_step_of(L).cf;
     const int n = AXIS == 0 ? L.nx : L.ny;
     const long lines = AXIS == 0 ? L.ny : L.nx;
     const long per = AXIS == 0 ? (long)(n + 1) : lines;
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
         double WL[5], WR[5];
         load5(L.in, at(c - 1), WL);
         load5(L.in, at(c), WR);
-        gr_force<AXIS, GEO>(WL, WR, x, y, ax, ay, L, F);
+        gr_force<AXIS, GEO>(WL, WR, x, y, ax, ay, L, cf, F);
     } else {
         // computeXSLICFlux (Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-35): the left
         // cell's right-face state evolved about the interface - h/2, the right cell's
@@ -135,12 +137,12 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
         load5(L.in, at(c - 2), W0);
         load5(L.in, at(c - 1), W1);
         load5(L.in, at(c), W2);
-        if (AXIS == 0) gr_half_step<AXIS, GEO>(W0, W1, W2, x - hh, y, ax - 1, ay, 1, L, pr);
-        else gr_half_step<AXIS, GEO>(W0, W1, W2, x, y - hh, ax, ay - 1, 1, L, pr);
+        if (AXIS == 0) gr_half_step<AXIS, GEO>(W0, W1, W2, x - hh, y, ax - 1, ay, 1, L, cf, pr);
+        else gr_half_step<AXIS, GEO>(W0, W1, W2, x, y - hh, ax, ay - 1, 1, L, cf, pr);
         load5(L.in, at(c + 1), W0);
-        if (AXIS == 0) gr_half_step<AXIS, GEO>(W1, W2, W0, x + hh, y, ax + 1, ay, 0, L, pl);
-        else gr_half_step<AXIS, GEO>(W1, W2, W0, x, y + hh, ax, ay + 1, 0, L, pl);
-        gr_force<AXIS, GEO>(pr, pl, x, y, ax, ay, L, F);
+        if (AXIS == 0) gr_half_step<AXIS, GEO>(W1, W2, W0, x + hh, y, ax + 1, ay, 0, L, cf, pl);
+        else gr_half_step<AXIS, GEO>(W1, W2, W0, x, y + hh, ax, ay + 1, 0, L, cf, pl);
+        gr_force<AXIS, GEO>(pr, pl, x, y, ax, ay, L, cf, F);
         if (L.half) {
             const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;
 #pragma unroll
@@ -159,6 +161,8 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
 // the cell centre) where its stencil is clear; a copy elsewhere.
 template <int ORDER, int AXIS, int GEO>
 __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
+    if (gr_stopped(L)) return;
+    const double dt_dx = gr_step_of(L).cf.dt_dx;
     long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
     if (idx >= (long)L.nx * L.ny) return;
     int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
@@ -173,7 +177,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
         long fl = (long)i * L.fpitch + j;
         long fr = AXIS == 0 ? fl + 1 : fl + L.fpitch;
 #pragma unroll
-        for (int k = 0; k < 5; ++k) un[k] = U[k] + (L.coef.dt_dx * (L.flux.p[k][fl] - L.flux.p[k][fr]));
+        for (int k = 0; k < 5; ++k) un[k] = U[k] + (dt_dx * (L.flux.p[k][fl] - L.flux.p[k][fr]));
         gr_c2p(un, g.sdet, L.P, W);
     }
     store5(L.out, cell, W);
@@ -187,6 +191,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_update(GRLaunch L) {
 // inside evolveConservedVariableVector{,2D}, p2c at x -/+ h in each sweep).
 template <int DIM>
 __global__ __launch_bounds__(kGRThreads) void k_gr_roundtrip(GRLaunch L, double* q) {
+    if (gr_stopped(L)) return;
     const int qw = L.nx + 2, qh = DIM == 2 ? L.ny + 2 : 1;
     long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
     if (idx >= (long)qw * qh) return;
@@ -210,6 +215,8 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_roundtrip(GRLaunch L, double*
 // twice, then x 1/2 y 1 x 1/2), U + (k1 + 2 k2 + 2 k3 + k4) / 6.
 template <int DIM, int GEO>
 __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
+    if (gr_stopped(L)) return;
+    const GRStep S = gr_step_of(L);
     long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
     if (idx >= (long)L.nx * L.ny) return;
     int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
@@ -258,24 +265,24 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
         gr_p2c(W, gc, uM, L.P);
         gr_source(uM, sg, L.P, src);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
+        for (int k = 0; k < 5; ++k) kk[0][k] = S.dt * src[k];
         for (int st = 0; st < 3; ++st) {
             double tmp[5], Wm[5], e1[5], um[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
             gr_c2p(tmp, gc.sdet, L.P, Wm);
             if (DIM == 1) {
-                gr_fractional_core<0>(L.frac_dt_h[st], qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e1);
+                gr_fractional_core<0>(S.fx(st), qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e1);
             } else {
                 double e2[5], e3[5];
-                gr_fractional_core<0>(L.frac_dt_h[st], qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e2);
-                gr_fractional_core<1>(L.frac_dt_h_y[st], qn[2], e2, qn[3], gn[2], gc, gn[3], L, e3);
-                gr_fractional_core<0>(L.frac_dt_h[st], qn[0], e3, qn[1], gn[0], gc, gn[1], L, e1);
+                gr_fractional_core<0>(S.fx(st), qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e2);
+                gr_fractional_core<1>(S.fy(st), qn[2], e2, qn[3], gn[2], gc, gn[3], L, e3);
+                gr_fractional_core<0>(S.fx(st), qn[0], e3, qn[1], gn[0], gc, gn[1], L, e1);
             }
             gr_p2c(e1, gc, um, L.P);
             gr_source(um, sg, L.P, src);
 #pragma unroll
-            for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
+            for (int k = 0; k < 5; ++k) kk[st + 1][k] = S.dt * src[k];
         }
         double un[5];
 #pragma unroll
@@ -306,7 +313,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
     if (L.tab.src) source_from_table(L.tab.src + (long)i * L.tab.spitch + j, L.tab.splane, sg);
     else source_geo(b, x, y, 0.0, h, sg);
     double un[5];
-    gr_rk4_cons<DIM, 0>(uL, uM, uR, uT, uB, x, y, gc, sg, L, un);
+    gr_rk4_cons<DIM, 0>(uL, uM, uR, uT, uB, x, y, gc, sg, L, S, un);
     gr_c2p(un, gc.sdet, L.P, W);
     store5(L.out, cell, W);
 }
@@ -349,6 +356,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_src_build(GRLaunch L, double*
 template <int DIM>
 __global__ __launch_bounds__(kGRThreads) void k_gr_wavespeed(GRLaunch L, double dr, double dc,
                                                              unsigned long long* slots) {
+    if (gr_stopped(L)) return;
     double m = 0.0;
     long total = (long)L.nx * L.ny;
     for (long idx = (long)blockIdx.x * kGRThreads + threadIdx.x; idx < total;

@@ -76,7 +76,7 @@ CFV_HD void gr_slope_at(const double* WL, const double* WM, const double* WR, do
 // converted back at that face.
 template <int AXIS, int GEO>
 __host__ __device__ void gr_half_step(const double* WL, const double* WM, const double* WR, double x, double y, int ax,
-                                      int ay, int side, const GRLaunch& L, double* out) {
+                                      int ay, int side, const GRLaunch& L, const SweepCoef& cf, double* out) {
     const double h = L.h;
     double um[5], s[5];
     gr_slope_at<AXIS, GEO>(WL, WM, WR, x, y, ax, ay, L, um, s);
@@ -93,7 +93,7 @@ __host__ __device__ void gr_half_step(const double* WL, const double* WM, const 
     gr_flux<AXIS>(xr, gr, L.P, fr);
     double e[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + evolution_k(L.coef.hdt_dx, fl[k], fr[k]);
+    for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + evolution_k(cf.hdt_dx, fl[k], fr[k]);
     gr_c2p(e, side == 0 ? gl.sdet : gr.sdet, L.P, out);
 }
 
@@ -103,7 +103,7 @@ __host__ __device__ void gr_half_step(const double* WL, const double* WM, const 
 // flux at the interface.
 template <int AXIS, int GEO, int KIND = 2>
 __host__ __device__ void gr_force(const double* WL, const double* WR, double x, double y, int ax, int ay,
-                                  const GRLaunch& L, double* F) {
+                                  const GRLaunch& L, const SweepCoef& cf, double* F) {
     const double hh = L.h * 0.5;
     double ul[5], ur[5], fl[5], fr[5], us[5], fs[5];
     {
@@ -116,18 +116,18 @@ __host__ __device__ void gr_force(const double* WL, const double* WR, double x, 
     }
     if constexpr (KIND == 0) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) F[k] = lax_friedrichs_k(ul[k], ur[k], fl[k], fr[k], L.coef.dx_dt);
+        for (int k = 0; k < 5; ++k) F[k] = lax_friedrichs_k(ul[k], ur[k], fl[k], fr[k], cf.dx_dt);
         return;
     }
 #pragma unroll
-    for (int k = 0; k < 5; ++k) us[k] = richtmyer_k(ul[k], ur[k], fl[k], fr[k], L.coef.dt_dx);
+    for (int k = 0; k < 5; ++k) us[k] = richtmyer_k(ul[k], ur[k], fl[k], fr[k], cf.dt_dx);
     {
         Geo g = geo_pick<GEO>(L, x, y, ax, ay);
         gr_flux<AXIS>(us, g, L.P, KIND == 1 ? F : fs);
     }
     if constexpr (KIND == 2) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) F[k] = force_k(lax_friedrichs_k(ul[k], ur[k], fl[k], fr[k], L.coef.dx_dt), fs[k]);
+        for (int k = 0; k < 5; ++k) F[k] = force_k(lax_friedrichs_k(ul[k], ur[k], fl[k], fr[k], cf.dx_dt), fs[k]);
     }
 }
 
@@ -176,12 +176,13 @@ __host__ __device__ void gr_fractional(double fdt_h, const double* WL, const dou
 // vectors uL, uR (uT, uB) back to primitives at x -/+ h (y -/+ h), then RK4 on
 // the source term with each stage's middle state evolved by fractional sweeps
 // against them (1D: x by 1/2, 1/2, 1; 2D: x 1/4 y 1/2 x 1/4, twice, then
-// x 1/2 y 1 x 1/2: L.frac_dt_h, L.frac_dt_h_y), U + (k1 + 2 k2 + 2 k3 + k4) / 6.
-// uM: the cell's conserved vector at (x, y); gc: the Geo there; sg: its
-// SourceGeo.  un: the evolved conserved vector.
+// x 1/2 y 1 x 1/2: S.fx, S.fy), U + (k1 + 2 k2 + 2 k3 + k4) / 6 with k = S.dt times
+// the source.  uM: the cell's conserved vector at (x, y); gc: the Geo there;
+// sg: its SourceGeo.  un: the evolved conserved vector.
 template <int DIM, int GEO>
 CFV_HD void gr_rk4_cons(const double* uL, const double* uM, const double* uR, const double* uT, const double* uB,
-                        double x, double y, const Geo& gc, const SourceGeo& sg, const GRLaunch& L, double* un) {
+                        double x, double y, const Geo& gc, const SourceGeo& sg, const GRLaunch& L, const GRStep& S,
+                        double* un) {
     const Hole& b = L.hole;
     const double h = L.h;
     const double z = z_of<GEO>(L);
@@ -195,24 +196,24 @@ CFV_HD void gr_rk4_cons(const double* uL, const double* uM, const double* uR, co
     double kk[4][5], src[5];
     gr_source_of(uM, SrcRef{sg}, L.P, src);
 #pragma unroll
-    for (int k = 0; k < 5; ++k) kk[0][k] = L.dt * src[k];
+    for (int k = 0; k < 5; ++k) kk[0][k] = S.dt * src[k];
     for (int st = 0; st < 3; ++st) {
         double tmp[5], Wm[5], e1[5], um[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
         gr_c2p(tmp, gc.sdet, L.P, Wm);
         if (DIM == 1) {
-            gr_fractional<0, GEO>(L.frac_dt_h[st], WLn, Wm, WRn, x, y, L, e1);
+            gr_fractional<0, GEO>(S.fx(st), WLn, Wm, WRn, x, y, L, e1);
         } else {
             double e2[5], e3[5];
-            gr_fractional<0, GEO>(L.frac_dt_h[st], WLn, Wm, WRn, x, y, L, e2);
-            gr_fractional<1, GEO>(L.frac_dt_h_y[st], WTn, e2, WBn, x, y, L, e3);
-            gr_fractional<0, GEO>(L.frac_dt_h[st], WLn, e3, WRn, x, y, L, e1);
+            gr_fractional<0, GEO>(S.fx(st), WLn, Wm, WRn, x, y, L, e2);
+            gr_fractional<1, GEO>(S.fy(st), WTn, e2, WBn, x, y, L, e3);
+            gr_fractional<0, GEO>(S.fx(st), WLn, e3, WRn, x, y, L, e1);
         }
         gr_p2c(e1, gc, um, L.P);
         gr_source_of(um, SrcRef{sg}, L.P, src);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) kk[st + 1][k] = L.dt * src[k];
+        for (int k = 0; k < 5; ++k) kk[st + 1][k] = S.dt * src[k];
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k)

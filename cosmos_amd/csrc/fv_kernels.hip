@@ -416,6 +416,20 @@ __global__ __launch_bounds__(kHeadThreads) void k_step_begin(StepState* st) {
     st->full.dt_dx = dt / st->dx;
     st->full.dx_dt = st->dx / dt;
     st->full.hdt_dx = 0.5 * (dt / st->dx);
+    if (st->gr) {
+        // the forcing of a black-hole step: dt / 2 over its subcycles, and the
+        // stage fractions of evolveConservedVariableVector{,2D} times that over dx,
+        // each formed as the host forms them for a single forcing call
+        const double fdt = 0.5 * (dt / st->gr_subcycles);
+        const double f = fdt / st->dx;
+        st->gr_dt = fdt;
+        st->gr_frac[0] = (st->gr_dim == 1 ? 0.5 : 0.25) * f;
+        st->gr_frac[1] = (st->gr_dim == 1 ? 0.5 : 0.25) * f;
+        st->gr_frac[2] = (st->gr_dim == 1 ? 1.0 : 0.5) * f;
+        st->gr_frac_y[0] = 0.5 * f;
+        st->gr_frac_y[1] = 0.5 * f;
+        st->gr_frac_y[2] = 1.0 * f;
+    }
     st->t = st->t + dt;
     st->iter = st->iter + 1;
     st->active_steps = st->active_steps + 1;

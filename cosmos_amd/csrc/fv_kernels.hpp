@@ -97,6 +97,15 @@ struct StepState {
     unsigned long long* status;  // the solver's status block (null: none); [5] set on a NaN time step
     SweepCoef half;              // the outer x sweeps of a 2D step: 0.5 * dt
     SweepCoef full;              // the y sweep of a 2D step and the 1D sweep: dt
+    // black-hole loops (gr != 0): the forcing's step 0.5 * (dt / subcycles) and its
+    // RK4 stage fractions times that step over dx (Solvers/BlackHoleEulerForcingSolver.cpp)
+    int gr;
+    int gr_subcycles;
+    int gr_dim;
+    int pad;
+    double gr_dt;
+    double gr_frac[3];           // x (1D: 1/2 1/2 1; 2D: 1/4 1/4 1/2)
+    double gr_frac_y[3];         // 2D y: 1/2 1/2 1
 };
 
 struct Planes {
@@ -180,7 +189,51 @@ struct GRLaunch {
     GeoTab tab;                 // geometry cache of this h (tab.geo null: evaluate at the call sites)
     double z;                   // z of the evaluation positions of the point-level functions (the
                                 // sweeps and the forcing work in the z = 0 plane)
+    const StepState* step;      // a device-resident loop: the step's coefficients come from here, and
+                                // every launch returns at once after the loop stopped (null: coef, dt,
+                                // frac_dt_h and frac_dt_h_y above)
+    int use_full;               // with step: the sweep takes step->full (else step->half)
 };
+
+// The time-step coefficients of one black-hole launch: the sweep's and the
+// forcing's (its step and RK4 stage fractions), from the launch record or,
+// in a device-resident loop, from the step state.
+struct GRStep {
+    SweepCoef cf;
+    double dt;
+    double fx0, fx1, fx2;
+    double fy0, fy1, fy2;
+    CFV_HD double fx(int st) const { return st == 0 ? fx0 : (st == 1 ? fx1 : fx2); }
+    CFV_HD double fy(int st) const { return st == 0 ? fy0 : (st == 1 ? fy1 : fy2); }
+};
+
+CFV_HD GRStep gr_step_of(const GRLaunch& L) {
+    GRStep S;
+    if (L.step) {
+        const StepState& t = *L.step;
+        S.cf = L.use_full ? t.full : t.half;
+        S.dt = t.gr_dt;
+        S.fx0 = t.gr_frac[0];
+        S.fx1 = t.gr_frac[1];
+        S.fx2 = t.gr_frac[2];
+        S.fy0 = t.gr_frac_y[0];
+        S.fy1 = t.gr_frac_y[1];
+        S.fy2 = t.gr_frac_y[2];
+    } else {
+        S.cf = L.coef;
+        S.dt = L.dt;
+        S.fx0 = L.frac_dt_h[0];
+        S.fx1 = L.frac_dt_h[1];
+        S.fx2 = L.frac_dt_h[2];
+        S.fy0 = L.frac_dt_h_y[0];
+        S.fy1 = L.frac_dt_h_y[1];
+        S.fy2 = L.frac_dt_h_y[2];
+    }
+    return S;
+}
+
+// A launch of a device-resident loop after the loop stopped does nothing.
+CFV_HD bool gr_stopped(const GRLaunch& L) { return L.step && L.step->done; }
 
 hipError_t launch_gr_sweep(const GRLaunch& L, int axis, hipStream_t s);
 // Fill the geometry cache of L.h, L.hole: geo_at over the half lattice of the

@@ -125,9 +125,9 @@ void half_state(const double* WL, const double* WM, const double* WR, const Swee
 template <int AXIS>
 void gr_interface_flux(int kind, const double* WL, const double* WR, double x, double y, const GRLaunch& L,
                        double* out) {
-    if (kind == COSMOS_FLUX_LAX_FRIEDRICHS) gr_force<AXIS, 2, 0>(WL, WR, x, y, 0, 0, L, out);
-    else if (kind == COSMOS_FLUX_RICHTMYER) gr_force<AXIS, 2, 1>(WL, WR, x, y, 0, 0, L, out);
-    else gr_force<AXIS, 2, 2>(WL, WR, x, y, 0, 0, L, out);
+    if (kind == COSMOS_FLUX_LAX_FRIEDRICHS) gr_force<AXIS, 2, 0>(WL, WR, x, y, 0, 0, L, L.coef, out);
+    else if (kind == COSMOS_FLUX_RICHTMYER) gr_force<AXIS, 2, 1>(WL, WR, x, y, 0, 0, L, L.coef, out);
+    else gr_force<AXIS, 2, 2>(WL, WR, x, y, 0, 0, L, L.coef, out);
 }
 
 // compute{X,Y}SLICFlux (Solvers/BlackHoleEulerSecondOrderSolver.cpp:3-35) at the
@@ -139,13 +139,13 @@ void gr_slic(const double* WLL, const double* WL, const double* WR, const double
     const double hh = L.h * 0.5;
     double pr[5], pl[5];
     if (AXIS == 0) {
-        gr_half_step<0, 2>(WLL, WL, WR, x - hh, y, 0, 0, 1, L, pr);
-        gr_half_step<0, 2>(WL, WR, WRR, x + hh, y, 0, 0, 0, L, pl);
+        gr_half_step<0, 2>(WLL, WL, WR, x - hh, y, 0, 0, 1, L, L.coef, pr);
+        gr_half_step<0, 2>(WL, WR, WRR, x + hh, y, 0, 0, 0, L, L.coef, pl);
     } else {
-        gr_half_step<1, 2>(WLL, WL, WR, x, y - hh, 0, 0, 1, L, pr);
-        gr_half_step<1, 2>(WL, WR, WRR, x, y + hh, 0, 0, 0, L, pl);
+        gr_half_step<1, 2>(WLL, WL, WR, x, y - hh, 0, 0, 1, L, L.coef, pr);
+        gr_half_step<1, 2>(WL, WR, WRR, x, y + hh, 0, 0, 0, L, L.coef, pl);
     }
-    gr_force<AXIS, 2>(pr, pl, x, y, 0, 0, L, out);
+    gr_force<AXIS, 2>(pr, pl, x, y, 0, 0, L, L.coef, out);
 }
 
 }  // namespace
@@ -325,8 +325,8 @@ int cosmos_point_half_step(int physics, int axis, const double* left, const doub
         if (axis != 0 && axis != 1) return bad("axis must be 0 (x) or 1 (y)");
         const GRLaunch L =
             gr_point(adiabatic_index, spacetime, position, cell_spacing, time_step, bias, slope_limiter);
-        if (axis == 0) gr_half_step<0, 2>(left, middle, right, position[0], position[1], 0, 0, side, L, prim_out);
-        else gr_half_step<1, 2>(left, middle, right, position[0], position[1], 0, 0, side, L, prim_out);
+        if (axis == 0) gr_half_step<0, 2>(left, middle, right, position[0], position[1], 0, 0, side, L, L.coef, prim_out);
+        else gr_half_step<1, 2>(left, middle, right, position[0], position[1], 0, 0, side, L, L.coef, prim_out);
         return COSMOS_OK;
     }
     const PhysParams P = make_params(adiabatic_index);
@@ -394,8 +394,9 @@ int cosmos_point_forcing(int dim, const double* left, const double* middle, cons
     const Geo gc = geo_at(L.hole, x, y, z);
     SourceGeo sg;
     source_geo(L.hole, x, y, z, cell_spacing, sg);
-    if (dim == 1) gr_rk4_cons<1, 2>(left, middle, right, nullptr, nullptr, x, y, gc, sg, L, cons_out);
-    else gr_rk4_cons<2, 2>(left, middle, right, top, bottom, x, y, gc, sg, L, cons_out);
+    const GRStep S = gr_step_of(L);
+    if (dim == 1) gr_rk4_cons<1, 2>(left, middle, right, nullptr, nullptr, x, y, gc, sg, L, S, cons_out);
+    else gr_rk4_cons<2, 2>(left, middle, right, top, bottom, x, y, gc, sg, L, S, cons_out);
     return COSMOS_OK;
 }
 

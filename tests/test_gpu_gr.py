@@ -5,6 +5,8 @@ the 1e-9 relative bound).  Covers the sweeps (FORCE and SLIC, both axes, 1D
 and 2D, every limiter) with the excision region inside the grid, the RK4
 forcing step, the unit-square wave speed, and the forced solve loop, for a
 Schwarzschild and a near-extremal Kerr hole."""
+import math
+
 import numpy as np
 import pytest
 
@@ -100,6 +102,31 @@ def test_gr_solve_forced_1d_to_time(orc, spin):
     exp, te, ne = orc.gr_run(W, 2, h, 0.95, hole, tf=0.02, subcycles=1, gamma=G_GR)
     assert same(out, exp), rel_linf(out, exp)
     assert (t, n) == (te, ne) and t == 0.02
+
+
+@pytest.mark.parametrize("tf, max_steps, subcycles", [(0.35, -1, 1), (math.inf, 21, 2), (math.inf, 16, 0)])
+def test_gr_device_loop_across_chunks(orc, tf, max_steps, subcycles):
+    """The device-resident loop queues steps in chunks of 16 and stops on the
+    device (final time, step cap): runs that end inside a chunk, on a chunk
+    boundary and past several chunks, with 0, 1 and 2 forcing subcycles, and
+    (runs stopped by the step cap, whose last step is not clamped to a final
+    time) a resumed run continuing where the first stopped."""
+    W, h, hole = workloads.gr_accretion_1d(48, spin=True)
+    with solver_for(W, hole) as s:
+        t, n = s.solve_forced(2, h, 0.95, tf, max_steps=max_steps, subcycles=subcycles)
+        out = s.download()
+        exp, te, ne = orc.gr_run(W, 2, h, 0.95, hole, tf=tf, max_steps=max_steps if max_steps >= 0 else 2 ** 31 - 1,
+                                 subcycles=subcycles, gamma=G_GR)
+        assert same(out, exp), rel_linf(out, exp)
+        assert (t, n) == (te, ne) and n >= 16
+        if max_steps < 0:
+            return
+        t2, n2 = s.solve_forced(2, h, 0.95, math.inf, max_steps=3, subcycles=subcycles, start_time=t,
+                                start_iteration=n)
+        out2 = s.download()
+    exp2, te2, ne2 = orc.gr_run(W, 2, h, 0.95, hole, max_steps=n + 3, subcycles=subcycles, gamma=G_GR)
+    assert same(out2, exp2), rel_linf(out2, exp2)
+    assert (t2, n + n2) == (te2, ne2)
 
 
 def test_gr_random_field_workload_64(orc):
