@@ -233,7 +233,6 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
     double uM[5];
     SourceGeo sg;
     if constexpr (GEO == 1) {
-        double kk[4][5];
         double src[5];
         // every position-only value and the neighbours' chain read once, at the
         // start, from the tables (tab.geo, tab.src, tab.q); the RK4 chain then
@@ -264,30 +263,26 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
         source_from_table(L.tab.src + (long)i * L.tab.spitch + j, L.tab.splane, sg);
         gr_p2c(W, gc, uM, L.P);
         gr_source(uM, sg, L.P, src);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) kk[0][k] = S.dt * src[k];
+        RK4 rk;
+        rk.start(src, S);
         for (int st = 0; st < 3; ++st) {
             double tmp[5], Wm[5], e1[5], um[5];
-#pragma unroll
-            for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
+            rk.middle(st, uM, tmp);
             gr_c2p(tmp, gc.sdet, L.P, Wm);
             if (DIM == 1) {
-                gr_fractional_core<0>(S.fx(st), qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e1);
+                gr_fractional_core<0>(rk.fx, qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e1);
             } else {
                 double e2[5], e3[5];
-                gr_fractional_core<0>(S.fx(st), qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e2);
-                gr_fractional_core<1>(S.fy(st), qn[2], e2, qn[3], gn[2], gc, gn[3], L, e3);
-                gr_fractional_core<0>(S.fx(st), qn[0], e3, qn[1], gn[0], gc, gn[1], L, e1);
+                gr_fractional_core<0>(rk.fx, qn[0], Wm, qn[1], gn[0], gc, gn[1], L, e2);
+                gr_fractional_core<1>(rk.fy, qn[2], e2, qn[3], gn[2], gc, gn[3], L, e3);
+                gr_fractional_core<0>(rk.fx, qn[0], e3, qn[1], gn[0], gc, gn[1], L, e1);
             }
             gr_p2c(e1, gc, um, L.P);
             gr_source(um, sg, L.P, src);
-#pragma unroll
-            for (int k = 0; k < 5; ++k) kk[st + 1][k] = S.dt * src[k];
+            rk.next(st, src, S);
         }
         double un[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
+        rk.finish(uM, un);
         gr_c2p(un, gc.sdet, L.P, W);
         store5(L.out, cell, W);
         return;

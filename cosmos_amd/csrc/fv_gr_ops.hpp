@@ -171,6 +171,52 @@ __host__ __device__ void gr_fractional(double fdt_h, const double* WL, const dou
     gr_fractional_core<AXIS>(fdt_h, ul, WM, ur, gl, gc, gr, L, out);
 }
 
+// The RK4 bookkeeping of the forcing: k1 .. k4 as named registers and the
+// current stage's k and fractions carried through the stage loop (a
+// dynamically indexed k[4][5] or fraction table would live in scratch memory).
+// Stage st's middle state is U + k/2 (st = 0, 1) or U + k (st = 2) with k the
+// previous stage's; the result is U + (k1 + 2 k2 + 2 k3 + k4) / 6 associated
+// as the reference writes it.
+struct RK4 {
+    double k1[5], k2[5], k3[5], k4[5], kc[5];
+    double fx, fy, fx1, fx2, fy1, fy2;  // the current stage's fractions and the later ones
+    CFV_HD void start(const double* src, const GRStep& S) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            k1[k] = S.dt * src[k];
+            kc[k] = k1[k];
+            k2[k] = k3[k] = k4[k] = 0.0;
+        }
+        fx = S.fx0;
+        fy = S.fy0;
+        fx1 = S.fx1;
+        fx2 = S.fx2;
+        fy1 = S.fy1;
+        fy2 = S.fy2;
+    }
+    CFV_HD void middle(int st, const double* uM, double* tmp) const {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kc[k]) : uM[k] + kc[k];
+    }
+    // value selects on the uniform stage index (no store through a selected
+    // address, which would keep these arrays in memory)
+    CFV_HD void next(int st, const double* src, const GRStep& S) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            kc[k] = S.dt * src[k];
+            k2[k] = st == 0 ? kc[k] : k2[k];
+            k3[k] = st == 1 ? kc[k] : k3[k];
+            k4[k] = st == 2 ? kc[k] : k4[k];
+        }
+        fx = st == 0 ? fx1 : fx2;
+        fy = st == 0 ? fy1 : fy2;
+    }
+    CFV_HD void finish(const double* uM, double* un) const {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) un[k] = uM[k] + ((1.0 / 6.0) * (k1[k] + ((2.0 * k2[k]) + ((2.0 * k3[k]) + k4[k]))));
+    }
+};
+
 // evolveConservedVariableVector{,2D} (Solvers/BlackHoleEulerForcingSolver.cpp:3-185) at
 // (x, y) evaluated at the call sites (GEO 0 or 2): the neighbours' conserved
 // vectors uL, uR (uT, uB) back to primitives at x -/+ h (y -/+ h), then RK4 on
@@ -193,31 +239,27 @@ CFV_HD void gr_rk4_cons(const double* uL, const double* uM, const double* uR, co
         gr_c2p(uT, geo_at(b, x, y - h, z).sdet, L.P, WTn);
         gr_c2p(uB, geo_at(b, x, y + h, z).sdet, L.P, WBn);
     }
-    double kk[4][5], src[5];
+    double src[5];
     gr_source_of(uM, SrcRef{sg}, L.P, src);
-#pragma unroll
-    for (int k = 0; k < 5; ++k) kk[0][k] = S.dt * src[k];
+    RK4 rk;
+    rk.start(src, S);
     for (int st = 0; st < 3; ++st) {
         double tmp[5], Wm[5], e1[5], um[5];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) tmp[k] = st < 2 ? uM[k] + (0.5 * kk[st][k]) : uM[k] + kk[st][k];
+        rk.middle(st, uM, tmp);
         gr_c2p(tmp, gc.sdet, L.P, Wm);
         if (DIM == 1) {
-            gr_fractional<0, GEO>(S.fx(st), WLn, Wm, WRn, x, y, L, e1);
+            gr_fractional<0, GEO>(rk.fx, WLn, Wm, WRn, x, y, L, e1);
         } else {
             double e2[5], e3[5];
-            gr_fractional<0, GEO>(S.fx(st), WLn, Wm, WRn, x, y, L, e2);
-            gr_fractional<1, GEO>(S.fy(st), WTn, e2, WBn, x, y, L, e3);
-            gr_fractional<0, GEO>(S.fx(st), WLn, e3, WRn, x, y, L, e1);
+            gr_fractional<0, GEO>(rk.fx, WLn, Wm, WRn, x, y, L, e2);
+            gr_fractional<1, GEO>(rk.fy, WTn, e2, WBn, x, y, L, e3);
+            gr_fractional<0, GEO>(rk.fx, WLn, e3, WRn, x, y, L, e1);
         }
         gr_p2c(e1, gc, um, L.P);
         gr_source_of(um, SrcRef{sg}, L.P, src);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) kk[st + 1][k] = S.dt * src[k];
+        rk.next(st, src, S);
     }
-#pragma unroll
-    for (int k = 0; k < 5; ++k)
-        un[k] = uM[k] + ((1.0 / 6.0) * (kk[0][k] + ((2.0 * kk[1][k]) + ((2.0 * kk[2][k]) + kk[3][k]))));
+    rk.finish(uM, un);
 }
 
 }  // namespace cfv

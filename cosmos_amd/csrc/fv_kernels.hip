@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const int row = blk / tiles;
     const int tile = blk - row * tiles;
     Ev acc{};
-    Ev* const pacc = (Phys::kPhysics == 1 && L.status) ? &acc : nullptr;
+    Ev* const pacc = Phys::kPhysics == 1 ? &acc : nullptr;  // counted in registers, added only with L.status
     const double m = x_segment<Phys, ORDER, LIMMODE, DIM, WS, W0, NT>(L.in, L.out, nx, (long)row * L.pitch, tile, cf,
                                                                      L.bias, L.limiter, P, lds, pacc);
     if (Phys::kPhysics == 1) status_add(L.status, acc);
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(kYThreads) __attribute__((amdgpu_waves_per_eu(y_min
     const int i0 = by * ROWS;
     const int i1 = min(i0 + ROWS, ny);
     Ev acc{};
-    Ev* const pacc = (Phys::kPhysics == 1 && L.status) ? &acc : nullptr;
+    Ev* const pacc = Phys::kPhysics == 1 ? &acc : nullptr;  // counted in registers, added only with L.status
     const double wsmax = y_march<Phys, ORDER, LIMMODE, W0>(L.in, L.out, ny, L.pitch, j, colok, jj, i0, i1, WS, cf,
                                                           L.bias, L.limiter, P, pacc);
     if (Phys::kPhysics == 1) status_add(L.status, acc);
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
     const int br = blk / tc;
     const int bc = blk - br * tc;
     Ev acc{};
-    Ev* const pacc = (Phys::kPhysics == 1 && L.status) ? &acc : nullptr;
+    Ev* const pacc = Phys::kPhysics == 1 ? &acc : nullptr;  // counted in registers, added only with L.status
     const double m = y_tile_cells<Phys, ORDER, LIMMODE, W0, NT, TC>(L.in, L.out, L.nx, L.ny, L.pitch, bc * TC, br * TRV,
                                                                      WS ? DIM : 0, cf, L.bias, L.limiter, L.P, lds,
                                                                      pacc);

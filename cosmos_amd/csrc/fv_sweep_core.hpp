@@ -202,12 +202,16 @@ template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STR
 __device__ __forceinline__ void line_step(const double* W, double* Wn, const SweepCoef& cf, double bias, int limiter,
                                           const PhysParams& P, double (*lds)[NT], const Own* own = nullptr,
                                           Ev* acc = nullptr) {
+    // special-relativistic solves are counted into registers (the per-stage
+    // counters must not live behind a pointer that may be null: that would put
+    // them in scratch memory); acc, when given, receives the owned ones
+    constexpr bool kCount = Phys::kPhysics == 1;
     Ev eh{}, eg0{}, eg1{}, ei{}, eu{};
-    Ev* const ph = acc ? &eh : nullptr;
-    Ev* const pg0 = acc ? &eg0 : nullptr;
-    Ev* const pg1 = acc ? &eg1 : nullptr;
-    Ev* const pi = acc ? &ei : nullptr;
-    Ev* const pu = acc ? &eu : nullptr;
+    Ev* const ph = kCount ? &eh : nullptr;
+    Ev* const pg0 = kCount ? &eg0 : nullptr;
+    Ev* const pg1 = kCount ? &eg1 : nullptr;
+    Ev* const pi = kCount ? &ei : nullptr;
+    Ev* const pu = kCount ? &eu : nullptr;
     //   SLIC:  sA = rows 0-4   stage A U, then the interface fluxes
     //          sB = rows 5-14  stage B Q0, G0
     //   FORCE: sA = rows 0-9   U and F(U);  sB = rows 10-14 the interface fluxes
@@ -277,7 +281,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
 #pragma unroll
     for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : (ORDER == 2 ? sA[k][sl] : sB[k][sl]);
     update_cell<Phys, W0>(U, Fl, F, cf, P, Wn, pu);
-    if (acc) {
+    if (kCount && acc) {
         // FORCE: the cell's own flux counts where SLIC counts its half step
         ev_add(*acc, eh, own->half);
         ev_add(*acc, eg0, own->g0);
@@ -364,12 +368,16 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
                                           bool colok, int jj, int i0, int i1, bool ws, const SweepCoef& cf,
                                           double bias, int limiter, const PhysParams& P, Ev* acc = nullptr) {
     double wsmax = 0.0;
+    // special-relativistic solves are counted into registers (the per-stage
+    // counters must not live behind a pointer that may be null: that would put
+    // them in scratch memory); acc, when given, receives the owned ones
+    constexpr bool kCount = Phys::kPhysics == 1;
     Ev eh{}, eg0{}, eg1{}, ei{}, eu{};
-    Ev* const ph = acc ? &eh : nullptr;
-    Ev* const pg0 = acc ? &eg0 : nullptr;
-    Ev* const pg1 = acc ? &eg1 : nullptr;
-    Ev* const pi = acc ? &ei : nullptr;
-    Ev* const pu = acc ? &eu : nullptr;
+    Ev* const ph = kCount ? &eh : nullptr;
+    Ev* const pg0 = kCount ? &eg0 : nullptr;
+    Ev* const pg1 = kCount ? &eg1 : nullptr;
+    Ev* const pi = kCount ? &ei : nullptr;
+    Ev* const pu = kCount ? &eu : nullptr;
     const bool first = i0 == 0, last = i1 >= ny;
     auto owned = [&](int q) { return colok && ((q >= i0 && q < i1) || (q == -1 && first) || (q == ny && last)); };
     auto owned_iface = [&](int q) { return colok && ((q >= i0 && q < i1) || (q == -1 && first)); };
@@ -387,7 +395,7 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
             if (r >= i0) {
                 double Q0[5], G0[5], Q1[5], G1[5];
                 half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cf, bias, limiter, P, Q0, G0, Q1, G1, ph, pg0, pg1);
-                if (acc) {
+                if (kCount && acc) {
                     // the half step of row r - 1
                     const bool o = owned(r - 1);
                     ev_add(*acc, eh, o);
@@ -398,7 +406,7 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
                 if (r >= i0 + 1) {
                     double F[5];
                     force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cf, P, F, pi);
-                    if (acc) {
+                    if (kCount && acc) {
                         // interface (r - 2 | r - 1)
                         ev_add(*acc, ei, owned_iface(r - 2));
                         ei = Ev{};
@@ -406,7 +414,7 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
                     if (r >= i0 + 2) {
                         double Wo[5];
                         update_cell<Phys, W0>(Ua, Fp, F, cf, P, Wo, pu);
-                        if (acc) {
+                        if (kCount && acc) {
                             ev_add(*acc, eu, colok);
                             eu = Ev{};
                         }
@@ -442,7 +450,7 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
             if (r < i1) load_cell<W0>(in, (long)clamp1(r + 1, ny) * pitch + jj, Wn);
             Phys::p2c(W, U, P);
             Phys::template flux<1>(U, FU, Phys::inv_density(U), P, ph);
-            if (acc) {
+            if (kCount && acc) {
                 // the flux of row r's conserved vector
                 ev_add(*acc, eh, owned(r));
                 eh = Ev{};
@@ -450,7 +458,7 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
             if (r >= i0) {
                 double F[5];
                 force_flux<Phys, 1, W0>(Up, FUp, U, FU, cf, P, F, pi);
-                if (acc) {
+                if (kCount && acc) {
                     // interface (r - 1 | r)
                     ev_add(*acc, ei, owned_iface(r - 1));
                     ei = Ev{};
@@ -458,7 +466,7 @@ __device__ __forceinline__ double y_march(const ConstPlanes& in, const Planes& o
                 if (r >= i0 + 1) {
                     double Wo[5];
                     update_cell<Phys, W0>(Up, Fp, F, cf, P, Wo, pu);
-                    if (acc) {
+                    if (kCount && acc) {
                         ev_add(*acc, eu, colok);
                         eu = Ev{};
                     }
