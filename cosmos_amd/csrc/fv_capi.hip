@@ -84,6 +84,13 @@ struct cosmos_solver {
     int cur_begin = 0;
     long queued = 0;
     int run_flips = 0;                         // buffer flips per step of a black-hole loop (0: sweeps_per_step)
+    // special-relativistic solve loops: each sweep's dispatch order (blocks whose
+    // solves hit the iteration cap in the previous step first) and marks
+    unsigned char* hot_x = nullptr;
+    unsigned char* hot_y = nullptr;
+    int* order_x = nullptr;
+    int* order_y = nullptr;
+    int n_x = 0, n_y = 0;
     bool unsettled = false;
     // optional per-kernel timing of queued steps (HIP events around each launch)
     bool profiling = false;
@@ -173,6 +180,39 @@ SweepLaunch base_launch(cosmos_solver* s, int in_buf, int order, double bias, in
     return L;
 }
 
+// The special-relativistic loop's dispatch orders and marks for the sweep
+// grids of this order (zero marks: the first step keeps the plain order).
+int ensure_hot_order(cosmos_solver* s, int order) {
+    if (s->physics != COSMOS_RELATIVISTIC) return COSMOS_OK;
+    SweepLaunch L = base_launch(s, s->cur, order, 0.0, 1);
+    const int nx = cfv::sweep_x_grid(L), ny = s->dim == 2 ? cfv::sweep_y_tile_grid(L) : 0;
+    if (nx != s->n_x) {
+        hipFree(s->hot_x);
+        hipFree(s->order_x);
+        s->hot_x = nullptr;
+        s->order_x = nullptr;
+        s->n_x = 0;
+        HIPCHK(s, hipMalloc((void**)&s->hot_x, (size_t)nx));
+        HIPCHK(s, hipMalloc((void**)&s->order_x, sizeof(int) * (size_t)nx));
+        HIPCHK(s, hipMemsetAsync(s->hot_x, 0, (size_t)nx, s->stream));
+        s->n_x = nx;
+    }
+    if (ny != s->n_y) {
+        hipFree(s->hot_y);
+        hipFree(s->order_y);
+        s->hot_y = nullptr;
+        s->order_y = nullptr;
+        s->n_y = 0;
+        if (ny > 0) {
+            HIPCHK(s, hipMalloc((void**)&s->hot_y, (size_t)ny));
+            HIPCHK(s, hipMalloc((void**)&s->order_y, sizeof(int) * (size_t)ny));
+            HIPCHK(s, hipMemsetAsync(s->hot_y, 0, (size_t)ny, s->stream));
+        }
+        s->n_y = ny;
+    }
+    return COSMOS_OK;
+}
+
 int check_order(cosmos_solver* s, int order) {
     if (order != 1 && order != 2) return fail(s, COSMOS_ERR_ARGUMENT, "order must be 1 (FORCE) or 2 (SLIC)");
     if (order == 2 && (s->nx < 2 || (s->dim == 2 && s->ny < 2)))
@@ -225,7 +265,12 @@ bool fused_step(const cosmos_solver* s) {
 
 // Queue one solve-loop iteration reading its dt from the device StepState.
 int queue_step(cosmos_solver* s) {
-    hipError_t e = timed(s, kProfStep, [&] { return cfv::launch_step_begin(s->st, s->stream); });
+    hipError_t e = timed(s, kProfStep, [&] {
+        hipError_t r = cfv::launch_step_begin(s->st, s->stream);
+        if (r == hipSuccess && s->n_x) r = cfv::launch_hot_order(s->hot_x, s->order_x, s->n_x, s->stream);
+        if (r == hipSuccess && s->n_y) r = cfv::launch_hot_order(s->hot_y, s->order_y, s->n_y, s->stream);
+        return r;
+    });
     if (e != hipSuccess) return fail(s, COSMOS_ERR_DEVICE, std::string("step launch: ") + hipGetErrorString(e));
     const int spp = sweeps_per_step(s);
     const int b0 = (int)((s->cur_begin + (long)spp * s->queued) & 1);
@@ -244,6 +289,10 @@ int queue_step(cosmos_solver* s) {
         L.step = s->st;
         L.use_full = (spp == 1) || (k == 1);
         L.fuse_ws = (k == spp - 1);
+        const bool y = spp == 3 && k == 1;
+        L.block_order = y ? s->order_y : s->order_x;
+        L.order_n = y ? s->n_y : s->n_x;
+        L.hot = y ? s->hot_y : s->hot_x;
         int rc = run_sweep(s, L, (spp == 3 && k == 1) ? 1 : 0);
         if (rc) return rc;
     }
@@ -581,6 +630,10 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->flag);
     hipFree(s->status);
     hipFree(s->stage);
+    hipFree(s->hot_x);
+    hipFree(s->hot_y);
+    hipFree(s->order_x);
+    hipFree(s->order_y);
     hipFree(s->gr_flux);
     hipFree(s->geo_tab);
     hipFree(s->src_tab);
@@ -975,6 +1028,7 @@ int cosmos_begin_at(cosmos_solver* s, int order, double cell_spacing, double cfl
         HIPCHK(s, hipMemcpyAsync(s->st, &h, sizeof h, hipMemcpyHostToDevice, s->stream));
         rc = max_ws_impl(s, s->st_slots);
         if (rc) return rc;
+        if ((rc = ensure_hot_order(s, order))) return rc;
         s->run_order = order;
         s->run_bias = bias;
         s->run_limiter = slope_limiter;

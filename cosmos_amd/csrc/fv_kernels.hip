@@ -99,6 +99,13 @@ __device__ __forceinline__ void status_add(unsigned long long* st, const Ev& e) 
     if (e.p) atomicAdd(st + 3, (unsigned long long)e.p);
 }
 
+// The block this workgroup runs: from the launch's dispatch order where one
+// is given for this grid, else the XCD-contiguous renumbering.
+__device__ __forceinline__ int block_of(const SweepLaunch& L) {
+    if (L.block_order && L.order_n == (int)gridDim.x) return L.block_order[blockIdx.x];
+    return xcd_tile(blockIdx.x, gridDim.x);
+}
+
 __device__ __forceinline__ bool resolve_coef(const SweepLaunch& L, SweepCoef& cf) {
     cf = L.coef;
     if (L.step) {
@@ -124,14 +131,17 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const PhysParams& P = L.P;
     const int nx = L.nx;
     const int tiles = (nx + TW - 1) / TW;
-    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int blk = block_of(L);
     const int row = blk / tiles;
     const int tile = blk - row * tiles;
     Ev acc{};
     Ev* const pacc = Phys::kPhysics == 1 ? &acc : nullptr;  // counted in registers, added only with L.status
     const double m = x_segment<Phys, ORDER, LIMMODE, DIM, WS, W0, NT>(L.in, L.out, nx, (long)row * L.pitch, tile, cf,
                                                                      L.bias, L.limiter, P, lds, pacc);
-    if (Phys::kPhysics == 1) status_add(L.status, acc);
+    if (Phys::kPhysics == 1) {
+        status_add(L.status, acc);
+        if (L.hot && acc.hit) L.hot[blk] = 1;
+    }
     if constexpr (WS) {
         if (L.step->ws_slot_atomics) {
             wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
@@ -196,7 +206,7 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
     SweepCoef cf;
     if (!resolve_coef(L, cf)) return;
     const int tc = (L.nx + TC - 1) / TC;
-    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int blk = block_of(L);
     const int br = blk / tc;
     const int bc = blk - br * tc;
     Ev acc{};
@@ -204,7 +214,10 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
     const double m = y_tile_cells<Phys, ORDER, LIMMODE, W0, NT, TC>(L.in, L.out, L.nx, L.ny, L.pitch, bc * TC, br * TRV,
                                                                      WS ? DIM : 0, cf, L.bias, L.limiter, L.P, lds,
                                                                      pacc);
-    if (Phys::kPhysics == 1) status_add(L.status, acc);
+    if (Phys::kPhysics == 1) {
+        status_add(L.status, acc);
+        if (L.hot && acc.hit) L.hot[blk] = 1;
+    }
     if constexpr (WS) {
         if (L.step->ws_slot_atomics) wave_max_to_slot(m, L.step->ws_slots, blockIdx.x * (NT / 64) + (threadIdx.x >> 6));
         else block_max_store<NT>(m, L.step->ws_part, &lds[ORDER == 2 ? 5 : 0][0]);
@@ -628,6 +641,71 @@ hipError_t launch_fused_w(const SweepLaunch& L, hipStream_t s) {
 
 hipError_t launch_step_fused(const SweepLaunch& L, hipStream_t s) {
     return L.w_zero ? launch_fused_w<true>(L, s) : launch_fused_w<false>(L, s);
+}
+
+// The dispatch order of an n-block sweep: the marked blocks first, then the
+// others, each group in the XCD-contiguous order of xcd_tile; the marks are
+// cleared for the next step.  One block of kOrderThreads threads, a chunk of
+// slots at a time (ballot prefix counts within each wave, wave totals in LDS).
+constexpr int kOrderThreads = 1024;
+
+__global__ __launch_bounds__(kOrderThreads) void k_hot_order(unsigned char* hot, int* order, int n) {
+    __shared__ int wave_hot[kOrderThreads / 64], wave_cold[kOrderThreads / 64];
+    __shared__ int total_hot;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) total_hot = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int k = tid; k < n; k += kOrderThreads) mine += hot[k] ? 1 : 0;
+    if (mine) atomicAdd(&total_hot, mine);
+    __syncthreads();
+    const int nhot = total_hot;
+    int base_hot = 0, base_cold = 0;
+    for (int c = 0; c < n; c += kOrderThreads) {
+        const int slot = c + tid;
+        const bool valid = slot < n;
+        const int b = valid ? xcd_tile(slot, n) : 0;
+        const bool h = valid && hot[b];
+        const bool cold = valid && !h;
+        const unsigned long long mh = __ballot(h), mc = __ballot(cold);
+        const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int ph = __popcll(mh & below), pc = __popcll(mc & below);
+        if (lane == 0) {
+            wave_hot[wave] = __popcll(mh);
+            wave_cold[wave] = __popcll(mc);
+        }
+        __syncthreads();
+        int oh = base_hot, oc = base_cold;
+        for (int w = 0; w < wave; ++w) {
+            oh += wave_hot[w];
+            oc += wave_cold[w];
+        }
+        if (h) order[oh + ph] = b;
+        if (cold) order[nhot + oc + pc] = b;
+        for (int w = 0; w < kOrderThreads / 64; ++w) {
+            base_hot += wave_hot[w];
+            base_cold += wave_cold[w];
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < n; k += kOrderThreads) hot[k] = 0;
+}
+
+hipError_t launch_hot_order(unsigned char* hot, int* order, int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_hot_order, dim3(1), dim3(kOrderThreads), 0, s, hot, order, n);
+    return hipGetLastError();
+}
+
+int sweep_x_grid(const SweepLaunch& L) {
+    const int TW = kXThreads - 2 * (L.order == 2 ? 2 : 1);
+    return (L.nx + TW - 1) / TW * L.ny;
+}
+
+int sweep_y_tile_grid(const SweepLaunch& L) {
+    const int tc = y_tile_for(L);
+    if (tc != 8 && tc != 16) return 0;
+    const int trv = kXThreads / tc - 2 * (L.order == 2 ? 2 : 1);
+    return (L.nx + tc - 1) / tc * ((L.ny + trv - 1) / trv);
 }
 
 hipError_t launch_step_begin(StepState* st, hipStream_t s) {

@@ -137,6 +137,14 @@ struct SweepLaunch {
     int fuse_rows;  // fused step: output rows per block (0: chosen from the occupancy)
     int y_tile;                 // y-sweep layout: 0 column march, 8 or 16 columns per LDS tile; -1: by physics
     unsigned long long* status; // special relativity: con2prim event counters (cap, q, rho, p floors), or null
+    // Special relativity in a solve loop (else null): the dispatch order of the
+    // blocks, those whose solves ran to the iteration cap in the previous step
+    // first (their wandering Newton loops then overlap the other blocks instead
+    // of trailing the launch), and the marks for the next step's order.  Used
+    // when order_n equals the launch's grid; any order gives the same results.
+    const int* block_order;
+    int order_n;
+    unsigned char* hot;
 };
 
 // All return hipSuccess or the launch error.
@@ -145,6 +153,13 @@ hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);
+// The dispatch order of an n-block sweep from its blocks' marks: the marked
+// blocks first, then the rest in the XCD-contiguous order; clears the marks.
+hipError_t launch_hot_order(unsigned char* hot, int* order, int n, hipStream_t s);
+// Blocks of the sweeps launch_sweep_x / launch_sweep_y run for L (0 when the
+// y sweep is the column march, which takes no order).
+int sweep_x_grid(const SweepLaunch& L);
+int sweep_y_tile_grid(const SweepLaunch& L);
 // One whole 2D step X(dt/2) Y(dt) X(dt/2) of the fast Newtonian SLIC / van
 // Leer / bias 0 update in one kernel (coefficients from L.step, the wave-speed
 // maximum merged into the step's slots).  Needs nx, ny >= 8 and slot atomics.

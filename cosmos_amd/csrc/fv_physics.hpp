@@ -46,6 +46,7 @@ struct PhysParams {
 // functions take an optional Ev* and count into it; null counts nothing.
 struct Ev {
     unsigned cap, q, rho, p;
+    unsigned hit;  // any solve of the thread ran to the cap, owned or not (block scheduling only)
 };
 
 CFV_HD void ev_add(Ev& a, const Ev& b, bool on) {

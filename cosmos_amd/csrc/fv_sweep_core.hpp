@@ -282,6 +282,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
     for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : (ORDER == 2 ? sA[k][sl] : sB[k][sl]);
     update_cell<Phys, W0>(U, Fl, F, cf, P, Wn, pu);
     if (kCount && acc) {
+        acc->hit += ((eh.cap + eg0.cap) + (eg1.cap + ei.cap)) + eu.cap;
         // FORCE: the cell's own flux counts where SLIC counts its half step
         ev_add(*acc, eh, own->half);
         ev_add(*acc, eg0, own->g0);
