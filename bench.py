@@ -18,8 +18,9 @@ kernels' HBM bytes from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes
 / step time / 8 TB/s.  roofline = the dominant kernel's algorithmic bytes per
 launch (80 B per cell: the state read once and written once) over its average
 launch duration, timed with HIP events on the solver's stream; cpu_baseline =
-the reference's own sources (oracle/_ref, serial build) timed on this host on a
-bounded sample.
+the reference's own sources (oracle/_ref, built with their OpenMP loops on)
+timed on every host thread on a bounded sample, cpu_baseline_serial = the same
+sources' serial build on one core.
 """
 from __future__ import annotations
 
@@ -731,21 +732,34 @@ def adjoint_leg(args):
 
 
 def cpu_leg(args):
-    """The reference's own sources (serial build, oracle/_ref) on a bounded
-    sample of the same workload; the C restatement (oracle/liboracle.so,
-    OpenMP) on one full-size step for comparison."""
+    """The reference's own sources on a bounded sample of the same workload:
+    built with their OpenMP loops on, on every host thread (cpu_baseline), and
+    serial on one core (cpu_baseline_serial); the C restatement
+    (oracle/liboracle.so, OpenMP) on one full-size step for comparison."""
     from cosmos_amd import workloads
     import oracle
 
     out = {}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     ref = oracle.load_ref()
+    ref_omp = oracle.load_ref_omp()
     n = args.cpu_size
     W = workloads.quadrants_2d(n, seed=args.seed)
+    if ref_omp is not None:
+        used = ref_omp.set_threads(threads)
+        t0 = time.perf_counter()
+        _, _, steps = ref_omp.run(W, 0, 2, 1.0 / n, 0.9, max_steps=args.cpu_steps)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": n * n * steps / dt, "unit": "cell-updates/s", "cores": used, "kind": "reference",
+            "sample": f"reference sources built with their own OpenMP loops (-fopenmp, {used} threads) on the "
+                      f"same 4-quadrant SLIC workload at {n}x{n}, {steps} full steps, {dt:.1f} s",
+        }
     if ref is not None:
         t0 = time.perf_counter()
         _, _, steps = ref.run(W, 0, 2, 1.0 / n, 0.9, max_steps=args.cpu_steps)
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {
+        out["cpu_baseline" if "cpu_baseline" not in out else "cpu_baseline_serial"] = {
             "value": n * n * steps / dt, "unit": "cell-updates/s", "cores": 1, "kind": "reference",
             "sample": f"reference sources (serial build) on the same 4-quadrant SLIC workload at {n}x{n}, "
                       f"{steps} full steps, {dt:.1f} s",
@@ -770,7 +784,6 @@ def cpu_leg(args):
                 "sample": f"reference sources (serial) on the black-hole workload at {m}x{m} (its own "
                           f"excision radius 2M), {steps} steps, {dt:.1f} s"}
     orc = oracle.load_oracle()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     orc.set_threads(threads)
     N = args.size
     WF = workloads.quadrants_2d(N, seed=args.seed)

@@ -3,6 +3,8 @@
     load_oracle()  the C restatement, oracle/liboracle.so (orc_* functions)
     load_ref()     the reference's own sources built by oracle/Makefile,
                    oracle/_ref/libcosmos_ref.so (ref_* functions)
+    load_ref_omp() the same sources with OpenMP on, _ref/libcosmos_ref_omp.so
+                   (timing only: its parallel loops race)
 
 Both expose the same signatures (see cosmos_oracle.h); wrap() gives them a
 common numpy-facing interface.  Only tests, the golden-fixture script,
@@ -21,6 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libcosmos_ref.so")
+REF_OMP_SO = os.path.join(HERE, "_ref", "libcosmos_ref_omp.so")
 REFERENCE_ROOT = "/root/reference"
 
 _D = ctypes.c_double
@@ -92,6 +95,9 @@ class CpuPath:
             self.lib.ref_gr_amr_level2.restype = _I
             self.lib.ref_gr_amr_level2.argtypes = [_I, _P, _I, _D, _D, _D, _D, _D, _D, _D, _D, _D, _D, _P, _P, _P,
                                                    u8, u8, u8]
+        if prefix == "ref_" and hasattr(self.lib, "ref_set_threads"):
+            self.lib.ref_set_threads.restype = _I
+            self.lib.ref_set_threads.argtypes = [_I]
         if prefix == "ref_" and hasattr(self.lib, "ref_amr_level1"):
             self.lib.ref_amr_level1.restype = _I
             self.lib.ref_amr_level1.argtypes = [_I, _I, _I, _P, _I, _I, _D, _D, _D, _D, _D, _P, _P, u8, u8]
@@ -423,7 +429,11 @@ class CpuPath:
         return out[0], buf
 
     def set_threads(self, n):
+        """Threads of the OpenMP loops; returns the count they will use."""
+        if self.prefix == "ref_":
+            return self.lib.ref_set_threads(n)
         self.lib.orc_set_threads(n)
+        return n
 
     def gr_set_excision(self, radius):
         """Excision radius override (<= 0 restores the reference's horizon radius)."""
@@ -441,3 +451,12 @@ def load_ref() -> CpuPath | None:
     if not os.path.exists(REF_SO):
         return None
     return CpuPath(REF_SO, "ref_")
+
+
+def load_ref_omp() -> CpuPath | None:
+    """The reference's sources built with their OpenMP loops enabled, for
+    timing only (those loops race on shared maxima and flag vectors, so its
+    results are never compared), or None where it has not been built."""
+    if not os.path.exists(REF_OMP_SO):
+        return None
+    return CpuPath(REF_OMP_SO, "ref_")

@@ -16,6 +16,10 @@
 #include <sstream>
 #include <vector>
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include "Solvers/EulerFirstOrderSolver.hpp"
 #include "Solvers/EulerSecondOrderSolver.hpp"
 #include "Solvers/RelativisticEulerFirstOrderSolver.hpp"
@@ -871,4 +875,17 @@ extern "C" void ref_insert_boundary(int physics, int dim, int size, const double
         std::vector<std::vector<BlackHoleEulerStateVector> > r = BlackHoleEulerSolvers::insertBoundaryCells2D(c, size);
         from_2d(r, out);
     }
+}
+
+// Threads of the reference's own OpenMP loops (the -fopenmp build,
+// libcosmos_ref_omp.so; the serial build has one).  n <= 0 leaves the
+// runtime's default.  Returns the thread count the loops will use.
+extern "C" int ref_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
 }

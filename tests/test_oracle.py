@@ -32,6 +32,25 @@ def test_run_2d_matches_reference(ref, orc, physics, gamma, order, limiter):
     assert same(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
 
 
+def test_openmp_reference_build_is_the_same_code(ref):
+    """The reference's OpenMP build (timed as the all-cores CPU baseline) on
+    one thread gives the serial build's result bitwise; with several threads
+    its shared-maximum loops race, so only the run's completion is checked."""
+    from oracle import load_ref_omp
+
+    ro = load_ref_omp()
+    if ro is None:
+        pytest.skip("OpenMP reference build not present")
+    W = workloads.quadrants_2d(20, seed=9, ny=17)
+    a = ref.run(W, 0, 2, 1.0 / 20, 0.9, max_steps=3)
+    assert ro.set_threads(1) == 1
+    b = ro.run(W, 0, 2, 1.0 / 20, 0.9, max_steps=3)
+    assert same(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+    assert ro.set_threads(4) == 4
+    c = ro.run(W, 0, 2, 1.0 / 20, 0.9, max_steps=3)
+    assert c[2] == 3 and np.isfinite(c[0]).all()
+
+
 @pytest.mark.parametrize("physics,gamma", [(0, G_N), (1, G_SR)])
 @pytest.mark.parametrize("order", [1, 2])
 def test_run_1d_to_time_matches_reference(ref, orc, physics, gamma, order):
