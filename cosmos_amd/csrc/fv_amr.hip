@@ -1128,19 +1128,35 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
         a->dx = cell_spacing;
         const long nf2 = 4L * a->nx * (a->dim == 1 ? 1 : 4L * a->ny);
         if (!a->fine2) {
-            rc = cosmos_create(&a->fine2, a->physics, a->dim, 4 * a->nx, a->dim == 1 ? 1 : 4 * a->ny, a->P.gamma);
+            // built into locals and handed to the context only once every step
+            // has succeeded: a failed setup leaves no half-configured level
+            // behind for the next call to skip past
+            cosmos_solver* f2 = nullptr;
+            rc = cosmos_create(&f2, a->physics, a->dim, 4 * a->nx, a->dim == 1 ? 1 : 4 * a->ny, a->P.gamma);
             if (rc) return amr_fail(a, rc, std::string("AMR setup: ") + cosmos_last_error(nullptr));
+            auto give_up = [&](int code) {
+                const std::string msg = std::string("AMR setup: ") + cosmos_last_error(f2);
+                cosmos_destroy(f2);
+                return amr_fail(a, code, msg);
+            };
             if (a->P.c2p_tol != cfv::make_params(a->P.gamma).c2p_tol ||
                 a->P.c2p_maxit != cfv::make_params(a->P.gamma).c2p_maxit || a->P.gr_tol > 0.0) {
-                rc = cosmos_set_con2prim(a->fine2, a->P.c2p_tol, a->P.c2p_maxit);
-                if (rc) return from_solver(a, a->fine2, rc);
+                rc = cosmos_set_con2prim(f2, a->P.c2p_tol, a->P.c2p_maxit);
+                if (rc) return give_up(rc);
             }
             if (a->physics == COSMOS_BLACKHOLE) {
-                rc = cosmos_set_spacetime(a->fine2, a->hole.M, a->hole.a, a->hole.x, a->hole.y, a->hole.z);
-                if (!rc) rc = cosmos_set_excision(a->fine2, a->excision);
-                if (rc) return from_solver(a, a->fine2, rc);
+                rc = cosmos_set_spacetime(f2, a->hole.M, a->hole.a, a->hole.x, a->hole.y, a->hole.z);
+                if (!rc) rc = cosmos_set_excision(f2, a->excision);
+                if (rc) return give_up(rc);
             }
-            AMRCHK(a, hipMalloc((void**)&a->f2flags, (size_t)nf2));
+            unsigned char* flags2 = nullptr;
+            const hipError_t e = hipMalloc((void**)&flags2, (size_t)nf2);
+            if (e != hipSuccess) {
+                cosmos_destroy(f2);
+                return amr_fail(a, COSMOS_ERR_DEVICE, std::string("AMR setup: ") + hipGetErrorString(e));
+            }
+            a->fine2 = f2;
+            a->f2flags = flags2;
         }
         // every level starts from the coarse state: intermediate and fine cells
         // copies of their coarse cell, coarse flags set, the others clear (:597-636)

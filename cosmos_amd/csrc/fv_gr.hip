@@ -137,11 +137,15 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
         load5(L.in, at(c - 2), W0);
         load5(L.in, at(c - 1), W1);
         load5(L.in, at(c), W2);
-        if (AXIS == 0) gr_half_step<AXIS, GEO>(W0, W1, W2, x - hh, y, ax - 1, ay, 1, L, cf, pr);
-        else gr_half_step<AXIS, GEO>(W0, W1, W2, x, y - hh, ax, ay - 1, 1, L, cf, pr);
+        // the two face states are independent: both evolved, then their
+        // primitive recoveries run as one interleaved pair
+        double er[5], el[5], sr, sl;
+        if (AXIS == 0) gr_half_evolve<AXIS, GEO>(W0, W1, W2, x - hh, y, ax - 1, ay, 1, L, cf, er, sr);
+        else gr_half_evolve<AXIS, GEO>(W0, W1, W2, x, y - hh, ax, ay - 1, 1, L, cf, er, sr);
         load5(L.in, at(c + 1), W0);
-        if (AXIS == 0) gr_half_step<AXIS, GEO>(W1, W2, W0, x + hh, y, ax + 1, ay, 0, L, cf, pl);
-        else gr_half_step<AXIS, GEO>(W1, W2, W0, x, y + hh, ax, ay + 1, 0, L, cf, pl);
+        if (AXIS == 0) gr_half_evolve<AXIS, GEO>(W1, W2, W0, x + hh, y, ax + 1, ay, 0, L, cf, el, sl);
+        else gr_half_evolve<AXIS, GEO>(W1, W2, W0, x, y + hh, ax, ay + 1, 0, L, cf, el, sl);
+        gr_c2p2(er, sr, el, sl, L.P, pr, pl);
         gr_force<AXIS, GEO>(pr, pl, x, y, ax, ay, L, cf, F);
         if (L.half) {
             const long f = AXIS == 0 ? (long)i * L.fpitch + c : (long)c * L.fpitch + j;

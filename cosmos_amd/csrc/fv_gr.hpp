@@ -250,6 +250,36 @@ CFV_HD void gr_con2prim_rec(const double* U, double sdet, double tol, const Phys
     c2p_finish(c, x, fd, P);
 }
 
+// Two independent primitive recoveries side by side.  Each chain takes exactly
+// the steps of gr_con2prim_rec (the same iterates, the same kept x and f'(x),
+// the same finish), so each record is bitwise what gr_con2prim_rec gives; the
+// Newton steps of the two are interleaved so that one chain's dependent FP64
+// latency hides the other's.  A chain that has stopped is held by selects
+// while the other continues (its extra steps are computed and discarded).
+CFV_HD void gr_con2prim_rec2(const double* Ua, double sdeta, double tola, const double* Ub, double sdetb,
+                             double tolb, const PhysParams& P, C2PRec& ca, C2PRec& cb) {
+    C2PCoef ka, kb;
+    c2p_setup(Ua, sdeta, P, ca, ka);
+    c2p_setup(Ub, sdetb, P, cb, kb);
+    double xa = 1.0, fda = 1.0, xb = 1.0, fdb = 1.0;
+    bool ra = true, rb = true;
+    for (int it = 0; it < P.c2p_maxit; ++it) {
+        double ga, gb;
+        const double xna = c2p_newton(ca, ka, P.sr_c0, xa, ga);
+        const double xnb = c2p_newton(cb, kb, P.sr_c0, xb, gb);
+        const bool sa = __builtin_fabs(xa - xna) < tola, sb = __builtin_fabs(xb - xnb) < tolb;
+        fda = ra ? ga : fda;
+        fdb = rb ? gb : fdb;
+        xa = (ra && !sa) ? xna : xa;
+        xb = (rb && !sb) ? xnb : xb;
+        ra = ra && !sa;
+        rb = rb && !sb;
+        if (!ra && !rb) break;
+    }
+    c2p_finish(ca, xa, fda, P);
+    c2p_finish(cb, xb, fdb, P);
+}
+
 // R = (rho, v^x, v^y, v^z, p); lf = W; h = specific enthalpy.
 CFV_HD void gr_con2prim(const double* U, double sdet, double tol, const PhysParams& P, double* R, double& lf_out,
                         double& h_out) {
@@ -266,6 +296,19 @@ CFV_HD double gr_tol(const PhysParams& P, double reference) { return P.gr_tol > 
 CFV_HD void gr_c2p(const double* U, double sdet, const PhysParams& P, double* W) {
     double lf, h;
     gr_con2prim(U, sdet, gr_tol(P, 1e-8), P, W, lf, h);
+}
+
+// Two independent gr_c2p side by side (bitwise each gr_c2p).
+CFV_HD void gr_c2p2(const double* Ua, double sdeta, const double* Ub, double sdetb, const PhysParams& P, double* Wa,
+                    double* Wb) {
+    C2PRec ca, cb;
+    const double tol = gr_tol(P, 1e-8);
+    gr_con2prim_rec2(Ua, sdeta, tol, Ub, sdetb, tol, P, ca, cb);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        Wa[k] = ca.R[k];
+        Wb[k] = cb.R[k];
+    }
 }
 
 // compute{X,Y}FluxVector(cons) (:159-257 x, tolerance 1e-10; :259-357 y, 1e-8):
@@ -299,6 +342,18 @@ CFV_HD void gr_flux(const double* U, const Geo& G, const PhysParams& P, double* 
     C2PRec c;
     gr_con2prim_rec(U, G.sdet, gr_flux_tol(P, AXIS), P, c);
     gr_flux_of<AXIS>(c, G, F);
+}
+
+// Two independent fluxes along one axis (their Newton solves interleaved):
+// bitwise gr_flux of each.
+template <int AXIS>
+CFV_HD void gr_flux2(const double* Ua, const Geo& Ga, const double* Ub, const Geo& Gb, const PhysParams& P,
+                     double* Fa, double* Fb) {
+    C2PRec ca, cb;
+    const double tol = gr_flux_tol(P, AXIS);
+    gr_con2prim_rec2(Ua, Ga.sdet, tol, Ub, Gb.sdet, tol, P, ca, cb);
+    gr_flux_of<AXIS>(ca, Ga, Fa);
+    gr_flux_of<AXIS>(cb, Gb, Fb);
 }
 
 // ---------------------------------------------------------------------------

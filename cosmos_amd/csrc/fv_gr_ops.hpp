@@ -73,10 +73,12 @@ CFV_HD void gr_slope_at(const double* WL, const double* WM, const double* WR, do
 // evolveStateByHalf{X,Y}TimeStep (Solvers/BlackHoleEulerSolvers.cpp:219-307) of the
 // middle cell centred at (x, y): slope from p2c at x - h, x, x + h; face fluxes
 // at x -/+ h/2; the selected face evolved by 0.5 (dt/h) (F_L - F_R) and
-// converted back at that face.
+// converted back at that face.  gr_half_evolve stops before that conversion:
+// e, the evolved conserved face state, and sdet, sqrt(det gamma) at the face.
 template <int AXIS, int GEO>
-__host__ __device__ void gr_half_step(const double* WL, const double* WM, const double* WR, double x, double y, int ax,
-                                      int ay, int side, const GRLaunch& L, const SweepCoef& cf, double* out) {
+__host__ __device__ void gr_half_evolve(const double* WL, const double* WM, const double* WR, double x, double y,
+                                        int ax, int ay, int side, const GRLaunch& L, const SweepCoef& cf, double* e,
+                                        double& sdet) {
     const double h = L.h;
     double um[5], s[5];
     gr_slope_at<AXIS, GEO>(WL, WM, WR, x, y, ax, ay, L, um, s);
@@ -89,12 +91,18 @@ __host__ __device__ void gr_half_step(const double* WL, const double* WM, const 
     const double hh = h * 0.5;
     Geo gl = geo_off<AXIS, GEO>(L, x, y, -hh, ax, ay, -1);
     Geo gr = geo_off<AXIS, GEO>(L, x, y, hh, ax, ay, 1);
-    gr_flux<AXIS>(xl, gl, L.P, fl);
-    gr_flux<AXIS>(xr, gr, L.P, fr);
-    double e[5];
+    gr_flux2<AXIS>(xl, gl, xr, gr, L.P, fl, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + evolution_k(cf.hdt_dx, fl[k], fr[k]);
-    gr_c2p(e, side == 0 ? gl.sdet : gr.sdet, L.P, out);
+    sdet = side == 0 ? gl.sdet : gr.sdet;
+}
+
+template <int AXIS, int GEO>
+__host__ __device__ void gr_half_step(const double* WL, const double* WM, const double* WR, double x, double y, int ax,
+                                      int ay, int side, const GRLaunch& L, const SweepCoef& cf, double* out) {
+    double e[5], sdet;
+    gr_half_evolve<AXIS, GEO>(WL, WM, WR, x, y, ax, ay, side, L, cf, e, sdet);
+    gr_c2p(e, sdet, L.P, out);
 }
 
 // The Lax-Friedrichs flux (KIND 0), the Richtmyer flux (1) or the FORCE flux
@@ -111,8 +119,7 @@ __host__ __device__ void gr_force(const double* WL, const double* WR, double x, 
         Geo gr = geo_off<AXIS, GEO>(L, x, y, hh, ax, ay, 1);
         gr_p2c(WL, gl, ul, L.P);
         gr_p2c(WR, gr, ur, L.P);
-        gr_flux<AXIS>(ul, gl, L.P, fl);
-        gr_flux<AXIS>(ur, gr, L.P, fr);
+        gr_flux2<AXIS>(ul, gl, ur, gr, L.P, fl, fr);
     }
     if constexpr (KIND == 0) {
 #pragma unroll
@@ -147,8 +154,7 @@ CFV_HD void gr_fractional_core(double fdt_h, const double* ul, const double* WM,
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    gr_flux<AXIS>(xl, gl, L.P, fl);
-    gr_flux<AXIS>(xr, gr, L.P, fr);
+    gr_flux2<AXIS>(xl, gl, xr, gr, L.P, fl, fr);
     double e[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = um[k] + evolution_k(fdt_h, fl[k], fr[k]);
