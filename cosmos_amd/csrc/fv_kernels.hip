@@ -247,7 +247,8 @@ template <class Phys, int LIMMODE, bool W0>
 __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_step_fused(SweepLaunch L,
                                                                                                      int RB) {
     constexpr int NT = kXThreads;
-    __shared__ double lds[15][NT];
+    constexpr int NC = W0 ? 4 : 5;  // stored components
+    __shared__ double lds[slic3r_lds_rows<W0>()][NT];
     if (L.step->done) return;
     const SweepCoef ch = L.step->half, cfu = L.step->full;
     const PhysParams& P = L.P;
@@ -267,15 +268,17 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
 
     // index of the thread's cell of (mirrored) input row rr
     auto at_row = [&](int rr) { return (long)mirror2(rr, ny) * pitch + pc; };
-    // first x sweep of one input row (the thread's primitives Wr) -> Wx
+    // first x sweep of one input row (the thread's primitives Wr) -> Wx; the
+    // passes of line_step_3r need no barrier between them
     auto x1_row = [&](const double* Wr, double* Wx) {
-        __syncthreads();
-        line_step<Phys, 2, LIMMODE, W0, NT, 0, 1>(Wr, Wx, ch, 0.0, 1, P, lds);
+        line_step_3r<Phys, LIMMODE, W0, NT>(Wr, Wx, ch, P, lds);
         if (edge) {
-            __syncthreads();
+            // through the R2 rows of line_step_3r (free until the next pass's
+            // first barrier; its last reads were before this pass's last one)
+            double(*X)[NT] = lds + NC;
 #pragma unroll
             for (int k = 0; k < 5; ++k)
-                if (!(W0 && k == 3)) lds[k][t] = Wx[k];
+                if (!(W0 && k == 3)) X[(W0 && k > 3) ? k - 1 : k][t] = Wx[k];
             __syncthreads();
             if (p < 0 || p >= nx) {
                 // the mirror column's slot: in [4, 8) on the left, >= 1 on the
@@ -285,7 +288,7 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
                 src = src < 0 ? 0 : (src > NT - 1 ? NT - 1 : src);
 #pragma unroll
                 for (int k = 0; k < 5; ++k)
-                    if (!(W0 && k == 3)) Wx[k] = lds[k][src];
+                    if (!(W0 && k == 3)) Wx[k] = X[(W0 && k > 3) ? k - 1 : k][src];
             }
         }
     };
@@ -317,8 +320,7 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
                     // y output of row r - 2 -> second x sweep -> HBM
                     double Wo[5], Wz[5];
                     update_cell<Phys, W0>(Ua, Fp, F, cfu, P, Wo);
-                    __syncthreads();
-                    line_step<Phys, 2, LIMMODE, W0, NT, 0, 1>(Wo, Wz, ch, 0.0, 1, P, lds);
+                    line_step_3r<Phys, LIMMODE, W0, NT>(Wo, Wz, ch, P, lds);
                     if (t >= 4 && t < NT - 4 && p < nx) {
                         store_cell<W0>(L.out, (long)(r - 2) * pitch + p, Wz);
                         const double w = Phys::template wavespeed<2>(Wz, P);

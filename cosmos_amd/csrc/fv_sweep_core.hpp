@@ -292,6 +292,77 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
     }
 }
 
+// Rows of LDS (of NT doubles) line_step_3r uses: the conserved vectors, the
+// left-face states and fluxes, and the interface fluxes, each of the stored
+// components (four when the z velocity is identically zero).
+template <bool W0>
+constexpr int slic3r_lds_rows() {
+    return 4 * (W0 ? 4 : 5);
+}
+
+// line_step for a SLIC pass along x at bias 0 with van Leer (the fused step's
+// sweeps), its stages in three disjoint LDS regions: R1 the conserved vectors
+// (rows [0, NC)), R2 the left-face states and fluxes ([NC, 3 NC)), R3 the
+// interface fluxes ([3 NC, 4 NC)).  Each region is written only after a
+// barrier that follows the previous pass's last read of it, so consecutive
+// passes need no barrier between them; a caller that reads or writes these
+// rows itself between passes uses R2 (free from the end of one pass to the
+// first barrier of the next) and barriers before and after.  Every value is
+// computed by the functions and in the order line_step uses: bitwise equal.
+template <class Phys, int LIMMODE, bool W0, int NT>
+__device__ __forceinline__ void line_step_3r(const double* W, double* Wn, const SweepCoef& cf, const PhysParams& P,
+                                             double (*lds)[NT]) {
+    constexpr int NC = W0 ? 4 : 5;
+    double(*R1)[NT] = lds;
+    double(*Rq)[NT] = lds + NC;
+    double(*Rg)[NT] = lds + 2 * NC;
+    double(*R3)[NT] = lds + 3 * NC;
+    auto slot = [](int k) { return (W0 && k > 3) ? k - 1 : k; };
+    const int s = threadIdx.x;
+    const int sl = s >= 1 ? s - 1 : 0;
+    const int sr = s < NT - 1 ? s + 1 : NT - 1;
+    double U[5], F[5];
+    Phys::p2c(W, U, P);
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (!(W0 && k == 3)) R1[slot(k)][s] = U[k];
+    __syncthreads();
+    double Q1[5], G1[5];
+    {
+        double ul[5], ur[5], Q0[5], G0[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            ul[k] = (W0 && k == 3) ? 0.0 : R1[slot(k)][sl];
+            ur[k] = (W0 && k == 3) ? 0.0 : R1[slot(k)][sr];
+        }
+        half_step<Phys, 0, LIMMODE, W0>(ul, U, ur, cf, 0.0, 1, P, Q0, G0, Q1, G1);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            if (W0 && k == 3) continue;
+            Rq[slot(k)][s] = Q0[k];
+            Rg[slot(k)][s] = G0[k];
+        }
+    }
+    __syncthreads();
+    {
+        double UR[5], FR[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            UR[k] = (W0 && k == 3) ? 0.0 : Rq[slot(k)][sr];
+            FR[k] = (W0 && k == 3) ? 0.0 : Rg[slot(k)][sr];
+        }
+        force_flux<Phys, 0, W0>(Q1, G1, UR, FR, cf, P, F);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+        if (!(W0 && k == 3)) R3[slot(k)][s] = F[k];
+    __syncthreads();
+    double Fl[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : R3[slot(k)][sl];
+    update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
+}
+
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>
 __device__ __forceinline__ double line_cells(const ConstPlanes& in, const Planes& out, long ld_idx, long st_idx,
                                           bool valid, int ws_dim, const SweepCoef& cf, double bias, int limiter,
