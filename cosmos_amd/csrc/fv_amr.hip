@@ -28,6 +28,14 @@
 #include "fv_kernels.hpp"
 
 namespace cfv {
+// solver internals for the AMR levels (fv_capi.hip)
+int amr_masked_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
+                     int slope_limiter, const unsigned char* mask);
+hipError_t amr_mask_writeback(cosmos_solver* s, const unsigned char* mask);
+int amr_share_stream(cosmos_solver* s, cosmos_solver* owner);
+}  // namespace cfv
+
+namespace cfv {
 namespace {
 
 constexpr int kAmrThreads = 256;
@@ -587,6 +595,12 @@ int from_solver(cosmos_amr* a, cosmos_solver* s, int rc) {
     return amr_fail(a, rc, cosmos_last_error(s));
 }
 
+// Wait for the queued work of every level (they share the coarse handle's stream).
+int amr_sync(cosmos_amr* a) {
+    AMRCHK(a, hipStreamSynchronize((hipStream_t)cosmos_stream(a->coarse)));
+    return COSMOS_OK;
+}
+
 int grid_of(cosmos_amr* a, cosmos_solver* s, Grid& g) {
     double* planes[5];
     long pitch = 0;
@@ -603,16 +617,12 @@ int grid_of(cosmos_amr* a, cosmos_solver* s, Grid& g) {
 long coarse_cells(const cosmos_amr* a) { return (long)a->nx * a->ny; }
 long fine_cells(const cosmos_amr* a) { return (long)(2 * a->nx) * (a->dim == 1 ? 1 : 2 * a->ny); }
 
-// Kernels that touch both levels (or the flags) run on the coarse handle's
-// stream after both handles' queued work has finished; the solver calls of
-// either level likewise start only after that stream is idle, so every
-// access to a level's state or flags is ordered.
-hipStream_t order_streams(cosmos_amr* a) {
-    if (a->fine2) cosmos_synchronize(a->fine2);
-    cosmos_synchronize(a->fine);
-    cosmos_synchronize(a->coarse);
-    return (hipStream_t)cosmos_stream(a->coarse);
-}
+// Every level queues its work on the coarse handle's stream (the finer
+// levels share it from their creation), so the kernels that touch several
+// levels or the flags are ordered after each level's work without a host
+// synchronisation: they go on that stream too.
+hipStream_t order_streams(cosmos_amr* a) { return (hipStream_t)cosmos_stream(a->coarse); }
+
 
 template <class F>
 int amr_guarded(cosmos_amr* a, F&& f) {
@@ -764,10 +774,23 @@ unsigned char* level_flags(cosmos_amr* a, int level) {
     return level == 0 ? a->cflags : level == 1 ? a->fflags : a->f2flags;
 }
 
-// One sweep of a level restricted to that level's flags.
+// One sweep of a level restricted to that level's flags, in one of two forms
+// with the same result.  The finer levels, whose flags mark the refined region
+// (usually a small part of the level): only the tiles holding a flagged cell
+// are swept and only flagged cells written, then written back into the level's
+// state (Newtonian and special relativistic).  The coarse level, whose flags
+// mark every cell outside the refined region, and black-hole levels: the
+// level's full sweep, then the unflagged cells restored.
 int amr_sweep(cosmos_amr* a, int level, int order, int axis, double dx, double dt, double bias, int lim) {
     cosmos_solver* s = level_solver(a, level);
-    order_streams(a);
+    if (a->physics != COSMOS_BLACKHOLE && level > 0) {
+        const unsigned char* mask = level_flags(a, level);
+        int rc = cfv::amr_masked_sweep(s, order, axis, dx, dt, bias, lim, mask);
+        if (rc) return from_solver(a, s, rc);
+        AMRCHK(a, amr_timed(a, kAmrProfMerge, (hipStream_t)cosmos_stream(s),
+                            [&] { return cfv::amr_mask_writeback(s, mask); }));
+        return COSMOS_OK;
+    }
     Grid before, after;
     int rc = grid_of(a, s, before);
     if (rc) return rc;
@@ -877,6 +900,7 @@ int cosmos_amr_create(cosmos_amr** out, int physics, int dim, int nx, int ny, do
     a->P = cfv::make_params(adiabatic_index);
     int rc = cosmos_create(&a->coarse, physics, dim, nx, ny, adiabatic_index);
     if (!rc) rc = cosmos_create(&a->fine, physics, dim, 2 * nx, dim == 1 ? 1 : 2 * ny, adiabatic_index);
+    if (!rc) rc = cfv::amr_share_stream(a->fine, a->coarse);
     hipError_t e = hipSuccess;
     if (!rc) e = hipMalloc((void**)&a->cflags, (size_t)coarse_cells(a));
     if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->fflags, (size_t)fine_cells(a));
@@ -903,9 +927,10 @@ int cosmos_amr_destroy(cosmos_amr* a) {
     hipFree(a->fflags);
     hipFree(a->f2flags);
     hipFree(a->any);
-    cosmos_destroy(a->coarse);
-    cosmos_destroy(a->fine);
+    // the finer levels queue on the coarse handle's stream: they go first
     cosmos_destroy(a->fine2);
+    cosmos_destroy(a->fine);
+    cosmos_destroy(a->coarse);
     delete a;
     return COSMOS_OK;
 }
@@ -922,7 +947,7 @@ int cosmos_amr_profile_enable(cosmos_amr* a, int on) {
 int cosmos_amr_profile_read(cosmos_amr* a, double* total_ms, int* launches) {
     return amr_guarded(a, [&] {
         if (!total_ms || !launches) return amr_fail(a, COSMOS_ERR_ARGUMENT, "null output");
-        order_streams(a);
+        if (int rs = amr_sync(a)) return rs;
         for (int k = 0; k < kAmrProfKinds; ++k) {
             total_ms[k] = 0.0;
             launches[k] = 0;
@@ -1014,7 +1039,7 @@ int cosmos_amr_upload(cosmos_amr* a, const double* prim) {
 int cosmos_amr_download(cosmos_amr* a, double* coarse, double* fine, unsigned char* coarse_flags,
                         unsigned char* fine_flags) {
     return amr_guarded(a, [&] {
-        order_streams(a);
+        if (int rs = amr_sync(a)) return rs;
         int rc = coarse ? cosmos_download(a->coarse, coarse) : COSMOS_OK;
         if (rc) return from_solver(a, a->coarse, rc);
         rc = fine ? cosmos_download(a->fine, fine) : COSMOS_OK;
@@ -1109,7 +1134,7 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
             if ((rc = amr_restrict(a, cell_spacing))) return rc;
             if ((rc = amr_derefine(a, cell_spacing, tolerance))) return rc;
         }
-        order_streams(a);
+        if ((rc = amr_sync(a))) return rc;
         if (time_out) *time_out = t;
         if (steps_out) *steps_out = it;
         return COSMOS_OK;
@@ -1134,6 +1159,11 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
             cosmos_solver* f2 = nullptr;
             rc = cosmos_create(&f2, a->physics, a->dim, 4 * a->nx, a->dim == 1 ? 1 : 4 * a->ny, a->P.gamma);
             if (rc) return amr_fail(a, rc, std::string("AMR setup: ") + cosmos_last_error(nullptr));
+            if ((rc = cfv::amr_share_stream(f2, a->coarse))) {
+                const std::string msg = std::string("AMR setup: ") + cosmos_last_error(f2);
+                cosmos_destroy(f2);
+                return amr_fail(a, rc, msg);
+            }
             auto give_up = [&](int code) {
                 const std::string msg = std::string("AMR setup: ") + cosmos_last_error(f2);
                 cosmos_destroy(f2);
@@ -1229,7 +1259,7 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
             if ((rc = l2_exchange(a, 0, cell_spacing, 0))) return rc;
             if ((rc = l2_derefine(a, 0, cell_spacing, dtol))) return rc;
         }
-        order_streams(a);
+        if ((rc = amr_sync(a))) return rc;
         if (time_out) *time_out = t;
         if (steps_out) *steps_out = it;
         return COSMOS_OK;

@@ -72,6 +72,7 @@ struct cosmos_solver {
     bool geo_built = false, geo_exact = false;
     double geo_key[6] = {0, 0, 0, 0, 0, 0};
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false: the stream belongs to another handle (the levels of an AMR context)
     int device = 0;
     // solve-loop parameters set by cosmos_begin.  Steps are queued without
     // knowing on the host whether the device loop has stopped; a stopped step
@@ -638,7 +639,7 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->geo_tab);
     hipFree(s->src_tab);
     hipFree(s->q_tab);
-    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->stream && s->own_stream) hipStreamDestroy(s->stream);
     delete s;
     return COSMOS_OK;
 }
@@ -946,6 +947,62 @@ int cosmos_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, dou
         return COSMOS_OK;
     });
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// AMR support (called by fv_amr.hip, not part of the C-ABI)
+// ---------------------------------------------------------------------------
+namespace cfv {
+
+// The masked sweep of a Newtonian or special-relativistic level
+// (compute{X,Y}{SLIC,FORCE}TimeStepAMR: each flagged cell updated from the
+// previous state through the full boundary copy, the others unchanged): only
+// tiles that hold a flagged cell are swept, and only flagged cells are written,
+// into the other buffer; amr_mask_writeback then copies them back, so the
+// state stays in the current buffer.  Not counted in the status word.
+int amr_masked_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
+                     int slope_limiter, const unsigned char* mask) {
+    return guarded(s, [&] {
+        int rc = check_order(s, order);
+        if (rc) return rc;
+        if (axis != 0 && axis != 1) return fail(s, COSMOS_ERR_ARGUMENT, "axis must be 0 (x) or 1 (y)");
+        if (axis == 1 && s->dim == 1) return fail(s, COSMOS_ERR_ARGUMENT, "a 1D solver has no y sweep");
+        if (s->physics == COSMOS_BLACKHOLE) return fail(s, COSMOS_ERR_ARGUMENT, "black-hole levels sweep unmasked");
+        rc = settle(s);
+        if (rc) return rc;
+        s->run_ready = false;
+        SweepLaunch L = base_launch(s, s->cur, order, bias, slope_limiter);
+        L.coef.dt_dx = time_step / cell_spacing;
+        L.coef.dx_dt = cell_spacing / time_step;
+        L.coef.hdt_dx = 0.5 * (time_step / cell_spacing);
+        L.status = nullptr;
+        L.mask = mask;
+        return run_sweep(s, L, axis);
+    });
+}
+
+// The masked sweep's flagged cells from the other buffer back into the current one.
+hipError_t amr_mask_writeback(cosmos_solver* s, const unsigned char* mask) {
+    return launch_mask_writeback(planes_of(s, s->cur), cplanes_of(s, s->cur ^ 1), mask, s->nx, s->ny, s->pitch,
+                                 s->stream);
+}
+
+// Queue s's work on owner's stream from now on (the levels of an AMR context
+// then need no host synchronisation between them); s must not outlive owner.
+int amr_share_stream(cosmos_solver* s, cosmos_solver* owner) {
+    return guarded(s, [&] {
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        if (s->own_stream) HIPCHK(s, hipStreamDestroy(s->stream));
+        s->stream = owner->stream;
+        s->own_stream = false;
+        return COSMOS_OK;
+    });
+}
+
+}  // namespace cfv
+
+extern "C" {
 
 static int max_ws_impl(cosmos_solver* s, unsigned long long* slots_dev) {
     int rc = settle(s);

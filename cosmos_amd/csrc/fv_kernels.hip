@@ -136,8 +136,9 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_x(SweepLaunch L) {
     const int tile = blk - row * tiles;
     Ev acc{};
     Ev* const pacc = Phys::kPhysics == 1 ? &acc : nullptr;  // counted in registers, added only with L.status
-    const double m = x_segment<Phys, ORDER, LIMMODE, DIM, WS, W0, NT>(L.in, L.out, nx, (long)row * L.pitch, tile, cf,
-                                                                     L.bias, L.limiter, P, lds, pacc);
+    const double m = x_segment<Phys, ORDER, LIMMODE, DIM, WS, W0, NT>(
+        L.in, L.out, nx, (long)row * L.pitch, tile, cf, L.bias, L.limiter, P, lds, pacc,
+        L.mask ? L.mask + (long)row * nx : nullptr);
     if (Phys::kPhysics == 1) {
         status_add(L.status, acc);
         if (L.hot && acc.hit) L.hot[blk] = 1;
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
     Ev* const pacc = Phys::kPhysics == 1 ? &acc : nullptr;  // counted in registers, added only with L.status
     const double m = y_tile_cells<Phys, ORDER, LIMMODE, W0, NT, TC>(L.in, L.out, L.nx, L.ny, L.pitch, bc * TC, br * TRV,
                                                                      WS ? DIM : 0, cf, L.bias, L.limiter, L.P, lds,
-                                                                     pacc);
+                                                                     pacc, L.mask);
     if (Phys::kPhysics == 1) {
         status_add(L.status, acc);
         if (L.hot && acc.hit) L.hot[blk] = 1;
@@ -478,6 +479,16 @@ __global__ __launch_bounds__(256) void k_zero_slots(unsigned long long* slots) {
     for (int k = threadIdx.x; k < kSlotWords; k += blockDim.x) slots[k] = 0ull;
 }
 
+__global__ __launch_bounds__(256) void k_mask_writeback(Planes dst, ConstPlanes src, const unsigned char* mask,
+                                                       int nx, int ny, long pitch) {
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (long)nx * ny || !mask[idx]) return;
+    const long i = idx / nx, j = idx - i * nx;
+    const long c = i * pitch + j;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dst.p[k][c] = src.p[k][c];
+}
+
 __global__ __launch_bounds__(256) void k_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag) {
     long total = (long)nx * ny;
     int found = 0;
@@ -519,6 +530,7 @@ int y_rows_for(const SweepLaunch& L) {
 // occupancy (2048^2 SR step 2.98 ms with the march, 2.22 ms with 8-column
 // and 2.35 ms with 16-column tiles).
 int y_tile_for(const SweepLaunch& L) {
+    if (L.mask) return L.y_tile == 8 ? 8 : 16;  // masked sweeps: tiles only
     if (L.y_tile == 0 || L.y_tile == 8 || L.y_tile == 16) return L.y_tile;
     return L.physics == 1 ? 8 : 0;
 }
@@ -578,6 +590,14 @@ hipError_t launch_sweep_x(const SweepLaunch& L, hipStream_t s) {
 
 hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s) {
     return L.physics == 0 ? dispatch_axis<NewtonianT, 1>(L, s) : dispatch_axis<RelativisticT, 1>(L, s);
+}
+
+hipError_t launch_mask_writeback(const Planes& dst, const ConstPlanes& src, const unsigned char* mask, int nx, int ny,
+                                 long pitch, hipStream_t s) {
+    const long n = (long)nx * ny;
+    hipLaunchKernelGGL(k_mask_writeback, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, mask, nx, ny,
+                       pitch);
+    return hipGetLastError();
 }
 
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,

@@ -145,11 +145,20 @@ struct SweepLaunch {
     const int* block_order;
     int order_n;
     unsigned char* hot;
+    // AMR masked sweeps (else null): the level's refinement flags ([ny][nx]
+    // bytes).  Only flagged cells are written to out; a block whose tile holds
+    // no flagged output cell returns at once.  The y sweep then runs as LDS
+    // tiles (the column march takes no mask).
+    const unsigned char* mask;
 };
 
 // All return hipSuccess or the launch error.
 hipError_t launch_sweep_x(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
+// dst[c] = src[c] for every cell c whose mask byte is set (the five planes of
+// an nx-by-ny grid of row pitch `pitch`): the write-back of a masked sweep.
+hipError_t launch_mask_writeback(const Planes& dst, const ConstPlanes& src, const unsigned char* mask, int nx, int ny,
+                                 long pitch, hipStream_t s);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);

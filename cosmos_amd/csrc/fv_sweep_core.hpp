@@ -396,7 +396,8 @@ __device__ __forceinline__ double x_cells(const ConstPlanes& in, const Planes& o
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int TC>
 __device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Planes& out, int nx, int ny, long pitch,
                                                int c0, int r0, int ws_dim, const SweepCoef& cf, double bias,
-                                               int limiter, const PhysParams& P, double (*lds)[NT], Ev* acc = nullptr) {
+                                               int limiter, const PhysParams& P, double (*lds)[NT], Ev* acc = nullptr,
+                                               const unsigned char* mask = nullptr) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TR = NT / TC;
     const int s = threadIdx.x;
@@ -405,7 +406,12 @@ __device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Plan
     const int j = c0 + (s - r * TC);
     const int jj = j < nx ? j : nx - 1;
     const int ii = ORDER == 2 ? mirror2(i, ny) : clamp1(i, ny);
-    const bool valid = (r >= G) && (r < TR - G) && (i < ny) && (j < nx);
+    bool valid = (r >= G) && (r < TR - G) && (i < ny) && (j < nx);
+    if (mask) {
+        // masked: only flagged cells are outputs; a tile without one is done
+        valid = valid && mask[(long)i * nx + j] != 0;
+        if (!__syncthreads_or(valid ? 1 : 0)) return 0.0;
+    }
     Own own = own_of(i, ny, valid, r0 == 0, r0 + (TR - 2 * G) >= ny);
     if (j >= nx) own = Own{false, false, false, false, false};
     return line_cells<Phys, ORDER, LIMMODE, W0, NT, 1, TC>(in, out, (long)ii * pitch + jj, (long)i * pitch + j, valid,
@@ -413,17 +419,23 @@ __device__ __forceinline__ double y_tile_cells(const ConstPlanes& in, const Plan
 }
 
 // One row segment of an x sweep: thread s holds cell tile * (NT - 2G) - G + s
-// of the row at rowbase.
+// of the row at rowbase.  mask (masked sweeps, else null): the row's flags;
+// only flagged cells are outputs, and a segment without one returns at once.
 template <class Phys, int ORDER, int LIMMODE, int DIM, bool WS, bool W0, int NT>
 __device__ __forceinline__ double x_segment(const ConstPlanes& in, const Planes& out, int nx, long rowbase, int tile,
                                             const SweepCoef& cf, double bias, int limiter, const PhysParams& P,
-                                            double (*lds)[NT], Ev* acc = nullptr) {
+                                            double (*lds)[NT], Ev* acc = nullptr,
+                                            const unsigned char* mask = nullptr) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TW = NT - 2 * G;
     const int s = threadIdx.x;
     const int c = tile * TW - G + s;
     const int cc = ORDER == 2 ? mirror2(c, nx) : clamp1(c, nx);
-    const bool valid = (s >= G) && (s < NT - G) && (c < nx);
+    bool valid = (s >= G) && (s < NT - G) && (c < nx);
+    if (mask) {
+        valid = valid && mask[c] != 0;
+        if (!__syncthreads_or(valid ? 1 : 0)) return 0.0;
+    }
     const Own own = own_of(c, nx, valid, tile == 0, (tile + 1) * TW >= nx);
     return x_cells<Phys, ORDER, LIMMODE, W0, NT>(in, out, rowbase + cc, rowbase + c, valid, WS ? DIM : 0, cf, bias,
                                                  limiter, P, lds, &own, acc);
