@@ -171,6 +171,17 @@ CFV_HD void flux_keep(const double* U, const Geo& G, const PhysParams& P, C2PRec
     gr_flux_of<AXIS>(c, G, F);
 }
 
+// Two independent fluxes with their records, the Newton solves interleaved
+// (gr_con2prim_rec2): bitwise flux_keep of each.
+template <int AXIS>
+CFV_HD void flux_keep2(const double* Ua, const Geo& Ga, C2PRec& ca, double* Fa, const double* Ub, const Geo& Gb,
+                       C2PRec& cb, double* Fb, const PhysParams& P) {
+    const double tol = gr_flux_tol(P, AXIS);
+    gr_con2prim_rec2(Ua, Ga.sdet, tol, Ub, Gb.sdet, tol, P, ca, cb);
+    gr_flux_of<AXIS>(ca, Ga, Fa);
+    gr_flux_of<AXIS>(cb, Gb, Fb);
+}
+
 // gU += (dS/dU)^T gS for S = gr_source(U, SG)
 CFV_HD void vjp_source(const double* U, const SourceGeo& S, const PhysParams& P, const double* gout, double* gU) {
     C2PRec c;
@@ -307,9 +318,8 @@ CFV_HD void force_fwd_g(const double* WL, const double* WR, const ForceGeo& G, c
                         double* F) {
     double ul[5], ur[5], fl[5], fr[5], us[5], fs[5];
     gr_p2c(WL, G.l, ul, P);
-    gr_flux<AXIS>(ul, G.l, P, fl);
     gr_p2c(WR, G.r, ur, P);
-    gr_flux<AXIS>(ur, G.r, P, fr);
+    gr_flux2<AXIS>(ul, G.l, ur, G.r, P, fl, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (cf.dt_dx * (fl[k] - fr[k])));
     gr_flux<AXIS>(us, G.c, P, fs);
@@ -323,9 +333,8 @@ CFV_HD void vjp_force_g(const double* WL, const double* WR, const ForceGeo& G, c
     double ul[5], ur[5], fl[5], fr[5], us[5];
     C2PRec cl, cr;
     gr_p2c(WL, G.l, ul, P);
-    flux_keep<AXIS>(ul, G.l, P, cl, fl);
     gr_p2c(WR, G.r, ur, P);
-    flux_keep<AXIS>(ur, G.r, P, cr, fr);
+    flux_keep2<AXIS>(ul, G.l, cl, fl, ur, G.r, cr, fr, P);
 #pragma unroll
     for (int k = 0; k < 5; ++k) us[k] = 0.5 * ((ul[k] + ur[k]) + (cf.dt_dx * (fl[k] - fr[k])));
     double gfs[5], gfl[5], gfr[5], gul[5], gur[5], gus[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -379,8 +388,7 @@ CFV_HD void half_fwd_g(const double* WL, const double* WM, const double* WR, int
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    gr_flux<AXIS>(xl, G.fl, P, fl);
-    gr_flux<AXIS>(xr, G.fr, P, fr);
+    gr_flux2<AXIS>(xl, G.fl, xr, G.fr, P, fl, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (cf.hdt_dx * (fl[k] - fr[k]));
     gr_c2p(e, side == 0 ? G.fl.sdet : G.fr.sdet, P, out);
@@ -402,8 +410,7 @@ CFV_HD void vjp_half_g(const double* WL, const double* WM, const double* WR, int
         xr[k] = um[k] + (0.5 * s[k]);
     }
     C2PRec cl, cr;
-    flux_keep<AXIS>(xl, G.fl, P, cl, fl);
-    flux_keep<AXIS>(xr, G.fr, P, cr, fr);
+    flux_keep2<AXIS>(xl, G.fl, cl, fl, xr, G.fr, cr, fr, P);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = (side == 0 ? xl[k] : xr[k]) + (cf.hdt_dx * (fl[k] - fr[k]));
     double ge[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -462,8 +469,7 @@ CFV_HD void frac_fwd_g(double fdt_h, const double* WL, const double* WM, const d
         xl[k] = um[k] - (0.5 * s[k]);
         xr[k] = um[k] + (0.5 * s[k]);
     }
-    gr_flux<AXIS>(xl, gl, P, fl);
-    gr_flux<AXIS>(xr, gr, P, fr);
+    gr_flux2<AXIS>(xl, gl, xr, gr, P, fl, fr);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = um[k] + (fdt_h * (fl[k] - fr[k]));
     gr_c2p(e, gc.sdet, P, out);
@@ -485,8 +491,7 @@ CFV_HD void vjp_frac_g(double fdt_h, const double* WL, const double* WM, const d
         xr[k] = um[k] + (0.5 * s[k]);
     }
     C2PRec cl, cr;
-    flux_keep<AXIS>(xl, gl, P, cl, fl);
-    flux_keep<AXIS>(xr, gr, P, cr, fr);
+    flux_keep2<AXIS>(xl, gl, cl, fl, xr, gr, cr, fr, P);
 #pragma unroll
     for (int k = 0; k < 5; ++k) e[k] = um[k] + (fdt_h * (fl[k] - fr[k]));
     double ge[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
