@@ -403,10 +403,26 @@ __global__ __launch_bounds__(kAmrThreads) void k_fill(u8* a, long n, u8 v) {
 }
 
 // flag[0] = 1 if any byte of a is nonzero (the caller zeroes flag first).
+// a is 16-byte aligned (a device allocation); 16 bytes per thread per pass
+// over a grid-stride loop, then the tail byte by byte.
 __global__ __launch_bounds__(kAmrThreads) void k_any(const u8* a, long n, int* flag) {
-    long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
-    bool v = idx < n && a[idx];
+    const long stride = (long)gridDim.x * kAmrThreads;
+    const long t = (long)blockIdx.x * kAmrThreads + threadIdx.x;
+    const long n16 = n / 16;
+    const uint4* q = reinterpret_cast<const uint4*>(a);
+    bool v = false;
+    for (long k = t; k < n16; k += stride) {
+        const uint4 w = q[k];
+        v = v || ((w.x | w.y | w.z | w.w) != 0u);
+    }
+    for (long k = n16 * 16 + t; k < n; k += stride) v = v || a[k] != 0;
     if (__any(v) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+// Blocks of a k_any launch over n bytes (at most 2048; the loop covers the rest).
+dim3 any_blocks(long n) {
+    const long b = (n / 16 + kAmrThreads - 1) / kAmrThreads;
+    return dim3((unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b)));
 }
 
 // Initial fine state: every child a copy of its coarse cell (the loops before
@@ -822,7 +838,7 @@ int amr_any_set(cosmos_amr* a, const unsigned char* flags, long n, bool& out) {
     hipStream_t s = order_streams(a);
     int zero = 0, v = 0;
     AMRCHK(a, hipMemcpyAsync(a->any, &zero, sizeof(int), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(cfv::k_any, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, s, flags, n, a->any);
+    hipLaunchKernelGGL(cfv::k_any, cfv::any_blocks(n), dim3(cfv::kAmrThreads), 0, s, flags, n, a->any);
     AMRCHK(a, hipGetLastError());
     AMRCHK(a, hipMemcpyAsync(&v, a->any, sizeof(int), hipMemcpyDeviceToHost, s));
     AMRCHK(a, hipStreamSynchronize(s));

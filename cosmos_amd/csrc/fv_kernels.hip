@@ -479,14 +479,35 @@ __global__ __launch_bounds__(256) void k_zero_slots(unsigned long long* slots) {
     for (int k = threadIdx.x; k < kSlotWords; k += blockDim.x) slots[k] = 0ull;
 }
 
+// A wave per 1024 consecutive cells: each lane tests 16 of their mask bytes
+// with one 16-byte read (mask is a device allocation, 16-byte aligned) and a
+// wave with no flagged cell leaves at once; otherwise the wave copies its
+// cells 64 consecutive ones at a time, each lane the five values of its cell
+// where flagged (coalesced rows).
 __global__ __launch_bounds__(256) void k_mask_writeback(Planes dst, ConstPlanes src, const unsigned char* mask,
                                                        int nx, int ny, long pitch) {
-    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (long)nx * ny || !mask[idx]) return;
-    const long i = idx / nx, j = idx - i * nx;
-    const long c = i * pitch + j;
+    const long n = (long)nx * ny;
+    const int lane = threadIdx.x & 63;
+    const long wbase = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * 1024;
+    if (wbase >= n) return;
+    const long b = wbase + 16L * lane;
+    bool any = false;
+    if (b + 16 <= n) {
+        const uint4 w = *reinterpret_cast<const uint4*>(mask + b);
+        any = (w.x | w.y | w.z | w.w) != 0u;
+    } else {
+        for (long e = b; e < n; ++e) any = any || mask[e] != 0;
+    }
+    if (!__any(any)) return;
+#pragma unroll 1
+    for (int e = 0; e < 16; ++e) {
+        const long idx = wbase + 64L * e + lane;
+        if (idx >= n || !mask[idx]) continue;
+        const long i = idx / nx, j = idx - i * nx;
+        const long c = i * pitch + j;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) dst.p[k][c] = src.p[k][c];
+        for (int k = 0; k < 5; ++k) dst.p[k][c] = src.p[k][c];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag) {
@@ -595,8 +616,8 @@ hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s) {
 hipError_t launch_mask_writeback(const Planes& dst, const ConstPlanes& src, const unsigned char* mask, int nx, int ny,
                                  long pitch, hipStream_t s) {
     const long n = (long)nx * ny;
-    hipLaunchKernelGGL(k_mask_writeback, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dst, src, mask, nx, ny,
-                       pitch);
+    const long blocks = (n + 4095) / 4096;  // 4 waves of 1024 cells
+    hipLaunchKernelGGL(k_mask_writeback, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, mask, nx, ny, pitch);
     return hipGetLastError();
 }
 
