@@ -692,14 +692,25 @@ hipError_t launch_step_fused(const SweepLaunch& L, hipStream_t s) {
 // slots at a time (ballot prefix counts within each wave, wave totals in LDS).
 constexpr int kOrderThreads = 1024;
 
-__global__ __launch_bounds__(kOrderThreads) void k_hot_order(unsigned char* hot, int* order, int n) {
+// With row > 0 a block counts as marked when it or one of its four neighbours
+// in the launch's block grid (row width `row` blocks) was: the unphysical
+// states whose solves run to a long cap sit at shocks that move less than a
+// cell per step, so they cross into a neighbouring block from one step to the
+// next.  row == 0: the block's own mark only.
+__global__ __launch_bounds__(kOrderThreads) void k_hot_order(unsigned char* hot, int* order, int n, int row) {
     __shared__ int wave_hot[kOrderThreads / 64], wave_cold[kOrderThreads / 64];
     __shared__ int total_hot;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    auto marked = [&](int b) {
+        if (hot[b]) return true;
+        if (row <= 0) return false;
+        return (b > 0 && hot[b - 1]) || (b + 1 < n && hot[b + 1]) || (b >= row && hot[b - row]) ||
+               (b + row < n && hot[b + row]);
+    };
     if (tid == 0) total_hot = 0;
     __syncthreads();
     int mine = 0;
-    for (int k = tid; k < n; k += kOrderThreads) mine += hot[k] ? 1 : 0;
+    for (int k = tid; k < n; k += kOrderThreads) mine += marked(k) ? 1 : 0;
     if (mine) atomicAdd(&total_hot, mine);
     __syncthreads();
     const int nhot = total_hot;
@@ -708,7 +719,7 @@ __global__ __launch_bounds__(kOrderThreads) void k_hot_order(unsigned char* hot,
         const int slot = c + tid;
         const bool valid = slot < n;
         const int b = valid ? xcd_tile(slot, n) : 0;
-        const bool h = valid && hot[b];
+        const bool h = valid && marked(b);
         const bool cold = valid && !h;
         const unsigned long long mh = __ballot(h), mc = __ballot(cold);
         const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -734,9 +745,18 @@ __global__ __launch_bounds__(kOrderThreads) void k_hot_order(unsigned char* hot,
     for (int k = tid; k < n; k += kOrderThreads) hot[k] = 0;
 }
 
-hipError_t launch_hot_order(unsigned char* hot, int* order, int n, hipStream_t s) {
-    hipLaunchKernelGGL(k_hot_order, dim3(1), dim3(kOrderThreads), 0, s, hot, order, n);
+hipError_t launch_hot_order(unsigned char* hot, int* order, int n, int row, hipStream_t s) {
+    hipLaunchKernelGGL(k_hot_order, dim3(1), dim3(kOrderThreads), 0, s, hot, order, n, row < 0 ? 0 : row);
     return hipGetLastError();
+}
+
+int sweep_row_blocks(const SweepLaunch& L, int axis) {
+    if (axis == 0) {
+        const int TW = kXThreads - 2 * (L.order == 2 ? 2 : 1);
+        return (L.nx + TW - 1) / TW;
+    }
+    const int tc = y_tile_for(L);
+    return tc == 8 || tc == 16 ? (L.nx + tc - 1) / tc : 0;
 }
 
 int sweep_x_grid(const SweepLaunch& L) {

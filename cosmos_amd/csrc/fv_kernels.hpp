@@ -164,7 +164,12 @@ hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, i
 hipError_t launch_step_begin(StepState* st, hipStream_t s);
 // The dispatch order of an n-block sweep from its blocks' marks: the marked
 // blocks first, then the rest in the XCD-contiguous order; clears the marks.
-hipError_t launch_hot_order(unsigned char* hot, int* order, int n, hipStream_t s);
+// row > 0: a block also counts as marked when a neighbour in the block grid of
+// row width `row` was (see k_hot_order); 0: its own mark only.
+hipError_t launch_hot_order(unsigned char* hot, int* order, int n, int row, hipStream_t s);
+// Blocks per row of the block grid of an x sweep (axis 0) or of a y-tile sweep
+// (axis 1; 0 when the y sweep is the column march).
+int sweep_row_blocks(const SweepLaunch& L, int axis);
 // Blocks of the sweeps launch_sweep_x / launch_sweep_y run for L (0 when the
 // y sweep is the column march, which takes no order).
 int sweep_x_grid(const SweepLaunch& L);

@@ -46,8 +46,14 @@ struct PhysParams {
 // functions take an optional Ev* and count into it; null counts nothing.
 struct Ev {
     unsigned cap, q, rho, p;
-    unsigned hit;  // any solve of the thread ran to the cap, owned or not (block scheduling only)
+    unsigned hit;   // any solve of the thread ran long (see slow), owned or not (block scheduling only)
+    unsigned slow;  // solves that took more than kSlowSolve iterations (block scheduling only)
 };
+
+// Newton iterations beyond which a special-relativistic con2prim counts as a
+// slow solve for the solve loop's block order (normal solves converge in a
+// handful; states with no root run to the cap).
+constexpr int kSlowSolve = 30;
 
 CFV_HD void ev_add(Ev& a, const Ev& b, bool on) {
     if (on) {
@@ -233,6 +239,7 @@ struct RelativisticT {
         h_out = h;
         if (ev) {
             ev->cap += it == P.c2p_maxit ? 1u : 0u;
+            ev->slow += it > kSlowSolve ? 1u : 0u;
             ev->q += q < 1e-8 ? 1u : 0u;
             ev->rho += rho < 1e-8 ? 1u : 0u;
             ev->p += p < 1e-8 ? 1u : 0u;

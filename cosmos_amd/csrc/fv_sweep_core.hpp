@@ -283,6 +283,7 @@ __device__ __forceinline__ void line_step(const double* W, double* Wn, const Swe
     update_cell<Phys, W0>(U, Fl, F, cf, P, Wn, pu);
     if (kCount && acc) {
         acc->hit += ((eh.cap + eg0.cap) + (eg1.cap + ei.cap)) + eu.cap;
+        acc->hit += ((eh.slow + eg0.slow) + (eg1.slow + ei.slow)) + eu.slow;
         // FORCE: the cell's own flux counts where SLIC counts its half step
         ev_add(*acc, eh, own->half);
         ev_add(*acc, eg0, own->g0);
