@@ -419,6 +419,10 @@ __global__ __launch_bounds__(kAmrThreads) void k_any(const u8* a, long n, int* f
     if (__any(v) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
 }
 
+__global__ __launch_bounds__(64) void k_zero_int(int* p) {
+    if (threadIdx.x == 0) *p = 0;
+}
+
 // Blocks of a k_any launch over n bytes (at most 2048; the loop covers the rest).
 dim3 any_blocks(long n) {
     const long b = (n / 16 + kAmrThreads - 1) / kAmrThreads;
@@ -579,6 +583,7 @@ struct cosmos_amr {
     unsigned char* fflags = nullptr;  // device, [2ny][2nx] (1D: [2nx])
     unsigned char* f2flags = nullptr; // device, [4nx]
     int* any = nullptr;               // device scratch flag
+    int* any_host = nullptr;          // page-locked host copy of it
     cfv::PhysParams P{};
     cfv::Hole hole = cfv::make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);  // COSMOS_BLACKHOLE (the solvers' default)
     double excision = 0.0;
@@ -836,13 +841,12 @@ int amr_flux_step(cosmos_amr* a, int level, int order, double dx, double dt) {
 
 int amr_any_set(cosmos_amr* a, const unsigned char* flags, long n, bool& out) {
     hipStream_t s = order_streams(a);
-    int zero = 0, v = 0;
-    AMRCHK(a, hipMemcpyAsync(a->any, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(cfv::k_zero_int, dim3(1), dim3(64), 0, s, a->any);
     hipLaunchKernelGGL(cfv::k_any, cfv::any_blocks(n), dim3(cfv::kAmrThreads), 0, s, flags, n, a->any);
     AMRCHK(a, hipGetLastError());
-    AMRCHK(a, hipMemcpyAsync(&v, a->any, sizeof(int), hipMemcpyDeviceToHost, s));
+    AMRCHK(a, hipMemcpyAsync(a->any_host, a->any, sizeof(int), hipMemcpyDeviceToHost, s));
     AMRCHK(a, hipStreamSynchronize(s));
-    out = v != 0;
+    out = *a->any_host != 0;
     return COSMOS_OK;
 }
 
@@ -921,6 +925,7 @@ int cosmos_amr_create(cosmos_amr** out, int physics, int dim, int nx, int ny, do
     if (!rc) e = hipMalloc((void**)&a->cflags, (size_t)coarse_cells(a));
     if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->fflags, (size_t)fine_cells(a));
     if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->any, sizeof(int));
+    if (!rc && e == hipSuccess) e = hipHostMalloc((void**)&a->any_host, sizeof(int));
     if (rc || e != hipSuccess) {
         std::string msg = rc ? std::string(cosmos_last_error(nullptr)) : std::string(hipGetErrorString(e));
         cosmos_amr_destroy(a);
@@ -943,6 +948,7 @@ int cosmos_amr_destroy(cosmos_amr* a) {
     hipFree(a->fflags);
     hipFree(a->f2flags);
     hipFree(a->any);
+    hipHostFree(a->any_host);
     // the finer levels queue on the coarse handle's stream: they go first
     cosmos_destroy(a->fine2);
     cosmos_destroy(a->fine);

@@ -44,6 +44,7 @@ struct cosmos_solver {
     unsigned long long* st_slots = nullptr;    // device, max wave speed slots of the solve loop
     double* st_part = nullptr;                 // device, per-block maxima of the solve loop's last sweep
     unsigned long long* ws_scratch = nullptr;  // device, slots for cosmos_max_wave_speed
+    unsigned long long* ws_host = nullptr;     // page-locked host copy of those slots (read back by DMA)
     int* flag = nullptr;                       // device, scratch flag
     unsigned long long* status = nullptr;      // device, the status block (cosmos_status_t layout, 8 words)
     int excised_known = 0;                     // black hole: excised_all computed for excised_key
@@ -600,6 +601,7 @@ int cosmos_create(cosmos_solver** out, int physics, int dim, int nx, int ny, dou
     if (e == hipSuccess) e = hipMalloc((void**)&s->st, sizeof(StepState));
     if (e == hipSuccess) e = hipMalloc((void**)&s->st_slots, sizeof(unsigned long long) * cfv::kSlotWords);
     if (e == hipSuccess) e = hipMalloc((void**)&s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&s->ws_host, sizeof(unsigned long long) * cfv::kSlotWords);
     if (e == hipSuccess) e = hipMalloc((void**)&s->flag, sizeof(int));
     if (e == hipSuccess) e = hipMalloc((void**)&s->status, sizeof(unsigned long long) * kStatusWords);
     if (e == hipSuccess)
@@ -635,6 +637,7 @@ int cosmos_destroy(cosmos_solver* s) {
     hipFree(s->st_slots);
     hipFree(s->st_part);
     hipFree(s->ws_scratch);
+    hipHostFree(s->ws_host);
     hipFree(s->flag);
     hipFree(s->status);
     hipFree(s->stage);
@@ -1032,11 +1035,10 @@ int cosmos_max_wave_speed(cosmos_solver* s, double* out) {
         if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "out is null");
         int rc = max_ws_impl(s, s->ws_scratch);
         if (rc) return rc;
-        std::vector<unsigned long long> words(cfv::kSlotWords);
-        HIPCHK(s, hipMemcpyAsync(words.data(), s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords,
+        HIPCHK(s, hipMemcpyAsync(s->ws_host, s->ws_scratch, sizeof(unsigned long long) * cfv::kSlotWords,
                                  hipMemcpyDeviceToHost, s->stream));
         HIPCHK(s, hipStreamSynchronize(s->stream));
-        *out = cfv::slots_max_host(words.data());
+        *out = cfv::slots_max_host(s->ws_host);
         return COSMOS_OK;
     });
 }
