@@ -30,8 +30,11 @@
 namespace cfv {
 // solver internals for the AMR levels (fv_capi.hip)
 int amr_masked_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
-                     int slope_limiter, const unsigned char* mask);
-hipError_t amr_mask_writeback(cosmos_solver* s, const unsigned char* mask);
+                     int slope_limiter, const unsigned char* mask, const int* blocks, int list_n);
+hipError_t amr_mask_writeback(cosmos_solver* s, const unsigned char* mask, const int* segs, int n_segs);
+int amr_mask_tc(cosmos_solver* s);
+hipError_t amr_mask_lists(cosmos_solver* s, const unsigned char* mask, int order, unsigned char* hit, int* xl,
+                          int* yl, int* wl, int* counts);
 int amr_share_stream(cosmos_solver* s, cosmos_solver* owner);
 }  // namespace cfv
 
@@ -345,16 +348,17 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, Fr
     int i = (int)(idx / c.nx), j = (int)(idx - (long)i * c.nx);
     const int dim = Fp.dim;
     const double t2 = tol * 0.5;
+    // only a refined cell can derefine: the flag first, the jump test after
+    const long first = dim == 1 ? 2L * j : (long)(2 * i) * f.nx + 2 * j;
+    if (!fflags[first]) return;
     double um[5], un[5];
     bool marked;
-    long first;
     if (dim == 1) {
         cons_at<Phys>(f, 0, 2 * j + 1, Fc, At{2 * j, 0}, P, um);
         cons_at<Phys>(f, 0, 2 * j, Fc, At{2 * j - 1, 0}, P, un);
         marked = jump(un, um, dx, t2);
         cons_at<Phys>(f, 0, clampc(2 * j + 2, f.nx), Fc, At{2 * j + 1, 0}, P, un);
         marked |= jump(um, un, dx, t2);
-        first = 2 * j;
     } else {
         const int fi = 2 * i + 1, fj = 2 * j + 1;
         cons_at<Phys>(f, fi, fj, Fc, At{fj, fi}, P, um);
@@ -366,9 +370,8 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_derefine(Grid c, Grid f, Fr
         marked |= jump(un, um, dx, t2);
         cons_at<Phys>(f, fi, clampc(fj + 1, f.nx), Fc, At{fj + 1, fi}, P, un);
         marked |= jump(um, un, dx, t2);
-        first = (long)(2 * i) * f.nx + 2 * j;
     }
-    if (!marked && fflags[first]) {
+    if (!marked) {
         if (dim == 1) {
             fflags[2 * j] = 0;
             fflags[2 * j + 1] = 0;
@@ -400,33 +403,6 @@ __global__ __launch_bounds__(kAmrThreads) void k_mask_merge(Grid before, Grid af
 __global__ __launch_bounds__(kAmrThreads) void k_fill(u8* a, long n, u8 v) {
     long idx = (long)blockIdx.x * kAmrThreads + threadIdx.x;
     if (idx < n) a[idx] = v;
-}
-
-// flag[0] = 1 if any byte of a is nonzero (the caller zeroes flag first).
-// a is 16-byte aligned (a device allocation); 16 bytes per thread per pass
-// over a grid-stride loop, then the tail byte by byte.
-__global__ __launch_bounds__(kAmrThreads) void k_any(const u8* a, long n, int* flag) {
-    const long stride = (long)gridDim.x * kAmrThreads;
-    const long t = (long)blockIdx.x * kAmrThreads + threadIdx.x;
-    const long n16 = n / 16;
-    const uint4* q = reinterpret_cast<const uint4*>(a);
-    bool v = false;
-    for (long k = t; k < n16; k += stride) {
-        const uint4 w = q[k];
-        v = v || ((w.x | w.y | w.z | w.w) != 0u);
-    }
-    for (long k = n16 * 16 + t; k < n; k += stride) v = v || a[k] != 0;
-    if (__any(v) && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
-}
-
-__global__ __launch_bounds__(64) void k_zero_int(int* p) {
-    if (threadIdx.x == 0) *p = 0;
-}
-
-// Blocks of a k_any launch over n bytes (at most 2048; the loop covers the rest).
-dim3 any_blocks(long n) {
-    const long b = (n / 16 + kAmrThreads - 1) / kAmrThreads;
-    return dim3((unsigned)(b < 1 ? 1 : (b > 2048 ? 2048 : b)));
 }
 
 // Initial fine state: every child a copy of its coarse cell (the loops before
@@ -513,17 +489,21 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, Fra
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
     const int dim = Fp.dim;
+    const bool restrict_here = !pflags[idx] && (!strict || children_all(cflags, c, dim, i, j));
+    const bool inject = !strict || pflags[idx];
+    if (!restrict_here && !inject) return;  // strict, parent clear: nothing written
     double W[5];
-    if (!pflags[idx] && (!strict || children_all(cflags, c, dim, i, j))) {
+    if (restrict_here) {
         restrict_children<Phys>(c, dim, i, j, Fc, Fp, P, W);
         store(p, i, j, W);
     } else {
         load(p, i, j, W);
     }
+    if (!inject) return;
     for (int a = 0; a < (dim == 1 ? 1 : 2); ++a)
         for (int b = 0; b < 2; ++b) {
             const int ci = dim == 1 ? 0 : 2 * i + a, cj = 2 * j + b;
-            if (!cflags[(long)ci * c.nx + cj] && (!strict || pflags[idx])) store(c, ci, cj, W);
+            if (!cflags[(long)ci * c.nx + cj]) store(c, ci, cj, W);
         }
 }
 
@@ -538,6 +518,8 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_derefine(Grid p, Grid c, Fra
     if (idx >= (long)p.nx * p.ny) return;
     int i = (int)(idx / p.nx), j = (int)(idx - (long)i * p.nx);
     const int dim = Fp.dim;
+    // only a parent whose children are all flagged can derefine: that first
+    if (!children_all(cflags, c, dim, i, j)) return;
     double um[5], un[5];
     bool marked;
     if (dim == 1) {
@@ -558,7 +540,7 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_derefine(Grid p, Grid c, Fra
         cons_at<Phys>(c, fi, clampc(fj + 1, c.nx), Fc, At{fj + 1, fi}, P, un);
         marked |= jump(um, un, dx, tol);
     }
-    if (!marked && children_all(cflags, c, dim, i, j)) {
+    if (!marked) {
         children_set(cflags, c, dim, i, j, 0);
         pflags[idx] = 1;
         double W[5];
@@ -574,6 +556,25 @@ dim3 blocks_of(long n) { return dim3((unsigned)((n + kAmrThreads - 1) / kAmrThre
 
 using cfv::Grid;
 
+// The blocks of a finer level's masked launches that hold a flagged cell
+// (x-sweep row segments, y-sweep tiles, 64-cell write-back segments:
+// launch_mask_lists),
+// built from the level's flags after their last change.  The build is queued
+// ahead of the host wait for the level's time step, whose copy brings the
+// three counts back with it; the masked sweeps and write-backs then launch
+// only the listed blocks.
+struct MaskLists {
+    int* blocks[3] = {nullptr, nullptr, nullptr};  // device: x segments, y tiles, write-back segments
+    long cap[3] = {0, 0, 0};                        // entries allocated
+    unsigned char* hit = nullptr;                   // device: the builder's mark maps (all zero between builds)
+    long hit_cap = 0;                               // their bytes
+    int n[3] = {0, 0, 0};                           // entries listed (once valid)
+    int order = 0, tc = 0;                          // the sweeps' order and y-tile width they were built for
+    bool queued = false;                            // built (or being built) on the device, counts not read
+    bool valid = false;                             // counts read, flags unchanged since the build
+    hipEvent_t copied = nullptr;                    // recorded after the counts' copy to the host
+};
+
 struct cosmos_amr {
     int physics = 0, dim = 2, nx = 0, ny = 0;
     cosmos_solver* coarse = nullptr;
@@ -581,9 +582,10 @@ struct cosmos_amr {
     cosmos_solver* fine2 = nullptr;   // 1D three-level runs: the 4nx-cell level (lazily created)
     unsigned char* cflags = nullptr;  // device, [ny][nx]
     unsigned char* fflags = nullptr;  // device, [2ny][2nx] (1D: [2nx])
-    unsigned char* f2flags = nullptr; // device, [4nx]
-    int* any = nullptr;               // device scratch flag
-    int* any_host = nullptr;          // page-locked host copy of it
+    unsigned char* f2flags = nullptr; // device, [4ny][4nx] (1D: [4nx])
+    MaskLists lists[3];               // levels 1 and 2 (index by level)
+    int* counts = nullptr;            // device, the lists' counts (three per level)
+    int* counts_host = nullptr;       // page-locked host copy of them
     cfv::PhysParams P{};
     cfv::Hole hole = cfv::make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);  // COSMOS_BLACKHOLE (the solvers' default)
     double excision = 0.0;
@@ -643,6 +645,7 @@ long fine_cells(const cosmos_amr* a) { return (long)(2 * a->nx) * (a->dim == 1 ?
 // levels or the flags are ordered after each level's work without a host
 // synchronisation: they go on that stream too.
 hipStream_t order_streams(cosmos_amr* a) { return (hipStream_t)cosmos_stream(a->coarse); }
+void lists_changed(cosmos_amr* a);
 
 
 template <class F>
@@ -733,6 +736,7 @@ int amr_refine(cosmos_amr* a, double dx, double tol) {
     int rc = grid_of(a, a->coarse, c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
+    lists_changed(a);
     // the special-relativistic and black-hole 2D drivers read the diagonal cell as "bottom"
     const int diag = a->physics == COSMOS_NEWTONIAN ? 0 : 1;
     if (a->physics == COSMOS_BLACKHOLE) {
@@ -780,6 +784,7 @@ int amr_derefine(cosmos_amr* a, double dx, double tol) {
     if (!rc) rc = grid_of(a, a->fine, f);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
+    lists_changed(a);
     AMRCHK(a, amr_timed(a, kAmrProfDerefine, s, [&] {
                return launch_physics<DerefineK>(a->physics, cfv::blocks_of(coarse_cells(a)), s, c, f,
                                                 frame(a, dx * 0.5), frame(a, dx), spacing_of(dx), tol, a->P,
@@ -795,21 +800,93 @@ unsigned char* level_flags(cosmos_amr* a, int level) {
     return level == 0 ? a->cflags : level == 1 ? a->fflags : a->f2flags;
 }
 
+// Any change of the flags of any level: every level's block lists are stale.
+void lists_changed(cosmos_amr* a) {
+    for (MaskLists& M : a->lists) {
+        M.valid = false;
+        M.queued = false;
+    }
+}
+
+// Queue the build of level's block lists (1 or 2) for sweeps of this order
+// from its current flags, with the copy of their counts to the host.
+int lists_queue(cosmos_amr* a, int level, int order) {
+    MaskLists& M = a->lists[level];
+    cosmos_solver* s = level_solver(a, level);
+    const int m = level == 1 ? 2 : 4;
+    const int nx = m * a->nx, ny = a->dim == 1 ? 1 : m * a->ny;
+    const int tc = cfv::amr_mask_tc(s);
+    long sz[3];
+    cfv::mask_list_sizes(nx, ny, a->dim, order, tc, sz);
+    hipStream_t st = order_streams(a);
+    for (int k = 0; k < 3; ++k) {
+        if (M.cap[k] >= sz[k] && M.blocks[k]) continue;
+        // grows once per level (and order): the old list may still be read by queued work
+        AMRCHK(a, hipStreamSynchronize(st));
+        hipFree(M.blocks[k]);
+        M.blocks[k] = nullptr;
+        M.cap[k] = 0;
+        const long want = sz[k] > 0 ? sz[k] : 1;
+        AMRCHK(a, hipMalloc((void**)&M.blocks[k], sizeof(int) * (size_t)want));
+        M.cap[k] = want;
+    }
+    const long hit_bytes = sz[0] + sz[1] + sz[2];
+    if (M.hit_cap < hit_bytes || !M.hit) {
+        AMRCHK(a, hipStreamSynchronize(st));
+        hipFree(M.hit);
+        M.hit = nullptr;
+        M.hit_cap = 0;
+        AMRCHK(a, hipMalloc((void**)&M.hit, (size_t)hit_bytes));
+        M.hit_cap = hit_bytes;
+        hipLaunchKernelGGL(cfv::k_fill, cfv::blocks_of(hit_bytes), dim3(cfv::kAmrThreads), 0, st, M.hit, hit_bytes,
+                           (unsigned char)0);
+        AMRCHK(a, hipGetLastError());
+    }
+    if (!M.copied) AMRCHK(a, hipEventCreateWithFlags(&M.copied, hipEventDisableTiming));
+    int* dev = a->counts + 3 * level;
+    AMRCHK(a, cfv::amr_mask_lists(s, level_flags(a, level), order, M.hit, M.blocks[0], M.blocks[1], M.blocks[2],
+                                  dev));
+    AMRCHK(a, hipMemcpyAsync(a->counts_host + 3 * level, dev, 3 * sizeof(int), hipMemcpyDeviceToHost, st));
+    AMRCHK(a, hipEventRecord(M.copied, st));
+    M.order = order;
+    M.tc = tc;
+    M.queued = true;
+    M.valid = false;
+    return COSMOS_OK;
+}
+
+// The counts of level's queued lists (normally already copied by the host
+// wait that followed the build).
+int lists_take(cosmos_amr* a, int level) {
+    MaskLists& M = a->lists[level];
+    if (!M.queued) return COSMOS_OK;
+    AMRCHK(a, hipEventSynchronize(M.copied));
+    for (int k = 0; k < 3; ++k) M.n[k] = a->counts_host[3 * level + k];
+    M.queued = false;
+    M.valid = true;
+    return COSMOS_OK;
+}
+
 // One sweep of a level restricted to that level's flags, in one of two forms
 // with the same result.  The finer levels, whose flags mark the refined region
 // (usually a small part of the level): only the tiles holding a flagged cell
 // are swept and only flagged cells written, then written back into the level's
-// state (Newtonian and special relativistic).  The coarse level, whose flags
+// state (Newtonian and special relativistic); with the level's block lists
+// current, only the listed blocks are launched.  The coarse level, whose flags
 // mark every cell outside the refined region, and black-hole levels: the
 // level's full sweep, then the unflagged cells restored.
 int amr_sweep(cosmos_amr* a, int level, int order, int axis, double dx, double dt, double bias, int lim) {
     cosmos_solver* s = level_solver(a, level);
     if (a->physics != COSMOS_BLACKHOLE && level > 0) {
         const unsigned char* mask = level_flags(a, level);
-        int rc = cfv::amr_masked_sweep(s, order, axis, dx, dt, bias, lim, mask);
+        const MaskLists& M = a->lists[level];
+        const bool listed = M.valid && M.order == order && M.tc == cfv::amr_mask_tc(s);
+        int rc = cfv::amr_masked_sweep(s, order, axis, dx, dt, bias, lim, mask, listed ? M.blocks[axis] : nullptr,
+                                       listed ? M.n[axis] : 0);
         if (rc) return from_solver(a, s, rc);
-        AMRCHK(a, amr_timed(a, kAmrProfMerge, (hipStream_t)cosmos_stream(s),
-                            [&] { return cfv::amr_mask_writeback(s, mask); }));
+        AMRCHK(a, amr_timed(a, kAmrProfMerge, (hipStream_t)cosmos_stream(s), [&] {
+                   return cfv::amr_mask_writeback(s, mask, listed ? M.blocks[2] : nullptr, listed ? M.n[2] : 0);
+               }));
         return COSMOS_OK;
     }
     Grid before, after;
@@ -839,26 +916,14 @@ int amr_flux_step(cosmos_amr* a, int level, int order, double dx, double dt) {
     return rc;
 }
 
-int amr_any_set(cosmos_amr* a, const unsigned char* flags, long n, bool& out) {
-    hipStream_t s = order_streams(a);
-    hipLaunchKernelGGL(cfv::k_zero_int, dim3(1), dim3(64), 0, s, a->any);
-    hipLaunchKernelGGL(cfv::k_any, cfv::any_blocks(n), dim3(cfv::kAmrThreads), 0, s, flags, n, a->any);
-    AMRCHK(a, hipGetLastError());
-    AMRCHK(a, hipMemcpyAsync(a->any_host, a->any, sizeof(int), hipMemcpyDeviceToHost, s));
-    AMRCHK(a, hipStreamSynchronize(s));
-    out = *a->any_host != 0;
-    return COSMOS_OK;
-}
-
-int amr_any_refined(cosmos_amr* a, bool& out) { return amr_any_set(a, a->fflags, fine_cells(a), out); }
-
-// The 1D parent / child passes of the three-level driver (parent level lp).
+// The parent / child passes of the three-level driver (parent level lp).
 int l2_refine(cosmos_amr* a, int lp, double dx, double tol, int only_flagged) {
     Grid p, c;
     int rc = grid_of(a, level_solver(a, lp), p);
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
+    lists_changed(a);
     AMRCHK(a, amr_timed(a, kAmrProfRefine, s, [&] {
                return launch_physics<L2RefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
                                                 frame(a, level_spacing(dx, lp)), level_flags(a, lp),
@@ -888,6 +953,7 @@ int l2_derefine(cosmos_amr* a, int lp, double dx, double tol) {
     if (!rc) rc = grid_of(a, level_solver(a, lp + 1), c);
     if (rc) return rc;
     hipStream_t s = order_streams(a);
+    lists_changed(a);
     AMRCHK(a, amr_timed(a, kAmrProfDerefine, s, [&] {
                return launch_physics<L2DerefineK>(a->physics, cfv::blocks_of((long)p.nx * p.ny), s, p, c,
                                                   frame(a, level_spacing(dx, lp + 1)), frame(a, level_spacing(dx, lp)),
@@ -924,8 +990,8 @@ int cosmos_amr_create(cosmos_amr** out, int physics, int dim, int nx, int ny, do
     hipError_t e = hipSuccess;
     if (!rc) e = hipMalloc((void**)&a->cflags, (size_t)coarse_cells(a));
     if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->fflags, (size_t)fine_cells(a));
-    if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->any, sizeof(int));
-    if (!rc && e == hipSuccess) e = hipHostMalloc((void**)&a->any_host, sizeof(int));
+    if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->counts, 9 * sizeof(int));
+    if (!rc && e == hipSuccess) e = hipHostMalloc((void**)&a->counts_host, 9 * sizeof(int));
     if (rc || e != hipSuccess) {
         std::string msg = rc ? std::string(cosmos_last_error(nullptr)) : std::string(hipGetErrorString(e));
         cosmos_amr_destroy(a);
@@ -947,8 +1013,13 @@ int cosmos_amr_destroy(cosmos_amr* a) {
     hipFree(a->cflags);
     hipFree(a->fflags);
     hipFree(a->f2flags);
-    hipFree(a->any);
-    hipHostFree(a->any_host);
+    for (MaskLists& M : a->lists) {
+        for (int k = 0; k < 3; ++k) hipFree(M.blocks[k]);
+        hipFree(M.hit);
+        if (M.copied) hipEventDestroy(M.copied);
+    }
+    hipFree(a->counts);
+    hipHostFree(a->counts_host);
     // the finer levels queue on the coarse handle's stream: they go first
     cosmos_destroy(a->fine2);
     cosmos_destroy(a->fine);
@@ -1046,6 +1117,7 @@ int cosmos_amr_upload(cosmos_amr* a, const double* prim) {
         if (!rc) rc = grid_of(a, a->fine, f);
         if (rc) return rc;
         hipStream_t s = order_streams(a);
+        lists_changed(a);
         hipLaunchKernelGGL(cfv::k_inject_all, cfv::blocks_of(coarse_cells(a)), dim3(cfv::kAmrThreads), 0, s, c, f,
                            a->dim);
         hipLaunchKernelGGL(cfv::k_fill, cfv::blocks_of(coarse_cells(a)), dim3(cfv::kAmrThreads), 0, s, a->cflags,
@@ -1079,6 +1151,7 @@ int cosmos_amr_download(cosmos_amr* a, double* coarse, double* fine, unsigned ch
 int cosmos_amr_set_flags(cosmos_amr* a, const unsigned char* coarse_flags, const unsigned char* fine_flags) {
     return amr_guarded(a, [&] {
         hipStream_t s = order_streams(a);
+        lists_changed(a);
         if (coarse_flags)
             AMRCHK(a, hipMemcpyAsync(a->cflags, coarse_flags, (size_t)coarse_cells(a), hipMemcpyHostToDevice, s));
         if (fine_flags)
@@ -1128,15 +1201,20 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
         double t = 0.0;
         int it = 0, fine_total = 0, rc = COSMOS_OK;
         while (t < final_time && it < cap) {
+            // the flags first (they read the coarse state, as the time step
+            // does, and write only flags), then the fine level's block lists,
+            // whose counts come back with the time step's wait
+            if ((rc = amr_refine(a, cell_spacing, tolerance))) return rc;
+            if ((rc = lists_queue(a, 1, order))) return rc;
             double cdt = 0.0;
             rc = cosmos_stable_time_step(a->coarse, cell_spacing, cfl, t, final_time, fine_total, &cdt);
             if (rc) return from_solver(a, a->coarse, rc);
-            if ((rc = amr_refine(a, cell_spacing, tolerance))) return rc;
+            if ((rc = lists_take(a, 1))) return rc;
             if ((rc = amr_flux_step(a, 0, order, cell_spacing, cdt))) return rc;
             // the 2D black-hole driver starts its "refined" test at true
-            // (AMR/BlackHoleEulerAMR.cpp:533): its fine level always sub-steps
-            bool refined = a->physics == COSMOS_BLACKHOLE && a->dim == 2;
-            if (!refined && (rc = amr_any_refined(a, refined))) return rc;
+            // (AMR/BlackHoleEulerAMR.cpp:533): its fine level always sub-steps;
+            // otherwise any flagged fine cell (a listed write-back chunk)
+            const bool refined = (a->physics == COSMOS_BLACKHOLE && a->dim == 2) || a->lists[1].n[2] > 0;
             if (refined) {
                 // fine sub-steps to the coarse time, less 1e-4 (pow(10.0, -4.0) in the reference)
                 double ft = t;
@@ -1220,6 +1298,7 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
         {
             hipStream_t s = order_streams(a);
             const long n = coarse_cells(a), nm = fine_cells(a);
+            lists_changed(a);
             hipLaunchKernelGGL(cfv::k_inject_all, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, s, c, m, a->dim);
             hipLaunchKernelGGL(cfv::k_inject_all, cfv::blocks_of(nm), dim3(cfv::kAmrThreads), 0, s, m, f, a->dim);
             hipLaunchKernelGGL(cfv::k_fill, cfv::blocks_of(n), dim3(cfv::kAmrThreads), 0, s, a->cflags, n,
@@ -1236,24 +1315,32 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
         double t = 0.0;
         int it = 0, fine_total = 0;
         while (t < final_time && it < cap) {
+            // each level's flag pass goes ahead of its time step (both read
+            // only that level's state; the pass writes only flags), and the
+            // block lists it leaves stale are rebuilt there, their counts
+            // coming back with the time step's wait
+            if ((rc = l2_refine(a, 0, cell_spacing, tolerance, 0))) return rc;
+            if ((rc = lists_queue(a, 1, order))) return rc;
             double cdt = 0.0;
             rc = cosmos_stable_time_step(a->coarse, cell_spacing, cfl, t, final_time, fine_total, &cdt);
             if (rc) return from_solver(a, a->coarse, rc);
-            if ((rc = l2_refine(a, 0, cell_spacing, tolerance, 0))) return rc;
+            if ((rc = lists_take(a, 1))) return rc;
             if ((rc = amr_flux_step(a, 0, order, cell_spacing, cdt))) return rc;
-            bool refined = false;
-            if ((rc = amr_any_set(a, a->fflags, fine_cells(a), refined))) return rc;
+            const bool refined = a->lists[1].n[2] > 0;  // any flagged intermediate cell
             if (refined) {
                 // intermediate sub-steps to the coarse time, less 1e-4
                 double mt = t;
                 while (mt < ((t + cdt) - 1e-4)) {
+                    if ((rc = l2_refine(a, 1, cell_spacing, itol, 1))) return rc;
+                    if ((rc = lists_queue(a, 1, order))) return rc;
+                    if ((rc = lists_queue(a, 2, order))) return rc;
                     double mdt = 0.0;
                     rc = cosmos_stable_time_step(a->fine, cell_spacing * 0.5, cfl, mt, t + cdt, fine_total, &mdt);
                     if (rc) return from_solver(a, a->fine, rc);
-                    if ((rc = l2_refine(a, 1, cell_spacing, itol, 1))) return rc;
+                    if ((rc = lists_take(a, 1))) return rc;
+                    if ((rc = lists_take(a, 2))) return rc;
                     if ((rc = amr_flux_step(a, 1, order, cell_spacing * 0.5, mdt))) return rc;
-                    bool refined2 = false;
-                    if ((rc = amr_any_set(a, a->f2flags, nf2, refined2))) return rc;
+                    const bool refined2 = a->lists[2].n[2] > 0;  // any flagged fine cell
                     if (refined2) {
                         double ft = mt;
                         while (ft < ((mt + mdt) - 1e-4)) {

@@ -971,8 +971,10 @@ namespace cfv {
 // tiles that hold a flagged cell are swept, and only flagged cells are written,
 // into the other buffer; amr_mask_writeback then copies them back, so the
 // state stays in the current buffer.  Not counted in the status word.
+// blocks (else null): the list_n blocks of this sweep that hold a flagged
+// cell (amr_mask_lists for this mask and order); only those are launched.
 int amr_masked_sweep(cosmos_solver* s, int order, int axis, double cell_spacing, double time_step, double bias,
-                     int slope_limiter, const unsigned char* mask) {
+                     int slope_limiter, const unsigned char* mask, const int* blocks, int list_n) {
     return guarded(s, [&] {
         int rc = check_order(s, order);
         if (rc) return rc;
@@ -988,14 +990,31 @@ int amr_masked_sweep(cosmos_solver* s, int order, int axis, double cell_spacing,
         L.coef.hdt_dx = 0.5 * (time_step / cell_spacing);
         L.status = nullptr;
         L.mask = mask;
+        if (blocks) {
+            L.block_order = blocks;
+            L.order_n = list_n;
+        }
         return run_sweep(s, L, axis);
     });
 }
 
-// The masked sweep's flagged cells from the other buffer back into the current one.
-hipError_t amr_mask_writeback(cosmos_solver* s, const unsigned char* mask) {
+// The masked sweep's flagged cells from the other buffer back into the
+// current one (segs: the n_segs 64-cell segments holding a flagged cell, or
+// null for all).
+hipError_t amr_mask_writeback(cosmos_solver* s, const unsigned char* mask, const int* segs, int n_segs) {
     return launch_mask_writeback(planes_of(s, s->cur), cplanes_of(s, s->cur ^ 1), mask, s->nx, s->ny, s->pitch,
-                                 s->stream);
+                                 segs, n_segs, s->stream);
+}
+
+// Columns of s's masked y-sweep tiles (the lists of amr_mask_lists depend on them).
+int amr_mask_tc(cosmos_solver* s) { return mask_tile_cols(base_launch(s, s->cur, 2, 0.0, 1)); }
+
+// The blocks of s's masked sweeps of this order, and the write-back
+// segments, that hold a flagged cell of mask, queued on s's stream
+// (launch_mask_lists; hit: its zeroed mark maps).
+hipError_t amr_mask_lists(cosmos_solver* s, const unsigned char* mask, int order, unsigned char* hit, int* xl,
+                          int* yl, int* wl, int* counts) {
+    return launch_mask_lists(mask, s->nx, s->ny, s->dim, order, amr_mask_tc(s), hit, xl, yl, wl, counts, s->stream);
 }
 
 // Queue s's work on owner's stream from now on (the levels of an AMR context

@@ -510,6 +510,110 @@ __global__ __launch_bounds__(256) void k_mask_writeback(Planes dst, ConstPlanes 
     }
 }
 
+// The candidate blocks of a masked level's launches that hold a flagged cell,
+// in two passes.  k_mask_mark: a thread per 16 mask bytes (one 16-byte read;
+// the mask is a device allocation, 16-byte aligned) marks, for each flagged
+// cell, its x-sweep row segment, its y-sweep tile and its 64-cell write-back
+// segment in the byte maps hx, hy, hw (benign same-value races), and clears
+// the three counts.  Flags are sparse, so most threads read zero and leave.
+__global__ __launch_bounds__(256) void k_mask_mark(const unsigned char* mask, long n, int nx, int tw, int tiles_x,
+                                                  int tc, int trv, int tcn, unsigned char* hx, unsigned char* hy,
+                                                  unsigned char* hw, int* counts) {
+    const long e0 = ((long)blockIdx.x * 256 + threadIdx.x) * 16;
+    if (blockIdx.x == 0 && threadIdx.x < 3) counts[threadIdx.x] = 0;  // k_mask_compact runs after this kernel
+    if (e0 >= n) return;
+    unsigned int q[4] = {0u, 0u, 0u, 0u};
+    if (e0 + 16 <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(mask + e0);
+        q[0] = v.x;
+        q[1] = v.y;
+        q[2] = v.z;
+        q[3] = v.w;
+    } else {
+        for (long e = e0; e < n; ++e) q[(e - e0) >> 2] |= (unsigned int)mask[e] << (8 * ((e - e0) & 3));
+    }
+    if ((q[0] | q[1] | q[2] | q[3]) == 0u) return;
+    long row = e0 / nx;
+    int col = (int)(e0 - row * nx);
+    for (int k = 0; k < 16; ++k, ++col) {
+        while (col >= nx) {
+            col -= nx;
+            ++row;
+        }
+        if (((q[k >> 2] >> (8 * (k & 3))) & 0xffu) == 0u) continue;
+        hx[row * tiles_x + col / tw] = 1;
+        if (hy) hy[(row / trv) * tcn + col / tc] = 1;
+        hw[(e0 + k) >> 6] = 1;
+    }
+}
+
+// k_mask_compact: eight consecutive candidates per thread, 2048 per block
+// (the x segments' blocks first, then the y tiles', then the write-back
+// segments'; a block never mixes kinds).  Each marked candidate is appended
+// to its kind's list and its mark cleared (the maps are all zero again for
+// the next build); a block takes its slots with one atomic and places its
+// candidates by a scan over its threads.
+constexpr int kCompactPer = 8;
+__global__ __launch_bounds__(256) void k_mask_compact(unsigned char* hx, unsigned char* hy, unsigned char* hw, long nxb,
+                                                     long nyb, long nwb, int bx, int by, int* xl, int* yl, int* wl,
+                                                     int* counts) {
+    __shared__ int wave_sum[4];
+    __shared__ int block_base;
+    const int blk = blockIdx.x;
+    const int kind = blk < bx ? 0 : (blk < bx + by ? 1 : 2);
+    const long b0 = ((long)(blk - (kind == 0 ? 0 : (kind == 1 ? bx : bx + by))) * 256 + threadIdx.x) * kCompactPer;
+    const long nb = kind == 0 ? nxb : (kind == 1 ? nyb : nwb);
+    unsigned char* hit = kind == 0 ? hx : (kind == 1 ? hy : hw);
+    int* list = kind == 0 ? xl : (kind == 1 ? yl : wl);
+    unsigned int marks = 0u;
+#pragma unroll
+    for (int u = 0; u < kCompactPer; ++u) {
+        const long b = b0 + u;
+        if (b < nb && hit[b] != 0) {
+            hit[b] = 0;
+            marks |= 1u << u;
+        }
+    }
+    const int mine = __popc(marks);
+    // inclusive scan of the counts over the wave, then over the block's waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wave_sum[wave] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        before += w < wave ? wave_sum[w] : 0;
+        total += wave_sum[w];
+    }
+    if (total == 0) return;  // uniform over the block
+    if (threadIdx.x == 0) block_base = atomicAdd(counts + kind, total);
+    __syncthreads();
+    int pos = block_base + before + incl - mine;
+    for (int u = 0; u < kCompactPer; ++u)
+        if (marks & (1u << u)) list[pos++] = (int)(b0 + u);
+}
+
+// A wave per listed 64-cell segment of the grid (segs[k] covers cells
+// [64 segs[k], 64 segs[k] + 64) in row-major order): each lane copies the five
+// values of its cell where flagged.
+__global__ __launch_bounds__(256) void k_mask_writeback_segs(Planes dst, ConstPlanes src, const unsigned char* mask,
+                                                            int nx, int ny, long pitch, const int* segs, int n_segs) {
+    const long wv = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wv >= n_segs) return;
+    const long idx = (long)segs[wv] * 64 + (threadIdx.x & 63);
+    if (idx >= (long)nx * ny || !mask[idx]) return;
+    const long i = idx / nx, j = idx - i * nx;
+    const long c = i * pitch + j;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) dst.p[k][c] = src.p[k][c];
+}
+
 __global__ __launch_bounds__(256) void k_any_nonzero(const double* plane, int nx, int ny, long pitch, int* flag) {
     long total = (long)nx * ny;
     int found = 0;
@@ -525,7 +629,9 @@ hipError_t dispatch_x(const SweepLaunch& L, hipStream_t s) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     const int TW = kXThreads - 2 * G;
     const int tiles = (L.nx + TW - 1) / TW;
-    dim3 grid((unsigned)(tiles * L.ny)), block(kXThreads);
+    const bool listed = L.mask && L.block_order;
+    if (listed && L.order_n <= 0) return hipSuccess;
+    dim3 grid((unsigned)(listed ? L.order_n : tiles * L.ny)), block(kXThreads);
     if (L.fuse_ws) {
         if (L.dim == 1) hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 1, true, W0>), grid, block, 0, s, L);
         else hipLaunchKernelGGL((k_sweep_x<Phys, ORDER, LIMMODE, 2, true, W0>), grid, block, 0, s, L);
@@ -545,15 +651,22 @@ int y_rows_for(const SweepLaunch& L) {
 
 // Columns per y-sweep LDS tile (0: the column march): the march for the
 // Newtonian sweeps (no halo recomputation; measured 3-4 % faster than tiles
-// at 4096^2), 8-column tiles (28 output rows of 32) for the
+// at 4096^2) on grids it fills, 16-column tiles below that, 8-column tiles (28 output rows of 32) for the
 // special-relativistic ones, which are FP64-bound: the march's register
 // window holds them to 2 waves per SIMD, the tiles run at the x sweep's
 // occupancy (2048^2 SR step 2.98 ms with the march, 2.22 ms with 8-column
 // and 2.35 ms with 16-column tiles).
+int masked_tile_cols(int y_tile) { return y_tile == 8 ? 8 : 16; }  // masked sweeps: tiles only
+
 int y_tile_for(const SweepLaunch& L) {
-    if (L.mask) return L.y_tile == 8 ? 8 : 16;  // masked sweeps: tiles only
+    if (L.mask) return masked_tile_cols(L.y_tile);
     if (L.y_tile == 0 || L.y_tile == 8 || L.y_tile == 16) return L.y_tile;
-    return L.physics == 1 ? 8 : 0;
+    if (L.physics == 1) return 8;
+    // Newtonian: the march, unless it would run fewer than four blocks per CU
+    // (1024 blocks; 2048^2 and smaller grids), where 16-column tiles fill the chip
+    const int rows = L.fuse_ws ? 64 : y_rows_for(L);
+    const long march = (long)((L.nx + kYThreads - 1) / kYThreads) * ((L.ny + rows - 1) / rows);
+    return march < 1024 ? 16 : 0;
 }
 
 template <class Phys, int ORDER, int LIMMODE, bool W0, int TC>
@@ -561,7 +674,9 @@ hipError_t dispatch_y_tile(const SweepLaunch& L, hipStream_t s) {
     constexpr int G = ORDER == 2 ? 2 : 1;
     constexpr int TRV = kXThreads / TC - 2 * G;
     const int tiles = (L.nx + TC - 1) / TC * ((L.ny + TRV - 1) / TRV);
-    dim3 grid((unsigned)tiles), block(kXThreads);
+    const bool listed = L.mask && L.block_order;
+    if (listed && L.order_n <= 0) return hipSuccess;
+    dim3 grid((unsigned)(listed ? L.order_n : tiles)), block(kXThreads);
     if (L.fuse_ws) hipLaunchKernelGGL((k_sweep_y_tile<Phys, ORDER, LIMMODE, 2, true, W0, TC>), grid, block, 0, s, L);
     else hipLaunchKernelGGL((k_sweep_y_tile<Phys, ORDER, LIMMODE, 2, false, W0, TC>), grid, block, 0, s, L);
     return hipGetLastError();
@@ -614,10 +729,46 @@ hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s) {
 }
 
 hipError_t launch_mask_writeback(const Planes& dst, const ConstPlanes& src, const unsigned char* mask, int nx, int ny,
-                                 long pitch, hipStream_t s) {
+                                 long pitch, const int* segs, int n_segs, hipStream_t s) {
+    if (segs) {
+        if (n_segs <= 0) return hipSuccess;
+        hipLaunchKernelGGL(k_mask_writeback_segs, dim3((unsigned)((n_segs + 3) / 4)), dim3(256), 0, s, dst, src, mask,
+                           nx, ny, pitch, segs, n_segs);
+        return hipGetLastError();
+    }
     const long n = (long)nx * ny;
     const long blocks = (n + 4095) / 4096;  // 4 waves of 1024 cells
     hipLaunchKernelGGL(k_mask_writeback, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, mask, nx, ny, pitch);
+    return hipGetLastError();
+}
+
+int mask_tile_cols(const SweepLaunch& L) { return masked_tile_cols(L.y_tile); }
+
+void mask_list_sizes(int nx, int ny, int dim, int order, int tc, long sizes[3]) {
+    const int g = order == 2 ? 2 : 1;
+    const int tw = kXThreads - 2 * g, trv = kXThreads / tc - 2 * g;
+    sizes[0] = (long)((nx + tw - 1) / tw) * ny;
+    sizes[1] = dim == 2 ? (long)((nx + tc - 1) / tc) * ((ny + trv - 1) / trv) : 0;
+    sizes[2] = ((long)nx * ny + 63) / 64;
+}
+
+hipError_t launch_mask_lists(const unsigned char* mask, int nx, int ny, int dim, int order, int tc,
+                             unsigned char* hit, int* xl, int* yl, int* wl, int* counts, hipStream_t s) {
+    const int g = order == 2 ? 2 : 1;
+    const int tw = kXThreads - 2 * g, trv = kXThreads / tc - 2 * g;
+    long sz[3];
+    mask_list_sizes(nx, ny, dim, order, tc, sz);
+    unsigned char* hx = hit;
+    unsigned char* hy = dim == 2 ? hit + sz[0] : nullptr;
+    unsigned char* hw = hit + sz[0] + sz[1];
+    const long n = (long)nx * ny;
+    hipLaunchKernelGGL(k_mask_mark, dim3((unsigned)((n + 4095) / 4096)), dim3(256), 0, s, mask, n, nx, tw,
+                       (nx + tw - 1) / tw, tc, trv, (nx + tc - 1) / tc, hx, hy, hw, counts);
+    constexpr long kPerBlock = 256 * kCompactPer;
+    const int bx = (int)((sz[0] + kPerBlock - 1) / kPerBlock), by = (int)((sz[1] + kPerBlock - 1) / kPerBlock),
+              bw = (int)((sz[2] + kPerBlock - 1) / kPerBlock);
+    hipLaunchKernelGGL(k_mask_compact, dim3((unsigned)(bx + by + bw)), dim3(256), 0, s, hx, hy, hw, sz[0], sz[1],
+                       sz[2], bx, by, xl, yl, wl, counts);
     return hipGetLastError();
 }
 

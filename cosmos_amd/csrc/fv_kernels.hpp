@@ -148,7 +148,10 @@ struct SweepLaunch {
     // AMR masked sweeps (else null): the level's refinement flags ([ny][nx]
     // bytes).  Only flagged cells are written to out; a block whose tile holds
     // no flagged output cell returns at once.  The y sweep then runs as LDS
-    // tiles (the column march takes no mask).
+    // tiles (the column march takes no mask).  With a mask, block_order may
+    // list the blocks that hold a flagged output cell (from launch_mask_lists):
+    // the launch then runs order_n blocks, block k the listed block_order[k],
+    // and none at all when order_n is 0.
     const unsigned char* mask;
 };
 
@@ -157,8 +160,24 @@ hipError_t launch_sweep_x(const SweepLaunch& L, hipStream_t s);
 hipError_t launch_sweep_y(const SweepLaunch& L, hipStream_t s);
 // dst[c] = src[c] for every cell c whose mask byte is set (the five planes of
 // an nx-by-ny grid of row pitch `pitch`): the write-back of a masked sweep.
+// segs (else null): the n_segs 64-cell segments (row-major cell order)
+// holding a flagged cell (from launch_mask_lists); only those are visited.
 hipError_t launch_mask_writeback(const Planes& dst, const ConstPlanes& src, const unsigned char* mask, int nx, int ny,
-                                 long pitch, hipStream_t s);
+                                 long pitch, const int* segs, int n_segs, hipStream_t s);
+// The blocks of a masked level's launches that hold a flagged cell of mask
+// ([ny][nx] bytes), for sweeps of this order (1 or 2) and y-tile width tc
+// (8 or 16; 2D only): counts[0] x-sweep row segments into xl, counts[1]
+// y-sweep tiles into yl, counts[2] 64-cell write-back segments into wl, each
+// list in no particular order (the blocks write disjoint cells).  counts are
+// cleared first.  xl, yl and wl need mask_list_sizes() entries; hit is their
+// total in bytes of mark maps, all zero on entry and left all zero.
+hipError_t launch_mask_lists(const unsigned char* mask, int nx, int ny, int dim, int order, int tc,
+                             unsigned char* hit, int* xl, int* yl, int* wl, int* counts, hipStream_t s);
+// Candidate blocks of each list: x-sweep segments, y-sweep tiles (0 in 1D),
+// 64-cell write-back segments.
+void mask_list_sizes(int nx, int ny, int dim, int order, int tc, long sizes[3]);
+// Columns of the masked y sweep's tiles for this launch record (8 or 16).
+int mask_tile_cols(const SweepLaunch& L);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);

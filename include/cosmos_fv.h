@@ -82,8 +82,8 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode);
  * it): 0 one thread per column marching down y_rows rows with a register
  * window; 8 or 16: one thread per cell over LDS tiles of that many columns
  * (re-evaluating 4 halo rows per tile for SLIC, at the x sweep's occupancy);
- * -1: the default (the march for Newtonian, 8-column tiles for special
- * relativistic). */
+ * -1: the default (for Newtonian the march where it runs at least 1024
+ * blocks, else 16-column tiles; 8-column tiles for special relativistic). */
 int cosmos_set_y_tile(cosmos_solver* s, int columns);
 /* Step fusion (no counterpart in the reference; results do not depend on it):
  * with 1 (the default), a 2D fast-mode Newtonian SLIC step with van Leer and
