@@ -36,6 +36,9 @@ int amr_mask_tc(cosmos_solver* s);
 hipError_t amr_mask_lists(cosmos_solver* s, const unsigned char* mask, int order, unsigned char* hit, int* xl,
                           int* yl, int* wl, int* counts);
 int amr_share_stream(cosmos_solver* s, cosmos_solver* owner);
+int amr_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, double current_time, double final_time,
+                         int current_iteration, const unsigned char* mask, const int* segs, int n_segs,
+                         unsigned long long* rest, int full, double* out);
 }  // namespace cfv
 
 namespace cfv {
@@ -586,6 +589,7 @@ struct cosmos_amr {
     MaskLists lists[3];               // levels 1 and 2 (index by level)
     int* counts = nullptr;            // device, the lists' counts (three per level)
     int* counts_host = nullptr;       // page-locked host copy of them
+    unsigned long long* rest = nullptr;  // device, per level kSlotWords: wave-speed maximum of its unflagged cells
     cfv::PhysParams P{};
     cfv::Hole hole = cfv::make_hole(1.0, 0.0, 0.5, 0.5, 0.0, 0.0);  // COSMOS_BLACKHOLE (the solvers' default)
     double excision = 0.0;
@@ -867,6 +871,24 @@ int lists_take(cosmos_amr* a, int level) {
     return COSMOS_OK;
 }
 
+// The time step of a finer level's sub-step (cosmos_stable_time_step).  first:
+// the first sub-step since its parent's step began; later sub-steps change
+// only flagged cells, so for masked levels (Newtonian, special relativistic)
+// with current block lists the wave speed is re-read only at flagged cells,
+// joined with the unflagged cells' maximum kept by the first.
+int level_time_step(cosmos_amr* a, int level, double dx, double cfl, double t, double tf, int iter, bool first,
+                    double* dt) {
+    cosmos_solver* s = level_solver(a, level);
+    const MaskLists& M = a->lists[level];
+    int rc;
+    if (a->physics != COSMOS_BLACKHOLE && M.valid)
+        rc = cfv::amr_stable_time_step(s, dx, cfl, t, tf, iter, level_flags(a, level), M.blocks[2], M.n[2],
+                                       a->rest + (long)level * cfv::kSlotWords, first ? 1 : 0, dt);
+    else
+        rc = cosmos_stable_time_step(s, dx, cfl, t, tf, iter, dt);
+    return from_solver(a, s, rc);
+}
+
 // One sweep of a level restricted to that level's flags, in one of two forms
 // with the same result.  The finer levels, whose flags mark the refined region
 // (usually a small part of the level): only the tiles holding a flagged cell
@@ -992,6 +1014,7 @@ int cosmos_amr_create(cosmos_amr** out, int physics, int dim, int nx, int ny, do
     if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->fflags, (size_t)fine_cells(a));
     if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->counts, 9 * sizeof(int));
     if (!rc && e == hipSuccess) e = hipHostMalloc((void**)&a->counts_host, 9 * sizeof(int));
+    if (!rc && e == hipSuccess) e = hipMalloc((void**)&a->rest, 3 * sizeof(unsigned long long) * cfv::kSlotWords);
     if (rc || e != hipSuccess) {
         std::string msg = rc ? std::string(cosmos_last_error(nullptr)) : std::string(hipGetErrorString(e));
         cosmos_amr_destroy(a);
@@ -1020,6 +1043,7 @@ int cosmos_amr_destroy(cosmos_amr* a) {
     }
     hipFree(a->counts);
     hipHostFree(a->counts_host);
+    hipFree(a->rest);
     // the finer levels queue on the coarse handle's stream: they go first
     cosmos_destroy(a->fine2);
     cosmos_destroy(a->fine);
@@ -1218,10 +1242,12 @@ int cosmos_amr_solve_level1(cosmos_amr* a, int order, double cell_spacing, doubl
             if (refined) {
                 // fine sub-steps to the coarse time, less 1e-4 (pow(10.0, -4.0) in the reference)
                 double ft = t;
+                bool first = true;
                 while (ft < ((t + cdt) - 1e-4)) {
                     double fdt = 0.0;
-                    rc = cosmos_stable_time_step(a->fine, cell_spacing * 0.5, cfl, ft, t + cdt, fine_total, &fdt);
-                    if (rc) return from_solver(a, a->fine, rc);
+                    rc = level_time_step(a, 1, cell_spacing * 0.5, cfl, ft, t + cdt, fine_total, first, &fdt);
+                    if (rc) return rc;
+                    first = false;
                     if ((rc = amr_flux_step(a, 1, order, cell_spacing * 0.5, fdt))) return rc;
                     ft += fdt;
                     fine_total += 1;
@@ -1343,11 +1369,13 @@ int cosmos_amr_solve_level2(cosmos_amr* a, int order, double cell_spacing, doubl
                     const bool refined2 = a->lists[2].n[2] > 0;  // any flagged fine cell
                     if (refined2) {
                         double ft = mt;
+                        bool first = true;
                         while (ft < ((mt + mdt) - 1e-4)) {
                             double fdt = 0.0;
-                            rc = cosmos_stable_time_step(a->fine2, cell_spacing * 0.25, cfl, ft, mt + mdt, fine_total,
-                                                         &fdt);
-                            if (rc) return from_solver(a, a->fine2, rc);
+                            rc = level_time_step(a, 2, cell_spacing * 0.25, cfl, ft, mt + mdt, fine_total, first,
+                                                 &fdt);
+                            if (rc) return rc;
+                            first = false;
                             if ((rc = amr_flux_step(a, 2, order, cell_spacing * 0.25, fdt))) return rc;
                             ft += fdt;
                             fine_total += 1;

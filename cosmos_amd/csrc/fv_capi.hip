@@ -1033,6 +1033,15 @@ int amr_share_stream(cosmos_solver* s, cosmos_solver* owner) {
 
 extern "C" {
 
+// EulerSolvers::computeStableTimeStep (Solvers/EulerSolvers.cpp:130-145)
+static double stable_dt(double smax, double cell_spacing, double cfl, double current_time, double final_time,
+                        int current_iteration) {
+    double dt = cfl * (cell_spacing / smax);
+    if (current_iteration <= 5) dt *= 0.2;
+    if ((current_time + dt) > final_time) dt = final_time - current_time;
+    return dt;
+}
+
 static int max_ws_impl(cosmos_solver* s, unsigned long long* slots_dev) {
     int rc = settle(s);
     if (rc) return rc;
@@ -1069,14 +1078,51 @@ int cosmos_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, d
         double smax = 0.0;
         int rc = cosmos_max_wave_speed(s, &smax);
         if (rc) return rc;
-        // EulerSolvers::computeStableTimeStep (Solvers/EulerSolvers.cpp:130-145)
-        double dt = cfl * (cell_spacing / smax);
-        if (current_iteration <= 5) dt *= 0.2;
-        if ((current_time + dt) > final_time) dt = final_time - current_time;
-        *out = dt;
+        *out = stable_dt(smax, cell_spacing, cfl, current_time, final_time, current_iteration);
         return COSMOS_OK;
     });
 }
+
+}  // extern "C"
+
+namespace cfv {
+
+// cosmos_stable_time_step of a masked Newtonian or special-relativistic AMR
+// level, the same time step from fewer reads: full reads every cell and keeps
+// the maximum over the unflagged ones in rest (kSlotWords device words); otherwise only
+// the flagged cells of the n_segs listed 64-cell segments are read and rest
+// joins them, which is the full maximum while no unflagged cell has changed
+// since the full pass (between the masked sub-steps of one parent step).
+int amr_stable_time_step(cosmos_solver* s, double cell_spacing, double cfl, double current_time, double final_time,
+                         int current_iteration, const unsigned char* mask, const int* segs, int n_segs,
+                         unsigned long long* rest, int full, double* out) {
+    return guarded(s, [&] {
+        if (!out) return fail(s, COSMOS_ERR_ARGUMENT, "out is null");
+        if (s->physics == COSMOS_BLACKHOLE) return fail(s, COSMOS_ERR_ARGUMENT, "black-hole levels are unmasked");
+        int rc = settle(s);
+        if (rc) return rc;
+        hipError_t e = launch_zero_slots(s->ws_scratch, s->stream);
+        if (e == hipSuccess && full) e = launch_zero_slots(rest, s->stream);
+        const ConstPlanes in = cplanes_of(s, s->cur);
+        if (e == hipSuccess) {
+            e = full ? launch_wavespeed_split(in, s->nx, s->ny, s->pitch, s->physics, s->dim, s->P, mask,
+                                              s->ws_scratch, rest, s->stream)
+                     : launch_wavespeed_segs(in, s->nx, s->ny, s->pitch, s->physics, s->dim, s->P, mask, segs,
+                                             n_segs, rest, s->ws_scratch, s->stream);
+        }
+        if (e != hipSuccess)
+            return fail(s, COSMOS_ERR_DEVICE, std::string("wave speed launch: ") + hipGetErrorString(e));
+        HIPCHK(s, hipMemcpyAsync(s->ws_host, s->ws_scratch, sizeof(unsigned long long) * kSlotWords,
+                                 hipMemcpyDeviceToHost, s->stream));
+        HIPCHK(s, hipStreamSynchronize(s->stream));
+        *out = stable_dt(slots_max_host(s->ws_host), cell_spacing, cfl, current_time, final_time, current_iteration);
+        return COSMOS_OK;
+    });
+}
+
+}  // namespace cfv
+
+extern "C" {
 
 int cosmos_begin(cosmos_solver* s, int order, double cell_spacing, double cfl, double final_time, int max_steps,
                  double bias, int slope_limiter) {

@@ -349,19 +349,126 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
 // ---------------------------------------------------------------------------
 // Stand-alone wave speed (computeMaximumWaveSpeed{,2D}) and the step driver
 // ---------------------------------------------------------------------------
+// The wave-speed kernels' tiles: 256 columns by kWsRows rows per block, a
+// thread per column (no index division per cell; the rows' loads overlap).
+constexpr int kWsRows = 8;
+
+struct WsTile {
+    int j, i0, rows;  // column, first row, rows in range (0 when the column is outside the grid)
+};
+
+__device__ __forceinline__ WsTile ws_tile(int nx, int ny) {
+    const int gx = (nx + 255) / 256;
+    const int by = blockIdx.x / gx, bx = blockIdx.x - by * gx;
+    WsTile t;
+    t.j = bx * 256 + threadIdx.x;
+    t.i0 = by * kWsRows;
+    t.rows = t.j < nx ? min(kWsRows, ny - t.i0) : 0;
+    return t;
+}
+
+int ws_blocks(int nx, int ny) { return ((nx + 255) / 256) * ((ny + kWsRows - 1) / kWsRows); }
+
 template <class Phys, int DIM>
 __global__ __launch_bounds__(256) void k_wavespeed(ConstPlanes in, int nx, int ny, long pitch, PhysParams P,
                                                    unsigned long long* slots) {
+    const WsTile t = ws_tile(nx, ny);
     double m = 0.0;
-    long total = (long)nx * ny;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-        long i = idx / nx, j = idx - i * nx;
-        double W[5];
-        load_cell<false>(in, i * pitch + j, W);
-        double w = Phys::template wavespeed<DIM>(W, P);
-        if (w > m) m = w;
+    if (t.rows == kWsRows) {
+#pragma unroll
+        for (int r = 0; r < kWsRows; ++r) {
+            double W[5];
+            load_cell<false>(in, (long)(t.i0 + r) * pitch + t.j, W);
+            const double w = Phys::template wavespeed<DIM>(W, P);
+            if (w > m) m = w;
+        }
+    } else {
+        for (int r = 0; r < t.rows; ++r) {
+            double W[5];
+            load_cell<false>(in, (long)(t.i0 + r) * pitch + t.j, W);
+            const double w = Phys::template wavespeed<DIM>(W, P);
+            if (w > m) m = w;
+        }
     }
     block_max_to<256>(m, slots);
+}
+
+// The maximum wave speed of a masked AMR level in two forms whose results
+// combine into k_wavespeed's.  k_wavespeed_split: every cell, as
+// k_wavespeed, into slots, and the maximum over the cells whose mask byte is
+// clear into the slots rest (same layout).  While only flagged cells change
+// (a masked level's sub-steps), k_wavespeed_segs then reads only the flagged
+// cells of the listed 64-cell segments (a wave per segment) and merges the
+// rest slots in: the same maximum, as max is exact in any order.
+template <class Phys, int DIM>
+__global__ __launch_bounds__(256) void k_wavespeed_split(ConstPlanes in, int nx, int ny, long pitch, PhysParams P,
+                                                         const unsigned char* mask, unsigned long long* slots,
+                                                         unsigned long long* rest) {
+    __shared__ double red[2][4];
+    const WsTile t = ws_tile(nx, ny);
+    double m = 0.0, r = 0.0;
+    auto row = [&](int k) {
+        const int i = t.i0 + k;
+        double W[5];
+        load_cell<false>(in, (long)i * pitch + t.j, W);
+        const bool flagged = mask[(long)i * nx + t.j] != 0;
+        const double w = Phys::template wavespeed<DIM>(W, P);
+        if (w > m) m = w;
+        if (!flagged && w > r) r = w;
+    };
+    if (t.rows == kWsRows) {
+#pragma unroll
+        for (int k = 0; k < kWsRows; ++k) row(k);
+    } else {
+        for (int k = 0; k < t.rows; ++k) row(k);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64), q = __shfl_xor(r, off, 64);
+        m = (o > m) ? o : m;
+        r = (q > r) ? q : r;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = m;
+        red[1][wave] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = red[0][0], c = red[1][0];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            b = (red[0][w] > b) ? red[0][w] : b;
+            c = (red[1][w] > c) ? red[1][w] : c;
+        }
+        if (b > 0.0)
+            atomicMax(slots + (blockIdx.x % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(b));
+        if (c > 0.0)
+            atomicMax(rest + (blockIdx.x % kMaxSlots) * kSlotStride, (unsigned long long)__double_as_longlong(c));
+    }
+}
+
+template <class Phys, int DIM>
+__global__ __launch_bounds__(256) void k_wavespeed_segs(ConstPlanes in, int nx, int ny, long pitch, PhysParams P,
+                                                        const unsigned char* mask, const int* segs, int n_segs,
+                                                        const unsigned long long* rest, unsigned long long* slots) {
+    const long wv = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    double m = 0.0;
+    if (wv < n_segs) {
+        const long idx = (long)segs[wv] * 64 + (threadIdx.x & 63);
+        if (idx < (long)nx * ny && mask[idx]) {
+            const long i = idx / nx, j = idx - i * nx;
+            double W[5];
+            load_cell<false>(in, i * pitch + j, W);
+            const double w = Phys::template wavespeed<DIM>(W, P);
+            if (w > m) m = w;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kMaxSlots) {
+        const double q = __longlong_as_double((long long)rest[threadIdx.x * kSlotStride]);
+        if (q > m) m = q;
+    }
+    wave_max_to_slot(m, slots, (int)wv);
 }
 
 // One iteration of the solve loop's head (Solvers/EulerSecondOrderSolver.cpp:162-172 with
@@ -774,10 +881,7 @@ hipError_t launch_mask_lists(const unsigned char* mask, int nx, int ny, int dim,
 
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s) {
-    long total = (long)nx * ny;
-    long blocks = (total + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
-    dim3 grid((unsigned)blocks), block(256);
+    dim3 grid((unsigned)ws_blocks(nx, ny)), block(256);
     if (physics == 0) {
         if (dim == 1) hipLaunchKernelGGL((k_wavespeed<Newtonian, 1>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
         else hipLaunchKernelGGL((k_wavespeed<Newtonian, 2>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
@@ -785,6 +889,42 @@ hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, i
         if (dim == 1) hipLaunchKernelGGL((k_wavespeed<Relativistic, 1>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
         else hipLaunchKernelGGL((k_wavespeed<Relativistic, 2>), grid, block, 0, s, in, nx, ny, pitch, P, slots);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_wavespeed_split(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
+                                  const PhysParams& P, const unsigned char* mask, unsigned long long* slots,
+                                  unsigned long long* rest, hipStream_t s) {
+    const dim3 grid((unsigned)ws_blocks(nx, ny)), block(256);
+#define COSMOS_WS_SPLIT(PH, D) \
+    hipLaunchKernelGGL((k_wavespeed_split<PH, D>), grid, block, 0, s, in, nx, ny, pitch, P, mask, slots, rest)
+    if (physics == 0) {
+        if (dim == 1) COSMOS_WS_SPLIT(Newtonian, 1);
+        else COSMOS_WS_SPLIT(Newtonian, 2);
+    } else {
+        if (dim == 1) COSMOS_WS_SPLIT(Relativistic, 1);
+        else COSMOS_WS_SPLIT(Relativistic, 2);
+    }
+#undef COSMOS_WS_SPLIT
+    return hipGetLastError();
+}
+
+hipError_t launch_wavespeed_segs(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
+                                 const PhysParams& P, const unsigned char* mask, const int* segs, int n_segs,
+                                 const unsigned long long* rest, unsigned long long* slots, hipStream_t s) {
+    const int blocks = n_segs > 0 ? (n_segs + 3) / 4 : 1;  // one block at least: it merges rest
+    const dim3 grid((unsigned)blocks), block(256);
+#define COSMOS_WS_SEGS(PH, D) \
+    hipLaunchKernelGGL((k_wavespeed_segs<PH, D>), grid, block, 0, s, in, nx, ny, pitch, P, mask, segs, n_segs, rest, \
+                       slots)
+    if (physics == 0) {
+        if (dim == 1) COSMOS_WS_SEGS(Newtonian, 1);
+        else COSMOS_WS_SEGS(Newtonian, 2);
+    } else {
+        if (dim == 1) COSMOS_WS_SEGS(Relativistic, 1);
+        else COSMOS_WS_SEGS(Relativistic, 2);
+    }
+#undef COSMOS_WS_SEGS
     return hipGetLastError();
 }
 

@@ -180,6 +180,18 @@ void mask_list_sizes(int nx, int ny, int dim, int order, int tc, long sizes[3]);
 int mask_tile_cols(const SweepLaunch& L);
 hipError_t launch_wavespeed(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
                             const PhysParams& P, unsigned long long* slots, hipStream_t s);
+// The wave-speed maximum of a masked AMR level (Newtonian or special
+// relativistic), in two forms that give launch_wavespeed's slots: split
+// reads every cell and also leaves the maximum over the unflagged cells in
+// the slots rest (kSlotWords words, zeroed by the caller); segs reads only
+// the flagged cells of the n_segs listed 64-cell segments and merges rest,
+// valid while no unflagged cell has changed since the split pass.
+hipError_t launch_wavespeed_split(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
+                                  const PhysParams& P, const unsigned char* mask, unsigned long long* slots,
+                                  unsigned long long* rest, hipStream_t s);
+hipError_t launch_wavespeed_segs(const ConstPlanes& in, int nx, int ny, long pitch, int physics, int dim,
+                                 const PhysParams& P, const unsigned char* mask, const int* segs, int n_segs,
+                                 const unsigned long long* rest, unsigned long long* slots, hipStream_t s);
 hipError_t launch_step_begin(StepState* st, hipStream_t s);
 // The dispatch order of an n-block sweep from its blocks' marks: the marked
 // blocks first, then the rest in the XCD-contiguous order; clears the marks.
