@@ -123,6 +123,40 @@ def test_amr_2d_larger_newtonian(orc):
 from test_oracle_amr import LEVEL2_NAMES, level2_cases  # noqa: E402
 
 
+@pytest.mark.parametrize("physics", [0, 1])
+def test_amr_drivers_on_levels_wider_than_a_segment(orc, physics):
+    """Finer levels wider than one x-sweep row segment (252 cells) and taller
+    than one y-sweep tile, a few percent of their cells flagged: the masked
+    sweeps then launch only the listed blocks, and every fine sub-step after
+    the first re-reads only the flagged cells' wave speeds.  Both drivers,
+    six coarse steps with sub-steps, every level and flag bitwise the
+    oracle's."""
+    g = G_N if physics == 0 else G_SR
+    W = workloads.quadrants_2d(140, seed=21, ny=72)
+    if physics:
+        W[..., 1:3] *= 0.5
+    exp = orc.amr_level1(W, physics, 2, 1.0 / 140, 0.8, 1.0, 8.0, g, max_steps=6)
+    assert 0.0 < (exp[3] != 0).mean() < 0.2
+    with amr_for(W, physics, g) as a:
+        t, n = a.solve_level1(2, 1.0 / 140, 0.8, 1.0, 8.0, 6)
+        got = a.download()
+    for k in range(4):
+        assert same(got[k], exp[k]), k
+    assert (t, n) == (exp[4], exp[5])
+    W = workloads.quadrants_2d(72, seed=22, ny=40)
+    if physics:
+        W[..., 1:3] *= 0.5
+    exp = orc.amr_level2(W, physics, 2, 1.0 / 72, 0.8, 1.0, 8.0, g, max_steps=6)
+    assert 0.0 < (exp[5] != 0).mean() < 0.2
+    with Amr(physics, 2, 72, 40, g) as a:
+        a.upload(W)
+        t, k = a.solve_level2(2, 1.0 / 72, 0.8, 1.0, 8.0, 6)
+        got = a.download_level2()
+    for k2, name in enumerate(LEVEL2_NAMES):
+        assert same(got[k2], exp[k2]), name
+    assert (t, k) == (exp[6], exp[7])
+
+
 @pytest.mark.parametrize("case", range(7))
 def test_amr_level2_bitwise(orc, case):
     tag, physics, order, W, tol, g, tf = level2_cases()[case]
