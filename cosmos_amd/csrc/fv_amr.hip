@@ -65,6 +65,23 @@ __device__ __forceinline__ void store(const Grid& g, int i, int j, const double*
     for (int k = 0; k < 5; ++k) g.p[k][(long)i * g.pitch + j] = W[k];
 }
 
+// W into the children (i, j2) and (i, j2 + 1) of a row (j2 even) where put0 /
+// put1: one 16-byte store per plane when both are written (rows start 16-byte
+// aligned: the pitch is a multiple of 16 doubles).
+__device__ __forceinline__ void store_pair(const Grid& g, int i, int j2, const double* W, bool put0, bool put1) {
+    const long o = (long)i * g.pitch + j2;
+    if (put0 && put1) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) *reinterpret_cast<double2*>(g.p[k] + o) = make_double2(W[k], W[k]);
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        if (put0) g.p[k][o] = W[k];
+        if (put1) g.p[k][o + 1] = W[k];
+    }
+}
+
 // |(b - a) / dx| >= tol for any component, the test of the reference's flag
 // loops (AMR/EulerAMR.cpp:215-216, :412-415).  Spacing carries dx and, when
 // it is exact (dx a power of two with a normal reciprocal), 1 / dx: then
@@ -330,11 +347,11 @@ __global__ __launch_bounds__(kAmrThreads) void k_amr_restrict(Grid c, Grid f, Fr
         load(c, i, j, W);
     }
     const int rows = dim == 1 ? 1 : 2;
-    for (int a = 0; a < rows; ++a)
-        for (int b = 0; b < 2; ++b) {
-            int fi = dim == 1 ? 0 : 2 * i + a, fj = 2 * j + b;
-            if (!fflags[(long)fi * f.nx + fj]) store(f, fi, fj, W);
-        }
+    for (int a = 0; a < rows; ++a) {
+        const int fi = dim == 1 ? 0 : 2 * i + a;
+        const long fo = (long)fi * f.nx + 2 * j;
+        store_pair(f, fi, 2 * j, W, !fflags[fo], !fflags[fo + 1]);
+    }
 }
 
 // A2: derefinement of quiet refined cells, tested on fine cell 2j+1
@@ -417,8 +434,7 @@ __global__ __launch_bounds__(kAmrThreads) void k_inject_all(Grid c, Grid f, int 
     double W[5];
     load(c, i, j, W);
     const int rows = dim == 1 ? 1 : 2;
-    for (int a = 0; a < rows; ++a)
-        for (int b = 0; b < 2; ++b) store(f, dim == 1 ? 0 : 2 * i + a, 2 * j + b, W);
+    for (int a = 0; a < rows; ++a) store_pair(f, dim == 1 ? 0 : 2 * i + a, 2 * j, W, true, true);
 }
 
 // Three-level AMR (EulerAMR::solve{,2D}Level2AMR, AMR/EulerAMR.cpp:593-1381;
@@ -503,11 +519,11 @@ __global__ __launch_bounds__(kAmrThreads) void k_l2_exchange(Grid p, Grid c, Fra
         load(p, i, j, W);
     }
     if (!inject) return;
-    for (int a = 0; a < (dim == 1 ? 1 : 2); ++a)
-        for (int b = 0; b < 2; ++b) {
-            const int ci = dim == 1 ? 0 : 2 * i + a, cj = 2 * j + b;
-            if (!cflags[(long)ci * c.nx + cj]) store(c, ci, cj, W);
-        }
+    for (int a = 0; a < (dim == 1 ? 1 : 2); ++a) {
+        const int ci = dim == 1 ? 0 : 2 * i + a;
+        const long co = (long)ci * c.nx + 2 * j;
+        store_pair(c, ci, 2 * j, W, !cflags[co], !cflags[co + 1]);
+    }
 }
 
 // Derefinement: child (2i+1, 2j+1) (1D: 2j+1) against its neighbours (copy
