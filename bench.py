@@ -953,8 +953,9 @@ def main():
         "parity": {"exact": "bitwise equal to the reference arithmetic (tests/test_gpu_parity.py)",
                    "fast": "relative L-inf per conserved variable <= 1e-11 after 100 steps at 4096^2 "
                            "(tests/test_gpu_parity.py::test_fast_mode_config2_full_size_100_steps); the fused "
-                           "step kernel is bitwise the three fast sweeps "
-                           "(tests/test_gpu_parity.py::test_fused_step_is_bitwise_the_three_sweeps)"},
+                           "step kernel hands conserved vectors between its sweeps and agrees with the three "
+                           "fast sweeps to 1e-12 (tests/test_gpu_parity.py::"
+                           "test_fused_step_matches_the_three_sweeps)"},
         "timing": "HIP events on the solver stream; per-kernel averages from a second identical region with "
                   "every launch bracketed by events",
     }

@@ -233,9 +233,11 @@ __global__ __launch_bounds__(kXThreads) void k_sweep_y_tile(SweepLaunch L) {
 // result into the y sweep's three-row register window (as k_sweep_y), and
 // every y output row goes straight through the second x sweep and out to HBM
 // with the wave-speed maximum.  Per-cell arithmetic is the three separate
-// sweeps', so the result is bitwise theirs; each cell is read and written once
-// per step instead of three times, at the cost of recomputing the first x sweep
-// on 4 halo rows per RB and every sweep on 8 halo columns per 256.
+// sweeps', except that each sweep hands its updated conserved vector straight
+// to the next (the separate sweeps store primitives and convert back), so the
+// two differ by rounding; each cell is read and written once per step instead
+// of three times, at the cost of recomputing the first x sweep on 4 halo rows
+// per RB and every sweep on 8 halo columns per 256.
 // Boundaries: columns through mirror2 at load (the first sweep's ghosts); the
 // second sweep's ghost columns are the y outputs of the mirrored columns, which
 // the y sweep produces from the first sweep's outputs of those columns: blocks
@@ -269,10 +271,11 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
 
     // index of the thread's cell of (mirrored) input row rr
     auto at_row = [&](int rr) { return (long)mirror2(rr, ny) * pitch + pc; };
-    // first x sweep of one input row (the thread's primitives Wr) -> Wx; the
-    // passes of line_step_3r need no barrier between them
+    // first x sweep of one input row (the thread's primitives Wr) -> Wx, its
+    // updated conserved vector; the passes of line_step_3r need no barrier
+    // between them
     auto x1_row = [&](const double* Wr, double* Wx) {
-        line_step_3r<Phys, LIMMODE, W0, NT>(Wr, Wx, ch, P, lds);
+        line_step_3r<Phys, LIMMODE, W0, NT, false, true>(Wr, Wx, ch, P, lds);
         if (edge) {
             // through the R2 rows of line_step_3r (free until the next pass's
             // first barrier; its last reads were before this pass's last one)
@@ -303,14 +306,13 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
     load_cell<W0>(L.in, at_row(i0 - 1), Wpre);
 #pragma unroll 1
     for (int r = i0 - 2; r <= i1 + 1; ++r) {
-        double W[5], U[5];
+        double U[5];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) W[k] = Wn[k];
+        for (int k = 0; k < 5; ++k) U[k] = Wn[k];
         if (r < i1 + 1) {
             x1_row(Wpre, Wn);  // row r + 1
             if (r < i1) load_cell<W0>(L.in, at_row(r + 2), Wpre);
         }
-        Phys::p2c(W, U, P);
         if (r >= i0) {
             double Q0[5], G0[5], Q1[5], G1[5];
             half_step<Phys, 1, LIMMODE, W0>(Ua, Ub, U, cfu, 0.0, 1, P, Q0, G0, Q1, G1);
@@ -319,9 +321,9 @@ __global__ __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(3, 8)
                 force_flux<Phys, 1, W0>(Q1p, G1p, Q0, G0, cfu, P, F);
                 if (r >= i0 + 2) {
                     // y output of row r - 2 -> second x sweep -> HBM
-                    double Wo[5], Wz[5];
-                    update_cell<Phys, W0>(Ua, Fp, F, cfu, P, Wo);
-                    line_step_3r<Phys, LIMMODE, W0, NT>(Wo, Wz, ch, P, lds);
+                    double Uo[5], Wz[5];
+                    update_cons<Phys, W0>(Ua, Fp, F, cfu, Uo);
+                    line_step_3r<Phys, LIMMODE, W0, NT, true, false>(Uo, Wz, ch, P, lds);
                     if (t >= 4 && t < NT - 4 && p < nx) {
                         store_cell<W0>(L.out, (long)(r - 2) * pitch + p, Wz);
                         const double w = Phys::template wavespeed<2>(Wz, P);

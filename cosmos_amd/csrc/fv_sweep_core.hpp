@@ -163,6 +163,16 @@ CFV_HD void update_cell(const double* U, const double* Fl, const double* Fr, con
     Phys::c2p(un, W, Phys::inv_density(un), P, eu);
 }
 
+// The conservative update of update_cell without its con2prim (fast mode):
+// the updated conserved vector, for a following sweep that starts from it.
+template <class Phys, bool W0>
+__device__ __forceinline__ void update_cons(const double* U, const double* Fl, const double* Fr,
+                                            const SweepCoef& cf, double* un) {
+    static_assert(Phys::kFast, "conserved hand-off between sweeps is a fast-mode re-association");
+#pragma unroll
+    for (int k = 0; k < 5; ++k) un[k] = (W0 && k == 3) ? 0.0 : __builtin_fma(cf.dt_dx, Fl[k] - Fr[k], U[k]);
+}
+
 // Which of a thread's con2prim solves it owns for the event counts: every
 // distinct solve of a line is counted once, in the sites the oracle
 // evaluates (cells -1 .. n of the line for the half steps and, FORCE, the cell
@@ -302,7 +312,9 @@ constexpr int slic3r_lds_rows() {
 }
 
 // line_step for a SLIC pass along x at bias 0 with van Leer (the fused step's
-// sweeps), its stages in three disjoint LDS regions: R1 the conserved vectors
+// sweeps; IN_CONS / OUT_CONS: W is the conserved vector / Wn receives the
+// updated conserved vector, skipping the con2prim of the previous sweep and
+// the prim2con of this one), its stages in three disjoint LDS regions: R1 the conserved vectors
 // (rows [0, NC)), R2 the left-face states and fluxes ([NC, 3 NC)), R3 the
 // interface fluxes ([3 NC, 4 NC)).  Each region is written only after a
 // barrier that follows the previous pass's last read of it, so consecutive
@@ -310,7 +322,7 @@ constexpr int slic3r_lds_rows() {
 // rows itself between passes uses R2 (free from the end of one pass to the
 // first barrier of the next) and barriers before and after.  Every value is
 // computed by the functions and in the order line_step uses: bitwise equal.
-template <class Phys, int LIMMODE, bool W0, int NT>
+template <class Phys, int LIMMODE, bool W0, int NT, bool IN_CONS = false, bool OUT_CONS = false>
 __device__ __forceinline__ void line_step_3r(const double* W, double* Wn, const SweepCoef& cf, const PhysParams& P,
                                              double (*lds)[NT]) {
     constexpr int NC = W0 ? 4 : 5;
@@ -323,7 +335,12 @@ __device__ __forceinline__ void line_step_3r(const double* W, double* Wn, const 
     const int sl = s >= 1 ? s - 1 : 0;
     const int sr = s < NT - 1 ? s + 1 : NT - 1;
     double U[5], F[5];
-    Phys::p2c(W, U, P);
+    if constexpr (IN_CONS) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) U[k] = W[k];
+    } else {
+        Phys::p2c(W, U, P);
+    }
 #pragma unroll
     for (int k = 0; k < 5; ++k)
         if (!(W0 && k == 3)) R1[slot(k)][s] = U[k];
@@ -361,7 +378,8 @@ __device__ __forceinline__ void line_step_3r(const double* W, double* Wn, const 
     double Fl[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) Fl[k] = (W0 && k == 3) ? 0.0 : R3[slot(k)][sl];
-    update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
+    if constexpr (OUT_CONS) update_cons<Phys, W0>(U, Fl, F, cf, Wn);
+    else update_cell<Phys, W0>(U, Fl, F, cf, P, Wn);
 }
 
 template <class Phys, int ORDER, int LIMMODE, bool W0, int NT, int AXIS, int STRIDE>

@@ -85,11 +85,13 @@ int cosmos_set_launch(cosmos_solver* s, int y_rows, int ws_mode);
  * -1: the default (for Newtonian the march where it runs at least 1024
  * blocks, else 16-column tiles; 8-column tiles for special relativistic). */
 int cosmos_set_y_tile(cosmos_solver* s, int columns);
-/* Step fusion (no counterpart in the reference; results do not depend on it):
- * with 1 (the default), a 2D fast-mode Newtonian SLIC step with van Leer and
- * bias 0 queued by cosmos_begin / cosmos_advance / cosmos_solve on a grid of at
- * least 8 x 8 cells runs its X(dt/2) Y(dt) X(dt/2) sweeps as one kernel that
- * reads and writes each cell once; 0 queues the three sweeps.  rows_per_block:
+/* Step fusion (no counterpart in the reference): with 1 (the default), a 2D
+ * fast-mode Newtonian SLIC step with van Leer and bias 0 queued by cosmos_begin
+ * / cosmos_advance / cosmos_solve on a grid of at least 8 x 8 cells runs its
+ * X(dt/2) Y(dt) X(dt/2) sweeps as one kernel that reads and writes each cell
+ * once and hands conserved vectors between its sweeps (the result differs from
+ * the three sweeps' by rounding; both stay within the fast mode's tolerance of
+ * the reference); 0 queues the three sweeps.  rows_per_block:
  * output rows of each block of the fused kernel, a launch shape only (0: chosen
  * from the grid and the kernel's occupancy; else 4 .. 1024). */
 int cosmos_set_step_fusion(cosmos_solver* s, int enable, int rows_per_block);

@@ -309,48 +309,69 @@ def test_fast_mode_config2_full_size_100_steps(orc):
     assert k == 100 and err <= FAST_TOL, err
 
 
-def fast_pair(W, steps, tf=math.inf, rows=0):
-    """The same fast-mode solve with the fused step kernel (rows per block:
-    rows, 0 automatic) and with three sweeps."""
+def fast_solve(W, steps, fuse, tf=math.inf, rows=0):
+    """A fast-mode solve with the fused step kernel (rows per block: rows, 0
+    automatic) or with three sweeps: (state, time, steps)."""
     from cosmos_amd import FAST
 
     ny, nx = W.shape[:2]
-    res = []
-    for fuse in (True, False):
-        with Solver(NEWTONIAN, 2, nx, ny, G_N) as s:
-            s.set_mode(FAST)
-            s.set_step_fusion(fuse, rows)
-            s.upload(W)
-            t, k = s.solve(2, 1.0 / max(nx, ny), 0.9, tf, steps)
-            res.append((s.download(), t, k))
-    return res
+    with Solver(NEWTONIAN, 2, nx, ny, G_N) as s:
+        s.set_mode(FAST)
+        s.set_step_fusion(fuse, rows)
+        s.upload(W)
+        t, k = s.solve(2, 1.0 / max(nx, ny), 0.9, tf, steps)
+        return s.download(), t, k
+
+
+def fast_pair(W, steps, tf=math.inf, rows=0):
+    """The same fast-mode solve with the fused step kernel and with three sweeps."""
+    return [fast_solve(W, steps, fuse, tf, rows) for fuse in (True, False)]
+
+
+# The fused step hands each sweep's updated conserved vector straight to the
+# next sweep, where the three sweeps store primitives between them and convert
+# back: the two differ by rounding only (each against the oracle: FAST_TOL).
+FUSED_TOL = 1e-12
 
 
 @pytest.mark.parametrize("zvel", [False, True])
-def test_fused_step_is_bitwise_the_three_sweeps(zvel):
+def test_fused_step_matches_the_three_sweeps(zvel):
     """The fused X(dt/2) Y(dt) X(dt/2) kernel against the three fast sweeps:
-    bitwise equal states, times and step counts, on shapes at and around its
-    248-column strips and 64-row blocks (edge strips that need the mirrored
-    ghost columns on both sides, a single strip narrower than 248, a last
-    strip of one column), with and without the z-velocity plane."""
+    equal step counts, states and times within rounding, on shapes at and
+    around its 248-column strips and 64-row blocks (edge strips that need the
+    mirrored ghost columns on both sides, a single strip narrower than 248, a
+    last strip of one column), with and without the z-velocity plane; a wrong
+    halo or ghost column would differ by the size of the state's jumps."""
+    from conftest import cons_rel_linf
+
     shapes = ((8, 8), (9, 13), (247, 70), (248, 65), (249, 130), (250, 64), (253, 9), (497, 129), (744, 200),
               (1000, 67))
+    worst = 0.0
     for idx, (nx, ny) in enumerate(shapes):
         W = workloads.quadrants_2d(nx, seed=20 + idx, ny=ny)
         if zvel:
             W[..., 3] = 0.1 * W[..., 2]
         (a, ta, ka), (b, tb, kb) = fast_pair(W, 9)
-        assert ka == kb == 9 and ta == tb, (nx, ny, ta, tb)
-        assert same(a, b), (nx, ny, rel_linf(a, b))
-    # rows per block are a launch shape only
-    for rows in (4, 13, 200):
-        W = workloads.quadrants_2d(253, seed=40 + rows, ny=150)
-        (a, ta, ka), (b, tb, kb) = fast_pair(W, 5, rows=rows)
-        assert ka == kb and ta == tb and same(a, b), rows
+        err = cons_rel_linf(a, b)
+        worst = max(worst, err)
+        assert ka == kb == 9 and abs(ta - tb) <= FUSED_TOL * tb, (nx, ny, ta, tb)
+        assert err <= FUSED_TOL, (nx, ny, err)
+    print(f"fused against three sweeps, 9 steps: worst rel L-inf {worst:.3e}")
     # final-time clamp and early stop inside the queued steps
     W = workloads.quadrants_2d(300, seed=31, ny=140)
     (a, ta, ka), (b, tb, kb) = fast_pair(W, 1000, tf=0.01)
-    assert ka == kb and ta == tb and 0 < ka < 1000 and same(a, b)
+    assert ka == kb and 0 < ka < 1000 and abs(ta - tb) <= FUSED_TOL * tb
+    assert cons_rel_linf(a, b) <= FUSED_TOL
+
+
+def test_fused_step_rows_per_block_is_a_launch_shape():
+    """Rows per block of the fused kernel change only the launch: bitwise
+    equal states, times and step counts."""
+    for rows in (4, 13, 200):
+        W = workloads.quadrants_2d(253, seed=40 + rows, ny=150)
+        a, ta, ka = fast_solve(W, 5, True, rows=rows)
+        b, tb, kb = fast_solve(W, 5, True)
+        assert ka == kb and ta == tb and same(a, b), rows
 
 
 def test_consecutive_solves_continue_from_the_last_state(orc):
@@ -377,18 +398,22 @@ def test_consecutive_solves_continue_from_the_last_state(orc):
             outs.append((s.solve(2, 1.0 / 96, 0.9, math.inf, 7), s.download()))
     from conftest import cons_rel_linf
 
-    assert outs[0][0] == outs[1][0] and same(outs[0][1], outs[1][1])
+    (t0, k0), (t1, k1) = outs[0][0], outs[1][0]
+    assert k0 == k1 and abs(t0 - t1) <= FUSED_TOL * t1
+    assert cons_rel_linf(outs[0][1], outs[1][1]) <= FUSED_TOL
     assert cons_rel_linf(outs[0][1], e2) <= FAST_TOL
 
 
 def test_fused_step_full_size_against_oracle(orc):
-    """Fused fast steps at 1024^2 against the reference arithmetic within 1e-11."""
+    """Fused fast steps at 1024^2 against the reference arithmetic within 1e-11
+    (and against the three fast sweeps within rounding)."""
     from conftest import cons_rel_linf
 
     W = workloads.quadrants_2d(1024, seed=5)
     (a, ta, ka), (b, tb, kb) = fast_pair(W, 40)
     exp, te, ke = orc.run(W, 0, 2, 1.0 / 1024, 0.9, max_steps=40)
-    assert same(a, b) and ka == ke == 40 and abs(ta - te) <= FAST_TOL * te
+    assert ka == kb == ke == 40 and abs(ta - te) <= FAST_TOL * te
+    assert cons_rel_linf(a, b) <= FUSED_TOL
     assert cons_rel_linf(a, exp) <= FAST_TOL
 
 
