@@ -15,11 +15,13 @@
 // cells.
 //
 // Kernel layout: a 256-cell tile of one line per workgroup with a 3-cell
-// halo (250 outputs), five LDS stages: U and nu; the half steps; the FORCE
+// halo (250 outputs) for x; for y, tiles of 4 neighbouring columns by 64
+// rows (58 outputs per column), each thread's line neighbours 4 threads
+// apart, so a wave reads 32 contiguous bytes per row; five LDS stages: U and nu; the half steps; the FORCE
 // adjoints (written in place over the face states: each slot is read and
 // written by one thread); the half-step adjoints; the per-cell totals and the
-// ghost fold.  x lines are contiguous; y lines are strided, with
-// neighbouring lines in consecutive workgroups so that they share cache lines.
+// ghost fold.  x lines are contiguous; y column groups are strided, with
+// neighbouring groups in consecutive workgroups so that they share cache lines.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -61,25 +63,32 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return xcd * base + (xcd < extra ? xcd : extra) + k;
 }
 
-template <int AXIS, int ORDER>
+template <int AXIS, int ORDER, int TC>
 __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
     __shared__ double sU[5][kAT], sN[5][kAT], sA[5][kAT], sB[5][kAT], sM[5][kAT];
-    const int TW = kAT - 2 * kAH;
+    constexpr int R = kAT / TC;    // positions along the line per tile
+    constexpr int TW = R - 2 * kAH;
     const int tiles = (L.n + TW - 1) / TW;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
+    const int s = threadIdx.x;
+    const int q = s / TC;          // position along the line within the tile
     int line, tile;
     if (AXIS == 0) {
         line = blk / tiles;
         tile = blk - line * tiles;
     } else {
-        tile = blk / L.lines;
-        line = blk - tile * L.lines;
+        // TC neighbouring columns per tile; consecutive workgroups take
+        // neighbouring column groups of one band, which share cache lines
+        const int groups = (L.lines + TC - 1) / TC;
+        tile = blk / groups;
+        line = (blk - tile * groups) * TC + (s - q * TC);
     }
-    const int s = threadIdx.x;
-    const int c = tile * TW - kAH + s;
+    const bool line_ok = line < L.lines;
+    const long lc = line_ok ? line : L.lines - 1;
+    const int c = tile * TW - kAH + q;
     const int n = L.n;
     const int cm = ORDER == 2 ? mirror_c(c, n) : clamp_c(c, n);
-    const long addr = AXIS == 0 ? (long)line * L.pitch + cm : (long)cm * L.pitch + line;
+    const long addr = AXIS == 0 ? lc * L.pitch + cm : (long)cm * L.pitch + lc;
     const adj::Coef& C = L.C;
 
     // stage 1: U of every position; nu of real cells (0 for ghosts)
@@ -87,7 +96,7 @@ __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) W[k] = L.W.p[k][addr];
     adj::p2c(W, U, C);
-    const bool real = c >= 0 && c < n;
+    const bool real = c >= 0 && c < n && line_ok;
     if (real) {
         double O[5], g[5];
 #pragma unroll
@@ -108,16 +117,16 @@ __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
     __syncthreads();
 
     double ul[5], ur[5];
-    if (s > 0 && s < kAT - 1) {
+    if (q > 0 && q < R - 1) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            ul[k] = sU[k][s - 1];
-            ur[k] = sU[k][s + 1];
+            ul[k] = sU[k][s - TC];
+            ur[k] = sU[k][s + TC];
         }
     }
     if constexpr (ORDER == 2) {
         // stage 2: the half step of each position -> (a, b)
-        if (s > 0 && s < kAT - 1) {
+        if (q > 0 && q < R - 1) {
             double a[5], b[5];
             adj::half_step<AXIS>(ul, U, ur, C, a, b);
 #pragma unroll
@@ -130,25 +139,25 @@ __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
     }
     // stage 3: adjoint of interface (s, s+1), phi = k (nu_{s+1} - nu_s);
     // the state adjoints overwrite the slots this thread alone reads
-    if (s > 0 && s < kAT - 2) {
+    if (q > 0 && q < R - 2) {
         double UL[5], UR[5], phi[5], gl[5], gr[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             UL[k] = ORDER == 2 ? sB[k][s] : U[k];
-            UR[k] = ORDER == 2 ? sA[k][s + 1] : ur[k];
-            phi[k] = C.k * (sN[k][s + 1] - nu[k]);
+            UR[k] = ORDER == 2 ? sA[k][s + TC] : ur[k];
+            phi[k] = C.k * (sN[k][s + TC] - nu[k]);
         }
         adj::vjp_force<AXIS>(UL, UR, phi, C, gl, gr);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             sB[k][s] = gl[k];
-            sA[k][s + 1] = gr[k];
+            sA[k][s + TC] = gr[k];
         }
     }
     __syncthreads();
     if constexpr (ORDER == 2) {
         // stage 4: half-step adjoint of each position -> contributions to s-1, s, s+1
-        if (s > 1 && s < kAT - 2) {
+        if (q > 1 && q < R - 2) {
             double ga[5], gb[5], gL[5], gM[5], gR[5];
 #pragma unroll
             for (int k = 0; k < 5; ++k) {
@@ -174,10 +183,10 @@ __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
     }
     // stage 5: the total adjoint of each position's U (into sN, own slot)
     double tot[5];
-    if (s > 0 && s < kAT - 1) {
+    if (q > 0 && q < R - 1) {
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
-            if (ORDER == 2) tot[k] = ((nu[k] + sB[k][s - 1]) + sM[k][s]) + sA[k][s + 1];
+            if (ORDER == 2) tot[k] = ((nu[k] + sB[k][s - TC]) + sM[k][s]) + sA[k][s + TC];
             else tot[k] = (nu[k] + sB[k][s]) + sA[k][s];
         }
     } else {
@@ -189,15 +198,15 @@ __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
     for (int k = 0; k < 5; ++k) sN[k][s] = tot[k];
     __syncthreads();
     // stage 6: outputs fold the ghost positions that copy them
-    if (s >= kAH && s < kAT - kAH && real) {
+    if (q >= kAH && q < R - kAH && real) {
         if (ORDER == 2) {
-            if (c == 0) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 1];
-            if (c == 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 3];
-            if (c == n - 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 1];
-            if (c == n - 2) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 3];
+            if (c == 0) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - TC];
+            if (c == 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 3 * TC];
+            if (c == n - 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + TC];
+            if (c == n - 2) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 3 * TC];
         } else {
-            if (c == 0) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - 1];
-            if (c == n - 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + 1];
+            if (c == 0) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s - TC];
+            if (c == n - 1) for (int k = 0; k < 5; ++k) tot[k] += sN[k][s + TC];
         }
         double gW[5];
         adj::vjp_p2c(W, tot, gW, C);
@@ -206,20 +215,25 @@ __global__ __launch_bounds__(kAT) void k_adjoint_sweep(AdjLaunch L) {
     }
 }
 
-dim3 adj_grid(const AdjLaunch& L) {
-    const int TW = kAT - 2 * kAH;
+// Columns per tile of the y reverse sweep (x lines are contiguous: 1).
+constexpr int kAdjYCols = 4;
+
+dim3 adj_grid(const AdjLaunch& L, int tc) {
+    const int TW = kAT / tc - 2 * kAH;
     const long tiles = (L.n + TW - 1) / TW;
-    return dim3((unsigned)(tiles * L.lines));
+    return dim3((unsigned)(tiles * ((L.lines + tc - 1) / tc)));
 }
 
 hipError_t launch_adjoint_sweep(const AdjLaunch& L, hipStream_t s) {
-    const dim3 g = adj_grid(L), b(kAT);
+    const dim3 b(kAT);
     if (L.axis == 0) {
-        if (L.order == 2) hipLaunchKernelGGL((k_adjoint_sweep<0, 2>), g, b, 0, s, L);
-        else hipLaunchKernelGGL((k_adjoint_sweep<0, 1>), g, b, 0, s, L);
+        const dim3 g = adj_grid(L, 1);
+        if (L.order == 2) hipLaunchKernelGGL((k_adjoint_sweep<0, 2, 1>), g, b, 0, s, L);
+        else hipLaunchKernelGGL((k_adjoint_sweep<0, 1, 1>), g, b, 0, s, L);
     } else {
-        if (L.order == 2) hipLaunchKernelGGL((k_adjoint_sweep<1, 2>), g, b, 0, s, L);
-        else hipLaunchKernelGGL((k_adjoint_sweep<1, 1>), g, b, 0, s, L);
+        const dim3 g = adj_grid(L, kAdjYCols);
+        if (L.order == 2) hipLaunchKernelGGL((k_adjoint_sweep<1, 2, kAdjYCols>), g, b, 0, s, L);
+        else hipLaunchKernelGGL((k_adjoint_sweep<1, 1, kAdjYCols>), g, b, 0, s, L);
     }
     return hipGetLastError();
 }
@@ -408,6 +422,7 @@ int forward_sweep_dev(cosmos_adjoint* a, int axis, double dt, const double* in, 
     L.coef.dt_dx = dt / a->dx;
     L.coef.dx_dt = a->dx / dt;
     L.coef.hdt_dx = 0.5 * (dt / a->dx);
+    L.y_tile = -1;  // the solvers' default y layout for this grid
     hipError_t e = axis == 0 ? cfv::launch_sweep_x(L, stream_of(a)) : cfv::launch_sweep_y(L, stream_of(a));
     ADJCHK(a, e);
     return COSMOS_OK;
