@@ -14,6 +14,19 @@
 #include "fv_physics.hpp"
 
 namespace cfv {
+
+// 1 / x for the Newtonian reverse sweep: on the GPU the reciprocal estimate
+// with one Newton step (within an ulp of the quotient, fast_rcp), on the host
+// (the finite-difference checks of these functions) the quotient.  That
+// kernel's own re-evaluations of forward values need no bitwise agreement
+// with the forward pass, only its branches up to the kinks.
+CFV_HD double adj_rcp(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return fast_rcp(x);
+#else
+    return 1.0 / x;
+#endif
+}
 namespace adj {
 
 struct Coef {
@@ -49,7 +62,7 @@ CFV_HD void vjp_p2c(const double* W, const double* gU, double* gW, const Coef& C
 
 // primitives of U (rho, velocity, p) with 1/rho
 CFV_HD void c2p(const double* U, double* W, double& irho, const Coef& C) {
-    irho = 1.0 / U[0];
+    irho = adj_rcp(U[0]);
     W[0] = U[0];
     W[1] = U[1] * irho;
     W[2] = U[2] * irho;
@@ -59,7 +72,7 @@ CFV_HD void c2p(const double* U, double* W, double& irho, const Coef& C) {
 
 // gU = (dW/dU)^T gW, the Jacobian of c2p taken at the primitives W = c2p(U)
 CFV_HD void vjp_c2p(const double* W, const double* gW, double* gU, const Coef& C) {
-    const double ir = 1.0 / W[0];
+    const double ir = adj_rcp(W[0]);
     const double q = 0.5 * ((W[1] * W[1] + W[2] * W[2]) + W[3] * W[3]);
     const double gp = gW[4] * C.gm1;
     gU[0] = gW[0] - ir * ((gW[1] * W[1] + gW[2] * W[2]) + gW[3] * W[3]) + gp * q;
@@ -109,10 +122,15 @@ CFV_HD void vjp_flux_acc(const double* U, const double* gF, double* gU, const Co
 
 // Limiter value phi(r) and its derivative on the branch the forward selects
 // (SlopeLimiters::computeSlopeLimiter, Solvers/SlopeLimiters.cpp:3-69).
+// FASTDIV: quotients by the reciprocal estimate (adj_rcp), for the Newtonian
+// reverse sweep; the black-hole adjoint, whose taping forward passes
+// re-evaluate the forward slopes bitwise, keeps the quotients.
+template <bool FASTDIV = false>
 CFV_HD double limiter_d(double r, double bias, int limiter, double& dphi) {
     const double den = (1.0 - bias) + (1.0 + bias) * r;
-    const double xi = 2.0 / den;
-    const double dxi = -2.0 * (1.0 + bias) / (den * den);
+    const double rden = FASTDIV ? adj_rcp(den) : 1.0;
+    const double xi = FASTDIV ? 2.0 * rden : 2.0 / den;
+    const double dxi = FASTDIV ? (-2.0 * (1.0 + bias)) * (rden * rden) : -2.0 * (1.0 + bias) / (den * den);
     dphi = 0.0;
     if (r < 0.0) return 0.0;
     if (limiter == 0) {  // SuperBee
@@ -131,12 +149,13 @@ CFV_HD double limiter_d(double r, double bias, int limiter, double& dphi) {
         return m;
     }
     if (limiter == 1) {  // van Leer
-        const double a = (2.0 * r) / (1.0 + r);
+        const double r1 = FASTDIV ? adj_rcp(1.0 + r) : 1.0;
+        const double a = FASTDIV ? (2.0 * r) * r1 : (2.0 * r) / (1.0 + r);
         if (xi < a) {
             dphi = dxi;
             return xi;
         }
-        dphi = 2.0 / ((1.0 + r) * (1.0 + r));
+        dphi = FASTDIV ? 2.0 * (r1 * r1) : 2.0 / ((1.0 + r) * (1.0 + r));
         return a;
     }
     if (r < 1.0) {  // MinBee
@@ -152,17 +171,19 @@ CFV_HD double limiter_d(double r, double bias, int limiter, double& dphi) {
 
 // One component of the limited slope Delta = s phi(r) (SlopeLimiters::computeSlopeVector,
 // Solvers/SlopeLimiters.cpp:71-101) and its partial derivatives by dl = m - l and dr = r - m.
+template <bool FASTDIV = false>
 CFV_HD double slope_d(double ul, double um, double ur, double bias, int limiter, double& d_dl,
                                           double& d_dr) {
     const double dl = um - ul, dr = ur - um;
     const double s = 0.5 * (1.0 + bias) * dl + 0.5 * (1.0 - bias) * dr;
     const bool cl = __builtin_fabs(dl) < 1e-5, cr = __builtin_fabs(dr) < 1e-5;
     const double num = cl ? 1e-5 : dl, den = cr ? 1e-5 : dr;
-    const double r = num / den;
+    const double rden = FASTDIV ? adj_rcp(den) : 1.0;
+    const double r = FASTDIV ? num * rden : num / den;
     double dphi;
-    const double phi = limiter_d(r, bias, limiter, dphi);
-    const double drdl = cl ? 0.0 : 1.0 / den;
-    const double drdr = cr ? 0.0 : -r / den;
+    const double phi = limiter_d<FASTDIV>(r, bias, limiter, dphi);
+    const double drdl = cl ? 0.0 : (FASTDIV ? rden : 1.0 / den);
+    const double drdr = cr ? 0.0 : (FASTDIV ? -r * rden : -r / den);
     d_dl = 0.5 * (1.0 + bias) * phi + s * dphi * drdl;
     d_dr = 0.5 * (1.0 - bias) * phi + s * dphi * drdr;
     return s * phi;
@@ -178,7 +199,7 @@ CFV_HD void half_step(const double* UL, const double* UM, const double* UR, cons
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         double d0, d1;
-        const double D = slope_d(UL[k], UM[k], UR[k], C.bias, C.limiter, d0, d1);
+        const double D = slope_d<true>(UL[k], UM[k], UR[k], C.bias, C.limiter, d0, d1);
         xl[k] = UM[k] - 0.5 * D;
         xr[k] = UM[k] + 0.5 * D;
     }
@@ -200,7 +221,7 @@ CFV_HD void vjp_half_step(const double* UL, const double* UM, const double* UR, 
     double xl[5], xr[5], dd0[5], dd1[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        const double D = slope_d(UL[k], UM[k], UR[k], C.bias, C.limiter, dd0[k], dd1[k]);
+        const double D = slope_d<true>(UL[k], UM[k], UR[k], C.bias, C.limiter, dd0[k], dd1[k]);
         xl[k] = UM[k] - 0.5 * D;
         xr[k] = UM[k] + 0.5 * D;
     }
