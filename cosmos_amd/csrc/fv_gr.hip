@@ -201,6 +201,10 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_roundtrip(GRLaunch L, double*
     if (idx >= (long)qw * qh) return;
     const int r = (int)(idx / qw), cc = (int)(idx - (long)r * qw);
     const int i = DIM == 2 ? r - 1 : 0, j = cc - 1;
+    // the forcing reads this chain only from cells whose stencil is clear, so
+    // never at an excised position (whose recoveries near the singularity can
+    // run to the iteration cap): skipped there
+    if (excised(L.hole, j * L.h, DIM == 2 ? i * L.h : 0.0, 0.0)) return;
     double W[5], U[5], R[5];
     load5(L.in, (long)clamp1_gr(i, L.ny) * L.pitch + clamp1_gr(j, L.nx), W);
     const Geo g = geo_from_table(L.tab, DIM, 2 * j, 2 * i);
