@@ -34,6 +34,10 @@ namespace cfv {
 namespace {
 
 constexpr int kGRThreads = 256;
+// The forcing runs at one wave per SIMD (its RK4 chain holds ~490 registers
+// per lane): one-wave blocks let each SIMD start its next wave as soon as its
+// own finishes, instead of waiting for the slowest of a 4-wave block.
+constexpr int kForceThreads = 64;
 
 __device__ __forceinline__ int mirror2_gr(int c, int n) {
     if (c < 0) c = -c - 1;
@@ -222,10 +226,10 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_roundtrip(GRLaunch L, double*
 // the step-start neighbours (1D: x by 1/2, 1/2, 1; 2D: x 1/4 y 1/2 x 1/4,
 // twice, then x 1/2 y 1 x 1/2), U + (k1 + 2 k2 + 2 k3 + k4) / 6.
 template <int DIM, int GEO>
-__global__ __launch_bounds__(kGRThreads) void k_gr_forcing(GRLaunch L) {
+__global__ __launch_bounds__(kForceThreads) void k_gr_forcing(GRLaunch L) {
     if (gr_stopped(L)) return;
     const GRStep S = gr_step_of(L);
-    long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
+    long idx = (long)blockIdx.x * kForceThreads + threadIdx.x;
     if (idx >= (long)L.nx * L.ny) return;
     int i = (int)(idx / L.nx), j = (int)(idx - (long)i * L.nx);
     long cell = (long)i * L.pitch + j;
@@ -381,6 +385,7 @@ __global__ __launch_bounds__(kGRThreads) void k_gr_wavespeed(GRLaunch L, double 
 }
 
 dim3 blocks_for(long work) { return dim3((unsigned)((work + kGRThreads - 1) / kGRThreads)); }
+dim3 force_blocks(long cells) { return dim3((unsigned)((cells + kForceThreads - 1) / kForceThreads)); }
 
 }  // namespace
 
@@ -426,12 +431,12 @@ hipError_t launch_gr_forcing(const GRLaunch& L, hipStream_t s) {
         else hipLaunchKernelGGL((k_gr_roundtrip<2>), blocks_for(qn), dim3(kGRThreads), 0, s, L, q);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1, 1>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
-        else hipLaunchKernelGGL((k_gr_forcing<2, 1>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
+        if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1, 1>), force_blocks(cells), dim3(kForceThreads), 0, s, L);
+        else hipLaunchKernelGGL((k_gr_forcing<2, 1>), force_blocks(cells), dim3(kForceThreads), 0, s, L);
         return hipGetLastError();
     }
-    if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1, 0>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
-    else hipLaunchKernelGGL((k_gr_forcing<2, 0>), blocks_for(cells), dim3(kGRThreads), 0, s, L);
+    if (L.dim == 1) hipLaunchKernelGGL((k_gr_forcing<1, 0>), force_blocks(cells), dim3(kForceThreads), 0, s, L);
+    else hipLaunchKernelGGL((k_gr_forcing<2, 0>), force_blocks(cells), dim3(kForceThreads), 0, s, L);
     return hipGetLastError();
 }
 
