@@ -26,7 +26,7 @@
 namespace cfv {
 namespace {
 
-constexpr int kGAT = 128;
+constexpr int kGAT = 64;  // one-wave blocks: the reverse kernels run at one wave per SIMD
 
 __device__ __forceinline__ int mirror2_a(int c, int n) {
     if (c < 0) c = -c - 1;
