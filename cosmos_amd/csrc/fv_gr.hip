@@ -38,6 +38,8 @@ constexpr int kGRThreads = 256;
 // per lane): one-wave blocks let each SIMD start its next wave as soon as its
 // own finishes, instead of waiting for the slowest of a 4-wave block.
 constexpr int kForceThreads = 64;
+// The interface fluxes run at two waves per SIMD; one-wave blocks likewise.
+constexpr int kIfaceThreads = 64;
 
 __device__ __forceinline__ int mirror2_gr(int c, int n) {
     if (c < 0) c = -c - 1;
@@ -80,14 +82,14 @@ __device__ __forceinline__ bool stencil_clear(const Hole& b, int dim, int i, int
 // y sweep: thread -> (interface c, column j), c in [0, ny].  Interface c lies
 // between cells c - 1 and c at (c - 0.5) * h.
 template <int ORDER, int AXIS, int GEO>
-__global__ __launch_bounds__(kGRThreads) void k_gr_iface(GRLaunch L) {
+__global__ __launch_bounds__(kIfaceThreads) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_gr_iface(GRLaunch L) {
     if (gr_stopped(L)) return;
     const SweepCoef cf = gr_step_of(L).cf;
     const int n = AXIS == 0 ? L.nx : L.ny;
     const long lines = AXIS == 0 ? L.ny : L.nx;
     const long per = AXIS == 0 ? (long)(n + 1) : lines;
     const long total = (long)(n + 1) * lines;
-    long idx = (long)blockIdx.x * kGRThreads + threadIdx.x;
+    long idx = (long)blockIdx.x * kIfaceThreads + threadIdx.x;
     if (idx >= total) return;
     int c, i, j;
     if (AXIS == 0) {
@@ -395,12 +397,13 @@ template <int GEO>
 hipError_t launch_gr_sweep_t(const GRLaunch& L, int axis, hipStream_t s) {
     const dim3 block(kGRThreads);
     const long ifaces = axis == 0 ? (long)(L.nx + 1) * L.ny : (long)(L.ny + 1) * L.nx;
+    const dim3 iblock(kIfaceThreads), igrid((unsigned)((ifaces + kIfaceThreads - 1) / kIfaceThreads));
     if (L.order == 2) {
-        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<2, 0, GEO>), blocks_for(ifaces), block, 0, s, L);
-        else hipLaunchKernelGGL((k_gr_iface<2, 1, GEO>), blocks_for(ifaces), block, 0, s, L);
+        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<2, 0, GEO>), igrid, iblock, 0, s, L);
+        else hipLaunchKernelGGL((k_gr_iface<2, 1, GEO>), igrid, iblock, 0, s, L);
     } else {
-        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<1, 0, GEO>), blocks_for(ifaces), block, 0, s, L);
-        else hipLaunchKernelGGL((k_gr_iface<1, 1, GEO>), blocks_for(ifaces), block, 0, s, L);
+        if (axis == 0) hipLaunchKernelGGL((k_gr_iface<1, 0, GEO>), igrid, iblock, 0, s, L);
+        else hipLaunchKernelGGL((k_gr_iface<1, 1, GEO>), igrid, iblock, 0, s, L);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
